@@ -1,0 +1,232 @@
+/*
+ * newt_gpu.h — C ABI of the MI355X batched newt step.
+ *
+ * The reference engine has one mutable `World` (model and state together,
+ * newt/src/world.rs:108-204) advanced by `World::step(&mut self)`
+ * (newt/src/world.rs:1459-1537).  This boundary splits it into an immutable
+ * `newt_model` (the non-state fields of `World`: dt, gravity, integrator,
+ * solver config, trees' links, geoms, pair filters) and a `newt_data`
+ * holding the mutable state of N independent worlds that share that model.
+ * `newt_step` advances every world of a `newt_data` by `n_steps` calls of
+ * the reference's `World::step`.
+ *
+ * All functions return a `newt_status`; on failure `newt_last_error()`
+ * returns a thread-local message.  The reference panics instead
+ * (newt/src/solver.rs:123-131, newt/src/world.rs:874-898,
+ * newt/src/solver.rs:2856-2858); here a panic-class condition inside a
+ * world sets that world's fault flag, freezes it, and the other worlds
+ * continue.
+ *
+ * Plain C types only: pointers, sizes, doubles.  Host arrays are row-major
+ * [world][slot].
+ */
+#ifndef NEWT_GPU_H
+#define NEWT_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum newt_status {
+    NEWT_OK = 0,
+    NEWT_ERR_INVALID_CONFIG = 1,  /* SolverConfig::validate (solver.rs:123-131) */
+    NEWT_ERR_UNSUPPORTED_PAIR = 2,/* World::assert_pairs_supported (world.rs:874-898) */
+    NEWT_ERR_NOT_PD = 3,          /* tree contact solve, non-PD M (solver.rs:2856) */
+    NEWT_ERR_NEWTON_FAIL = 4,     /* Newton solve failure (solver.rs:1379) */
+    NEWT_ERR_OOM = 5,
+    NEWT_ERR_DEVICE = 6,
+    NEWT_ERR_INVALID_ARG = 7,
+    NEWT_ERR_CAPACITY = 8,        /* a world exceeded a compiled capacity */
+    NEWT_ERR_UNSUPPORTED = 9      /* a model feature this build does not step */
+} newt_status;
+
+/* JointKind (newt/src/joint.rs:40-110). */
+typedef enum newt_joint_kind {
+    NEWT_JOINT_FREE = 0,
+    NEWT_JOINT_FIXED = 1,
+    NEWT_JOINT_HINGE = 2,
+    NEWT_JOINT_SLIDE = 3,
+    NEWT_JOINT_BALL = 4
+} newt_joint_kind;
+
+/* GeomShape (newt/src/geom.rs:189). */
+typedef enum newt_shape_kind {
+    NEWT_SHAPE_PLANE = 0,
+    NEWT_SHAPE_SPHERE = 1,
+    NEWT_SHAPE_BOX = 2,
+    NEWT_SHAPE_CAPSULE = 3
+} newt_shape_kind;
+
+/* Integrator (newt/src/world.rs:76-94). */
+typedef enum newt_integrator {
+    NEWT_INTEGRATOR_EULER = 0,
+    NEWT_INTEGRATOR_IMPLICITFAST = 1,
+    NEWT_INTEGRATOR_RK4 = 2
+} newt_integrator;
+
+/* SolverMode (newt/src/solver.rs:62-73). */
+typedef enum newt_solver_mode {
+    NEWT_SOLVER_PENALTY = 0,
+    NEWT_SOLVER_PGS = 1,
+    NEWT_SOLVER_NEWTON = 2
+} newt_solver_mode;
+
+/* ConeKind (newt/src/solver.rs:77-89). */
+typedef enum newt_cone {
+    NEWT_CONE_PYRAMIDAL = 0,
+    NEWT_CONE_ELLIPTIC = 1
+} newt_cone;
+
+/* Arithmetic of a newt_data. */
+typedef enum newt_precision {
+    /* Every quantity in FP64; the drop-in for World::step. */
+    NEWT_PRECISION_FP64 = 0,
+    /* FP32 kinematics, ABA, narrowphase, Delassus and PGS sweeps; FP64 mass
+     * matrix Cholesky, qpos/quaternion accumulation and stored state. */
+    NEWT_PRECISION_MIXED = 1
+} newt_precision;
+
+/* World options: the scalar, non-state fields of World
+ * (newt/src/world.rs:110-166). */
+typedef struct newt_options {
+    double dt;               /* World::dt */
+    double gravity[3];       /* World::gravity */
+    double penetration_slop; /* World::penetration_slop */
+    int32_t integrator;      /* newt_integrator */
+    int32_t solver_mode;     /* newt_solver_mode */
+    int32_t iterations;      /* SolverConfig::iterations */
+    int32_t cone;            /* newt_cone */
+} newt_options;
+
+/* One Link (newt/src/tree.rs:79-123) plus its joint (joint.rs:40-110).
+ * Quaternions are (x, y, z, w); the inertia is column-major like Mat3
+ * (newt/src/math.rs:386). */
+typedef struct newt_link_desc {
+    int32_t parent;          /* -1 for the root */
+    int32_t joint;           /* newt_joint_kind */
+    double axis[3];          /* hinge/slide axis in the parent frame */
+    double damping;          /* JointKind damping */
+    double armature;         /* JointKind armature */
+    double free_damping;     /* Link::free_damping */
+    int32_t has_range;       /* JointKind range Some(..) */
+    int32_t pad0;
+    double range[2];
+    double parent_pos[3];    /* joint_offset_in_parent.0 */
+    double parent_quat[4];   /* joint_offset_in_parent.1 */
+    double child_pos[3];     /* joint_offset_in_child.0 */
+    double child_quat[4];    /* joint_offset_in_child.1 */
+    double mass;
+    double inertia[9];       /* inertia_body, column-major */
+} newt_link_desc;
+
+/* One Geom (newt/src/geom.rs:352-419). */
+typedef struct newt_geom_desc {
+    int32_t shape;           /* newt_shape_kind */
+    int32_t tree;            /* attached tree index, -1 for none */
+    int32_t link;            /* attached link index within that tree */
+    int32_t condim;          /* 1, 3, 4 or 6 */
+    double size[3];          /* sphere: r; box: half extents; capsule: r, half_height */
+    double pos[3];           /* local_offset */
+    double quat[4];          /* local_orientation (x, y, z, w) */
+    double friction;
+    double margin;
+    double gap;
+    double solref[2];        /* (timeconst, dampratio) */
+    double solimp[4];        /* (dmin, dmax, width, midpoint) */
+    int32_t solimp_power;    /* SolImp::power */
+    uint32_t group;          /* collision_group */
+    uint32_t mask;           /* collision_mask */
+    int32_t pad0;
+} newt_geom_desc;
+
+typedef struct newt_model newt_model;
+typedef struct newt_data newt_data;
+
+/* Thread-local description of the last failure. */
+const char* newt_last_error(void);
+
+/* ---- model: the immutable half of World ------------------------------ */
+
+/* World::new + the option fields (newt/src/world.rs:326-361). */
+newt_status newt_model_create(const newt_options* options, newt_model** out);
+
+/* World::add_tree (newt/src/world.rs:965).  `self_collide` = 0 adds the
+ * tree to World::disabled_self_collision (newt/src/world.rs:130). */
+newt_status newt_model_add_tree(newt_model* model, const newt_link_desc* links,
+                                int32_t n_links, int32_t self_collide,
+                                int32_t* tree_index);
+
+/* World::add_geom (newt/src/world.rs:974). */
+newt_status newt_model_add_geom(newt_model* model, const newt_geom_desc* geom,
+                                int32_t* geom_index);
+
+/* Validate (SolverConfig::validate, assert_pairs_supported) and compile the
+ * model into device tables.  No further add_* calls are accepted. */
+newt_status newt_model_finalize(newt_model* model);
+
+/* Sizes of one world: nq, nv, link count, geom count. */
+newt_status newt_model_dims(const newt_model* model, int32_t* nq, int32_t* nv,
+                            int32_t* n_links, int32_t* n_geoms);
+
+void newt_model_destroy(newt_model* model);
+
+/* ---- data: the mutable half of World, for n_worlds worlds ------------ */
+
+/* Allocates device state for n_worlds worlds.  Each world starts at the
+ * default state of Tree::push_link (newt/src/tree.rs:275-330). */
+newt_status newt_data_create(const newt_model* model, int32_t n_worlds,
+                             int32_t precision, newt_data** out);
+
+void newt_data_destroy(newt_data* data);
+
+/* Host copies of the generalized state: qpos [n_worlds*nq], qvel
+ * [n_worlds*nv].  set_state also rebuilds each world's broadphase fat
+ * bounds from the new state, as World::add_geom and the public state
+ * setters do (newt/src/world.rs:974-979, 1119-1136), and clears faults. */
+newt_status newt_data_set_state(newt_data* data, const double* qpos,
+                                const double* qvel);
+newt_status newt_data_get_state(const newt_data* data, double* qpos,
+                                double* qvel);
+
+/* Generalized acceleration of the last step's integration: the qddot that
+ * euler_step_with_workspace adds to qdot (newt/src/tree.rs:1407-1417).
+ * [n_worlds*nv]. */
+newt_status newt_data_get_qacc(const newt_data* data, double* qacc);
+
+/* Per-world fault flags [n_worlds] (0 = healthy; else a newt_status). */
+newt_status newt_data_get_faults(const newt_data* data, int32_t* faults);
+
+/* Contacts consumed by the last step's solver, in narrow-phase order
+ * (newt/src/world.rs:2476-2528): for world w, counts[w] contacts; entry
+ * [w*cap + k] holds (geom_a, geom_b, slot) in `ids` (3 int32 each) and
+ * (position xyz, normal xyz, penetration) in `geo` (7 doubles each).
+ * `cap` is the per-world capacity reported by newt_data_contact_capacity. */
+newt_status newt_data_contact_capacity(const newt_data* data, int32_t* cap);
+newt_status newt_data_get_contacts(const newt_data* data, int32_t* counts,
+                                   int32_t* ids, double* geo);
+
+/* Broadphase candidate pairs of the last step (World::active_pairs,
+ * newt/src/world.rs:1173-1226), sorted (min, max): counts[w] pairs, entry
+ * [w*pair_cap + k] = (a, b). */
+newt_status newt_data_pair_capacity(const newt_data* data, int32_t* cap);
+newt_status newt_data_get_pairs(const newt_data* data, int32_t* counts,
+                                int32_t* pairs);
+
+/* ---- step --------------------------------------------------------------- */
+
+/* n_steps × World::step (newt/src/world.rs:1459) on every world of `data`,
+ * enqueued on `stream` (a hipStream_t; NULL = the default stream).  Returns
+ * once the work is enqueued; faults are read with newt_data_get_faults. */
+newt_status newt_step(const newt_model* model, newt_data* data,
+                      int32_t n_steps, void* stream);
+
+/* Blocks until every step enqueued on `data` has finished. */
+newt_status newt_data_synchronize(newt_data* data);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEWT_GPU_H */

@@ -1,0 +1,244 @@
+// golden_scenes.cpp — the reference's golden scenes rebuilt on the oracle.
+//
+// TEST INFRASTRUCTURE ONLY.  Each function rebuilds one scene from the
+// reference's golden tests in `float` and serializes the state exactly as
+// the test does; tests/test_oracle_golden.py compares the bytes with the
+// fixture files under tests/golden/.
+#include "newt_world.hpp"
+
+#include <cstring>
+#include <string>
+
+using namespace oracle;
+using F = float;
+
+namespace {
+
+Link<F> mk_link(int parent, int joint, V3<F> pj, Q<F> pq, V3<F> cj, F mass, M3<F> inertia) {
+    Link<F> l;
+    l.parent = parent;
+    l.joint = joint;
+    l.pj_pos = pj;
+    l.pj_quat = pq;
+    l.cj_pos = cj;
+    l.cj_quat = qidentity<F>();
+    l.mass = mass;
+    l.inertia = inertia;
+    return l;
+}
+Link<F> hinge_link(int parent, V3<F> axis, V3<F> pj, V3<F> cj, F mass, M3<F> inertia) {
+    Link<F> l = mk_link(parent, J_HINGE, pj, qidentity<F>(), cj, mass, inertia);
+    l.axis = axis;
+    return l;
+}
+void put(std::vector<uint8_t>& out, F v) {
+    uint8_t b[4];
+    std::memcpy(b, &v, 4);
+    out.insert(out.end(), b, b + 4);
+}
+void put_u32(std::vector<uint8_t>& out, uint32_t v) {
+    uint8_t b[4];
+    std::memcpy(b, &v, 4);
+    out.insert(out.end(), b, b + 4);
+}
+void snap_tree(std::vector<uint8_t>& out, const Tree<F>& t) {
+    for (F v : t.q) put(out, v);
+    for (F v : t.qdot) put(out, v);
+}
+
+// newt/tests/joints_golden.rs:23-135
+std::vector<uint8_t> joints_chain_and_floating() {
+    Tree<F> chain;
+    chain.push_link(mk_link(-1, J_FIXED, vzero<F>(), qidentity<F>(), vzero<F>(), 1.0f, m3diag(1.0f, 1.0f, 1.0f)));
+    const F masses[3] = {1.1f, 0.7f, 1.3f};
+    const F lengths[3] = {0.6f, 0.4f, 0.5f};
+    for (int i = 0; i < 3; ++i) {
+        F l = lengths[i], m = masses[i];
+        F ip = (1.0f / 12.0f) * m * l * l;
+        V3<F> pa = i == 0 ? vzero<F>() : v3(0.0f, 0.0f, -lengths[i - 1] * 0.5f);
+        chain.push_link(hinge_link(i, v3(1.0f, 0.0f, 0.0f), pa, v3(0.0f, 0.0f, l * 0.5f), m, m3diag(ip, ip, 1e-6f)));
+    }
+    chain.q[chain.q_offset[1]] = 0.4f;
+    chain.q[chain.q_offset[2]] = -0.3f;
+    chain.q[chain.q_offset[3]] = 0.2f;
+    chain.qdot[chain.v_offset[2]] = 0.5f;
+
+    Tree<F> fl;
+    F hx = 0.3f, hy = 0.2f, hz = 0.15f, m0 = 2.0f;
+    F hx2 = hx * hx, hy2 = hy * hy, hz2 = hz * hz;
+    M3<F> box_i = m3diag((m0 / 3.0f) * (hy2 + hz2), (m0 / 3.0f) * (hx2 + hz2), (m0 / 3.0f) * (hx2 + hy2));
+    Q<F> q0 = from_axis_angle(v3(1.0f, 0.4f, -0.3f), 0.6f);
+    fl.push_link(mk_link(-1, J_FREE, v3(0.5f, -1.0f, 4.5f), q0, vzero<F>(), m0, box_i));
+    F L = 0.8f, m1 = 0.7f;
+    F ip = (1.0f / 12.0f) * m1 * L * L;
+    V3<F> axis = normalize(v3(1.0f, 0.3f, 0.0f));
+    fl.push_link(hinge_link(0, axis, v3(0.0f, 0.0f, -hz), v3(0.0f, 0.0f, L * 0.5f), m1, m3diag(ip, ip, 1e-4f)));
+    fl.qdot[fl.v_offset[1]] = 4.0f;
+
+    std::vector<uint8_t> out;
+    snap_tree(out, chain);
+    snap_tree(out, fl);
+    V3<F> gc{0.0f, 0.0f, -9.81f}, gf{0.0f, 0.0f, 0.0f};
+    AbaWs<F> w1, w2;
+    F dt = 0.005f;
+    for (int s = 0; s < 1000; ++s) {
+        rk4_step(chain, gc, dt, w1);
+        rk4_step(fl, gf, dt, w2);
+        if (s == 99) { snap_tree(out, chain); snap_tree(out, fl); }
+    }
+    snap_tree(out, chain);
+    snap_tree(out, fl);
+    return out;
+}
+
+// newt/tests/joints_mixed_golden.rs
+std::vector<uint8_t> joints_mixed() {
+    Tree<F> t;
+    t.push_link(mk_link(-1, J_FREE, v3(0.1f, -0.2f, 1.4f), from_axis_angle(normalize(v3(1.0f, 0.3f, -0.2f)), 0.4f),
+                        vzero<F>(), 1.7f, m3diag(0.4f, 0.3f, 0.35f)));
+    F w0[6] = {0.2f, 0.15f, -0.05f, 0.05f, -0.1f, 0.03f};
+    for (int k = 0; k < 6; ++k) t.qdot[k] = w0[k];
+    t.push_link(hinge_link(0, normalize(v3(0.7f, 0.2f, 0.15f)), v3(0.2f, 0.0f, -0.15f), v3(0.0f, 0.0f, 0.3f), 0.9f,
+                           m3diag(0.08f, 0.08f, 0.005f)));
+    t.q[t.q_offset[1]] = 0.4f;
+    t.qdot[t.v_offset[1]] = -0.3f;
+    Link<F> sl = mk_link(1, J_SLIDE, v3(0.0f, 0.0f, -0.3f), qidentity<F>(), vzero<F>(), 0.6f, m3diag(0.02f, 0.02f, 0.02f));
+    sl.axis = normalize(v3(0.3f, 0.9f, -0.1f));
+    t.push_link(sl);
+    t.q[t.q_offset[2]] = 0.15f;
+    t.qdot[t.v_offset[2]] = 0.25f;
+    Link<F> bl = mk_link(2, J_BALL, v3(0.05f, 0.0f, -0.1f), qidentity<F>(), v3(0.0f, 0.0f, 0.25f), 0.5f, m3diag(0.03f, 0.03f, 0.004f));
+    bl.damping = 0.05f;
+    t.push_link(bl);
+    Q<F> qb = renormalize(from_axis_angle(v3(1.0f, 0.0f, 0.0f), 0.35f) * from_axis_angle(v3(0.0f, 0.0f, 1.0f), 0.2f));
+    int o = t.q_offset[3];
+    t.q[o] = qb.x; t.q[o + 1] = qb.y; t.q[o + 2] = qb.z; t.q[o + 3] = qb.w;
+    int vo = t.v_offset[3];
+    t.qdot[vo] = 0.3f; t.qdot[vo + 1] = 0.6f; t.qdot[vo + 2] = -0.4f;
+
+    std::vector<uint8_t> out;
+    auto ser = [&]() {
+        put_u32(out, (uint32_t)t.q.size());
+        for (F v : t.q) put(out, v);
+        put_u32(out, (uint32_t)t.qdot.size());
+        for (F v : t.qdot) put(out, v);
+    };
+    ser();
+    AbaWs<F> ws;
+    V3<F> g{0.0f, 0.0f, -9.81f};
+    const int deltas[3] = {100, 400, 500};
+    for (int d : deltas) {
+        for (int s = 0; s < d; ++s) rk4_step(t, g, 0.005f, ws);
+        ser();
+    }
+    return out;
+}
+
+M3<F> box_inertia(F mass, F hx, F hy, F hz) {
+    F hx2 = hx * hx, hy2 = hy * hy, hz2 = hz * hz;
+    return m3diag((mass / 3.0f) * (hy2 + hz2), (mass / 3.0f) * (hx2 + hz2), (mass / 3.0f) * (hx2 + hy2));
+}
+
+// newt/tests/dynamics_golden.rs
+std::vector<uint8_t> dynamics_mass_bias() {
+    Tree<F> t;
+    t.push_link(mk_link(-1, J_FIXED, vzero<F>(), qidentity<F>(), vzero<F>(), 1.0f, m3diag(1.0f, 1.0f, 1.0f)));
+    Link<F> h = hinge_link(0, normalize(v3(1.0f, 0.3f, -0.2f)), v3(0.05f, 0.0f, -0.1f), v3(-0.15f, 0.0f, 0.3f), 0.9f,
+                           box_inertia(0.9f, 0.15f, 0.1f, 0.3f));
+    h.armature = 0.02f;
+    t.push_link(h);
+    Link<F> s = mk_link(1, J_SLIDE, v3(0.0f, 0.0f, -0.3f), qidentity<F>(), v3(0.1f, -0.2f, 0.15f), 0.7f,
+                        box_inertia(0.7f, 0.1f, 0.2f, 0.15f));
+    s.axis = normalize(v3(0.2f, 1.0f, 0.4f));
+    s.armature = 0.05f;
+    t.push_link(s);
+    Link<F> b = mk_link(2, J_BALL, v3(0.1f, -0.2f, -0.15f), qidentity<F>(), v3(-0.05f, 0.1f, 0.2f), 0.5f,
+                        m3diag(0.02f, 0.03f, 0.04f));
+    b.armature = 0.01f;
+    t.push_link(b);
+    t.q[t.q_offset[1]] = 0.35f;
+    t.qdot[t.v_offset[1]] = 0.6f;
+    t.q[t.q_offset[2]] = 0.12f;
+    t.qdot[t.v_offset[2]] = -0.4f;
+    Q<F> qb = renormalize(from_axis_angle(normalize(v3(0.4f, 1.0f, -0.3f)), 0.55f));
+    int o = t.q_offset[3];
+    t.q[o] = qb.x; t.q[o + 1] = qb.y; t.q[o + 2] = qb.z; t.q[o + 3] = qb.w;
+    int vo = t.v_offset[3];
+    t.qdot[vo] = 0.3f; t.qdot[vo + 1] = -0.5f; t.qdot[vo + 2] = 0.7f;
+    std::vector<F> m = mass_matrix(t);
+    std::vector<F> qdd(t.nv(), 0.0f);
+    std::vector<std::pair<V3<F>, V3<F>>> ext(t.links.size(), {vzero<F>(), vzero<F>()});
+    std::vector<F> bias = inverse_dynamics(t, qdd, v3(0.0f, 0.0f, -9.81f), ext);
+    std::vector<uint8_t> out;
+    for (F v : m) put(out, v);
+    for (F v : bias) put(out, v);
+    return out;
+}
+
+Geom<F> static_plane(F friction) {  // Geom::static_plane(ZERO, Z, friction), geom.rs:465
+    Geom<F> g;
+    g.shape = S_PLANE;
+    g.pos = vzero<F>();
+    g.quat = qidentity<F>();  // quat_align_z_to(+Z) = identity
+    g.friction = friction;
+    return g;
+}
+Geom<F> box_on_link(int tree, int link, V3<F> he, V3<F> off, Q<F> q, F friction) {  // geom.rs:604
+    Geom<F> g;
+    g.shape = S_BOX;
+    g.tree = tree;
+    g.link = link;
+    g.size = he;
+    g.pos = off;
+    g.quat = q;
+    g.friction = friction;
+    return g;
+}
+
+// newt/tests/solver_golden.rs:93-144 + record_tree (272-297), PGS, RK4.
+std::vector<uint8_t> solver_tree() {
+    World<F> w;
+    w.dt = 0.005f;
+    w.gravity = v3(0.0f, 0.0f, -9.81f);
+    w.mode = M_PGS;
+    w.iterations = 30;
+    w.cone = C_PYRAMIDAL;
+    w.integrator = I_RK4;
+    Geom<F> plane = static_plane(0.7f);
+    plane.solref = {0.05f, 1.5f};
+    w.add_geom(plane);
+    Tree<F> t;
+    Q<F> ro = from_axis_angle(normalize(v3(1.0f, 0.4f, -0.2f)), 0.35f);
+    t.push_link(mk_link(-1, J_FREE, v3(0.13f, -0.07f, 0.78f), ro, vzero<F>(), 1.2f, m3diag(0.08f, 0.09f, 0.1f)));
+    t.push_link(hinge_link(0, v3(0.0f, 1.0f, 0.0f), v3(0.0f, 0.0f, -0.3f), v3(0.0f, 0.0f, 0.22f), 0.75f,
+                           m3diag(0.02f, 0.025f, 0.018f)));
+    t.q[t.q_offset[1]] = 0.37f;
+    t.qdot[t.v_offset[1]] = -0.25f;
+    int ti = w.add_tree(t, true);
+    w.add_geom(box_on_link(ti, 0, v3(0.22f, 0.17f, 0.19f), v3(0.08f, -0.06f, 0.0f), from_axis_angle(v3(0.0f, 1.0f, 0.0f), 0.2f), 0.7f));
+    w.add_geom(box_on_link(ti, 1, v3(0.14f, 0.12f, 0.24f), v3(0.0f, 0.0f, -0.18f), from_axis_angle(v3(1.0f, 0.0f, 0.0f), -0.15f), 0.7f));
+    std::vector<uint8_t> out;
+    auto snap = [&]() { for (auto& tr : w.trees) snap_tree(out, tr); };
+    snap();
+    for (int s = 0; s < 1000; ++s) {
+        w.step();
+        if (s == 49 || s == 199) snap();
+    }
+    snap();
+    return out;
+}
+
+}  // namespace
+
+extern "C" int oracle_golden_bytes(const char* name, uint8_t* out, int cap) {
+    std::vector<uint8_t> b;
+    std::string n(name);
+    if (n == "joints_chain_and_floating") b = joints_chain_and_floating();
+    else if (n == "joints_mixed") b = joints_mixed();
+    else if (n == "dynamics_mass_bias") b = dynamics_mass_bias();
+    else if (n == "solver_tree") b = solver_tree();
+    else return -1;
+    if ((int)b.size() > cap) return -2;
+    std::memcpy(out, b.data(), b.size());
+    return (int)b.size();
+}

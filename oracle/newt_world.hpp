@@ -1,0 +1,809 @@
+// newt_world.hpp — CPU oracle, world level: broadphase, narrowphase, the
+// tree-contact PGS solve and World::step for articulated trees.
+//
+// TEST INFRASTRUCTURE ONLY (see newt_ref.hpp).  Restates:
+//   newt/src/broadphase.rs   Aabb, fat-bound hysteresis, pair filter
+//   newt/src/world.rs        step, active_pairs, populate_broadphase_tree,
+//                            swept_geom_bound, collect_contacts_full,
+//                            step_trees / step_trees_euler, tangent_basis
+//   newt/src/contact.rs      dispatch_narrow_phase, sphere/box/capsule-plane,
+//                            box-box SAT + face clipping manifold
+//   newt/src/solver.rs       SolImp/SolRef math, solve_tree_contacts (PGS)
+#pragma once
+
+#include "newt_ref.hpp"
+
+namespace oracle {
+
+// ---------------------------------------------------------------------------
+// Aabb (broadphase.rs:22-122)
+// ---------------------------------------------------------------------------
+
+template <class T> struct Aabb {
+    V3<T> mn, mx;
+};
+template <class T> inline Aabb<T> aabb_unbounded() {
+    T inf = std::numeric_limits<T>::infinity();
+    return {{-inf, -inf, -inf}, {inf, inf, inf}};
+}
+template <class T> inline Aabb<T> aabb_center_ext(V3<T> c, V3<T> e) { return {c - e, c + e}; }
+template <class T> inline Aabb<T> expanded(Aabb<T> a, T m) {  // broadphase.rs:41-44
+    V3<T> mm{m, m, m};
+    return {a.mn - mm, a.mx + mm};
+}
+template <class T> inline Aabb<T> fatten(Aabb<T> a) {  // broadphase.rs:46-54
+    const T FATNESS = T(0.1), EPS = T(1.0e-4);
+    V3<T> e = a.mx - a.mn;
+    V3<T> m{rmax(e.x * FATNESS, EPS), rmax(e.y * FATNESS, EPS), rmax(e.z * FATNESS, EPS)};
+    return {a.mn - m, a.mx + m};
+}
+template <class T> inline Aabb<T> aunion(Aabb<T> a, Aabb<T> b) {  // broadphase.rs:56-69
+    return {{rmin(a.mn.x, b.mn.x), rmin(a.mn.y, b.mn.y), rmin(a.mn.z, b.mn.z)},
+            {rmax(a.mx.x, b.mx.x), rmax(a.mx.y, b.mx.y), rmax(a.mx.z, b.mx.z)}};
+}
+template <class T> inline bool overlaps(Aabb<T> a, Aabb<T> b) {  // broadphase.rs:71-78
+    return a.mn.x <= b.mx.x && a.mx.x >= b.mn.x && a.mn.y <= b.mx.y && a.mx.y >= b.mn.y &&
+           a.mn.z <= b.mx.z && a.mx.z >= b.mn.z;
+}
+template <class T> inline bool contains(Aabb<T> a, Aabb<T> b) {  // broadphase.rs:80-87
+    return a.mn.x <= b.mn.x && a.mn.y <= b.mn.y && a.mn.z <= b.mn.z && a.mx.x >= b.mx.x &&
+           a.mx.y >= b.mx.y && a.mx.z >= b.mx.z;
+}
+
+// geom_aabb (broadphase.rs:708-770) for the supported shapes.
+template <class T> Aabb<T> geom_aabb(const Geom<T>& g, const Pose<T>& p) {
+    switch (g.shape) {
+        case S_PLANE: return aabb_unbounded<T>();
+        case S_SPHERE: return aabb_center_ext(p.p, V3<T>{g.size.x, g.size.x, g.size.x});
+        case S_BOX: {
+            V3<T> x = vabs(rotate(p.q, v3(T(1), T(0), T(0)))) * g.size.x;
+            V3<T> y = vabs(rotate(p.q, v3(T(0), T(1), T(0)))) * g.size.y;
+            V3<T> z = vabs(rotate(p.q, v3(T(0), T(0), T(1)))) * g.size.z;
+            return aabb_center_ext(p.p, (x + y) + z);
+        }
+        case S_CAPSULE: {
+            V3<T> axis = rotate(p.q, v3(T(0), T(0), T(1)));
+            T r = g.size.x;
+            return aabb_center_ext(p.p, vabs(axis) * g.size.y + V3<T>{r, r, r});
+        }
+    }
+    fail("geom_aabb: unknown shape");
+}
+
+// ---------------------------------------------------------------------------
+// SolRef / SolImp math (geom.rs:156-185, solver.rs:242-384)
+// ---------------------------------------------------------------------------
+
+template <class T> SolRef<T> combine_solref(SolRef<T> a, SolRef<T> b) {  // geom.rs:156-185
+    if (a.is_direct() || b.is_direct()) {
+        if (a.is_direct() && b.is_direct())
+            return {a.timeconst <= b.timeconst ? a.timeconst : b.timeconst,
+                    a.dampratio <= b.dampratio ? a.dampratio : b.dampratio};
+        return a.is_direct() ? a : b;
+    }
+    return {a.timeconst <= b.timeconst ? a.timeconst : b.timeconst,
+            a.dampratio <= b.dampratio ? a.dampratio : b.dampratio};
+}
+template <class T> inline T fmin_lt(T a, T b) { return a < b ? a : b; }  // solver.rs:253
+template <class T> SolImp<T> combine_solimp(SolImp<T> a, SolImp<T> b) {  // solver.rs:242-250
+    SolImp<T> o;
+    o.dmin = fmin_lt(a.dmin, b.dmin);
+    o.dmax = fmin_lt(a.dmax, b.dmax);
+    o.width = fmin_lt(a.width, b.width);
+    o.midpoint = a.midpoint;
+    o.power = a.power;
+    return o;
+}
+template <class T> T powi(T x, uint32_t n) {  // solver.rs:261-268
+    T r = T(1.0);
+    for (uint32_t i = 0; i < n; ++i) r *= x;
+    return r;
+}
+template <class T> T sigmoid(T x, uint32_t power, T mid) {  // solver.rs:273-286
+    if (x <= T(0.0)) return T(0.0);
+    if (x >= T(1.0)) return T(1.0);
+    if (x < mid) return mid * powi(x / mid, power);
+    return T(1.0) - (T(1.0) - mid) * powi((T(1.0) - x) / (T(1.0) - mid), power);
+}
+template <class T> SolImp<T> effective_solimp(SolImp<T> s) {  // solver.rs:319-328
+    const T MN = T(1.0e-4), MX = T(0.9999);
+    SolImp<T> o;
+    o.dmin = rclamp(s.dmin, MN, MX);
+    o.dmax = rclamp(s.dmax, MN, MX);
+    o.width = s.width > T(0.0) ? s.width : T(0.0);
+    o.midpoint = rclamp(s.midpoint, MN, MX);
+    o.power = s.power < 1 ? 1 : s.power;
+    return o;
+}
+template <class T> T impedance_at_position(T position, T margin, SolImp<T> s0) {  // solver.rs:301-309
+    SolImp<T> s = effective_solimp(s0);
+    T x = (position - margin) / s.width;
+    if (x < T(0.0)) x = -x;
+    T y = sigmoid(x, s.power, s.midpoint);
+    return s.dmin + (s.dmax - s.dmin) * y;
+}
+template <class T> SolRef<T> safe_solref(SolRef<T> r, T dt) {  // solver.rs:331-337
+    if (!r.is_direct() && r.timeconst < T(2.0) * dt) return {T(2.0) * dt, r.dampratio};
+    return r;
+}
+template <class T> void reference_coefficients(T position, SolRef<T> r, SolImp<T> s0, T* b, T* k, T* d) {  // solver.rs:363-384
+    SolImp<T> s = effective_solimp(s0);
+    *d = impedance_at_position(position, T(0.0), s);
+    if (r.is_direct()) {
+        *b = -r.dampratio / s.dmax;
+        *k = -r.timeconst / (s.dmax * s.dmax);
+    } else {
+        T tc = r.timeconst, dmax = s.dmax;
+        *b = T(2.0) / (dmax * tc);
+        *k = *d / (dmax * dmax * tc * tc * r.dampratio * r.dampratio);
+    }
+}
+template <class T> T reference_accel(T position, T velocity, SolRef<T> r, SolImp<T> s) {  // solver.rs:353-361
+    T b, k, d;
+    reference_coefficients(position, r, s, &b, &k, &d);
+    return -b * velocity - k * position;
+}
+template <class T> T contact_regularization_from_diag(T base_diag, T imp, T mu, bool pyr3) {  // solver.rs:1961-1974
+    T base = (T(1.0) - imp) / imp * base_diag;
+    return pyr3 ? T(2.0) * mu * mu * base : base;
+}
+
+// tangent_basis (world.rs:3104-3111).
+template <class T> std::pair<V3<T>, V3<T>> tangent_basis(V3<T> n) {
+    bool use_z = rabs(n.z) < T(0.9);
+    V3<T> ref = use_z ? v3(T(0), T(0), T(1)) : v3(T(1), T(0), T(0));
+    V3<T> t1 = normalize(cross(ref, n));
+    return {t1, cross(n, t1)};
+}
+
+// ---------------------------------------------------------------------------
+// Contacts (contact.rs:82-214, 218-403, 716-834, 2509-2806)
+// ---------------------------------------------------------------------------
+
+template <class T> struct Contact {
+    int geom_a, geom_b;
+    int slot;  // index within the pair's ContactBuf (narrow-phase order)
+    V3<T> pos, normal;
+    T penetration, friction, gap;
+};
+
+template <class T> inline T combine_friction(T a, T b) { return a < b ? a : b; }  // contact.rs:218-220
+template <class T> inline T combine_max(T a, T b) { return a > b ? a : b; }       // contact.rs:225-227
+
+// Plane frame: outward normal = local +Z (contact.rs:260-265).
+template <class T> inline void plane_world(const Pose<T>& pp, V3<T>* n, V3<T>* p0) {
+    *n = rotate(pp.q, v3(T(0), T(0), T(1)));
+    *p0 = pp.p;
+}
+
+template <class T>
+void sphere_plane(int ia, const Pose<T>& ps, T radius, T fr, T margin, T gap, int ib, const Pose<T>& pp,
+                  std::vector<Contact<T>>& out) {  // contact.rs:269-301
+    V3<T> n, p0;
+    plane_world(pp, &n, &p0);
+    T signed_d = dot(ps.p - p0, n);
+    T raw = signed_d - radius;
+    if (raw <= margin) {
+        V3<T> cp = ps.p - n * (radius + raw * T(0.5));
+        out.push_back({ia, ib, (int)out.size(), cp, n, margin - raw, fr, gap});
+    }
+}
+
+template <class T>
+void box_plane(int ia, const Pose<T>& pb, V3<T> he, T fr, T margin, T gap, int ib, const Pose<T>& pp,
+               std::vector<Contact<T>>& out) {  // contact.rs:311-362
+    V3<T> n, p0;
+    plane_world(pp, &n, &p0);
+    T dist = dot(pb.p - p0, n);
+    for (int i = 0; i < 8; ++i) {
+        V3<T> local{(i & 1) == 0 ? -he.x : he.x, (i & 2) == 0 ? -he.y : he.y, (i & 4) == 0 ? -he.z : he.z};
+        V3<T> co = rotate(pb.q, local);
+        T ld = dot(n, co);
+        T raw = dist + ld;
+        if (raw > margin || ld > T(0.0)) continue;
+        V3<T> corner = pb.p + co;
+        V3<T> cp = corner - n * (raw * T(0.5));
+        if (out.size() < 4) out.push_back({ia, ib, (int)out.size(), cp, n, margin - raw, fr, gap});
+    }
+}
+
+template <class T>
+void capsule_plane(int ia, const Pose<T>& pc, T radius, T hh, T fr, T margin, T gap, int ib,
+                   const Pose<T>& pp, std::vector<Contact<T>>& out) {  // contact.rs:367-403
+    V3<T> n, p0;
+    plane_world(pp, &n, &p0);
+    V3<T> aw = rotate(pc.q, v3(T(0), T(0), T(1)));
+    V3<T> ends[2] = {pc.p + aw * hh, pc.p - aw * hh};
+    for (int e = 0; e < 2; ++e) {
+        T signed_d = dot(ends[e] - p0, n);
+        T raw = signed_d - radius;
+        if (raw <= margin) {
+            V3<T> cp = ends[e] - n * (radius + raw * T(0.5));
+            out.push_back({ia, ib, (int)out.size(), cp, n, margin - raw, fr, gap});
+        }
+    }
+}
+
+// closest_points_on_segments (contact.rs:2342-2385), clamp01 (2324-2334).
+template <class T> inline T clamp01(T x) { return x < T(0.0) ? T(0.0) : (x > T(1.0) ? T(1.0) : x); }
+template <class T>
+std::pair<V3<T>, V3<T>> closest_points_on_segments(V3<T> a0, V3<T> a1, V3<T> b0, V3<T> b1) {
+    V3<T> d1 = a1 - a0, d2 = b1 - b0, r = a0 - b0;
+    T a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+    const T EPS = T(1.0e-12);
+    if (a <= EPS && e <= EPS) return {a0, b0};
+    if (a <= EPS) return {a0, b0 + d2 * clamp01(f / e)};
+    T c = dot(d1, r);
+    if (e <= EPS) return {a0 + d1 * clamp01(-c / a), b0};
+    T b_ = dot(d1, d2);
+    T denom = a * e - b_ * b_;
+    T s = denom != T(0.0) ? clamp01((b_ * f - c * e) / denom) : T(0.0);
+    T tn = b_ * s + f;
+    T t, sf;
+    if (tn < T(0.0)) { t = T(0.0); sf = clamp01(-c / a); }
+    else if (tn > e) { t = T(1.0); sf = clamp01((b_ - c) / a); }
+    else { t = tn / e; sf = s; }
+    return {a0 + d1 * sf, b0 + d2 * t};
+}
+
+template <class T> struct FV2 {
+    T u, v, depth;
+};
+// sutherland_hodgman_axis_rect (contact.rs:1115-1167).
+template <class T> std::vector<FV2<T>> sutherland_hodgman(const FV2<T> subject[4], T hu, T hv) {
+    auto sgn = [&](int edge, FV2<T> p) {
+        switch (edge) {
+            case 0: return p.u + hu;
+            case 1: return hu - p.u;
+            case 2: return p.v + hv;
+            default: return hv - p.v;
+        }
+    };
+    auto lerp = [](FV2<T> a, FV2<T> b, T t) {
+        return FV2<T>{a.u + (b.u - a.u) * t, a.v + (b.v - a.v) * t, a.depth + (b.depth - a.depth) * t};
+    };
+    std::vector<FV2<T>> out(subject, subject + 4);
+    for (int edge = 0; edge < 4; ++edge) {
+        if (out.empty()) break;
+        std::vector<FV2<T>> in = out;
+        out.clear();
+        size_t n = in.size();
+        for (size_t i = 0; i < n; ++i) {
+            FV2<T> cur = in[i], prev = in[(i + n - 1) % n];
+            T cs = sgn(edge, cur), ps = sgn(edge, prev);
+            bool ci = cs >= T(0.0), pi = ps >= T(0.0);
+            if (ci) {
+                if (!pi) out.push_back(lerp(prev, cur, ps / (ps - cs)));
+                out.push_back(cur);
+            } else if (pi) {
+                out.push_back(lerp(prev, cur, ps / (ps - cs)));
+            }
+        }
+    }
+    return out;
+}
+inline void other_two(int k, int* a, int* b) {
+    if (k == 0) { *a = 1; *b = 2; } else if (k == 1) { *a = 0; *b = 2; } else { *a = 0; *b = 1; }
+}
+// edge_midpoint (contact.rs:1173-1196).
+template <class T> V3<T> edge_midpoint(V3<T> c, const V3<T>* basis, const T* half, int ei, V3<T> dir) {
+    int j, k;
+    other_two(ei, &j, &k);
+    T sj = dot(dir, basis[j]) >= T(0.0) ? T(1.0) : T(-1.0);
+    T sk = dot(dir, basis[k]) >= T(0.0) ? T(1.0) : T(-1.0);
+    return (c + basis[j] * (sj * half[j])) + basis[k] * (sk * half[k]);
+}
+
+// box_box_full_manifold -> box_box_sat_fallback (contact.rs:569-583, 716-868),
+// box_box_edge_edge_contact (872-910), box_box_face_reference_contacts (917-1079).
+template <class T>
+void box_box(int ia, const Pose<T>& pa, V3<T> ha3, int ib, const Pose<T>& pb, V3<T> hb3, T fr, T margin, T gap,
+             std::vector<Contact<T>>& out) {
+    V3<T> ax[3] = {rotate(pa.q, v3(T(1), T(0), T(0))), rotate(pa.q, v3(T(0), T(1), T(0))), rotate(pa.q, v3(T(0), T(0), T(1)))};
+    V3<T> bx[3] = {rotate(pb.q, v3(T(1), T(0), T(0))), rotate(pb.q, v3(T(0), T(1), T(0))), rotate(pb.q, v3(T(0), T(0), T(1)))};
+    T ha[3] = {ha3.x, ha3.y, ha3.z}, hb[3] = {hb3.x, hb3.y, hb3.z};
+    V3<T> delta = pa.p - pb.p;
+    auto proj = [](V3<T> l, const V3<T>* u, const T* h) {
+        return (h[0] * rabs(dot(l, u[0])) + h[1] * rabs(dot(l, u[1]))) + h[2] * rabs(dot(l, u[2]));
+    };
+    T best = std::numeric_limits<T>::infinity();
+    V3<T> best_axis = v3(T(0), T(0), T(1));
+    int kind = 0, k1 = 0, k2 = 0;  // 0 face A, 1 face B, 2 edge-edge
+    auto test = [&](V3<T> l, int knd, int i, int j) {
+        T sgn = dot(delta, l);
+        T ov = (proj(l, ax, ha) + proj(l, bx, hb)) - rabs(sgn);
+        if (ov < -margin) return true;
+        if (ov < best) {
+            best = ov;
+            best_axis = sgn >= T(0.0) ? l : -l;
+            kind = knd; k1 = i; k2 = j;
+        }
+        return false;
+    };
+    for (int k = 0; k < 3; ++k) if (test(ax[k], 0, k, 0)) return;
+    for (int k = 0; k < 3; ++k) if (test(bx[k], 1, k, 0)) return;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            V3<T> raw = cross(ax[i], bx[j]);
+            T len2 = dot(raw, raw);
+            if (len2 < T(1.0e-6)) continue;
+            if (test(raw / std::sqrt(len2), 2, i, j)) return;
+        }
+    if (!std::isfinite(best)) return;
+    T pen_shift = best + margin;
+    if (pen_shift <= T(0.0)) return;
+    V3<T> nw = best_axis;
+    if (kind == 2) {
+        V3<T> ma = edge_midpoint(pa.p, ax, ha, k1, -nw);
+        V3<T> mb = edge_midpoint(pb.p, bx, hb, k2, nw);
+        V3<T> ea0 = ma - ax[k1] * ha[k1], ea1 = ma + ax[k1] * ha[k1];
+        V3<T> eb0 = mb - bx[k2] * hb[k2], eb1 = mb + bx[k2] * hb[k2];
+        V3<T> pbp = closest_points_on_segments(ea0, ea1, eb0, eb1).second;
+        out.push_back({ia, ib, (int)out.size(), pbp, nw, pen_shift, fr, gap});
+        return;
+    }
+    bool ref_a = kind == 0;
+    int rk = k1;
+    V3<T> rc = ref_a ? pa.p : pb.p;
+    const V3<T>* rb = ref_a ? ax : bx;
+    const T* rh = ref_a ? ha : hb;
+    V3<T> ic = ref_a ? pb.p : pa.p;
+    const V3<T>* ibs = ref_a ? bx : ax;
+    const T* ih = ref_a ? hb : ha;
+    V3<T> ron = ref_a ? -nw : nw;
+    V3<T> rfa = rb[rk];
+    T rs = dot(ron, rfa) >= T(0.0) ? T(1.0) : T(-1.0);
+    V3<T> rfc = rc + rfa * (rs * rh[rk]);
+    int ru, rv;
+    other_two(rk, &ru, &rv);
+    int iax = 0;
+    T isg = T(1.0), idot = std::numeric_limits<T>::infinity();
+    for (int k = 0; k < 3; ++k) {
+        const T signs[2] = {T(1.0), T(-1.0)};
+        for (T s : signs) {
+            T d = dot(ibs[k] * s, ron);
+            if (d < idot) { idot = d; iax = k; isg = s; }
+        }
+    }
+    V3<T> ifc = ic + ibs[iax] * (isg * ih[iax]);
+    int iu, iv;
+    other_two(iax, &iu, &iv);
+    V3<T> U = ibs[iu], Vv = ibs[iv];
+    T hu = ih[iu], hv = ih[iv];
+    V3<T> corners[4] = {(ifc - U * hu) - Vv * hv, (ifc + U * hu) - Vv * hv, (ifc + U * hu) + Vv * hv, (ifc - U * hu) + Vv * hv};
+    FV2<T> subj[4];
+    for (int i = 0; i < 4; ++i) {
+        V3<T> rel = corners[i] - rfc;
+        subj[i] = {dot(rel, rb[ru]), dot(rel, rb[rv]), -dot(rel, ron)};
+    }
+    std::vector<FV2<T>> clipped = sutherland_hodgman(subj, rh[ru], rh[rv]);
+    T cpen[8];
+    V3<T> cpos[8];
+    int count = 0;
+    for (const FV2<T>& cv : clipped) {
+        T pen = cv.depth + margin;
+        if (pen <= T(0.0)) continue;
+        V3<T> por = (rfc + rb[ru] * cv.u) + rb[rv] * cv.v;
+        cpen[count] = pen;
+        cpos[count] = ref_a ? por + nw * (pen - margin) : por;
+        if (++count == 8) break;
+    }
+    int order[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+    for (int i = 1; i < count; ++i) {
+        int j = i;
+        while (j > 0 && cpen[order[j]] > cpen[order[j - 1]]) {
+            std::swap(order[j - 1], order[j]);
+            --j;
+        }
+    }
+    int take = count > 4 ? 4 : count;
+    for (int i = 0; i < take; ++i)
+        out.push_back({ia, ib, (int)out.size(), cpos[order[i]], nw, cpen[order[i]], fr, gap});
+}
+
+// One-direction primitive (try_narrow_phase, contact.rs:2592-2806).  Returns
+// false when (a, b) is not a primitive in this order.
+template <class T>
+bool try_narrow(int ia, const Geom<T>& ga, const Pose<T>& pa, int ib, const Geom<T>& gb, const Pose<T>& pb,
+                T fr, T margin, T gap, std::vector<Contact<T>>& out) {
+    if (ga.shape == S_SPHERE && gb.shape == S_PLANE) { sphere_plane(ia, pa, ga.size.x, fr, margin, gap, ib, pb, out); return true; }
+    if (ga.shape == S_BOX && gb.shape == S_PLANE) { box_plane(ia, pa, ga.size, fr, margin, gap, ib, pb, out); return true; }
+    if (ga.shape == S_CAPSULE && gb.shape == S_PLANE) { capsule_plane(ia, pa, ga.size.x, ga.size.y, fr, margin, gap, ib, pb, out); return true; }
+    if (ga.shape == S_BOX && gb.shape == S_BOX) {
+        box_box(ia, pa, ga.size, ib, pb, gb.size, fr, margin, gap, out);
+        return true;
+    }
+    return false;
+}
+
+// dispatch_narrow_phase (contact.rs:2509-2590), FullManifold mode.
+template <class T>
+void narrow_phase(int ia, const Geom<T>& ga, const Pose<T>& pa, int ib, const Geom<T>& gb, const Pose<T>& pb,
+                  std::vector<Contact<T>>& out) {
+    out.clear();
+    T fr = combine_friction(ga.friction, gb.friction);
+    T margin = combine_max(ga.margin, gb.margin);
+    T gap = combine_max(ga.gap, gb.gap);
+    if (try_narrow(ia, ga, pa, ib, gb, pb, fr, margin, gap, out)) return;
+    if (try_narrow(ib, gb, pb, ia, ga, pa, fr, margin, gap, out)) {
+        for (auto& c : out) {
+            std::swap(c.geom_a, c.geom_b);
+            c.normal = -c.normal;
+        }
+        return;
+    }
+    fail("unsupported shape pair");
+}
+
+// is_pair_supported (contact.rs:232-257) for the shapes restated here.
+inline bool pair_supported(int a, int b) {
+    auto one = [](int x, int y) {
+        return (x == S_SPHERE && y == S_PLANE) || (x == S_BOX && y == S_PLANE) || (x == S_CAPSULE && y == S_PLANE) ||
+               (x == S_SPHERE && y == S_SPHERE) || (x == S_SPHERE && y == S_CAPSULE) ||
+               (x == S_CAPSULE && y == S_CAPSULE) || (x == S_BOX && y == S_BOX);
+    };
+    return one(a, b) || one(b, a);
+}
+
+// ---------------------------------------------------------------------------
+// World
+// ---------------------------------------------------------------------------
+
+template <class T> struct World {
+    T dt = T(0.005);
+    V3<T> gravity{T(0.0), T(0.0), T(-9.81)};
+    T penetration_slop = T(0.0);
+    int integrator = I_RK4;
+    int mode = M_PENALTY;
+    uint32_t iterations = 20;
+    int cone = C_PYRAMIDAL;
+    std::vector<Tree<T>> trees;
+    std::vector<Geom<T>> geoms;
+    std::vector<bool> self_collide;  // per tree; false => in disabled_self_collision
+    // Broadphase proxies: fat AABB per geom (DynamicAabbTree leaf bounds).
+    std::vector<Aabb<T>> fat;
+    std::vector<bool> present;
+    // Outputs of the last step.
+    std::vector<std::pair<int, int>> last_pairs;
+    std::vector<Contact<T>> last_contacts;
+    std::vector<T> last_qacc;  // flattened over trees
+    std::vector<AbaWs<T>> ws;
+
+    int add_tree(const Tree<T>& t, bool collide) {
+        trees.push_back(t);
+        self_collide.push_back(collide);
+        ws.emplace_back();
+        return (int)trees.size() - 1;
+    }
+    int add_geom(const Geom<T>& g) {  // world.rs:974-979
+        geoms.push_back(g);
+        refresh_broadphase();
+        return (int)geoms.size() - 1;
+    }
+
+    void tree_poses(std::vector<std::vector<Pose<T>>>& poses) const {
+        poses.resize(trees.size());
+        for (size_t t = 0; t < trees.size(); ++t) forward_kinematics(trees[t], poses[t]);
+    }
+    Pose<T> geom_pose(int gi, const std::vector<std::vector<Pose<T>>>& poses) const {
+        const Geom<T>& g = geoms[gi];
+        if (!g.is_link()) return geom_world_pose(g, vzero<T>(), qidentity<T>());
+        const Pose<T>& lp = poses[g.tree][g.link];
+        return geom_world_pose(g, lp.p, lp.q);
+    }
+
+    // populate_broadphase_tree + swept_geom_bound (world.rs:1254-1358).
+    void populate_broadphase() {
+        std::vector<std::vector<Pose<T>>> poses;
+        tree_poses(poses);
+        std::vector<std::vector<std::pair<V3<T>, V3<T>>>> vel(trees.size());
+        for (size_t t = 0; t < trees.size(); ++t) link_world_velocities(trees[t], poses[t], vel[t]);
+        fat.resize(geoms.size());
+        present.resize(geoms.size(), false);
+        for (size_t gi = 0; gi < geoms.size(); ++gi) {
+            const Geom<T>& g = geoms[gi];
+            Pose<T> start = geom_pose((int)gi, poses);
+            Pose<T> end = start;
+            T ang_speed = T(0.0);
+            V3<T> pivot = start.p;
+            if (g.is_link()) {
+                V3<T> v = vel[g.tree][g.link].first, w = vel[g.tree][g.link].second;
+                const Pose<T>& lp = poses[g.tree][g.link];
+                end = geom_world_pose(g, lp.p + v * dt,
+                                      integrate_body_angular_velocity(lp.q, inverse_rotate(lp.q, w), dt));
+                ang_speed = length(w);
+                pivot = lp.p;
+            }
+            Aabb<T> sb = geom_aabb(g, start), eb = geom_aabb(g, end);
+            V3<T> bc = (sb.mn + sb.mx) * T(0.5);
+            T br = length((sb.mx - sb.mn) * T(0.5));
+            T pr = length(bc - pivot) + br;
+            T sweep = (ang_speed > T(0.0) && std::isfinite(pr)) ? pr * ang_speed * dt : T(0.0);
+            Aabb<T> bound = expanded(expanded(aunion(sb, eb), sweep), g.margin);
+            // update_with_filter_and_tree (broadphase.rs:320-346).
+            if (!present[gi] || !contains(fat[gi], bound)) {
+                fat[gi] = fatten(bound);
+                present[gi] = true;
+            }
+        }
+    }
+    void refresh_broadphase() { populate_broadphase(); }  // world.rs:1119-1136
+
+    // update_broadphase + compute_pairs (world.rs:1190-1226, broadphase.rs:384-455):
+    // all (min,max) pairs whose fat bounds overlap and that pass the filters.
+    std::vector<std::pair<int, int>> active_pairs() {
+        populate_broadphase();
+        std::vector<std::pair<int, int>> out;
+        int n = (int)geoms.size();
+        for (int a = 0; a < n; ++a)
+            for (int b = a + 1; b < n; ++b) {
+                const Geom<T>& ga = geoms[a];
+                const Geom<T>& gb = geoms[b];
+                if (!overlaps(fat[a], fat[b])) continue;
+                if (!((ga.group & gb.mask) != 0 && (gb.group & ga.mask) != 0)) continue;
+                if (ga.is_link() && gb.is_link() && ga.tree == gb.tree && !self_collide[ga.tree]) continue;
+                bool same_att = (ga.is_link() == gb.is_link()) &&
+                                (!ga.is_link() || (ga.tree == gb.tree && ga.link == gb.link));
+                if (same_att) continue;
+                out.push_back({a, b});
+            }
+        return out;
+    }
+
+    // collect_contacts_full (world.rs:2476-2528).
+    std::vector<Contact<T>> detect_contacts(const std::vector<std::pair<int, int>>& pairs) const {
+        std::vector<std::vector<Pose<T>>> poses;
+        tree_poses(poses);
+        std::vector<Pose<T>> gp(geoms.size());
+        for (size_t gi = 0; gi < geoms.size(); ++gi) gp[gi] = geom_pose((int)gi, poses);
+        std::vector<Contact<T>> out, buf;
+        for (auto [a, b] : pairs) {
+            narrow_phase(a, geoms[a], gp[a], b, geoms[b], gp[b], buf);
+            for (auto c : buf) {
+                if (penetration_slop > T(0.0) && c.gap < penetration_slop) c.gap = penetration_slop;
+                out.push_back(c);
+            }
+        }
+        return out;
+    }
+
+    struct Comp {  // WorldTreeJacobian (solver.rs:2768-2774)
+        int tree, link;
+        std::vector<T> coeff;
+        V3<T> wrench_linear;
+    };
+    struct Row {  // WorldContactRow (solver.rs:2746-2752)
+        std::vector<Comp> comps;
+        T reg = T(0.0), diag = T(0.0), bias = T(0.0);
+    };
+
+    // tree_body_invweight0 (solver.rs:3719-3754).
+    std::pair<T, T> body_invweight0(int ti, int link, const std::vector<Pose<T>>& poses, const std::vector<T>& factor) const {
+        const Tree<T>& t = trees[ti];
+        int nv = t.nv();
+        std::vector<V3<T>> jt, jr;
+        point_jacobian(t, poses, link, vzero<T>(), jt, jr);
+        T tr = T(0.0), rot = T(0.0);
+        for (int axis = 0; axis < 3; ++axis) {
+            std::vector<T> lin(nv), ang(nv);
+            for (int s = 0; s < nv; ++s) {
+                lin[s] = axis == 0 ? jt[s].x : axis == 1 ? jt[s].y : jt[s].z;
+                ang[s] = axis == 0 ? jr[s].x : axis == 1 ? jr[s].y : jr[s].z;
+            }
+            std::vector<T> lr = cholesky_solve(factor, nv, lin), ar = cholesky_solve(factor, nv, ang);
+            RSum<T> s1, s2;
+            for (int s = 0; s < nv; ++s) s1.add(lin[s] * lr[s]);
+            for (int s = 0; s < nv; ++s) s2.add(ang[s] * ar[s]);
+            tr += s1.acc;
+            rot += s2.acc;
+        }
+        return {tr / T(3.0), rot / T(3.0)};
+    }
+
+    // solve_tree_contacts (solver.rs:2805-3366), PGS, trees only.
+    // Returns tree_qfrc per tree.
+    std::vector<std::vector<T>> solve_tree_contacts(const std::vector<Contact<T>>& contacts, int tree_implicit /*-1 none,0,1*/) const {
+        std::vector<std::vector<T>> qfrc(trees.size());
+        for (size_t t = 0; t < trees.size(); ++t) qfrc[t].assign(trees[t].nv(), T(0.0));
+        if (contacts.empty() || dt <= T(0.0) || trees.empty()) return qfrc;
+        if (mode == M_NEWTON) fail("Newton tree-contact solve is not restated");
+        if (cone != C_PYRAMIDAL) fail("elliptic cone is not restated");
+        std::vector<std::vector<Pose<T>>> poses;
+        tree_poses(poses);
+        std::vector<std::vector<T>> factors(trees.size()), dfactors(trees.size()), freev(trees.size());
+        for (size_t t = 0; t < trees.size(); ++t) {
+            int n = trees[t].nv();
+            if (n == 0) continue;
+            std::vector<T> rm = tree_implicit < 0 ? mass_matrix(trees[t]) : implicit_mass_matrix(trees[t], dt);
+            if (!cholesky(rm, n, factors[t])) fail("tree contact solve failed: mass matrix is not positive definite");
+            if (!cholesky(mass_matrix(trees[t]), n, dfactors[t])) fail("tree diagApprox failed");
+            // tree_free_velocity_delta (solver.rs:3602-3622).
+            Tree<T> ft = trees[t];
+            ft.disable_penalty_limits = true;
+            std::vector<Pose<T>> fp;
+            forward_kinematics(ft, fp);
+            std::vector<std::pair<V3<T>, V3<T>>> ext(ft.links.size(), {vzero<T>(), vzero<T>()});
+            std::vector<T> qdd;
+            AbaWs<T> tmp;
+            aba_tree(ft, fp, gravity, ext, tree_implicit < 0 ? VI_EXPLICIT : (tree_implicit ? VI_JOINT_DAMPING_ACT : VI_JOINT_DAMPING), dt, tmp, qdd);
+            freev[t].resize(n);
+            for (int s = 0; s < n; ++s) freev[t][s] = qdd[s] * dt;
+        }
+        struct Block {
+            int start;
+            SolRef<T> solref;
+            SolImp<T> solimp;
+            T penetration, mu;
+        };
+        std::vector<Row> rows;
+        std::vector<Block> blocks;
+        for (size_t ci = 0; ci < contacts.size(); ++ci) {
+            const Contact<T>& c = contacts[ci];
+            const Geom<T>& ga = geoms[c.geom_a];
+            const Geom<T>& gb = geoms[c.geom_b];
+            if (!ga.is_link() && !gb.is_link()) continue;
+            T pen = c.penetration - c.gap;
+            if (pen <= T(0.0)) continue;
+            SolRef<T> sr = combine_solref(ga.solref, gb.solref);
+            SolImp<T> si = combine_solimp(ga.solimp, gb.solimp);
+            int condim = ga.condim < gb.condim ? ga.condim : gb.condim;
+            if (condim != 3) fail("condim other than 3 is not restated");
+            T mu = combine_friction(ga.friction, gb.friction);
+            // contact_friction_axes without anisotropy (solver.rs:3441-3465).
+            V3<T> t1 = tangent_basis(c.normal).first;
+            V3<T> t2 = cross(c.normal, t1);
+            V3<T> dirs[4] = {c.normal + t1 * mu, c.normal - t1 * mu, c.normal + t2 * mu, c.normal - t2 * mu};
+            int start = (int)rows.size();
+            for (int k = 0; k < 4; ++k) {
+                // world_jacobian_for_side (solver.rs:3537-3600), side A then
+                // side B; a static side contributes no component.
+                Row r;
+                for (int side = 0; side < 2; ++side) {
+                    const Geom<T>& g = side == 0 ? ga : gb;
+                    T sign = side == 0 ? T(1.0) : T(-1.0);
+                    if (!g.is_link()) continue;
+                    const Pose<T>& lp = poses[g.tree][g.link];
+                    V3<T> local = inverse_rotate(lp.q, c.pos - lp.p);
+                    std::vector<V3<T>> jt, jr;
+                    point_jacobian(trees[g.tree], poses[g.tree], g.link, local, jt, jr);
+                    Comp cp;
+                    cp.tree = g.tree;
+                    cp.link = g.link;
+                    cp.coeff.resize(jt.size());
+                    for (size_t s = 0; s < jt.size(); ++s) cp.coeff[s] = dot(jt[s], dirs[k]) * sign;
+                    cp.wrench_linear = dirs[k] * sign;
+                    r.comps.push_back(cp);
+                }
+                rows.push_back(r);
+            }
+            blocks.push_back({start, sr, si, pen, mu});
+        }
+        if (rows.empty()) return qfrc;
+        int nr = (int)rows.size();
+        std::vector<T> A((size_t)nr * nr, T(0.0));
+        for (int col = 0; col < nr; ++col) {
+            // apply_world_impulse (solver.rs:3799-3836) with impulse 1.
+            std::vector<std::vector<T>> delta(trees.size());
+            for (size_t t = 0; t < trees.size(); ++t) delta[t].assign(trees[t].nv(), T(0.0));
+            for (const Comp& cc : rows[col].comps) {
+                int nv = trees[cc.tree].nv();
+                std::vector<T> gi(nv);
+                for (int s = 0; s < nv; ++s) gi[s] = cc.coeff[s] * T(1.0);
+                std::vector<T> resp = cholesky_solve(factors[cc.tree], nv, gi);
+                for (int s = 0; s < nv; ++s) delta[cc.tree][s] += resp[s];
+            }
+            for (int ri = 0; ri < nr; ++ri) {
+                // world_velocity_from_delta (solver.rs:3687-3714).
+                RSum<T> outer;
+                for (const Comp& rc : rows[ri].comps) {
+                    RSum<T> s;
+                    for (int k = 0; k < trees[rc.tree].nv(); ++k) s.add(rc.coeff[k] * delta[rc.tree][k]);
+                    outer.add(T(0.0) + s.acc);
+                }
+                A[(size_t)ri * nr + col] = outer.acc;
+            }
+        }
+        for (const Block& b : blocks) {
+            for (int off = 0; off < 4; ++off) {
+                int ri = b.start + off;
+                Row& r = rows[ri];
+                T imp = impedance_at_position(-b.penetration, T(0.0), b.solimp);
+                // tree_contact_diag_approx (solver.rs:3756-3796).
+                T tr = T(0.0), rot = T(0.0);
+                for (const Comp& cp : r.comps) {
+                    auto iw = body_invweight0(cp.tree, cp.link, poses[cp.tree], dfactors[cp.tree]);
+                    if (!is_zero_vec(cp.wrench_linear)) tr += iw.first; else rot += iw.second;
+                }
+                (void)rot;
+                T diag_approx = tr * (T(1.0) + b.mu * b.mu);
+                r.reg = contact_regularization_from_diag(diag_approx, imp, b.mu, true);
+                r.diag = A[(size_t)ri * nr + ri] + r.reg;
+                // world_current_velocity / world_free_velocity (solver.rs:3624-3685).
+                RSum<T> vel, fvel;
+                for (const Comp& cp : r.comps) {
+                    const Tree<T>& t = trees[cp.tree];
+                    RSum<T> sv, sf;
+                    for (int s = 0; s < t.nv(); ++s) sv.add(cp.coeff[s] * t.qdot[s]);
+                    for (int s = 0; s < t.nv(); ++s) sf.add(cp.coeff[s] * freev[cp.tree][s]);
+                    vel.add(T(0.0) + sv.acc);
+                    fvel.add(T(0.0) + sf.acc);
+                }
+                T position = -b.penetration;
+                T ref = reference_accel(position, vel.acc, safe_solref(b.solref, dt), b.solimp);
+                r.bias = fvel.acc - ref * dt;
+            }
+        }
+        // PGS sweeps (solver.rs:3269-3331; world_pgs_non_negative 3851-3861).
+        std::vector<T> f(nr, T(0.0));
+        for (uint32_t it = 0; it < iterations; ++it)
+            for (const Block& b : blocks)
+                for (int off = 0; off < 4; ++off) {
+                    int i = b.start + off;
+                    RSum<T> s;
+                    for (int col = 0; col < nr; ++col) s.add(A[(size_t)i * nr + col] * f[col]);
+                    T residual = (rows[i].bias + s.acc) + rows[i].reg * f[i];
+                    f[i] = rmax(f[i] - residual / rows[i].diag, T(0.0));
+                }
+        // Force recovery (solver.rs:3345-3364).
+        for (int i = 0; i < nr; ++i) {
+            T force = f[i] / dt;
+            for (const Comp& cp : rows[i].comps)
+                for (int s = 0; s < trees[cp.tree].nv(); ++s) qfrc[cp.tree][s] += cp.coeff[s] * force;
+        }
+        return qfrc;
+    }
+
+    // World::step (world.rs:1459-1537) for trees.
+    void step() {
+        if (iterations == 0) fail("solver iterations must be >= 1");
+        if (mode == M_NEWTON && cone == C_ELLIPTIC) fail("newton + elliptic");
+        for (size_t a = 0; a < geoms.size(); ++a)
+            for (size_t b = a + 1; b < geoms.size(); ++b)
+                if (!pair_supported(geoms[a].shape, geoms[b].shape)) {
+                    // validate_supported_pairs checks only pairs that can be active.
+                    if (!(geoms[a].shape == S_PLANE && geoms[b].shape == S_PLANE)) fail("unsupported pair");
+                }
+        std::vector<std::pair<int, int>> pairs = active_pairs();
+        last_pairs = pairs;
+        bool constraint = mode == M_PGS || mode == M_NEWTON;
+        std::vector<std::vector<T>> tq;
+        last_contacts.clear();
+        if (constraint) {
+            std::vector<Contact<T>> contacts = detect_contacts(pairs);
+            std::vector<Contact<T>> tc;
+            for (auto& c : contacts)
+                if (geoms[c.geom_a].is_link() || geoms[c.geom_b].is_link()) tc.push_back(c);
+            last_contacts = tc;
+            int ti = integrator == I_RK4 ? -1 : integrator == I_IMPLICITFAST ? 1 : 0;
+            tq = solve_tree_contacts(tc, ti);
+        } else {
+            if (!pairs.empty()) fail("penalty contact forces are not restated");
+        }
+        last_qacc.clear();
+        for (size_t t = 0; t < trees.size(); ++t) {
+            Tree<T>& tr = trees[t];
+            std::vector<T> delta;
+            bool prior = tr.disable_penalty_limits;
+            if (constraint) {
+                for (auto& l : tr.links)
+                    if (l.has_range) fail("joint-limit rows are not restated");
+                delta.assign(tr.nv(), T(0.0));  // solve_tree_limits with no rows
+                for (int s = 0; s < tr.nv(); ++s) delta[s] += tq[t][s];
+                for (int s = 0; s < tr.nv(); ++s) tr.qfrc_applied[s] += delta[s];
+                tr.disable_penalty_limits = true;
+            }
+            std::vector<T> qacc;
+            if (integrator == I_RK4) {
+                rk4_step(tr, gravity, dt, ws[t]);
+            } else {
+                euler_step(tr, gravity, dt, integrator == I_IMPLICITFAST, ws[t], qacc);
+                last_qacc.insert(last_qacc.end(), qacc.begin(), qacc.end());
+            }
+            for (size_t s = 0; s < delta.size(); ++s) tr.qfrc_applied[s] -= delta[s];
+            tr.disable_penalty_limits = prior;
+        }
+        refresh_broadphase();
+    }
+};
+
+}  // namespace oracle
