@@ -529,6 +529,17 @@ template <class T> struct World {
     }
     void refresh_broadphase() { populate_broadphase(); }  // world.rs:1119-1136
 
+    // The state-independent pair filters: different attachment, group/mask
+    // (should_collide, broadphase.rs:16-18) and disabled self collision.
+    bool static_pair_ok(int a, int b) const {
+        const Geom<T>& ga = geoms[a];
+        const Geom<T>& gb = geoms[b];
+        if (!((ga.group & gb.mask) != 0 && (gb.group & ga.mask) != 0)) return false;
+        if (ga.is_link() && gb.is_link() && ga.tree == gb.tree && !self_collide[ga.tree]) return false;
+        bool same_att = (ga.is_link() == gb.is_link()) && (!ga.is_link() || (ga.tree == gb.tree && ga.link == gb.link));
+        return !same_att;
+    }
+
     // update_broadphase + compute_pairs (world.rs:1190-1226, broadphase.rs:384-455):
     // all (min,max) pairs whose fat bounds overlap and that pass the filters.
     std::vector<std::pair<int, int>> active_pairs() {
@@ -537,14 +548,8 @@ template <class T> struct World {
         int n = (int)geoms.size();
         for (int a = 0; a < n; ++a)
             for (int b = a + 1; b < n; ++b) {
-                const Geom<T>& ga = geoms[a];
-                const Geom<T>& gb = geoms[b];
                 if (!overlaps(fat[a], fat[b])) continue;
-                if (!((ga.group & gb.mask) != 0 && (gb.group & ga.mask) != 0)) continue;
-                if (ga.is_link() && gb.is_link() && ga.tree == gb.tree && !self_collide[ga.tree]) continue;
-                bool same_att = (ga.is_link() == gb.is_link()) &&
-                                (!ga.is_link() || (ga.tree == gb.tree && ga.link == gb.link));
-                if (same_att) continue;
+                if (!static_pair_ok(a, b)) continue;
                 out.push_back({a, b});
             }
         return out;
@@ -757,12 +762,11 @@ template <class T> struct World {
     void step() {
         if (iterations == 0) fail("solver iterations must be >= 1");
         if (mode == M_NEWTON && cone == C_ELLIPTIC) fail("newton + elliptic");
+        // assert_pairs_supported over auto_pairs (world.rs:874-898, 930-947, 1042-1069).
         for (size_t a = 0; a < geoms.size(); ++a)
             for (size_t b = a + 1; b < geoms.size(); ++b)
-                if (!pair_supported(geoms[a].shape, geoms[b].shape)) {
-                    // validate_supported_pairs checks only pairs that can be active.
-                    if (!(geoms[a].shape == S_PLANE && geoms[b].shape == S_PLANE)) fail("unsupported pair");
-                }
+                if (static_pair_ok((int)a, (int)b) && !pair_supported(geoms[a].shape, geoms[b].shape))
+                    fail("unsupported pair");
         std::vector<std::pair<int, int>> pairs = active_pairs();
         last_pairs = pairs;
         bool constraint = mode == M_PGS || mode == M_NEWTON;

@@ -1,0 +1,236 @@
+// oracle_capi.cpp — C entry points of the CPU oracle, for ctypes.
+//
+// TEST INFRASTRUCTURE ONLY.  Builds N independent oracle worlds from the same
+// newt_options / newt_link_desc / newt_geom_desc records the product's C ABI
+// takes (include/newt_gpu.h), steps them on host threads, and returns state,
+// qacc, pairs and contacts for comparison with the HIP path.  Two
+// instantiations: precision 0 = double (the FP64 oracle), 1 = float (newt's
+// own arithmetic).
+#include "newt_world.hpp"
+
+#include <atomic>
+#include <chrono>
+#include <memory>
+#include <thread>
+
+#include "../include/newt_gpu.h"
+
+using namespace oracle;
+
+namespace {
+
+struct Base {
+    virtual ~Base() = default;
+    virtual void add_tree(const newt_link_desc* l, int n, int collide) = 0;
+    virtual void add_geom(const newt_geom_desc* g) = 0;
+    virtual void finalize(int n_worlds) = 0;
+    virtual int nq() const = 0;
+    virtual int nv() const = 0;
+    virtual void set_state(const double* qpos, const double* qvel) = 0;
+    virtual void get_state(double* qpos, double* qvel) const = 0;
+    virtual void get_qacc(double* qacc) const = 0;
+    virtual void step(int n_steps, int threads) = 0;
+    virtual int contacts(int world, int cap, int* ids, double* geo) const = 0;
+    virtual int pairs(int world, int cap, int* out) const = 0;
+    virtual void mass_matrix(int world, double* out) const = 0;
+    virtual void dynamics(int world, double* qacc, double* M, double* bias) const = 0;
+};
+
+template <class T> struct Impl : Base {
+    World<T> proto;
+    std::vector<World<T>> worlds;
+
+    explicit Impl(const newt_options& o) {
+        proto.dt = T(o.dt);
+        proto.gravity = v3(T(o.gravity[0]), T(o.gravity[1]), T(o.gravity[2]));
+        proto.penetration_slop = T(o.penetration_slop);
+        proto.integrator = o.integrator;
+        proto.mode = o.solver_mode;
+        proto.iterations = (uint32_t)o.iterations;
+        proto.cone = o.cone;
+    }
+    void add_tree(const newt_link_desc* ld, int n, int collide) override {
+        Tree<T> t;
+        for (int i = 0; i < n; ++i) {
+            const newt_link_desc& d = ld[i];
+            Link<T> l;
+            l.parent = d.parent;
+            l.joint = d.joint;
+            l.axis = v3(T(d.axis[0]), T(d.axis[1]), T(d.axis[2]));
+            l.damping = T(d.damping);
+            l.armature = T(d.armature);
+            l.free_damping = T(d.free_damping);
+            l.has_range = d.has_range != 0;
+            l.range_lo = T(d.range[0]);
+            l.range_hi = T(d.range[1]);
+            l.pj_pos = v3(T(d.parent_pos[0]), T(d.parent_pos[1]), T(d.parent_pos[2]));
+            l.pj_quat = Q<T>{T(d.parent_quat[0]), T(d.parent_quat[1]), T(d.parent_quat[2]), T(d.parent_quat[3])};
+            l.cj_pos = v3(T(d.child_pos[0]), T(d.child_pos[1]), T(d.child_pos[2]));
+            l.cj_quat = Q<T>{T(d.child_quat[0]), T(d.child_quat[1]), T(d.child_quat[2]), T(d.child_quat[3])};
+            l.mass = T(d.mass);
+            for (int k = 0; k < 9; ++k) l.inertia.d[k] = T(d.inertia[k]);
+            t.push_link(l);
+        }
+        proto.add_tree(t, collide != 0);
+    }
+    void add_geom(const newt_geom_desc* d) override {
+        Geom<T> g;
+        g.shape = d->shape;
+        g.tree = d->tree;
+        g.link = d->link;
+        g.size = v3(T(d->size[0]), T(d->size[1]), T(d->size[2]));
+        g.pos = v3(T(d->pos[0]), T(d->pos[1]), T(d->pos[2]));
+        g.quat = Q<T>{T(d->quat[0]), T(d->quat[1]), T(d->quat[2]), T(d->quat[3])};
+        g.friction = T(d->friction);
+        g.margin = T(d->margin);
+        g.gap = T(d->gap);
+        g.solref = {T(d->solref[0]), T(d->solref[1])};
+        g.solimp.dmin = T(d->solimp[0]);
+        g.solimp.dmax = T(d->solimp[1]);
+        g.solimp.width = T(d->solimp[2]);
+        g.solimp.midpoint = T(d->solimp[3]);
+        g.solimp.power = (uint32_t)d->solimp_power;
+        g.condim = d->condim;
+        g.group = d->group;
+        g.mask = d->mask;
+        proto.add_geom(g);
+    }
+    void finalize(int n) override { worlds.assign(n, proto); }
+    int nq() const override {
+        int s = 0;
+        for (auto& t : proto.trees) s += t.nq();
+        return s;
+    }
+    int nv() const override {
+        int s = 0;
+        for (auto& t : proto.trees) s += t.nv();
+        return s;
+    }
+    void set_state(const double* qpos, const double* qvel) override {
+        int NQ = nq(), NV = nv();
+        for (size_t w = 0; w < worlds.size(); ++w) {
+            int oq = 0, ov = 0;
+            for (auto& t : worlds[w].trees) {
+                for (int k = 0; k < t.nq(); ++k) t.q[k] = T(qpos[w * NQ + oq + k]);
+                for (int k = 0; k < t.nv(); ++k) t.qdot[k] = T(qvel[w * NV + ov + k]);
+                oq += t.nq();
+                ov += t.nv();
+            }
+            // Public state setters refresh the query proxies (world.rs:1119-1136).
+            worlds[w].refresh_broadphase();
+        }
+    }
+    void get_state(double* qpos, double* qvel) const override {
+        int NQ = nq(), NV = nv();
+        for (size_t w = 0; w < worlds.size(); ++w) {
+            int oq = 0, ov = 0;
+            for (auto& t : worlds[w].trees) {
+                for (int k = 0; k < t.nq(); ++k) qpos[w * NQ + oq + k] = (double)t.q[k];
+                for (int k = 0; k < t.nv(); ++k) qvel[w * NV + ov + k] = (double)t.qdot[k];
+                oq += t.nq();
+                ov += t.nv();
+            }
+        }
+    }
+    void get_qacc(double* qacc) const override {
+        int NV = nv();
+        for (size_t w = 0; w < worlds.size(); ++w)
+            for (int k = 0; k < NV; ++k)
+                qacc[w * NV + k] = k < (int)worlds[w].last_qacc.size() ? (double)worlds[w].last_qacc[k] : 0.0;
+    }
+    void step(int n_steps, int threads) override {
+        int n = (int)worlds.size();
+        if (threads <= 1 || n <= 1) {
+            for (int w = 0; w < n; ++w)
+                for (int s = 0; s < n_steps; ++s) worlds[w].step();
+            return;
+        }
+        std::atomic<int> next{0};
+        std::vector<std::thread> pool;
+        for (int k = 0; k < threads; ++k)
+            pool.emplace_back([&]() {
+                for (;;) {
+                    int w = next.fetch_add(1);
+                    if (w >= n) break;
+                    for (int s = 0; s < n_steps; ++s) worlds[w].step();
+                }
+            });
+        for (auto& th : pool) th.join();
+    }
+    int contacts(int world, int cap, int* ids, double* geo) const override {
+        const auto& cs = worlds[world].last_contacts;
+        int n = (int)cs.size();
+        for (int k = 0; k < n && k < cap; ++k) {
+            ids[3 * k] = cs[k].geom_a;
+            ids[3 * k + 1] = cs[k].geom_b;
+            ids[3 * k + 2] = cs[k].slot;
+            double* g = geo + 7 * k;
+            g[0] = cs[k].pos.x; g[1] = cs[k].pos.y; g[2] = cs[k].pos.z;
+            g[3] = cs[k].normal.x; g[4] = cs[k].normal.y; g[5] = cs[k].normal.z;
+            g[6] = cs[k].penetration;
+        }
+        return n;
+    }
+    int pairs(int world, int cap, int* out) const override {
+        const auto& ps = worlds[world].last_pairs;
+        int n = (int)ps.size();
+        for (int k = 0; k < n && k < cap; ++k) {
+            out[2 * k] = ps[k].first;
+            out[2 * k + 1] = ps[k].second;
+        }
+        return n;
+    }
+    // Explicit ABA (no external wrench), CRB mass matrix and RNE bias of
+    // tree 0 at the world's current state.
+    void dynamics(int world, double* qacc, double* Mout, double* bias) const override {
+        const Tree<T>& t = worlds[world].trees[0];
+        std::vector<Pose<T>> poses;
+        forward_kinematics(t, poses);
+        std::vector<std::pair<V3<T>, V3<T>>> ext(t.links.size(), {vzero<T>(), vzero<T>()});
+        AbaWs<T> ws;
+        std::vector<T> a;
+        aba_tree(t, poses, worlds[world].gravity, ext, VI_EXPLICIT, worlds[world].dt, ws, a);
+        std::vector<T> m = oracle::mass_matrix(t);
+        std::vector<T> z(t.nv(), T(0));
+        std::vector<T> h = inverse_dynamics(t, z, worlds[world].gravity, ext);
+        for (int k = 0; k < t.nv(); ++k) { qacc[k] = (double)a[k]; bias[k] = (double)h[k]; }
+        for (size_t k = 0; k < m.size(); ++k) Mout[k] = (double)m[k];
+    }
+    void mass_matrix(int world, double* out) const override {
+        const Tree<T>& t = worlds[world].trees[0];
+        std::vector<T> m = oracle::mass_matrix(t);
+        for (size_t k = 0; k < m.size(); ++k) out[k] = (double)m[k];
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+void* oracle_create(const newt_options* o, int precision) {
+    if (precision == 1) return new Impl<float>(*o);
+    return new Impl<double>(*o);
+}
+void oracle_destroy(void* h) { delete static_cast<Base*>(h); }
+void oracle_add_tree(void* h, const newt_link_desc* l, int n, int collide) { static_cast<Base*>(h)->add_tree(l, n, collide); }
+void oracle_add_geom(void* h, const newt_geom_desc* g) { static_cast<Base*>(h)->add_geom(g); }
+void oracle_finalize(void* h, int n_worlds) { static_cast<Base*>(h)->finalize(n_worlds); }
+int oracle_nq(void* h) { return static_cast<Base*>(h)->nq(); }
+int oracle_nv(void* h) { return static_cast<Base*>(h)->nv(); }
+void oracle_set_state(void* h, const double* qpos, const double* qvel) { static_cast<Base*>(h)->set_state(qpos, qvel); }
+void oracle_get_state(void* h, double* qpos, double* qvel) { static_cast<Base*>(h)->get_state(qpos, qvel); }
+void oracle_get_qacc(void* h, double* qacc) { static_cast<Base*>(h)->get_qacc(qacc); }
+void oracle_step(void* h, int n_steps, int threads) { static_cast<Base*>(h)->step(n_steps, threads); }
+int oracle_contacts(void* h, int world, int cap, int* ids, double* geo) { return static_cast<Base*>(h)->contacts(world, cap, ids, geo); }
+int oracle_pairs(void* h, int world, int cap, int* out) { return static_cast<Base*>(h)->pairs(world, cap, out); }
+void oracle_mass_matrix(void* h, int world, double* out) { static_cast<Base*>(h)->mass_matrix(world, out); }
+void oracle_dynamics(void* h, int world, double* qacc, double* M, double* bias) {
+    static_cast<Base*>(h)->dynamics(world, qacc, M, bias);
+}
+
+// Deterministic sin/cos of the oracle, for the kernel's polynomial check.
+double oracle_nsin64(double x) { return nsin(x); }
+double oracle_ncos64(double x) { return ncos(x); }
+float oracle_nsin32(float x) { return nsin(x); }
+float oracle_ncos32(float x) { return ncos(x); }
+}

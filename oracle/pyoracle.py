@@ -1,0 +1,143 @@
+"""ctypes wrapper of oracle/_build/liboracle.so.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  Never imported by tooling_amd.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+
+
+def build():
+    """Compile the oracle with its Makefile (IEEE-strict g++)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} is missing: run `make -C oracle`")
+        L = C.CDLL(LIB)
+        P, I, PD, PI = C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)
+        sig = {
+            "oracle_create": (P, [P, I]),
+            "oracle_destroy": (None, [P]),
+            "oracle_add_tree": (None, [P, P, I, I]),
+            "oracle_add_geom": (None, [P, P]),
+            "oracle_finalize": (None, [P, I]),
+            "oracle_nq": (I, [P]),
+            "oracle_nv": (I, [P]),
+            "oracle_set_state": (None, [P, PD, PD]),
+            "oracle_get_state": (None, [P, PD, PD]),
+            "oracle_get_qacc": (None, [P, PD]),
+            "oracle_step": (None, [P, I, I]),
+            "oracle_contacts": (I, [P, I, I, PI, PD]),
+            "oracle_pairs": (I, [P, I, I, PI]),
+            "oracle_mass_matrix": (None, [P, I, PD]),
+            "oracle_dynamics": (None, [P, I, PD, PD, PD]),
+            "oracle_golden_bytes": (I, [C.c_char_p, C.POINTER(C.c_uint8), I]),
+            "oracle_nsin64": (C.c_double, [C.c_double]),
+            "oracle_ncos64": (C.c_double, [C.c_double]),
+            "oracle_nsin32": (C.c_float, [C.c_float]),
+            "oracle_ncos32": (C.c_float, [C.c_float]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def golden_bytes(name: str) -> bytes:
+    buf = (C.c_uint8 * 65536)()
+    n = lib().oracle_golden_bytes(name.encode(), buf, len(buf))
+    if n < 0:
+        raise KeyError(name)
+    return bytes(buf[:n])
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class OracleWorlds:
+    """N oracle worlds built from a tooling_amd.scenes.Scene.
+    precision: 0 = double (FP64 oracle), 1 = float (newt's arithmetic)."""
+
+    def __init__(self, scene, n_worlds, precision=0):
+        L = lib()
+        self.L = L
+        self.n = n_worlds
+        opts = scene.options_struct()
+        self.h = L.oracle_create(C.byref(opts), precision)
+        links = scene.link_array()
+        L.oracle_add_tree(self.h, C.cast(links, C.c_void_p), len(scene.links), scene.self_collide)
+        for g in scene.geom_structs():
+            L.oracle_add_geom(self.h, C.cast(C.pointer(g), C.c_void_p))
+        L.oracle_finalize(self.h, n_worlds)
+        self.nq = L.oracle_nq(self.h)
+        self.nv = L.oracle_nv(self.h)
+
+    def set_state(self, qpos, qvel):
+        qpos = np.ascontiguousarray(qpos, dtype=np.float64)
+        qvel = np.ascontiguousarray(qvel, dtype=np.float64)
+        self.L.oracle_set_state(self.h, _dp(qpos), _dp(qvel))
+
+    def state(self):
+        qpos = np.zeros((self.n, self.nq))
+        qvel = np.zeros((self.n, self.nv))
+        self.L.oracle_get_state(self.h, _dp(qpos), _dp(qvel))
+        return qpos, qvel
+
+    def qacc(self):
+        out = np.zeros((self.n, self.nv))
+        self.L.oracle_get_qacc(self.h, _dp(out))
+        return out
+
+    def step(self, n_steps=1, threads=1):
+        self.L.oracle_step(self.h, n_steps, threads)
+
+    def contacts(self, world, cap=4096):
+        ids = np.zeros((cap, 3), dtype=np.int32)
+        geo = np.zeros((cap, 7))
+        n = self.L.oracle_contacts(self.h, world, cap, ids.ctypes.data_as(C.POINTER(C.c_int)), _dp(geo))
+        return ids[:n].copy(), geo[:n].copy()
+
+    def pairs(self, world, cap=65536):
+        out = np.zeros((cap, 2), dtype=np.int32)
+        n = self.L.oracle_pairs(self.h, world, cap, out.ctypes.data_as(C.POINTER(C.c_int)))
+        return out[:n].copy()
+
+    def mass_matrix(self, world):
+        out = np.zeros((self.nv, self.nv))
+        self.L.oracle_mass_matrix(self.h, world, _dp(out))
+        return out
+
+    def dynamics(self, world):
+        """(explicit ABA qacc, CRB mass matrix, RNE bias) at the current state."""
+        a = np.zeros(self.nv)
+        M = np.zeros((self.nv, self.nv))
+        h = np.zeros(self.nv)
+        self.L.oracle_dynamics(self.h, world, _dp(a), _dp(M), _dp(h))
+        return a, M, h
+
+    def close(self):
+        if self.h:
+            self.L.oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

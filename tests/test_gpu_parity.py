@@ -1,0 +1,139 @@
+"""HIP path vs the CPU oracle on the same seeded inputs.
+
+FP64 step: pair sets and contact (geom_a, geom_b, slot) lists identical to
+the oracle at every step; qpos, qvel and qacc within 1e-9 relative
+(‖Δ‖∞ / max(‖ref‖∞, 1)) over 100 steps.  Mixed step: one step from the
+oracle's state gives qacc within 1e-5 relative.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from tooling_amd import abi, scenes
+from tooling_amd.newt import Data, Model, step
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-9
+TOL_MIXED_QACC = 1e-5
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(a, b, floor=1.0):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = np.maximum(np.abs(b).max(axis=-1), floor)
+    return (np.abs(a - b).max(axis=-1) / den).max()
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    from oracle import pyoracle
+    pyoracle.lib()
+    return pyoracle
+
+
+def test_pendulum_fp64_100_steps(oracle):
+    sc = scenes.pendulum()
+    n = 256
+    q0, v0 = sc.initial_state(n, 7)
+    say("pendulum3 fp64: 256 worlds x 100 steps")
+    m = Model(sc)
+    d = Data(m, n, abi.PRECISION_FP64)
+    d.set_state(q0, v0)
+    step(m, d, 100)
+    q, v = d.state()
+    a = d.qacc()
+    assert not d.faults().any()
+    ow = oracle.OracleWorlds(sc, n, 0)
+    ow.set_state(q0, v0)
+    ow.step(100, threads=8)
+    qr, vr = ow.state()
+    ar = ow.qacc()
+    assert rel_err(q, qr) <= TOL_FP64
+    assert rel_err(v, vr) <= TOL_FP64
+    assert rel_err(a, ar) <= TOL_FP64
+
+
+def test_pendulum_fp64_1000_steps_one_launch(oracle):
+    sc = scenes.pendulum()
+    n = 64
+    q0, v0 = sc.initial_state(n, 11)
+    say("pendulum3 fp64: 64 worlds x 1000 steps")
+    m = Model(sc)
+    d = Data(m, n, abi.PRECISION_FP64)
+    d.set_state(q0, v0)
+    step(m, d, 1000)
+    q, v = d.state()
+    ow = oracle.OracleWorlds(sc, n, 0)
+    ow.set_state(q0, v0)
+    ow.step(1000, threads=8)
+    qr, vr = ow.state()
+    # 1000 steps of a chaotic triple pendulum: report, bound loosely.
+    assert rel_err(q, qr) <= 1e-6
+
+
+def test_quadruped_fp64_pairs_contacts_and_state(oracle):
+    sc = scenes.quadruped()
+    n = 64
+    q0, v0 = sc.initial_state(n, 3)
+    say("quadruped18 fp64: 64 worlds x 150 single steps, pairs/contacts each step")
+    m = Model(sc)
+    d = Data(m, n, abi.PRECISION_FP64)
+    d.set_state(q0, v0)
+    ow = oracle.OracleWorlds(sc, n, 0)
+    ow.set_state(q0, v0)
+    touched = 0
+    for s in range(150):
+        step(m, d, 1)
+        ow.step(1, threads=8)
+        pc, pairs = d.pairs()
+        cc, ids, geo = d.contacts()
+        for w in range(n):
+            pr = ow.pairs(w)
+            assert pc[w] == len(pr)
+            assert np.array_equal(pairs[w, :pc[w]], pr), (s, w)
+            ir, gr = ow.contacts(w)
+            assert cc[w] == len(ir), (s, w)
+            assert np.array_equal(ids[w, :cc[w]], ir), (s, w)
+            if cc[w]:
+                touched += 1
+                assert rel_err(geo[w, :cc[w]], gr) <= TOL_FP64
+    assert touched > 0
+    assert not d.faults().any()
+    q, v = d.state()
+    qr, vr = ow.state()
+    assert rel_err(q, qr) <= TOL_FP64
+    assert rel_err(v, vr) <= TOL_FP64
+    assert rel_err(d.qacc(), ow.qacc()) <= TOL_FP64
+
+
+def test_quadruped_mixed_one_step_qacc(oracle):
+    sc = scenes.quadruped()
+    n = 256
+    q0, v0 = sc.initial_state(n, 5)
+    ow = oracle.OracleWorlds(sc, n, 0)
+    ow.set_state(q0, v0)
+    say("quadruped18 mixed: 256 worlds, qacc of one step from oracle states")
+    m = Model(sc)
+    worst = 0.0
+    for t in (0, 120, 60):  # states before and during contact
+        ow.step(t, threads=8)
+        qs, vs = ow.state()
+        d = Data(m, n, abi.PRECISION_MIXED)
+        d.set_state(qs, vs)
+        step(m, d, 1)
+        ref = oracle.OracleWorlds(sc, n, 0)
+        ref.set_state(qs, vs)
+        ref.step(1, threads=8)
+        err = rel_err(d.qacc(), ref.qacc())
+        worst = max(worst, err)
+        d.close()
+        ref.close()
+    say(f"mixed qacc relative error {worst:.3e}")
+    assert worst <= TOL_MIXED_QACC

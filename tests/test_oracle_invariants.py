@@ -1,0 +1,96 @@
+"""Pins of the oracle beyond the golden files: closed forms and identities
+that a dropped term, a wrong sign or a transposed operand would break.
+
+* newt's sin/cos polynomial against the true functions (math.rs:45-134).
+* ABA (aba.rs) vs CRB mass matrix (dynamics.rs:474) vs RNE bias
+  (dynamics.rs:232): M·qacc_ABA + h = 0 with no applied force, M symmetric
+  positive definite — three independent algorithms must agree.
+* single hinge pendulum: M = m l² + I_xx and qacc = -m g l sin(q) / M.
+* semi-implicit Euler of a free root under gravity: v_n = g n dt,
+  z_n = z_0 + g dt² n(n+1)/2 (the position update uses the new velocity).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tooling_amd import abi, scenes
+
+
+def test_sin_cos_polynomial(oracle_lib):
+    L = oracle_lib.lib()
+    xs = np.linspace(-10.0, 10.0, 20001)
+    es = max(abs(L.oracle_nsin64(x) - math.sin(x)) for x in xs)
+    ec = max(abs(L.oracle_ncos64(x) - math.cos(x)) for x in xs)
+    # degree-7/8 Taylor on |r| <= pi/4 (3.1e-7) plus newt's pi/2 split
+    # constants (c1+c2+c3 differs from pi/2 by 4.8e-8 per quadrant).
+    assert es < 1e-6 and ec < 1e-6
+    # exact at zero; odd/even symmetry
+    assert L.oracle_nsin64(0.0) == 0.0 and L.oracle_ncos64(0.0) == 1.0
+    for x in (0.3, 1.7, 4.2):
+        assert L.oracle_nsin64(-x) == -L.oracle_nsin64(x)
+        assert L.oracle_ncos64(-x) == L.oracle_ncos64(x)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_aba_crb_rne_agree_on_quadruped(oracle_lib, seed):
+    sc = scenes.quadruped()
+    n = 4
+    q, v = sc.initial_state(n, seed)
+    rng = np.random.default_rng(seed)
+    v = rng.normal(0.0, 1.0, size=v.shape)
+    # random unit root orientation
+    quat = rng.normal(size=(n, 4))
+    q[:, 3:7] = quat / np.linalg.norm(quat, axis=1, keepdims=True)
+    ow = oracle_lib.OracleWorlds(sc, n, 0)
+    ow.set_state(q, v)
+    for w in range(n):
+        a, M, h = ow.dynamics(w)
+        assert np.abs(M - M.T).max() <= 1e-12 * np.abs(M).max()
+        assert np.linalg.eigvalsh(M).min() > 0.0
+        resid = M @ a + h
+        assert np.abs(resid).max() <= 1e-9 * max(1.0, np.abs(h).max())
+
+
+def _single_pendulum(l, m, ixx):
+    links = [scenes.link(-1, abi.JOINT_FIXED),
+             scenes.link(0, abi.JOINT_HINGE, axis=(1.0, 0.0, 0.0), child_pos=(0.0, 0.0, l), mass=m,
+                         inertia=(ixx, 0, 0, 0, ixx, 0, 0, 0, 0.01))]
+    opts = dict(dt=0.001, gravity=(0.0, 0.0, -9.81), integrator=abi.INTEGRATOR_EULER,
+                solver=abi.SOLVER_PGS, iterations=1)
+    return scenes.Scene("single", opts, links, [], 0, None)
+
+
+def test_single_pendulum_closed_form(oracle_lib):
+    l, m, ixx = 0.7, 1.3, 0.05
+    sc = _single_pendulum(l, m, ixx)
+    angles = np.array([[-2.5], [-0.4], [0.0], [0.3], [1.2], [3.0]])
+    ow = oracle_lib.OracleWorlds(sc, len(angles), 0)
+    ow.set_state(angles, np.zeros_like(angles))
+    L = oracle_lib.lib()
+    for w, th in enumerate(angles[:, 0]):
+        a, M, h = ow.dynamics(w)
+        assert M[0, 0] == pytest.approx(m * l * l + ixx, rel=1e-14)
+        # COM hangs at -l (child anchor above the COM); gravity torque about
+        # +X.  newt forms sin(q) from the half angle through the rotation
+        # matrix (aba.rs:762-781): sin q = 2 sin(q/2) cos(q/2).
+        sin_q = 2.0 * L.oracle_nsin64(th * 0.5) * L.oracle_ncos64(th * 0.5)
+        expected = -m * 9.81 * l * sin_q / (m * l * l + ixx)
+        assert a[0] == pytest.approx(expected, rel=1e-12, abs=1e-14)
+
+
+def test_free_fall_semi_implicit_euler(oracle_lib):
+    links = [scenes.link(-1, abi.JOINT_FREE, parent_pos=(0.0, 0.0, 10.0), mass=2.0,
+                         inertia=(0.1, 0, 0, 0, 0.1, 0, 0, 0, 0.1))]
+    dt, g, n = 0.002, -9.81, 250
+    opts = dict(dt=dt, gravity=(0.0, 0.0, g), integrator=abi.INTEGRATOR_EULER, solver=abi.SOLVER_PGS,
+                iterations=1)
+    sc = scenes.Scene("fall", opts, links, [], 0, None)
+    ow = oracle_lib.OracleWorlds(sc, 1, 0)
+    q0 = np.array([[0.0, 0.0, 10.0, 0.0, 0.0, 0.0, 1.0]])
+    ow.set_state(q0, np.zeros((1, 6)))
+    ow.step(n)
+    q, v = ow.state()
+    assert v[0, 5] == pytest.approx(g * dt * n, rel=1e-12)
+    assert q[0, 2] == pytest.approx(10.0 + g * dt * dt * n * (n + 1) / 2, rel=1e-12)
+    assert np.allclose(q[0, 3:7], [0, 0, 0, 1])

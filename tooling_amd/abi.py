@@ -1,0 +1,138 @@
+"""ctypes mirror of include/newt_gpu.h.
+
+The library is loaded from the package's _build directory; a missing
+library raises instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libnewt_gpu.so")
+
+# newt_status
+OK = 0
+STATUS_NAMES = {
+    0: "OK", 1: "INVALID_CONFIG", 2: "UNSUPPORTED_PAIR", 3: "NOT_PD", 4: "NEWTON_FAIL",
+    5: "OOM", 6: "DEVICE", 7: "INVALID_ARG", 8: "CAPACITY", 9: "UNSUPPORTED",
+}
+JOINT_FREE, JOINT_FIXED, JOINT_HINGE, JOINT_SLIDE, JOINT_BALL = range(5)
+SHAPE_PLANE, SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE = range(4)
+INTEGRATOR_EULER, INTEGRATOR_IMPLICITFAST, INTEGRATOR_RK4 = range(3)
+SOLVER_PENALTY, SOLVER_PGS, SOLVER_NEWTON = range(3)
+CONE_PYRAMIDAL, CONE_ELLIPTIC = range(2)
+PRECISION_FP64, PRECISION_MIXED = range(2)
+
+D3 = C.c_double * 3
+D4 = C.c_double * 4
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double),
+        ("gravity", D3),
+        ("penetration_slop", C.c_double),
+        ("integrator", C.c_int32),
+        ("solver_mode", C.c_int32),
+        ("iterations", C.c_int32),
+        ("cone", C.c_int32),
+    ]
+
+
+class LinkDesc(C.Structure):
+    _fields_ = [
+        ("parent", C.c_int32),
+        ("joint", C.c_int32),
+        ("axis", D3),
+        ("damping", C.c_double),
+        ("armature", C.c_double),
+        ("free_damping", C.c_double),
+        ("has_range", C.c_int32),
+        ("pad0", C.c_int32),
+        ("range", C.c_double * 2),
+        ("parent_pos", D3),
+        ("parent_quat", D4),
+        ("child_pos", D3),
+        ("child_quat", D4),
+        ("mass", C.c_double),
+        ("inertia", C.c_double * 9),
+    ]
+
+
+class GeomDesc(C.Structure):
+    _fields_ = [
+        ("shape", C.c_int32),
+        ("tree", C.c_int32),
+        ("link", C.c_int32),
+        ("condim", C.c_int32),
+        ("size", D3),
+        ("pos", D3),
+        ("quat", D4),
+        ("friction", C.c_double),
+        ("margin", C.c_double),
+        ("gap", C.c_double),
+        ("solref", C.c_double * 2),
+        ("solimp", C.c_double * 4),
+        ("solimp_power", C.c_int32),
+        ("group", C.c_uint32),
+        ("mask", C.c_uint32),
+        ("pad0", C.c_int32),
+    ]
+
+
+# Every symbol include/newt_gpu.h declares, with (restype, argtypes).
+_P = C.c_void_p
+_I = C.c_int32
+_PI = C.POINTER(C.c_int32)
+_PD = C.POINTER(C.c_double)
+SIGNATURES = {
+    "newt_last_error": (C.c_char_p, []),
+    "newt_model_create": (_I, [C.POINTER(Options), C.POINTER(_P)]),
+    "newt_model_add_tree": (_I, [_P, C.POINTER(LinkDesc), _I, _I, _PI]),
+    "newt_model_add_geom": (_I, [_P, C.POINTER(GeomDesc), _PI]),
+    "newt_model_finalize": (_I, [_P]),
+    "newt_model_dims": (_I, [_P, _PI, _PI, _PI, _PI]),
+    "newt_model_destroy": (None, [_P]),
+    "newt_data_create": (_I, [_P, _I, _I, C.POINTER(_P)]),
+    "newt_data_destroy": (None, [_P]),
+    "newt_data_set_state": (_I, [_P, _PD, _PD]),
+    "newt_data_get_state": (_I, [_P, _PD, _PD]),
+    "newt_data_get_qacc": (_I, [_P, _PD]),
+    "newt_data_get_faults": (_I, [_P, _PI]),
+    "newt_data_contact_capacity": (_I, [_P, _PI]),
+    "newt_data_get_contacts": (_I, [_P, _PI, _PI, _PD]),
+    "newt_data_pair_capacity": (_I, [_P, _PI]),
+    "newt_data_get_pairs": (_I, [_P, _PI, _PI]),
+    "newt_step": (_I, [_P, _P, _I, _P]),
+    "newt_data_synchronize": (_I, [_P]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libnewt_gpu.so and declare its signatures; raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build the HIP extension first (python -m tooling_amd.build)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class NewtError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def check(status: int):
+    if status != OK:
+        msg = load_library().newt_last_error()
+        raise NewtError(status, msg.decode() if msg else "")

@@ -1,0 +1,219 @@
+// newt_math.hpp — scalar, vector, quaternion and spatial algebra shared by the
+// host model compiler and the gfx950 kernels.
+//
+// Templated on the arithmetic type R (double for the FP64 step, float for the
+// FP32 bulk of the mixed step).  Operation order follows newt's expression
+// trees (newt/src/math.rs, newt/src/spatial.rs, newt/src/aba.rs) so the FP64
+// kernel reproduces the CPU step's rounding; the kernels are compiled with
+// FP contraction off for the same reason.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NEWT_HD __host__ __device__ __forceinline__
+
+namespace newt {
+
+template <class R> NEWT_HD R rsqrt_(R x);
+template <> NEWT_HD double rsqrt_<double>(double x) { return __builtin_sqrt(x); }
+template <> NEWT_HD float rsqrt_<float>(float x) { return __builtin_sqrtf(x); }
+template <class R> NEWT_HD R nsqrt(R x) { return rsqrt_<R>(x); }
+template <class R> NEWT_HD R nabs(R x) { return __builtin_fabs(x); }
+template <> NEWT_HD float nabs<float>(float x) { return __builtin_fabsf(x); }
+// IEEE maxNum / minNum (newt's f32::max / f32::min ignore a NaN operand).
+template <class R> NEWT_HD R nmax(R a, R b) { return __builtin_fmax(a, b); }
+template <class R> NEWT_HD R nmin(R a, R b) { return __builtin_fmin(a, b); }
+template <> NEWT_HD float nmax<float>(float a, float b) { return __builtin_fmaxf(a, b); }
+template <> NEWT_HD float nmin<float>(float a, float b) { return __builtin_fminf(a, b); }
+template <class R> NEWT_HD bool nfinite(R x) { return x == x && x - x == R(0); }
+
+// `x as i32`: truncation, saturating, NaN -> 0 (newt/src/math.rs:126-130).
+template <class R> NEWT_HD int32_t sat_i32(R x) {
+    if (x != x) return 0;
+    if (x >= R(2147483647.0)) return 2147483647;
+    if (x <= R(-2147483648.0)) return -2147483647 - 1;
+    return (int32_t)x;
+}
+
+// Deterministic sin/cos: Cody-Waite reduction by pi/2 with newt's three
+// constants, then degree-7 / degree-8 Taylor polynomials in Horner form
+// (newt/src/math.rs:45-134).  Device libm is never used: the polynomial is
+// part of the step's definition.
+template <class R> NEWT_HD R poly_sin(R x) {
+    R x2 = x * x;
+    R a7 = R(-1.0) / R(5040.0), a5 = R(1.0) / R(120.0), a3 = R(-1.0) / R(6.0);
+    return x * (R(1.0) + x2 * (a3 + x2 * (a5 + x2 * a7)));
+}
+template <class R> NEWT_HD R poly_cos(R x) {
+    R x2 = x * x;
+    R b8 = R(1.0) / R(40320.0), b6 = R(-1.0) / R(720.0), b4 = R(1.0) / R(24.0), b2 = R(-1.0) / R(2.0);
+    return R(1.0) + x2 * (b2 + x2 * (b4 + x2 * (b6 + x2 * b8)));
+}
+template <class R> NEWT_HD R reduce_quadrant(R x, int32_t* quadrant) {
+    const R two_over_pi = R(0.63661974);
+    const R c1 = R(1.5703125), c2 = R(4.8370362e-4), c3 = R(7.5497895e-8);
+    R scaled = x * two_over_pi;
+    int32_t q = scaled >= R(0.0) ? sat_i32(scaled + R(0.5)) : sat_i32(scaled - R(0.5));
+    R k = (R)q;
+    *quadrant = q;
+    return ((x - k * c1) - k * c2) - k * c3;
+}
+template <class R> NEWT_HD void nsincos(R x, R* s, R* c) {
+    int32_t q;
+    R r = reduce_quadrant(x, &q);
+    R ps = poly_sin(r), pc = poly_cos(r);
+    switch (q & 3) {
+        case 0: *s = ps; *c = pc; break;
+        case 1: *s = pc; *c = -ps; break;
+        case 2: *s = -ps; *c = -pc; break;
+        default: *s = -pc; *c = ps; break;
+    }
+}
+
+template <class R> struct V3 {
+    R x, y, z;
+};
+template <class R> NEWT_HD V3<R> mk3(R x, R y, R z) { return V3<R>{x, y, z}; }
+template <class R> NEWT_HD V3<R> z3() { return V3<R>{R(0), R(0), R(0)}; }
+template <class R> NEWT_HD V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> NEWT_HD V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> NEWT_HD V3<R> operator-(V3<R> a) { return {-a.x, -a.y, -a.z}; }
+template <class R> NEWT_HD V3<R> operator*(V3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class R> NEWT_HD V3<R> operator/(V3<R> a, R s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class R> NEWT_HD R dot3(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> NEWT_HD V3<R> cross3(V3<R> a, V3<R> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class R> NEWT_HD R len3(V3<R> a) { return nsqrt(dot3(a, a)); }
+template <class R> NEWT_HD V3<R> unit3(V3<R> a) {
+    R l = len3(a);
+    return l == R(0) ? z3<R>() : a / l;
+}
+template <class R> NEWT_HD V3<R> abs3(V3<R> a) { return {nabs(a.x), nabs(a.y), nabs(a.z)}; }
+template <class R> NEWT_HD bool is_zero3(V3<R> a) { return a.x == R(0) && a.y == R(0) && a.z == R(0); }
+
+// Quaternion (x, y, z, w), Hamilton product (newt/src/math.rs:560-683).
+template <class R> struct Qt {
+    R x, y, z, w;
+};
+template <class R> NEWT_HD Qt<R> qid() { return Qt<R>{R(0), R(0), R(0), R(1)}; }
+template <class R> NEWT_HD Qt<R> qmul(Qt<R> a, Qt<R> r) {
+    return {a.w * r.x + a.x * r.w + a.y * r.z - a.z * r.y, a.w * r.y - a.x * r.z + a.y * r.w + a.z * r.x,
+            a.w * r.z + a.x * r.y - a.y * r.x + a.z * r.w, a.w * r.w - a.x * r.x - a.y * r.y - a.z * r.z};
+}
+template <class R> NEWT_HD Qt<R> qconj(Qt<R> q) { return {-q.x, -q.y, -q.z, q.w}; }
+template <class R> NEWT_HD Qt<R> qaxis(V3<R> axis, R angle) {
+    V3<R> a = unit3(axis);
+    R s, c;
+    nsincos(angle * R(0.5), &s, &c);
+    return {a.x * s, a.y * s, a.z * s, c};
+}
+template <class R> NEWT_HD Qt<R> qnormalize(Qt<R> q) {
+    R n = nsqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    if (n == R(0)) return qid<R>();
+    return {q.x / n, q.y / n, q.z / n, q.w / n};
+}
+template <class R> NEWT_HD V3<R> qrot(Qt<R> q, V3<R> v) {
+    V3<R> qv{q.x, q.y, q.z};
+    V3<R> t = cross3(qv, v) * R(2.0);
+    return v + t * q.w + cross3(qv, t);
+}
+template <class R> NEWT_HD V3<R> qrot_inv(Qt<R> q, V3<R> v) { return qrot(qconj(q), v); }
+template <class R> NEWT_HD Qt<R> qintegrate(Qt<R> q, V3<R> w, R dt) {
+    return qnormalize(qmul(q, qaxis(w, dt * len3(w))));
+}
+
+// 3x3 in row-major m[r*3+c]; built from a quaternion with Quat::to_mat3's
+// entry formulas (newt/src/math.rs:620-633).
+template <class R> struct M33 {
+    R m[9];
+};
+template <class R> NEWT_HD M33<R> qmat(Qt<R> q) {
+    R x = q.x, y = q.y, z = q.z, w = q.w;
+    M33<R> o;
+    o.m[0] = R(1.0) - R(2.0) * (y * y + z * z);
+    o.m[3] = R(2.0) * (x * y + z * w);
+    o.m[6] = R(2.0) * (x * z - y * w);
+    o.m[1] = R(2.0) * (x * y - z * w);
+    o.m[4] = R(1.0) - R(2.0) * (x * x + z * z);
+    o.m[7] = R(2.0) * (y * z + x * w);
+    o.m[2] = R(2.0) * (x * z + y * w);
+    o.m[5] = R(2.0) * (y * z - x * w);
+    o.m[8] = R(1.0) - R(2.0) * (x * x + y * y);
+    return o;
+}
+template <class R> NEWT_HD V3<R> mv(const M33<R>& a, V3<R> v) {
+    return {a.m[0] * v.x + a.m[1] * v.y + a.m[2] * v.z, a.m[3] * v.x + a.m[4] * v.y + a.m[5] * v.z,
+            a.m[6] * v.x + a.m[7] * v.y + a.m[8] * v.z};
+}
+template <class R> NEWT_HD V3<R> mtv(const M33<R>& a, V3<R> v) {
+    return {a.m[0] * v.x + a.m[3] * v.y + a.m[6] * v.z, a.m[1] * v.x + a.m[4] * v.y + a.m[7] * v.z,
+            a.m[2] * v.x + a.m[5] * v.y + a.m[8] * v.z};
+}
+
+// Spatial motion (angular, linear) and force (torque, linear).
+template <class R> struct SM {
+    V3<R> w, v;
+};
+template <class R> struct SF {
+    V3<R> t, f;
+};
+template <class R> NEWT_HD SM<R> smz() { return {z3<R>(), z3<R>()}; }
+template <class R> NEWT_HD SF<R> sfz() { return {z3<R>(), z3<R>()}; }
+template <class R> NEWT_HD SM<R> operator+(SM<R> a, SM<R> b) { return {a.w + b.w, a.v + b.v}; }
+template <class R> NEWT_HD SM<R> operator*(SM<R> a, R s) { return {a.w * s, a.v * s}; }
+template <class R> NEWT_HD SF<R> operator+(SF<R> a, SF<R> b) { return {a.t + b.t, a.f + b.f}; }
+template <class R> NEWT_HD SF<R> operator-(SF<R> a, SF<R> b) { return {a.t - b.t, a.f - b.f}; }
+template <class R> NEWT_HD SF<R> operator*(SF<R> a, R s) { return {a.t * s, a.f * s}; }
+template <class R> NEWT_HD SM<R> crm(SM<R> a, SM<R> b) { return {cross3(a.w, b.w), cross3(a.w, b.v) + cross3(a.v, b.w)}; }
+template <class R> NEWT_HD SF<R> crf(SM<R> a, SF<R> b) { return {cross3(a.w, b.t) + cross3(a.v, b.f), cross3(a.w, b.f)}; }
+template <class R> NEWT_HD R sdot(SM<R> m, SF<R> f) { return dot3(m.w, f.t) + dot3(m.v, f.f); }
+
+// Plücker transform parent -> child: rotation (row-major) and translation.
+template <class R> struct Xf {
+    M33<R> E;
+    V3<R> r;
+};
+template <class R> NEWT_HD SM<R> xmotion(const Xf<R>& x, SM<R> m) {
+    V3<R> w = mv(x.E, m.w);
+    return {w, mv(x.E, m.v) + cross3(x.r, w)};
+}
+template <class R> NEWT_HD SF<R> xtforce(const Xf<R>& x, SF<R> f) {
+    V3<R> fl = mtv(x.E, f.f);
+    V3<R> tq = mtv(x.E, f.t - cross3(x.r, f.f));
+    return {tq, fl};
+}
+
+// 6x6 (row-major) times motion; accumulation from +0 like newt's loops.
+template <class R> NEWT_HD SF<R> m6mul(const R* a, SM<R> m) {
+    R v[6] = {m.w.x, m.w.y, m.w.z, m.v.x, m.v.y, m.v.z};
+    R o[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        R s = R(0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) s = s + a[i * 6 + j] * v[j];
+        o[i] = s;
+    }
+    return {{o[0], o[1], o[2]}, {o[3], o[4], o[5]}};
+}
+
+// Xᵀ A X by basis columns (newt/src/spatial.rs:457-481, aba.rs:551-613).
+template <class R> NEWT_HD void m6pullback(const R* a, const Xf<R>& x, R* out) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        SM<R> e = smz<R>();
+        R* comp[6] = {&e.w.x, &e.w.y, &e.w.z, &e.v.x, &e.v.y, &e.v.z};
+        *comp[c] = R(1);
+        SF<R> f = xtforce(x, m6mul(a, xmotion(x, e)));
+        out[0 * 6 + c] = f.t.x;
+        out[1 * 6 + c] = f.t.y;
+        out[2 * 6 + c] = f.t.z;
+        out[3 * 6 + c] = f.f.x;
+        out[4 * 6 + c] = f.f.y;
+        out[5 * 6 + c] = f.f.z;
+    }
+}
+
+}  // namespace newt

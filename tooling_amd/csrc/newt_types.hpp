@@ -1,0 +1,82 @@
+// newt_types.hpp — device tables of a compiled model and the per-world
+// workspace layout.
+//
+// A model is compiled once on the host (newt_model_finalize) into flat
+// arrays that every world of a batch shares: link records, geom records, the
+// statically filtered broadphase candidate pairs and each link's ancestor
+// chain.  Per-world state and scratch live in "lane-interleaved" arrays:
+// element i of world w sits at base[i * W + w], so the lanes of a wavefront,
+// which step consecutive worlds, touch consecutive addresses.
+#pragma once
+
+#include <stdint.h>
+
+namespace newt {
+
+enum : int { JF_FREE = 0, JF_FIXED = 1, JF_HINGE = 2, JF_SLIDE = 3, JF_BALL = 4 };
+enum : int { SH_PLANE = 0, SH_SPHERE = 1, SH_BOX = 2, SH_CAPSULE = 3 };
+
+struct LinkT {
+    int parent, joint, q_off, v_off;
+    int chain_off, chain_len;  // ancestors root..self in ModelT::chain
+    int pad0, pad1;
+    double axis[3];            // as given (joint subspace)
+    double pj[3], cj[3];       // joint anchor in parent / child frame
+    double pjq[4], cjq[4];     // anchor orientations (fixed joints, root pose)
+    double xfE[9], xfr[3];     // fixed-joint parent->child transform
+    double si[36];             // 6x6 spatial inertia at the COM
+    double mass, damping, armature, free_damping;
+};
+
+struct GeomT {
+    int shape, link;           // link = -1 for a static geom
+    int condim, max_contacts;
+    double size[3], pos[3], quat[4];
+    double friction, margin, gap;
+    double solref[2], solimp[4];
+    int power, pad0;
+};
+
+struct PairT {
+    int a, b;
+};
+
+struct ModelT {
+    int n_links, nq, nv, n_geoms;
+    int n_pairs, cap_c, cap_r, n_chain;
+    int integrator, mode, iterations, cone;
+    double dt, slop;
+    double gravity[3];
+    const LinkT* links;
+    const GeomT* geoms;
+    const PairT* pairs;
+    const int* chain;
+};
+
+// Offsets (in elements) of the per-world scratch regions.
+struct Layout {
+    // real-valued workspace (arithmetic type of the step)
+    int poses, lvel, xup, S, v, c, a, ia, pa, d, fs, tau, qacc_free, jt;
+    int cgeo, coef, rreg, rdiag, rbias, f, A, iw, delta, gvec, gres;
+    int total_r;
+    // factorization workspace (always double)
+    int M, Li, Ld, dvec, dres;
+    int total_d;
+    // integer workspace
+    int cids, rinfo, pairs, iwf;
+    int total_i;
+};
+
+struct DataT {
+    int W;            // worlds in this batch
+    double* qpos;     // [nq][W]
+    double* qvel;     // [nv][W]
+    double* qacc;     // [nv][W]
+    double* qfrc;     // [nv][W] user qfrc_applied
+    double* fat;      // [n_geoms*6][W] broadphase fat bounds
+    int* fault;       // [W]
+    int* ncontact;    // [W] contacts of the last step
+    int* npair;       // [W] pairs of the last step
+};
+
+}  // namespace newt

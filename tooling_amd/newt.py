@@ -1,0 +1,127 @@
+"""Model / Data / step over the C ABI — the host-side mirror of newt's World.
+
+newt keeps model and state in one `World` and advances it with
+`World::step(&mut self)` (newt/src/world.rs:1459).  Here a :class:`Model`
+is the immutable half (options, tree, geoms; compiled into device tables)
+and a :class:`Data` holds the state of N worlds on the GPU; ``step(model,
+data, n)`` advances all of them by ``n`` World::step calls.  Errors raise
+:class:`tooling_amd.abi.NewtError` where newt panics.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .scenes import Scene
+
+
+class Model:
+    def __init__(self, scene: Scene):
+        self.lib = abi.load_library()
+        self.scene = scene
+        h = C.c_void_p()
+        opts = scene.options_struct()
+        abi.check(self.lib.newt_model_create(C.byref(opts), C.byref(h)))
+        self.handle = h
+        links = scene.link_array()
+        abi.check(self.lib.newt_model_add_tree(h, links, len(scene.links), scene.self_collide, None))
+        for g in scene.geom_structs():
+            abi.check(self.lib.newt_model_add_geom(h, C.byref(g), None))
+        abi.check(self.lib.newt_model_finalize(h))
+        nq, nv, nl, ng = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        abi.check(self.lib.newt_model_dims(h, C.byref(nq), C.byref(nv), C.byref(nl), C.byref(ng)))
+        self.nq, self.nv, self.n_links, self.n_geoms = nq.value, nv.value, nl.value, ng.value
+
+    def close(self):
+        if self.handle:
+            self.lib.newt_model_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Data:
+    def __init__(self, model: Model, n_worlds: int, precision: int = abi.PRECISION_FP64):
+        self.model = model
+        self.lib = model.lib
+        self.n_worlds = n_worlds
+        self.precision = precision
+        h = C.c_void_p()
+        abi.check(self.lib.newt_data_create(model.handle, n_worlds, precision, C.byref(h)))
+        self.handle = h
+
+    @staticmethod
+    def _dp(a):
+        return a.ctypes.data_as(C.POINTER(C.c_double))
+
+    def set_state(self, qpos, qvel):
+        qpos = np.ascontiguousarray(qpos, dtype=np.float64).reshape(self.n_worlds, self.model.nq)
+        qvel = np.ascontiguousarray(qvel, dtype=np.float64).reshape(self.n_worlds, self.model.nv)
+        abi.check(self.lib.newt_data_set_state(self.handle, self._dp(qpos), self._dp(qvel)))
+
+    def state(self):
+        qpos = np.zeros((self.n_worlds, self.model.nq))
+        qvel = np.zeros((self.n_worlds, self.model.nv))
+        abi.check(self.lib.newt_data_get_state(self.handle, self._dp(qpos), self._dp(qvel)))
+        return qpos, qvel
+
+    def qacc(self):
+        out = np.zeros((self.n_worlds, self.model.nv))
+        abi.check(self.lib.newt_data_get_qacc(self.handle, self._dp(out)))
+        return out
+
+    def faults(self):
+        out = np.zeros(self.n_worlds, dtype=np.int32)
+        abi.check(self.lib.newt_data_get_faults(self.handle, out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out
+
+    def contacts(self):
+        """Per world: list of (geom_a, geom_b, slot) and an array of
+        (pos xyz, normal xyz, penetration) rows, in narrow-phase order."""
+        cap = C.c_int32()
+        abi.check(self.lib.newt_data_contact_capacity(self.handle, C.byref(cap)))
+        cap = cap.value
+        counts = np.zeros(self.n_worlds, dtype=np.int32)
+        ids = np.zeros((self.n_worlds, cap, 3), dtype=np.int32)
+        geo = np.zeros((self.n_worlds, cap, 7))
+        abi.check(self.lib.newt_data_get_contacts(
+            self.handle, counts.ctypes.data_as(C.POINTER(C.c_int32)),
+            ids.ctypes.data_as(C.POINTER(C.c_int32)), self._dp(geo)))
+        return counts, ids, geo
+
+    def pairs(self):
+        cap = C.c_int32()
+        abi.check(self.lib.newt_data_pair_capacity(self.handle, C.byref(cap)))
+        cap = cap.value
+        counts = np.zeros(self.n_worlds, dtype=np.int32)
+        pairs = np.zeros((self.n_worlds, cap, 2), dtype=np.int32)
+        abi.check(self.lib.newt_data_get_pairs(
+            self.handle, counts.ctypes.data_as(C.POINTER(C.c_int32)),
+            pairs.ctypes.data_as(C.POINTER(C.c_int32))))
+        return counts, pairs
+
+    def synchronize(self):
+        abi.check(self.lib.newt_data_synchronize(self.handle))
+
+    def close(self):
+        if self.handle:
+            self.lib.newt_data_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def step(model: Model, data: Data, n_steps: int = 1, stream=None):
+    """Enqueue n_steps World::step calls for every world of `data`."""
+    s = C.c_void_p(stream) if stream is not None else None
+    abi.check(model.lib.newt_step(model.handle, data.handle, int(n_steps), s))

@@ -1,0 +1,228 @@
+"""Scene builders for the benchmark workloads, in newt's model vocabulary.
+
+A :class:`Scene` holds the World options, the tree's links, the geoms and a
+seeded initial-state sampler.  The same records feed the HIP library and the
+CPU oracle, so both step identical doubles.
+
+Inertia helpers restate newt/src/geom.rs:942-1000 (solid box, solid capsule).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Callable, List
+
+import numpy as np
+
+from . import abi
+
+IDENTITY_Q = (0.0, 0.0, 0.0, 1.0)
+
+
+def solid_box_inertia(mass, he):
+    """Column-major diag inertia of a solid box (geom.rs:942-951)."""
+    hx2, hy2, hz2 = he[0] * he[0], he[1] * he[1], he[2] * he[2]
+    return _diag((mass / 3.0) * (hy2 + hz2), (mass / 3.0) * (hx2 + hz2), (mass / 3.0) * (hx2 + hy2))
+
+
+def solid_capsule_inertia(mass, r, h):
+    """Column-major diag inertia of a solid capsule along local Z (geom.rs:966-981)."""
+    vol_cyl = r * r * (2.0 * h)
+    vol_caps = (4.0 / 3.0) * r * r * r
+    total = vol_cyl + vol_caps
+    m_c = mass * (vol_cyl / total)
+    m_s = mass - m_c
+    i_zz = 0.5 * m_c * r * r + (2.0 / 5.0) * m_s * r * r
+    offset = h + 3.0 / 8.0 * r
+    hemi = (83.0 / 320.0) * m_s * r * r
+    i_xx = (1.0 / 12.0) * m_c * (3.0 * r * r + 4.0 * h * h) + hemi + m_s * offset * offset
+    return i_xx, i_zz
+
+
+def _diag(x, y, z):
+    return (x, 0.0, 0.0, 0.0, y, 0.0, 0.0, 0.0, z)
+
+
+def quat_axis_angle(axis, angle):
+    n = math.sqrt(sum(a * a for a in axis))
+    s, c = math.sin(angle * 0.5), math.cos(angle * 0.5)
+    return (axis[0] / n * s, axis[1] / n * s, axis[2] / n * s, c)
+
+
+def link(parent, joint, *, axis=(0.0, 0.0, 0.0), parent_pos=(0.0, 0.0, 0.0), parent_quat=IDENTITY_Q,
+         child_pos=(0.0, 0.0, 0.0), mass=1.0, inertia=_diag(1.0, 1.0, 1.0), damping=0.0, armature=0.0,
+         free_damping=0.0):
+    return dict(parent=parent, joint=joint, axis=axis, parent_pos=parent_pos, parent_quat=parent_quat,
+                child_pos=child_pos, child_quat=IDENTITY_Q, mass=mass, inertia=inertia, damping=damping,
+                armature=armature, free_damping=free_damping)
+
+
+def geom(shape, *, tree=-1, link=-1, size=(0.0, 0.0, 0.0), pos=(0.0, 0.0, 0.0), quat=IDENTITY_Q,
+         friction=1.0, margin=0.0, gap=0.0, solref=(0.02, 1.0), solimp=(0.9, 0.95, 0.001, 0.5), power=2,
+         condim=3):
+    """Geom defaults follow Geom::static_plane / *_on_link (geom.rs:465-700)."""
+    return dict(shape=shape, tree=tree, link=link, size=size, pos=pos, quat=quat, friction=friction,
+                margin=margin, gap=gap, solref=solref, solimp=solimp, power=power, condim=condim)
+
+
+@dataclass
+class Scene:
+    name: str
+    options: dict
+    links: List[dict]
+    geoms: List[dict]
+    self_collide: int = 0
+    sampler: Callable[[int, int], tuple] = None
+    description: str = ""
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def nq(self):
+        return sum(7 if l["joint"] == abi.JOINT_FREE else 0 if l["joint"] == abi.JOINT_FIXED
+                   else 4 if l["joint"] == abi.JOINT_BALL else 1 for l in self.links)
+
+    @property
+    def nv(self):
+        return sum(6 if l["joint"] == abi.JOINT_FREE else 0 if l["joint"] == abi.JOINT_FIXED
+                   else 3 if l["joint"] == abi.JOINT_BALL else 1 for l in self.links)
+
+    def initial_state(self, n_worlds: int, seed: int):
+        return self.sampler(n_worlds, seed)
+
+    # -- ctypes records --------------------------------------------------
+    def options_struct(self) -> abi.Options:
+        o = self.options
+        return abi.Options(dt=o["dt"], gravity=abi.D3(*o["gravity"]), penetration_slop=o.get("slop", 0.0),
+                           integrator=o["integrator"], solver_mode=o["solver"], iterations=o["iterations"],
+                           cone=o.get("cone", abi.CONE_PYRAMIDAL))
+
+    def link_array(self):
+        arr = (abi.LinkDesc * len(self.links))()
+        for i, l in enumerate(self.links):
+            d = arr[i]
+            d.parent = l["parent"]
+            d.joint = l["joint"]
+            d.axis = abi.D3(*l["axis"])
+            d.damping = l["damping"]
+            d.armature = l["armature"]
+            d.free_damping = l["free_damping"]
+            d.has_range = 0
+            d.parent_pos = abi.D3(*l["parent_pos"])
+            d.parent_quat = abi.D4(*l["parent_quat"])
+            d.child_pos = abi.D3(*l["child_pos"])
+            d.child_quat = abi.D4(*l["child_quat"])
+            d.mass = l["mass"]
+            d.inertia = (abi.C.c_double * 9)(*l["inertia"])
+        return arr
+
+    def geom_structs(self):
+        out = []
+        for g in self.geoms:
+            d = abi.GeomDesc()
+            d.shape = g["shape"]
+            d.tree = g["tree"]
+            d.link = g["link"]
+            d.condim = g["condim"]
+            d.size = abi.D3(*g["size"])
+            d.pos = abi.D3(*g["pos"])
+            d.quat = abi.D4(*g["quat"])
+            d.friction = g["friction"]
+            d.margin = g["margin"]
+            d.gap = g["gap"]
+            d.solref = (abi.C.c_double * 2)(*g["solref"])
+            d.solimp = (abi.C.c_double * 4)(*g["solimp"])
+            d.solimp_power = g["power"]
+            d.group = 0xFFFFFFFF
+            d.mask = 0xFFFFFFFF
+            out.append(d)
+        return out
+
+
+def _rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+# ---------------------------------------------------------------------------
+# Workload 1: 3-link planar revolute pendulum, no contacts.
+# ---------------------------------------------------------------------------
+def pendulum(dt=0.002) -> Scene:
+    """Fixed root + 3 hinges about +X; capsule links 0.3 m (anchor to anchor),
+    radius 0.02 m, 1 kg, COM at mid-link; self collision off, no plane."""
+    half = 0.15
+    ixx, izz = solid_capsule_inertia(1.0, 0.02, half)
+    links = [link(-1, abi.JOINT_FIXED)]
+    for i in range(3):
+        links.append(link(i, abi.JOINT_HINGE, axis=(1.0, 0.0, 0.0),
+                          parent_pos=(0.0, 0.0, 0.0) if i == 0 else (0.0, 0.0, -half),
+                          child_pos=(0.0, 0.0, half), mass=1.0, inertia=_diag(ixx, ixx, izz)))
+    geoms = [geom(abi.SHAPE_CAPSULE, tree=0, link=i + 1, size=(0.02, half, 0.0)) for i in range(3)]
+
+    def sampler(n, seed):
+        r = _rng(seed)
+        qpos = r.uniform(-math.pi, math.pi, size=(n, 3))
+        qvel = r.normal(0.0, 0.5, size=(n, 3))
+        return qpos, qvel
+
+    opts = dict(dt=dt, gravity=(0.0, 0.0, -9.81), integrator=abi.INTEGRATOR_EULER,
+                solver=abi.SOLVER_PGS, iterations=20)
+    return Scene("pendulum3", opts, links, geoms, 0, sampler,
+                 "3-link planar revolute pendulum, capsule links 0.3 m x 0.02 m, 1 kg, Euler dt 2 ms")
+
+
+# ---------------------------------------------------------------------------
+# Workload 2: 18-DoF floating-base quadruped on a plane.
+# ---------------------------------------------------------------------------
+TORSO_HALF = (0.25, 0.15, 0.05)
+LIMB_R = 0.03
+LIMB_HALF = 0.1   # capsule cylinder half length: 0.2 m limbs
+
+
+def quadruped(dt=0.002, iterations=50, friction=0.8) -> Scene:
+    """Free-root 12 kg box torso and four legs of (hip yaw about Z, hip
+    pitch about X, knee about X) hinges; every hinge link is a 0.2 m capsule
+    of radius 0.03 m and 1 kg.  The hip and thigh capsules point sideways
+    from the torso's long edges, the shank hangs down, so the zero pose
+    stands on the four shanks.  Self collision is off (the MJCF loader's
+    convention); the ground plane has friction 0.8."""
+    links = [link(-1, abi.JOINT_FREE, parent_pos=(0.0, 0.0, 0.4), mass=12.0,
+                  inertia=solid_box_inertia(12.0, TORSO_HALF))]
+    geoms = [geom(abi.SHAPE_PLANE, friction=friction)]
+    geoms.append(geom(abi.SHAPE_BOX, tree=0, link=0, size=TORSO_HALF, friction=1.0))
+    ixx, izz = solid_capsule_inertia(1.0, LIMB_R, LIMB_HALF)
+    along_y = _diag(ixx, izz, ixx)      # capsule axis along the link's Y
+    along_z = _diag(ixx, ixx, izz)      # capsule axis along the link's Z
+    to_y = quat_axis_angle((1.0, 0.0, 0.0), -math.pi / 2)   # local Z -> Y
+    for sx in (1.0, -1.0):
+        for sy in (1.0, -1.0):
+            hip = len(links)
+            links.append(link(0, abi.JOINT_HINGE, axis=(0.0, 0.0, 1.0),
+                              parent_pos=(sx * TORSO_HALF[0], sy * TORSO_HALF[1], 0.0),
+                              child_pos=(0.0, -sy * LIMB_HALF, 0.0), mass=1.0, inertia=along_y))
+            thigh = len(links)
+            links.append(link(hip, abi.JOINT_HINGE, axis=(1.0, 0.0, 0.0),
+                              parent_pos=(0.0, sy * LIMB_HALF, 0.0),
+                              child_pos=(0.0, -sy * LIMB_HALF, 0.0), mass=1.0, inertia=along_y))
+            shank = len(links)
+            links.append(link(thigh, abi.JOINT_HINGE, axis=(1.0, 0.0, 0.0),
+                              parent_pos=(0.0, sy * LIMB_HALF, 0.0),
+                              child_pos=(0.0, 0.0, LIMB_HALF), mass=1.0, inertia=along_z))
+            geoms.append(geom(abi.SHAPE_CAPSULE, tree=0, link=hip, size=(LIMB_R, LIMB_HALF, 0.0), quat=to_y))
+            geoms.append(geom(abi.SHAPE_CAPSULE, tree=0, link=thigh, size=(LIMB_R, LIMB_HALF, 0.0), quat=to_y))
+            geoms.append(geom(abi.SHAPE_CAPSULE, tree=0, link=shank, size=(LIMB_R, LIMB_HALF, 0.0)))
+
+    def sampler(n, seed):
+        r = _rng(seed)
+        qpos = np.zeros((n, 19))
+        qpos[:, 2] = r.uniform(0.3, 0.5, size=n)
+        qpos[:, 6] = 1.0
+        qpos[:, 7:] = r.normal(0.0, 0.2, size=(n, 12))
+        qvel = np.zeros((n, 18))
+        return qpos, qvel
+
+    opts = dict(dt=dt, gravity=(0.0, 0.0, -9.81), integrator=abi.INTEGRATOR_EULER,
+                solver=abi.SOLVER_PGS, iterations=iterations)
+    return Scene("quadruped18", opts, links, geoms, 0, sampler,
+                 "18-DoF floating-base quadruped on a plane (mu 0.8), PGS 50, Euler dt 2 ms")
+
+
+SCENES = {"pendulum3": pendulum, "quadruped18": quadruped}
