@@ -2,8 +2,11 @@
 
 FP64 step: pair sets and contact (geom_a, geom_b, slot) lists identical to
 the oracle at every step; qpos, qvel and qacc within 1e-9 relative
-(‖Δ‖∞ / max(‖ref‖∞, 1)) over 100 steps.  Mixed step: one step from the
-oracle's state gives qacc within 1e-5 relative.
+(‖Δ‖∞ / max(‖ref‖∞, 1), max over worlds) over 100+ steps.  Mixed step: one
+step from the oracle's state gives qacc within 1e-5 relative, at states from
+free fall to lying flat on the plane (the worst-conditioned contact sets).
+For scale: newt's own f32 step is ~2e-3 from the FP64 oracle on the same
+states.
 """
 import sys
 
@@ -122,7 +125,7 @@ def test_quadruped_mixed_one_step_qacc(oracle):
     say("quadruped18 mixed: 256 worlds, qacc of one step from oracle states")
     m = Model(sc)
     worst = 0.0
-    for t in (0, 120, 60):  # states before and during contact
+    for t in (0, 60, 60, 60, 60, 60, 100, 100):  # free fall, landing, lying flat
         ow.step(t, threads=8)
         qs, vs = ow.state()
         d = Data(m, n, abi.PRECISION_MIXED)

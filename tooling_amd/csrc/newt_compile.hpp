@@ -1,0 +1,267 @@
+// newt_compile.hpp — host-side compilation of a model description
+// (include/newt_gpu.h records) into the flat tables of newt_types.hpp, and
+// the per-world workspace layout.  No device calls: newt_gpu.hip uploads the
+// tables; the same code serves any host-side consumer of the tables.
+#pragma once
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/newt_gpu.h"
+#include "newt_math.hpp"
+#include "newt_types.hpp"
+
+namespace newt {
+
+struct HostModel {
+    newt_options opt{};
+    std::vector<std::vector<newt_link_desc>> trees;
+    std::vector<int> tree_collide;
+    std::vector<newt_geom_desc> geoms;
+    ModelT dev{};  // pointers into the vectors below
+    std::vector<LinkT> links;
+    std::vector<GeomT> hgeoms;
+    std::vector<PairT> pairs;
+    std::vector<int> chain;
+    std::vector<double> q_default;
+};
+
+inline newt_status cfail(std::string* err, newt_status s, const std::string& msg) {
+    if (err) *err = msg;
+    return s;
+}
+
+// Largest contact count one candidate pair can emit (ContactBuf, contact.rs:108-124).
+inline int pair_max_contacts(int sa, int sb) {
+    int p = sa == SH_PLANE ? sb : sa;
+    if (p == SH_SPHERE) return 1;
+    if (p == SH_CAPSULE) return 2;
+    return 4;
+}
+
+// Parent->child transform of a fixed joint (tree.rs:1124-1131), host double.
+inline void fixed_xup(const newt_link_desc& d, double E[9], double r[3]) {
+    Qt<double> pq{d.parent_quat[0], d.parent_quat[1], d.parent_quat[2], d.parent_quat[3]};
+    Qt<double> cq{d.child_quat[0], d.child_quat[1], d.child_quat[2], d.child_quat[3]};
+    Qt<double> rel = qmul(pq, qconj(cq));
+    M33<double> m = qmat(qconj(rel));
+    for (int k = 0; k < 9; ++k) E[k] = m.m[k];
+    V3<double> pj = mk3(d.parent_pos[0], d.parent_pos[1], d.parent_pos[2]);
+    V3<double> cj = mk3(d.child_pos[0], d.child_pos[1], d.child_pos[2]);
+    V3<double> t = cj - mv(m, pj);
+    r[0] = t.x; r[1] = t.y; r[2] = t.z;
+}
+
+// Mat6::from_spatial_inertia with a zero COM offset (spatial.rs:355-381),
+// evaluated with the reference's operations so the signs of its zero
+// entries match: [I + ([c]x[c]x)(-m), [c]x m; [c]x (-m), m 1], c = 0.
+inline void spatial_inertia(const newt_link_desc& d, double si[36]) {
+    const double m = d.mass;
+    const double z = 0.0;
+    // skew(0) in column-major order (math.rs:418-424)
+    const double cx[9] = {0.0, z, -z, -z, 0.0, z, z, -z, 0.0};
+    auto get = [&](const double* a, int r, int c) { return a[c * 3 + r]; };
+    double cxcx[9];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += get(cx, r, k) * get(cx, k, c);
+            cxcx[c * 3 + r] = s;
+        }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            si[r * 6 + c] = d.inertia[c * 3 + r] + get(cxcx, r, c) * (-m);
+            si[r * 6 + c + 3] = get(cx, r, c) * m;
+            si[(r + 3) * 6 + c] = get(cx, r, c) * (-m);
+            si[(r + 3) * 6 + c + 3] = r == c ? m : 0.0;
+        }
+}
+
+inline newt_status compile_model(HostModel* M, std::string* err) {
+    if (M->trees.size() != 1)
+        return cfail(err, NEWT_ERR_UNSUPPORTED, "this build steps models with exactly one tree");
+    const newt_options& o = M->opt;
+    if (o.iterations < 1) return cfail(err, NEWT_ERR_INVALID_CONFIG, "solver iterations must be >= 1");
+    if (o.solver_mode == NEWT_SOLVER_NEWTON && o.cone == NEWT_CONE_ELLIPTIC)
+        return cfail(err, NEWT_ERR_INVALID_CONFIG, "solver=newton with cone=elliptic is not supported yet; use cone=pyramidal");
+    if (o.integrator != NEWT_INTEGRATOR_EULER)
+        return cfail(err, NEWT_ERR_UNSUPPORTED, "the GPU step integrates with semi-implicit Euler only");
+    if (o.solver_mode == NEWT_SOLVER_NEWTON) return cfail(err, NEWT_ERR_UNSUPPORTED, "Newton solver is not built for trees");
+    if (o.cone != NEWT_CONE_PYRAMIDAL) return cfail(err, NEWT_ERR_UNSUPPORTED, "elliptic cone is not built");
+    const auto& T = M->trees[0];
+    int nq = 0, nv = 0;
+    M->links.clear();
+    M->chain.clear();
+    M->q_default.clear();
+    for (size_t i = 0; i < T.size(); ++i) {
+        const newt_link_desc& d = T[i];
+        if (i == 0 && !(d.joint == NEWT_JOINT_FREE || d.joint == NEWT_JOINT_FIXED))
+            return cfail(err, NEWT_ERR_INVALID_ARG, "root joint must be Free or Fixed");
+        if (i > 0 && !(d.parent >= 0 && d.parent < (int)i))
+            return cfail(err, NEWT_ERR_INVALID_ARG, "parent index must precede its link");
+        if (i > 0 && d.joint == NEWT_JOINT_FREE) return cfail(err, NEWT_ERR_INVALID_ARG, "free joint only at the root");
+        if (d.joint == NEWT_JOINT_BALL) return cfail(err, NEWT_ERR_UNSUPPORTED, "ball joints are not built on the GPU path");
+        if (d.has_range) return cfail(err, NEWT_ERR_UNSUPPORTED, "joint ranges (limit rows) are not built on the GPU path");
+        LinkT l{};
+        l.parent = d.parent;
+        l.joint = d.joint;
+        l.q_off = nq;
+        l.v_off = nv;
+        for (int k = 0; k < 3; ++k) {
+            l.axis[k] = d.axis[k];
+            l.pj[k] = d.parent_pos[k];
+            l.cj[k] = d.child_pos[k];
+        }
+        for (int k = 0; k < 4; ++k) {
+            l.pjq[k] = d.parent_quat[k];
+            l.cjq[k] = d.child_quat[k];
+        }
+        fixed_xup(d, l.xfE, l.xfr);
+        spatial_inertia(d, l.si);
+        l.mass = d.mass;
+        l.damping = d.damping;
+        l.armature = d.armature;
+        l.free_damping = d.free_damping;
+        // ancestor chain root..self
+        std::vector<int> ch;
+        for (int j = (int)i; j >= 0; j = T[j].parent) ch.push_back(j);
+        std::reverse(ch.begin(), ch.end());
+        l.chain_off = (int)M->chain.size();
+        l.chain_len = (int)ch.size();
+        M->chain.insert(M->chain.end(), ch.begin(), ch.end());
+        // default q (tree.rs:275-330)
+        if (d.joint == NEWT_JOINT_FREE) {
+            for (int k = 0; k < 3; ++k) M->q_default.push_back(d.parent_pos[k]);
+            for (int k = 0; k < 4; ++k) M->q_default.push_back(d.parent_quat[k]);
+            nq += 7;
+            nv += 6;
+        } else if (d.joint == NEWT_JOINT_HINGE || d.joint == NEWT_JOINT_SLIDE) {
+            M->q_default.push_back(0.0);
+            nq += 1;
+            nv += 1;
+        }
+        M->links.push_back(l);
+    }
+    // geoms
+    M->hgeoms.clear();
+    for (const newt_geom_desc& g : M->geoms) {
+        if (g.shape < NEWT_SHAPE_PLANE || g.shape > NEWT_SHAPE_CAPSULE) return cfail(err, NEWT_ERR_INVALID_ARG, "unknown geom shape");
+        if (g.tree >= 1) return cfail(err, NEWT_ERR_INVALID_ARG, "geom attached to a missing tree");
+        if (g.tree == 0 && (g.link < 0 || g.link >= (int)T.size())) return cfail(err, NEWT_ERR_INVALID_ARG, "geom link out of range");
+        if (g.condim != 3 && g.condim != 1 && g.condim != 4 && g.condim != 6)
+            return cfail(err, NEWT_ERR_INVALID_ARG, "condim must be 1, 3, 4 or 6");
+        GeomT h{};
+        h.shape = g.shape;
+        h.link = g.tree == 0 ? g.link : -1;
+        h.condim = g.condim;
+        for (int k = 0; k < 3; ++k) {
+            h.size[k] = g.size[k];
+            h.pos[k] = g.pos[k];
+        }
+        for (int k = 0; k < 4; ++k) h.quat[k] = g.quat[k];
+        h.friction = g.friction;
+        h.margin = g.margin;
+        h.gap = g.gap;
+        h.solref[0] = g.solref[0];
+        h.solref[1] = g.solref[1];
+        for (int k = 0; k < 4; ++k) h.solimp[k] = g.solimp[k];
+        h.power = g.solimp_power;
+        M->hgeoms.push_back(h);
+    }
+    // Static candidate pairs: group/mask filter, disabled self collision,
+    // same attachment (broadphase.rs:16-18, 426-437; world.rs:1219-1225).
+    M->pairs.clear();
+    int cap_c = 0;
+    bool self_collide = M->tree_collide[0] != 0;
+    for (size_t a = 0; a < M->geoms.size(); ++a)
+        for (size_t b = a + 1; b < M->geoms.size(); ++b) {
+            const newt_geom_desc& ga = M->geoms[a];
+            const newt_geom_desc& gb = M->geoms[b];
+            if (!((ga.group & gb.mask) != 0 && (gb.group & ga.mask) != 0)) continue;
+            bool la = ga.tree >= 0, lb = gb.tree >= 0;
+            if (la && lb && !self_collide) continue;
+            if (la == lb && (!la || ga.link == gb.link)) continue;
+            bool plane_pair = (ga.shape == NEWT_SHAPE_PLANE) != (gb.shape == NEWT_SHAPE_PLANE);
+            if (!plane_pair)
+                return cfail(err, NEWT_ERR_UNSUPPORTED_PAIR,
+                            "contact pair geom " + std::to_string(a) + " x geom " + std::to_string(b) +
+                                " is not built on the GPU path (plane-primitive pairs only)");
+            if (std::min(ga.condim, gb.condim) != 3)
+                return cfail(err, NEWT_ERR_UNSUPPORTED, "only condim 3 pyramidal contacts are built");
+            M->pairs.push_back({(int)a, (int)b});
+            cap_c += pair_max_contacts(ga.shape, gb.shape);
+        }
+    if (o.solver_mode == NEWT_SOLVER_PENALTY && !M->pairs.empty())
+        return cfail(err, NEWT_ERR_UNSUPPORTED, "penalty contact forces are not built; use the PGS solver");
+    ModelT& d = M->dev;
+    d.n_links = (int)T.size();
+    d.nq = nq;
+    d.nv = nv;
+    d.n_geoms = (int)M->geoms.size();
+    d.n_pairs = (int)M->pairs.size();
+    d.cap_c = std::max(cap_c, 1);
+    d.cap_r = std::max(4 * cap_c, 1);
+    d.n_chain = (int)M->chain.size();
+    d.integrator = o.integrator;
+    d.mode = o.solver_mode;
+    d.iterations = o.iterations;
+    d.cone = o.cone;
+    d.dt = o.dt;
+    d.slop = o.penetration_slop;
+    for (int k = 0; k < 3; ++k) d.gravity[k] = o.gravity[k];
+    d.links = M->links.data();
+    d.geoms = M->hgeoms.data();
+    d.pairs = M->pairs.data();
+    d.chain = M->chain.data();
+    return NEWT_OK;
+}
+
+inline Layout make_layout(const ModelT& m) {
+    Layout L{};
+    int r = 0, dd = 0, ii = 0;
+    auto R = [&](int n) { int o = r; r += n; return o; };
+    auto D = [&](int n) { int o = dd; dd += n; return o; };
+    auto I = [&](int n) { int o = ii; ii += n; return o; };
+    int n = m.n_links, nv = std::max(m.nv, 1);
+    L.coef = R(2 * nv * m.cap_r);
+    L.A = R(m.cap_r * m.cap_r);
+    L.total_r = r;
+    L.poses = D(7 * n);
+    L.lvel = D(6 * n);
+    L.xup = D(12 * n);
+    L.S = D(6 * n);
+    L.v = D(6 * n);
+    L.c = D(6 * n);
+    L.a = D(6 * n);
+    L.ia = D(36 * n);
+    L.pa = D(6 * n);
+    L.d = D(n);
+    L.fs = D(n);
+    L.tau = D(nv);
+    L.qacc_free = D(nv);
+    L.delta = D(nv);
+    L.gres = D(nv);
+    L.jt = D(3 * nv);
+    L.cgeo = D(9 * m.cap_c);
+    L.iwc = D(n);
+    L.rreg = D(m.cap_r);
+    L.rdiag = D(m.cap_r);
+    L.rbias = D(m.cap_r);
+    L.f = D(m.cap_r);
+    L.M = D(nv * nv);
+    L.Li = D(nv * nv);
+    L.Ld = D(nv * nv);
+    L.dvec = D(nv);
+    L.dres = D(nv);
+    L.total_d = dd;
+    L.cids = I(3 * m.cap_c);
+    L.rinfo = I(3 * m.cap_r);
+    L.pairs = I(std::max(m.n_pairs, 1));
+    L.iwf = I(n + 2);
+    L.total_i = ii;
+    return L;
+}
+
+
+}  // namespace newt

@@ -55,12 +55,12 @@ struct ModelT {
 
 // Offsets (in elements) of the per-world scratch regions.
 struct Layout {
-    // real-valued workspace (arithmetic type of the step)
-    int poses, lvel, xup, S, v, c, a, ia, pa, d, fs, tau, qacc_free, jt;
-    int cgeo, coef, rreg, rdiag, rbias, f, A, iw, delta, gvec, gres;
+    // Jacobian rows and Delassus matrix (type R of the step: FP64 or FP32)
+    int coef, A;
     int total_r;
-    // factorization workspace (always double)
-    int M, Li, Ld, dvec, dres;
+    // FP64 workspace: kinematics, dynamics, contact geometry, factorization
+    int poses, lvel, xup, S, v, c, a, ia, pa, d, fs, tau, qacc_free, delta, gres;
+    int jt, cgeo, iwc, rreg, rdiag, rbias, f, M, Li, Ld, dvec, dres;
     int total_d;
     // integer workspace
     int cids, rinfo, pairs, iwf;
