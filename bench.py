@@ -68,26 +68,29 @@ def time_gpu(model, scene, worlds, precision, steps, warmup, chunk, q0, v0):
                 value=worlds * total_steps / (gpu_ms / 1e3), faults=faults)
 
 
-def cpu_baseline(scene, q0, v0, steps=200):
+def cpu_baseline(scene, q0, v0, warmup, steps=100):
     from oracle import pyoracle
     threads = max(1, min(os.cpu_count() or 1, 16))
     n = 32 * threads
     ow = pyoracle.OracleWorlds(scene, n, 1)  # float: newt's arithmetic
     ow.set_state(q0[:n], v0[:n])
+    ow.step(warmup, threads=threads)
     t0 = time.perf_counter()
     ow.step(steps, threads=threads)
     dt = time.perf_counter() - t0
     ow.close()
     return dict(value=n * steps / dt, unit="world-steps/s", cores=threads, kind="port",
-                sample=f"{n} worlds x {steps} steps of the same seeded quadruped batch, C++ float "
-                       f"restatement of newt's step on {threads} host threads ({dt:.1f} s)")
+                sample=f"{n} worlds of the same seeded quadruped batch, steps {warmup}..{warmup + steps} "
+                       f"(after the same untimed warmup as the GPU), C++ float restatement of newt's "
+                       f"step on {threads} host threads ({dt:.1f} s)")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=300,
+                    help="untimed steps; 300 steps take the batch through landing into contact")
     ap.add_argument("--worlds", type=int, default=8192)
     ap.add_argument("--chunk", type=int, default=10, help="steps per kernel launch")
     ap.add_argument("--seed", type=int, default=2026)
@@ -124,7 +127,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(scene, q0, v0)
+        cpu = cpu_baseline(scene, q0, v0, args.warmup)
         log(f"cpu: {cpu['value']:.4g} world-steps/s on {cpu['cores']} threads")
     out = {
         "metric": "world-steps/s mixed-prec vs FP64 GPU at qacc err<=1e-5; p99 rollout latency",

@@ -81,11 +81,18 @@ typedef enum newt_cone {
 
 /* Arithmetic of a newt_data. */
 typedef enum newt_precision {
-    /* Every quantity in FP64; the drop-in for World::step. */
+    /* Every quantity in FP64; the drop-in for World::step.  The contact
+     * solve runs one wavefront per world (parallel Delassus assembly,
+     * incremental-residual PGS); results agree with the CPU step to ~1e-12. */
     NEWT_PRECISION_FP64 = 0,
-    /* FP32 kinematics, ABA, narrowphase, Delassus and PGS sweeps; FP64 mass
-     * matrix Cholesky, qpos/quaternion accumulation and stored state. */
-    NEWT_PRECISION_MIXED = 1
+    /* FP64 kinematics, dynamics (CRB, ABA), Cholesky, residuals, impulses and
+     * qpos/quaternion accumulation; Jacobian rows and the Delassus matrix
+     * are stored in FP32.  qacc within 1e-5 of the FP64 step. */
+    NEWT_PRECISION_MIXED = 1,
+    /* FP64 with newt's operation order everywhere, one lane per world:
+     * bit-identical to the CPU restatement of World::step.  For
+     * certification; slow when contact sets are large. */
+    NEWT_PRECISION_FP64_EXACT = 2
 } newt_precision;
 
 /* World options: the scalar, non-state fields of World

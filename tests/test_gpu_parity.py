@@ -41,13 +41,14 @@ def oracle():
     return pyoracle
 
 
-def test_pendulum_fp64_100_steps(oracle):
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP64, abi.PRECISION_FP64_EXACT])
+def test_pendulum_fp64_100_steps(oracle, precision):
     sc = scenes.pendulum()
     n = 256
     q0, v0 = sc.initial_state(n, 7)
-    say("pendulum3 fp64: 256 worlds x 100 steps")
+    say(f"pendulum3 precision {precision}: 256 worlds x 100 steps")
     m = Model(sc)
-    d = Data(m, n, abi.PRECISION_FP64)
+    d = Data(m, n, precision)
     d.set_state(q0, v0)
     step(m, d, 100)
     q, v = d.state()
@@ -61,6 +62,8 @@ def test_pendulum_fp64_100_steps(oracle):
     assert rel_err(q, qr) <= TOL_FP64
     assert rel_err(v, vr) <= TOL_FP64
     assert rel_err(a, ar) <= TOL_FP64
+    if precision == abi.PRECISION_FP64_EXACT:
+        assert np.array_equal(q, qr) and np.array_equal(v, vr) and np.array_equal(a, ar)
 
 
 def test_pendulum_fp64_1000_steps_one_launch(oracle):
@@ -81,18 +84,19 @@ def test_pendulum_fp64_1000_steps_one_launch(oracle):
     assert rel_err(q, qr) <= 1e-6
 
 
-def test_quadruped_fp64_pairs_contacts_and_state(oracle):
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP64_EXACT, abi.PRECISION_FP64])
+def test_quadruped_fp64_pairs_contacts_and_state(oracle, precision):
     sc = scenes.quadruped()
     n = 64
     q0, v0 = sc.initial_state(n, 3)
-    say("quadruped18 fp64: 64 worlds x 150 single steps, pairs/contacts each step")
+    say(f"quadruped18 precision {precision}: 64 worlds x 250 single steps, pairs/contacts each step")
     m = Model(sc)
-    d = Data(m, n, abi.PRECISION_FP64)
+    d = Data(m, n, precision)
     d.set_state(q0, v0)
     ow = oracle.OracleWorlds(sc, n, 0)
     ow.set_state(q0, v0)
     touched = 0
-    for s in range(150):
+    for s in range(250):
         step(m, d, 1)
         ow.step(1, threads=8)
         pc, pairs = d.pairs()
@@ -111,6 +115,31 @@ def test_quadruped_fp64_pairs_contacts_and_state(oracle):
     assert not d.faults().any()
     q, v = d.state()
     qr, vr = ow.state()
+    assert rel_err(q, qr) <= TOL_FP64
+    assert rel_err(v, vr) <= TOL_FP64
+    assert rel_err(d.qacc(), ow.qacc()) <= TOL_FP64
+    if precision == abi.PRECISION_FP64_EXACT:
+        assert np.array_equal(q, qr) and np.array_equal(v, vr)
+
+
+def test_quadruped_fp64_steady_state_100_steps(oracle):
+    """100 steps from a contact-rich state (robots lying on the plane)."""
+    sc = scenes.quadruped()
+    n = 128
+    q0, v0 = sc.initial_state(n, 9)
+    ow = oracle.OracleWorlds(sc, n, 0)
+    ow.set_state(q0, v0)
+    ow.step(400, threads=8)
+    qs, vs = ow.state()
+    say("quadruped18 fp64: 128 worlds x 100 steps from step 400")
+    m = Model(sc)
+    d = Data(m, n, abi.PRECISION_FP64)
+    d.set_state(qs, vs)
+    step(m, d, 100)
+    ow.step(100, threads=8)
+    q, v = d.state()
+    qr, vr = ow.state()
+    assert not d.faults().any()
     assert rel_err(q, qr) <= TOL_FP64
     assert rel_err(v, vr) <= TOL_FP64
     assert rel_err(d.qacc(), ow.qacc()) <= TOL_FP64

@@ -20,7 +20,7 @@ SHAPE_PLANE, SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE = range(4)
 INTEGRATOR_EULER, INTEGRATOR_IMPLICITFAST, INTEGRATOR_RK4 = range(3)
 SOLVER_PENALTY, SOLVER_PGS, SOLVER_NEWTON = range(3)
 CONE_PYRAMIDAL, CONE_ELLIPTIC = range(2)
-PRECISION_FP64, PRECISION_MIXED = range(2)
+PRECISION_FP64, PRECISION_MIXED, PRECISION_FP64_EXACT = range(3)
 
 D3 = C.c_double * 3
 D4 = C.c_double * 4

@@ -24,6 +24,7 @@ struct HostModel {
     std::vector<GeomT> hgeoms;
     std::vector<PairT> pairs;
     std::vector<int> chain;
+    std::vector<int> level_off, level_links;
     std::vector<double> q_default;
 };
 
@@ -143,6 +144,29 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
         }
         M->links.push_back(l);
     }
+    // level schedule and sibling ranks
+    {
+        int n = (int)M->links.size();
+        std::vector<int> nchild(n, 0);
+        int maxd = 0;
+        for (int i = 0; i < n; ++i) {
+            LinkT& l = M->links[i];
+            l.depth = l.parent < 0 ? 0 : M->links[l.parent].depth + 1;
+            l.rank = l.parent < 0 ? 0 : nchild[l.parent]++;
+            maxd = std::max(maxd, l.depth);
+        }
+        int max_rank = 1;
+        for (int c : nchild) max_rank = std::max(max_rank, c);
+        M->level_off.assign(1, 0);
+        M->level_links.clear();
+        for (int dl = 0; dl <= maxd; ++dl) {
+            for (int i = 0; i < n; ++i)
+                if (M->links[i].depth == dl) M->level_links.push_back(i);
+            M->level_off.push_back((int)M->level_links.size());
+        }
+        M->dev.n_levels = maxd + 1;
+        M->dev.max_rank = max_rank;
+    }
     // geoms
     M->hgeoms.clear();
     for (const newt_geom_desc& g : M->geoms) {
@@ -214,10 +238,12 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     d.geoms = M->hgeoms.data();
     d.pairs = M->pairs.data();
     d.chain = M->chain.data();
+    d.level_off = M->level_off.data();
+    d.level_links = M->level_links.data();
     return NEWT_OK;
 }
 
-inline Layout make_layout(const ModelT& m) {
+inline Layout make_layout(const ModelT& m, int real_bytes) {
     Layout L{};
     int r = 0, dd = 0, ii = 0;
     auto R = [&](int n) { int o = r; r += n; return o; };
@@ -260,6 +286,13 @@ inline Layout make_layout(const ModelT& m) {
     L.pairs = I(std::max(m.n_pairs, 1));
     L.iwf = I(n + 2);
     L.total_i = ii;
+    // team hand-off: Li (nv*nv f64), bias/reg (64 f64 each), J rows (64*nv R)
+    int off = 0;
+    L.sb_Li = off; off += 8 * nv * nv;
+    L.sb_bias = off; off += 8 * 64;
+    L.sb_reg = off; off += 8 * 64;
+    L.sb_J = off; off += real_bytes * 64 * nv;
+    L.sb_stride = (off + 255) / 256 * 256;
     return L;
 }
 

@@ -12,6 +12,7 @@
 #include "../../include/newt_gpu.h"
 #include "newt_compile.hpp"
 #include "newt_step.hpp"
+#include "newt_solve.hpp"
 
 using namespace newt;
 
@@ -43,6 +44,8 @@ struct newt_model {
     GeomT* d_geoms = nullptr;
     PairT* d_pairs = nullptr;
     int* d_chain = nullptr;
+    int* d_level_off = nullptr;
+    int* d_level_links = nullptr;
 };
 
 struct newt_data {
@@ -54,18 +57,27 @@ struct newt_data {
     void* ws = nullptr;      // real workspace (double or float)
     double* dws = nullptr;   // factorization workspace
     int* iws = nullptr;      // integer workspace
+    char* sb = nullptr;      // team-solve hand-off blocks, world-major
+    int* team_rows = nullptr;
     hipStream_t last_stream = nullptr;
 };
 
 namespace {
 
 template <class R>
+__device__ __forceinline__ World<R> world_of(const ModelT& m, const Layout& L, const DataT& d, R* ws, double* dws,
+                                             int* iws, int w) {
+    return World<R>{m, L, {ws, d.W, w}, {dws, d.W, w}, {iws, d.W, w}, {d.qpos, d.W, w}, {d.qvel, d.W, w},
+                    {d.qacc, d.W, w}, {d.qfrc, d.W, w}, {d.fat, d.W, w}};
+}
+
+// Bit-exact single kernel: n_steps whole World::steps per lane.
+template <class R>
 __global__ void __launch_bounds__(kBlock) k_step(ModelT m, Layout L, DataT d, R* ws, double* dws, int* iws,
                                                  int n_steps, int init) {
     int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= d.W) return;
-    World<R> wd{m, L, {ws, d.W, w}, {dws, d.W, w}, {iws, d.W, w}, {d.qpos, d.W, w}, {d.qvel, d.W, w},
-                {d.qacc, d.W, w}, {d.qfrc, d.W, w}, {d.fat, d.W, w}};
+    World<R> wd = world_of<R>(m, L, d, ws, dws, iws, w);
     if (d.fault[w] != 0) return;
     const R dt = (R)m.dt;
     wd.forward_kinematics();
@@ -90,17 +102,81 @@ __global__ void __launch_bounds__(kBlock) k_step(ModelT m, Layout L, DataT d, R*
     d.ncontact[w] = iws[(size_t)(L.iwf + m.n_links + 1) * d.W + w];
 }
 
-newt_status launch(newt_data* D, int n_steps, hipStream_t stream, int init) {
+// Lane per world: pairs, contacts, CRB, factorizations, free velocity and
+// the contact rows, handed to k_solve.  `first` runs the start-of-step
+// broadphase update (needed after a state change).
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_pre(ModelT m, Layout L, DataT d, R* ws, double* dws, int* iws, int first,
+                                                char* sb, int* team_rows) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= d.W) return;
+    team_rows[w] = 0;
+    if (d.fault[w] != 0) return;
+    World<R> wd = world_of<R>(m, L, d, ws, dws, iws, w);
+    if (first) {
+        wd.forward_kinematics();
+        wd.link_velocities();
+        wd.populate((R)m.dt, false);
+    }
+    int tr = 0;
+    int st = wd.pre(true, sb, &tr);
+    if (st != 0) {
+        d.fault[w] = st;
+        return;
+    }
+    team_rows[w] = tr;
+}
+
+// Lane per world: ABA with the contact force, Euler, FK, broadphase refresh.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_post(ModelT m, Layout L, DataT d, R* ws, double* dws, int* iws) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= d.W) return;
+    if (d.fault[w] != 0) return;
+    World<R> wd = world_of<R>(m, L, d, ws, dws, iws, w);
+    wd.post();
+    d.npair[w] = iws[(size_t)(L.iwf + m.n_links) * d.W + w];
+    d.ncontact[w] = iws[(size_t)(L.iwf + m.n_links + 1) * d.W + w];
+}
+
+template <class R>
+newt_status launch_team(newt_data* D, int n_steps, hipStream_t stream) {
     const newt_model* M = D->model;
     dim3 grid((D->W + kBlock - 1) / kBlock), block(kBlock);
-    if (D->precision == NEWT_PRECISION_FP64)
-        hipLaunchKernelGGL(k_step<double>, grid, block, 0, stream, M->dev, D->L, D->d, (double*)D->ws, D->dws, D->iws,
-                           n_steps, init);
-    else
+    R* ws = (R*)D->ws;
+    const int nv = M->dev.nv;
+    for (int s = 0; s < n_steps; ++s) {
+        hipLaunchKernelGGL(k_pre<R>, grid, block, 0, stream, M->dev, D->L, D->d, ws, D->dws, D->iws, s == 0 ? 1 : 0,
+                           D->sb, D->team_rows);
+        if (M->dev.n_pairs > 0 && M->dev.mode == NEWT_SOLVER_PGS) {
+            if (nv <= 24)
+                hipLaunchKernelGGL((k_solve<R, 24>), dim3(D->W), dim3(64), 0, stream, M->dev, D->L, D->team_rows, D->sb,
+                                   D->dws, D->W);
+            else
+                hipLaunchKernelGGL((k_solve<R, 64>), dim3(D->W), dim3(64), 0, stream, M->dev, D->L, D->team_rows, D->sb,
+                                   D->dws, D->W);
+        }
+        hipLaunchKernelGGL(k_post<R>, grid, block, 0, stream, M->dev, D->L, D->d, ws, D->dws, D->iws);
+    }
+    HIP_TRY(hipGetLastError());
+    return NEWT_OK;
+}
+
+newt_status launch(newt_data* D, int n_steps, hipStream_t stream, int init) {
+    const newt_model* M = D->model;
+    D->last_stream = stream;
+    if (!init && D->precision != NEWT_PRECISION_FP64_EXACT) {
+        if (D->precision == NEWT_PRECISION_FP64) return launch_team<double>(D, n_steps, stream);
+        return launch_team<float>(D, n_steps, stream);
+    }
+    dim3 grid((D->W + kBlock - 1) / kBlock), block(kBlock);
+    if (D->precision == NEWT_PRECISION_MIXED)
         hipLaunchKernelGGL(k_step<float>, grid, block, 0, stream, M->dev, D->L, D->d, (float*)D->ws, D->dws, D->iws,
                            n_steps, init);
+    else
+        hipLaunchKernelGGL(k_step<double>, grid, block, 0, stream, M->dev, D->L, D->d, (double*)D->ws, D->dws, D->iws,
+                           n_steps, init);
     HIP_TRY(hipGetLastError());
-    D->last_stream = stream;
     return NEWT_OK;
 }
 
@@ -165,11 +241,18 @@ newt_status newt_model_finalize(newt_model* model) {
     if (!H.pairs.empty())
         HIP_TRY(hipMemcpy(model->d_pairs, H.pairs.data(), sizeof(PairT) * H.pairs.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(model->d_chain, H.chain.data(), sizeof(int) * H.chain.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&model->d_level_off, sizeof(int) * H.level_off.size()));
+    HIP_TRY(hipMalloc(&model->d_level_links, sizeof(int) * H.level_links.size()));
+    HIP_TRY(hipMemcpy(model->d_level_off, H.level_off.data(), sizeof(int) * H.level_off.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(model->d_level_links, H.level_links.data(), sizeof(int) * H.level_links.size(),
+                      hipMemcpyHostToDevice));
     model->dev = H.dev;
     model->dev.links = model->d_links;
     model->dev.geoms = model->d_geoms;
     model->dev.pairs = model->d_pairs;
     model->dev.chain = model->d_chain;
+    model->dev.level_off = model->d_level_off;
+    model->dev.level_links = model->d_level_links;
     model->finalized = true;
     return NEWT_OK;
 }
@@ -189,29 +272,35 @@ void newt_model_destroy(newt_model* model) {
     (void)hipFree(model->d_geoms);
     (void)hipFree(model->d_pairs);
     (void)hipFree(model->d_chain);
+    (void)hipFree(model->d_level_off);
+    (void)hipFree(model->d_level_links);
     delete model;
 }
 
 newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t precision, newt_data** out) {
     if (!model || !out || n_worlds < 1) return fail(NEWT_ERR_INVALID_ARG, "bad data arguments");
     if (!model->finalized) return fail(NEWT_ERR_INVALID_ARG, "model is not finalized");
-    if (precision != NEWT_PRECISION_FP64 && precision != NEWT_PRECISION_MIXED)
+    if (precision != NEWT_PRECISION_FP64 && precision != NEWT_PRECISION_MIXED &&
+        precision != NEWT_PRECISION_FP64_EXACT)
         return fail(NEWT_ERR_INVALID_ARG, "unknown precision");
+    if (model->dev.nv > 64 && precision != NEWT_PRECISION_FP64_EXACT)
+        return fail(NEWT_ERR_UNSUPPORTED, "the wavefront contact solve handles nv <= 64");
     newt_data* D = new newt_data();
     D->model = model;
     D->W = n_worlds;
     D->precision = precision;
-    D->L = make_layout(model->dev);
+    D->L = make_layout(model->dev, precision == NEWT_PRECISION_MIXED ? 4 : 8);
     const ModelT& m = model->dev;
     size_t W = (size_t)n_worlds;
-    size_t rsz = precision == NEWT_PRECISION_FP64 ? sizeof(double) : sizeof(float);
+    size_t rsz = precision == NEWT_PRECISION_MIXED ? sizeof(float) : sizeof(double);
     auto alloc = [&](void** p, size_t bytes) -> bool { return hipMalloc(p, std::max<size_t>(bytes, 8)) == hipSuccess; };
     bool ok = alloc((void**)&D->d.qpos, W * m.nq * 8) && alloc((void**)&D->d.qvel, W * m.nv * 8) &&
               alloc((void**)&D->d.qacc, W * m.nv * 8) && alloc((void**)&D->d.qfrc, W * m.nv * 8) &&
               alloc((void**)&D->d.fat, W * m.n_geoms * 6 * 8) && alloc((void**)&D->d.fault, W * 4) &&
               alloc((void**)&D->d.ncontact, W * 4) && alloc((void**)&D->d.npair, W * 4) &&
               alloc(&D->ws, W * D->L.total_r * rsz) && alloc((void**)&D->dws, W * D->L.total_d * 8) &&
-              alloc((void**)&D->iws, W * D->L.total_i * 4);
+              alloc((void**)&D->iws, W * D->L.total_i * 4) && alloc((void**)&D->sb, W * D->L.sb_stride) &&
+              alloc((void**)&D->team_rows, W * 4);
     if (!ok) {
         newt_data_destroy(D);
         return fail(NEWT_ERR_OOM, "device allocation failed");
@@ -249,6 +338,8 @@ void newt_data_destroy(newt_data* D) {
     (void)hipFree(D->ws);
     (void)hipFree(D->dws);
     (void)hipFree(D->iws);
+    (void)hipFree(D->sb);
+    (void)hipFree(D->team_rows);
     delete D;
 }
 

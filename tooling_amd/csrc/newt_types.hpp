@@ -19,7 +19,7 @@ enum : int { SH_PLANE = 0, SH_SPHERE = 1, SH_BOX = 2, SH_CAPSULE = 3 };
 struct LinkT {
     int parent, joint, q_off, v_off;
     int chain_off, chain_len;  // ancestors root..self in ModelT::chain
-    int pad0, pad1;
+    int depth, rank;           // tree depth; index among the parent's children
     double axis[3];            // as given (joint subspace)
     double pj[3], cj[3];       // joint anchor in parent / child frame
     double pjq[4], cjq[4];     // anchor orientations (fixed joints, root pose)
@@ -51,6 +51,11 @@ struct ModelT {
     const GeomT* geoms;
     const PairT* pairs;
     const int* chain;
+    // Level schedule: links of depth d are level_links[level_off[d] ..
+    // level_off[d+1]); max_rank = largest sibling count.
+    int n_levels, max_rank;
+    const int* level_off;
+    const int* level_links;
 };
 
 // Offsets (in elements) of the per-world scratch regions.
@@ -65,6 +70,8 @@ struct Layout {
     // integer workspace
     int cids, rinfo, pairs, iwf;
     int total_i;
+    // world-major hand-off block of the team solve (byte offsets)
+    int sb_Li, sb_bias, sb_reg, sb_J, sb_stride;
 };
 
 struct DataT {
