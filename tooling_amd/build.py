@@ -1,8 +1,10 @@
 """Build the HIP library (tooling_amd/_build/libnewt_gpu.so) for gfx950.
 
 Run as ``python -m tooling_amd.build``.  hipcc cross-compiles on a host with
-no GPU.  FP contraction is off so the FP64 step keeps newt's rounding, and
-packed FP32 instructions are disabled.
+no GPU.  Packed FP32 instructions are disabled everywhere.  newt_gpu.hip holds
+the bit-exact lane-per-world step, compiled with FP contraction off so it keeps
+newt's rounding; newt_team.hip holds the tolerance-bound persistent step,
+compiled with contraction on (fused multiply-adds).
 """
 import os
 import subprocess
@@ -19,23 +21,42 @@ HIPCC_FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-shared",
-    "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
+# Per translation unit: FP contraction.
+SOURCES = {"newt_gpu.hip": ["-ffp-contract=off"], "newt_team.hip": ["-ffp-contract=fast"]}
+
 
 def build(verbose: bool = False) -> str:
+    """Compile each translation unit (in parallel) and link the library."""
     os.makedirs(OUT_DIR, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    jobs = []
+    for src, extra in SOURCES.items():
+        obj = os.path.join(OUT_DIR, src.replace(".hip", ".o"))
+        cmd = [hipcc, *compile_flags, *extra, "-c", "-o", obj, os.path.join(CSRC, src)]
+        jobs.append((src, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+    out = []
+    failed = False
+    for src, obj, p in jobs:
+        so, se = p.communicate()
+        out.append(so + se)
+        if p.returncode != 0:
+            sys.stderr.write(so + se)
+            failed = True
+    if failed:
+        raise RuntimeError("hipcc failed building libnewt_gpu.so")
     tmp = LIB + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, os.path.join(CSRC, "newt_gpu.hip")]
-    res = subprocess.run(cmd, capture_output=True, text=True)
+    res = subprocess.run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for _, o, _ in jobs],
+                         capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libnewt_gpu.so")
+        raise RuntimeError("linking libnewt_gpu.so failed")
     if verbose:
         noise = "packed-fp32-ops"
-        msg = "\n".join(l for l in (res.stdout + res.stderr).splitlines() if noise not in l)
+        msg = "\n".join(l for l in "".join(out).splitlines() if noise not in l)
         if msg.strip():
             print(msg)
     os.replace(tmp, LIB)

@@ -24,7 +24,7 @@ struct HostModel {
     std::vector<GeomT> hgeoms;
     std::vector<PairT> pairs;
     std::vector<int> chain;
-    std::vector<int> level_off, level_links;
+    std::vector<int> level_off, level_links, level_nrank, dof_link;
     std::vector<double> q_default;
 };
 
@@ -153,6 +153,7 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
             LinkT& l = M->links[i];
             l.depth = l.parent < 0 ? 0 : M->links[l.parent].depth + 1;
             l.rank = l.parent < 0 ? 0 : nchild[l.parent]++;
+            l.anc = (l.parent < 0 ? 0ull : M->links[l.parent].anc) | (i < 64 ? 1ull << i : 0ull);
             maxd = std::max(maxd, l.depth);
         }
         int max_rank = 1;
@@ -166,6 +167,21 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
         }
         M->dev.n_levels = maxd + 1;
         M->dev.max_rank = max_rank;
+        M->level_nrank.assign(maxd + 1, 0);
+        M->dof_link.clear();
+        for (int i = 0; i < n; ++i) {
+            const LinkT& l = M->links[i];
+            int cnt = l.joint == NEWT_JOINT_FREE ? 6 : (l.joint == NEWT_JOINT_FIXED ? 0 : 1);
+            for (int k = 0; k < cnt; ++k) M->dof_link.push_back(i);
+        }
+        M->dev.damped = 0;
+        for (int i = 0; i < n; ++i) {
+            const LinkT& l = M->links[i];
+            M->level_nrank[l.depth] = std::max(M->level_nrank[l.depth], l.rank + 1);
+            if ((l.joint == NEWT_JOINT_FREE && l.free_damping != 0.0) ||
+                ((l.joint == NEWT_JOINT_HINGE || l.joint == NEWT_JOINT_SLIDE) && l.damping != 0.0))
+                M->dev.damped = 1;
+        }
     }
     // geoms
     M->hgeoms.clear();
@@ -240,6 +256,8 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     d.chain = M->chain.data();
     d.level_off = M->level_off.data();
     d.level_links = M->level_links.data();
+    d.level_nrank = M->level_nrank.data();
+    d.dof_link = M->dof_link.data();
     return NEWT_OK;
 }
 
@@ -286,13 +304,7 @@ inline Layout make_layout(const ModelT& m, int real_bytes) {
     L.pairs = I(std::max(m.n_pairs, 1));
     L.iwf = I(n + 2);
     L.total_i = ii;
-    // team hand-off: Li (nv*nv f64), bias/reg (64 f64 each), J rows (64*nv R)
-    int off = 0;
-    L.sb_Li = off; off += 8 * nv * nv;
-    L.sb_bias = off; off += 8 * 64;
-    L.sb_reg = off; off += 8 * 64;
-    L.sb_J = off; off += real_bytes * 64 * nv;
-    L.sb_stride = (off + 255) / 256 * 256;
+    (void)real_bytes;
     return L;
 }
 

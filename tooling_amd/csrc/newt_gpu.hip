@@ -12,7 +12,16 @@
 #include "../../include/newt_gpu.h"
 #include "newt_compile.hpp"
 #include "newt_step.hpp"
-#include "newt_solve.hpp"
+#include "newt_team.hpp"
+
+namespace newt {
+// Defined in newt_team.hip (the persistent-step instantiations).
+hipError_t team_launch(bool mixed, int nv, dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const ModelT& m,
+                       const TeamLayout& T, const CacheLayout& C, const int4* blob, const Layout& L, const DataT& d,
+                       double* dws, int* iws, int n_steps, int wpb);
+hipError_t team_set_lds(bool mixed, int nv, int bytes);
+int team_max_worlds_for(int nv);
+}  // namespace newt
 
 using namespace newt;
 
@@ -33,6 +42,7 @@ newt_status fail(newt_status s, const std::string& msg) {
     } while (0)
 
 constexpr int kBlock = 64;  // one wavefront: 64 worlds
+constexpr int kMaxLds = 160 * 1024;  // LDS of one gfx950 workgroup
 
 }  // namespace
 
@@ -46,6 +56,10 @@ struct newt_model {
     int* d_chain = nullptr;
     int* d_level_off = nullptr;
     int* d_level_links = nullptr;
+    int* d_level_nrank = nullptr;
+    int* d_dof_link = nullptr;
+    CacheLayout cache{};     // model tables as the persistent step caches them in LDS
+    char* d_blob = nullptr;  // the same tables packed in that layout
 };
 
 struct newt_data {
@@ -53,12 +67,13 @@ struct newt_data {
     int W = 0;
     int precision = 0;
     Layout L{};
+    TeamLayout TL{};  // LDS layout of one world in the persistent step
+    int wpb = 1;      // worlds (wavefronts) per workgroup of the persistent step
+    int lds_bytes = 0;
     DataT d{};
     void* ws = nullptr;      // real workspace (double or float)
     double* dws = nullptr;   // factorization workspace
     int* iws = nullptr;      // integer workspace
-    char* sb = nullptr;      // team-solve hand-off blocks, world-major
-    int* team_rows = nullptr;
     hipStream_t last_stream = nullptr;
 };
 
@@ -102,73 +117,19 @@ __global__ void __launch_bounds__(kBlock) k_step(ModelT m, Layout L, DataT d, R*
     d.ncontact[w] = iws[(size_t)(L.iwf + m.n_links + 1) * d.W + w];
 }
 
-// Lane per world: pairs, contacts, CRB, factorizations, free velocity and
-// the contact rows, handed to k_solve.  `first` runs the start-of-step
-// broadphase update (needed after a state change).
-template <class R>
-__global__ void __launch_bounds__(kBlock) k_pre(ModelT m, Layout L, DataT d, R* ws, double* dws, int* iws, int first,
-                                                char* sb, int* team_rows) {
-    int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= d.W) return;
-    team_rows[w] = 0;
-    if (d.fault[w] != 0) return;
-    World<R> wd = world_of<R>(m, L, d, ws, dws, iws, w);
-    if (first) {
-        wd.forward_kinematics();
-        wd.link_velocities();
-        wd.populate((R)m.dt, false);
-    }
-    int tr = 0;
-    int st = wd.pre(true, sb, &tr);
-    if (st != 0) {
-        d.fault[w] = st;
-        return;
-    }
-    team_rows[w] = tr;
-}
-
-// Lane per world: ABA with the contact force, Euler, FK, broadphase refresh.
-template <class R>
-__global__ void __launch_bounds__(kBlock) k_post(ModelT m, Layout L, DataT d, R* ws, double* dws, int* iws) {
-    int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= d.W) return;
-    if (d.fault[w] != 0) return;
-    World<R> wd = world_of<R>(m, L, d, ws, dws, iws, w);
-    wd.post();
-    d.npair[w] = iws[(size_t)(L.iwf + m.n_links) * d.W + w];
-    d.ncontact[w] = iws[(size_t)(L.iwf + m.n_links + 1) * d.W + w];
-}
-
-template <class R>
-newt_status launch_team(newt_data* D, int n_steps, hipStream_t stream) {
+newt_status launch_persistent(newt_data* D, int n_steps, hipStream_t stream) {
     const newt_model* M = D->model;
-    dim3 grid((D->W + kBlock - 1) / kBlock), block(kBlock);
-    R* ws = (R*)D->ws;
-    const int nv = M->dev.nv;
-    for (int s = 0; s < n_steps; ++s) {
-        hipLaunchKernelGGL(k_pre<R>, grid, block, 0, stream, M->dev, D->L, D->d, ws, D->dws, D->iws, s == 0 ? 1 : 0,
-                           D->sb, D->team_rows);
-        if (M->dev.n_pairs > 0 && M->dev.mode == NEWT_SOLVER_PGS) {
-            if (nv <= 24)
-                hipLaunchKernelGGL((k_solve<R, 24>), dim3(D->W), dim3(64), 0, stream, M->dev, D->L, D->team_rows, D->sb,
-                                   D->dws, D->W);
-            else
-                hipLaunchKernelGGL((k_solve<R, 64>), dim3(D->W), dim3(64), 0, stream, M->dev, D->L, D->team_rows, D->sb,
-                                   D->dws, D->W);
-        }
-        hipLaunchKernelGGL(k_post<R>, grid, block, 0, stream, M->dev, D->L, D->d, ws, D->dws, D->iws);
-    }
-    HIP_TRY(hipGetLastError());
+    const int wpb = D->wpb;
+    dim3 grid((D->W + wpb - 1) / wpb), block(64 * wpb);
+    HIP_TRY(team_launch(D->precision == NEWT_PRECISION_MIXED, M->dev.nv, grid, block, D->lds_bytes, stream, M->dev,
+                        D->TL, M->cache, (const int4*)M->d_blob, D->L, D->d, D->dws, D->iws, n_steps, wpb));
     return NEWT_OK;
 }
 
 newt_status launch(newt_data* D, int n_steps, hipStream_t stream, int init) {
     const newt_model* M = D->model;
     D->last_stream = stream;
-    if (!init && D->precision != NEWT_PRECISION_FP64_EXACT) {
-        if (D->precision == NEWT_PRECISION_FP64) return launch_team<double>(D, n_steps, stream);
-        return launch_team<float>(D, n_steps, stream);
-    }
+    if (!init && D->precision != NEWT_PRECISION_FP64_EXACT) return launch_persistent(D, n_steps, stream);
     dim3 grid((D->W + kBlock - 1) / kBlock), block(kBlock);
     if (D->precision == NEWT_PRECISION_MIXED)
         hipLaunchKernelGGL(k_step<float>, grid, block, 0, stream, M->dev, D->L, D->d, (float*)D->ws, D->dws, D->iws,
@@ -246,6 +207,13 @@ newt_status newt_model_finalize(newt_model* model) {
     HIP_TRY(hipMemcpy(model->d_level_off, H.level_off.data(), sizeof(int) * H.level_off.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(model->d_level_links, H.level_links.data(), sizeof(int) * H.level_links.size(),
                       hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&model->d_level_nrank, sizeof(int) * H.level_nrank.size()));
+    HIP_TRY(hipMalloc(&model->d_dof_link, sizeof(int) * std::max<size_t>(1, H.dof_link.size())));
+    if (!H.dof_link.empty())
+        HIP_TRY(hipMemcpy(model->d_dof_link, H.dof_link.data(), sizeof(int) * H.dof_link.size(),
+                          hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(model->d_level_nrank, H.level_nrank.data(), sizeof(int) * H.level_nrank.size(),
+                      hipMemcpyHostToDevice));
     model->dev = H.dev;
     model->dev.links = model->d_links;
     model->dev.geoms = model->d_geoms;
@@ -253,6 +221,27 @@ newt_status newt_model_finalize(newt_model* model) {
     model->dev.chain = model->d_chain;
     model->dev.level_off = model->d_level_off;
     model->dev.level_links = model->d_level_links;
+    model->dev.level_nrank = model->d_level_nrank;
+    model->dev.dof_link = model->d_dof_link;
+    {
+        const ModelT& hd = H.dev;
+        CacheLayout C = make_cache_layout(hd);
+        std::vector<char> blob(C.bytes, 0);
+        auto put = [&](int off, const void* src, size_t bytes) {
+            if (bytes) std::memcpy(blob.data() + off, src, bytes);
+        };
+        put(C.links, H.links.data(), sizeof(LinkT) * H.links.size());
+        put(C.geoms, H.hgeoms.data(), sizeof(GeomT) * H.hgeoms.size());
+        put(C.pairs, H.pairs.data(), sizeof(PairT) * H.pairs.size());
+        put(C.chain, H.chain.data(), sizeof(int) * H.chain.size());
+        put(C.level_off, H.level_off.data(), sizeof(int) * H.level_off.size());
+        put(C.level_links, H.level_links.data(), sizeof(int) * H.level_links.size());
+        put(C.level_nrank, H.level_nrank.data(), sizeof(int) * H.level_nrank.size());
+        put(C.dof_link, H.dof_link.data(), sizeof(int) * H.dof_link.size());
+        HIP_TRY(hipMalloc(&model->d_blob, blob.size()));
+        HIP_TRY(hipMemcpy(model->d_blob, blob.data(), blob.size(), hipMemcpyHostToDevice));
+        model->cache = C;
+    }
     model->finalized = true;
     return NEWT_OK;
 }
@@ -274,6 +263,9 @@ void newt_model_destroy(newt_model* model) {
     (void)hipFree(model->d_chain);
     (void)hipFree(model->d_level_off);
     (void)hipFree(model->d_level_links);
+    (void)hipFree(model->d_level_nrank);
+    (void)hipFree(model->d_dof_link);
+    (void)hipFree(model->d_blob);
     delete model;
 }
 
@@ -283,13 +275,30 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
     if (precision != NEWT_PRECISION_FP64 && precision != NEWT_PRECISION_MIXED &&
         precision != NEWT_PRECISION_FP64_EXACT)
         return fail(NEWT_ERR_INVALID_ARG, "unknown precision");
-    if (model->dev.nv > 64 && precision != NEWT_PRECISION_FP64_EXACT)
-        return fail(NEWT_ERR_UNSUPPORTED, "the wavefront contact solve handles nv <= 64");
+    if ((model->dev.nv > 32 || model->dev.n_links > 64) && precision != NEWT_PRECISION_FP64_EXACT)
+        return fail(NEWT_ERR_UNSUPPORTED, "the wavefront-per-world step handles nv <= 32 and <= 64 links; "
+                                          "use NEWT_PRECISION_FP64_EXACT for larger trees");
+    const TeamLayout TL = make_team_layout(model->dev, precision == NEWT_PRECISION_MIXED ? 4 : 8);
+    const int cache_bytes = model->cache.bytes;
+    const int max_wpb = team_max_worlds_for(model->dev.nv);
+    const int wpb = std::min(max_wpb, (kMaxLds - cache_bytes) / std::max(TL.bytes, 1));
+    if (precision != NEWT_PRECISION_FP64_EXACT && wpb < 1)
+        return fail(NEWT_ERR_UNSUPPORTED, "one world's step state exceeds the 160 KiB of LDS of a workgroup");
     newt_data* D = new newt_data();
     D->model = model;
     D->W = n_worlds;
     D->precision = precision;
     D->L = make_layout(model->dev, precision == NEWT_PRECISION_MIXED ? 4 : 8);
+    D->TL = TL;
+    D->wpb = std::max(wpb, 1);
+    D->lds_bytes = cache_bytes + D->wpb * TL.bytes;
+    if (precision != NEWT_PRECISION_FP64_EXACT && D->lds_bytes > 65536) {
+        hipError_t e = team_set_lds(precision == NEWT_PRECISION_MIXED, model->dev.nv, D->lds_bytes);
+        if (e != hipSuccess) {
+            delete D;
+            return fail(NEWT_ERR_DEVICE, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+        }
+    }
     const ModelT& m = model->dev;
     size_t W = (size_t)n_worlds;
     size_t rsz = precision == NEWT_PRECISION_MIXED ? sizeof(float) : sizeof(double);
@@ -299,8 +308,7 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
               alloc((void**)&D->d.fat, W * m.n_geoms * 6 * 8) && alloc((void**)&D->d.fault, W * 4) &&
               alloc((void**)&D->d.ncontact, W * 4) && alloc((void**)&D->d.npair, W * 4) &&
               alloc(&D->ws, W * D->L.total_r * rsz) && alloc((void**)&D->dws, W * D->L.total_d * 8) &&
-              alloc((void**)&D->iws, W * D->L.total_i * 4) && alloc((void**)&D->sb, W * D->L.sb_stride) &&
-              alloc((void**)&D->team_rows, W * 4);
+              alloc((void**)&D->iws, W * D->L.total_i * 4);
     if (!ok) {
         newt_data_destroy(D);
         return fail(NEWT_ERR_OOM, "device allocation failed");
@@ -338,8 +346,6 @@ void newt_data_destroy(newt_data* D) {
     (void)hipFree(D->ws);
     (void)hipFree(D->dws);
     (void)hipFree(D->iws);
-    (void)hipFree(D->sb);
-    (void)hipFree(D->team_rows);
     delete D;
 }
 

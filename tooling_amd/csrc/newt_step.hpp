@@ -35,8 +35,6 @@ template <class R> struct Sum {
     NEWT_HD void add(R x) { acc = acc + x; }
 };
 
-constexpr int kTeamRows = 64;  // rows a wavefront-per-world solve holds in registers
-
 template <class R> struct World {
     const ModelT& m;
     const Layout& L;
@@ -636,7 +634,7 @@ template <class R> struct World {
 
     // Builds rows, Delassus, PGS; leaves the generalized contact force in
     // ws[L.delta].  Returns false if a mass matrix is not positive definite.
-    NEWT_HD bool solve_contacts(int nc, R dt, int* nrows_out, bool team, char* sb, int* team_rows) const {
+    NEWT_HD bool solve_contacts(int nc, R dt, int* nrows_out) const {
         const int nv = m.nv;
         const double dtd = m.dt;
         for (int s = 0; s < nv; ++s) dw[L.delta + s] = 0.0;
@@ -767,28 +765,6 @@ template <class R> struct World {
         }
         *nrows_out = nr;
         if (nr == 0) return true;
-        if (team && nr <= kTeamRows) {
-            // Hand the rows to the wavefront-per-world solve: J rows (components
-            // summed), bias, regularizer and the response factor, world-major.
-            char* base = sb + (size_t)ws.w * L.sb_stride;
-            double* sLi = (double*)(base + L.sb_Li);
-            double* sbias = (double*)(base + L.sb_bias);
-            double* sreg = (double*)(base + L.sb_reg);
-            R* sJ = (R*)(base + L.sb_J);
-            for (int k = 0; k < nv * nv; ++k) sLi[k] = dw[L.Li + k];
-            for (int row = 0; row < nr; ++row) {
-                sbias[row] = dw[L.rbias + row];
-                sreg[row] = dw[L.rreg + row];
-                int ncomp = iw[L.rinfo + 3 * row];
-                for (int s = 0; s < nv; ++s) {
-                    double v = 0.0;
-                    for (int cidx = 0; cidx < ncomp; ++cidx) v = v + (double)ws[L.coef + (row * 2 + cidx) * nv + s];
-                    sJ[row * nv + s] = (R)v;
-                }
-            }
-            *team_rows = nr;
-            return true;
-        }
         // Delassus A = J (M+dtB)^-1 Jᵀ by columns (solver.rs:3045-3065): FP64
         // solves and dot products, stored in R.
         const int cap = m.cap_r;
@@ -841,12 +817,11 @@ template <class R> struct World {
 
     // ---------------- one World::step --------------------------------------
     // Precondition: poses and link velocities describe the current state.
-    // pre: broadphase pairs, contacts, rows and (unless handed to the team
-    // solve) the contact solve; returns a fault code.
-    NEWT_HD int pre(bool team, char* sb, int* team_rows) const {
+    // pre: broadphase pairs, contacts and the contact solve; returns a fault
+    // code.
+    NEWT_HD int pre() const {
         const R dt = (R)m.dt;
         const int nv = m.nv;
-        *team_rows = 0;
         int np = active_pairs();
         int nc = 0;
         bool constraint = m.mode == 1;
@@ -854,7 +829,7 @@ template <class R> struct World {
         if (constraint) {
             nc = narrowphase(np);
             int nr;
-            if (!solve_contacts(nc, dt, &nr, team, sb, team_rows)) return 3;  // NEWT_ERR_NOT_PD
+            if (!solve_contacts(nc, dt, &nr)) return 3;  // NEWT_ERR_NOT_PD
         }
         iw[L.iwf + m.n_links] = np;
         iw[L.iwf + m.n_links + 1] = nc;
@@ -901,8 +876,7 @@ template <class R> struct World {
     }
 
     NEWT_HD int step() const {
-        int tr;
-        int st = pre(false, nullptr, &tr);
+        int st = pre();
         if (st != 0) return st;
         post();
         return 0;

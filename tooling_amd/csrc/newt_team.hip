@@ -1,0 +1,43 @@
+// newt_team.hip — instantiations of the persistent wavefront-per-world step
+// (newt_team.hpp) for the two register-array widths, and their launchers.
+#include <hip/hip_runtime.h>
+
+#include "newt_team.hpp"
+
+namespace newt {
+
+namespace {
+
+template <class R, int NV>
+hipError_t launch_nv(dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const ModelT& m, const TeamLayout& T,
+                     const CacheLayout& C, const int4* blob, const Layout& L, const DataT& d, double* dws, int* iws,
+                     int n_steps, int wpb) {
+    hipLaunchKernelGGL((k_team<R, NV>), grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, n_steps, wpb);
+    return hipGetLastError();
+}
+
+template <class R, int NV>
+hipError_t set_lds_nv(int bytes) {
+    return hipFuncSetAttribute((const void*)k_team<R, NV>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+}  // namespace
+
+int team_max_worlds_for(int nv) { return nv <= 24 ? team_max_worlds<24>() : team_max_worlds<32>(); }
+
+hipError_t team_launch(bool mixed, int nv, dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const ModelT& m,
+                       const TeamLayout& T, const CacheLayout& C, const int4* blob, const Layout& L, const DataT& d,
+                       double* dws, int* iws, int n_steps, int wpb) {
+    if (mixed)
+        return nv <= 24 ? launch_nv<float, 24>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, n_steps, wpb)
+                        : launch_nv<float, 32>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, n_steps, wpb);
+    return nv <= 24 ? launch_nv<double, 24>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, n_steps, wpb)
+                    : launch_nv<double, 32>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, n_steps, wpb);
+}
+
+hipError_t team_set_lds(bool mixed, int nv, int bytes) {
+    if (mixed) return nv <= 24 ? set_lds_nv<float, 24>(bytes) : set_lds_nv<float, 32>(bytes);
+    return nv <= 24 ? set_lds_nv<double, 24>(bytes) : set_lds_nv<double, 32>(bytes);
+}
+
+}  // namespace newt

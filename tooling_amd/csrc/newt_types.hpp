@@ -20,6 +20,7 @@ struct LinkT {
     int parent, joint, q_off, v_off;
     int chain_off, chain_len;  // ancestors root..self in ModelT::chain
     int depth, rank;           // tree depth; index among the parent's children
+    unsigned long long anc;    // bit j set when link j is this link or an ancestor (links < 64)
     double axis[3];            // as given (joint subspace)
     double pj[3], cj[3];       // joint anchor in parent / child frame
     double pjq[4], cjq[4];     // anchor orientations (fixed joints, root pose)
@@ -52,10 +53,15 @@ struct ModelT {
     const PairT* pairs;
     const int* chain;
     // Level schedule: links of depth d are level_links[level_off[d] ..
-    // level_off[d+1]); max_rank = largest sibling count.
+    // level_off[d+1]); max_rank = largest sibling count; level_nrank[d] =
+    // largest sibling count among the links of depth d.
     int n_levels, max_rank;
     const int* level_off;
     const int* level_links;
+    const int* level_nrank;
+    const int* dof_link;  // [nv] link owning each dof
+    int damped;  // some joint has damping or free_damping != 0
+    int pad0;
 };
 
 // Offsets (in elements) of the per-world scratch regions.
@@ -70,8 +76,6 @@ struct Layout {
     // integer workspace
     int cids, rinfo, pairs, iwf;
     int total_i;
-    // world-major hand-off block of the team solve (byte offsets)
-    int sb_Li, sb_bias, sb_reg, sb_J, sb_stride;
 };
 
 struct DataT {
