@@ -169,3 +169,59 @@ def test_quadruped_mixed_one_step_qacc(oracle):
         ref.close()
     say(f"mixed qacc relative error {worst:.3e}")
     assert worst <= TOL_MIXED_QACC
+
+
+def _settled(oracle, sc, n, seed, steps):
+    ow = oracle.OracleWorlds(sc, n, 0)
+    q0, v0 = sc.initial_state(n, seed)
+    ow.set_state(q0, v0)
+    ow.step(steps, threads=8)
+    return ow
+
+
+@pytest.mark.parametrize("n_boxes", [6, 20])
+def test_box_chain_fp64_contact_sets_over_64_rows(oracle, n_boxes):
+    """Up to 4*n_boxes contacts: rows past the 64 a wavefront keeps in
+    registers take the global-memory row path; 20 boxes (nv 25) also take
+    the wider register-array instantiation."""
+    sc = scenes.box_chain(n_boxes)
+    n = 32
+    ow = _settled(oracle, sc, n, 4, 60)
+    qs, vs = ow.state()
+    say(f"{sc.name} fp64: {n} worlds x 40 steps from a settled state")
+    m = Model(sc)
+    d = Data(m, n, abi.PRECISION_FP64)
+    d.set_state(qs, vs)
+    most = 0
+    for _ in range(4):
+        step(m, d, 10)
+        ow.step(10, threads=8)
+        cc, ids, _geo = d.contacts()
+        for w in range(n):
+            ir, _gr = ow.contacts(w)
+            assert cc[w] == len(ir)
+            assert np.array_equal(ids[w, :cc[w]], ir)
+        most = max(most, int(cc.max()))
+    assert most > 16, "no world reached more than 64 rows"
+    assert not d.faults().any()
+    q, v = d.state()
+    qr, vr = ow.state()
+    assert rel_err(q, qr) <= TOL_FP64
+    assert rel_err(v, vr) <= TOL_FP64
+    assert rel_err(d.qacc(), ow.qacc()) <= TOL_FP64
+
+
+def test_box_chain_mixed_one_step_qacc(oracle):
+    sc = scenes.box_chain(6)
+    n = 64
+    ow = _settled(oracle, sc, n, 6, 80)
+    qs, vs = ow.state()
+    say(f"{sc.name} mixed: {n} worlds, qacc of one step from settled states")
+    m = Model(sc)
+    d = Data(m, n, abi.PRECISION_MIXED)
+    d.set_state(qs, vs)
+    step(m, d, 1)
+    ow.step(1, threads=8)
+    err = rel_err(d.qacc(), ow.qacc())
+    say(f"mixed qacc relative error {err:.3e}")
+    assert err <= TOL_MIXED_QACC

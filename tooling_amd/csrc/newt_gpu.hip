@@ -18,7 +18,7 @@ namespace newt {
 // Defined in newt_team.hip (the persistent-step instantiations).
 hipError_t team_launch(bool mixed, int nv, dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const ModelT& m,
                        const TeamLayout& T, const CacheLayout& C, const int4* blob, const Layout& L, const DataT& d,
-                       double* dws, int* iws, int n_steps, int wpb);
+                       double* dws, int* iws, char* ovf, int n_steps, int wpb);
 hipError_t team_set_lds(bool mixed, int nv, int bytes);
 int team_max_worlds_for(int nv);
 }  // namespace newt
@@ -74,6 +74,7 @@ struct newt_data {
     void* ws = nullptr;      // real workspace (double or float)
     double* dws = nullptr;   // factorization workspace
     int* iws = nullptr;      // integer workspace
+    char* ovf = nullptr;     // per-world blocks for contact sets over 64 rows (persistent step)
     hipStream_t last_stream = nullptr;
 };
 
@@ -122,7 +123,7 @@ newt_status launch_persistent(newt_data* D, int n_steps, hipStream_t stream) {
     const int wpb = D->wpb;
     dim3 grid((D->W + wpb - 1) / wpb), block(64 * wpb);
     HIP_TRY(team_launch(D->precision == NEWT_PRECISION_MIXED, M->dev.nv, grid, block, D->lds_bytes, stream, M->dev,
-                        D->TL, M->cache, (const int4*)M->d_blob, D->L, D->d, D->dws, D->iws, n_steps, wpb));
+                        D->TL, M->cache, (const int4*)M->d_blob, D->L, D->d, D->dws, D->iws, D->ovf, n_steps, wpb));
     return NEWT_OK;
 }
 
@@ -308,7 +309,8 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
               alloc((void**)&D->d.fat, W * m.n_geoms * 6 * 8) && alloc((void**)&D->d.fault, W * 4) &&
               alloc((void**)&D->d.ncontact, W * 4) && alloc((void**)&D->d.npair, W * 4) &&
               alloc(&D->ws, W * D->L.total_r * rsz) && alloc((void**)&D->dws, W * D->L.total_d * 8) &&
-              alloc((void**)&D->iws, W * D->L.total_i * 4);
+              alloc((void**)&D->iws, W * D->L.total_i * 4) &&
+              alloc((void**)&D->ovf, W * (size_t)std::max(TL.ov_stride, 16));
     if (!ok) {
         newt_data_destroy(D);
         return fail(NEWT_ERR_OOM, "device allocation failed");
@@ -346,6 +348,7 @@ void newt_data_destroy(newt_data* D) {
     (void)hipFree(D->ws);
     (void)hipFree(D->dws);
     (void)hipFree(D->iws);
+    (void)hipFree(D->ovf);
     delete D;
 }
 

@@ -11,8 +11,9 @@ namespace {
 template <class R, int NV>
 hipError_t launch_nv(dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const ModelT& m, const TeamLayout& T,
                      const CacheLayout& C, const int4* blob, const Layout& L, const DataT& d, double* dws, int* iws,
-                     int n_steps, int wpb) {
-    hipLaunchKernelGGL((k_team<R, NV>), grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, n_steps, wpb);
+                     char* ovf, int n_steps, int wpb) {
+    hipLaunchKernelGGL((k_team<R, NV>), grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps,
+                       wpb);
     return hipGetLastError();
 }
 
@@ -27,12 +28,12 @@ int team_max_worlds_for(int nv) { return nv <= 24 ? team_max_worlds<24>() : team
 
 hipError_t team_launch(bool mixed, int nv, dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const ModelT& m,
                        const TeamLayout& T, const CacheLayout& C, const int4* blob, const Layout& L, const DataT& d,
-                       double* dws, int* iws, int n_steps, int wpb) {
+                       double* dws, int* iws, char* ovf, int n_steps, int wpb) {
     if (mixed)
-        return nv <= 24 ? launch_nv<float, 24>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, n_steps, wpb)
-                        : launch_nv<float, 32>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, n_steps, wpb);
-    return nv <= 24 ? launch_nv<double, 24>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, n_steps, wpb)
-                    : launch_nv<double, 32>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, n_steps, wpb);
+        return nv <= 24 ? launch_nv<float, 24>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb)
+                        : launch_nv<float, 32>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb);
+    return nv <= 24 ? launch_nv<double, 24>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb)
+                    : launch_nv<double, 32>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb);
 }
 
 hipError_t team_set_lds(bool mixed, int nv, int bytes) {

@@ -48,12 +48,16 @@ struct TeamLayout {
     int q, qd, qfrc, qacc, qfree, tau, u, dvec;
     int poses, lvel, xup, Li, Ld, Lr, Ldr, fat, cgeo, invw;
     int ia, v, c, a, pa, fs, dd;  // union, CRB/ABA side
-    int rbias, rreg, rdiag, f;    // union, solve side
+    int rbias, rreg, f;           // union, solve side (up to 64 rows)
+    int lg;                       // union: bias | reg | diag | impulse, cap_r each (over 64 rows)
     int total_d;                  // doubles, union included
-    int j_off;                    // byte offset of the rows: cap_r x nv of R (union, solve side)
+    int j_off;                    // byte offset of the rows: 64 x nv of R (union, solve side)
     int i_off;                    // byte offset of the ints
     int cids, ccnt, coff, plist, alist, misc;
     int bytes;                    // per world, multiple of 16
+    // Global block per world holding the rows of contact sets of more than
+    // 64 rows (cap_r x nv of R; stride 0 when the model cannot produce them).
+    int ov_stride;
 };
 
 inline TeamLayout make_team_layout(const ModelT& m, int real_bytes) {
@@ -61,6 +65,7 @@ inline TeamLayout make_team_layout(const ModelT& m, int real_bytes) {
     int d = 0;
     auto D = [&](int n) { int o = d; d += n > 0 ? n : 1; return o; };
     const int n = m.n_links, nv = m.nv, nq = m.nq, cr = m.cap_r > 0 ? m.cap_r : 1;
+    const int lr = cr < 64 ? cr : 64;  // rows held in LDS
     T.q = D(nq);
     T.qd = D(nv);
     T.qfrc = D(nv);
@@ -90,14 +95,15 @@ inline TeamLayout make_team_layout(const ModelT& m, int real_bytes) {
     T.dd = D(n);
     const int aba_end = d;
     d = u0;
-    T.rbias = D(cr);
-    T.rreg = D(cr);
-    T.rdiag = D(cr);
-    T.f = D(cr > 64 ? cr : 64);
+    T.rbias = D(lr);
+    T.rreg = D(lr);
+    T.f = D(64);
     d = (d + 1) / 2 * 2;
     T.j_off = 8 * d;
-    const int solve_end_bytes = T.j_off + real_bytes * cr * (nv > 0 ? nv : 1);
-    const int union_end_bytes = std::max(8 * aba_end, solve_end_bytes);
+    const int solve_end_bytes = T.j_off + real_bytes * lr * (nv > 0 ? nv : 1);
+    T.lg = u0;
+    const int large_end_bytes = cr > 64 ? 8 * (u0 + 4 * cr) : 0;
+    const int union_end_bytes = std::max(std::max(8 * aba_end, solve_end_bytes), large_end_bytes);
     T.total_d = (union_end_bytes + 7) / 8;
     T.i_off = (union_end_bytes + 15) / 16 * 16;
     int i = 0;
@@ -109,6 +115,7 @@ inline TeamLayout make_team_layout(const ModelT& m, int real_bytes) {
     T.alist = I(m.cap_c);
     T.misc = I(8);
     T.bytes = (T.i_off + 4 * i + 15) / 16 * 16;
+    if (cr > 64) T.ov_stride = (real_bytes * cr * (nv > 0 ? nv : 1) + 255) / 256 * 256;
     return T;
 }
 
@@ -159,6 +166,13 @@ __device__ __forceinline__ void wsync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The same for a wavefront's global-memory accesses (waits for them).
+__device__ __forceinline__ void gsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 __device__ __forceinline__ unsigned long long lanes_below(int lane) {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
@@ -168,6 +182,17 @@ __device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
     return v;
 }
+
+// Where a step's contact rows live: all in LDS for up to 64 rows; beyond,
+// the rows in the world's global overflow block and the per-row scalars in
+// LDS.
+template <class R> struct RowStore {
+    R* J;          // rows x nv
+    double* bias;  // per row
+    double* reg;
+    double* diag;  // only for more than 64 rows
+    double* f;
+};
 
 // The model tables as cached in the workgroup's LDS.
 struct Tables {
@@ -190,6 +215,7 @@ struct Team {
     double* sd;
     int* si;
     R* sJ;
+    RowStore<R> ov;  // this world's overflow block (contact sets over 64 rows)
 
     __device__ __forceinline__ V3<double> ld3(int o) const { return {sd[o], sd[o + 1], sd[o + 2]}; }
     __device__ __forceinline__ void st3(int o, V3<double> v) const { sd[o] = v.x; sd[o + 1] = v.y; sd[o + 2] = v.z; }
@@ -857,7 +883,7 @@ struct Team {
 
     // Pyramidal rows n ± μ t of every active contact (solver.rs:2805-3120):
     // Jacobian row into sJ, bias and regularizer.  One lane per row.
-    __device__ __forceinline__ void rows(int na) const {
+    __device__ __forceinline__ void rows(int na, const RowStore<R>& rs) const {
         const int nv = m.nv;
         const double dt = m.dt;
         for (int l = lane; l < 4 * na; l += 64) {
@@ -874,7 +900,7 @@ struct Team {
             const V3<double> t2 = cross3(n, t1);
             const V3<double> tt = r < 2 ? t1 : t2;
             const V3<double> dir = (r & 1) == 0 ? n + tt * mu : n - tt * mu;
-            R* row = sJ + l * nv;
+            R* row = rs.J + l * nv;
             for (int s = 0; s < nv; ++s) row[s] = R(0);
             double tr = 0.0;
             for (int side = 0; side < 2; ++side) {
@@ -959,9 +985,9 @@ struct Team {
             }
             const double diag_approx = tr * (1.0 + mu * mu);
             const double base = (1.0 - imp) / imp * diag_approx;
-            sd[T.rreg + l] = 2.0 * mu * mu * base;
+            rs.reg[l] = 2.0 * mu * mu * base;
             const double refacc = -bcoef * vel - kcoef * position;
-            sd[T.rbias + l] = fvel - refacc * dt;
+            rs.bias[l] = fvel - refacc * dt;
         }
         wsync();
     }
@@ -1054,16 +1080,17 @@ struct Team {
         wsync();
     }
 
-    // Same for nr > 64 rows, without forming A: residuals from z = Y f,
-    // maintained in lanes 0..nv-1 (r_i = b_i + y_i·z).
-    __device__ __forceinline__ void pgs_large(int nr) const {
+    // Same for nr > 64 rows (rows in the world's global overflow block,
+    // per-row scalars in LDS), without forming A: residuals from z = Y f,
+    // kept in lanes 0..nv-1 (r_i = b_i + y_i·z).
+    __device__ __forceinline__ void pgs_large(int nr, const RowStore<R>& rs) const {
         const int nv = m.nv;
         for (int base = 0; base < nr; base += 64) {
             const int j = base + lane;
             const bool active = j < nr;
             double y[NV];
 #pragma unroll
-            for (int s = 0; s < NV; ++s) y[s] = (s < nv && active) ? (double)sJ[j * nv + s] : 0.0;
+            for (int s = 0; s < NV; ++s) y[s] = (s < nv && active) ? (double)rs.J[j * nv + s] : 0.0;
 #pragma unroll
             for (int k = 0; k < NV; ++k) {
                 if (k < nv) {
@@ -1078,31 +1105,30 @@ struct Team {
             for (int s = 0; s < NV; ++s) {
                 const double ys = (double)(R)y[s];
                 diag += ys * ys;
-                if (active && s < nv) sJ[j * nv + s] = (R)y[s];
+                if (active && s < nv) rs.J[j * nv + s] = (R)y[s];
             }
             if (active) {
-                sd[T.rdiag + j] = (double)(R)diag + sd[T.rreg + j];
-                sd[T.f + j] = 0.0;
+                rs.diag[j] = (double)(R)diag + rs.reg[j];
+                rs.f[j] = 0.0;
             }
         }
-        wsync();
+        gsync();
         double z = 0.0;
         for (int it = 0; it < m.iterations; ++it) {
             for (int i = 0; i < nr; ++i) {
-                const double yi = lane < nv ? (double)sJ[i * nv + lane] : 0.0;
+                const double yi = lane < nv ? (double)rs.J[i * nv + lane] : 0.0;
                 const double dot = wave_sum(yi * z);
-                const double fi = sd[T.f + i];
-                double fn = fi - ((sd[T.rbias + i] + dot) + sd[T.rreg + i] * fi) / sd[T.rdiag + i];
+                const double fi = rs.f[i];
+                double fn = fi - ((rs.bias[i] + dot) + rs.reg[i] * fi) / rs.diag[i];
                 fn = fn > 0.0 ? fn : 0.0;
                 z += yi * (fn - fi);
-                sd[T.f + i] = fn;
+                rs.f[i] = fn;  // every lane stores the same value and reads back its own
             }
         }
-        wsync();
         const double dt = m.dt;
         for (int s = lane; s < nv; s += 64) {
             double acc = 0.0;
-            for (int j = 0; j < nr; ++j) acc += (double)sJ[j * nv + s] * (sd[T.f + j] / dt);
+            for (int j = 0; j < nr; ++j) acc += (double)rs.J[j * nv + s] * (rs.f[j] / dt);
             sd[T.u + s] = acc;
         }
         wsync();
@@ -1168,9 +1194,14 @@ struct Team {
             wsync();
             aba(T.qfree);
             invweights(T.Ld, T.Ldr);
-            rows(na);
-            if (4 * na <= 64) pgs_small(4 * na);
-            else pgs_large(4 * na);
+            if (4 * na <= 64) {
+                rows(na, RowStore<R>{sJ, sd + T.rbias, sd + T.rreg, nullptr, sd + T.f});
+                pgs_small(4 * na);
+            } else {
+                rows(na, ov);
+                gsync();
+                pgs_large(4 * na, ov);
+            }
             back_subst_add();
         } else {
             aba(T.qacc);
@@ -1197,7 +1228,7 @@ template <class R, int NV>
 __global__ void __launch_bounds__(64 * team_max_worlds<NV>())
     __attribute__((amdgpu_waves_per_eu(NV <= 24 ? 2 : 1)))
     k_team(ModelT m, TeamLayout T, CacheLayout C, const int4* __restrict__ blob, Layout L, DataT d,
-           double* __restrict__ dws, int* __restrict__ iws, int n_steps, int wpb) {
+           double* __restrict__ dws, int* __restrict__ iws, char* __restrict__ ovf, int n_steps, int wpb) {
     extern __shared__ __align__(16) double smem[];
     char* lds = (char*)smem;
     for (int k = threadIdx.x; k < C.bytes / 16; k += blockDim.x) ((int4*)lds)[k] = blob[k];
@@ -1215,7 +1246,10 @@ __global__ void __launch_bounds__(64 * team_max_worlds<NV>())
     double* sd = (double*)base;
     int* si = (int*)(base + T.i_off);
     R* sJ = (R*)(base + T.j_off);
-    Team<R, NV> t{m, T, tb, lane, sd, si, sJ};
+    const int cr = m.cap_r;
+    const RowStore<R> ov{(R*)(ovf + (size_t)w * T.ov_stride), sd + T.lg, sd + T.lg + cr, sd + T.lg + 2 * cr,
+                         sd + T.lg + 3 * cr};
+    Team<R, NV> t{m, T, tb, lane, sd, si, sJ, ov};
     for (int k = lane; k < m.nq; k += 64) sd[T.q + k] = d.qpos[(size_t)k * W + w];
     for (int k = lane; k < m.nv; k += 64) {
         sd[T.qd + k] = d.qvel[(size_t)k * W + w];

@@ -225,4 +225,43 @@ def quadruped(dt=0.002, iterations=50, friction=0.8) -> Scene:
                  "18-DoF floating-base quadruped on a plane (mu 0.8), PGS 50, Euler dt 2 ms")
 
 
-SCENES = {"pendulum3": pendulum, "quadruped18": quadruped}
+# ---------------------------------------------------------------------------
+# Large contact sets: a chain of boxes lying on the plane.
+# ---------------------------------------------------------------------------
+BOX_HALF = (0.1, 0.05, 0.03)
+
+
+def box_chain(n_boxes=6, dt=0.002, iterations=30) -> Scene:
+    """Free-root box and n_boxes-1 further boxes joined end to end by hinges
+    about the vertical axis, all lying on the plane: up to 4 corner contacts
+    per box (n_boxes=6: 24 contacts, 96 pyramidal rows), past the 64 rows a
+    wavefront holds in registers.  1 kg boxes, friction 1.0."""
+    hx = BOX_HALF[0]
+    inertia = solid_box_inertia(1.0, BOX_HALF)
+    links = [link(-1, abi.JOINT_FREE, parent_pos=(0.0, 0.0, BOX_HALF[2]), mass=1.0, inertia=inertia)]
+    for i in range(1, n_boxes):
+        links.append(link(i - 1, abi.JOINT_HINGE, axis=(0.0, 0.0, 1.0), parent_pos=(hx + 0.01, 0.0, 0.0),
+                          child_pos=(-hx - 0.01, 0.0, 0.0), mass=1.0, inertia=inertia))
+    geoms = [geom(abi.SHAPE_PLANE, friction=1.0)]
+    for i in range(n_boxes):
+        geoms.append(geom(abi.SHAPE_BOX, tree=0, link=i, size=BOX_HALF, friction=1.0))
+
+    def sampler(n, seed):
+        r = _rng(seed)
+        qpos = np.zeros((n, 7 + n_boxes - 1))
+        qpos[:, 0:2] = r.normal(0.0, 0.05, size=(n, 2))
+        qpos[:, 2] = BOX_HALF[2] + r.uniform(0.0, 0.01, size=n)
+        qpos[:, 6] = 1.0
+        qpos[:, 7:] = r.normal(0.0, 0.3, size=(n, n_boxes - 1))
+        qvel = np.zeros((n, 6 + n_boxes - 1))
+        qvel[:, 3:5] = r.normal(0.0, 0.2, size=(n, 2))
+        qvel[:, 6:] = r.normal(0.0, 0.5, size=(n, n_boxes - 1))
+        return qpos, qvel
+
+    opts = dict(dt=dt, gravity=(0.0, 0.0, -9.81), integrator=abi.INTEGRATOR_EULER,
+                solver=abi.SOLVER_PGS, iterations=iterations)
+    return Scene(f"boxchain{n_boxes}", opts, links, geoms, 0, sampler,
+                 f"{n_boxes} hinged boxes lying on a plane (up to {4 * n_boxes} contacts), PGS {iterations}")
+
+
+SCENES = {"pendulum3": pendulum, "quadruped18": quadruped, "boxchain6": box_chain}
