@@ -48,7 +48,7 @@ struct TeamLayout {
     int q, qd, qfrc, qacc, qfree, tau, u, dvec;
     int poses, lvel, xup, Li, Ld, Lr, Ldr, fat, cgeo, invw;
     int ia, v, c, a, pa, fs, dd;  // union, CRB/ABA side
-    int rbias, rreg, f;           // union, solve side (up to 64 rows)
+    int rbias, rreg, rdg, f;      // union, solve side (up to 64 rows)
     int lg;                       // union: bias | reg | diag | impulse, cap_r each (over 64 rows)
     int total_d;                  // doubles, union included
     int j_off;                    // byte offset of the rows: 64 x nv of R (union, solve side)
@@ -97,6 +97,7 @@ inline TeamLayout make_team_layout(const ModelT& m, int real_bytes) {
     d = u0;
     T.rbias = D(lr);
     T.rreg = D(lr);
+    T.rdg = D(lr);
     T.f = D(64);
     d = (d + 1) / 2 * 2;
     T.j_off = 8 * d;
@@ -158,6 +159,18 @@ __device__ __forceinline__ double put_lane(double v, double x, int lane, int i) 
     return lane == i ? x : v;
 }
 
+// v where i < lane, else 0 (same opaque lane as put_lane).
+__device__ __forceinline__ double below_lane(double v, int lane, int i) {
+    asm volatile("" : "+v"(lane));
+    return i < lane ? v : 0.0;
+}
+
+// v where i <= lane, else 0 (same opaque lane as put_lane).
+__device__ __forceinline__ double upto_lane(double v, int lane, int i) {
+    asm volatile("" : "+v"(lane));
+    return i <= lane ? v : 0.0;
+}
+
 // Orders this wavefront's LDS accesses: a wavefront's LDS operations execute
 // in issue order, so a fence for the compiler is all one world needs.
 __device__ __forceinline__ void wsync() {
@@ -190,7 +203,7 @@ template <class R> struct RowStore {
     R* J;          // rows x nv
     double* bias;  // per row
     double* reg;
-    double* diag;  // only for more than 64 rows
+    double* diag;  // Delassus diagonal (+ regularizer, over 64 rows)
     double* f;
 };
 
@@ -560,6 +573,21 @@ struct Team {
         wsync();
     }
 
+    // Column c of Xᵀ A X (m6pullback, spatial.rs:457-481) added into the
+    // row-major 6x6 at sd[dst]: the unit motion e_c through X, times A,
+    // back through Xᵀ as a force.  Six lanes, one per column, pull a link.
+    __device__ __forceinline__ void pull_col_add(const double* a, const Xf<double>& x, int c, int dst) const {
+        SM<double> e{mk3(c == 0 ? 1.0 : 0.0, c == 1 ? 1.0 : 0.0, c == 2 ? 1.0 : 0.0),
+                     mk3(c == 3 ? 1.0 : 0.0, c == 4 ? 1.0 : 0.0, c == 5 ? 1.0 : 0.0)};
+        const SF<double> f = xtforce(x, m6mul(a, xmotion(x, e)));
+        sd[dst + 0 * 6 + c] += f.t.x;
+        sd[dst + 1 * 6 + c] += f.t.y;
+        sd[dst + 2 * 6 + c] += f.t.z;
+        sd[dst + 3 * 6 + c] += f.f.x;
+        sd[dst + 4 * 6 + c] += f.f.y;
+        sd[dst + 5 * 6 + c] += f.f.z;
+    }
+
     // dynamics::mass_matrix (dynamics.rs:474-578), composite rigid bodies,
     // into sd[Li] (row-major nv x nv).
     __device__ __forceinline__ void mass_matrix() const {
@@ -567,20 +595,20 @@ struct Team {
         for (int k = lane; k < nv * nv; k += 64) sd[T.Li + k] = 0.0;
         for (int k = lane; k < n * 36; k += 64) sd[T.ia + k] = tb.links[k / 36].si[k % 36];
         wsync();
-        for (int lev = m.n_levels - 1; lev >= 1; --lev)
+        for (int lev = m.n_levels - 1; lev >= 1; --lev) {
+            const int k0 = tb.level_off[lev], cnt = tb.level_off[lev + 1] - k0;
             for (int rank = 0; rank < tb.level_nrank[lev]; ++rank) {
-                for (int k = tb.level_off[lev] + lane; k < tb.level_off[lev + 1]; k += 64) {
-                    int i = tb.level_links[k];
+                for (int idx = lane; idx < 6 * cnt; idx += 64) {
+                    const int i = tb.level_links[k0 + idx / 6];
                     const LinkT& l = tb.links[i];
                     if (l.rank != rank) continue;
-                    double a[36], out[36];
+                    double a[36];
                     ld36(T.ia + 36 * i, a);
-                    m6pullback(a, ldx(i), out);
-#pragma unroll
-                    for (int e = 0; e < 36; ++e) sd[T.ia + 36 * l.parent + e] += out[e];
+                    pull_col_add(a, ldx(i), idx % 6, T.ia + 36 * l.parent);
                 }
                 wsync();
             }
+        }
         if (tb.links[0].joint == JF_FREE) {
             int o = tb.links[0].v_off;
             for (int e = lane; e < 36; e += 64) sd[T.Li + (o + e / 6) * nv + (o + e % 6)] = sd[T.ia + e];
@@ -617,17 +645,18 @@ struct Team {
     }
 
     // Cholesky of the symmetric matrix in sd[Li] into its lower factor at
-    // sd[dst] (may be Li itself) and the pivots' reciprocals at sd[rdst].
+    // sd[dst] (may be Li itself) and the pivots' reciprocals at sd[rdst];
+    // lane k also keeps row k in a[] and 1/L_kk in rinv.
     // Right-looking, lane i holding row i in registers: at column j the pivot
     // comes from lane j by v_readlane and every lane updates its row; an
     // entry receives its updates in the same order as the reference's
     // Cholesky-Banachiewicz (dynamics.rs:602-622).  Lanes also carry values
     // above their diagonal; the stored factor has them zeroed.  Returns
     // false, uniformly, when a pivot is not positive.
-    __device__ __forceinline__ bool factor(int dst, int rdst) const {
+    __device__ __forceinline__ bool factor(int dst, int rdst, double (&a)[NV], double& rinv) const {
         const int n = m.nv;
         const int row = lane < n ? lane : n - 1;
-        double a[NV];
+        rinv = 0.0;
 #pragma unroll
         for (int s = 0; s < NV; ++s) a[s] = s < n ? sd[T.Li + row * n + s] : 0.0;
         bool ok = true;
@@ -639,21 +668,20 @@ struct Team {
                 const double piv = __builtin_sqrt(djj);
                 const double rp = 1.0 / piv;
                 a[j] = put_lane(a[j] * rp, piv, lane, j);
-                if (lane == 0) sd[rdst + j] = rp;
+                rinv = put_lane(rinv, rp, lane, j);
 #pragma unroll
                 for (int k = j + 1; k < NV; ++k)
                     if (k < n) a[k] = a[k] - a[j] * readlane_f64(a[j], k);
             }
         }
+        // Stored with the strict upper triangle zeroed (substitutions read
+        // whole rows).
         if (lane < n) {
 #pragma unroll
             for (int s = 0; s < NV; ++s)
-                if (s < n) sd[dst + lane * n + s] = a[s];
+                if (s < n) sd[dst + lane * n + s] = upto_lane(a[s], lane, s);
+            sd[rdst + lane] = rinv;
         }
-        wsync();
-        // Zero the strict upper triangle (substitutions read whole rows).
-        for (int e = lane; e < n * n; e += 64)
-            if (e % n > e / n) sd[dst + e] = 0.0;
         wsync();
         return ok;
     }
@@ -700,45 +728,48 @@ struct Team {
             }
             wsync();
         }
-        // articulated inertias: leaves -> root, siblings by rank
-        for (int lev = m.n_levels - 1; lev >= 1; --lev)
+        // articulated inertias: leaves -> root, siblings by rank; six lanes
+        // per link, one per column of the pulled-back inertia
+        for (int lev = m.n_levels - 1; lev >= 1; --lev) {
+            const int k0 = tb.level_off[lev], cnt = tb.level_off[lev + 1] - k0;
             for (int rank = 0; rank < tb.level_nrank[lev]; ++rank) {
-                for (int k = tb.level_off[lev] + lane; k < tb.level_off[lev + 1]; k += 64) {
-                    int i = tb.level_links[k];
+                for (int idx = lane; idx < 6 * cnt; idx += 64) {
+                    const int i = tb.level_links[k0 + idx / 6], col = idx % 6;
                     const LinkT& l = tb.links[i];
                     if (l.rank != rank) continue;
-                    double a[36], out6[36];
+                    double a[36];
                     ld36(T.ia + 36 * i, a);
-                    Xf<double> x = ldx(i);
-                    SF<double> pa = ldf(T.pa + 6 * i);
-                    SM<double> c = ldm(T.c + 6 * i);
+                    const Xf<double> x = ldx(i);
+                    const SF<double> pa = ldf(T.pa + 6 * i);
+                    const SM<double> c = ldm(T.c + 6 * i);
                     SF<double> paf;
                     if (l.joint == JF_FIXED) {
                         paf = pa + m6mul(a, c);
                     } else {
-                        SM<double> s = subspace(i);
-                        SF<double> ias = m6mul(a, s);
-                        double d = (sdot(s, ias) + l.armature) + dt * (l.damping + 0.0);
-                        SF<double> ps = pa + m6mul(a, c);
-                        double fs = (sd[T.tau + l.v_off] + 0.0) - l.damping * sd[T.qd + l.v_off];
-                        sd[T.fs + i] = fs;
-                        double qdd = (fs - sdot(s, ps)) / d;
+                        const SM<double> s = subspace(i);
+                        const SF<double> ias = m6mul(a, s);
+                        const double d = (sdot(s, ias) + l.armature) + dt * (l.damping + 0.0);
+                        const SF<double> ps = pa + m6mul(a, c);
+                        const double fs = (sd[T.tau + l.v_off] + 0.0) - l.damping * sd[T.qd + l.v_off];
+                        const double qdd = (fs - sdot(s, ps)) / d;
                         paf = ps + ias * qdd;
-                        double inv_d = 1.0 / d;
-                        double iv[6] = {ias.t.x, ias.t.y, ias.t.z, ias.f.x, ias.f.y, ias.f.z};
+                        const double inv_d = 1.0 / d;
+                        const double iv[6] = {ias.t.x, ias.t.y, ias.t.z, ias.f.x, ias.f.y, ias.f.z};
 #pragma unroll
                         for (int r = 0; r < 6; ++r)
 #pragma unroll
                             for (int cc = 0; cc < 6; ++cc) a[r * 6 + cc] -= (iv[r] * iv[cc]) * inv_d;
-                        sd[T.dd + i] = d;
+                        if (col == 0) {
+                            sd[T.fs + i] = fs;
+                            sd[T.dd + i] = d;
+                        }
                     }
-                    m6pullback(a, x, out6);
-#pragma unroll
-                    for (int e = 0; e < 36; ++e) sd[T.ia + 36 * l.parent + e] += out6[e];
-                    stf(T.pa + 6 * l.parent, ldf(T.pa + 6 * l.parent) + xtforce(x, paf));
+                    pull_col_add(a, x, col, T.ia + 36 * l.parent);
+                    if (col == 0) stf(T.pa + 6 * l.parent, ldf(T.pa + 6 * l.parent) + xtforce(x, paf));
                 }
                 wsync();
             }
+        }
         // root: free base solves its 6x6 system (GMat6::solve, aba.rs:614-665)
         for (int k = lane; k < m.nv; k += 64) sd[out + k] = 0.0;
         if (lane == 0) {
@@ -825,6 +856,97 @@ struct Team {
         }
     }
 
+    // Right-hand side of the free acceleration, rhs = τ − D q̇ − C(q, q̇), into
+    // sd[out]: C by recursive Newton-Euler with q̈ = 0 and gravity as each
+    // link's body force (the quantities of the ABA's first pass), forces
+    // summed leaves -> root by sibling rank.  With M + dt·D = L Lᵀ,
+    // (M + dt·D) q̈_free = rhs is the implicit-damping ABA's system
+    // (aba.rs:803-1102, tree.rs:1021-1116).  Needs store_xups().
+    __device__ __forceinline__ void rne_rhs(int out) const {
+        V3<double> g = v3of(m.gravity);
+        for (int lev = 0; lev < m.n_levels; ++lev) {
+            for (int k = tb.level_off[lev] + lane; k < tb.level_off[lev + 1]; k += 64) {
+                const int i = tb.level_links[k];
+                const LinkT& l = tb.links[i];
+                SM<double> v, acc = smz<double>();
+                if (l.joint == JF_FREE) {
+                    const int o = T.qd + l.v_off;
+                    v = {mk3(sd[o], sd[o + 1], sd[o + 2]), mk3(sd[o + 3], sd[o + 4], sd[o + 5])};
+                } else {
+                    const Xf<double> x = ldx(i);
+                    const SM<double> vp = l.parent >= 0 ? ldm(T.v + 6 * l.parent) : smz<double>();
+                    const SM<double> ap = l.parent >= 0 ? ldm(T.a + 6 * l.parent) : smz<double>();
+                    v = xmotion(x, vp);
+                    acc = xmotion(x, ap);
+                    if (l.joint != JF_FIXED) {
+                        const SM<double> sq = subspace(i) * sd[T.qd + l.v_off];
+                        v = v + sq;
+                        acc = acc + crm(v, sq);
+                    }
+                }
+                stm(T.v + 6 * i, v);
+                stm(T.a + 6 * i, acc);
+                const V3<double> fb = qrot_inv(ori(i), g * l.mass);
+                stf(T.pa + 6 * i, (m6mul(l.si, acc) + crf(v, m6mul(l.si, v))) - SF<double>{z3<double>(), fb});
+            }
+            wsync();
+        }
+        for (int lev = m.n_levels - 1; lev >= 1; --lev) {
+            for (int rank = 0; rank < tb.level_nrank[lev]; ++rank) {
+                for (int k = tb.level_off[lev] + lane; k < tb.level_off[lev + 1]; k += 64) {
+                    const int i = tb.level_links[k];
+                    const LinkT& l = tb.links[i];
+                    if (l.rank != rank) continue;
+                    stf(T.pa + 6 * l.parent, ldf(T.pa + 6 * l.parent) + xtforce(ldx(i), ldf(T.pa + 6 * i)));
+                }
+                wsync();
+            }
+        }
+        for (int i = lane; i < m.n_links; i += 64) {
+            const LinkT& l = tb.links[i];
+            const SF<double> f = ldf(T.pa + 6 * i);
+            if (l.joint == JF_FREE) {
+                const int o = l.v_off;
+                const double fc[6] = {f.t.x, f.t.y, f.t.z, f.f.x, f.f.y, f.f.z};
+                for (int k = 0; k < 6; ++k) {
+                    const double tau = m.mode == 1 ? 0.0 + sd[T.tau + o + k] : 0.0;
+                    sd[out + o + k] = (tau - l.free_damping * sd[T.qd + o + k]) - fc[k];
+                }
+            } else if (l.joint != JF_FIXED) {
+                sd[out + l.v_off] = ((sd[T.tau + l.v_off] + 0.0) - l.damping * sd[T.qd + l.v_off]) - sdot(subspace(i), f);
+            }
+        }
+        wsync();
+    }
+
+    // L⁻¹ x for one vector, x_k in lane k; L by rows in a[] (below the
+    // diagonal; the registers above it are not zeroed), 1/L_kk in rinv.
+    __device__ __forceinline__ double fwd_vec(double x, const double (&a)[NV], double rinv) const {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            if (k < m.nv) {
+                const double xk = readlane_f64(x, k) * readlane_f64(rinv, k);
+                x = put_lane(x - below_lane(a[k], lane, k) * xk, xk, lane, k);
+            }
+        }
+        return x;
+    }
+
+    // L⁻ᵀ x for one vector, x_k in lane k; L from sd[T.Li] (upper triangle
+    // zero), 1/L_kk from sd[T.Lr].
+    __device__ __forceinline__ double bwd_vec(double x) const {
+        const int nv = m.nv;
+#pragma unroll
+        for (int k = NV - 1; k >= 0; --k) {
+            if (k < nv) {
+                const double zk = readlane_f64(x, k) * sd[T.Lr + k];
+                // L[k][lane] is zero for lane > k: finished lanes keep x.
+                x = put_lane(x - sd[T.Li + k * nv + (lane < nv ? lane : 0)] * zk, zk, lane, k);
+            }
+        }
+        return x;
+    }
+
     // Translational Jacobian entry of dof s for a world point, along world
     // direction `dir` (jacobian.rs:44-110); zero when dof s is not on the
     // chain of the link whose ancestor mask is `anc`.
@@ -881,13 +1003,61 @@ struct Team {
         }
     }
 
-    // Pyramidal rows n ± μ t of every active contact (solver.rs:2805-3120):
-    // Jacobian row into sJ, bias and regularizer.  One lane per row.
-    __device__ __forceinline__ void rows(int na, const RowStore<R>& rs) const {
+    // y ← L⁻¹ y, column-oriented, with L held by rows across the lanes
+    // (lane k: a[] = row k of L, rinv = 1/L_kk): each L entry reaches every
+    // lane through v_readlane.
+    __device__ __forceinline__ void fwd_regs(double (&y)[NV], const double (&a)[NV], double rinv) const {
         const int nv = m.nv;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            if (k < nv) {
+                y[k] = y[k] * readlane_f64(rinv, k);
+#pragma unroll
+                for (int s = k + 1; s < NV; ++s)
+                    if (s < nv) y[s] = y[s] - readlane_f64(a[k], s) * y[k];
+            }
+        }
+    }
+
+    // invweights() with the factor in registers.
+    __device__ __forceinline__ void invweights_regs(const double (&la)[NV], double lrinv) const {
+        const int n = m.n_links, nv = m.nv;
+        for (int base = 0; base < 3 * n; base += 64) {
+            const int idx = base + lane;
+            const bool on = idx < 3 * n;
+            const int i = on ? idx / 3 : 0, ax = idx % 3;
+            const unsigned long long anc = tb.links[i].anc;
+            const V3<double> point = pos(i) + qrot(ori(i), z3<double>());
+            const V3<double> dir = mk3(ax == 0 ? 1.0 : 0.0, ax == 1 ? 1.0 : 0.0, ax == 2 ? 1.0 : 0.0);
+            double y[NV];
+#pragma unroll
+            for (int s = 0; s < NV; ++s) y[s] = (s < nv && on) ? jac_entry(s, anc, point, dir) : 0.0;
+            fwd_regs(y, la, lrinv);
+            double sq = 0.0;
+#pragma unroll
+            for (int s = 0; s < NV; ++s)
+                if (s < nv) sq += y[s] * y[s];
+            sd[T.f + lane] = sq;
+            wsync();
+            if (on && ax == 0) {
+                const int o = T.f + lane;
+                sd[T.invw + i] = ((sd[o] + sd[o + 1]) + sd[o + 2]) / 3.0;
+            }
+            wsync();
+        }
+    }
+
+    // Pyramidal rows n ± μ t of every active contact (solver.rs:2805-3120):
+    // the Jacobian row in FP64 registers gives the bias and, through the
+    // factor, the row y = L⁻¹ Jᵀ of Y, stored in R with the regularizer and
+    // bias.  One lane per row.
+    __device__ __forceinline__ void rows(int na, const RowStore<R>& rs, const double (&la)[NV], double lrinv) const {
+        const int nv = m.nv, nr = 4 * na;
         const double dt = m.dt;
-        for (int l = lane; l < 4 * na; l += 64) {
-            const int c = si[T.alist + (l >> 2)], r = l & 3;
+        for (int base = 0; base < nr; base += 64) {
+            const int l = base + lane;
+            const bool act = l < nr;
+            const int c = si[T.alist + ((act ? l : 0) >> 2)], r = l & 3;
             const int o = T.cgeo + 9 * c;
             const GeomT& ga = tb.geoms[si[T.cids + 3 * c]];
             const GeomT& gb = tb.geoms[si[T.cids + 3 * c + 1]];
@@ -900,34 +1070,33 @@ struct Team {
             const V3<double> t2 = cross3(n, t1);
             const V3<double> tt = r < 2 ? t1 : t2;
             const V3<double> dir = (r & 1) == 0 ? n + tt * mu : n - tt * mu;
-            R* row = rs.J + l * nv;
-            for (int s = 0; s < nv; ++s) row[s] = R(0);
+            // Jacobian row (J·dir over both attachments), FP64 in registers.
+            double y[NV];
+#pragma unroll
+            for (int s = 0; s < NV; ++s) y[s] = 0.0;
             double tr = 0.0;
+#pragma unroll
             for (int side = 0; side < 2; ++side) {
                 const GeomT& g = side == 0 ? ga : gb;
-                if (g.link < 0) continue;
-                const double sign = side == 0 ? 1.0 : -1.0;
-                const V3<double> lp = pos(g.link);
-                const Qt<double> lo = ori(g.link);
-                const V3<double> point = lp + qrot(lo, qrot_inv(lo, cp - lp));
-                const LinkT& tl = tb.links[g.link];
-                tr = tr + sd[T.invw + g.link];
-                for (int k = 0; k < tl.chain_len; ++k) {
-                    const int i = tb.chain[tl.chain_off + k];
-                    const LinkT& lk = tb.links[i];
-                    if (lk.joint == JF_FIXED) continue;
-                    const int cnt = lk.joint == JF_FREE ? 6 : 1;
-                    for (int q = 0; q < cnt; ++q) {
-                        const int s = lk.v_off + q;
-                        const double val = jac_entry(s, ~0ull, point, dir) * sign;
-                        row[s] = (R)((double)row[s] + (double)(R)val);
-                    }
+                if (g.link >= 0) {
+                    const double sign = side == 0 ? 1.0 : -1.0;
+                    const V3<double> lp = pos(g.link);
+                    const Qt<double> lo = ori(g.link);
+                    const V3<double> point = lp + qrot(lo, qrot_inv(lo, cp - lp));
+                    const unsigned long long anc = tb.links[g.link].anc;
+                    tr = tr + sd[T.invw + g.link];
+#pragma unroll
+                    for (int s = 0; s < NV; ++s)
+                        if (s < nv) y[s] += jac_entry(s, anc, point, dir) * sign;
                 }
             }
             double vel = -0.0, fvel = -0.0;
-            for (int s = 0; s < nv; ++s) {
-                vel = vel + (double)row[s] * sd[T.qd + s];
-                fvel = fvel + (double)row[s] * (sd[T.qfree + s] * dt);
+#pragma unroll
+            for (int s = 0; s < NV; ++s) {
+                if (s < nv) {
+                    vel = vel + y[s] * sd[T.qd + s];
+                    fvel = fvel + y[s] * (sd[T.qfree + s] * dt);
+                }
             }
             // SolRef / SolImp mixing and reference acceleration
             // (solver.rs:319-384, 3067-3120)
@@ -984,10 +1153,25 @@ struct Team {
                 kcoef = imp / (dmax * dmax * tc * tc * dr * dr);
             }
             const double diag_approx = tr * (1.0 + mu * mu);
-            const double base = (1.0 - imp) / imp * diag_approx;
-            rs.reg[l] = 2.0 * mu * mu * base;
+            const double rbase = (1.0 - imp) / imp * diag_approx;
             const double refacc = -bcoef * vel - kcoef * position;
-            rs.bias[l] = fvel - refacc * dt;
+            if (act) {
+                rs.reg[l] = 2.0 * mu * mu * rbase;
+                rs.bias[l] = fvel - refacc * dt;
+            }
+            // Y row: L y = J_l (solver.rs:3045-3065 factor solves), stored in
+            // R; the Delassus diagonal A_ll = ‖y‖² from the FP64 row.
+            fwd_regs(y, la, lrinv);
+            double dg = 0.0;
+#pragma unroll
+            for (int s = 0; s < NV; ++s)
+                if (s < nv) dg += y[s] * y[s];
+            if (act) {
+                rs.diag[l] = dg;
+#pragma unroll
+                for (int s = 0; s < NV; ++s)
+                    if (s < nv) rs.J[l * nv + s] = (R)y[s];
+            }
         }
         wsync();
     }
@@ -1020,25 +1204,8 @@ struct Team {
     __device__ __forceinline__ void pgs_small(int nr) const {
         const int nv = m.nv;
         const bool active = lane < nr;
-        double y[NV];
-#pragma unroll
-        for (int s = 0; s < NV; ++s) y[s] = (s < nv && active) ? (double)sJ[lane * nv + s] : 0.0;
-        // Y row of this lane: L y = J_lane, column-oriented substitution.
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            if (k < nv) {
-                y[k] = y[k] * sd[T.Lr + k];
-#pragma unroll
-                for (int s = k + 1; s < NV; ++s)
-                    if (s < nv) y[s] -= sd[T.Li + s * nv + k] * y[k];
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < NV; ++s)
-            if (active && s < nv) sJ[lane * nv + s] = (R)y[s];
-        wsync();
-        // Row `lane` of A = Y Yᵀ from the stored Y (rounded to R), eight rows
-        // at a time.
+        // Row `lane` of A = Y Yᵀ (FP64 sums of the stored R values), eight
+        // rows at a time.
         double a[64];
         const R* yl = sJ + (active ? lane : 0) * nv;
 #pragma unroll
@@ -1047,19 +1214,17 @@ struct Team {
 #pragma unroll
             for (int ii = 0; ii < 8; ++ii) acc[ii] = 0.0;
             if (ib < nr) {
-                for (int s = 0; s < nv; ++s) {
-                    const double ys = (double)yl[s];
+                for (int k = 0; k < nv; ++k) {
+                    const double yk = (double)yl[k];
 #pragma unroll
                     for (int ii = 0; ii < 8; ++ii)
-                        if (ib + ii < nr) acc[ii] += ys * (double)sJ[(ib + ii) * nv + s];
+                        if (ib + ii < nr) acc[ii] += yk * (double)sJ[(ib + ii) * nv + k];
                 }
             }
 #pragma unroll
-            for (int ii = 0; ii < 8; ++ii) a[ib + ii] = active ? (double)(R)acc[ii] : 0.0;
+            for (int ii = 0; ii < 8; ++ii) a[ib + ii] = (active && ib + ii < nr) ? acc[ii] : 0.0;
         }
-        double diag = 0.0;
-        for (int s = 0; s < nv; ++s) diag += (double)yl[s] * (double)yl[s];
-        diag = (double)(R)diag;
+        const double diag = active ? sd[T.rdg + lane] : 0.0;
         const double reg = active ? sd[T.rreg + lane] : 0.0;
         const double invd = active ? 1.0 / (diag + reg) : 0.0;
         const double cf = active ? 1.0 - reg * invd : 0.0;
@@ -1080,37 +1245,14 @@ struct Team {
         wsync();
     }
 
-    // Same for nr > 64 rows (rows in the world's global overflow block,
+    // Same for nr > 64 rows (Y rows in the world's global overflow block,
     // per-row scalars in LDS), without forming A: residuals from z = Y f,
     // kept in lanes 0..nv-1 (r_i = b_i + y_i·z).
     __device__ __forceinline__ void pgs_large(int nr, const RowStore<R>& rs) const {
         const int nv = m.nv;
-        for (int base = 0; base < nr; base += 64) {
-            const int j = base + lane;
-            const bool active = j < nr;
-            double y[NV];
-#pragma unroll
-            for (int s = 0; s < NV; ++s) y[s] = (s < nv && active) ? (double)rs.J[j * nv + s] : 0.0;
-#pragma unroll
-            for (int k = 0; k < NV; ++k) {
-                if (k < nv) {
-                    y[k] = y[k] * sd[T.Lr + k];
-#pragma unroll
-                    for (int s = k + 1; s < NV; ++s)
-                        if (s < nv) y[s] -= sd[T.Li + s * nv + k] * y[k];
-                }
-            }
-            double diag = 0.0;
-#pragma unroll
-            for (int s = 0; s < NV; ++s) {
-                const double ys = (double)(R)y[s];
-                diag += ys * ys;
-                if (active && s < nv) rs.J[j * nv + s] = (R)y[s];
-            }
-            if (active) {
-                rs.diag[j] = (double)(R)diag + rs.reg[j];
-                rs.f[j] = 0.0;
-            }
+        for (int j = lane; j < nr; j += 64) {
+            rs.diag[j] = rs.diag[j] + rs.reg[j];
+            rs.f[j] = 0.0;
         }
         gsync();
         double z = 0.0;
@@ -1134,19 +1276,10 @@ struct Team {
         wsync();
     }
 
-    // qacc = qfree + L⁻ᵀ u: back substitution with x_k in lane k.
+    // qacc = qfree + L⁻ᵀ u.
     __device__ __forceinline__ void back_subst_add() const {
-        const int nv = m.nv;
-        double x = lane < nv ? sd[T.u + lane] : 0.0;
-#pragma unroll
-        for (int k = NV - 1; k >= 0; --k) {
-            if (k < nv) {
-                const double zk = readlane_f64(x, k) * sd[T.Lr + k];
-                // L[k][lane] is zero for lane > k: finished lanes keep x.
-                x = put_lane(x - sd[T.Li + k * nv + (lane < nv ? lane : 0)] * zk, zk, lane, k);
-            }
-        }
-        if (lane < nv) sd[T.qacc + lane] = sd[T.qfree + lane] + x;
+        const double x = bwd_vec(lane < m.nv ? sd[T.u + lane] : 0.0);
+        if (lane < m.nv) sd[T.qacc + lane] = sd[T.qfree + lane] + x;
         wsync();
     }
 
@@ -1188,17 +1321,26 @@ struct Team {
         wsync();
         if (na > 0) {
             mass_matrix();
-            if (m.damped && !factor(T.Ld, T.Ldr)) return 3;  // NEWT_ERR_NOT_PD
-            if (m.damped) add_damping();
-            if (!factor(T.Li, T.Lr)) return 3;
-            wsync();
-            aba(T.qfree);
-            invweights(T.Ld, T.Ldr);
+            double la[NV], lrinv;
+            if (m.damped) {
+                if (!factor(T.Ld, T.Ldr, la, lrinv)) return 3;  // NEWT_ERR_NOT_PD
+                invweights(T.Ld, T.Ldr);
+                add_damping();
+            }
+            if (!factor(T.Li, T.Lr, la, lrinv)) return 3;
+            if (!m.damped) invweights_regs(la, lrinv);
+            // free acceleration: (M + dt·D) q̈ = τ − D q̇ − C
+            rne_rhs(T.qfree);
+            {
+                const double qf = bwd_vec(fwd_vec(lane < m.nv ? sd[T.qfree + lane] : 0.0, la, lrinv));
+                if (lane < m.nv) sd[T.qfree + lane] = qf;
+                wsync();
+            }
             if (4 * na <= 64) {
-                rows(na, RowStore<R>{sJ, sd + T.rbias, sd + T.rreg, nullptr, sd + T.f});
+                rows(na, RowStore<R>{sJ, sd + T.rbias, sd + T.rreg, sd + T.rdg, sd + T.f}, la, lrinv);
                 pgs_small(4 * na);
             } else {
-                rows(na, ov);
+                rows(na, ov, la, lrinv);
                 gsync();
                 pgs_large(4 * na, ov);
             }
