@@ -1,15 +1,28 @@
 """Throughput of the batched newt step on one MI355X.
 
 A step is one World::step of every world in the batch.  The workload is the
-18-DoF floating-base quadruped on a plane (8192 worlds, PGS 50 iterations,
-dt 2 ms, Euler).  `value` is the mixed-precision step's world-steps/s with
-the state resident in HBM; the FP64 step (the drop-in) is timed in the same
-run and reported beside it, with the ratio.  Timing uses HIP events on the
-stream the kernels run on.  The CPU baseline is the oracle's float
-instantiation (newt's own f32 arithmetic) stepping a bounded sample of the
-same worlds on the host's cores.
+18-DoF floating-base quadruped on a plane: 8192 worlds, PGS 50 iterations,
+dt 2 ms, Euler, seeded states, timed after a 300-step warmup that brings the
+batch into contact (newt/src/world.rs:1459-1537 per world and step).
 
-    python bench.py --gpus 1 --steps 200 --warmup 20
+`value` is the mixed step's world-steps/s with the state resident in HBM;
+the FP64 step (the drop-in) is timed in the same run and reported beside it.
+Each launch advances every world `--chunk` steps; HIP events on the stream
+the kernel runs on bracket every launch, so launch durations are the
+dominant kernel's durations (one kernel per launch).
+
+Reported beside the throughput:
+  roofline        HBM fraction of the algorithmic state traffic (736 B per
+                  world-step: qpos/qvel read and written, qacc written, FP64)
+  roofline_fp64   FP64 fraction of the algorithmic FLOPs (newt's own
+                  operation count, from the oracle's counting scalar on a
+                  sample of the same worlds and steps)
+  qacc_err        mixed step vs the FP64 oracle, one step from the timed
+                  states, relative (max |Δ| / max(max |ref|, 1) per world)
+  cpu_baseline    the oracle's float instantiation (newt's f32 arithmetic)
+                  stepping a sample of the same worlds on the host cores
+
+    python bench.py --gpus 1 --steps 200 --warmup 300
 """
 import argparse
 import json
@@ -20,22 +33,27 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic HBM bytes per world-step: state read + state write + qacc write
-# (qpos FP64; qvel/qacc FP64 in the FP64 step, FP32 in the mixed step).
-def bytes_per_world_step(nq, nv, precision):
-    if precision == 0:
-        return 2 * (nq * 8 + nv * 8) + nv * 8
-    return 2 * (nq * 8 + nv * 4) + nv * 4
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6    # MI355X dense FP64
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_team_quadruped18_traffic.json")
 
 
-HBM_PEAK_GBS = 8000.0
+def bytes_per_world_step(nq, nv):
+    """State read + state write + qacc write, FP64 (qpos, qvel, qacc)."""
+    return 2 * (nq * 8 + nv * 8) + nv * 8
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def time_gpu(model, scene, worlds, precision, steps, warmup, chunk, q0, v0):
+def percentile(xs, p):
+    s = sorted(xs)
+    k = min(len(s) - 1, max(0, int(round(p / 100.0 * (len(s) - 1)))))
+    return s[k]
+
+
+def time_gpu(model, worlds, precision, steps, warmup, chunk, q0, v0, keep_state=False):
     import torch
     from tooling_amd.newt import Data, step
 
@@ -47,6 +65,7 @@ def time_gpu(model, scene, worlds, precision, steps, warmup, chunk, q0, v0):
     if warmup:
         step(model, d, warmup, sp)
     torch.cuda.synchronize()
+    warm_state = d.state() if keep_state else None
     n_launch = max(1, steps // chunk)
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(n_launch)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(n_launch)]
@@ -63,15 +82,51 @@ def time_gpu(model, scene, worlds, precision, steps, warmup, chunk, q0, v0):
     d.close()
     total_steps = n_launch * chunk
     gpu_ms = sum(launch_ms)
-    return dict(steps=total_steps, ms_per_step=gpu_ms / total_steps, wall_s=wall,
-                launch_ms=sum(launch_ms) / len(launch_ms), steps_per_launch=chunk,
-                value=worlds * total_steps / (gpu_ms / 1e3), faults=faults)
+    return dict(steps=total_steps, ms_per_step=gpu_ms / total_steps, wall_s=wall, launch_ms=launch_ms,
+                steps_per_launch=chunk, value=worlds * total_steps / (gpu_ms / 1e3), faults=faults,
+                warm_state=warm_state)
 
 
-def cpu_baseline(scene, q0, v0, warmup, steps=100):
+def qacc_error(model, scene, qs, vs, n):
+    """Mixed step vs the FP64 oracle, one step from the given states."""
+    import numpy as np
+    from oracle import pyoracle
+    from tooling_amd import abi
+    from tooling_amd.newt import Data, step
+
+    d = Data(model, n, abi.PRECISION_MIXED)
+    d.set_state(qs[:n], vs[:n])
+    step(model, d, 1)
+    a = d.qacc()
+    d.close()
+    ow = pyoracle.OracleWorlds(scene, n, 0)
+    ow.set_state(qs[:n], vs[:n])
+    ow.step(1, threads=max(1, min(os.cpu_count() or 1, 16)))
+    r = ow.qacc()
+    ow.close()
+    err = np.abs(a - r).max(axis=1) / np.maximum(np.abs(r).max(axis=1), 1.0)
+    return dict(max=float(err.max()), p99=float(np.percentile(err, 99)), median=float(np.median(err)),
+                worlds=n, above_1e5=int((err > 1e-5).sum()))
+
+
+def flops_per_world_step(scene, q0, v0, warmup, n=16, steps=50):
+    """newt's operation count per world-step (oracle counting scalar) over
+    steps warmup..warmup+steps of the first n worlds."""
+    from oracle import pyoracle
+    ow = pyoracle.OracleWorlds(scene, n, 2)
+    ow.set_state(q0[:n], v0[:n])
+    ow.step(warmup, threads=1)
+    ow.flops(reset=True)
+    ow.step(steps, threads=1)
+    c = ow.flops(reset=True)
+    ow.close()
+    return sum(c) / (n * steps), c
+
+
+def cpu_baseline(scene, q0, v0, warmup, steps):
     from oracle import pyoracle
     threads = max(1, min(os.cpu_count() or 1, 16))
-    n = 32 * threads
+    n = 64 * threads
     ow = pyoracle.OracleWorlds(scene, n, 1)  # float: newt's arithmetic
     ow.set_state(q0[:n], v0[:n])
     ow.step(warmup, threads=threads)
@@ -80,9 +135,23 @@ def cpu_baseline(scene, q0, v0, warmup, steps=100):
     dt = time.perf_counter() - t0
     ow.close()
     return dict(value=n * steps / dt, unit="world-steps/s", cores=threads, kind="port",
-                sample=f"{n} worlds of the same seeded quadruped batch, steps {warmup}..{warmup + steps} "
-                       f"(after the same untimed warmup as the GPU), C++ float restatement of newt's "
-                       f"step on {threads} host threads ({dt:.1f} s)")
+                sample=f"{n} worlds of the same seeded quadruped batch, steps {warmup}..{warmup + steps} after "
+                       f"the same untimed warmup as the GPU; the oracle's float instantiation (newt's f32 "
+                       f"step, byte-identical to newt's goldens) on {threads} host threads, one world per "
+                       f"task ({dt:.1f} s)")
+
+
+def measured_traffic(worlds, chunk):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes, when they were taken on this workload."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("worlds") != worlds or t.get("steps_per_launch") != chunk:
+        return None
+    return t.get("bytes_per_launch")
 
 
 def main():
@@ -92,13 +161,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=300,
                     help="untimed steps; 300 steps take the batch through landing into contact")
     ap.add_argument("--worlds", type=int, default=8192)
-    ap.add_argument("--chunk", type=int, default=10, help="steps per kernel launch")
+    ap.add_argument("--chunk", type=int, default=10, help="steps per kernel launch (one rollout)")
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB device buffer 5 times (reference kernel for the HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: worlds are independent and shard by world index with no "
-              "per-step exchange, so multi-GPU runs are not part of this build.", file=sys.stderr)
+              "per-step exchange, so runs on more than one GPU are not part of this build.", file=sys.stderr)
         sys.exit(2)
     if args.steps % args.chunk:
         args.chunk = 1
@@ -106,29 +178,53 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_mib": args.copy_calibration, "copies": 5}))
+        return
     from tooling_amd import abi, scenes
     from tooling_amd.newt import Model
 
     scene = scenes.quadruped()
     q0, v0 = scene.initial_state(args.worlds, args.seed)
     model = Model(scene)
-    log(f"bench: {scene.name} x {args.worlds} worlds, warmup {args.warmup}, steps {args.steps}")
-    mixed = time_gpu(model, scene, args.worlds, abi.PRECISION_MIXED, args.steps, args.warmup, args.chunk, q0, v0)
+    log(f"bench: {scene.name} x {args.worlds} worlds, warmup {args.warmup}, steps {args.steps}, "
+        f"{args.chunk} steps per launch")
+    mixed = time_gpu(model, args.worlds, abi.PRECISION_MIXED, args.steps, args.warmup, args.chunk, q0, v0,
+                     keep_state=True)
     log(f"mixed: {mixed['value']:.4g} world-steps/s, {mixed['ms_per_step']:.4f} ms/step")
-    fp64 = time_gpu(model, scene, args.worlds, abi.PRECISION_FP64, args.steps, args.warmup, args.chunk, q0, v0)
+    fp64 = time_gpu(model, args.worlds, abi.PRECISION_FP64, args.steps, args.warmup, args.chunk, q0, v0)
     log(f"fp64: {fp64['value']:.4g} world-steps/s, {fp64['ms_per_step']:.4f} ms/step")
 
-    B = bytes_per_world_step(model.nq, model.nv, abi.PRECISION_MIXED)
-    achieved = B * args.worlds * mixed["steps_per_launch"] / (mixed["launch_ms"] / 1e3) / 1e9
+    qs, vs = mixed["warm_state"]
+    err = qacc_error(model, scene, qs, vs, min(512, args.worlds))
+    log(f"mixed qacc error vs FP64 oracle: max {err['max']:.3g}, p99 {err['p99']:.3g} over {err['worlds']} worlds")
+
+    launch_ms = sum(mixed["launch_ms"]) / len(mixed["launch_ms"])
+    steps_per_launch = mixed["steps_per_launch"]
+    B = bytes_per_world_step(model.nq, model.nv)
+    achieved = B * args.worlds * steps_per_launch / (launch_ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "k_step<float>", "bytes_per_world_step": B,
-                "launch_ms": mixed["launch_ms"], "steps_per_launch": mixed["steps_per_launch"]}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.worlds, args.chunk),
+                "kernel": "newt::k_team<float, 24>", "bytes_per_world_step": B,
+                "launch_ms": launch_ms, "steps_per_launch": steps_per_launch, "worlds": args.worlds}
+    fpw, counts = flops_per_world_step(scene, q0, v0, args.warmup)
+    tflops = fpw * args.worlds * steps_per_launch / (launch_ms / 1e3) / 1e12
+    roofline_fp64 = {"bound": "fp64", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / FP64_PEAK_TFLOPS, "flop_per_world_step": fpw,
+                     "flop_counts_add_mul_div_sqrt": counts,
+                     "source": f"oracle counting scalar, 16 worlds, steps {args.warmup}..{args.warmup + 50}"}
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(scene, q0, v0, args.warmup)
+        cpu = cpu_baseline(scene, q0, v0, args.warmup, args.cpu_steps)
         log(f"cpu: {cpu['value']:.4g} world-steps/s on {cpu['cores']} threads")
+    lat = mixed["launch_ms"]
     out = {
         "metric": "world-steps/s mixed-prec vs FP64 GPU at qacc err<=1e-5; p99 rollout latency",
         "value": mixed["value"],
@@ -140,17 +236,26 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f64 (f32 Delassus rows)",
         "data": "synthetic",
         "config": {"workload": f"{scene.name}: 18-DoF floating-base quadruped on a plane, "
                                f"{args.worlds} worlds, PGS {scene.options['iterations']} iterations, "
-                               f"Euler dt {scene.options['dt']} s, seed {args.seed}",
-                   "worlds": args.worlds, "precision": "mixed (FP32 bulk, FP64 Cholesky and qpos)",
-                   "parallelism": "1 GPU, one lane per world"},
-        "fp64": {"value": fp64["value"], "ms_per_step": fp64["ms_per_step"], "dtype": "f64"},
+                               f"Euler dt {scene.options['dt']} s, seed {args.seed}, timed after {args.warmup} "
+                               f"steps",
+                   "worlds": args.worlds, "steps_per_launch": args.chunk,
+                   "precision": "mixed: FP64 state, kinematics, dynamics, factor and PGS; rows of "
+                                "Y = L^-1 J^T stored in FP32",
+                   "parallelism": "1 GPU, one wavefront per world, 8 worlds per workgroup"},
+        "fp64": {"value": fp64["value"], "ms_per_step": fp64["ms_per_step"], "dtype": "f64",
+                 "launch_ms": sum(fp64["launch_ms"]) / len(fp64["launch_ms"])},
         "speedup_vs_fp64": mixed["value"] / fp64["value"],
+        "qacc_err": err,
+        "rollout_latency_ms": {"steps_per_rollout": args.chunk, "worlds": args.worlds,
+                               "p50": percentile(lat, 50), "p99": percentile(lat, 99), "max": max(lat),
+                               "samples": len(lat)},
         "faults": {"mixed": mixed["faults"], "fp64": fp64["faults"]},
         "roofline": roofline,
+        "roofline_fp64": roofline_fp64,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -6,6 +6,7 @@
 // qacc, pairs and contacts for comparison with the HIP path.  Two
 // instantiations: precision 0 = double (the FP64 oracle), 1 = float (newt's
 // own arithmetic).
+#include "flop_count.hpp"
 #include "newt_world.hpp"
 
 #include <atomic>
@@ -165,9 +166,9 @@ template <class T> struct Impl : Base {
             ids[3 * k + 1] = cs[k].geom_b;
             ids[3 * k + 2] = cs[k].slot;
             double* g = geo + 7 * k;
-            g[0] = cs[k].pos.x; g[1] = cs[k].pos.y; g[2] = cs[k].pos.z;
-            g[3] = cs[k].normal.x; g[4] = cs[k].normal.y; g[5] = cs[k].normal.z;
-            g[6] = cs[k].penetration;
+            g[0] = (double)cs[k].pos.x; g[1] = (double)cs[k].pos.y; g[2] = (double)cs[k].pos.z;
+            g[3] = (double)cs[k].normal.x; g[4] = (double)cs[k].normal.y; g[5] = (double)cs[k].normal.z;
+            g[6] = (double)cs[k].penetration;
         }
         return n;
     }
@@ -209,6 +210,7 @@ extern "C" {
 
 void* oracle_create(const newt_options* o, int precision) {
     if (precision == 1) return new Impl<float>(*o);
+    if (precision == 2) return new Impl<Flop>(*o);
     return new Impl<double>(*o);
 }
 void oracle_destroy(void* h) { delete static_cast<Base*>(h); }
@@ -226,6 +228,17 @@ int oracle_pairs(void* h, int world, int cap, int* out) { return static_cast<Bas
 void oracle_mass_matrix(void* h, int world, double* out) { static_cast<Base*>(h)->mass_matrix(world, out); }
 void oracle_dynamics(void* h, int world, double* qacc, double* M, double* bias) {
     static_cast<Base*>(h)->dynamics(world, qacc, M, bias);
+}
+
+// Operation counts of the calling thread's Flop arithmetic since the last
+// reset: add/sub, mul, div, sqrt.
+void oracle_flops(unsigned long long* out, int reset) {
+    FlopCounts& c = flop_counts();
+    out[0] = c.add;
+    out[1] = c.mul;
+    out[2] = c.div;
+    out[3] = c.sqrt;
+    if (reset) c = FlopCounts{};
 }
 
 // Deterministic sin/cos of the oracle, for the kernel's polynomial check.

@@ -45,6 +45,7 @@ def lib():
             "oracle_mass_matrix": (None, [P, I, PD]),
             "oracle_dynamics": (None, [P, I, PD, PD, PD]),
             "oracle_golden_bytes": (I, [C.c_char_p, C.POINTER(C.c_uint8), I]),
+            "oracle_flops": (None, [C.POINTER(C.c_ulonglong), I]),
             "oracle_nsin64": (C.c_double, [C.c_double]),
             "oracle_ncos64": (C.c_double, [C.c_double]),
             "oracle_nsin32": (C.c_float, [C.c_float]),
@@ -72,7 +73,8 @@ def _dp(a):
 
 class OracleWorlds:
     """N oracle worlds built from a tooling_amd.scenes.Scene.
-    precision: 0 = double (FP64 oracle), 1 = float (newt's arithmetic)."""
+    precision: 0 = double (FP64 oracle), 1 = float (newt's arithmetic),
+    2 = double with operation counts (see flops())."""
 
     def __init__(self, scene, n_worlds, precision=0):
         L = lib()
@@ -130,6 +132,13 @@ class OracleWorlds:
         h = np.zeros(self.nv)
         self.L.oracle_dynamics(self.h, world, _dp(a), _dp(M), _dp(h))
         return a, M, h
+
+    def flops(self, reset=True):
+        """(add/sub, mul, div, sqrt) counted on this thread since the last
+        reset, for precision 2 worlds stepped with threads=1."""
+        out = (C.c_ulonglong * 4)()
+        self.L.oracle_flops(out, 1 if reset else 0)
+        return tuple(int(x) for x in out)
 
     def close(self):
         if self.h:
