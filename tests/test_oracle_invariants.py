@@ -94,3 +94,50 @@ def test_free_fall_semi_implicit_euler(oracle_lib):
     assert v[0, 5] == pytest.approx(g * dt * n, rel=1e-12)
     assert q[0, 2] == pytest.approx(10.0 + g * dt * dt * n * (n + 1) / 2, rel=1e-12)
     assert np.allclose(q[0, 3:7], [0, 0, 0, 1])
+
+
+def _single_body(shape, size, quat):
+    from tooling_amd import abi
+    links = [scenes.link(-1, abi.JOINT_FREE, mass=1.0, inertia=scenes._diag(0.01, 0.01, 0.01))]
+    geoms = [scenes.geom(abi.SHAPE_PLANE), scenes.geom(shape, tree=0, link=0, size=size, quat=quat)]
+    opts = dict(dt=0.002, gravity=(0.0, 0.0, -9.81), integrator=abi.INTEGRATOR_EULER, solver=abi.SOLVER_PGS,
+                iterations=10)
+    return scenes.Scene("single", opts, links, geoms, 0, None, "")
+
+
+def _contacts_at(oracle_lib, sc, pos, quat):
+    ow = oracle_lib.OracleWorlds(sc, 1, 0)
+    q = np.array([[*pos, *quat]])
+    ow.set_state(q, np.zeros((1, 6)))
+    ow.step(1)  # contacts are collected from the state the step starts from
+    return ow.contacts(0)
+
+
+def test_plane_primitive_contacts_closed_form(oracle_lib):
+    """Plane contacts of a capsule lying flat, a sphere and an axis-aligned
+    box, each sunk by d: contact.rs:269-403 place a contact at every
+    penetrating endpoint / corner (corners in bit order), halfway between the
+    two surfaces, normal from the body (B) to the plane (A), penetration d."""
+    from tooling_amd import abi
+    r, h, d = 0.03, 0.1, 0.004
+    ident = (0.0, 0.0, 0.0, 1.0)
+    # Capsule axis (local Z) turned onto world X.
+    to_x = scenes.quat_axis_angle((0.0, 1.0, 0.0), np.pi / 2)
+    ids, geo = _contacts_at(oracle_lib, _single_body(abi.SHAPE_CAPSULE, (r, h, 0.0), to_x), (0.0, 0.0, r - d), ident)
+    assert [tuple(i) for i in ids] == [(0, 1, 0), (0, 1, 1)]
+    ends = sorted(geo[:, 0])
+    assert np.allclose(ends, [-h, h], atol=1e-12)
+    assert np.allclose(geo[:, 2], -d / 2, atol=1e-12)
+    assert np.allclose(geo[:, 3:6], [0.0, 0.0, -1.0], atol=1e-12)
+    assert np.allclose(geo[:, 6], d, atol=1e-12)
+    ids, geo = _contacts_at(oracle_lib, _single_body(abi.SHAPE_SPHERE, (r, 0.0, 0.0), ident), (0.1, -0.2, r - d), ident)
+    assert [tuple(i) for i in ids] == [(0, 1, 0)]
+    assert np.allclose(geo[0], [0.1, -0.2, -d / 2, 0.0, 0.0, -1.0, d], atol=1e-12)
+    he = (0.2, 0.1, 0.05)
+    ids, geo = _contacts_at(oracle_lib, _single_body(abi.SHAPE_BOX, he, ident), (0.0, 0.0, he[2] - d), ident)
+    assert [tuple(i) for i in ids] == [(0, 1, k) for k in range(4)]
+    # bottom corners in bit order: x flips fastest, then y
+    expect = [(-he[0], -he[1]), (he[0], -he[1]), (-he[0], he[1]), (he[0], he[1])]
+    assert np.allclose(geo[:, :2], expect, atol=1e-12)
+    assert np.allclose(geo[:, 2], -d / 2, atol=1e-12)
+    assert np.allclose(geo[:, 6], d, atol=1e-12)
