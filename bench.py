@@ -154,6 +154,113 @@ def measured_traffic(worlds, chunk):
     return t.get("bytes_per_launch")
 
 
+def bench_chains(args):
+    """16384 worlds of fixed-root hinge chains, n ~ U{2..64} links (one model
+    per length), PGS 30, dt 1 ms; every model's batch advances `chunk` steps
+    per launch, launches spread over four streams.  Timed with the device
+    synchronized on both sides of the K steps."""
+    import numpy as np
+    import torch
+    from oracle import pyoracle
+    from tooling_amd import abi, scenes
+    from tooling_amd.newt import Data, Model, step
+
+    worlds = args.worlds if args.worlds != 8192 else 16384
+    rng = np.random.Generator(np.random.PCG64(args.seed))
+    lengths = rng.integers(2, 65, size=worlds)
+    groups = [(n, int((lengths == n).sum())) for n in range(2, 65) if (lengths == n).sum() > 0]
+    streams = [torch.cuda.Stream() for _ in range(4)]
+    log(f"bench chains: {worlds} worlds in {len(groups)} models, warmup {args.warmup}, steps {args.steps}")
+
+    def run(precision):
+        batches = []
+        for n, cnt in groups:
+            sc = scenes.hinge_chain(n)
+            m = Model(sc)
+            d = Data(m, cnt, precision)
+            q0, v0 = sc.initial_state(cnt, args.seed + n)
+            d.set_state(q0, v0)
+            batches.append((n, sc, m, d, q0, v0))
+        torch.cuda.synchronize()
+        for i, (_, _, m, d, _, _) in enumerate(batches):
+            step(m, d, args.warmup, streams[i % 4].cuda_stream)
+        torch.cuda.synchronize()
+        warm = [d.state() for (_, _, _, d, _, _) in batches]
+        t0 = time.perf_counter()
+        for _ in range(args.steps // args.chunk):
+            for i, (_, _, m, d, _, _) in enumerate(batches):
+                step(m, d, args.chunk, streams[i % 4].cuda_stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        faults = sum(int(d.faults().sum()) for (_, _, _, d, _, _) in batches)
+        return batches, warm, dt, faults
+
+    mixed, warm, t_mixed, f_mixed = run(abi.PRECISION_MIXED)
+    total = worlds * (args.steps // args.chunk) * args.chunk
+    log(f"mixed: {total / t_mixed:.4g} world-steps/s")
+    errs = []
+    for (n, sc, m, d, _, _), (qs, vs) in zip(mixed, warm):
+        k = min(4, len(qs))
+        dd = Data(m, k, abi.PRECISION_MIXED)
+        dd.set_state(qs[:k], vs[:k])
+        step(m, dd, 1)
+        a = dd.qacc()
+        dd.close()
+        ow = pyoracle.OracleWorlds(sc, k, 0)
+        ow.set_state(qs[:k], vs[:k])
+        ow.step(1, threads=k)
+        r = ow.qacc()
+        ow.close()
+        errs.extend((np.abs(a - r).max(axis=1) / np.maximum(np.abs(r).max(axis=1), 1.0)).tolist())
+    for b in mixed:
+        b[3].close()
+    fp64, _, t_fp64, f_fp64 = run(abi.PRECISION_FP64)
+    for b in fp64:
+        b[3].close()
+    log(f"fp64: {total / t_fp64:.4g} world-steps/s")
+    mean_n = float(lengths.mean())
+    B = 40.0 * mean_n  # 2*(n*8 + n*8) + n*8 bytes per world-step, FP64 state
+    achieved = B * total / t_mixed / 1e9
+    cpu = None
+    if not args.no_cpu:
+        t_cpu, ws = 0.0, 0
+        threads = max(1, min(os.cpu_count() or 1, 16))
+        for n, cnt in groups[::4]:
+            sc = scenes.hinge_chain(n)
+            k = min(cnt, threads)
+            q0, v0 = sc.initial_state(k, args.seed + n)
+            ow = pyoracle.OracleWorlds(sc, k, 1)
+            ow.set_state(q0, v0)
+            ow.step(args.warmup, threads=k)
+            t0 = time.perf_counter()
+            ow.step(100, threads=k)
+            t_cpu += time.perf_counter() - t0
+            ws += k * 100
+            ow.close()
+        cpu = dict(value=ws / t_cpu, unit="world-steps/s", cores=threads, kind="port",
+                   sample=f"up to {threads} worlds of every 4th chain length, 100 steps after the warmup, oracle "
+                          f"float (newt's f32 step), one world per host thread ({t_cpu:.1f} s)")
+    out = {
+        "metric": "world-steps/s mixed-prec vs FP64 GPU at qacc err<=1e-5; p99 rollout latency",
+        "value": total / t_mixed, "unit": "world-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t_mixed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (f32 Delassus rows)", "data": "synthetic",
+        "config": {"workload": f"chains: {worlds} worlds, fixed-root hinge chains n ~ U{{2..64}} (mean "
+                               f"{mean_n:.1f}), {len(groups)} models, PGS 30, Euler dt 1 ms, seed {args.seed}",
+                   "worlds": worlds, "steps_per_launch": args.chunk, "streams": 4,
+                   "parallelism": "1 GPU, one wavefront per world, one launch per model per chunk"},
+        "fp64": {"value": total / t_fp64, "ms_per_step": t_fp64 / args.steps * 1e3, "dtype": "f64"},
+        "speedup_vs_fp64": t_fp64 / t_mixed,
+        "qacc_err": {"max": float(max(errs)), "median": float(np.median(errs)), "worlds": len(errs)},
+        "faults": {"mixed": f_mixed, "fp64": f_fp64},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_world_step": B, "note": "wall-clock over all models' launches"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +272,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["quadruped", "chains"], default="quadruped",
+                    help="quadruped: 8192 quadrupeds (the headline); chains: fixed-root hinge chains of "
+                         "2..64 links, one model per length, batched over streams")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 5 times (reference kernel for the HBM counters)")
     args = ap.parse_args()
@@ -186,6 +296,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_mib": args.copy_calibration, "copies": 5}))
+        return
+    if args.workload == "chains":
+        bench_chains(args)
         return
     from tooling_amd import abi, scenes
     from tooling_amd.newt import Model

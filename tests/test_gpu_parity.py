@@ -225,3 +225,49 @@ def test_box_chain_mixed_one_step_qacc(oracle):
     err = rel_err(d.qacc(), ow.qacc())
     say(f"mixed qacc relative error {err:.3e}")
     assert err <= TOL_MIXED_QACC
+
+
+@pytest.mark.parametrize("n_links", [40, 64])
+def test_hinge_chain_fp64_wide(oracle, n_links):
+    """Chains of 40 and 64 hinges: the 64-wide register instantiation, LDS
+    above 64 KiB per workgroup, long level schedules."""
+    sc = scenes.hinge_chain(n_links)
+    n = 16
+    ow = _settled(oracle, sc, n, 8, 100)
+    qs, vs = ow.state()
+    say(f"{sc.name} fp64: {n} worlds x 60 steps from step 100")
+    m = Model(sc)
+    d = Data(m, n, abi.PRECISION_FP64)
+    d.set_state(qs, vs)
+    touched = 0
+    for _ in range(6):
+        step(m, d, 10)
+        ow.step(10, threads=8)
+        cc, ids, _geo = d.contacts()
+        for w in range(n):
+            ir, _gr = ow.contacts(w)
+            assert cc[w] == len(ir)
+            assert np.array_equal(ids[w, :cc[w]], ir)
+            touched += int(cc[w] > 0)
+    assert touched > 0
+    assert not d.faults().any()
+    q, v = d.state()
+    qr, vr = ow.state()
+    assert rel_err(q, qr) <= TOL_FP64
+    assert rel_err(v, vr) <= TOL_FP64
+
+
+def test_hinge_chain_mixed_one_step_qacc(oracle):
+    sc = scenes.hinge_chain(40)
+    n = 32
+    ow = _settled(oracle, sc, n, 9, 200)
+    qs, vs = ow.state()
+    say(f"{sc.name} mixed: {n} worlds, qacc of one step from step 200")
+    m = Model(sc)
+    d = Data(m, n, abi.PRECISION_MIXED)
+    d.set_state(qs, vs)
+    step(m, d, 1)
+    ow.step(1, threads=8)
+    err = rel_err(d.qacc(), ow.qacc())
+    say(f"mixed qacc relative error {err:.3e}")
+    assert err <= TOL_MIXED_QACC

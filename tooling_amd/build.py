@@ -25,7 +25,8 @@ HIPCC_FLAGS = [
 ]
 
 # Per translation unit: FP contraction.
-SOURCES = {"newt_gpu.hip": ["-ffp-contract=off"], "newt_team.hip": ["-ffp-contract=fast"]}
+SOURCES = {"newt_gpu.hip": ["-ffp-contract=off"], "newt_team.hip": ["-ffp-contract=fast"],
+           "newt_team64f.hip": ["-ffp-contract=fast"], "newt_team64d.hip": ["-ffp-contract=fast"]}
 
 
 def build(verbose: bool = False) -> str:

@@ -153,7 +153,8 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
             LinkT& l = M->links[i];
             l.depth = l.parent < 0 ? 0 : M->links[l.parent].depth + 1;
             l.rank = l.parent < 0 ? 0 : nchild[l.parent]++;
-            l.anc = (l.parent < 0 ? 0ull : M->links[l.parent].anc) | (i < 64 ? 1ull << i : 0ull);
+            for (int w = 0; w < 2; ++w) l.anc[w] = l.parent < 0 ? 0ull : M->links[l.parent].anc[w];
+            if (i < 128) l.anc[i >> 6] |= 1ull << (i & 63);
             maxd = std::max(maxd, l.depth);
         }
         int max_rank = 1;

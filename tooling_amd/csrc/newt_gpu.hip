@@ -234,7 +234,6 @@ newt_status newt_model_finalize(newt_model* model) {
         put(C.links, H.links.data(), sizeof(LinkT) * H.links.size());
         put(C.geoms, H.hgeoms.data(), sizeof(GeomT) * H.hgeoms.size());
         put(C.pairs, H.pairs.data(), sizeof(PairT) * H.pairs.size());
-        put(C.chain, H.chain.data(), sizeof(int) * H.chain.size());
         put(C.level_off, H.level_off.data(), sizeof(int) * H.level_off.size());
         put(C.level_links, H.level_links.data(), sizeof(int) * H.level_links.size());
         put(C.level_nrank, H.level_nrank.data(), sizeof(int) * H.level_nrank.size());
@@ -276,8 +275,8 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
     if (precision != NEWT_PRECISION_FP64 && precision != NEWT_PRECISION_MIXED &&
         precision != NEWT_PRECISION_FP64_EXACT)
         return fail(NEWT_ERR_INVALID_ARG, "unknown precision");
-    if ((model->dev.nv > 32 || model->dev.n_links > 64) && precision != NEWT_PRECISION_FP64_EXACT)
-        return fail(NEWT_ERR_UNSUPPORTED, "the wavefront-per-world step handles nv <= 32 and <= 64 links; "
+    if ((model->dev.nv > 64 || model->dev.n_links > 128) && precision != NEWT_PRECISION_FP64_EXACT)
+        return fail(NEWT_ERR_UNSUPPORTED, "the wavefront-per-world step handles nv <= 64 and <= 128 links; "
                                           "use NEWT_PRECISION_FP64_EXACT for larger trees");
     const TeamLayout TL = make_team_layout(model->dev, precision == NEWT_PRECISION_MIXED ? 4 : 8);
     const int cache_bytes = model->cache.bytes;
@@ -294,7 +293,9 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
     D->wpb = std::max(wpb, 1);
     D->lds_bytes = cache_bytes + D->wpb * TL.bytes;
     if (precision != NEWT_PRECISION_FP64_EXACT && D->lds_bytes > 65536) {
-        hipError_t e = team_set_lds(precision == NEWT_PRECISION_MIXED, model->dev.nv, D->lds_bytes);
+        // The attribute is per kernel, shared by every newt_data: allow the
+        // whole workgroup LDS so that no data narrows another's launch.
+        hipError_t e = team_set_lds(precision == NEWT_PRECISION_MIXED, model->dev.nv, kMaxLds);
         if (e != hipSuccess) {
             delete D;
             return fail(NEWT_ERR_DEVICE, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));

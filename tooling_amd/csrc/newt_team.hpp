@@ -123,7 +123,7 @@ inline TeamLayout make_team_layout(const ModelT& m, int real_bytes) {
 // Byte offsets of the model tables as cached in LDS (shared by the worlds of
 // a workgroup); the same layout is the device blob they are copied from.
 struct CacheLayout {
-    int links, geoms, pairs, chain, level_off, level_links, level_nrank, dof_link;
+    int links, geoms, pairs, level_off, level_links, level_nrank, dof_link;
     int bytes;
 };
 
@@ -134,7 +134,6 @@ inline CacheLayout make_cache_layout(const ModelT& m) {
     C.links = B((int)sizeof(LinkT) * m.n_links);
     C.geoms = B((int)sizeof(GeomT) * m.n_geoms);
     C.pairs = B((int)sizeof(PairT) * m.n_pairs);
-    C.chain = B(4 * m.n_chain);
     C.level_off = B(4 * (m.n_levels + 1));
     C.level_links = B(4 * m.n_links);
     C.level_nrank = B(4 * m.n_levels);
@@ -212,7 +211,6 @@ struct Tables {
     const LinkT* links;
     const GeomT* geoms;
     const PairT* pairs;
-    const int* chain;
     const int* level_off;
     const int* level_links;
     const int* level_nrank;
@@ -949,10 +947,11 @@ struct Team {
 
     // Translational Jacobian entry of dof s for a world point, along world
     // direction `dir` (jacobian.rs:44-110); zero when dof s is not on the
-    // chain of the link whose ancestor mask is `anc`.
-    __device__ __forceinline__ double jac_entry(int s, unsigned long long anc, V3<double> point, V3<double> dir) const {
+    // chain of the link whose ancestor mask is (anc0, anc1).
+    __device__ __forceinline__ double jac_entry(int s, unsigned long long anc0, unsigned long long anc1, V3<double> point,
+                                                V3<double> dir) const {
         const int j = tb.dof_link[s];
-        if (!((anc >> j) & 1ull)) return 0.0;
+        if (!(((j < 64 ? anc0 : anc1) >> (j & 63)) & 1ull)) return 0.0;
         const LinkT& l = tb.links[j];
         if (l.joint == JF_FREE) {
             const int k = s - l.v_off;
@@ -972,16 +971,17 @@ struct Team {
     // suffices.  One lane per (link, axis); L in sd[ld], 1/L_kk in sd[ldr].
     __device__ __forceinline__ void invweights(int ld, int ldr) const {
         const int n = m.n_links, nv = m.nv;
-        for (int base = 0; base < 3 * n; base += 64) {
+        // Passes of 63 lanes keep each link's three axes in one pass.
+        for (int base = 0; base < 3 * n; base += 63) {
             const int idx = base + lane;
-            const bool on = idx < 3 * n;
+            const bool on = lane < 63 && idx < 3 * n;
             const int i = on ? idx / 3 : 0, ax = idx % 3;
-            const unsigned long long anc = tb.links[i].anc;
+            const unsigned long long anc0 = tb.links[i].anc[0], anc1 = tb.links[i].anc[1];
             const V3<double> point = pos(i) + qrot(ori(i), z3<double>());
             const V3<double> dir = mk3(ax == 0 ? 1.0 : 0.0, ax == 1 ? 1.0 : 0.0, ax == 2 ? 1.0 : 0.0);
             double y[NV];
 #pragma unroll
-            for (int s = 0; s < NV; ++s) y[s] = (s < nv && on) ? jac_entry(s, anc, point, dir) : 0.0;
+            for (int s = 0; s < NV; ++s) y[s] = (s < nv && on) ? jac_entry(s, anc0, anc1, point, dir) : 0.0;
             double sq = 0.0;
 #pragma unroll
             for (int k = 0; k < NV; ++k) {
@@ -1022,16 +1022,17 @@ struct Team {
     // invweights() with the factor in registers.
     __device__ __forceinline__ void invweights_regs(const double (&la)[NV], double lrinv) const {
         const int n = m.n_links, nv = m.nv;
-        for (int base = 0; base < 3 * n; base += 64) {
+        // Passes of 63 lanes keep each link's three axes in one pass.
+        for (int base = 0; base < 3 * n; base += 63) {
             const int idx = base + lane;
-            const bool on = idx < 3 * n;
+            const bool on = lane < 63 && idx < 3 * n;
             const int i = on ? idx / 3 : 0, ax = idx % 3;
-            const unsigned long long anc = tb.links[i].anc;
+            const unsigned long long anc0 = tb.links[i].anc[0], anc1 = tb.links[i].anc[1];
             const V3<double> point = pos(i) + qrot(ori(i), z3<double>());
             const V3<double> dir = mk3(ax == 0 ? 1.0 : 0.0, ax == 1 ? 1.0 : 0.0, ax == 2 ? 1.0 : 0.0);
             double y[NV];
 #pragma unroll
-            for (int s = 0; s < NV; ++s) y[s] = (s < nv && on) ? jac_entry(s, anc, point, dir) : 0.0;
+            for (int s = 0; s < NV; ++s) y[s] = (s < nv && on) ? jac_entry(s, anc0, anc1, point, dir) : 0.0;
             fwd_regs(y, la, lrinv);
             double sq = 0.0;
 #pragma unroll
@@ -1083,11 +1084,11 @@ struct Team {
                     const V3<double> lp = pos(g.link);
                     const Qt<double> lo = ori(g.link);
                     const V3<double> point = lp + qrot(lo, qrot_inv(lo, cp - lp));
-                    const unsigned long long anc = tb.links[g.link].anc;
+                    const unsigned long long anc0 = tb.links[g.link].anc[0], anc1 = tb.links[g.link].anc[1];
                     tr = tr + sd[T.invw + g.link];
 #pragma unroll
                     for (int s = 0; s < NV; ++s)
-                        if (s < nv) y[s] += jac_entry(s, anc, point, dir) * sign;
+                        if (s < nv) y[s] += jac_entry(s, anc0, anc1, point, dir) * sign;
                 }
             }
             double vel = -0.0, fvel = -0.0;
@@ -1375,10 +1376,10 @@ __global__ void __launch_bounds__(64 * team_max_worlds<NV>())
     char* lds = (char*)smem;
     for (int k = threadIdx.x; k < C.bytes / 16; k += blockDim.x) ((int4*)lds)[k] = blob[k];
     __syncthreads();
-    const Tables tb{(const LinkT*)(lds + C.links),      (const GeomT*)(lds + C.geoms),
-                    (const PairT*)(lds + C.pairs),      (const int*)(lds + C.chain),
-                    (const int*)(lds + C.level_off),    (const int*)(lds + C.level_links),
-                    (const int*)(lds + C.level_nrank),  (const int*)(lds + C.dof_link)};
+    const Tables tb{(const LinkT*)(lds + C.links),     (const GeomT*)(lds + C.geoms),
+                    (const PairT*)(lds + C.pairs),     (const int*)(lds + C.level_off),
+                    (const int*)(lds + C.level_links), (const int*)(lds + C.level_nrank),
+                    (const int*)(lds + C.dof_link)};
     // The wavefront index is uniform within a wavefront; readfirstlane lets
     // the compiler treat everything derived from it as scalar.
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63, W = d.W;

@@ -20,7 +20,7 @@ struct LinkT {
     int parent, joint, q_off, v_off;
     int chain_off, chain_len;  // ancestors root..self in ModelT::chain
     int depth, rank;           // tree depth; index among the parent's children
-    unsigned long long anc;    // bit j set when link j is this link or an ancestor (links < 64)
+    unsigned long long anc[2]; // bit j set when link j is this link or an ancestor (links < 128)
     double axis[3];            // as given (joint subspace)
     double pj[3], cj[3];       // joint anchor in parent / child frame
     double pjq[4], cjq[4];     // anchor orientations (fixed joints, root pose)

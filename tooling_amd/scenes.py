@@ -264,4 +264,48 @@ def box_chain(n_boxes=6, dt=0.002, iterations=30) -> Scene:
                  f"{n_boxes} hinged boxes lying on a plane (up to {4 * n_boxes} contacts), PGS {iterations}")
 
 
-SCENES = {"pendulum3": pendulum, "quadruped18": quadruped, "boxchain6": box_chain}
+# ---------------------------------------------------------------------------
+# Workload 4: fixed-root hinge chains of 2..64 links draped on a plane.
+# ---------------------------------------------------------------------------
+CHAIN_LINK = 0.1     # anchor-to-anchor link length
+CHAIN_R = 0.02
+CHAIN_ARMATURE = 0.01  # joint armature: without it the longest chains diverge at dt 1 ms (in newt's step too)
+
+
+def hinge_chain(n, dt=0.001, iterations=30, seed=2026) -> Scene:
+    """Fixed root and n hinge links (axes alternating X and Y), each a
+    capsule of radius 0.02 m spanning its 0.1 m link, masses ~ U[0.5, 2] kg
+    (seeded per chain length), joint armature 0.01 kg m².  With k = max(4, n // 4 + 2) the root hangs
+    max(n - k, 0) link lengths (plus a radius and 5 mm) above the plane and
+    the sampled states bend the chain by a right angle at that link, so the
+    last (up to) k links start just above the plane and settle onto it."""
+    rm = _rng(seed * 131 + n)
+    masses = rm.uniform(0.5, 2.0, size=n)
+    half = CHAIN_LINK / 2
+    bend = max(n - max(4, n // 4 + 2), 0)
+    height = bend * CHAIN_LINK + CHAIN_R + 0.005
+    links = [link(-1, abi.JOINT_FIXED, parent_pos=(0.0, 0.0, height))]
+    geoms = [geom(abi.SHAPE_PLANE, friction=0.8)]
+    for i in range(n):
+        ixx, izz = solid_capsule_inertia(masses[i], CHAIN_R, half - CHAIN_R)
+        axis = (1.0, 0.0, 0.0) if i % 2 == 0 else (0.0, 1.0, 0.0)
+        links.append(link(i, abi.JOINT_HINGE, axis=axis,
+                          parent_pos=(0.0, 0.0, 0.0) if i == 0 else (0.0, 0.0, -half),
+                          child_pos=(0.0, 0.0, half), mass=float(masses[i]), inertia=_diag(ixx, ixx, izz),
+                          armature=CHAIN_ARMATURE))
+        geoms.append(geom(abi.SHAPE_CAPSULE, tree=0, link=i + 1, size=(CHAIN_R, half - CHAIN_R, 0.0), friction=0.8))
+
+    def sampler(nw, seed_):
+        r = _rng(seed_)
+        q = r.normal(0.0, 0.05, size=(nw, n))
+        q[:, bend] += math.pi / 2
+        return q, np.zeros((nw, n))
+
+    opts = dict(dt=dt, gravity=(0.0, 0.0, -9.81), integrator=abi.INTEGRATOR_EULER,
+                solver=abi.SOLVER_PGS, iterations=iterations)
+    return Scene(f"chain{n}", opts, links, geoms, 0, sampler,
+                 f"fixed-root {n}-hinge capsule chain draped on a plane, PGS {iterations}, Euler dt {dt}")
+
+
+SCENES = {"pendulum3": pendulum, "quadruped18": quadruped, "boxchain6": box_chain,
+          "chain40": lambda: hinge_chain(40), "chain64": lambda: hinge_chain(64)}
