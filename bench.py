@@ -261,6 +261,73 @@ def bench_chains(args):
     print(json.dumps(out))
 
 
+def bench_serve(args):
+    """Rollout requests arriving as a Poisson process (`--rate` per second):
+    model uniform over the 3-link pendulum, the quadruped and fixed-root
+    hinge chains of U{2..64} links, worlds ~ U{1..512}, horizon
+    ~ U{100..2000} steps.  Served by continuous batching (one resident pool
+    per model, four streams) in the mixed step, then in the FP64 step; latency
+    is arrival to results on the host."""
+    import numpy as np
+    from tooling_amd import abi, serve
+
+    def requests():
+        return serve.rollout_mix(args.requests, args.seed, rate=args.rate)
+
+    reqs = requests()
+    keys = sorted({r.key for r in reqs})
+    total = sum(r.n_worlds * r.horizon for r in reqs)
+    log(f"bench serve: {len(reqs)} requests over {len(keys)} models at {args.rate}/s, {total} world-steps, "
+        f"chunk {args.serve_chunk}")
+    pools = {k: (serve.scene_for(k), 2048 if k.startswith("chain") else 8192) for k in keys}
+    runs = {}
+    for name, precision in (("mixed", abi.PRECISION_MIXED), ("fp64", abi.PRECISION_FP64)):
+        srv = serve.Server(pools, precision=precision, chunk=args.serve_chunk, n_streams=4)
+        try:
+            # Load every kernel once before the clock starts.
+            srv.run([serve.Request(k, r.qpos[:1], r.qvel[:1], 1) for k in keys
+                     for r in [next(x for x in reqs if x.key == k)]])
+            srv.launches = srv.world_steps = 0
+            rs = requests()
+            last = [time.perf_counter()]
+
+            def progress(left, t):
+                if time.perf_counter() - last[0] > 20.0:
+                    last[0] = time.perf_counter()
+                    log(f"  {name}: {left} requests left at {t:.1f} s")
+
+            elapsed = srv.run(rs, time_limit=600.0, progress=progress)
+            faults = int(sum(int((r.faults != 0).sum()) for r in rs))
+            by_family = {}
+            for r in rs:
+                fam = "chain" if r.key.startswith("chain") else r.key
+                by_family.setdefault(fam, []).append(r.latency * 1e3)
+            runs[name] = {"value": total / elapsed, "elapsed_s": elapsed, "latency_ms": serve.latency_summary(rs),
+                          "latency_p99_ms_by_family": {f: float(np.percentile(v, 99)) for f, v in by_family.items()},
+                          "launches": srv.launches, "device_bytes": srv.device_bytes, "faults": faults}
+        finally:
+            srv.close()
+        log(f"{name}: {runs[name]['value']:.4g} world-steps/s over {runs[name]['elapsed_s']:.2f} s, "
+            f"p50 {runs[name]['latency_ms']['p50_ms']:.1f} ms, p99 {runs[name]['latency_ms']['p99_ms']:.1f} ms")
+    m, f = runs["mixed"], runs["fp64"]
+    out = {
+        "metric": "world-steps/s mixed-prec vs FP64 GPU at qacc err<=1e-5; p99 rollout latency",
+        "value": m["value"], "unit": "world-steps/s", "n_gpus": 1, "steps": None, "warmup": None,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (f32 Delassus rows)", "data": "synthetic",
+        "config": {"workload": f"serve: {len(reqs)} rollout requests, Poisson {args.rate}/s, model ~ U{{pendulum3, "
+                               f"quadruped18, chain2..64}}, worlds ~ U{{1..512}}, horizon ~ U{{100..2000}}, "
+                               f"seed {args.seed}",
+                   "models": len(keys), "world_steps": total, "steps_per_launch": args.serve_chunk,
+                   "streams": 4, "parallelism": "1 GPU, continuous batching: one resident pool per model"},
+        "latency_ms": m["latency_ms"], "latency_p99_ms_by_family": m["latency_p99_ms_by_family"],
+        "launches": m["launches"], "device_bytes": m["device_bytes"], "faults": {"mixed": m["faults"],
+                                                                                 "fp64": f["faults"]},
+        "fp64": f, "speedup_vs_fp64": m["value"] / f["value"],
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,9 +339,13 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["quadruped", "chains"], default="quadruped",
+    ap.add_argument("--workload", choices=["quadruped", "chains", "serve"], default="quadruped",
                     help="quadruped: 8192 quadrupeds (the headline); chains: fixed-root hinge chains of "
-                         "2..64 links, one model per length, batched over streams")
+                         "2..64 links, one model per length, batched over streams; serve: Poisson rollout "
+                         "requests served by continuous batching")
+    ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
+    ap.add_argument("--serve-chunk", type=int, default=50, help="serve: steps per launch")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 5 times (reference kernel for the HBM counters)")
     args = ap.parse_args()
@@ -299,6 +370,9 @@ def main():
         return
     if args.workload == "chains":
         bench_chains(args)
+        return
+    if args.workload == "serve":
+        bench_serve(args)
         return
     from tooling_amd import abi, scenes
     from tooling_amd.newt import Model

@@ -188,14 +188,46 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds,
 
 void newt_data_destroy(newt_data* data);
 
+/* newt_data_get_faults value of a parked world slot (see
+ * newt_data_park_worlds); not a newt_status. */
+#define NEWT_WORLD_PARKED (-1)
+
 /* Host copies of the generalized state: qpos [n_worlds*nq], qvel
- * [n_worlds*nv].  set_state also rebuilds each world's broadphase fat
+ * [n_worlds*nv].  The getters (and set_state) are ordered after the work
+ * last enqueued for this data and wait only for that stream.  set_state also rebuilds each world's broadphase fat
  * bounds from the new state, as World::add_geom and the public state
  * setters do (newt/src/world.rs:974-979, 1119-1136), and clears faults. */
 newt_status newt_data_set_state(newt_data* data, const double* qpos,
                                 const double* qvel);
 newt_status newt_data_get_state(const newt_data* data, double* qpos,
                                 double* qvel);
+
+/* set_state enqueued on `stream` (a hipStream_t; NULL = the default
+ * stream): the host arrays are copied into a pinned staging buffer of the
+ * data before the call returns and are free to reuse; the device copy and
+ * the fault reset run in stream order, so steps enqueued later on `stream`
+ * see the new state.  For serving loops that must not wait on other
+ * streams.  Same semantics as newt_data_set_state (world.rs:1119-1136). */
+newt_status newt_data_set_state_async(newt_data* data, const double* qpos,
+                                      const double* qvel, void* stream);
+
+/* World slots, for serving loops that batch many requests into one
+ * newt_data.  set_worlds_async starts `count` new worlds in slots
+ * [first, first+count) from qpos [count*nq] / qvel [count*nv], stream-ordered
+ * like newt_data_set_state_async: zero applied force, fresh broadphase
+ * proxies (World::new + add_geom + set state, world.rs:326-361,974-979),
+ * faults cleared.  park_worlds marks slots NEWT_WORLD_PARKED: newt_step
+ * leaves them untouched until set_worlds_async reuses them.  get_worlds
+ * copies a slot range's state out after the work enqueued for this data.
+ * No counterpart in newt, which has one World per object. */
+newt_status newt_data_set_worlds_async(newt_data* data, int32_t first, int32_t count,
+                                       const double* qpos, const double* qvel, void* stream);
+newt_status newt_data_park_worlds(newt_data* data, int32_t first, int32_t count, void* stream);
+newt_status newt_data_get_worlds(const newt_data* data, int32_t first, int32_t count,
+                                 double* qpos, double* qvel);
+
+/* Device memory allocated by this data, in bytes (admission control). */
+newt_status newt_data_device_bytes(const newt_data* data, uint64_t* bytes);
 
 /* Generalized acceleration of the last step's integration: the qddot that
  * euler_step_with_workspace adds to qdot (newt/src/tree.rs:1407-1417).
@@ -229,7 +261,8 @@ newt_status newt_data_get_pairs(const newt_data* data, int32_t* counts,
 newt_status newt_step(const newt_model* model, newt_data* data,
                       int32_t n_steps, void* stream);
 
-/* Blocks until every step enqueued on `data` has finished. */
+/* Blocks until every step enqueued on `data` has finished (waits on the
+ * stream of its last step). */
 newt_status newt_data_synchronize(newt_data* data);
 
 #ifdef __cplusplus

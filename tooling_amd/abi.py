@@ -21,6 +21,7 @@ INTEGRATOR_EULER, INTEGRATOR_IMPLICITFAST, INTEGRATOR_RK4 = range(3)
 SOLVER_PENALTY, SOLVER_PGS, SOLVER_NEWTON = range(3)
 CONE_PYRAMIDAL, CONE_ELLIPTIC = range(2)
 PRECISION_FP64, PRECISION_MIXED, PRECISION_FP64_EXACT = range(3)
+WORLD_PARKED = -1
 
 D3 = C.c_double * 3
 D4 = C.c_double * 4
@@ -96,6 +97,11 @@ SIGNATURES = {
     "newt_data_destroy": (None, [_P]),
     "newt_data_set_state": (_I, [_P, _PD, _PD]),
     "newt_data_get_state": (_I, [_P, _PD, _PD]),
+    "newt_data_set_state_async": (_I, [_P, _PD, _PD, _P]),
+    "newt_data_set_worlds_async": (_I, [_P, _I, _I, _PD, _PD, _P]),
+    "newt_data_park_worlds": (_I, [_P, _I, _I, _P]),
+    "newt_data_get_worlds": (_I, [_P, _I, _I, _PD, _PD]),
+    "newt_data_device_bytes": (_I, [_P, C.POINTER(C.c_uint64)]),
     "newt_data_get_qacc": (_I, [_P, _PD]),
     "newt_data_get_faults": (_I, [_P, _PI]),
     "newt_data_contact_capacity": (_I, [_P, _PI]),
@@ -117,6 +123,11 @@ def load_library(path: str = LIB_PATH):
     if not os.path.exists(path):
         raise RuntimeError(
             f"{path} is missing: build the HIP extension first (python -m tooling_amd.build)")
+    # torch ships its own HIP runtime under the soname of the system one
+    # (libamdhip64.so.7).  Loaded after this library, torch would map a second
+    # runtime and report no device; loaded first, its runtime serves both, so
+    # torch streams and events are valid handles for this library.
+    import torch  # noqa: F401
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

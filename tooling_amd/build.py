@@ -29,19 +29,25 @@ SOURCES = {"newt_gpu.hip": ["-ffp-contract=off"], "newt_team.hip": ["-ffp-contra
            "newt_team64f.hip": ["-ffp-contract=fast"], "newt_team64d.hip": ["-ffp-contract=fast"]}
 
 
-def build(verbose: bool = False) -> str:
-    """Compile each translation unit (in parallel) and link the library."""
+def build(verbose: bool = False, only=None) -> str:
+    """Compile each translation unit (in parallel) and link the library.
+    ``only``: names of the units to recompile; the others' objects are reused."""
     os.makedirs(OUT_DIR, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
     jobs = []
     for src, extra in SOURCES.items():
         obj = os.path.join(OUT_DIR, src.replace(".hip", ".o"))
+        if only and src not in only and os.path.exists(obj):
+            jobs.append((src, obj, None))
+            continue
         cmd = [hipcc, *compile_flags, *extra, "-c", "-o", obj, os.path.join(CSRC, src)]
         jobs.append((src, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
     out = []
     failed = False
     for src, obj, p in jobs:
+        if p is None:
+            continue
         so, se = p.communicate()
         out.append(so + se)
         if p.returncode != 0:
@@ -65,4 +71,4 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, only=sys.argv[1:] or None))

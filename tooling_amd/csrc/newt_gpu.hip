@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -75,6 +76,10 @@ struct newt_data {
     double* dws = nullptr;   // factorization workspace
     int* iws = nullptr;      // integer workspace
     char* ovf = nullptr;     // per-world blocks for contact sets over 64 rows (persistent step)
+    uint64_t device_bytes = 0;     // device memory this data allocated
+    std::vector<double> fat_default;  // fat bounds of one world at the default state
+    double* staging = nullptr;     // pinned host copy of the last asynchronous state upload
+    hipEvent_t staged = nullptr;   // completes when that upload has left the staging buffer
     hipStream_t last_stream = nullptr;
 };
 
@@ -151,6 +156,27 @@ void to_lanes(const double* src, std::vector<double>& dst, int W, int n) {
 void from_lanes(const std::vector<double>& src, double* dst, int W, int n) {
     for (int w = 0; w < W; ++w)
         for (int k = 0; k < n; ++k) dst[(size_t)w * n + k] = src[(size_t)k * W + w];
+}
+
+// Device-to-host copy ordered after the work enqueued on this data's stream
+// (other streams keep running).
+hipError_t d2h(const newt_data* D, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, D->last_stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(D->last_stream);
+}
+
+// Pinned staging for stream-ordered uploads, sized for every world's qpos,
+// qvel and fat bounds; waits until the previous upload has left it.
+hipError_t stage(newt_data* D) {
+    if (D->staging) return hipEventSynchronize(D->staged);
+    const ModelT& m = D->model->dev;
+    hipError_t e = hipHostMalloc((void**)&D->staging,
+                                 std::max<size_t>((size_t)D->W * (m.nq + m.nv + 6 * m.n_geoms) * 8, 8),
+                                 hipHostMallocDefault);
+    if (e != hipSuccess) return e;
+    return hipEventCreateWithFlags(&D->staged, hipEventDisableTiming);
 }
 
 }  // namespace
@@ -304,7 +330,10 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
     const ModelT& m = model->dev;
     size_t W = (size_t)n_worlds;
     size_t rsz = precision == NEWT_PRECISION_MIXED ? sizeof(float) : sizeof(double);
-    auto alloc = [&](void** p, size_t bytes) -> bool { return hipMalloc(p, std::max<size_t>(bytes, 8)) == hipSuccess; };
+    auto alloc = [&](void** p, size_t bytes) -> bool {
+        D->device_bytes += std::max<size_t>(bytes, 8);
+        return hipMalloc(p, std::max<size_t>(bytes, 8)) == hipSuccess;
+    };
     bool ok = alloc((void**)&D->d.qpos, W * m.nq * 8) && alloc((void**)&D->d.qvel, W * m.nv * 8) &&
               alloc((void**)&D->d.qacc, W * m.nv * 8) && alloc((void**)&D->d.qfrc, W * m.nv * 8) &&
               alloc((void**)&D->d.fat, W * m.n_geoms * 6 * 8) && alloc((void**)&D->d.fault, W * 4) &&
@@ -333,6 +362,9 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
     s = launch(D, 0, nullptr, 1);
     if (s != NEWT_OK) return s;
     HIP_TRY(hipDeviceSynchronize());
+    D->fat_default.resize((size_t)6 * m.n_geoms);
+    if (m.n_geoms)
+        HIP_TRY(hipMemcpy2D(D->fat_default.data(), 8, D->d.fat, W * 8, 8, 6 * m.n_geoms, hipMemcpyDeviceToHost));
     return NEWT_OK;
 }
 
@@ -350,6 +382,11 @@ void newt_data_destroy(newt_data* D) {
     (void)hipFree(D->dws);
     (void)hipFree(D->iws);
     (void)hipFree(D->ovf);
+    if (D->staged) {
+        (void)hipEventSynchronize(D->staged);
+        (void)hipEventDestroy(D->staged);
+    }
+    (void)hipHostFree(D->staging);
     delete D;
 }
 
@@ -359,24 +396,118 @@ newt_status newt_data_set_state(newt_data* D, const double* qpos, const double* 
     std::vector<double> a, b;
     to_lanes(qpos, a, D->W, m.nq);
     to_lanes(qvel, b, D->W, m.nv);
-    HIP_TRY(hipDeviceSynchronize());
-    if (m.nq) HIP_TRY(hipMemcpy(D->d.qpos, a.data(), a.size() * 8, hipMemcpyHostToDevice));
-    if (m.nv) HIP_TRY(hipMemcpy(D->d.qvel, b.data(), b.size() * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(D->d.fault, 0, (size_t)D->W * 4));
+    if (m.nq) HIP_TRY(hipMemcpyAsync(D->d.qpos, a.data(), a.size() * 8, hipMemcpyHostToDevice, D->last_stream));
+    if (m.nv) HIP_TRY(hipMemcpyAsync(D->d.qvel, b.data(), b.size() * 8, hipMemcpyHostToDevice, D->last_stream));
+    HIP_TRY(hipMemsetAsync(D->d.fault, 0, (size_t)D->W * 4, D->last_stream));
+    HIP_TRY(hipStreamSynchronize(D->last_stream));
+    return NEWT_OK;
+}
+
+newt_status newt_data_set_state_async(newt_data* D, const double* qpos, const double* qvel, void* stream) {
+    if (!D || !qpos || !qvel) return fail(NEWT_ERR_INVALID_ARG, "null argument");
+    const ModelT& m = D->model->dev;
+    const size_t nq = (size_t)D->W * m.nq, nv = (size_t)D->W * m.nv;
+    HIP_TRY(stage(D));
+    for (int w = 0; w < D->W; ++w) {
+        for (int k = 0; k < m.nq; ++k) D->staging[(size_t)k * D->W + w] = qpos[(size_t)w * m.nq + k];
+        for (int k = 0; k < m.nv; ++k) D->staging[nq + (size_t)k * D->W + w] = qvel[(size_t)w * m.nv + k];
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (nq) HIP_TRY(hipMemcpyAsync(D->d.qpos, D->staging, nq * 8, hipMemcpyHostToDevice, s));
+    if (nv) HIP_TRY(hipMemcpyAsync(D->d.qvel, D->staging + nq, nv * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(D->d.fault, 0, (size_t)D->W * 4, s));
+    HIP_TRY(hipEventRecord(D->staged, s));
+    D->last_stream = s;
+    return NEWT_OK;
+}
+
+newt_status newt_data_set_worlds_async(newt_data* D, int32_t first, int32_t count, const double* qpos,
+                                       const double* qvel, void* stream) {
+    if (!D || !qpos || !qvel) return fail(NEWT_ERR_INVALID_ARG, "null argument");
+    if (first < 0 || count < 1 || first + count > D->W) return fail(NEWT_ERR_INVALID_ARG, "world range out of bounds");
+    const ModelT& m = D->model->dev;
+    const size_t W = (size_t)D->W, c = (size_t)count;
+    const size_t nq = c * m.nq, nv = c * m.nv, nf = c * 6 * m.n_geoms;
+    HIP_TRY(stage(D));
+    double* sq = D->staging;
+    double* sv = sq + nq;
+    double* sf = sv + nv;
+    for (size_t w = 0; w < c; ++w) {
+        for (int k = 0; k < m.nq; ++k) sq[(size_t)k * c + w] = qpos[w * m.nq + k];
+        for (int k = 0; k < m.nv; ++k) sv[(size_t)k * c + w] = qvel[w * m.nv + k];
+    }
+    // A new world's proxies are the ones World::add_geom inserts at the
+    // default state (world.rs:974-979); the next step refits them to qpos.
+    for (int k = 0; k < 6 * m.n_geoms; ++k)
+        for (size_t w = 0; w < c; ++w) sf[(size_t)k * c + w] = D->fat_default[k];
+    hipStream_t s = (hipStream_t)stream;
+    if (m.nq) HIP_TRY(hipMemcpy2DAsync(D->d.qpos + first, W * 8, sq, c * 8, c * 8, m.nq, hipMemcpyHostToDevice, s));
+    if (m.nv) HIP_TRY(hipMemcpy2DAsync(D->d.qvel + first, W * 8, sv, c * 8, c * 8, m.nv, hipMemcpyHostToDevice, s));
+    if (nf)
+        HIP_TRY(hipMemcpy2DAsync(D->d.fat + first, W * 8, sf, c * 8, c * 8, 6 * m.n_geoms, hipMemcpyHostToDevice, s));
+    if (m.nv) {
+        HIP_TRY(hipMemset2DAsync(D->d.qfrc + first, W * 8, 0, c * 8, m.nv, s));
+        HIP_TRY(hipMemset2DAsync(D->d.qacc + first, W * 8, 0, c * 8, m.nv, s));
+    }
+    HIP_TRY(hipMemsetAsync(D->d.fault + first, 0, c * 4, s));
+    HIP_TRY(hipMemsetAsync(D->d.ncontact + first, 0, c * 4, s));
+    HIP_TRY(hipMemsetAsync(D->d.npair + first, 0, c * 4, s));
+    HIP_TRY(hipEventRecord(D->staged, s));
+    D->last_stream = s;
+    return NEWT_OK;
+}
+
+newt_status newt_data_park_worlds(newt_data* D, int32_t first, int32_t count, void* stream) {
+    if (!D) return fail(NEWT_ERR_INVALID_ARG, "null data");
+    if (first < 0 || count < 1 || first + count > D->W) return fail(NEWT_ERR_INVALID_ARG, "world range out of bounds");
+    std::vector<int32_t> parked((size_t)count, NEWT_WORLD_PARKED);
+    HIP_TRY(stage(D));
+    std::memcpy(D->staging, parked.data(), parked.size() * 4);
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipMemcpyAsync(D->d.fault + first, D->staging, (size_t)count * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipEventRecord(D->staged, s));
+    D->last_stream = s;
+    return NEWT_OK;
+}
+
+newt_status newt_data_get_worlds(const newt_data* D, int32_t first, int32_t count, double* qpos, double* qvel) {
+    if (!D) return fail(NEWT_ERR_INVALID_ARG, "null data");
+    if (first < 0 || count < 1 || first + count > D->W) return fail(NEWT_ERR_INVALID_ARG, "world range out of bounds");
+    const ModelT& m = D->model->dev;
+    const size_t W = (size_t)D->W, c = (size_t)count;
+    std::vector<double> a(c * m.nq), b(c * m.nv);
+    if (qpos && m.nq)
+        HIP_TRY(hipMemcpy2DAsync(a.data(), c * 8, D->d.qpos + first, W * 8, c * 8, m.nq, hipMemcpyDeviceToHost,
+                                 D->last_stream));
+    if (qvel && m.nv)
+        HIP_TRY(hipMemcpy2DAsync(b.data(), c * 8, D->d.qvel + first, W * 8, c * 8, m.nv, hipMemcpyDeviceToHost,
+                                 D->last_stream));
+    HIP_TRY(hipStreamSynchronize(D->last_stream));
+    for (size_t w = 0; w < c; ++w) {
+        if (qpos)
+            for (int k = 0; k < m.nq; ++k) qpos[w * m.nq + k] = a[(size_t)k * c + w];
+        if (qvel)
+            for (int k = 0; k < m.nv; ++k) qvel[w * m.nv + k] = b[(size_t)k * c + w];
+    }
+    return NEWT_OK;
+}
+
+newt_status newt_data_device_bytes(const newt_data* D, uint64_t* bytes) {
+    if (!D || !bytes) return fail(NEWT_ERR_INVALID_ARG, "null argument");
+    *bytes = D->device_bytes;
     return NEWT_OK;
 }
 
 newt_status newt_data_get_state(const newt_data* D, double* qpos, double* qvel) {
     if (!D) return fail(NEWT_ERR_INVALID_ARG, "null data");
     const ModelT& m = D->model->dev;
-    HIP_TRY(hipDeviceSynchronize());
     std::vector<double> a((size_t)D->W * m.nq), b((size_t)D->W * m.nv);
     if (qpos && m.nq) {
-        HIP_TRY(hipMemcpy(a.data(), D->d.qpos, a.size() * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(d2h(D, a.data(), D->d.qpos, a.size() * 8));
         from_lanes(a, qpos, D->W, m.nq);
     }
     if (qvel && m.nv) {
-        HIP_TRY(hipMemcpy(b.data(), D->d.qvel, b.size() * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(d2h(D, b.data(), D->d.qvel, b.size() * 8));
         from_lanes(b, qvel, D->W, m.nv);
     }
     return NEWT_OK;
@@ -385,10 +516,9 @@ newt_status newt_data_get_state(const newt_data* D, double* qpos, double* qvel) 
 newt_status newt_data_get_qacc(const newt_data* D, double* qacc) {
     if (!D || !qacc) return fail(NEWT_ERR_INVALID_ARG, "null argument");
     const ModelT& m = D->model->dev;
-    HIP_TRY(hipDeviceSynchronize());
     std::vector<double> a((size_t)D->W * m.nv);
     if (m.nv) {
-        HIP_TRY(hipMemcpy(a.data(), D->d.qacc, a.size() * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(d2h(D, a.data(), D->d.qacc, a.size() * 8));
         from_lanes(a, qacc, D->W, m.nv);
     }
     return NEWT_OK;
@@ -396,8 +526,7 @@ newt_status newt_data_get_qacc(const newt_data* D, double* qacc) {
 
 newt_status newt_data_get_faults(const newt_data* D, int32_t* faults) {
     if (!D || !faults) return fail(NEWT_ERR_INVALID_ARG, "null argument");
-    HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipMemcpy(faults, D->d.fault, (size_t)D->W * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(d2h(D, faults, D->d.fault, (size_t)D->W * 4));
     return NEWT_OK;
 }
 
@@ -413,12 +542,11 @@ newt_status newt_data_get_contacts(const newt_data* D, int32_t* counts, int32_t*
     const Layout& L = D->L;
     size_t W = (size_t)D->W;
     int cap = m.cap_c;
-    HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipMemcpy(counts, D->d.ncontact, W * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(d2h(D, counts, D->d.ncontact, W * 4));
     std::vector<int> ih(W * 3 * cap);
-    HIP_TRY(hipMemcpy(ih.data(), D->iws + (size_t)L.cids * W, ih.size() * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(d2h(D, ih.data(), D->iws + (size_t)L.cids * W, ih.size() * 4));
     std::vector<double> gh(W * 9 * cap);
-    HIP_TRY(hipMemcpy(gh.data(), D->dws + (size_t)L.cgeo * W, gh.size() * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(d2h(D, gh.data(), D->dws + (size_t)L.cgeo * W, gh.size() * 8));
     for (size_t w = 0; w < W; ++w)
         for (int k = 0; k < cap; ++k) {
             for (int j = 0; j < 3; ++j) ids[(w * cap + k) * 3 + j] = ih[(size_t)(3 * k + j) * W + w];
@@ -438,10 +566,9 @@ newt_status newt_data_get_pairs(const newt_data* D, int32_t* counts, int32_t* pa
     const newt_model* M = D->model;
     size_t W = (size_t)D->W;
     int cap = std::max(M->dev.n_pairs, 1);
-    HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipMemcpy(counts, D->d.npair, W * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(d2h(D, counts, D->d.npair, W * 4));
     std::vector<int> ih(W * cap);
-    HIP_TRY(hipMemcpy(ih.data(), D->iws + (size_t)D->L.pairs * W, ih.size() * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(d2h(D, ih.data(), D->iws + (size_t)D->L.pairs * W, ih.size() * 4));
     for (size_t w = 0; w < W; ++w)
         for (int k = 0; k < cap; ++k) {
             int idx = k < counts[w] ? ih[(size_t)k * W + w] : 0;
@@ -460,7 +587,7 @@ newt_status newt_step(const newt_model* model, newt_data* data, int32_t n_steps,
 
 newt_status newt_data_synchronize(newt_data* data) {
     if (!data) return fail(NEWT_ERR_INVALID_ARG, "null data");
-    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipStreamSynchronize(data->last_stream));
     return NEWT_OK;
 }
 

@@ -65,6 +65,41 @@ class Data:
         qvel = np.ascontiguousarray(qvel, dtype=np.float64).reshape(self.n_worlds, self.model.nv)
         abi.check(self.lib.newt_data_set_state(self.handle, self._dp(qpos), self._dp(qvel)))
 
+    def set_state_async(self, qpos, qvel, stream=None):
+        """set_state ordered on `stream` (a hipStream_t handle or None); the
+        arrays are staged before the call returns."""
+        qpos = np.ascontiguousarray(qpos, dtype=np.float64).reshape(self.n_worlds, self.model.nq)
+        qvel = np.ascontiguousarray(qvel, dtype=np.float64).reshape(self.n_worlds, self.model.nv)
+        abi.check(self.lib.newt_data_set_state_async(self.handle, self._dp(qpos), self._dp(qvel),
+                                                     C.c_void_p(stream) if stream else None))
+
+    def set_worlds_async(self, first, qpos, qvel, stream=None):
+        """Start len(qpos) new worlds in slots [first, first+len(qpos)),
+        ordered on `stream`; the arrays are staged before the call returns."""
+        qpos = np.ascontiguousarray(qpos, dtype=np.float64).reshape(-1, self.model.nq)
+        qvel = np.ascontiguousarray(qvel, dtype=np.float64).reshape(-1, self.model.nv)
+        if qpos.shape[0] != qvel.shape[0]:
+            raise ValueError("qpos and qvel hold different world counts")
+        abi.check(self.lib.newt_data_set_worlds_async(self.handle, int(first), qpos.shape[0], self._dp(qpos),
+                                                      self._dp(qvel), C.c_void_p(stream) if stream else None))
+
+    def park_worlds(self, first, count, stream=None):
+        """Stop stepping slots [first, first+count) until they are reused."""
+        abi.check(self.lib.newt_data_park_worlds(self.handle, int(first), int(count),
+                                                 C.c_void_p(stream) if stream else None))
+
+    def worlds(self, first, count):
+        qpos = np.zeros((count, self.model.nq))
+        qvel = np.zeros((count, self.model.nv))
+        abi.check(self.lib.newt_data_get_worlds(self.handle, int(first), int(count), self._dp(qpos),
+                                                self._dp(qvel)))
+        return qpos, qvel
+
+    def device_bytes(self):
+        out = C.c_uint64()
+        abi.check(self.lib.newt_data_device_bytes(self.handle, C.byref(out)))
+        return out.value
+
     def state(self):
         qpos = np.zeros((self.n_worlds, self.model.nq))
         qvel = np.zeros((self.n_worlds, self.model.nv))
