@@ -1059,7 +1059,8 @@ template <class T> void tree_deriv(const Tree<T>& t, const std::vector<T>& qdd, 
 }
 
 // rk4_step_with_workspace (tree.rs:1234-1364), zero external wrenches, no actuators.
-template <class T> void rk4_step(Tree<T>& t, V3<T> gravity, T dt, AbaWs<T>& ws) {
+// k1_out (test instrumentation, not in newt): the first stage's qddot.
+template <class T> void rk4_step(Tree<T>& t, V3<T> gravity, T dt, AbaWs<T>& ws, std::vector<T>* k1_out = nullptr) {
     std::vector<T> q0 = t.q, qd0 = t.qdot;
     std::vector<std::pair<V3<T>, V3<T>>> ext(t.links.size(), {vzero<T>(), vzero<T>()});
     std::vector<Pose<T>> poses;
@@ -1068,6 +1069,7 @@ template <class T> void rk4_step(Tree<T>& t, V3<T> gravity, T dt, AbaWs<T>& ws) 
     for (int s = 0; s < 4; ++s) {
         forward_kinematics(t, poses);
         aba_tree(t, poses, gravity, ext, VI_EXPLICIT, dt, ws, k);
+        if (s == 0 && k1_out) *k1_out = k;
         tree_deriv(t, k, dq[s], dv[s]);
         if (s < 3) {  // advance_tree_state (tree.rs:1551-1573)
             T h = stage_dt[s];

@@ -798,7 +798,8 @@ template <class T> struct World {
             }
             std::vector<T> qacc;
             if (integrator == I_RK4) {
-                rk4_step(tr, gravity, dt, ws[t]);
+                rk4_step(tr, gravity, dt, ws[t], &qacc);
+                last_qacc.insert(last_qacc.end(), qacc.begin(), qacc.end());
             } else {
                 euler_step(tr, gravity, dt, integrator == I_IMPLICITFAST, ws[t], qacc);
                 last_qacc.insert(last_qacc.end(), qacc.begin(), qacc.end());

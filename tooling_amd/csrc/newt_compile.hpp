@@ -86,8 +86,11 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     if (o.iterations < 1) return cfail(err, NEWT_ERR_INVALID_CONFIG, "solver iterations must be >= 1");
     if (o.solver_mode == NEWT_SOLVER_NEWTON && o.cone == NEWT_CONE_ELLIPTIC)
         return cfail(err, NEWT_ERR_INVALID_CONFIG, "solver=newton with cone=elliptic is not supported yet; use cone=pyramidal");
-    if (o.integrator != NEWT_INTEGRATOR_EULER)
-        return cfail(err, NEWT_ERR_UNSUPPORTED, "the GPU step integrates with semi-implicit Euler only");
+    // ImplicitFast differs from Euler only through actuator velocity terms;
+    // models here have no actuators, so both run the Euler step.
+    if (o.integrator != NEWT_INTEGRATOR_EULER && o.integrator != NEWT_INTEGRATOR_IMPLICITFAST &&
+        o.integrator != NEWT_INTEGRATOR_RK4)
+        return cfail(err, NEWT_ERR_INVALID_ARG, "unknown integrator");
     if (o.solver_mode == NEWT_SOLVER_NEWTON) return cfail(err, NEWT_ERR_UNSUPPORTED, "Newton solver is not built for trees");
     if (o.cone != NEWT_CONE_PYRAMIDAL) return cfail(err, NEWT_ERR_UNSUPPORTED, "elliptic cone is not built");
     const auto& T = M->trees[0];
@@ -179,8 +182,10 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
         for (int i = 0; i < n; ++i) {
             const LinkT& l = M->links[i];
             M->level_nrank[l.depth] = std::max(M->level_nrank[l.depth], l.rank + 1);
-            if ((l.joint == NEWT_JOINT_FREE && l.free_damping != 0.0) ||
-                ((l.joint == NEWT_JOINT_HINGE || l.joint == NEWT_JOINT_SLIDE) && l.damping != 0.0))
+            // RK4 solves contacts with M itself (no implicit damping term).
+            if (o.integrator != NEWT_INTEGRATOR_RK4 &&
+                ((l.joint == NEWT_JOINT_FREE && l.free_damping != 0.0) ||
+                 ((l.joint == NEWT_JOINT_HINGE || l.joint == NEWT_JOINT_SLIDE) && l.damping != 0.0)))
                 M->dev.damped = 1;
         }
     }
@@ -299,6 +304,13 @@ inline Layout make_layout(const ModelT& m, int real_bytes) {
     L.Ld = D(nv * nv);
     L.dvec = D(nv);
     L.dres = D(nv);
+    // RK4 (tree.rs:1234-1364): start state, stage derivative, weighted sums.
+    const bool rk4 = m.integrator == NEWT_INTEGRATOR_RK4;
+    L.rq0 = rk4 ? D(m.nq) : 0;
+    L.rv0 = rk4 ? D(nv) : 0;
+    L.rdq = rk4 ? D(m.nq) : 0;
+    L.raq = rk4 ? D(m.nq) : 0;
+    L.rav = rk4 ? D(nv) : 0;
     L.total_d = dd;
     L.cids = I(3 * m.cap_c);
     L.rinfo = I(3 * m.cap_r);

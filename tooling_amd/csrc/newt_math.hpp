@@ -120,6 +120,11 @@ template <class R> NEWT_HD V3<R> qrot(Qt<R> q, V3<R> v) {
     return v + t * q.w + cross3(qv, t);
 }
 template <class R> NEWT_HD V3<R> qrot_inv(Qt<R> q, V3<R> v) { return qrot(qconj(q), v); }
+// Quat::derivative (math.rs:640-643): q ⊗ (ω, 0) / 2.
+template <class R> NEWT_HD Qt<R> qderiv(Qt<R> q, V3<R> w) {
+    const Qt<R> p = qmul(q, Qt<R>{w.x, w.y, w.z, R(0.0)});
+    return {p.x * R(0.5), p.y * R(0.5), p.z * R(0.5), p.w * R(0.5)};
+}
 template <class R> NEWT_HD Qt<R> qintegrate(Qt<R> q, V3<R> w, R dt) {
     return qnormalize(qmul(q, qaxis(w, dt * len3(w))));
 }

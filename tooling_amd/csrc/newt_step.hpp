@@ -838,10 +838,83 @@ template <class R> struct World {
 
     // post: implicit-damping ABA with the contact force in dw[L.delta],
     // semi-implicit Euler, and the broadphase refresh on the new state.
+    // tree_deriv (tree.rs:1485-1540) at the current state into dw[L.rdq]
+    // (position rates); the velocity rates are the accelerations.
+    NEWT_HD void deriv() const {
+        for (int i = 0; i < m.n_links; ++i) {
+            const LinkT& l = m.links[i];
+            const int qo = l.q_off, vo = l.v_off;
+            if (l.joint == JF_FREE) {
+                Qt<double> o{qpos[qo + 3], qpos[qo + 4], qpos[qo + 5], qpos[qo + 6]};
+                V3<double> vw = qrot(o, mk3(qvel[vo + 3], qvel[vo + 4], qvel[vo + 5]));
+                Qt<double> dq = qderiv(o, mk3(qvel[vo], qvel[vo + 1], qvel[vo + 2]));
+                dw[L.rdq + qo] = vw.x;
+                dw[L.rdq + qo + 1] = vw.y;
+                dw[L.rdq + qo + 2] = vw.z;
+                dw[L.rdq + qo + 3] = dq.x;
+                dw[L.rdq + qo + 4] = dq.y;
+                dw[L.rdq + qo + 5] = dq.z;
+                dw[L.rdq + qo + 6] = dq.w;
+            } else if (l.joint == JF_HINGE || l.joint == JF_SLIDE) {
+                dw[L.rdq + qo] = qvel[vo];
+            }
+        }
+    }
+
+    // The tree part of step_trees (world.rs:1826-1946) with
+    // rk4_step_with_workspace (tree.rs:1234-1364): the contact force held
+    // constant over four explicit ABA stages; the first stage's
+    // acceleration is reported as qacc.
+    NEWT_HD void rk4(bool constraint) const {
+        const int nv = m.nv, nq = m.nq;
+        const double dt = m.dt;
+        for (int s = 0; s < nv; ++s) dw[L.tau + s] = qfrc[s] + (0.0 + dw[L.delta + s]);
+        for (int k = 0; k < nq; ++k) dw[L.rq0 + k] = qpos[k];
+        for (int s = 0; s < nv; ++s) dw[L.rv0 + s] = qvel[s];
+        const double h[3] = {dt * 0.5, dt * 0.5, dt};
+        for (int st = 0; st < 4; ++st) {
+            forward_kinematics();
+            aba(constraint, false, dt, L.delta);
+            if (st == 0)
+                for (int s = 0; s < nv; ++s) qacc[s] = dw[L.delta + s];
+            deriv();
+            // ((k1 + 2 k2) + 2 k3) + k4, accumulated in stage order.
+            for (int k = 0; k < nq; ++k) {
+                const double d = dw[L.rdq + k];
+                dw[L.raq + k] = st == 0 ? d : (st == 3 ? dw[L.raq + k] + d : dw[L.raq + k] + 2.0 * d);
+            }
+            for (int s = 0; s < nv; ++s) {
+                const double d = dw[L.delta + s];
+                dw[L.rav + s] = st == 0 ? d : (st == 3 ? dw[L.rav + s] + d : dw[L.rav + s] + 2.0 * d);
+            }
+            if (st < 3) {  // advance_tree_state (tree.rs:1551-1573)
+                for (int k = 0; k < nq; ++k) qpos[k] = dw[L.rq0 + k] + dw[L.rdq + k] * h[st];
+                for (int s = 0; s < nv; ++s) qvel[s] = dw[L.rv0 + s] + dw[L.delta + s] * h[st];
+            }
+        }
+        const double w6 = dt * (1.0 / 6.0);
+        for (int k = 0; k < nq; ++k) qpos[k] = dw[L.rq0 + k] + dw[L.raq + k] * w6;
+        for (int s = 0; s < nv; ++s) qvel[s] = dw[L.rv0 + s] + dw[L.rav + s] * w6;
+        if (m.links[0].joint == JF_FREE) {
+            Qt<double> o = qnormalize(Qt<double>{qpos[3], qpos[4], qpos[5], qpos[6]});
+            qpos[3] = o.x;
+            qpos[4] = o.y;
+            qpos[5] = o.z;
+            qpos[6] = o.w;
+        }
+    }
+
     NEWT_HD void post() const {
         const R dt = (R)m.dt;
         const int nv = m.nv;
         bool constraint = m.mode == 1;
+        if (m.integrator == 2) {
+            rk4(constraint);
+            forward_kinematics();
+            link_velocities();
+            populate(dt, false);
+            return;
+        }
         // step_trees_euler (world.rs:1952-2051): applied + (0 + delta).
         for (int s = 0; s < nv; ++s) dw[L.tau + s] = qfrc[s] + (0.0 + dw[L.delta + s]);
         aba(constraint, true, m.dt, L.delta);

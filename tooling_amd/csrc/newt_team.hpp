@@ -54,6 +54,7 @@ struct TeamLayout {
     int j_off;                    // byte offset of the rows: 64 x nv of R (union, solve side)
     int i_off;                    // byte offset of the ints
     int cids, ccnt, coff, plist, alist, misc;
+    int q0, qd0, aq, av;          // RK4: start state and weighted stage sums
     int bytes;                    // per world, multiple of 16
     // Global block per world holding the rows of contact sets of more than
     // 64 rows (cap_r x nv of R; stride 0 when the model cannot produce them).
@@ -84,6 +85,11 @@ inline TeamLayout make_team_layout(const ModelT& m, int real_bytes) {
     T.fat = D(6 * m.n_geoms);
     T.cgeo = D(9 * m.cap_c);
     T.invw = D(n);
+    const bool rk4 = m.integrator == 2;
+    T.q0 = rk4 ? D(nq) : 0;
+    T.qd0 = rk4 ? D(nv) : 0;
+    T.aq = rk4 ? D(nq) : 0;
+    T.av = rk4 ? D(nv) : 0;
     d = (d + 1) / 2 * 2;  // 16-byte aligned union
     const int u0 = d;
     T.ia = D(36 * n);
@@ -694,7 +700,7 @@ struct Team {
     // ABA (aba.rs:803-1102) with generalized force sd[tau] and implicit
     // joint damping (Euler); the free root reads tau under the constraint
     // solver (tree.rs:1021-1116).  qddot into sd[out].  Needs store_xups().
-    __device__ __forceinline__ void aba(int out) const {
+    __device__ __forceinline__ void aba(int out, bool implicit) const {
         const double dt = m.dt;
         V3<double> g = v3of(m.gravity);
         // velocities, bias forces, rigid inertias: root -> leaves
@@ -746,7 +752,7 @@ struct Team {
                     } else {
                         const SM<double> s = subspace(i);
                         const SF<double> ias = m6mul(a, s);
-                        const double d = (sdot(s, ias) + l.armature) + dt * (l.damping + 0.0);
+                        const double d = (sdot(s, ias) + l.armature) + (implicit ? dt * (l.damping + 0.0) : 0.0);
                         const SF<double> ps = pa + m6mul(a, c);
                         const double fs = (sd[T.tau + l.v_off] + 0.0) - l.damping * sd[T.qd + l.v_off];
                         const double qdd = (fs - sdot(s, ps)) / d;
@@ -786,7 +792,7 @@ struct Team {
                     const int q0 = T.qd + o;
                     SM<double> qr{mk3(sd[q0], sd[q0 + 1], sd[q0 + 2]), mk3(sd[q0 + 3], sd[q0 + 4], sd[q0 + 5])};
                     rhs = rhs - SF<double>{qr.w * fd, qr.v * fd};
-                    for (int k = 0; k < 6; ++k) A[k][k] += dt * fd;
+                    for (int k = 0; k < 6; ++k) A[k][k] += implicit ? dt * fd : 0.0;
                 }
                 double rv[6] = {rhs.t.x, rhs.t.y, rhs.t.z, rhs.f.x, rhs.f.y, rhs.f.z};
                 for (int r = 0; r < 6; ++r) A[r][6] = rv[r];
@@ -1312,6 +1318,86 @@ struct Team {
         wsync();
     }
 
+    // The contact force recovered as a generalized force, Jᵀ f/dt = L·(Y f/dt)
+    // with u = Y f/dt in sd[T.u] (solver.rs:3334-3364), added to the applied
+    // force as step_trees does (world.rs:1899-1906).
+    __device__ __forceinline__ void add_contact_force() const {
+        const int nv = m.nv;
+        if (lane < nv) {
+            double s = 0.0;
+            for (int k = 0; k <= lane; ++k) s += sd[T.Li + lane * nv + k] * sd[T.u + k];
+            sd[T.tau + lane] = sd[T.qfrc + lane] + (0.0 + s);
+        }
+        wsync();
+    }
+
+    // rk4_step_with_workspace (tree.rs:1234-1364): sd[T.tau] held over four
+    // explicit ABA stages (no implicit damping); qacc reports the first
+    // stage.  Poses and xups describe the start state on entry.
+    __device__ __forceinline__ void rk4() const {
+        const double dt = m.dt;
+        for (int k = lane; k < m.nq; k += 64) sd[T.q0 + k] = sd[T.q + k];
+        for (int k = lane; k < m.nv; k += 64) sd[T.qd0 + k] = sd[T.qd + k];
+        wsync();
+        for (int st = 0; st < 4; ++st) {
+            if (st > 0) {
+                kinematics();
+                store_xups();
+            }
+            const int out = st == 0 ? T.qacc : T.qfree;
+            aba(out, false);
+            const double h = st == 2 ? dt : dt * 0.5;  // advance_tree_state (tree.rs:1551-1573)
+            // tree_deriv (tree.rs:1485-1540), lane per link; sums
+            // ((k1 + 2 k2) + 2 k3) + k4 in stage order.
+            for (int i = lane; i < m.n_links; i += 64) {
+                const LinkT& l = tb.links[i];
+                if (l.joint == JF_FIXED) continue;
+                const int qo = l.q_off, vo = l.v_off;
+                const int nqi = l.joint == JF_FREE ? 7 : 1, nvi = l.joint == JF_FREE ? 6 : 1;
+                double dq[7];
+                if (l.joint == JF_FREE) {
+                    const Qt<double> o{sd[T.q + qo + 3], sd[T.q + qo + 4], sd[T.q + qo + 5], sd[T.q + qo + 6]};
+                    const V3<double> vw = qrot(o, ld3(T.qd + vo + 3));
+                    const Qt<double> dqo = qderiv(o, ld3(T.qd + vo));
+                    dq[0] = vw.x;
+                    dq[1] = vw.y;
+                    dq[2] = vw.z;
+                    dq[3] = dqo.x;
+                    dq[4] = dqo.y;
+                    dq[5] = dqo.z;
+                    dq[6] = dqo.w;
+                } else {
+                    dq[0] = sd[T.qd + vo];
+                }
+                for (int k = 0; k < nqi; ++k) {
+                    const int a = qo + k;
+                    sd[T.aq + a] = st == 0 ? dq[k] : (st == 3 ? sd[T.aq + a] + dq[k] : sd[T.aq + a] + 2.0 * dq[k]);
+                    if (st < 3) sd[T.q + a] = sd[T.q0 + a] + dq[k] * h;
+                }
+                for (int k = 0; k < nvi; ++k) {
+                    const int a = vo + k;
+                    const double dv = sd[out + a];
+                    sd[T.av + a] = st == 0 ? dv : (st == 3 ? sd[T.av + a] + dv : sd[T.av + a] + 2.0 * dv);
+                    if (st < 3) sd[T.qd + a] = sd[T.qd0 + a] + dv * h;
+                }
+            }
+            wsync();
+        }
+        const double w6 = dt * (1.0 / 6.0);
+        for (int k = lane; k < m.nq; k += 64) sd[T.q + k] = sd[T.q0 + k] + sd[T.aq + k] * w6;
+        for (int k = lane; k < m.nv; k += 64) sd[T.qd + k] = sd[T.qd0 + k] + sd[T.av + k] * w6;
+        wsync();
+        if (lane == 0 && tb.links[0].joint == JF_FREE) {
+            const int qo = T.q + tb.links[0].q_off;
+            const Qt<double> o = qnormalize(Qt<double>{sd[qo + 3], sd[qo + 4], sd[qo + 5], sd[qo + 6]});
+            sd[qo + 3] = o.x;
+            sd[qo + 4] = o.y;
+            sd[qo + 5] = o.z;
+            sd[qo + 6] = o.w;
+        }
+        wsync();
+    }
+
     // One World::step (world.rs:1459-1537); poses and link velocities
     // describe the current state on entry and on exit.  Returns a fault code.
     __device__ __forceinline__ int step(int* np, int* ncon) const {
@@ -1345,11 +1431,17 @@ struct Team {
                 gsync();
                 pgs_large(4 * na, ov);
             }
-            back_subst_add();
-        } else {
-            aba(T.qacc);
+            if (m.integrator == 2)
+                add_contact_force();
+            else
+                back_subst_add();
+        } else if (m.integrator != 2) {
+            aba(T.qacc, true);
         }
-        integrate();
+        if (m.integrator == 2)
+            rk4();
+        else
+            integrate();
         kinematics();
         populate(false);
         *np = __builtin_amdgcn_readfirstlane(si[T.misc]);

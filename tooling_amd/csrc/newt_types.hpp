@@ -72,6 +72,7 @@ struct Layout {
     // FP64 workspace: kinematics, dynamics, contact geometry, factorization
     int poses, lvel, xup, S, v, c, a, ia, pa, d, fs, tau, qacc_free, delta, gres;
     int jt, cgeo, iwc, rreg, rdiag, rbias, f, M, Li, Ld, dvec, dres;
+    int rq0, rv0, rdq, raq, rav;  // RK4 only
     int total_d;
     // integer workspace
     int cids, rinfo, pairs, iwf;

@@ -307,5 +307,14 @@ def hinge_chain(n, dt=0.001, iterations=30, seed=2026) -> Scene:
                  f"fixed-root {n}-hinge capsule chain draped on a plane, PGS {iterations}, Euler dt {dt}")
 
 
+def with_options(sc: Scene, suffix: str, **options) -> Scene:
+    """A copy of `sc` with World options replaced, e.g.
+    ``with_options(pendulum(), "_rk4", integrator=abi.INTEGRATOR_RK4)``."""
+    o = dict(sc.options)
+    o.update(options)
+    return Scene(sc.name + suffix, o, sc.links, sc.geoms, sc.self_collide, sc.sampler,
+                 f"{sc.description}; options {options}")
+
+
 SCENES = {"pendulum3": pendulum, "quadruped18": quadruped, "boxchain6": box_chain,
           "chain40": lambda: hinge_chain(40), "chain64": lambda: hinge_chain(64)}
