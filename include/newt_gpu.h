@@ -126,6 +126,12 @@ typedef struct newt_link_desc {
     double child_quat[4];    /* joint_offset_in_child.1 */
     double mass;
     double inertia[9];       /* inertia_body, column-major */
+    /* JointLimit::solref / solimp of the PGS limit row (joint.rs:124-141);
+     * limit_custom = 0 takes SolRef::DEFAULT and SolImp::DEFAULT. */
+    int32_t limit_custom;
+    int32_t limit_solimp_power;
+    double limit_solref[2];
+    double limit_solimp[4];
 } newt_link_desc;
 
 /* One Geom (newt/src/geom.rs:352-419). */
@@ -187,6 +193,13 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds,
                              int32_t precision, newt_data** out);
 
 void newt_data_destroy(newt_data* data);
+
+/* Tree::qfrc_applied (tree.rs:181-237), [n_worlds*nv]: the user's
+ * generalized force, held across steps.  A step adds its constraint force
+ * to it and takes it back off (world.rs:1899-1946, 2020-2049), so it may
+ * change by rounding as newt's does. */
+newt_status newt_data_set_qfrc_applied(newt_data* data, const double* qfrc);
+newt_status newt_data_get_qfrc_applied(const newt_data* data, double* qfrc);
 
 /* newt_data_get_faults value of a parked world slot (see
  * newt_data_park_worlds); not a newt_status. */

@@ -491,6 +491,9 @@ template <class T> struct Link {
     T damping = T(0), armature = T(0), free_damping = T(0);
     bool has_range = false;
     T range_lo = T(0), range_hi = T(0);
+    T limit_solref[2] = {T(0.02), T(1.0)};                    // JointLimit::solref or SolRef::DEFAULT
+    T limit_solimp[4] = {T(0.9), T(0.95), T(0.001), T(0.5)};  // JointLimit::solimp or SolImp::DEFAULT
+    int limit_power = 2;
     V3<T> pj_pos = vzero<T>();
     Q<T> pj_quat = qidentity<T>();
     V3<T> cj_pos = vzero<T>();

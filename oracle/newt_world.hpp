@@ -758,6 +758,87 @@ template <class T> struct World {
         return qfrc;
     }
 
+    // solve_tree_limits (solver.rs:2509-2628) over the limit rows of
+    // build_tree_solver_rows (solver.rs:2337-2373): one row per hinge/slide
+    // outside its range, in link order (coefficient +1 below the range, -1
+    // above it); A = E M⁻¹ Eᵀ from Cholesky solves; PGS with f >= 0.
+    // Returns the per-dof force delta (zeros when no row is active).
+    std::vector<T> solve_tree_limits(const Tree<T>& tr) const {
+        const int nv = tr.nv();
+        std::vector<T> qfrc(nv, T(0.0));
+        if (nv == 0 || !(dt > T(0.0))) return qfrc;
+        struct LRow {
+            int slot;
+            T coeff, violation;
+            SolRef<T> solref;
+            SolImp<T> solimp;
+        };
+        std::vector<LRow> rows;
+        for (size_t li = 0; li < tr.links.size(); ++li) {
+            const Link<T>& l = tr.links[li];
+            if (!(l.joint == J_HINGE || l.joint == J_SLIDE) || !l.has_range) continue;
+            LRow r;
+            r.slot = tr.v_offset[li];
+            r.solref.timeconst = l.limit_solref[0];
+            r.solref.dampratio = l.limit_solref[1];
+            r.solimp.dmin = l.limit_solimp[0];
+            r.solimp.dmax = l.limit_solimp[1];
+            r.solimp.width = l.limit_solimp[2];
+            r.solimp.midpoint = l.limit_solimp[3];
+            r.solimp.power = (uint32_t)l.limit_power;
+            const T q = tr.q[tr.q_offset[li]];
+            if (q < l.range_lo) {
+                r.coeff = T(1.0);
+                r.violation = l.range_lo - q;
+            } else if (q > l.range_hi) {
+                r.coeff = T(-1.0);
+                r.violation = q - l.range_hi;
+            } else {
+                continue;
+            }
+            rows.push_back(r);
+        }
+        if (rows.empty()) return qfrc;
+        std::vector<T> lfac;
+        if (!cholesky(mass_matrix(tr), nv, lfac)) return qfrc;
+        const int n = (int)rows.size();
+        std::vector<std::vector<T>> dvec(n);
+        for (int i = 0; i < n; ++i) {
+            std::vector<T> e(nv, T(0.0));
+            if (rows[i].coeff != T(0.0)) e[rows[i].slot] += rows[i].coeff;
+            dvec[i] = cholesky_solve(lfac, nv, e);
+        }
+        // TreeRow::dot_response / dot_qdot (solver.rs:2214-2230): sums from 0.0.
+        std::vector<T> a((size_t)n * n), bias(n), diag(n);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) a[(size_t)i * n + j] = T(0.0) + rows[i].coeff * dvec[j][rows[i].slot];
+        for (int i = 0; i < n; ++i) {
+            const T v_row = T(0.0) + rows[i].coeff * tr.qdot[rows[i].slot];
+            const T a_ref = reference_accel(-rows[i].violation, v_row, safe_solref(rows[i].solref, dt), rows[i].solimp);
+            bias[i] = -a_ref * dt;
+            const T a_ii = a[(size_t)i * n + i];
+            const T d = impedance_at_position(-rows[i].violation, T(0.0), rows[i].solimp);  // impedance()
+            const T r = d > T(0.0) ? (T(1.0) - d) / d * a_ii : T(0.0);
+            diag[i] = a_ii + r;
+        }
+        std::vector<T> f(n, T(0.0));
+        for (uint32_t it = 0; it < iterations; ++it)
+            for (int i = 0; i < n; ++i) {
+                T ri = bias[i];
+                for (int j = 0; j < n; ++j) ri += a[(size_t)i * n + j] * f[j];
+                const T extra = (diag[i] - a[(size_t)i * n + i]) * f[i];
+                ri += extra;
+                const T delta = -ri / diag[i];
+                const T nf = f[i] + delta;
+                f[i] = nf < T(0.0) ? T(0.0) : nf;
+            }
+        for (int i = 0; i < n; ++i) {
+            const T f_dt = f[i] / dt;
+            if (rows[i].coeff != T(0.0)) qfrc[rows[i].slot] += rows[i].coeff * f_dt;
+        }
+        return qfrc;
+    }
+
     // World::step (world.rs:1459-1537) for trees.
     void step() {
         if (iterations == 0) fail("solver iterations must be >= 1");
@@ -789,9 +870,10 @@ template <class T> struct World {
             std::vector<T> delta;
             bool prior = tr.disable_penalty_limits;
             if (constraint) {
-                for (auto& l : tr.links)
-                    if (l.has_range) fail("joint-limit rows are not restated");
-                delta.assign(tr.nv(), T(0.0));  // solve_tree_limits with no rows
+                if (mode == M_NEWTON)
+                    for (auto& l : tr.links)
+                        if (l.has_range) fail("Newton joint-limit rows are not restated");
+                delta = solve_tree_limits(tr);
                 for (int s = 0; s < tr.nv(); ++s) delta[s] += tq[t][s];
                 for (int s = 0; s < tr.nv(); ++s) tr.qfrc_applied[s] += delta[s];
                 tr.disable_penalty_limits = true;

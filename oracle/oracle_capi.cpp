@@ -30,6 +30,8 @@ struct Base {
     virtual void set_state(const double* qpos, const double* qvel) = 0;
     virtual void get_state(double* qpos, double* qvel) const = 0;
     virtual void get_qacc(double* qacc) const = 0;
+    virtual void set_qfrc(const double* qfrc) = 0;
+    virtual void get_qfrc(double* qfrc) const = 0;
     virtual void step(int n_steps, int threads) = 0;
     virtual int contacts(int world, int cap, int* ids, double* geo) const = 0;
     virtual int pairs(int world, int cap, int* out) const = 0;
@@ -64,6 +66,11 @@ template <class T> struct Impl : Base {
             l.has_range = d.has_range != 0;
             l.range_lo = T(d.range[0]);
             l.range_hi = T(d.range[1]);
+            // SolRef::DEFAULT (geom.rs:99-102), SolImp::DEFAULT (solver.rs:187-193)
+            const double sr[2] = {0.02, 1.0}, si[4] = {0.9, 0.95, 0.001, 0.5};
+            for (int k = 0; k < 2; ++k) l.limit_solref[k] = T(d.limit_custom ? d.limit_solref[k] : sr[k]);
+            for (int k = 0; k < 4; ++k) l.limit_solimp[k] = T(d.limit_custom ? d.limit_solimp[k] : si[k]);
+            l.limit_power = d.limit_custom ? d.limit_solimp_power : 2;
             l.pj_pos = v3(T(d.parent_pos[0]), T(d.parent_pos[1]), T(d.parent_pos[2]));
             l.pj_quat = Q<T>{T(d.parent_quat[0]), T(d.parent_quat[1]), T(d.parent_quat[2]), T(d.parent_quat[3])};
             l.cj_pos = v3(T(d.child_pos[0]), T(d.child_pos[1]), T(d.child_pos[2]));
@@ -119,6 +126,27 @@ template <class T> struct Impl : Base {
             }
             // Public state setters refresh the query proxies (world.rs:1119-1136).
             worlds[w].refresh_broadphase();
+        }
+    }
+    // Tree::qfrc_applied of every world, [world][nv].
+    void set_qfrc(const double* qfrc) override {
+        int NV = nv();
+        for (size_t w = 0; w < worlds.size(); ++w) {
+            int ov = 0;
+            for (auto& t : worlds[w].trees) {
+                for (int k = 0; k < t.nv(); ++k) t.qfrc_applied[k] = T(qfrc[w * NV + ov + k]);
+                ov += t.nv();
+            }
+        }
+    }
+    void get_qfrc(double* qfrc) const override {
+        int NV = nv();
+        for (size_t w = 0; w < worlds.size(); ++w) {
+            int ov = 0;
+            for (auto& t : worlds[w].trees) {
+                for (int k = 0; k < t.nv(); ++k) qfrc[w * NV + ov + k] = (double)t.qfrc_applied[k];
+                ov += t.nv();
+            }
         }
     }
     void get_state(double* qpos, double* qvel) const override {
@@ -222,6 +250,8 @@ int oracle_nv(void* h) { return static_cast<Base*>(h)->nv(); }
 void oracle_set_state(void* h, const double* qpos, const double* qvel) { static_cast<Base*>(h)->set_state(qpos, qvel); }
 void oracle_get_state(void* h, double* qpos, double* qvel) { static_cast<Base*>(h)->get_state(qpos, qvel); }
 void oracle_get_qacc(void* h, double* qacc) { static_cast<Base*>(h)->get_qacc(qacc); }
+void oracle_set_qfrc(void* h, const double* qfrc) { static_cast<Base*>(h)->set_qfrc(qfrc); }
+void oracle_get_qfrc(void* h, double* qfrc) { static_cast<Base*>(h)->get_qfrc(qfrc); }
 void oracle_step(void* h, int n_steps, int threads) { static_cast<Base*>(h)->step(n_steps, threads); }
 int oracle_contacts(void* h, int world, int cap, int* ids, double* geo) { return static_cast<Base*>(h)->contacts(world, cap, ids, geo); }
 int oracle_pairs(void* h, int world, int cap, int* out) { return static_cast<Base*>(h)->pairs(world, cap, out); }

@@ -39,6 +39,8 @@ def lib():
             "oracle_set_state": (None, [P, PD, PD]),
             "oracle_get_state": (None, [P, PD, PD]),
             "oracle_get_qacc": (None, [P, PD]),
+            "oracle_set_qfrc": (None, [P, PD]),
+            "oracle_get_qfrc": (None, [P, PD]),
             "oracle_step": (None, [P, I, I]),
             "oracle_contacts": (I, [P, I, I, PI, PD]),
             "oracle_pairs": (I, [P, I, I, PI]),
@@ -104,6 +106,16 @@ class OracleWorlds:
     def qacc(self):
         out = np.zeros((self.n, self.nv))
         self.L.oracle_get_qacc(self.h, _dp(out))
+        return out
+
+    def set_qfrc(self, qfrc):
+        """Tree::qfrc_applied of every world, [n_worlds, nv]."""
+        qfrc = np.ascontiguousarray(np.broadcast_to(qfrc, (self.n, self.nv)), dtype=np.float64)
+        self.L.oracle_set_qfrc(self.h, _dp(qfrc))
+
+    def qfrc(self):
+        out = np.zeros((self.n, self.nv))
+        self.L.oracle_get_qfrc(self.h, _dp(out))
         return out
 
     def step(self, n_steps=1, threads=1):

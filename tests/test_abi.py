@@ -26,8 +26,34 @@ def test_python_signatures_cover_the_header():
     assert sorted(abi.SIGNATURES) == declared_symbols()
 
 
-def test_struct_layouts_match_the_header_sizes():
-    # Sizes implied by the header's field list (all 8-byte aligned).
-    assert ctypes.sizeof(abi.Options) == 8 + 24 + 8 + 16
-    assert ctypes.sizeof(abi.LinkDesc) == 4 + 4 + 24 + 8 * 3 + 8 + 16 + 24 + 32 + 24 + 32 + 8 + 72
-    assert ctypes.sizeof(abi.GeomDesc) == 16 + 24 * 2 + 32 + 24 + 16 + 32 + 16
+def test_struct_layouts_match_the_header():
+    """Size and every field offset of the ctypes records against the C
+    compiler's layout of include/newt_gpu.h."""
+    import shutil
+    import subprocess
+    import tempfile
+    import pytest
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"newt_options": abi.Options, "newt_link_desc": abi.LinkDesc, "newt_geom_desc": abi.GeomDesc}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} sizeof() %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "layout.c"), os.path.join(d, "layout")
+        open(src, "w").write("\n".join(lines))
+        subprocess.run([cc, "-std=c99", "-o", exe, src], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line.strip():
+            cname, field, value = line.split()
+            got[(cname, field)] = int(value)
+    for cname, py in structs.items():
+        assert got[(cname, "sizeof()")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)

@@ -1,0 +1,157 @@
+"""Joint-limit rows (solve_tree_limits, newt/src/solver.rs:2337-2628) and a
+nonzero qfrc_applied on the GPU vs the CPU oracle, same seeded inputs.
+
+EXACT is bitwise (state, qacc and the rounding of qfrc_applied that the
+step's add-then-subtract leaves), FP64 within 1e-9 relative, mixed one-step
+qacc within 1e-5.  Scenes: the 3-link pendulum with ranges on every hinge
+and random states (most start outside their range), and the quadruped with
+ranges on hips and knees landing on the plane (limits and contacts in one
+step), each with Euler and RK4, plain and damped.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from tooling_amd import abi, scenes
+from tooling_amd.newt import Data, Model, step
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-9
+TOL_MIXED_QACC = 1e-5
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(a, b, floor=1.0):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = np.maximum(np.abs(b).max(axis=-1), floor)
+    return (np.abs(a - b).max(axis=-1) / den).max()
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    from oracle import pyoracle
+    pyoracle.lib()
+    return pyoracle
+
+
+def ranged(sc, lo, hi, damping=0.0, solref=None):
+    links = [dict(l, range=(lo, hi) if l["joint"] == abi.JOINT_HINGE else None, limit_solref=solref,
+                  damping=damping if l["joint"] == abi.JOINT_HINGE else 0.0) for l in sc.links]
+    return scenes.Scene(sc.name + "_ranged", dict(sc.options), links, sc.geoms, sc.self_collide, sc.sampler,
+                        sc.description + f"; hinge ranges ({lo}, {hi})")
+
+
+def _run(oracle, sc, n, seed, steps, precision, qfrc_scale=0.0, warm=0):
+    q0, v0 = sc.initial_state(n, seed)
+    rng = np.random.default_rng(seed)
+    ow = oracle.OracleWorlds(sc, n, 0)
+    ow.set_state(q0, v0)
+    qfrc = rng.normal(0.0, qfrc_scale, size=(n, ow.nv)) if qfrc_scale else np.zeros((n, ow.nv))
+    ow.set_qfrc(qfrc)
+    if warm:
+        ow.step(warm, threads=8)
+    m = Model(sc)
+    d = Data(m, n, precision)
+    d.set_state(*ow.state())
+    d.set_qfrc(ow.qfrc())
+    step(m, d, steps)
+    ow.step(steps, threads=8)
+    out = d.state() + (d.qacc(), d.qfrc(), d.faults())
+    ref = ow.state() + (ow.qacc(), ow.qfrc())
+    d.close()
+    ow.close()
+    return out, ref
+
+
+def _check(out, ref, precision):
+    (q, v, a, qf, f), (qr, vr, ar, qfr) = out, ref
+    assert not f.any()
+    assert rel_err(q, qr) <= TOL_FP64 and rel_err(v, vr) <= TOL_FP64 and rel_err(a, ar) <= TOL_FP64
+    assert rel_err(qf, qfr) <= TOL_FP64
+    if precision == abi.PRECISION_FP64_EXACT:
+        assert np.array_equal(q, qr) and np.array_equal(v, vr) and np.array_equal(a, ar)
+        assert np.array_equal(qf, qfr)
+
+
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP64_EXACT, abi.PRECISION_FP64])
+@pytest.mark.parametrize("integrator", [abi.INTEGRATOR_EULER, abi.INTEGRATOR_RK4])
+def test_pendulum_limits(oracle, precision, integrator):
+    sc = scenes.with_options(ranged(scenes.pendulum(), -1.0, 1.2), f"_i{integrator}", integrator=integrator)
+    say(f"{sc.name} precision {precision}: 128 worlds x 100 steps, qfrc_applied ~ N(0, 2)")
+    out, ref = _run(oracle, sc, 128, 61, 100, precision, qfrc_scale=2.0)
+    _check(out, ref, precision)
+
+
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP64_EXACT, abi.PRECISION_FP64])
+@pytest.mark.parametrize("integrator", [abi.INTEGRATOR_EULER, abi.INTEGRATOR_RK4])
+@pytest.mark.parametrize("damping", [0.0, 0.4])
+def test_quadruped_limits_and_contacts(oracle, precision, integrator, damping):
+    """Joint angles ~ N(0, 0.2) against ranges (-0.15, 0.15): several
+    limit rows per world from the first step, contacts from landing on."""
+    sc = scenes.with_options(ranged(scenes.quadruped(), -0.15, 0.15, damping=damping), f"_i{integrator}",
+                             integrator=integrator)
+    for warm, steps in ((0, 120), (300, 50)):
+        say(f"{sc.name} damping {damping} precision {precision}: 64 worlds x {steps} steps from step {warm}")
+        out, ref = _run(oracle, sc, 64, 71, steps, precision, qfrc_scale=0.5, warm=warm)
+        _check(out, ref, precision)
+
+
+def test_quadruped_limits_mixed_one_step_qacc(oracle):
+    sc = ranged(scenes.quadruped(), -0.15, 0.15)
+    n = 128
+    q0, v0 = sc.initial_state(n, 81)
+    ow = oracle.OracleWorlds(sc, n, 0)
+    ow.set_state(q0, v0)
+    m = Model(sc)
+    worst = 0.0
+    for t in (0, 80, 80, 240):
+        ow.step(t, threads=8)
+        qs, vs = ow.state()
+        d = Data(m, n, abi.PRECISION_MIXED)
+        d.set_state(qs, vs)
+        step(m, d, 1)
+        ref = oracle.OracleWorlds(sc, n, 0)
+        ref.set_state(qs, vs)
+        ref.step(1, threads=8)
+        worst = max(worst, rel_err(d.qacc(), ref.qacc()))
+        d.close()
+        ref.close()
+    say(f"limits mixed qacc relative error {worst:.3e}")
+    assert worst <= TOL_MIXED_QACC
+
+
+def test_reference_anchor_on_gpu(oracle):
+    """newt/tests/solver_joint_limits.rs:96-133 in FP64 on the GPU: released
+    above its range under a 12 N·m outward torque, the hinge stays below
+    pi/3 + 0.2 and follows the FP64 oracle over 6000 steps."""
+    links = [scenes.link(-1, abi.JOINT_FIXED),
+             scenes.link(0, abi.JOINT_HINGE, axis=(1.0, 0.0, 0.0), child_pos=(0.0, 0.0, 1.0), mass=1.0,
+                         inertia=(0.01, 0, 0, 0, 0.01, 0, 0, 0, 0.01), damping=0.5,
+                         range=(-math.pi / 4, math.pi / 4))]
+    opts = dict(dt=0.005, gravity=(0.0, 0.0, -9.81), integrator=abi.INTEGRATOR_RK4, solver=abi.SOLVER_PGS,
+                iterations=30)
+    sc = scenes.Scene("anchor", opts, links, [], 0, None)
+    m = Model(sc)
+    d = Data(m, 1, abi.PRECISION_FP64)
+    d.set_state(np.array([[math.pi / 3]]), np.zeros((1, 1)))
+    d.set_qfrc(np.array([[12.0]]))
+    ow = oracle.OracleWorlds(sc, 1, 0)
+    ow.set_state(np.array([[math.pi / 3]]), np.zeros((1, 1)))
+    ow.set_qfrc(np.array([[12.0]]))
+    peak = math.pi / 3
+    for _ in range(60):
+        step(m, d, 100)
+        ow.step(100)
+        q = d.state()[0][0, 0]
+        peak = max(peak, q)
+        assert abs(q - ow.state()[0][0, 0]) <= TOL_FP64
+    say(f"anchor peak {peak:.4f}, final {q:.4f}")
+    assert peak < math.pi / 3 + 0.2

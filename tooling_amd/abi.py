@@ -56,6 +56,10 @@ class LinkDesc(C.Structure):
         ("child_quat", D4),
         ("mass", C.c_double),
         ("inertia", C.c_double * 9),
+        ("limit_custom", C.c_int32),
+        ("limit_solimp_power", C.c_int32),
+        ("limit_solref", C.c_double * 2),
+        ("limit_solimp", C.c_double * 4),
     ]
 
 
@@ -103,6 +107,8 @@ SIGNATURES = {
     "newt_data_get_worlds": (_I, [_P, _I, _I, _PD, _PD]),
     "newt_data_device_bytes": (_I, [_P, C.POINTER(C.c_uint64)]),
     "newt_data_get_qacc": (_I, [_P, _PD]),
+    "newt_data_set_qfrc_applied": (_I, [_P, _PD]),
+    "newt_data_get_qfrc_applied": (_I, [_P, _PD]),
     "newt_data_get_faults": (_I, [_P, _PI]),
     "newt_data_contact_capacity": (_I, [_P, _PI]),
     "newt_data_get_contacts": (_I, [_P, _PI, _PI, _PD]),

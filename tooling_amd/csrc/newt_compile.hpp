@@ -25,6 +25,7 @@ struct HostModel {
     std::vector<PairT> pairs;
     std::vector<int> chain;
     std::vector<int> level_off, level_links, level_nrank, dof_link;
+    std::vector<LimitT> limits;
     std::vector<double> q_default;
 };
 
@@ -96,6 +97,7 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     const auto& T = M->trees[0];
     int nq = 0, nv = 0;
     M->links.clear();
+    M->limits.clear();
     M->chain.clear();
     M->q_default.clear();
     for (size_t i = 0; i < T.size(); ++i) {
@@ -106,8 +108,27 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
             return cfail(err, NEWT_ERR_INVALID_ARG, "parent index must precede its link");
         if (i > 0 && d.joint == NEWT_JOINT_FREE) return cfail(err, NEWT_ERR_INVALID_ARG, "free joint only at the root");
         if (d.joint == NEWT_JOINT_BALL) return cfail(err, NEWT_ERR_UNSUPPORTED, "ball joints are not built on the GPU path");
-        if (d.has_range) return cfail(err, NEWT_ERR_UNSUPPORTED, "joint ranges (limit rows) are not built on the GPU path");
+        if (d.has_range && !(d.joint == NEWT_JOINT_HINGE || d.joint == NEWT_JOINT_SLIDE))
+            return cfail(err, NEWT_ERR_INVALID_ARG, "a range needs a hinge or slide joint");
+        if (d.has_range && o.solver_mode != NEWT_SOLVER_PGS)
+            return cfail(err, NEWT_ERR_UNSUPPORTED,
+                         "joint ranges are built as PGS limit rows only (penalty limit torques and Newton "
+                         "limit rows are not built)");
         LinkT l{};
+        l.lim = -1;
+        if (d.has_range) {
+            LimitT lt{};
+            lt.lo = d.range[0];
+            lt.hi = d.range[1];
+            // SolRef::DEFAULT (geom.rs:99-102), SolImp::DEFAULT (solver.rs:187-193)
+            const double sr[2] = {0.02, 1.0}, sim[4] = {0.9, 0.95, 0.001, 0.5};
+            for (int k = 0; k < 2; ++k) lt.solref[k] = d.limit_custom ? d.limit_solref[k] : sr[k];
+            for (int k = 0; k < 4; ++k) lt.solimp[k] = d.limit_custom ? d.limit_solimp[k] : sim[k];
+            lt.power = d.limit_custom ? d.limit_solimp_power : 2;
+            lt.link = (int)i;
+            l.lim = (int)M->limits.size();
+            M->limits.push_back(lt);
+        }
         l.parent = d.parent;
         l.joint = d.joint;
         l.q_off = nq;
@@ -250,6 +271,7 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     d.cap_r = std::max(4 * cap_c, 1);
     d.n_chain = (int)M->chain.size();
     d.integrator = o.integrator;
+    d.n_ranged = (int)M->limits.size();
     d.mode = o.solver_mode;
     d.iterations = o.iterations;
     d.cone = o.cone;
@@ -311,11 +333,22 @@ inline Layout make_layout(const ModelT& m, int real_bytes) {
     L.rdq = rk4 ? D(m.nq) : 0;
     L.raq = rk4 ? D(m.nq) : 0;
     L.rav = rk4 ? D(nv) : 0;
+    const int nr = m.n_ranged;
+    L.lim = D(nv);
+    L.ldv = D(std::max(nr * nv, 1));
+    L.lA = D(std::max(nr * nr, 1));
+    L.lb = D(std::max(nr, 1));
+    L.ldg = D(std::max(nr, 1));
+    L.lf = D(std::max(nr, 1));
+    L.lc = D(std::max(nr, 1));
+    L.lvi = D(std::max(nr, 1));
     L.total_d = dd;
     L.cids = I(3 * m.cap_c);
     L.rinfo = I(3 * m.cap_r);
     L.pairs = I(std::max(m.n_pairs, 1));
     L.iwf = I(n + 2);
+    L.lsl = I(std::max(m.n_ranged, 1));
+    L.llm = I(std::max(m.n_ranged, 1));
     L.total_i = ii;
     (void)real_bytes;
     return L;

@@ -59,6 +59,7 @@ struct newt_model {
     int* d_level_links = nullptr;
     int* d_level_nrank = nullptr;
     int* d_dof_link = nullptr;
+    LimitT* d_limits = nullptr;
     CacheLayout cache{};     // model tables as the persistent step caches them in LDS
     char* d_blob = nullptr;  // the same tables packed in that layout
 };
@@ -241,6 +242,9 @@ newt_status newt_model_finalize(newt_model* model) {
                           hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(model->d_level_nrank, H.level_nrank.data(), sizeof(int) * H.level_nrank.size(),
                       hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&model->d_limits, sizeof(LimitT) * std::max<size_t>(1, H.limits.size())));
+    if (!H.limits.empty())
+        HIP_TRY(hipMemcpy(model->d_limits, H.limits.data(), sizeof(LimitT) * H.limits.size(), hipMemcpyHostToDevice));
     model->dev = H.dev;
     model->dev.links = model->d_links;
     model->dev.geoms = model->d_geoms;
@@ -250,6 +254,7 @@ newt_status newt_model_finalize(newt_model* model) {
     model->dev.level_links = model->d_level_links;
     model->dev.level_nrank = model->d_level_nrank;
     model->dev.dof_link = model->d_dof_link;
+    model->dev.limits = model->d_limits;
     {
         const ModelT& hd = H.dev;
         CacheLayout C = make_cache_layout(hd);
@@ -264,6 +269,7 @@ newt_status newt_model_finalize(newt_model* model) {
         put(C.level_links, H.level_links.data(), sizeof(int) * H.level_links.size());
         put(C.level_nrank, H.level_nrank.data(), sizeof(int) * H.level_nrank.size());
         put(C.dof_link, H.dof_link.data(), sizeof(int) * H.dof_link.size());
+        put(C.limits, H.limits.data(), sizeof(LimitT) * H.limits.size());
         HIP_TRY(hipMalloc(&model->d_blob, blob.size()));
         HIP_TRY(hipMemcpy(model->d_blob, blob.data(), blob.size(), hipMemcpyHostToDevice));
         model->cache = C;
@@ -291,6 +297,7 @@ void newt_model_destroy(newt_model* model) {
     (void)hipFree(model->d_level_links);
     (void)hipFree(model->d_level_nrank);
     (void)hipFree(model->d_dof_link);
+    (void)hipFree(model->d_limits);
     (void)hipFree(model->d_blob);
     delete model;
 }
@@ -400,6 +407,25 @@ newt_status newt_data_set_state(newt_data* D, const double* qpos, const double* 
     if (m.nv) HIP_TRY(hipMemcpyAsync(D->d.qvel, b.data(), b.size() * 8, hipMemcpyHostToDevice, D->last_stream));
     HIP_TRY(hipMemsetAsync(D->d.fault, 0, (size_t)D->W * 4, D->last_stream));
     HIP_TRY(hipStreamSynchronize(D->last_stream));
+    return NEWT_OK;
+}
+
+newt_status newt_data_set_qfrc_applied(newt_data* D, const double* qfrc) {
+    if (!D || !qfrc) return fail(NEWT_ERR_INVALID_ARG, "null argument");
+    const ModelT& m = D->model->dev;
+    std::vector<double> a;
+    to_lanes(qfrc, a, D->W, m.nv);
+    if (m.nv) HIP_TRY(hipMemcpyAsync(D->d.qfrc, a.data(), a.size() * 8, hipMemcpyHostToDevice, D->last_stream));
+    HIP_TRY(hipStreamSynchronize(D->last_stream));
+    return NEWT_OK;
+}
+
+newt_status newt_data_get_qfrc_applied(const newt_data* D, double* qfrc) {
+    if (!D || !qfrc) return fail(NEWT_ERR_INVALID_ARG, "null argument");
+    const ModelT& m = D->model->dev;
+    std::vector<double> a((size_t)D->W * m.nv);
+    HIP_TRY(d2h(D, a.data(), D->d.qfrc, a.size() * 8));
+    from_lanes(a, qfrc, D->W, m.nv);
     return NEWT_OK;
 }
 

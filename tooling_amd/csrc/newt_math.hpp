@@ -221,4 +221,51 @@ template <class R> NEWT_HD void m6pullback(const R* a, const Xf<R>& x, R* out) {
     }
 }
 
+// SolImp with MuJoCo's clamps (effective_solimp, solver.rs:319-328).
+struct SolImpE {
+    double dmin, dmax, width, mid;
+    int power;
+};
+NEWT_HD inline SolImpE solimp_effective(const double* si, int power) {
+    const double lo = 1.0e-4, hi = 0.9999;
+    auto cl = [&](double v) { return v < lo ? lo : (v > hi ? hi : v); };
+    return SolImpE{cl(si[0]), cl(si[1]), si[2] > 0.0 ? si[2] : 0.0, cl(si[3]), power < 1 ? 1 : power};
+}
+// impedance_at_position (solver.rs:261-309) at zero margin.
+NEWT_HD inline double impedance_at(const SolImpE& s, double position) {
+    double x = position / s.width;
+    if (x < 0.0) x = -x;
+    double y;
+    if (x <= 0.0) {
+        y = 0.0;
+    } else if (x >= 1.0) {
+        y = 1.0;
+    } else if (x < s.mid) {
+        double b = x / s.mid, pw = 1.0;
+        for (int k = 0; k < s.power; ++k) pw = pw * b;
+        y = s.mid * pw;
+    } else {
+        double b = (1.0 - x) / (1.0 - s.mid), pw = 1.0;
+        for (int k = 0; k < s.power; ++k) pw = pw * b;
+        y = 1.0 - (1.0 - s.mid) * pw;
+    }
+    return s.dmin + (s.dmax - s.dmin) * y;
+}
+// reference_accel with safe_solref (solver.rs:331-384).
+NEWT_HD inline double reference_accel(double position, double velocity, double tc, double dr, const SolImpE& s,
+                                      double dt) {
+    const bool direct = tc < 0.0;
+    if (!direct && tc < 2.0 * dt) tc = 2.0 * dt;
+    double b, k;
+    if (direct) {
+        b = -dr / s.dmax;
+        k = -tc / (s.dmax * s.dmax);
+    } else {
+        const double d = impedance_at(s, position);
+        b = 2.0 / (s.dmax * tc);
+        k = d / (s.dmax * s.dmax * tc * tc * dr * dr);
+    }
+    return -b * velocity - k * position;
+}
+
 }  // namespace newt

@@ -833,11 +833,89 @@ template <class R> struct World {
         }
         iw[L.iwf + m.n_links] = np;
         iw[L.iwf + m.n_links + 1] = nc;
+        if (constraint)
+            solve_limits();
+        else
+            for (int s = 0; s < nv; ++s) dw[L.lim + s] = 0.0;
         return 0;
     }
 
     // post: implicit-damping ABA with the contact force in dw[L.delta],
     // semi-implicit Euler, and the broadphase refresh on the new state.
+    // solve_tree_limits (solver.rs:2509-2628) over the limit rows of
+    // build_tree_solver_rows (2337-2373): one row per hinge/slide outside its
+    // range, in link order; A from Cholesky solves of M; PGS with f >= 0.
+    // The per-dof force into dw[L.lim] (zero without active rows).
+    NEWT_HD void solve_limits() const {
+        const int nv = m.nv;
+        const double dt = m.dt;
+        for (int s = 0; s < nv; ++s) dw[L.lim + s] = 0.0;
+        if (m.n_ranged == 0 || nv == 0 || !(dt > 0.0)) return;
+        int n = 0;
+        for (int i = 0; i < m.n_links; ++i) {
+            const LinkT& l = m.links[i];
+            if (l.lim < 0) continue;
+            const LimitT& lt = m.limits[l.lim];
+            const double q = qpos[l.q_off];
+            double c, viol;
+            if (q < lt.lo) {
+                c = 1.0;
+                viol = lt.lo - q;
+            } else if (q > lt.hi) {
+                c = -1.0;
+                viol = q - lt.hi;
+            } else {
+                continue;
+            }
+            iw[L.lsl + n] = l.v_off;
+            iw[L.llm + n] = l.lim;
+            dw[L.lc + n] = c;
+            dw[L.lvi + n] = viol;
+            ++n;
+        }
+        if (n == 0) return;
+        mass_matrix();
+        if (!cholesky(L.M, L.Ld, false, 0.0)) return;  // no limit force (solver.rs:2529-2532)
+        for (int i = 0; i < n; ++i) {
+            for (int s = 0; s < nv; ++s) dw[L.dres + s] = 0.0;
+            const int sl = iw[L.lsl + i];
+            dw[L.dres + sl] = dw[L.dres + sl] + dw[L.lc + i];
+            chol_solve(L.Ld, L.dres, L.ldv + i * nv);
+        }
+        // TreeRow::dot_response / dot_qdot (solver.rs:2214-2230)
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j)
+                dw[L.lA + i * n + j] = 0.0 + dw[L.lc + i] * dw[L.ldv + j * nv + iw[L.lsl + i]];
+        for (int i = 0; i < n; ++i) {
+            const LimitT& lt = m.limits[iw[L.llm + i]];
+            const SolImpE si = solimp_effective(lt.solimp, lt.power);
+            const double viol = dw[L.lvi + i];
+            const double vrow = 0.0 + dw[L.lc + i] * qvel[iw[L.lsl + i]];
+            const double aref = reference_accel(-viol, vrow, lt.solref[0], lt.solref[1], si, dt);
+            dw[L.lb + i] = -aref * dt;
+            const double aii = dw[L.lA + i * n + i];
+            const double d = impedance_at(si, -viol);
+            const double r = d > 0.0 ? (1.0 - d) / d * aii : 0.0;
+            dw[L.ldg + i] = aii + r;
+            dw[L.lf + i] = 0.0;
+        }
+        for (int it = 0; it < m.iterations; ++it)
+            for (int i = 0; i < n; ++i) {
+                double ri = dw[L.lb + i];
+                for (int j = 0; j < n; ++j) ri = ri + dw[L.lA + i * n + j] * dw[L.lf + j];
+                const double extra = (dw[L.ldg + i] - dw[L.lA + i * n + i]) * dw[L.lf + i];
+                ri = ri + extra;
+                const double delta = -ri / dw[L.ldg + i];
+                const double nf = dw[L.lf + i] + delta;
+                dw[L.lf + i] = nf < 0.0 ? 0.0 : nf;
+            }
+        for (int i = 0; i < n; ++i) {
+            const double fdt = dw[L.lf + i] / dt;
+            const int sl = iw[L.lsl + i];
+            dw[L.lim + sl] = dw[L.lim + sl] + dw[L.lc + i] * fdt;
+        }
+    }
+
     // tree_deriv (tree.rs:1485-1540) at the current state into dw[L.rdq]
     // (position rates); the velocity rates are the accelerations.
     NEWT_HD void deriv() const {
@@ -868,7 +946,6 @@ template <class R> struct World {
     NEWT_HD void rk4(bool constraint) const {
         const int nv = m.nv, nq = m.nq;
         const double dt = m.dt;
-        for (int s = 0; s < nv; ++s) dw[L.tau + s] = qfrc[s] + (0.0 + dw[L.delta + s]);
         for (int k = 0; k < nq; ++k) dw[L.rq0 + k] = qpos[k];
         for (int s = 0; s < nv; ++s) dw[L.rv0 + s] = qvel[s];
         const double h[3] = {dt * 0.5, dt * 0.5, dt};
@@ -908,16 +985,24 @@ template <class R> struct World {
         const R dt = (R)m.dt;
         const int nv = m.nv;
         bool constraint = m.mode == 1;
+        // step_trees_euler / step_trees (world.rs:1826-2051): the limit force
+        // plus the contact force is added to qfrc_applied for the step and
+        // taken back off afterwards.  dw[L.lim] keeps that delta.
+        for (int s = 0; s < nv; ++s) {
+            const double dl = dw[L.lim + s] + dw[L.delta + s];
+            dw[L.lim + s] = dl;
+            dw[L.tau + s] = qfrc[s] + dl;
+        }
         if (m.integrator == 2) {
             rk4(constraint);
+            for (int s = 0; s < nv; ++s) qfrc[s] = dw[L.tau + s] - dw[L.lim + s];
             forward_kinematics();
             link_velocities();
             populate(dt, false);
             return;
         }
-        // step_trees_euler (world.rs:1952-2051): applied + (0 + delta).
-        for (int s = 0; s < nv; ++s) dw[L.tau + s] = qfrc[s] + (0.0 + dw[L.delta + s]);
         aba(constraint, true, m.dt, L.delta);
+        for (int s = 0; s < nv; ++s) qfrc[s] = dw[L.tau + s] - dw[L.lim + s];
         for (int s = 0; s < nv; ++s) qacc[s] = dw[L.delta + s];
         // euler_step_with_workspace (tree.rs:1407-1418) + integrate_tree_positions
         // (1422-1475).  Velocity and position accumulation is FP64 in both

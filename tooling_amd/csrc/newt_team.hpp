@@ -55,6 +55,9 @@ struct TeamLayout {
     int i_off;                    // byte offset of the ints
     int cids, ccnt, coff, plist, alist, misc;
     int q0, qd0, aq, av;          // RK4: start state and weighted stage sums
+    int qlim, lsg, lvi;           // joint limits: force [nv]; row coefficient, violation
+    int ylim;                     // union, solve side: rows of L⁻¹ Eᵀ, FP64, n_ranged x nv
+    int lsl, llm;                 // ints: row dof slot, limit index
     int bytes;                    // per world, multiple of 16
     // Global block per world holding the rows of contact sets of more than
     // 64 rows (cap_r x nv of R; stride 0 when the model cannot produce them).
@@ -90,6 +93,10 @@ inline TeamLayout make_team_layout(const ModelT& m, int real_bytes) {
     T.qd0 = rk4 ? D(nv) : 0;
     T.aq = rk4 ? D(nq) : 0;
     T.av = rk4 ? D(nv) : 0;
+    const int nrl = m.n_ranged;
+    T.qlim = nrl ? D(nv) : 0;
+    T.lsg = nrl ? D(nrl) : 0;
+    T.lvi = nrl ? D(nrl) : 0;
     d = (d + 1) / 2 * 2;  // 16-byte aligned union
     const int u0 = d;
     T.ia = D(36 * n);
@@ -110,7 +117,10 @@ inline TeamLayout make_team_layout(const ModelT& m, int real_bytes) {
     const int solve_end_bytes = T.j_off + real_bytes * lr * (nv > 0 ? nv : 1);
     T.lg = u0;
     const int large_end_bytes = cr > 64 ? 8 * (u0 + 4 * cr) : 0;
-    const int union_end_bytes = std::max(std::max(8 * aba_end, solve_end_bytes), large_end_bytes);
+    T.ylim = u0;
+    const int lim_end_bytes = 8 * (u0 + nrl * nv);
+    const int union_end_bytes =
+        std::max(std::max(std::max(8 * aba_end, solve_end_bytes), large_end_bytes), lim_end_bytes);
     T.total_d = (union_end_bytes + 7) / 8;
     T.i_off = (union_end_bytes + 15) / 16 * 16;
     int i = 0;
@@ -121,6 +131,8 @@ inline TeamLayout make_team_layout(const ModelT& m, int real_bytes) {
     T.plist = I(m.n_pairs);
     T.alist = I(m.cap_c);
     T.misc = I(8);
+    T.lsl = I(nrl);
+    T.llm = I(nrl);
     T.bytes = (T.i_off + 4 * i + 15) / 16 * 16;
     if (cr > 64) T.ov_stride = (real_bytes * cr * (nv > 0 ? nv : 1) + 255) / 256 * 256;
     return T;
@@ -129,7 +141,7 @@ inline TeamLayout make_team_layout(const ModelT& m, int real_bytes) {
 // Byte offsets of the model tables as cached in LDS (shared by the worlds of
 // a workgroup); the same layout is the device blob they are copied from.
 struct CacheLayout {
-    int links, geoms, pairs, level_off, level_links, level_nrank, dof_link;
+    int links, geoms, pairs, level_off, level_links, level_nrank, dof_link, limits;
     int bytes;
 };
 
@@ -144,6 +156,7 @@ inline CacheLayout make_cache_layout(const ModelT& m) {
     C.level_links = B(4 * m.n_links);
     C.level_nrank = B(4 * m.n_levels);
     C.dof_link = B(4 * m.nv);
+    C.limits = B((int)sizeof(LimitT) * m.n_ranged);
     C.bytes = b > 0 ? b : 16;
     return C;
 }
@@ -221,6 +234,7 @@ struct Tables {
     const int* level_links;
     const int* level_nrank;
     const int* dof_link;
+    const LimitT* limits;
 };
 
 template <class R, int NV>
@@ -1318,15 +1332,110 @@ struct Team {
         wsync();
     }
 
+    // The limit rows of build_tree_solver_rows (solver.rs:2337-2373): one
+    // per hinge/slide outside its range, in link order (coefficient +1 below
+    // the range, -1 above).  Returns the count, uniform.
+    __device__ __forceinline__ int limit_rows() const {
+        int base = 0;
+        for (int i0 = 0; i0 < m.n_links; i0 += 64) {
+            const int i = i0 + lane;
+            double c = 0.0, viol = 0.0;
+            int slot = 0, li = -1;
+            if (i < m.n_links && tb.links[i].lim >= 0) {
+                const LinkT& l = tb.links[i];
+                const LimitT& lt = tb.limits[l.lim];
+                const double q = sd[T.q + l.q_off];
+                if (q < lt.lo) {
+                    c = 1.0;
+                    viol = lt.lo - q;
+                    li = l.lim;
+                } else if (q > lt.hi) {
+                    c = -1.0;
+                    viol = q - lt.hi;
+                    li = l.lim;
+                }
+                slot = l.v_off;
+            }
+            const unsigned long long bal = __ballot(li >= 0);
+            if (li >= 0) {
+                const int k = base + __popcll(bal & lanes_below(lane));
+                si[T.lsl + k] = slot;
+                si[T.llm + k] = li;
+                sd[T.lsg + k] = c;
+                sd[T.lvi + k] = viol;
+            }
+            base += __popcll(bal);
+        }
+        wsync();
+        return __builtin_amdgcn_readfirstlane(base);
+    }
+
+    // solve_tree_limits (solver.rs:2509-2628) for nl rows: lane i holds row
+    // i, y_i = L⁻¹(c_i e_slot) with L of M (no damping term) from sd[ld],
+    // 1/L_kk from sd[ldr]; A = Y Yᵀ in FP64; the PGS update of sweeps()
+    // (same projection, f >= 0) with R_i = (1-d)/d A_ii.  The per-dof force
+    // goes to sd[T.qlim].
+    __device__ __forceinline__ void limit_solve(int nl, int ld, int ldr) const {
+        const int nv = m.nv;
+        const double dt = m.dt;
+        const bool act = lane < nl;
+        double* y = sd + T.ylim + (act ? lane : 0) * nv;
+        int slot = 0;
+        double c = 0.0;
+        if (act) {
+            slot = si[T.lsl + lane];
+            c = sd[T.lsg + lane];
+            for (int s = 0; s < nv; ++s) y[s] = 0.0;
+            y[slot] = 0.0 + c;
+            for (int k = slot; k < nv; ++k) {
+                const double yk = y[k] * sd[ldr + k];
+                y[k] = yk;
+                for (int s = k + 1; s < nv; ++s) y[s] -= sd[ld + s * nv + k] * yk;
+            }
+        }
+        wsync();
+        double a[64];
+#pragma unroll
+        for (int j = 0; j < 64; ++j) {
+            double acc = 0.0;
+            if (j < NV && act && j < nl) {
+                const double* yj = sd + T.ylim + j * nv;
+                for (int k = 0; k < nv; ++k) acc += y[k] * yj[k];
+            }
+            a[j] = acc;
+        }
+        double dg = 0.0, cf = 0.0, invd = 0.0, r = 0.0, f = 0.0;
+        if (act) {
+            for (int k = 0; k < nv; ++k) dg += y[k] * y[k];
+            const LimitT& lt = tb.limits[si[T.llm + lane]];
+            const SolImpE sim = solimp_effective(lt.solimp, lt.power);
+            const double viol = sd[T.lvi + lane];
+            const double vrow = 0.0 + c * sd[T.qd + slot];
+            r = -reference_accel(-viol, vrow, lt.solref[0], lt.solref[1], sim, dt) * dt;
+            const double d = impedance_at(sim, -viol);
+            const double reg = d > 0.0 ? (1.0 - d) / d * dg : 0.0;
+            invd = 1.0 / (dg + reg);
+            cf = 1.0 - reg * invd;
+        }
+        if (nl <= 16) sweeps<16>(a, cf, invd, r, f);
+        else if (NV <= 32 || nl <= 32) sweeps<32>(a, cf, invd, r, f);
+        else if (nl <= 48) sweeps<48>(a, cf, invd, r, f);
+        else sweeps<64>(a, cf, invd, r, f);
+        for (int s = lane; s < nv; s += 64) sd[T.qlim + s] = 0.0;
+        wsync();
+        if (act) sd[T.qlim + slot] = 0.0 + c * (f / dt);
+        wsync();
+    }
+
     // The contact force recovered as a generalized force, Jᵀ f/dt = L·(Y f/dt)
     // with u = Y f/dt in sd[T.u] (solver.rs:3334-3364), added to the applied
     // force as step_trees does (world.rs:1899-1906).
-    __device__ __forceinline__ void add_contact_force() const {
+    __device__ __forceinline__ void add_contact_force(bool with_limits) const {
         const int nv = m.nv;
         if (lane < nv) {
             double s = 0.0;
             for (int k = 0; k <= lane; ++k) s += sd[T.Li + lane * nv + k] * sd[T.u + k];
-            sd[T.tau + lane] = sd[T.qfrc + lane] + (0.0 + s);
+            sd[T.tau + lane] = sd[T.qfrc + lane] + ((with_limits ? sd[T.qlim + lane] : 0.0) + s);
         }
         wsync();
     }
@@ -1403,42 +1512,71 @@ struct Team {
     __device__ __forceinline__ int step(int* np, int* ncon) const {
         const int nc = contacts();
         const int na = m.mode == 1 ? active_contacts(nc) : 0;
+        const bool rk4_ = m.integrator == 2;
         store_xups();
         for (int s = lane; s < m.nv; s += 64) sd[T.tau + s] = sd[T.qfrc + s];
         wsync();
-        if (na > 0) {
+        // Limit rows at the start state (step_trees_euler / step_trees,
+        // world.rs:1826-2051); PGS mode only.
+        int nl = m.n_ranged > 0 && m.mode == 1 ? limit_rows() : 0;
+        if (na > 0 || nl > 0) {
             mass_matrix();
             double la[NV], lrinv;
             if (m.damped) {
-                if (!factor(T.Ld, T.Ldr, la, lrinv)) return 3;  // NEWT_ERR_NOT_PD
-                invweights(T.Ld, T.Ldr);
+                if (!factor(T.Ld, T.Ldr, la, lrinv)) {
+                    if (na > 0) return 3;  // NEWT_ERR_NOT_PD
+                    nl = 0;                // no limit force (solver.rs:2529-2532)
+                } else {
+                    if (na > 0) invweights(T.Ld, T.Ldr);
+                    if (nl > 0) limit_solve(nl, T.Ld, T.Ldr);
+                }
                 add_damping();
             }
-            if (!factor(T.Li, T.Lr, la, lrinv)) return 3;
-            if (!m.damped) invweights_regs(la, lrinv);
-            // free acceleration: (M + dt·D) q̈ = τ − D q̇ − C
-            rne_rhs(T.qfree);
-            {
-                const double qf = bwd_vec(fwd_vec(lane < m.nv ? sd[T.qfree + lane] : 0.0, la, lrinv));
-                if (lane < m.nv) sd[T.qfree + lane] = qf;
-                wsync();
+            if (!factor(T.Li, T.Lr, la, lrinv)) {
+                if (na > 0) return 3;
+                nl = 0;
+            } else if (!m.damped) {
+                if (na > 0) invweights_regs(la, lrinv);
+                if (nl > 0) limit_solve(nl, T.Li, T.Lr);
             }
-            if (4 * na <= 64) {
-                rows(na, RowStore<R>{sJ, sd + T.rbias, sd + T.rreg, sd + T.rdg, sd + T.f}, la, lrinv);
-                pgs_small(4 * na);
+            if (na > 0) {
+                // free acceleration: (M + dt·D) q̈ = τ − D q̇ − C
+                rne_rhs(T.qfree);
+                {
+                    const double qf = bwd_vec(fwd_vec(lane < m.nv ? sd[T.qfree + lane] : 0.0, la, lrinv));
+                    if (lane < m.nv) sd[T.qfree + lane] = qf;
+                    wsync();
+                }
+                if (4 * na <= 64) {
+                    rows(na, RowStore<R>{sJ, sd + T.rbias, sd + T.rreg, sd + T.rdg, sd + T.f}, la, lrinv);
+                    pgs_small(4 * na);
+                } else {
+                    rows(na, ov, la, lrinv);
+                    gsync();
+                    pgs_large(4 * na, ov);
+                }
+                if (rk4_) {
+                    add_contact_force(nl > 0);
+                } else {
+                    if (nl > 0) {
+                        // u += L⁻¹ τ_limit: q̈ = q̈_free + (M + dt·D)⁻¹(Jᵀf/dt + τ_limit)
+                        const double x = fwd_vec(lane < m.nv ? sd[T.qlim + lane] : 0.0, la, lrinv);
+                        if (lane < m.nv) sd[T.u + lane] = sd[T.u + lane] + x;
+                        wsync();
+                    }
+                    back_subst_add();
+                }
             } else {
-                rows(na, ov, la, lrinv);
-                gsync();
-                pgs_large(4 * na, ov);
+                if (nl > 0) {
+                    for (int s = lane; s < m.nv; s += 64) sd[T.tau + s] = sd[T.qfrc + s] + (sd[T.qlim + s] + 0.0);
+                    wsync();
+                }
+                if (!rk4_) aba(T.qacc, true);
             }
-            if (m.integrator == 2)
-                add_contact_force();
-            else
-                back_subst_add();
-        } else if (m.integrator != 2) {
+        } else if (!rk4_) {
             aba(T.qacc, true);
         }
-        if (m.integrator == 2)
+        if (rk4_)
             rk4();
         else
             integrate();
@@ -1471,7 +1609,7 @@ __global__ void __launch_bounds__(64 * team_max_worlds<NV>())
     const Tables tb{(const LinkT*)(lds + C.links),     (const GeomT*)(lds + C.geoms),
                     (const PairT*)(lds + C.pairs),     (const int*)(lds + C.level_off),
                     (const int*)(lds + C.level_links), (const int*)(lds + C.level_nrank),
-                    (const int*)(lds + C.dof_link)};
+                    (const int*)(lds + C.dof_link),    (const LimitT*)(lds + C.limits)};
     // The wavefront index is uniform within a wavefront; readfirstlane lets
     // the compiler treat everything derived from it as scalar.
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63, W = d.W;

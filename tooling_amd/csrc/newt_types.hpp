@@ -27,6 +27,15 @@ struct LinkT {
     double xfE[9], xfr[3];     // fixed-joint parent->child transform
     double si[36];             // 6x6 spatial inertia at the COM
     double mass, damping, armature, free_damping;
+    int lim, pad1;             // index into ModelT::limits, -1 without a range
+};
+
+// Range of a hinge/slide with its PGS limit-row parameters (JointKind range,
+// JointLimit::solref/solimp, joint.rs:124-141).
+struct LimitT {
+    double lo, hi;
+    double solref[2], solimp[4];
+    int power, link;
 };
 
 struct GeomT {
@@ -60,8 +69,9 @@ struct ModelT {
     const int* level_links;
     const int* level_nrank;
     const int* dof_link;  // [nv] link owning each dof
-    int damped;  // some joint has damping or free_damping != 0
-    int pad0;
+    int damped;    // some joint has damping or free_damping != 0 (Euler)
+    int n_ranged;  // hinges/slides with a range
+    const LimitT* limits;
 };
 
 // Offsets (in elements) of the per-world scratch regions.
@@ -73,9 +83,13 @@ struct Layout {
     int poses, lvel, xup, S, v, c, a, ia, pa, d, fs, tau, qacc_free, delta, gres;
     int jt, cgeo, iwc, rreg, rdiag, rbias, f, M, Li, Ld, dvec, dres;
     int rq0, rv0, rdq, raq, rav;  // RK4 only
+    // joint-limit rows (up to n_ranged): force delta [nv], M⁻¹ e_i, A, and
+    // per-row bias, diagonal, impulse, coefficient, violation
+    int lim, ldv, lA, lb, ldg, lf, lc, lvi;
     int total_d;
     // integer workspace
     int cids, rinfo, pairs, iwf;
+    int lsl, llm;  // limit rows: dof slot, limit index
     int total_i;
 };
 

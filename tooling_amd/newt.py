@@ -106,6 +106,16 @@ class Data:
         abi.check(self.lib.newt_data_get_state(self.handle, self._dp(qpos), self._dp(qvel)))
         return qpos, qvel
 
+    def set_qfrc(self, qfrc):
+        """Tree::qfrc_applied of every world, [n_worlds, nv] (broadcast)."""
+        qfrc = np.ascontiguousarray(np.broadcast_to(qfrc, (self.n_worlds, self.model.nv)), dtype=np.float64)
+        abi.check(self.lib.newt_data_set_qfrc_applied(self.handle, self._dp(qfrc)))
+
+    def qfrc(self):
+        out = np.zeros((self.n_worlds, self.model.nv))
+        abi.check(self.lib.newt_data_get_qfrc_applied(self.handle, self._dp(out)))
+        return out
+
     def qacc(self):
         out = np.zeros((self.n_worlds, self.model.nv))
         abi.check(self.lib.newt_data_get_qacc(self.handle, self._dp(out)))
