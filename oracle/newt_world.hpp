@@ -246,6 +246,36 @@ std::pair<V3<T>, V3<T>> closest_points_on_segments(V3<T> a0, V3<T> a1, V3<T> b0,
     return {a0 + d1 * sf, b0 + d2 * t};
 }
 
+// closest_point_on_segment (contact.rs:2313-2322).
+template <class T> V3<T> closest_point_on_segment(V3<T> p, V3<T> a, V3<T> b) {
+    V3<T> ab = b - a;
+    T denom = dot(ab, ab);
+    if (denom == T(0.0)) return a;
+    T t = clamp01(dot(p - a, ab) / denom);
+    return a + ab * t;
+}
+
+// sphere_vs_sphere_at (contact.rs:434-467): one contact on B's surface,
+// normal B -> A (+Z when the centers coincide).
+template <class T>
+void sphere_vs_sphere_at(int ia, V3<T> ca, T ra, int ib, V3<T> cb, T rb, T fr, T margin, T gap,
+                         std::vector<Contact<T>>& out) {
+    V3<T> delta = ca - cb;
+    T dist2 = dot(delta, delta);
+    T sum_r_margin = (ra + rb) + margin;
+    if (dist2 >= sum_r_margin * sum_r_margin) return;
+    T dist = std::sqrt(dist2);
+    V3<T> n = dist > T(0.0) ? delta / dist : V3<T>{T(0.0), T(0.0), T(1.0)};
+    out.push_back({ia, ib, (int)out.size(), cb + n * rb, n, sum_r_margin - dist, fr, gap});
+}
+
+// Capsule segment end points: position ∓ axis·half_height, axis = local +Z.
+template <class T> void capsule_segment(const Pose<T>& p, T half_h, V3<T>* a, V3<T>* b) {
+    V3<T> ax = rotate(p.q, V3<T>{T(0.0), T(0.0), T(1.0)});
+    *a = p.p - ax * half_h;
+    *b = p.p + ax * half_h;
+}
+
 template <class T> struct FV2 {
     T u, v, depth;
 };
@@ -409,6 +439,27 @@ bool try_narrow(int ia, const Geom<T>& ga, const Pose<T>& pa, int ib, const Geom
     if (ga.shape == S_SPHERE && gb.shape == S_PLANE) { sphere_plane(ia, pa, ga.size.x, fr, margin, gap, ib, pb, out); return true; }
     if (ga.shape == S_BOX && gb.shape == S_PLANE) { box_plane(ia, pa, ga.size, fr, margin, gap, ib, pb, out); return true; }
     if (ga.shape == S_CAPSULE && gb.shape == S_PLANE) { capsule_plane(ia, pa, ga.size.x, ga.size.y, fr, margin, gap, ib, pb, out); return true; }
+    // sphere_sphere (contact.rs:407-427), sphere_capsule (473-499),
+    // capsule_capsule (1249-1272).
+    if (ga.shape == S_SPHERE && gb.shape == S_SPHERE) {
+        sphere_vs_sphere_at(ia, pa.p, ga.size.x, ib, pb.p, gb.size.x, fr, margin, gap, out);
+        return true;
+    }
+    if (ga.shape == S_SPHERE && gb.shape == S_CAPSULE) {
+        V3<T> b0, b1;
+        capsule_segment(pb, gb.size.y, &b0, &b1);
+        sphere_vs_sphere_at(ia, pa.p, ga.size.x, ib, closest_point_on_segment(pa.p, b0, b1), gb.size.x, fr, margin,
+                            gap, out);
+        return true;
+    }
+    if (ga.shape == S_CAPSULE && gb.shape == S_CAPSULE) {
+        V3<T> a0, a1, b0, b1;
+        capsule_segment(pa, ga.size.y, &a0, &a1);
+        capsule_segment(pb, gb.size.y, &b0, &b1);
+        std::pair<V3<T>, V3<T>> cp = closest_points_on_segments(a0, a1, b0, b1);
+        sphere_vs_sphere_at(ia, cp.first, ga.size.x, ib, cp.second, gb.size.x, fr, margin, gap, out);
+        return true;
+    }
     if (ga.shape == S_BOX && gb.shape == S_BOX) {
         box_box(ia, pa, ga.size, ib, pb, gb.size, fr, margin, gap, out);
         return true;

@@ -221,6 +221,93 @@ template <class R> NEWT_HD void m6pullback(const R* a, const Xf<R>& x, R* out) {
     }
 }
 
+// clamp01 (contact.rs:2324-2334): NaN passes through.
+NEWT_HD inline double clamp01(double x) { return x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x); }
+// closest_point_on_segment (contact.rs:2313-2322).
+NEWT_HD inline V3<double> closest_on_segment(V3<double> p, V3<double> a, V3<double> b) {
+    const V3<double> ab = b - a;
+    const double denom = dot3(ab, ab);
+    if (denom == 0.0) return a;
+    return a + ab * clamp01(dot3(p - a, ab) / denom);
+}
+// closest_points_on_segments (contact.rs:2342-2385): points on [a0,a1] and
+// [b0,b1], clamp-and-recheck with newt's 1e-12 degeneracy threshold.
+NEWT_HD inline void closest_on_segments(V3<double> a0, V3<double> a1, V3<double> b0, V3<double> b1, V3<double>* pa,
+                                        V3<double>* pb) {
+    const V3<double> d1 = a1 - a0, d2 = b1 - b0, r = a0 - b0;
+    const double a = dot3(d1, d1), e = dot3(d2, d2), f = dot3(d2, r);
+    const double eps = 1.0e-12;
+    if (a <= eps && e <= eps) {
+        *pa = a0;
+        *pb = b0;
+        return;
+    }
+    if (a <= eps) {
+        *pa = a0;
+        *pb = b0 + d2 * clamp01(f / e);
+        return;
+    }
+    const double c = dot3(d1, r);
+    if (e <= eps) {
+        *pa = a0 + d1 * clamp01(-c / a);
+        *pb = b0;
+        return;
+    }
+    const double bb = dot3(d1, d2);
+    const double denom = a * e - bb * bb;
+    const double s = denom != 0.0 ? clamp01((bb * f - c * e) / denom) : 0.0;
+    const double tn = bb * s + f;
+    double t, sf;
+    if (tn < 0.0) {
+        t = 0.0;
+        sf = clamp01(-c / a);
+    } else if (tn > e) {
+        t = 1.0;
+        sf = clamp01((bb - c) / a);
+    } else {
+        t = tn / e;
+        sf = s;
+    }
+    *pa = a0 + d1 * sf;
+    *pb = b0 + d2 * t;
+}
+// Sphere and capsule pairs, cap_* telling capsules from spheres
+// (sphere_sphere contact.rs:407-427, sphere_capsule
+// 473-499, capsule_capsule 1249-1272), reduced to sphere_vs_sphere_at
+// (434-467).  The pair is (a, b) in candidate order; a capsule-sphere pair
+// runs as sphere-capsule and is relabelled with the normal negated
+// (dispatch_narrow_phase, 2579-2588).  Returns false when the spheres do not
+// touch; else the contact point on b's side, the normal b -> a and the
+// penetration.
+NEWT_HD inline bool round_pair(bool cap_a, V3<double> pa, Qt<double> qa, const double* size_a, bool cap_b,
+                               V3<double> pb, Qt<double> qb, const double* size_b, double margin, V3<double>* point,
+                               V3<double>* normal, double* pen) {
+    const bool swap = cap_a && !cap_b;
+    const bool cx_cap = swap ? cap_b : cap_a, cy_cap = swap ? cap_a : cap_b;
+    const V3<double> px = swap ? pb : pa, py = swap ? pa : pb;
+    const Qt<double> qx = swap ? qb : qa, qy = swap ? qa : qb;
+    const double* zx = swap ? size_b : size_a;
+    const double* zy = swap ? size_a : size_b;
+    V3<double> cx = px, cy = py;
+    if (!cx_cap && cy_cap) {
+        const V3<double> ay = qrot(qy, mk3(0.0, 0.0, 1.0));
+        cy = closest_on_segment(px, py - ay * zy[1], py + ay * zy[1]);
+    } else if (cx_cap && cy_cap) {
+        const V3<double> ax = qrot(qx, mk3(0.0, 0.0, 1.0)), ay = qrot(qy, mk3(0.0, 0.0, 1.0));
+        closest_on_segments(px - ax * zx[1], px + ax * zx[1], py - ay * zy[1], py + ay * zy[1], &cx, &cy);
+    }
+    const V3<double> delta = cx - cy;
+    const double dist2 = dot3(delta, delta);
+    const double srm = (zx[0] + zy[0]) + margin;
+    if (dist2 >= srm * srm) return false;
+    const double dist = nsqrt(dist2);
+    const V3<double> n = dist > 0.0 ? delta / dist : mk3(0.0, 0.0, 1.0);
+    *point = cy + n * zy[0];
+    *normal = swap ? -n : n;
+    *pen = srm - dist;
+    return true;
+}
+
 // SolImp with MuJoCo's clamps (effective_solimp, solver.rs:319-328).
 struct SolImpE {
     double dmin, dmax, width, mid;

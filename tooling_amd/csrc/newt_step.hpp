@@ -316,8 +316,20 @@ template <class R> struct World {
             double margin = (double)ga.margin > (double)gb.margin ? (double)ga.margin : (double)gb.margin;
             double gap = (double)ga.gap > (double)gb.gap ? (double)ga.gap : (double)gb.gap;
             // (primitive, plane) in the given order, else swapped.
-            if (gb.shape == SH_PLANE) plane_primitive(&nc, pr.a, ga, pr.b, gb, false, fr, margin, gap);
-            else plane_primitive(&nc, pr.b, gb, pr.a, ga, true, fr, margin, gap);
+            if (ga.shape != SH_PLANE && gb.shape != SH_PLANE) {
+                V3<double> pa, pb, pt, nrm;
+                Qt<double> qa, qb;
+                double pen;
+                geom_pose(ga, &pa, &qa);
+                geom_pose(gb, &pb, &qb);
+                if (round_pair(ga.shape == SH_CAPSULE, pa, qa, ga.size, gb.shape == SH_CAPSULE, pb, qb, gb.size,
+                               margin, &pt, &nrm, &pen))
+                    emit(&nc, pr.a, pr.b, 0, pt, nrm, pen, fr, gap);
+            } else if (gb.shape == SH_PLANE) {
+                plane_primitive(&nc, pr.a, ga, pr.b, gb, false, fr, margin, gap);
+            } else {
+                plane_primitive(&nc, pr.b, gb, pr.a, ga, true, fr, margin, gap);
+            }
         }
         return nc;
     }

@@ -459,6 +459,32 @@ struct Team {
         double fr = ga.friction < gb.friction ? ga.friction : gb.friction;
         double margin = ga.margin > gb.margin ? ga.margin : gb.margin;
         double gap = ga.gap > gb.gap ? ga.gap : gb.gap;
+        if (ga.shape != SH_PLANE && gb.shape != SH_PLANE) {
+            V3<double> pa, pb, pt, nrm;
+            Qt<double> qa, qb;
+            double pen;
+            geom_pose(ga, &pa, &qa);
+            geom_pose(gb, &pb, &qb);
+            if (!round_pair(ga.shape == SH_CAPSULE, pa, qa, ga.size, gb.shape == SH_CAPSULE, pb, qb, gb.size, margin,
+                            &pt, &nrm, &pen))
+                return 0;
+            if (out >= 0) {
+                si[T.cids + 3 * out] = pr.a;
+                si[T.cids + 3 * out + 1] = pr.b;
+                si[T.cids + 3 * out + 2] = 0;
+                double* g = sd + T.cgeo + 9 * out;
+                g[0] = pt.x;
+                g[1] = pt.y;
+                g[2] = pt.z;
+                g[3] = nrm.x;
+                g[4] = nrm.y;
+                g[5] = nrm.z;
+                g[6] = pen;
+                g[7] = fr;
+                g[8] = gap;
+            }
+            return 1;
+        }
         // (primitive, plane) as given, else swapped with the normal negated
         // (contact.rs:2579-2588).
         bool flip = gb.shape != SH_PLANE;

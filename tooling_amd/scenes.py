@@ -63,10 +63,12 @@ def link(parent, joint, *, axis=(0.0, 0.0, 0.0), parent_pos=(0.0, 0.0, 0.0), par
 
 def geom(shape, *, tree=-1, link=-1, size=(0.0, 0.0, 0.0), pos=(0.0, 0.0, 0.0), quat=IDENTITY_Q,
          friction=1.0, margin=0.0, gap=0.0, solref=(0.02, 1.0), solimp=(0.9, 0.95, 0.001, 0.5), power=2,
-         condim=3):
-    """Geom defaults follow Geom::static_plane / *_on_link (geom.rs:465-700)."""
+         condim=3, group=0xFFFFFFFF, mask=0xFFFFFFFF):
+    """Geom defaults follow Geom::static_plane / *_on_link (geom.rs:465-700);
+    `group`/`mask` are collision_group / collision_mask (broadphase.rs:16-18)."""
     return dict(shape=shape, tree=tree, link=link, size=size, pos=pos, quat=quat, friction=friction,
-                margin=margin, gap=gap, solref=solref, solimp=solimp, power=power, condim=condim)
+                margin=margin, gap=gap, solref=solref, solimp=solimp, power=power, condim=condim, group=group,
+                mask=mask)
 
 
 @dataclass
@@ -147,8 +149,8 @@ class Scene:
             d.solref = (abi.C.c_double * 2)(*g["solref"])
             d.solimp = (abi.C.c_double * 4)(*g["solimp"])
             d.solimp_power = g["power"]
-            d.group = 0xFFFFFFFF
-            d.mask = 0xFFFFFFFF
+            d.group = g.get("group", 0xFFFFFFFF)
+            d.mask = g.get("mask", 0xFFFFFFFF)
             out.append(d)
         return out
 
@@ -322,6 +324,45 @@ def hinge_chain(n, dt=0.001, iterations=30, seed=2026) -> Scene:
                  f"fixed-root {n}-hinge capsule chain draped on a plane, PGS {iterations}, Euler dt {dt}")
 
 
+def hinge_coil(n=10, dt=0.001, iterations=30) -> Scene:
+    """Planar chain of n hinges (axes X) under a fixed root, folded like an
+    accordion (bends of about ±2.6 rad) so that links two apart lie close
+    together; capsule links (0.1 m, radius 0.02 m, 0.5 kg) and a sphere of
+    radius 0.025 m at the tip of every even link.  Self collision is on;
+    collision groups let geoms of links of equal parity collide with each
+    other (never neighbours) and every geom with the plane below, giving
+    capsule-capsule, sphere-capsule, sphere-sphere and plane contacts."""
+    half = CHAIN_LINK / 2
+    height = 0.35
+    links = [link(-1, abi.JOINT_FIXED, parent_pos=(0.0, 0.0, height))]
+    plane_bit = 4
+    geoms = [geom(abi.SHAPE_PLANE, friction=0.8, group=7, mask=7)]
+    ixx, izz = solid_capsule_inertia(0.5, CHAIN_R, half - CHAIN_R)
+    for i in range(n):
+        links.append(link(i, abi.JOINT_HINGE, axis=(1.0, 0.0, 0.0),
+                          parent_pos=(0.0, 0.0, 0.0) if i == 0 else (0.0, 0.0, -half),
+                          child_pos=(0.0, 0.0, half), mass=0.5, inertia=_diag(ixx, ixx, izz),
+                          armature=CHAIN_ARMATURE))
+        grp = 1 << (i % 2)
+        geoms.append(geom(abi.SHAPE_CAPSULE, tree=0, link=i + 1, size=(CHAIN_R, half - CHAIN_R, 0.0), friction=0.8,
+                          group=grp, mask=grp | plane_bit))
+        if i % 2 == 0:
+            geoms.append(geom(abi.SHAPE_SPHERE, tree=0, link=i + 1, size=(0.025, 0.0, 0.0), pos=(0.0, 0.0, -half),
+                              friction=0.8, group=grp, mask=grp | plane_bit))
+
+    def sampler(nw, seed_):
+        r = _rng(seed_)
+        sign = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+        q = sign * 2.6 + r.normal(0.0, 0.15, size=(nw, n))
+        q[:, 0] = r.normal(0.0, 0.3, size=nw)
+        return q, r.normal(0.0, 0.5, size=(nw, n))
+
+    opts = dict(dt=dt, gravity=(0.0, 0.0, -9.81), integrator=abi.INTEGRATOR_EULER,
+                solver=abi.SOLVER_PGS, iterations=iterations)
+    return Scene(f"coil{n}", opts, links, geoms, 1, sampler,
+                 f"folded {n}-hinge capsule chain with self collision, PGS {iterations}, Euler dt {dt}")
+
+
 def with_options(sc: Scene, suffix: str, **options) -> Scene:
     """A copy of `sc` with World options replaced, e.g.
     ``with_options(pendulum(), "_rk4", integrator=abi.INTEGRATOR_RK4)``."""
@@ -331,5 +372,5 @@ def with_options(sc: Scene, suffix: str, **options) -> Scene:
                  f"{sc.description}; options {options}")
 
 
-SCENES = {"pendulum3": pendulum, "quadruped18": quadruped, "boxchain6": box_chain,
+SCENES = {"pendulum3": pendulum, "quadruped18": quadruped, "boxchain6": box_chain, "coil10": hinge_coil,
           "chain40": lambda: hinge_chain(40), "chain64": lambda: hinge_chain(64)}
