@@ -36,6 +36,7 @@ inline newt_status cfail(std::string* err, newt_status s, const std::string& msg
 
 // Largest contact count one candidate pair can emit (ContactBuf, contact.rs:108-124).
 inline int pair_max_contacts(int sa, int sb) {
+    if (sa == SH_BOX && sb == SH_BOX) return 4;      // box_box keeps the four deepest
     if (sa != SH_PLANE && sb != SH_PLANE) return 1;  // sphere_vs_sphere_at
     int p = sa == SH_PLANE ? sb : sa;
     if (p == SH_SPHERE) return 1;
@@ -254,10 +255,9 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
             const bool round_a = ga.shape == NEWT_SHAPE_SPHERE || ga.shape == NEWT_SHAPE_CAPSULE;
             const bool round_b = gb.shape == NEWT_SHAPE_SPHERE || gb.shape == NEWT_SHAPE_CAPSULE;
             const std::string which = "contact pair geom " + std::to_string(a) + " x geom " + std::to_string(b);
-            if (ga.shape == NEWT_SHAPE_BOX && gb.shape == NEWT_SHAPE_BOX)
-                return cfail(err, NEWT_ERR_UNSUPPORTED, which + ": box-box contacts are not built on the GPU path");
+            const bool boxes = ga.shape == NEWT_SHAPE_BOX && gb.shape == NEWT_SHAPE_BOX;
             // is_pair_supported (contact.rs:232-257) for plane, sphere, box, capsule
-            if (!(pa != pb || (round_a && round_b)))
+            if (!(pa != pb || (round_a && round_b) || boxes))
                 return cfail(err, NEWT_ERR_UNSUPPORTED_PAIR, which + " is an unsupported shape pair");
             if (std::min(ga.condim, gb.condim) != 3)
                 return cfail(err, NEWT_ERR_UNSUPPORTED, "only condim 3 pyramidal contacts are built");

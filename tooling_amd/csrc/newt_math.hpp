@@ -271,6 +271,168 @@ NEWT_HD inline void closest_on_segments(V3<double> a0, V3<double> a1, V3<double>
     *pa = a0 + d1 * sf;
     *pb = b0 + d2 * t;
 }
+// Box-box contact manifold: box_box_full_manifold -> box_box_sat_fallback
+// (contact.rs:569-583, 716-868), edge_midpoint (1173-1196),
+// box_box_edge_edge_contact (872-910), box_box_face_reference_contacts
+// (917-1079) with sutherland_hodgman_axis_rect (1115-1167).  15-axis SAT with
+// the minimum overlap winning; an edge-edge winner gives one contact at the
+// edges' closest points, a face winner clips the most anti-parallel face of
+// the other box against the reference face and keeps the four deepest
+// points (stable order).  Writes up to four points on b's side, their
+// penetrations and the shared normal b -> a; returns the count.
+struct ClipV {
+    double u, v, depth;
+};
+NEWT_HD inline void other_two(int k, int* a, int* b) {
+    *a = k == 0 ? 1 : 0;
+    *b = k == 2 ? 1 : 2;
+}
+NEWT_HD inline V3<double> edge_midpoint(V3<double> c, const V3<double>* basis, const double* half, int ei,
+                                        V3<double> dir) {
+    int j, k;
+    other_two(ei, &j, &k);
+    const double sj = dot3(dir, basis[j]) >= 0.0 ? 1.0 : -1.0;
+    const double sk = dot3(dir, basis[k]) >= 0.0 ? 1.0 : -1.0;
+    return (c + basis[j] * (sj * half[j])) + basis[k] * (sk * half[k]);
+}
+NEWT_HD inline int box_box(V3<double> pa, Qt<double> qa, const double* ha, V3<double> pb, Qt<double> qb,
+                           const double* hb, double margin, V3<double>* cpos_out, double* pen_out,
+                           V3<double>* normal) {
+    const V3<double> ex = mk3(1.0, 0.0, 0.0), ey = mk3(0.0, 1.0, 0.0), ez = mk3(0.0, 0.0, 1.0);
+    const V3<double> ax[3] = {qrot(qa, ex), qrot(qa, ey), qrot(qa, ez)};
+    const V3<double> bx[3] = {qrot(qb, ex), qrot(qb, ey), qrot(qb, ez)};
+    const V3<double> delta = pa - pb;
+    auto proj = [](V3<double> l, const V3<double>* u, const double* h) {
+        return (h[0] * nabs(dot3(l, u[0])) + h[1] * nabs(dot3(l, u[1]))) + h[2] * nabs(dot3(l, u[2]));
+    };
+    double best = __builtin_inf();
+    V3<double> best_axis = ez;
+    int kind = 0, k1 = 0, k2 = 0;  // 0 face of a, 1 face of b, 2 edge-edge
+    auto test = [&](V3<double> l, int knd, int i, int j) {
+        const double sgn = dot3(delta, l);
+        const double ov = (proj(l, ax, ha) + proj(l, bx, hb)) - nabs(sgn);
+        if (ov < -margin) return true;
+        if (ov < best) {
+            best = ov;
+            best_axis = sgn >= 0.0 ? l : -l;
+            kind = knd;
+            k1 = i;
+            k2 = j;
+        }
+        return false;
+    };
+    for (int k = 0; k < 3; ++k)
+        if (test(ax[k], 0, k, 0)) return 0;
+    for (int k = 0; k < 3; ++k)
+        if (test(bx[k], 1, k, 0)) return 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const V3<double> raw = cross3(ax[i], bx[j]);
+            const double len2 = dot3(raw, raw);
+            if (len2 < 1.0e-6) continue;
+            if (test(raw / nsqrt(len2), 2, i, j)) return 0;
+        }
+    if (!__builtin_isfinite(best)) return 0;
+    const double pen_shift = best + margin;
+    if (pen_shift <= 0.0) return 0;
+    const V3<double> nw = best_axis;
+    *normal = nw;
+    if (kind == 2) {
+        const V3<double> ma = edge_midpoint(pa, ax, ha, k1, -nw);
+        const V3<double> mb = edge_midpoint(pb, bx, hb, k2, nw);
+        V3<double> sa, sb;
+        closest_on_segments(ma - ax[k1] * ha[k1], ma + ax[k1] * ha[k1], mb - bx[k2] * hb[k2], mb + bx[k2] * hb[k2],
+                            &sa, &sb);
+        cpos_out[0] = sb;
+        pen_out[0] = pen_shift;
+        return 1;
+    }
+    const bool ref_a = kind == 0;
+    const int rk = k1;
+    const V3<double> rc = ref_a ? pa : pb;
+    const V3<double>* rb = ref_a ? ax : bx;
+    const double* rh = ref_a ? ha : hb;
+    const V3<double> ic = ref_a ? pb : pa;
+    const V3<double>* ibs = ref_a ? bx : ax;
+    const double* ih = ref_a ? hb : ha;
+    const V3<double> ron = ref_a ? -nw : nw;
+    const V3<double> rfa = rb[rk];
+    const double rs = dot3(ron, rfa) >= 0.0 ? 1.0 : -1.0;
+    const V3<double> rfc = rc + rfa * (rs * rh[rk]);
+    int ru, rv;
+    other_two(rk, &ru, &rv);
+    int iax = 0;
+    double isg = 1.0, idot = __builtin_inf();
+    for (int k = 0; k < 3; ++k)
+        for (int e = 0; e < 2; ++e) {
+            const double sg = e == 0 ? 1.0 : -1.0;
+            const double d = dot3(ibs[k] * sg, ron);
+            if (d < idot) {
+                idot = d;
+                iax = k;
+                isg = sg;
+            }
+        }
+    const V3<double> ifc = ic + ibs[iax] * (isg * ih[iax]);
+    int iu, iv;
+    other_two(iax, &iu, &iv);
+    const V3<double> U = ibs[iu], Vv = ibs[iv];
+    const double hu = ih[iu], hv = ih[iv];
+    const V3<double> corners[4] = {(ifc - U * hu) - Vv * hv, (ifc + U * hu) - Vv * hv, (ifc + U * hu) + Vv * hv,
+                                   (ifc - U * hu) + Vv * hv};
+    ClipV poly[8], tmp[8];
+    int np = 4;
+    for (int i = 0; i < 4; ++i) {
+        const V3<double> rel = corners[i] - rfc;
+        poly[i] = ClipV{dot3(rel, rb[ru]), dot3(rel, rb[rv]), -dot3(rel, ron)};
+    }
+    const double ru_h = rh[ru], rv_h = rh[rv];
+    for (int edge = 0; edge < 4; ++edge) {
+        if (np == 0) break;
+        for (int i = 0; i < np; ++i) tmp[i] = poly[i];
+        const int n = np;
+        np = 0;
+        auto side = [&](const ClipV& p) {
+            return edge == 0 ? p.u + ru_h : (edge == 1 ? ru_h - p.u : (edge == 2 ? p.v + rv_h : rv_h - p.v));
+        };
+        for (int i = 0; i < n; ++i) {
+            const ClipV cur = tmp[i], prev = tmp[(i + n - 1) % n];
+            const double cs = side(cur), ps = side(prev);
+            const bool ci = cs >= 0.0, pi = ps >= 0.0;
+            if (ci != pi) {  // the edge crosses the clip line
+                const double t = ps / (ps - cs);
+                poly[np++] = ClipV{prev.u + (cur.u - prev.u) * t, prev.v + (cur.v - prev.v) * t,
+                                   prev.depth + (cur.depth - prev.depth) * t};
+            }
+            if (ci) poly[np++] = cur;
+        }
+    }
+    double cpen[8];
+    V3<double> cpos[8];
+    int count = 0;
+    for (int i = 0; i < np; ++i) {
+        const double pen = poly[i].depth + margin;
+        if (pen <= 0.0) continue;
+        const V3<double> por = (rfc + rb[ru] * poly[i].u) + rb[rv] * poly[i].v;
+        cpen[count] = pen;
+        cpos[count] = ref_a ? por + nw * (pen - margin) : por;
+        if (++count == 8) break;
+    }
+    int order[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+    for (int i = 1; i < count; ++i)
+        for (int j = i; j > 0 && cpen[order[j]] > cpen[order[j - 1]]; --j) {
+            const int t = order[j - 1];
+            order[j - 1] = order[j];
+            order[j] = t;
+        }
+    const int take = count > 4 ? 4 : count;
+    for (int i = 0; i < take; ++i) {
+        cpos_out[i] = cpos[order[i]];
+        pen_out[i] = cpen[order[i]];
+    }
+    return take;
+}
+
 // Sphere and capsule pairs, cap_* telling capsules from spheres
 // (sphere_sphere contact.rs:407-427, sphere_capsule
 // 473-499, capsule_capsule 1249-1272), reduced to sphere_vs_sphere_at

@@ -255,6 +255,10 @@ template <class R> struct World {
 
     NEWT_HD void emit(int* nc, int ga, int gb, int slot, V3<double> p, V3<double> n, double pen, double fr, double gap) const {
         int k = *nc;
+        if (k >= m.cap_c) {  // beyond the compiled capacity: counted, not stored (a fault)
+            *nc = k + 1;
+            return;
+        }
         iw[L.cids + 3 * k] = ga;
         iw[L.cids + 3 * k + 1] = gb;
         iw[L.cids + 3 * k + 2] = slot;
@@ -316,7 +320,15 @@ template <class R> struct World {
             double margin = (double)ga.margin > (double)gb.margin ? (double)ga.margin : (double)gb.margin;
             double gap = (double)ga.gap > (double)gb.gap ? (double)ga.gap : (double)gb.gap;
             // (primitive, plane) in the given order, else swapped.
-            if (ga.shape != SH_PLANE && gb.shape != SH_PLANE) {
+            if (ga.shape == SH_BOX && gb.shape == SH_BOX) {
+                V3<double> pa, pb, nrm, cp[4];
+                Qt<double> qa, qb;
+                double pen[4];
+                geom_pose(ga, &pa, &qa);
+                geom_pose(gb, &pb, &qb);
+                const int c = box_box(pa, qa, ga.size, pb, qb, gb.size, margin, cp, pen, &nrm);
+                for (int i = 0; i < c; ++i) emit(&nc, pr.a, pr.b, i, cp[i], nrm, pen[i], fr, gap);
+            } else if (ga.shape != SH_PLANE && gb.shape != SH_PLANE) {
                 V3<double> pa, pb, pt, nrm;
                 Qt<double> qa, qb;
                 double pen;
@@ -840,6 +852,7 @@ template <class R> struct World {
         for (int s = 0; s < nv; ++s) dw[L.delta + s] = 0.0;
         if (constraint) {
             nc = narrowphase(np);
+            if (nc > m.cap_c) return 8;  // NEWT_ERR_CAPACITY
             int nr;
             if (!solve_contacts(nc, dt, &nr)) return 3;  // NEWT_ERR_NOT_PD
         }

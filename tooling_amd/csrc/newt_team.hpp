@@ -459,6 +459,32 @@ struct Team {
         double fr = ga.friction < gb.friction ? ga.friction : gb.friction;
         double margin = ga.margin > gb.margin ? ga.margin : gb.margin;
         double gap = ga.gap > gb.gap ? ga.gap : gb.gap;
+        if (ga.shape == SH_BOX && gb.shape == SH_BOX) {
+            V3<double> pa, pb, nrm, cp[4];
+            Qt<double> qa, qb;
+            double pen[4];
+            geom_pose(ga, &pa, &qa);
+            geom_pose(gb, &pb, &qb);
+            const int c = box_box(pa, qa, ga.size, pb, qb, gb.size, margin, cp, pen, &nrm);
+            if (out >= 0)
+                for (int i = 0; i < c; ++i) {
+                    const int o = out + i;
+                    si[T.cids + 3 * o] = pr.a;
+                    si[T.cids + 3 * o + 1] = pr.b;
+                    si[T.cids + 3 * o + 2] = i;
+                    double* g = sd + T.cgeo + 9 * o;
+                    g[0] = cp[i].x;
+                    g[1] = cp[i].y;
+                    g[2] = cp[i].z;
+                    g[3] = nrm.x;
+                    g[4] = nrm.y;
+                    g[5] = nrm.z;
+                    g[6] = pen[i];
+                    g[7] = fr;
+                    g[8] = gap;
+                }
+            return c;
+        }
         if (ga.shape != SH_PLANE && gb.shape != SH_PLANE) {
             V3<double> pa, pb, pt, nrm;
             Qt<double> qa, qb;
@@ -559,6 +585,9 @@ struct Team {
             si[T.misc + 1] = off;
         }
         wsync();
+        // The compiled capacity bounds every pair's count; a larger total is
+        // reported as a fault by step() and nothing past it is written.
+        if (__builtin_amdgcn_readfirstlane(si[T.misc + 1]) > m.cap_c) return m.cap_c + 1;
         for (int k = lane; k < m.n_pairs; k += 64)
             if (si[T.ccnt + k] > 0) pair_contacts(k, si[T.coff + k]);
         wsync();
@@ -1537,6 +1566,7 @@ struct Team {
     // describe the current state on entry and on exit.  Returns a fault code.
     __device__ __forceinline__ int step(int* np, int* ncon) const {
         const int nc = contacts();
+        if (nc > m.cap_c) return 8;  // NEWT_ERR_CAPACITY
         const int na = m.mode == 1 ? active_contacts(nc) : 0;
         const bool rk4_ = m.integrator == 2;
         store_xups();

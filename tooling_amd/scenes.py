@@ -363,6 +363,36 @@ def hinge_coil(n=10, dt=0.001, iterations=30) -> Scene:
                  f"folded {n}-hinge capsule chain with self collision, PGS {iterations}, Euler dt {dt}")
 
 
+def box_corridor(n_boxes=6, dt=0.002, iterations=30) -> Scene:
+    """The hinged box chain of box_chain() between two static box walls
+    (half extents 1.0 x 0.03 x 0.1 m, 4 mm beyond the boxes' sides) that the
+    swinging boxes strike, with self collision between boxes two or more
+    links apart (collision groups): box-box contacts with one side on a link
+    and the other static or on another link, beside box-plane contacts."""
+    base = box_chain(n_boxes, dt, iterations)
+    geoms = [dict(base.geoms[0], group=7, mask=7)]
+    for i, g in enumerate(base.geoms[1:]):
+        grp = 1 << (i % 2)
+        geoms.append(dict(g, group=grp, mask=grp | 4))
+    wall_y = BOX_HALF[1] + 0.03 + 0.004
+    cx = (n_boxes - 1) * (BOX_HALF[0] + 0.01)
+    for sgn in (1.0, -1.0):
+        geoms.append(geom(abi.SHAPE_BOX, size=(1.0, 0.03, 0.1), pos=(cx, sgn * wall_y, 0.1), friction=0.6,
+                          group=7, mask=7))
+
+    def sampler(n, seed):
+        q, v = base.sampler(n, seed)
+        r = _rng(seed + 1)
+        q[:, 7:] *= 0.3
+        v[:, 3:5] = 0.0
+        v[:, 4] = r.normal(0.0, 0.3, size=n)
+        v[:, 6:] = r.normal(0.0, 1.5, size=(n, n_boxes - 1))
+        return q, v
+
+    return Scene(f"boxcorridor{n_boxes}", dict(base.options), base.links, geoms, 1, sampler,
+                 f"{n_boxes} hinged boxes between two static walls, self collision on, PGS {iterations}")
+
+
 def with_options(sc: Scene, suffix: str, **options) -> Scene:
     """A copy of `sc` with World options replaced, e.g.
     ``with_options(pendulum(), "_rk4", integrator=abi.INTEGRATOR_RK4)``."""
