@@ -3,8 +3,9 @@
 Run as ``python -m tooling_amd.build``.  hipcc cross-compiles on a host with
 no GPU.  Packed FP32 instructions are disabled everywhere.  newt_gpu.hip holds
 the bit-exact lane-per-world step, compiled with FP contraction off so it keeps
-newt's rounding; newt_team.hip holds the tolerance-bound persistent step,
-compiled with contraction on (fused multiply-adds).
+newt's rounding; the newt_team*.hip units hold the instantiations of the
+tolerance-bound persistent step, compiled with contraction on (fused
+multiply-adds), one or two per unit so they compile in parallel.
 """
 import os
 import subprocess
@@ -25,8 +26,10 @@ HIPCC_FLAGS = [
 ]
 
 # Per translation unit: FP contraction.
-SOURCES = {"newt_gpu.hip": ["-ffp-contract=off"], "newt_team.hip": ["-ffp-contract=fast"],
-           "newt_team64f.hip": ["-ffp-contract=fast"], "newt_team64d.hip": ["-ffp-contract=fast"]}
+SOURCES = {"newt_gpu.hip": ["-ffp-contract=off"]}
+for _unit in ("newt_team.hip", "newt_team24x.hip", "newt_team32.hip", "newt_team64f.hip", "newt_team64fx.hip",
+              "newt_team64d.hip", "newt_team64dx.hip"):
+    SOURCES[_unit] = ["-ffp-contract=fast"]
 
 
 def build(verbose: bool = False, only=None) -> str:

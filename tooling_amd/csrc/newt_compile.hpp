@@ -277,6 +277,9 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     d.n_chain = (int)M->chain.size();
     d.integrator = o.integrator;
     d.n_ranged = (int)M->limits.size();
+    d.ext = d.n_ranged > 0 || o.integrator == NEWT_INTEGRATOR_RK4 ? 1 : 0;
+    for (const PairT& pr : M->pairs)
+        if (M->hgeoms[pr.a].shape != SH_PLANE && M->hgeoms[pr.b].shape != SH_PLANE) d.ext = 1;
     d.mode = o.solver_mode;
     d.iterations = o.iterations;
     d.cone = o.cone;

@@ -17,10 +17,10 @@
 
 namespace newt {
 // Defined in newt_team.hip (the persistent-step instantiations).
-hipError_t team_launch(bool mixed, int nv, dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const ModelT& m,
-                       const TeamLayout& T, const CacheLayout& C, const int4* blob, const Layout& L, const DataT& d,
-                       double* dws, int* iws, char* ovf, int n_steps, int wpb);
-hipError_t team_set_lds(bool mixed, int nv, int bytes);
+hipError_t team_launch(bool mixed, int nv, bool ext, dim3 grid, dim3 block, int lds_bytes, hipStream_t stream,
+                       const ModelT& m, const TeamLayout& T, const CacheLayout& C, const int4* blob, const Layout& L,
+                       const DataT& d, double* dws, int* iws, char* ovf, int n_steps, int wpb);
+hipError_t team_set_lds(bool mixed, int nv, bool ext, int bytes);
 int team_max_worlds_for(int nv);
 }  // namespace newt
 
@@ -128,8 +128,9 @@ newt_status launch_persistent(newt_data* D, int n_steps, hipStream_t stream) {
     const newt_model* M = D->model;
     const int wpb = D->wpb;
     dim3 grid((D->W + wpb - 1) / wpb), block(64 * wpb);
-    HIP_TRY(team_launch(D->precision == NEWT_PRECISION_MIXED, M->dev.nv, grid, block, D->lds_bytes, stream, M->dev,
-                        D->TL, M->cache, (const int4*)M->d_blob, D->L, D->d, D->dws, D->iws, D->ovf, n_steps, wpb));
+    HIP_TRY(team_launch(D->precision == NEWT_PRECISION_MIXED, M->dev.nv, M->dev.ext != 0, grid, block, D->lds_bytes,
+                        stream, M->dev, D->TL, M->cache, (const int4*)M->d_blob, D->L, D->d, D->dws, D->iws, D->ovf,
+                        n_steps, wpb));
     return NEWT_OK;
 }
 
@@ -328,7 +329,7 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
     if (precision != NEWT_PRECISION_FP64_EXACT && D->lds_bytes > 65536) {
         // The attribute is per kernel, shared by every newt_data: allow the
         // whole workgroup LDS so that no data narrows another's launch.
-        hipError_t e = team_set_lds(precision == NEWT_PRECISION_MIXED, model->dev.nv, kMaxLds);
+        hipError_t e = team_set_lds(precision == NEWT_PRECISION_MIXED, model->dev.nv, model->dev.ext != 0, kMaxLds);
         if (e != hipSuccess) {
             delete D;
             return fail(NEWT_ERR_DEVICE, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));

@@ -1,58 +1,62 @@
-// newt_team.hip — instantiations of the persistent wavefront-per-world step
-// (newt_team.hpp) for register-array widths 24 and 32, and the launcher that
-// dispatches on the model's dof count.  Width 64 is instantiated in
-// newt_team64f.hip / newt_team64d.hip (compiled in parallel).
+// newt_team.hip — the launcher of the persistent wavefront-per-world step
+// (newt_team.hpp), dispatching on precision, register-array width (24, 32
+// or 64 dofs) and the extended-step flag, plus the width-24 plain-step
+// instantiations.  The other instantiations live in newt_team24x.hip,
+// newt_team32.hip and newt_team64{f,fx,d,dx}.hip so the units compile in
+// parallel.
 #include <hip/hip_runtime.h>
 
 #include "newt_team.hpp"
 
 namespace newt {
 
-template <class R, int NV>
-hipError_t launch_nv(dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const ModelT& m, const TeamLayout& T,
-                     const CacheLayout& C, const int4* blob, const Layout& L, const DataT& d, double* dws, int* iws,
-                     char* ovf, int n_steps, int wpb) {
-    hipLaunchKernelGGL((k_team<R, NV>), grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps,
-                       wpb);
-    return hipGetLastError();
-}
-
-template <class R, int NV>
-hipError_t set_lds_nv(int bytes) {
-    return hipFuncSetAttribute((const void*)k_team<R, NV>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-}
-
-// Width 64, in their own translation units.
-hipError_t launch64_64f(dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const ModelT& m, const TeamLayout& T,
-                        const CacheLayout& C, const int4* blob, const Layout& L, const DataT& d, double* dws, int* iws,
-                        char* ovf, int n_steps, int wpb);
-hipError_t launch64_64d(dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const ModelT& m, const TeamLayout& T,
-                        const CacheLayout& C, const int4* blob, const Layout& L, const DataT& d, double* dws, int* iws,
-                        char* ovf, int n_steps, int wpb);
-hipError_t set_lds64_64f(int bytes);
-hipError_t set_lds64_64d(int bytes);
+NEWT_TEAM_INSTANCE(float, 24, false)
+NEWT_TEAM_INSTANCE(double, 24, false)
+NEWT_TEAM_EXTERN(float, 24, true)
+NEWT_TEAM_EXTERN(double, 24, true)
+NEWT_TEAM_EXTERN(float, 32, false)
+NEWT_TEAM_EXTERN(double, 32, false)
+NEWT_TEAM_EXTERN(float, 32, true)
+NEWT_TEAM_EXTERN(double, 32, true)
+NEWT_TEAM_EXTERN(float, 64, false)
+NEWT_TEAM_EXTERN(double, 64, false)
+NEWT_TEAM_EXTERN(float, 64, true)
+NEWT_TEAM_EXTERN(double, 64, true)
 
 int team_max_worlds_for(int nv) {
     return nv <= 24 ? team_max_worlds<24>() : nv <= 32 ? team_max_worlds<32>() : team_max_worlds<64>();
 }
 
-hipError_t team_launch(bool mixed, int nv, dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const ModelT& m,
-                       const TeamLayout& T, const CacheLayout& C, const int4* blob, const Layout& L, const DataT& d,
-                       double* dws, int* iws, char* ovf, int n_steps, int wpb) {
-    if (nv > 32)
-        return mixed ? launch64_64f(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb)
-                     : launch64_64d(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb);
+namespace {
+template <class R, bool X>
+hipError_t launch_r(int nv, dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const ModelT& m,
+                    const TeamLayout& T, const CacheLayout& C, const int4* blob, const Layout& L, const DataT& d,
+                    double* dws, int* iws, char* ovf, int n_steps, int wpb) {
+    if (nv <= 24) return launch_team<R, 24, X>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb);
+    if (nv <= 32) return launch_team<R, 32, X>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb);
+    return launch_team<R, 64, X>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb);
+}
+template <class R, bool X>
+hipError_t set_lds_r(int nv, int bytes) {
+    if (nv <= 24) return set_team_lds<R, 24, X>(bytes);
+    if (nv <= 32) return set_team_lds<R, 32, X>(bytes);
+    return set_team_lds<R, 64, X>(bytes);
+}
+}  // namespace
+
+hipError_t team_launch(bool mixed, int nv, bool ext, dim3 grid, dim3 block, int lds_bytes, hipStream_t stream,
+                       const ModelT& m, const TeamLayout& T, const CacheLayout& C, const int4* blob, const Layout& L,
+                       const DataT& d, double* dws, int* iws, char* ovf, int n_steps, int wpb) {
     if (mixed)
-        return nv <= 24 ? launch_nv<float, 24>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb)
-                        : launch_nv<float, 32>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb);
-    return nv <= 24 ? launch_nv<double, 24>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb)
-                    : launch_nv<double, 32>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb);
+        return ext ? launch_r<float, true>(nv, grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb)
+                   : launch_r<float, false>(nv, grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb);
+    return ext ? launch_r<double, true>(nv, grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb)
+               : launch_r<double, false>(nv, grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb);
 }
 
-hipError_t team_set_lds(bool mixed, int nv, int bytes) {
-    if (nv > 32) return mixed ? set_lds64_64f(bytes) : set_lds64_64d(bytes);
-    if (mixed) return nv <= 24 ? set_lds_nv<float, 24>(bytes) : set_lds_nv<float, 32>(bytes);
-    return nv <= 24 ? set_lds_nv<double, 24>(bytes) : set_lds_nv<double, 32>(bytes);
+hipError_t team_set_lds(bool mixed, int nv, bool ext, int bytes) {
+    if (mixed) return ext ? set_lds_r<float, true>(nv, bytes) : set_lds_r<float, false>(nv, bytes);
+    return ext ? set_lds_r<double, true>(nv, bytes) : set_lds_r<double, false>(nv, bytes);
 }
 
 }  // namespace newt

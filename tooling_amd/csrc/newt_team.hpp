@@ -237,7 +237,10 @@ struct Tables {
     const LimitT* limits;
 };
 
-template <class R, int NV>
+// X: the extended step (joint-limit rows, RK4, sphere/capsule and box-box
+// pairs).  Models without them run the X = false instantiation, whose
+// register allocation is that of the plane-contact Euler step alone.
+template <class R, int NV, bool X>
 struct Team {
     const ModelT& m;      // scalars of the model (kernel argument)
     const TeamLayout& T;  // kernel argument
@@ -459,6 +462,7 @@ struct Team {
         double fr = ga.friction < gb.friction ? ga.friction : gb.friction;
         double margin = ga.margin > gb.margin ? ga.margin : gb.margin;
         double gap = ga.gap > gb.gap ? ga.gap : gb.gap;
+        if constexpr (X) {
         if (ga.shape == SH_BOX && gb.shape == SH_BOX) {
             V3<double> pa, pb, nrm, cp[4];
             Qt<double> qa, qb;
@@ -510,6 +514,7 @@ struct Team {
                 g[8] = gap;
             }
             return 1;
+        }
         }
         // (primitive, plane) as given, else swapped with the normal negated
         // (contact.rs:2579-2588).
@@ -1568,13 +1573,14 @@ struct Team {
         const int nc = contacts();
         if (nc > m.cap_c) return 8;  // NEWT_ERR_CAPACITY
         const int na = m.mode == 1 ? active_contacts(nc) : 0;
-        const bool rk4_ = m.integrator == 2;
+        const bool rk4_ = X && m.integrator == 2;
         store_xups();
         for (int s = lane; s < m.nv; s += 64) sd[T.tau + s] = sd[T.qfrc + s];
         wsync();
         // Limit rows at the start state (step_trees_euler / step_trees,
         // world.rs:1826-2051); PGS mode only.
-        int nl = m.n_ranged > 0 && m.mode == 1 ? limit_rows() : 0;
+        int nl = 0;
+        if constexpr (X) nl = m.n_ranged > 0 && m.mode == 1 ? limit_rows() : 0;
         if (na > 0 || nl > 0) {
             mass_matrix();
             double la[NV], lrinv;
@@ -1584,7 +1590,8 @@ struct Team {
                     nl = 0;                // no limit force (solver.rs:2529-2532)
                 } else {
                     if (na > 0) invweights(T.Ld, T.Ldr);
-                    if (nl > 0) limit_solve(nl, T.Ld, T.Ldr);
+                    if constexpr (X)
+                        if (nl > 0) limit_solve(nl, T.Ld, T.Ldr);
                 }
                 add_damping();
             }
@@ -1593,7 +1600,8 @@ struct Team {
                 nl = 0;
             } else if (!m.damped) {
                 if (na > 0) invweights_regs(la, lrinv);
-                if (nl > 0) limit_solve(nl, T.Li, T.Lr);
+                if constexpr (X)
+                    if (nl > 0) limit_solve(nl, T.Li, T.Lr);
             }
             if (na > 0) {
                 // free acceleration: (M + dt·D) q̈ = τ − D q̇ − C
@@ -1612,9 +1620,9 @@ struct Team {
                     pgs_large(4 * na, ov);
                 }
                 if (rk4_) {
-                    add_contact_force(nl > 0);
+                    if constexpr (X) add_contact_force(nl > 0);
                 } else {
-                    if (nl > 0) {
+                    if (X && nl > 0) {
                         // u += L⁻¹ τ_limit: q̈ = q̈_free + (M + dt·D)⁻¹(Jᵀf/dt + τ_limit)
                         const double x = fwd_vec(lane < m.nv ? sd[T.qlim + lane] : 0.0, la, lrinv);
                         if (lane < m.nv) sd[T.u + lane] = sd[T.u + lane] + x;
@@ -1632,10 +1640,14 @@ struct Team {
         } else if (!rk4_) {
             aba(T.qacc, true);
         }
-        if (rk4_)
-            rk4();
-        else
+        if constexpr (X) {
+            if (rk4_)
+                rk4();
+            else
+                integrate();
+        } else {
             integrate();
+        }
         kinematics();
         populate(false);
         *np = __builtin_amdgcn_readfirstlane(si[T.misc]);
@@ -1653,7 +1665,7 @@ template <int NV> constexpr int team_max_worlds() { return NV <= 24 ? 8 : 4; }
 // each world's state follows at C.bytes + wave*T.bytes.  Global state is
 // lane-interleaved ([element][W]).  At NV <= 24 the register budget is held
 // to 256 so two wavefronts fit a SIMD (eight per CU).
-template <class R, int NV>
+template <class R, int NV, bool X>
 __global__ void __launch_bounds__(64 * team_max_worlds<NV>())
     __attribute__((amdgpu_waves_per_eu(NV <= 24 ? 2 : 1)))
     k_team(ModelT m, TeamLayout T, CacheLayout C, const int4* __restrict__ blob, Layout L, DataT d,
@@ -1678,7 +1690,7 @@ __global__ void __launch_bounds__(64 * team_max_worlds<NV>())
     const int cr = m.cap_r;
     const RowStore<R> ov{(R*)(ovf + (size_t)w * T.ov_stride), sd + T.lg, sd + T.lg + cr, sd + T.lg + 2 * cr,
                          sd + T.lg + 3 * cr};
-    Team<R, NV> t{m, T, tb, lane, sd, si, sJ, ov};
+    Team<R, NV, X> t{m, T, tb, lane, sd, si, sJ, ov};
     for (int k = lane; k < m.nq; k += 64) sd[T.q + k] = d.qpos[(size_t)k * W + w];
     for (int k = lane; k < m.nv; k += 64) {
         sd[T.qd + k] = d.qvel[(size_t)k * W + w];
@@ -1715,5 +1727,32 @@ __global__ void __launch_bounds__(64 * team_max_worlds<NV>())
         if (fault) d.fault[w] = fault;
     }
 }
+
+// Launch / LDS-limit helpers of one instantiation.  Each is instantiated in
+// one of the newt_team*.hip units (so the units compile in parallel) and
+// declared extern below for the dispatcher.
+template <class R, int NV, bool X>
+hipError_t launch_team(dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const ModelT& m, const TeamLayout& T,
+                       const CacheLayout& C, const int4* blob, const Layout& L, const DataT& d, double* dws, int* iws,
+                       char* ovf, int n_steps, int wpb) {
+    hipLaunchKernelGGL((k_team<R, NV, X>), grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps,
+                       wpb);
+    return hipGetLastError();
+}
+template <class R, int NV, bool X>
+hipError_t set_team_lds(int bytes) {
+    return hipFuncSetAttribute((const void*)k_team<R, NV, X>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+#define NEWT_TEAM_EXTERN(R, NV, X)                                                                                   \
+    extern template hipError_t launch_team<R, NV, X>(dim3, dim3, int, hipStream_t, const ModelT&, const TeamLayout&,  \
+                                                     const CacheLayout&, const int4*, const Layout&, const DataT&,    \
+                                                     double*, int*, char*, int, int);                                 \
+    extern template hipError_t set_team_lds<R, NV, X>(int);
+#define NEWT_TEAM_INSTANCE(R, NV, X)                                                                          \
+    template hipError_t launch_team<R, NV, X>(dim3, dim3, int, hipStream_t, const ModelT&, const TeamLayout&, \
+                                              const CacheLayout&, const int4*, const Layout&, const DataT&,   \
+                                              double*, int*, char*, int, int);                                \
+    template hipError_t set_team_lds<R, NV, X>(int);
 
 }  // namespace newt

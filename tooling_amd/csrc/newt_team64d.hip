@@ -1,21 +1,10 @@
-// newt_team64d.hip — the persistent step at register-array width 64 (double
-// Delassus rows): models with 33..64 dofs.
+// newt_team64d.hip — the persistent step at width 64, FP64 Delassus rows, plain step.
 #include <hip/hip_runtime.h>
 
 #include "newt_team.hpp"
 
 namespace newt {
 
-hipError_t launch64_64d(dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const ModelT& m, const TeamLayout& T,
-                       const CacheLayout& C, const int4* blob, const Layout& L, const DataT& d, double* dws, int* iws,
-                       char* ovf, int n_steps, int wpb) {
-    hipLaunchKernelGGL((k_team<double, 64>), grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps,
-                       wpb);
-    return hipGetLastError();
-}
-
-hipError_t set_lds64_64d(int bytes) {
-    return hipFuncSetAttribute((const void*)k_team<double, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-}
+NEWT_TEAM_INSTANCE(double, 64, false)
 
 }  // namespace newt

@@ -72,6 +72,8 @@ struct ModelT {
     int damped;    // some joint has damping or free_damping != 0 (Euler)
     int n_ranged;  // hinges/slides with a range
     const LimitT* limits;
+    int ext;       // needs the extended persistent step: limits, RK4 or non-plane pairs
+    int pad1;
 };
 
 // Offsets (in elements) of the per-world scratch regions.
