@@ -58,6 +58,7 @@ struct TeamLayout {
     int qlim, lsg, lvi;           // joint limits: force [nv]; row coefficient, violation
     int ylim;                     // union, solve side: rows of L⁻¹ Eᵀ, FP64, n_ranged x nv
     int lsl, llm;                 // ints: row dof slot, limit index
+    int lflag, lused;             // ints: links touched by active contacts (flags, list)
     int bytes;                    // per world, multiple of 16
     // Global block per world holding the rows of contact sets of more than
     // 64 rows (cap_r x nv of R; stride 0 when the model cannot produce them).
@@ -133,6 +134,8 @@ inline TeamLayout make_team_layout(const ModelT& m, int real_bytes) {
     T.misc = I(8);
     T.lsl = I(nrl);
     T.llm = I(nrl);
+    T.lflag = I(n);
+    T.lused = I(n);
     T.bytes = (T.i_off + 4 * i + 15) / 16 * 16;
     if (cr > 64) T.ov_stride = (real_bytes * cr * (nv > 0 ? nv : 1) + 255) / 256 * 256;
     return T;
@@ -613,6 +616,31 @@ struct Team {
         return na;
     }
 
+    // Links that carry a geom of an active contact, in link order, into
+    // si[T.lused..]; returns their count.  The rows read the inverse weights
+    // of these links only.
+    __device__ __forceinline__ int contact_links(int na) const {
+        for (int i = lane; i < m.n_links; i += 64) si[T.lflag + i] = 0;
+        wsync();
+        for (int k = lane; k < na; k += 64) {
+            const int c = si[T.alist + k];
+            const int la = tb.geoms[si[T.cids + 3 * c]].link, lb = tb.geoms[si[T.cids + 3 * c + 1]].link;
+            if (la >= 0) si[T.lflag + la] = 1;
+            if (lb >= 0) si[T.lflag + lb] = 1;
+        }
+        wsync();
+        int nu = 0;
+        for (int i0 = 0; i0 < m.n_links; i0 += 64) {
+            const int i = i0 + lane;
+            const bool used = i < m.n_links && si[T.lflag + i] != 0;
+            const unsigned long long bal = __ballot(used);
+            if (used) si[T.lused + nu + __popcll(bal & lanes_below(lane))] = i;
+            nu += __popcll(bal);
+        }
+        wsync();
+        return __builtin_amdgcn_readfirstlane(nu);
+    }
+
     // ---------------- dynamics ---------------------------------------------
     // Parent-to-child Plücker transform (aba.rs:762-791).
     __device__ __forceinline__ Xf<double> joint_xup(int i) const {
@@ -1049,13 +1077,13 @@ struct Team {
     // every link: trace(J M⁻¹ Jᵀ)/3 at the link origin.  With M = L Lᵀ the
     // trace is Σ_axis ‖L⁻¹ J_axisᵀ‖², so one forward substitution per axis
     // suffices.  One lane per (link, axis); L in sd[ld], 1/L_kk in sd[ldr].
-    __device__ __forceinline__ void invweights(int ld, int ldr) const {
-        const int n = m.n_links, nv = m.nv;
+    __device__ __forceinline__ void invweights(int ld, int ldr, int nu) const {
+        const int n = nu, nv = m.nv;
         // Passes of 63 lanes keep each link's three axes in one pass.
         for (int base = 0; base < 3 * n; base += 63) {
             const int idx = base + lane;
             const bool on = lane < 63 && idx < 3 * n;
-            const int i = on ? idx / 3 : 0, ax = idx % 3;
+            const int i = on ? si[T.lused + idx / 3] : 0, ax = idx % 3;
             const unsigned long long anc0 = tb.links[i].anc[0], anc1 = tb.links[i].anc[1];
             const V3<double> point = pos(i) + qrot(ori(i), z3<double>());
             const V3<double> dir = mk3(ax == 0 ? 1.0 : 0.0, ax == 1 ? 1.0 : 0.0, ax == 2 ? 1.0 : 0.0);
@@ -1100,13 +1128,13 @@ struct Team {
     }
 
     // invweights() with the factor in registers.
-    __device__ __forceinline__ void invweights_regs(const double (&la)[NV], double lrinv) const {
-        const int n = m.n_links, nv = m.nv;
+    __device__ __forceinline__ void invweights_regs(const double (&la)[NV], double lrinv, int nu) const {
+        const int n = nu, nv = m.nv;
         // Passes of 63 lanes keep each link's three axes in one pass.
         for (int base = 0; base < 3 * n; base += 63) {
             const int idx = base + lane;
             const bool on = lane < 63 && idx < 3 * n;
-            const int i = on ? idx / 3 : 0, ax = idx % 3;
+            const int i = on ? si[T.lused + idx / 3] : 0, ax = idx % 3;
             const unsigned long long anc0 = tb.links[i].anc[0], anc1 = tb.links[i].anc[1];
             const V3<double> point = pos(i) + qrot(ori(i), z3<double>());
             const V3<double> dir = mk3(ax == 0 ? 1.0 : 0.0, ax == 1 ? 1.0 : 0.0, ax == 2 ? 1.0 : 0.0);
@@ -1589,7 +1617,7 @@ struct Team {
                     if (na > 0) return 3;  // NEWT_ERR_NOT_PD
                     nl = 0;                // no limit force (solver.rs:2529-2532)
                 } else {
-                    if (na > 0) invweights(T.Ld, T.Ldr);
+                    if (na > 0) invweights(T.Ld, T.Ldr, contact_links(na));
                     if constexpr (X)
                         if (nl > 0) limit_solve(nl, T.Ld, T.Ldr);
                 }
@@ -1599,7 +1627,7 @@ struct Team {
                 if (na > 0) return 3;
                 nl = 0;
             } else if (!m.damped) {
-                if (na > 0) invweights_regs(la, lrinv);
+                if (na > 0) invweights_regs(la, lrinv, contact_links(na));
                 if constexpr (X)
                     if (nl > 0) limit_solve(nl, T.Li, T.Lr);
             }
