@@ -278,11 +278,12 @@ def bench_serve(args):
     keys = sorted({r.key for r in reqs})
     total = sum(r.n_worlds * r.horizon for r in reqs)
     log(f"bench serve: {len(reqs)} requests over {len(keys)} models at {args.rate}/s, {total} world-steps, "
-        f"chunk {args.serve_chunk}")
+        f"chunk <= {args.serve_chunk}, quantum {args.serve_quantum_ms} ms")
     pools = {k: (serve.scene_for(k), 2048 if k.startswith("chain") else 8192) for k in keys}
     runs = {}
     for name, precision in (("mixed", abi.PRECISION_MIXED), ("fp64", abi.PRECISION_FP64)):
-        srv = serve.Server(pools, precision=precision, chunk=args.serve_chunk, n_streams=4)
+        srv = serve.Server(pools, precision=precision, chunk=args.serve_chunk, n_streams=4,
+                           quantum=args.serve_quantum_ms / 1e3 if args.serve_quantum_ms > 0 else None)
         try:
             # Load every kernel once before the clock starts.
             srv.run([serve.Request(k, r.qpos[:1], r.qvel[:1], 1) for k in keys
@@ -318,7 +319,8 @@ def bench_serve(args):
         "config": {"workload": f"serve: {len(reqs)} rollout requests, Poisson {args.rate}/s, model ~ U{{pendulum3, "
                                f"quadruped18, chain2..64}}, worlds ~ U{{1..512}}, horizon ~ U{{100..2000}}, "
                                f"seed {args.seed}",
-                   "models": len(keys), "world_steps": total, "steps_per_launch": args.serve_chunk,
+                   "models": len(keys), "world_steps": total, "max_steps_per_launch": args.serve_chunk,
+                   "launch_quantum_ms": args.serve_quantum_ms,
                    "streams": 4, "parallelism": "1 GPU, continuous batching: one resident pool per model"},
         "latency_ms": m["latency_ms"], "latency_p99_ms_by_family": m["latency_p99_ms_by_family"],
         "launches": m["launches"], "device_bytes": m["device_bytes"], "faults": {"mixed": m["faults"],
@@ -345,7 +347,9 @@ def main():
                          "requests served by continuous batching")
     ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
-    ap.add_argument("--serve-chunk", type=int, default=50, help="serve: steps per launch")
+    ap.add_argument("--serve-chunk", type=int, default=400, help="serve: most steps per launch")
+    ap.add_argument("--serve-quantum-ms", type=float, default=50.0,
+                    help="serve: target launch length; 0 runs every launch at --serve-chunk steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 5 times (reference kernel for the HBM counters)")
     args = ap.parse_args()

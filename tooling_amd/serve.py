@@ -16,7 +16,15 @@ memory:
   the pool's chunk size and the shortest remaining horizon among its
   requests, so each request leaves exactly at its horizon.
 * Launches go to ``n_streams`` streams with one chunk in flight per stream.
-  An idle stream takes the idle pool whose oldest request arrived first.
+  Pools not yet measured go first.  Otherwise stream 0 takes the idle pool
+  that will finish a request soonest: the shortest remaining horizon among
+  its requests times the pool's measured time per step.  This keeps short
+  rollouts' latency low.  The other streams take the idle pool with the
+  most remaining work, so the longest rollouts, which bound the tail
+  latency, run from the start.
+* Chunks are sized to about ``quantum`` seconds of the pool's step time.
+  Cheap models (the pendulum) run long chunks and a 64-link chain short
+  ones, so no pool holds a stream for much longer than ``quantum``.
 * A finished request's worlds are copied out, then its slots are parked and
   freed.
 
@@ -113,12 +121,23 @@ class Pool:
         self.active: List[list] = []   # [request, first slot, remaining steps]
         self.to_park: List[tuple] = []  # released ranges not yet parked on the device
         self.busy = False
+        self.step_s: Optional[float] = None  # measured seconds per step of one launch (moving average)
 
-    def oldest(self) -> float:
-        t = [r.arrival for r, _, _ in self.active]
+    def remaining_work(self) -> float:
+        """Estimated seconds of stepping left for this pool's requests."""
+        rem = [a[2] for a in self.active] + [r.horizon for r in self.queue]
+        if not rem:
+            return 0.0
+        return float("inf") if self.step_s is None else max(rem) * self.step_s
+
+    def next_finish(self) -> float:
+        """Estimated seconds until this pool's next request finishes."""
+        rem = [a[2] for a in self.active]
         if self.queue:
-            t.append(self.queue[0].arrival)
-        return min(t) if t else float("inf")
+            rem.append(self.queue[0].horizon)
+        if not rem:
+            return float("inf")
+        return 0.0 if self.step_s is None else min(rem) * self.step_s
 
     def close(self):
         self.data.close()
@@ -131,6 +150,7 @@ class _Lane:
     event: object
     pool: Optional[Pool] = None
     k: int = 0
+    t0: float = 0.0
 
 
 class Server:
@@ -138,10 +158,13 @@ class Server:
 
     ``pools``: key -> (scene, capacity in worlds).  ``hbm_budget``: device
     bytes the pools may allocate together; start-up fails if they need more.
+    ``chunk``: steps per launch, or with ``quantum`` (seconds) the largest
+    chunk; a pool's chunk is then sized to about ``quantum`` of its measured
+    step time.
     """
 
     def __init__(self, pools: Dict[str, tuple], precision: int = abi.PRECISION_MIXED, chunk: int = 50,
-                 n_streams: int = 4, hbm_budget: int = 200 << 30):
+                 n_streams: int = 4, hbm_budget: int = 200 << 30, quantum: Optional[float] = None):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("the serving loop needs a GPU")
@@ -149,6 +172,7 @@ class Server:
             raise ValueError("chunk must be at least 1")
         self.precision = precision
         self.chunk = chunk
+        self.quantum = quantum
         self.hbm_budget = hbm_budget
         self.pools: Dict[str, Pool] = {}
         self.device_bytes = 0
@@ -179,6 +203,8 @@ class Server:
         p = lane.pool
         lane.pool = None
         p.busy = False
+        per_step = (now() - lane.t0) / lane.k
+        p.step_s = per_step if p.step_s is None else 0.7 * p.step_s + 0.3 * per_step
         done = []
         for a in p.active:
             a[2] -= lane.k
@@ -200,11 +226,13 @@ class Server:
         return len(done)
 
     def _dispatch(self, lane: _Lane, now: float) -> bool:
+        short = lane is self.lanes[0]
+        key = (lambda q: q.next_finish()) if short else (lambda q: -q.remaining_work())
         best = None
         for p in self.pools.values():
             if p.busy or not (p.active or (p.queue and p.slots.fits(p.queue[0].n_worlds))):
                 continue
-            if best is None or p.oldest() < best.oldest():
+            if best is None or key(p) < key(best):
                 best = p
         if best is None:
             return False
@@ -218,10 +246,13 @@ class Server:
             if first < 0:
                 break
             self._admit(p, p.queue.popleft(), first, s, now)
-        k = min(self.chunk, min(a[2] for a in p.active))
+        k = self.chunk
+        if self.quantum is not None and p.step_s:
+            k = max(1, min(self.chunk, int(self.quantum / p.step_s)))
+        k = min(k, min(a[2] for a in p.active))
         step(p.model, p.data, k, s)
         lane.event.record(lane.stream)
-        lane.pool, lane.k = p, k
+        lane.pool, lane.k, lane.t0 = p, k, now
         p.busy = True
         self.launches += 1
         self.world_steps += k * sum(a[0].n_worlds for a in p.active)
