@@ -134,6 +134,24 @@ typedef struct newt_link_desc {
     double limit_solimp[4];
 } newt_link_desc;
 
+/* One free Body (newt/src/body.rs:22-47): inertia about the COM in body
+ * axes (column-major, inverted on the host as Body::new does, body.rs:53-70),
+ * the initial state every world starts from, the optional speed caps and
+ * the gravity scale.  rolling_friction is not modelled (None). */
+typedef struct newt_body_desc {
+    double mass;
+    double inertia[9];
+    double position[3];              /* world COM */
+    double orientation[4];           /* body -> world (x, y, z, w) */
+    double linear_velocity[3];       /* world frame */
+    double angular_velocity_body[3]; /* body frame */
+    int32_t has_max_linear_velocity;
+    int32_t has_max_angular_velocity;
+    double max_linear_velocity;
+    double max_angular_velocity;
+    double gravity_scale;            /* 1.0 in Body::new */
+} newt_body_desc;
+
 /* One Geom (newt/src/geom.rs:352-419). */
 typedef struct newt_geom_desc {
     int32_t shape;           /* newt_shape_kind */
@@ -151,7 +169,7 @@ typedef struct newt_geom_desc {
     int32_t solimp_power;    /* SolImp::power */
     uint32_t group;          /* collision_group */
     uint32_t mask;           /* collision_mask */
-    int32_t pad0;
+    int32_t body;            /* Geom::body: attached free body, -1 for none */
 } newt_geom_desc;
 
 typedef struct newt_model newt_model;
@@ -171,6 +189,11 @@ newt_status newt_model_add_tree(newt_model* model, const newt_link_desc* links,
                                 int32_t n_links, int32_t self_collide,
                                 int32_t* tree_index);
 
+/* World::add_body (newt/src/world.rs:950-954).  A model holds trees or
+ * free bodies, not both. */
+newt_status newt_model_add_body(newt_model* model, const newt_body_desc* body,
+                                int32_t* body_index);
+
 /* World::add_geom (newt/src/world.rs:974). */
 newt_status newt_model_add_geom(newt_model* model, const newt_geom_desc* geom,
                                 int32_t* geom_index);
@@ -179,7 +202,10 @@ newt_status newt_model_add_geom(newt_model* model, const newt_geom_desc* geom,
  * model into device tables.  No further add_* calls are accepted. */
 newt_status newt_model_finalize(newt_model* model);
 
-/* Sizes of one world: nq, nv, link count, geom count. */
+/* Sizes of one world: nq, nv, link count, geom count.  Free body i holds
+ * qpos[7i..7i+7) = (position, orientation x y z w) and qvel[6i..6i+6) =
+ * (linear velocity in world, angular velocity in body frame), after the
+ * tree's coordinates. */
 newt_status newt_model_dims(const newt_model* model, int32_t* nq, int32_t* nv,
                             int32_t* n_links, int32_t* n_geoms);
 

@@ -228,6 +228,111 @@ std::vector<uint8_t> solver_tree() {
     return out;
 }
 
+// Body state as the reference's golden tests serialize it: position,
+// orientation (x, y, z, w), linear velocity, body angular velocity.
+void snap_bodies(std::vector<uint8_t>& out, const World<F>& w) {
+    for (const Body<F>& b : w.bodies) {
+        const F v[13] = {b.p.x, b.p.y, b.p.z, b.q.x, b.q.y, b.q.z, b.q.w, b.v.x, b.v.y, b.v.z, b.w.x, b.w.y, b.w.z};
+        for (F x : v) put(out, x);
+    }
+}
+Body<F> solid_box(F mass, V3<F> h, V3<F> p, Q<F> q) { return make_body(mass, solid_box_inertia(mass, h), p, q); }
+Geom<F> box_on_body(int body, V3<F> he, F friction) {  // Geom::r#box (geom.rs:575-600)
+    Geom<F> g;
+    g.shape = S_BOX;
+    g.body = body;
+    g.size = he;
+    g.friction = friction;
+    return g;
+}
+std::vector<uint8_t> record_at(World<F>& w, std::initializer_list<int> marks) {
+    std::vector<uint8_t> out;
+    snap_bodies(out, w);
+    int done = 0;
+    for (int m : marks) {
+        for (; done < m; ++done) w.step();
+        snap_bodies(out, w);
+    }
+    return out;
+}
+
+// newt/tests/golden.rs:18-93: three tumbling boxes, RK4, no contacts.
+std::vector<uint8_t> tumbling_3_body() {
+    World<F> w;
+    w.dt = 0.005f;
+    w.gravity = v3(0.0f, 0.0f, -9.81f);
+    Body<F> b0 = solid_box(1.0f, v3(0.4f, 0.3f, 0.2f), v3(0.0f, 0.0f, 5.0f), qidentity<F>());
+    b0.v = v3(1.0f, 0.5f, 3.0f);
+    b0.w = v3(1.0f, 2.0f, 0.5f);
+    Body<F> b1 = solid_box(2.0f, v3(0.3f, 0.3f, 0.3f), v3(1.5f, -1.0f, 4.0f), from_axis_angle(v3(1.0f, 0.0f, 0.0f), 0.3f));
+    b1.v = v3(-0.5f, 1.5f, 2.0f);
+    b1.w = v3(0.2f, 3.5f, 0.1f);
+    Body<F> b2 = solid_box(0.5f, v3(0.5f, 0.2f, 0.1f), v3(-1.0f, 2.0f, 6.0f), from_axis_angle(v3(0.0f, 1.0f, 1.0f), 0.7f));
+    b2.v = v3(2.0f, -1.0f, 0.5f);
+    b2.w = v3(0.05f, 0.1f, 4.0f);
+    w.add_body(b0);
+    w.add_body(b1);
+    w.add_body(b2);
+    return record_at(w, {100, 1000});
+}
+
+// newt/tests/integrators_golden.rs:10-63: one body per integrator.
+std::vector<uint8_t> integrator_golden(int integrator) {
+    World<F> w;
+    w.integrator = integrator;
+    w.dt = 0.005f;
+    w.gravity = v3(0.0f, 0.0f, -9.81f);
+    Body<F> b = make_body(1.25f, m3diag(0.7f, 1.3f, 2.1f), v3(0.4f, -0.7f, 2.0f),
+                          from_axis_angle(v3(1.0f, 0.2f, -0.4f), 0.6f));
+    b.v = v3(0.8f, -0.4f, 1.2f);
+    b.w = v3(0.4f, 1.1f, -0.3f);
+    w.add_body(b);
+    return record_at(w, {20, 100});
+}
+
+// newt/tests/solver_golden.rs:27-79 + record (170-190): PGS 30, pyramidal, RK4.
+std::vector<uint8_t> solver_stack() {
+    World<F> w;
+    w.dt = 0.005f;
+    w.gravity = v3(0.0f, 0.0f, -9.81f);
+    w.mode = M_PGS;
+    w.iterations = 30;
+    w.cone = C_PYRAMIDAL;
+    SolRef<F> sr{0.05f, 1.5f};
+    Geom<F> plane = static_plane(0.8f);
+    plane.solref = sr;
+    w.add_geom(plane);
+    V3<F> half{0.25f, 0.25f, 0.25f};
+    for (int k = 0; k < 3; ++k) {
+        V3<F> off = v3(0.01f * (F)k, 0.0f, 0.25f + 0.5f * (F)k);
+        int idx = w.add_body(solid_box(1.0f, half, off, qidentity<F>()));
+        Geom<F> g = box_on_body(idx, half, 0.8f);
+        g.solref = sr;
+        w.add_geom(g);
+    }
+    return record_at(w, {50, 200, 1000});
+}
+std::vector<uint8_t> solver_incline() {
+    World<F> w;
+    w.dt = 0.005f;
+    F angle = 32.0f * 3.14159265358979323846f / 180.0f;
+    F g = 9.81f;
+    w.gravity = v3(g * nsin(angle), 0.0f, -g * ncos(angle));
+    w.mode = M_PGS;
+    w.iterations = 30;
+    w.cone = C_PYRAMIDAL;
+    SolRef<F> sr{0.05f, 1.5f};
+    Geom<F> plane = static_plane(0.5f);
+    plane.solref = sr;
+    w.add_geom(plane);
+    V3<F> half{0.2f, 0.21f, 0.19f};
+    int bi = w.add_body(solid_box(1.0f, half, v3(0.01f, 0.0f, 0.2f), qidentity<F>()));
+    Geom<F> bg = box_on_body(bi, half, 0.5f);
+    bg.solref = sr;
+    w.add_geom(bg);
+    return record_at(w, {50, 200, 1000});
+}
+
 }  // namespace
 
 extern "C" int oracle_golden_bytes(const char* name, uint8_t* out, int cap) {
@@ -237,6 +342,12 @@ extern "C" int oracle_golden_bytes(const char* name, uint8_t* out, int cap) {
     else if (n == "joints_mixed") b = joints_mixed();
     else if (n == "dynamics_mass_bias") b = dynamics_mass_bias();
     else if (n == "solver_tree") b = solver_tree();
+    else if (n == "tumbling_3_body") b = tumbling_3_body();
+    else if (n == "integrator_euler") b = integrator_golden(I_EULER);
+    else if (n == "integrator_rk4") b = integrator_golden(I_RK4);
+    else if (n == "integrator_implicitfast") b = integrator_golden(I_IMPLICITFAST);
+    else if (n == "solver_stack") b = solver_stack();
+    else if (n == "solver_incline") b = solver_incline();
     else return -1;
     if ((int)b.size() > cap) return -2;
     std::memcpy(out, b.data(), b.size());

@@ -547,6 +547,7 @@ template <class T> struct SolImp {  // solver.rs:187-193
 template <class T> struct Geom {
     int shape = S_PLANE;
     int tree = -1, link = -1;  // -1: static
+    int body = -1;             // free body it rides on (Geom::body), -1: none
     V3<T> size = vzero<T>();   // sphere r; box half extents; capsule (r, half_height)
     V3<T> pos = vzero<T>();
     Q<T> quat = qidentity<T>();
@@ -556,6 +557,7 @@ template <class T> struct Geom {
     int condim = 3;
     uint32_t group = 0xffffffffu, mask = 0xffffffffu;
     bool is_link() const { return tree >= 0; }
+    bool is_body() const { return body >= 0; }
 };
 
 template <class T> struct Pose {
@@ -565,7 +567,7 @@ template <class T> struct Pose {
 
 // geom_world_pose (geom.rs:887-898).
 template <class T> inline Pose<T> geom_world_pose(const Geom<T>& g, V3<T> pp, Q<T> pq) {
-    if (!g.is_link()) return {g.pos, g.quat};
+    if (!g.is_link() && !g.is_body()) return {g.pos, g.quat};
     return {pp + rotate(pq, g.pos), pq * g.quat};
 }
 

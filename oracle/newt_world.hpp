@@ -497,6 +497,340 @@ inline bool pair_supported(int a, int b) {
 }
 
 // ---------------------------------------------------------------------------
+// Free bodies (body.rs:22-47) and their constraint solve
+// (solve_free_bodies_diag_mode, solver.rs:703-1257) and integration
+// (world.rs:1705-1817, 3209-3286).
+// ---------------------------------------------------------------------------
+
+template <class T> struct Body {
+    T mass = T(1.0);
+    M3<T> inertia = m3identity<T>(), inv_inertia = m3identity<T>();
+    V3<T> p = vzero<T>(), v = vzero<T>();
+    Q<T> q = qidentity<T>();
+    V3<T> w = vzero<T>();  // body frame
+    bool has_max_lin = false, has_max_ang = false;
+    T max_lin = T(0.0), max_ang = T(0.0);
+    T gravity_scale = T(1.0);
+};
+
+// Body::new (body.rs:53-70): the inverse inertia is precomputed.
+template <class T> Body<T> make_body(T mass, const M3<T>& inertia, V3<T> p, Q<T> q) {
+    Body<T> b;
+    b.mass = mass;
+    b.inertia = inertia;
+    if (!m3inverse(inertia, &b.inv_inertia)) fail("body inertia tensor must be invertible");
+    b.p = p;
+    b.q = q;
+    return b;
+}
+
+// solid_box_inertia (geom.rs:942-950).
+template <class T> M3<T> solid_box_inertia(T mass, V3<T> h) {
+    T hx2 = h.x * h.x, hy2 = h.y * h.y, hz2 = h.z * h.z;
+    return m3diag((mass / T(3.0)) * (hy2 + hz2), (mass / T(3.0)) * (hx2 + hz2), (mass / T(3.0)) * (hx2 + hy2));
+}
+
+template <class T> struct BodyDeriv {  // Deriv (world.rs:3209-3215)
+    V3<T> dp, dv;
+    Q<T> dq;
+    V3<T> dw;
+};
+
+template <class T> inline V3<T> body_gravity(const Body<T>& b, V3<T> g) {
+    return b.gravity_scale == T(1.0) ? g : g * b.gravity_scale;
+}
+
+// evaluate (world.rs:3223-3245).
+template <class T> BodyDeriv<T> evaluate_body(const Body<T>& s, V3<T> gravity, V3<T> f, V3<T> tau) {
+    BodyDeriv<T> d;
+    d.dv = body_gravity(s, gravity) + f / s.mass;
+    V3<T> iw = s.inertia * s.w;
+    V3<T> gyro = -cross(s.w, iw);
+    V3<T> tau_body = inverse_rotate(s.q, tau);
+    d.dw = s.inv_inertia * (tau_body + gyro);
+    d.dq = qderivative(s.q, s.w);
+    d.dp = s.v;
+    return d;
+}
+
+// clamp_velocity (world.rs:3273-3286).
+template <class T> V3<T> clamp_velocity(V3<T> v, bool has_cap, T cap) {
+    if (!has_cap) return v;
+    if (cap <= T(0.0)) return vzero<T>();
+    T speed = length(v);
+    return speed > cap ? normalize(v) * cap : v;
+}
+
+template <class T> inline Q<T> qadd(Q<T> a, Q<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+
+// advance_all for one body (world.rs:3256-3271): origin + deriv * h.
+template <class T> Body<T> advance_body(const Body<T>& s, const BodyDeriv<T>& d, T h) {
+    Body<T> o = s;
+    o.p = s.p + d.dp * h;
+    o.v = s.v + d.dv * h;
+    o.q = qadd(s.q, qscale(d.dq, h));
+    o.w = s.w + d.dw * h;
+    return o;
+}
+
+// One constraint row of the free-body solve (ConstraintRow, solver.rs:1863-1885);
+// contact rows are always linear (RowGeom::Linear) under the pyramidal cone.
+template <class T> struct BodyRow {
+    V3<T> dir;
+    int body_a, body_b;  // -1: static side
+    V3<T> arm_a, arm_b;
+    T reg = T(0.0), diag = T(0.0), bias = T(0.0);
+};
+template <class T> struct BodyDelta {  // BodyDelta: dv_lin_world, dw_ang_body
+    V3<T> dv = vzero<T>(), dw = vzero<T>();
+};
+
+template <class T> struct FreeBodySolve {
+    const std::vector<Body<T>>& bodies;
+    std::vector<M3<T>> inv_i_world;
+    std::vector<BodyRow<T>> rows;
+
+    // row_body_diagonal (solver.rs:1892-1928), linear rows.
+    T row_body_diagonal(const BodyRow<T>& r) const {
+        T a = T(0.0);
+        if (r.body_a >= 0) {
+            V3<T> rc = cross(r.arm_a, r.dir);
+            a += dot(rc, inv_i_world[r.body_a] * rc) + T(1.0) / bodies[r.body_a].mass;
+        }
+        if (r.body_b >= 0) {
+            V3<T> rc = cross(r.arm_b, r.dir);
+            a += dot(rc, inv_i_world[r.body_b] * rc) + T(1.0) / bodies[r.body_b].mass;
+        }
+        return a;
+    }
+    // row_residual (solver.rs:1980-2020), linear rows.
+    T row_residual(const BodyRow<T>& r, const std::vector<BodyDelta<T>>& bd) const {
+        T s = T(0.0);
+        if (r.body_a >= 0) {
+            V3<T> wa = rotate(bodies[r.body_a].q, bd[r.body_a].dw);
+            V3<T> vp = bd[r.body_a].dv + cross(wa, r.arm_a);
+            s += dot(vp, r.dir);
+        }
+        if (r.body_b >= 0) {
+            V3<T> wb = rotate(bodies[r.body_b].q, bd[r.body_b].dw);
+            V3<T> vp = bd[r.body_b].dv + cross(wb, r.arm_b);
+            s -= dot(vp, r.dir);
+        }
+        return s;
+    }
+    // apply_impulse_delta (solver.rs:2022-2069), linear rows.
+    void apply(const BodyRow<T>& r, T df, std::vector<BodyDelta<T>>& bd) const {
+        if (df == T(0.0)) return;
+        if (r.body_a >= 0) {
+            int i = r.body_a;
+            V3<T> f = r.dir * df;
+            V3<T> tq = cross(r.arm_a, f);
+            bd[i].dv = bd[i].dv + f / bodies[i].mass;
+            bd[i].dw = bd[i].dw + inverse_rotate(bodies[i].q, inv_i_world[i] * tq);
+        }
+        if (r.body_b >= 0) {
+            int i = r.body_b;
+            V3<T> f = (-r.dir) * df;
+            V3<T> tq = cross(r.arm_b, f);
+            bd[i].dv = bd[i].dv + f / bodies[i].mass;
+            bd[i].dw = bd[i].dw + inverse_rotate(bodies[i].q, inv_i_world[i] * tq);
+        }
+    }
+    // normal_velocity_at_point (solver.rs:2131-2154).
+    T current_velocity(const BodyRow<T>& r) const {
+        V3<T> va = vzero<T>(), vb = vzero<T>();
+        if (r.body_a >= 0) {
+            const Body<T>& b = bodies[r.body_a];
+            va = b.v + cross(rotate(b.q, b.w), r.arm_a);
+        }
+        if (r.body_b >= 0) {
+            const Body<T>& b = bodies[r.body_b];
+            vb = b.v + cross(rotate(b.q, b.w), r.arm_b);
+        }
+        return dot(va - vb, r.dir);
+    }
+    // free_step_velocity_contribution (solver.rs:4012-4036) with the
+    // gravity-only free step: dw_body_free = 0.
+    T free_step_velocity(const BodyRow<T>& r, const std::vector<V3<T>>& dv_free) const {
+        T s = T(0.0);
+        if (r.body_a >= 0) {
+            V3<T> dw = rotate(bodies[r.body_a].q, vzero<T>());
+            V3<T> vp = dv_free[r.body_a] + cross(dw, r.arm_a);
+            s += dot(vp, r.dir);
+        }
+        if (r.body_b >= 0) {
+            V3<T> dw = rotate(bodies[r.body_b].q, vzero<T>());
+            V3<T> vp = dv_free[r.body_b] + cross(dw, r.arm_b);
+            s -= dot(vp, r.dir);
+        }
+        return s;
+    }
+    // row_cross_response_into (solver.rs:1604-1618).
+    T cross_response(const BodyRow<T>& r, const BodyRow<T>& applied) const {
+        std::vector<BodyDelta<T>> scratch(bodies.size());
+        apply(applied, T(1.0), scratch);
+        return row_residual(r, scratch);
+    }
+    // pgs_step_non_negative (solver.rs:1448-1466).
+    void step_non_negative(int ri, std::vector<T>& f, std::vector<BodyDelta<T>>& bd) const {
+        const BodyRow<T>& r = rows[ri];
+        T res = row_residual(r, bd) + r.reg * f[ri] + r.bias;
+        T delta = -res / r.diag;
+        T nf = f[ri] + delta;
+        T pr = nf < T(0.0) ? T(0.0) : nf;
+        delta = pr - f[ri];
+        f[ri] = pr;
+        apply(r, delta, bd);
+    }
+    // pgs_step_pyramidal_pair (solver.rs:1476-1520).
+    void step_pair(int ri, T h01, std::vector<T>& f, std::vector<BodyDelta<T>>& bd) const {
+        int rj = ri + 1;
+        T res_i = row_residual(rows[ri], bd) + rows[ri].reg * f[ri] + rows[ri].bias;
+        T res_j = row_residual(rows[rj], bd) + rows[rj].reg * f[rj] + rows[rj].bias;
+        T h00 = rows[ri].diag, h11 = rows[rj].diag;
+        T det = h00 * h11 - h01 * h01;
+        if (det <= T(0.0)) {
+            step_non_negative(ri, f, bd);
+            step_non_negative(rj, f, bd);
+            return;
+        }
+        T di = (-res_i * h11 + h01 * res_j) / det;
+        T dj = (h01 * res_i - h00 * res_j) / det;
+        T ni = f[ri] + di, nj = f[rj] + dj;
+        T a, b;
+        if (ni >= T(0.0) && nj >= T(0.0)) {
+            a = ni;
+            b = nj;
+        } else if (ni < T(0.0) && nj < T(0.0)) {
+            a = T(0.0);
+            b = T(0.0);
+        } else if (ni < T(0.0)) {
+            a = T(0.0);
+            b = rmax(f[rj] - res_j / h11, T(0.0));
+        } else {
+            a = rmax(f[ri] - res_i / h00, T(0.0));
+            b = T(0.0);
+        }
+        T ai = a - f[ri], aj = b - f[rj];
+        f[ri] = a;
+        f[rj] = b;
+        apply(rows[ri], ai, bd);
+        apply(rows[rj], aj, bd);
+    }
+};
+
+// solve_free_bodies (PGS; solver.rs:703-1257) for condim-3 contacts under the
+// pyramidal cone: per-body world wrenches (force, torque).
+template <class T>
+std::vector<std::pair<V3<T>, V3<T>>> solve_free_bodies(const std::vector<Body<T>>& bodies,
+                                                      const std::vector<Geom<T>>& geoms,
+                                                      const std::vector<Contact<T>>& contacts, V3<T> gravity,
+                                                      T dt, int cone, uint32_t iterations) {
+    const size_t nb = bodies.size();
+    std::vector<std::pair<V3<T>, V3<T>>> wrench(nb, {vzero<T>(), vzero<T>()});
+    if (contacts.empty() || dt <= T(0.0)) return wrench;
+    if (cone != C_PYRAMIDAL) fail("elliptic cone is not restated for free bodies");
+    std::vector<V3<T>> dv_free(nb);
+    for (size_t i = 0; i < nb; ++i) dv_free[i] = body_gravity(bodies[i], gravity) * dt;
+    FreeBodySolve<T> S{bodies, {}, {}};
+    struct Block {
+        int start;
+        SolRef<T> solref;
+        SolImp<T> solimp;
+        T pen, mu;
+        T h01 = T(0.0), h23 = T(0.0);
+    };
+    std::vector<Block> blocks;
+    for (const Contact<T>& c : contacts) {
+        const Geom<T>& ga = geoms[c.geom_a];
+        const Geom<T>& gb = geoms[c.geom_b];
+        int ba = ga.body, bb = gb.body;
+        if (ba < 0 && bb < 0) continue;
+        T pen = c.penetration - c.gap;
+        if (pen <= T(0.0)) continue;
+        int condim = ga.condim < gb.condim ? ga.condim : gb.condim;
+        if (condim != 3) fail("condim other than 3 is not restated for free bodies");
+        T mu = combine_friction(ga.friction, gb.friction);
+        // contact_friction_axes without anisotropy (solver.rs:3441-3465)
+        V3<T> t1 = tangent_basis(c.normal).first;
+        V3<T> t2 = cross(c.normal, t1);
+        V3<T> arm_a = ba >= 0 ? c.pos - bodies[ba].p : vzero<T>();
+        V3<T> arm_b = bb >= 0 ? c.pos - bodies[bb].p : vzero<T>();
+        int start = (int)S.rows.size();
+        // contact_row_directions (solver.rs:3401-3417)
+        const V3<T> dirs[4] = {c.normal + t1 * mu, c.normal - t1 * mu, c.normal + t2 * mu, c.normal - t2 * mu};
+        for (int k = 0; k < 4; ++k) S.rows.push_back(BodyRow<T>{dirs[k], ba, bb, arm_a, arm_b});
+        Block b;
+        b.start = start;
+        b.solref = combine_solref(ga.solref, gb.solref);
+        b.solimp = combine_solimp(ga.solimp, gb.solimp);
+        b.pen = pen;
+        b.mu = mu;
+        blocks.push_back(b);
+    }
+    const int nr = (int)S.rows.size();
+    if (nr == 0) return wrench;
+    S.inv_i_world.resize(nb);
+    for (size_t i = 0; i < nb; ++i) {
+        M3<T> r = to_mat3(bodies[i].q);
+        S.inv_i_world[i] = (r * bodies[i].inv_inertia) * transpose(r);
+    }
+    for (Block& b : blocks) {
+        for (int k = 0; k < 4; ++k) {
+            BodyRow<T>& r = S.rows[b.start + k];
+            T a_ii = S.row_body_diagonal(r);
+            T d = impedance_at_position(-b.pen, T(0.0), b.solimp);
+            // contact_regularization (solver.rs:1930-1959), pyramidal condim 3
+            T reg = T(0.0);
+            if (d > T(0.0)) {
+                T tran = T(0.0);
+                if (r.body_a >= 0) tran += T(1.0) / bodies[r.body_a].mass;
+                if (r.body_b >= 0) tran += T(1.0) / bodies[r.body_b].mass;
+                reg = contact_regularization_from_diag(tran * (T(1.0) + b.mu * b.mu), d, b.mu, true);
+            }
+            r.reg = reg;
+            r.diag = a_ii + reg;
+        }
+        for (int k = 0; k < 4; ++k) {
+            BodyRow<T>& r = S.rows[b.start + k];
+            T vcur = S.current_velocity(r);
+            T dvf = S.free_step_velocity(r, dv_free);
+            T a_ref = reference_accel(-b.pen, vcur, safe_solref(b.solref, dt), b.solimp);
+            r.bias = dvf - a_ref * dt;
+        }
+    }
+    for (Block& b : blocks) {
+        b.h01 = S.cross_response(S.rows[b.start], S.rows[b.start + 1]);
+        b.h23 = S.cross_response(S.rows[b.start + 2], S.rows[b.start + 3]);
+    }
+    std::vector<T> f(nr, T(0.0));
+    std::vector<BodyDelta<T>> bd(nb);
+    for (uint32_t it = 0; it < iterations; ++it) {
+        for (const Block& b : blocks) {
+            S.step_non_negative(b.start, f, bd);
+            S.step_pair(b.start, b.h01, f, bd);
+            S.step_pair(b.start + 2, b.h23, f, bd);
+        }
+    }
+    for (int ri = 0; ri < nr; ++ri) {
+        T fi = f[ri];
+        if (fi == T(0.0)) continue;
+        const BodyRow<T>& r = S.rows[ri];
+        V3<T> fa = r.dir * (fi / dt);
+        if (r.body_a >= 0) {
+            wrench[r.body_a].first = wrench[r.body_a].first + fa;
+            wrench[r.body_a].second = wrench[r.body_a].second + cross(r.arm_a, fa);
+        }
+        if (r.body_b >= 0) {
+            V3<T> fb = -fa;
+            wrench[r.body_b].first = wrench[r.body_b].first + fb;
+            wrench[r.body_b].second = wrench[r.body_b].second + cross(r.arm_b, fb);
+        }
+    }
+    return wrench;
+}
+
+// ---------------------------------------------------------------------------
 // World
 // ---------------------------------------------------------------------------
 
@@ -509,6 +843,7 @@ template <class T> struct World {
     uint32_t iterations = 20;
     int cone = C_PYRAMIDAL;
     std::vector<Tree<T>> trees;
+    std::vector<Body<T>> bodies;
     std::vector<Geom<T>> geoms;
     std::vector<bool> self_collide;  // per tree; false => in disabled_self_collision
     // Broadphase proxies: fat AABB per geom (DynamicAabbTree leaf bounds).
@@ -526,6 +861,10 @@ template <class T> struct World {
         ws.emplace_back();
         return (int)trees.size() - 1;
     }
+    int add_body(const Body<T>& b) {  // world.rs:950-954
+        bodies.push_back(b);
+        return (int)bodies.size() - 1;
+    }
     int add_geom(const Geom<T>& g) {  // world.rs:974-979
         geoms.push_back(g);
         refresh_broadphase();
@@ -538,6 +877,7 @@ template <class T> struct World {
     }
     Pose<T> geom_pose(int gi, const std::vector<std::vector<Pose<T>>>& poses) const {
         const Geom<T>& g = geoms[gi];
+        if (g.is_body()) return geom_world_pose(g, bodies[g.body].p, bodies[g.body].q);
         if (!g.is_link()) return geom_world_pose(g, vzero<T>(), qidentity<T>());
         const Pose<T>& lp = poses[g.tree][g.link];
         return geom_world_pose(g, lp.p, lp.q);
@@ -564,6 +904,11 @@ template <class T> struct World {
                                       integrate_body_angular_velocity(lp.q, inverse_rotate(lp.q, w), dt));
                 ang_speed = length(w);
                 pivot = lp.p;
+            } else if (g.is_body()) {
+                const Body<T>& b = bodies[g.body];
+                end = geom_world_pose(g, b.p + b.v * dt, integrate_body_angular_velocity(b.q, b.w, dt));
+                ang_speed = length(rotate(b.q, b.w));
+                pivot = b.p;
             }
             Aabb<T> sb = geom_aabb(g, start), eb = geom_aabb(g, end);
             V3<T> bc = (sb.mn + sb.mx) * T(0.5);
@@ -587,7 +932,10 @@ template <class T> struct World {
         const Geom<T>& gb = geoms[b];
         if (!((ga.group & gb.mask) != 0 && (gb.group & ga.mask) != 0)) return false;
         if (ga.is_link() && gb.is_link() && ga.tree == gb.tree && !self_collide[ga.tree]) return false;
-        bool same_att = (ga.is_link() == gb.is_link()) && (!ga.is_link() || (ga.tree == gb.tree && ga.link == gb.link));
+        // GeomAttach equality: Static, Body(i) or Link(tree, link).
+        auto kind = [](const Geom<T>& g) { return g.is_link() ? 2 : g.is_body() ? 1 : 0; };
+        bool same_att = kind(ga) == kind(gb) && (!ga.is_link() || (ga.tree == gb.tree && ga.link == gb.link)) &&
+                        (!ga.is_body() || ga.body == gb.body);
         return !same_att;
     }
 
@@ -890,6 +1238,46 @@ template <class T> struct World {
         return qfrc;
     }
 
+    // step_bodies (RK4, world.rs:1705-1785) and step_bodies_euler (Euler and
+    // ImplicitFast, 1790-1817) with the solver wrenches held over the step.
+    // last_qacc gets (dlinear_velocity, dangular_velocity_body) of the
+    // first evaluation per body.
+    void step_bodies(const std::vector<std::pair<V3<T>, V3<T>>>& ext) {
+        for (size_t i = 0; i < bodies.size(); ++i) {
+            Body<T>& b = bodies[i];
+            const V3<T> f = ext[i].first, tq = ext[i].second;
+            BodyDeriv<T> k1 = evaluate_body(b, gravity, f, tq);
+            const V3<T> a1[2] = {k1.dv, k1.dw};
+            for (int k = 0; k < 2; ++k) {
+                last_qacc.push_back(a1[k].x);
+                last_qacc.push_back(a1[k].y);
+                last_qacc.push_back(a1[k].z);
+            }
+            if (integrator == I_RK4) {
+                const Body<T> s0 = b;
+                BodyDeriv<T> k2 = evaluate_body(advance_body(s0, k1, dt * T(0.5)), gravity, f, tq);
+                BodyDeriv<T> k3 = evaluate_body(advance_body(s0, k2, dt * T(0.5)), gravity, f, tq);
+                BodyDeriv<T> k4 = evaluate_body(advance_body(s0, k3, dt), gravity, f, tq);
+                const T sixth = T(1.0) / T(6.0);
+                V3<T> dp = (k1.dp + k2.dp * T(2.0) + k3.dp * T(2.0) + k4.dp) * sixth;
+                V3<T> dv = (k1.dv + k2.dv * T(2.0) + k3.dv * T(2.0) + k4.dv) * sixth;
+                Q<T> dq = qscale(qadd(qadd(qadd(k1.dq, qscale(k2.dq, T(2.0))), qscale(k3.dq, T(2.0))), k4.dq), sixth);
+                V3<T> dw = (k1.dw + k2.dw * T(2.0) + k3.dw * T(2.0) + k4.dw) * sixth;
+                b.p = s0.p + dp * dt;
+                b.v = clamp_velocity(s0.v + dv * dt, s0.has_max_lin, s0.max_lin);
+                b.q = renormalize(qadd(s0.q, qscale(dq, dt)));
+                b.w = clamp_velocity(s0.w + dw * dt, s0.has_max_ang, s0.max_ang);
+            } else {
+                V3<T> v = b.v + k1.dv * dt;
+                V3<T> w = b.w + k1.dw * dt;
+                b.p = b.p + v * dt;
+                b.q = integrate_body_angular_velocity(b.q, w, dt);
+                b.v = clamp_velocity(v, b.has_max_lin, b.max_lin);
+                b.w = clamp_velocity(w, b.has_max_ang, b.max_ang);
+            }
+        }
+    }
+
     // World::step (world.rs:1459-1537) for trees.
     void step() {
         if (iterations == 0) fail("solver iterations must be >= 1");
@@ -903,15 +1291,25 @@ template <class T> struct World {
         last_pairs = pairs;
         bool constraint = mode == M_PGS || mode == M_NEWTON;
         std::vector<std::vector<T>> tq;
+        std::vector<std::pair<V3<T>, V3<T>>> bw(bodies.size(), {vzero<T>(), vzero<T>()});
         last_contacts.clear();
         if (constraint) {
             std::vector<Contact<T>> contacts = detect_contacts(pairs);
-            std::vector<Contact<T>> tc;
-            for (auto& c : contacts)
-                if (geoms[c.geom_a].is_link() || geoms[c.geom_b].is_link()) tc.push_back(c);
+            std::vector<Contact<T>> tc, fc;
+            for (auto& c : contacts) {
+                const Geom<T>& ga = geoms[c.geom_a];
+                const Geom<T>& gb = geoms[c.geom_b];
+                if ((ga.is_link() && gb.is_body()) || (ga.is_body() && gb.is_link()))
+                    fail("tree-body contacts are not restated");
+                if (ga.is_link() || gb.is_link()) tc.push_back(c);
+                else fc.push_back(c);  // compute_solver_wrenches (world.rs:2097-2149)
+            }
             last_contacts = tc;
+            last_contacts.insert(last_contacts.end(), fc.begin(), fc.end());
             int ti = integrator == I_RK4 ? -1 : integrator == I_IMPLICITFAST ? 1 : 0;
             tq = solve_tree_contacts(tc, ti);
+            if (mode == M_NEWTON && !fc.empty()) fail("Newton free-body solve is not restated");
+            bw = solve_free_bodies(bodies, geoms, fc, gravity, dt, cone, iterations);
         } else {
             if (!pairs.empty()) fail("penalty contact forces are not restated");
         }
@@ -940,6 +1338,7 @@ template <class T> struct World {
             for (size_t s = 0; s < delta.size(); ++s) tr.qfrc_applied[s] -= delta[s];
             tr.disable_penalty_limits = prior;
         }
+        step_bodies(bw);
         refresh_broadphase();
     }
 };

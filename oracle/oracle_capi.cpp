@@ -24,6 +24,7 @@ struct Base {
     virtual ~Base() = default;
     virtual void add_tree(const newt_link_desc* l, int n, int collide) = 0;
     virtual void add_geom(const newt_geom_desc* g) = 0;
+    virtual void add_body(const newt_body_desc* b) = 0;
     virtual void finalize(int n_worlds) = 0;
     virtual int nq() const = 0;
     virtual int nv() const = 0;
@@ -101,19 +102,36 @@ template <class T> struct Impl : Base {
         g.condim = d->condim;
         g.group = d->group;
         g.mask = d->mask;
+        g.body = d->body;
         proto.add_geom(g);
     }
+    void add_body(const newt_body_desc* d) override {
+        M3<T> inertia;
+        for (int k = 0; k < 9; ++k) inertia.d[k] = T(d->inertia[k]);
+        Body<T> b = make_body(T(d->mass), inertia, v3(T(d->position[0]), T(d->position[1]), T(d->position[2])),
+                              Q<T>{T(d->orientation[0]), T(d->orientation[1]), T(d->orientation[2]), T(d->orientation[3])});
+        b.v = v3(T(d->linear_velocity[0]), T(d->linear_velocity[1]), T(d->linear_velocity[2]));
+        b.w = v3(T(d->angular_velocity_body[0]), T(d->angular_velocity_body[1]), T(d->angular_velocity_body[2]));
+        b.has_max_lin = d->has_max_linear_velocity != 0;
+        b.has_max_ang = d->has_max_angular_velocity != 0;
+        b.max_lin = T(d->max_linear_velocity);
+        b.max_ang = T(d->max_angular_velocity);
+        b.gravity_scale = T(d->gravity_scale);
+        proto.add_body(b);
+    }
     void finalize(int n) override { worlds.assign(n, proto); }
-    int nq() const override {
+    int tree_nq() const {
         int s = 0;
         for (auto& t : proto.trees) s += t.nq();
         return s;
     }
-    int nv() const override {
+    int tree_nv() const {
         int s = 0;
         for (auto& t : proto.trees) s += t.nv();
         return s;
     }
+    int nq() const override { return tree_nq() + 7 * (int)proto.bodies.size(); }
+    int nv() const override { return tree_nv() + 6 * (int)proto.bodies.size(); }
     void set_state(const double* qpos, const double* qvel) override {
         int NQ = nq(), NV = nv();
         for (size_t w = 0; w < worlds.size(); ++w) {
@@ -123,6 +141,16 @@ template <class T> struct Impl : Base {
                 for (int k = 0; k < t.nv(); ++k) t.qdot[k] = T(qvel[w * NV + ov + k]);
                 oq += t.nq();
                 ov += t.nv();
+            }
+            for (auto& b : worlds[w].bodies) {
+                const double* q = qpos + w * NQ + oq;
+                const double* v = qvel + w * NV + ov;
+                b.p = v3(T(q[0]), T(q[1]), T(q[2]));
+                b.q = Q<T>{T(q[3]), T(q[4]), T(q[5]), T(q[6])};
+                b.v = v3(T(v[0]), T(v[1]), T(v[2]));
+                b.w = v3(T(v[3]), T(v[4]), T(v[5]));
+                oq += 7;
+                ov += 6;
             }
             // Public state setters refresh the query proxies (world.rs:1119-1136).
             worlds[w].refresh_broadphase();
@@ -142,6 +170,7 @@ template <class T> struct Impl : Base {
     void get_qfrc(double* qfrc) const override {
         int NV = nv();
         for (size_t w = 0; w < worlds.size(); ++w) {
+            for (int k = 0; k < NV; ++k) qfrc[w * NV + k] = 0.0;
             int ov = 0;
             for (auto& t : worlds[w].trees) {
                 for (int k = 0; k < t.nv(); ++k) qfrc[w * NV + ov + k] = (double)t.qfrc_applied[k];
@@ -158,6 +187,16 @@ template <class T> struct Impl : Base {
                 for (int k = 0; k < t.nv(); ++k) qvel[w * NV + ov + k] = (double)t.qdot[k];
                 oq += t.nq();
                 ov += t.nv();
+            }
+            for (const auto& b : worlds[w].bodies) {
+                double* q = qpos + w * NQ + oq;
+                double* v = qvel + w * NV + ov;
+                q[0] = (double)b.p.x; q[1] = (double)b.p.y; q[2] = (double)b.p.z;
+                q[3] = (double)b.q.x; q[4] = (double)b.q.y; q[5] = (double)b.q.z; q[6] = (double)b.q.w;
+                v[0] = (double)b.v.x; v[1] = (double)b.v.y; v[2] = (double)b.v.z;
+                v[3] = (double)b.w.x; v[4] = (double)b.w.y; v[5] = (double)b.w.z;
+                oq += 7;
+                ov += 6;
             }
         }
     }
@@ -244,6 +283,7 @@ void* oracle_create(const newt_options* o, int precision) {
 void oracle_destroy(void* h) { delete static_cast<Base*>(h); }
 void oracle_add_tree(void* h, const newt_link_desc* l, int n, int collide) { static_cast<Base*>(h)->add_tree(l, n, collide); }
 void oracle_add_geom(void* h, const newt_geom_desc* g) { static_cast<Base*>(h)->add_geom(g); }
+void oracle_add_body(void* h, const newt_body_desc* b) { static_cast<Base*>(h)->add_body(b); }
 void oracle_finalize(void* h, int n_worlds) { static_cast<Base*>(h)->finalize(n_worlds); }
 int oracle_nq(void* h) { return static_cast<Base*>(h)->nq(); }
 int oracle_nv(void* h) { return static_cast<Base*>(h)->nv(); }

@@ -33,6 +33,7 @@ def lib():
             "oracle_destroy": (None, [P]),
             "oracle_add_tree": (None, [P, P, I, I]),
             "oracle_add_geom": (None, [P, P]),
+            "oracle_add_body": (None, [P, P]),
             "oracle_finalize": (None, [P, I]),
             "oracle_nq": (I, [P]),
             "oracle_nv": (I, [P]),
@@ -84,8 +85,11 @@ class OracleWorlds:
         self.n = n_worlds
         opts = scene.options_struct()
         self.h = L.oracle_create(C.byref(opts), precision)
-        links = scene.link_array()
-        L.oracle_add_tree(self.h, C.cast(links, C.c_void_p), len(scene.links), scene.self_collide)
+        if scene.links:
+            links = scene.link_array()
+            L.oracle_add_tree(self.h, C.cast(links, C.c_void_p), len(scene.links), scene.self_collide)
+        for b in scene.body_structs():
+            L.oracle_add_body(self.h, C.cast(C.pointer(b), C.c_void_p))
         for g in scene.geom_structs():
             L.oracle_add_geom(self.h, C.cast(C.pointer(g), C.c_void_p))
         L.oracle_finalize(self.h, n_worlds)

@@ -12,6 +12,17 @@ float instantiation rebuilds the same scene and must reproduce every byte:
   solver_tree                broadphase, box-plane + box-box manifold, tree
                              contact PGS (Delassus, diagApprox, SolImp/SolRef),
                              RK4 (newt/tests/solver_golden.rs:93-144)
+  tumbling_3_body            free-body RK4: Euler's equations, quaternion
+                             derivative, renormalize (newt/tests/golden.rs)
+  integrator_euler / _rk4 /  one free body under each integrator, incl.
+  integrator_implicitfast    integrate_body_angular_velocity
+                             (newt/tests/integrators_golden.rs)
+  solver_stack               free-body PGS: box-plane and box-box contacts,
+                             rows, regularization, pyramidal facet pairs, body
+                             deltas, wrench recovery, RK4 ZOH
+                             (newt/tests/solver_golden.rs:27-46)
+  solver_incline             free-body PGS friction under tilted gravity
+                             (newt/tests/solver_golden.rs:48-79)
 """
 import os
 
@@ -20,7 +31,9 @@ import pytest
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
-@pytest.mark.parametrize("name", ["joints_chain_and_floating", "joints_mixed", "dynamics_mass_bias", "solver_tree"])
+@pytest.mark.parametrize("name", ["joints_chain_and_floating", "joints_mixed", "dynamics_mass_bias", "solver_tree",
+                                  "tumbling_3_body", "integrator_euler", "integrator_rk4",
+                                  "integrator_implicitfast", "solver_stack", "solver_incline"])
 def test_golden_bytes(oracle_lib, name):
     expected = open(os.path.join(GOLDEN, name + ".bin"), "rb").read()
     actual = oracle_lib.golden_bytes(name)

@@ -22,6 +22,8 @@ SOLVER_PENALTY, SOLVER_PGS, SOLVER_NEWTON = range(3)
 CONE_PYRAMIDAL, CONE_ELLIPTIC = range(2)
 PRECISION_FP64, PRECISION_MIXED, PRECISION_FP64_EXACT = range(3)
 WORLD_PARKED = -1
+(ERR_INVALID_CONFIG, ERR_UNSUPPORTED_PAIR, ERR_NOT_PD, ERR_NEWTON_FAIL, ERR_OOM, ERR_DEVICE, ERR_INVALID_ARG,
+ ERR_CAPACITY, ERR_UNSUPPORTED) = range(1, 10)
 
 D3 = C.c_double * 3
 D4 = C.c_double * 4
@@ -80,7 +82,23 @@ class GeomDesc(C.Structure):
         ("solimp_power", C.c_int32),
         ("group", C.c_uint32),
         ("mask", C.c_uint32),
-        ("pad0", C.c_int32),
+        ("body", C.c_int32),
+    ]
+
+
+class BodyDesc(C.Structure):
+    _fields_ = [
+        ("mass", C.c_double),
+        ("inertia", C.c_double * 9),
+        ("position", D3),
+        ("orientation", D4),
+        ("linear_velocity", D3),
+        ("angular_velocity_body", D3),
+        ("has_max_linear_velocity", C.c_int32),
+        ("has_max_angular_velocity", C.c_int32),
+        ("max_linear_velocity", C.c_double),
+        ("max_angular_velocity", C.c_double),
+        ("gravity_scale", C.c_double),
     ]
 
 
@@ -94,6 +112,7 @@ SIGNATURES = {
     "newt_model_create": (_I, [C.POINTER(Options), C.POINTER(_P)]),
     "newt_model_add_tree": (_I, [_P, C.POINTER(LinkDesc), _I, _I, _PI]),
     "newt_model_add_geom": (_I, [_P, C.POINTER(GeomDesc), _PI]),
+    "newt_model_add_body": (_I, [_P, C.POINTER(BodyDesc), _PI]),
     "newt_model_finalize": (_I, [_P]),
     "newt_model_dims": (_I, [_P, _PI, _PI, _PI, _PI]),
     "newt_model_destroy": (None, [_P]),

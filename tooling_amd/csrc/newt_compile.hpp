@@ -27,6 +27,9 @@ struct HostModel {
     std::vector<int> level_off, level_links, level_nrank, dof_link;
     std::vector<LimitT> limits;
     std::vector<double> q_default;
+    std::vector<double> v_default;  // zero for trees; the bodies' initial velocities
+    std::vector<newt_body_desc> bodies;
+    std::vector<BodyT> hbodies;
 };
 
 inline newt_status cfail(std::string* err, newt_status s, const std::string& msg) {
@@ -82,11 +85,152 @@ inline void spatial_inertia(const newt_link_desc& d, double si[36]) {
         }
 }
 
+// Mat3::inverse (math.rs:427-453) of a column-major 3x3, in double; the
+// result row-major.  False when the determinant is zero or not finite.
+inline bool body_inverse(const double* cm, double* out_rm) {
+    auto get = [&](int r, int c) { return cm[c * 3 + r]; };
+    double a = get(0, 0), b = get(0, 1), c = get(0, 2);
+    double d = get(1, 0), e = get(1, 1), f = get(1, 2);
+    double g = get(2, 0), h = get(2, 1), i = get(2, 2);
+    double det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+    if (det == 0.0 || !std::isfinite(det)) return false;
+    double inv = 1.0 / det;
+    const double col[9] = {(e * i - f * h) * inv, (f * g - d * i) * inv, (d * h - e * g) * inv,
+                           (c * h - b * i) * inv, (a * i - c * g) * inv, (b * g - a * h) * inv,
+                           (b * f - c * e) * inv, (c * d - a * f) * inv, (a * e - b * d) * inv};
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) out_rm[r * 3 + k] = col[k * 3 + r];
+    return true;
+}
+
+// A model of free bodies with static geoms (World::add_body, world.rs:950):
+// state layout, body tables, geoms riding on bodies, static candidate pairs.
+inline newt_status compile_bodies(HostModel* M, std::string* err) {
+    const newt_options& o = M->opt;
+    if (!M->trees.empty()) return cfail(err, NEWT_ERR_UNSUPPORTED, "a model holds a tree or free bodies, not both");
+    if (o.iterations < 1) return cfail(err, NEWT_ERR_INVALID_CONFIG, "solver iterations must be >= 1");
+    if (o.penetration_slop != 0.0) return cfail(err, NEWT_ERR_UNSUPPORTED, "penetration_slop is not built");
+    if (o.solver_mode == NEWT_SOLVER_NEWTON && o.cone == NEWT_CONE_ELLIPTIC)
+        return cfail(err, NEWT_ERR_INVALID_CONFIG, "solver=newton with cone=elliptic is not supported yet; use cone=pyramidal");
+    if (o.integrator != NEWT_INTEGRATOR_EULER && o.integrator != NEWT_INTEGRATOR_IMPLICITFAST &&
+        o.integrator != NEWT_INTEGRATOR_RK4)
+        return cfail(err, NEWT_ERR_INVALID_ARG, "unknown integrator");
+    if (o.solver_mode == NEWT_SOLVER_NEWTON) return cfail(err, NEWT_ERR_UNSUPPORTED, "the Newton free-body solve is not built");
+    if (o.cone != NEWT_CONE_PYRAMIDAL) return cfail(err, NEWT_ERR_UNSUPPORTED, "elliptic cone is not built");
+    const int nb = (int)M->bodies.size();
+    M->hbodies.clear();
+    M->q_default.clear();
+    M->v_default.clear();
+    for (int i = 0; i < nb; ++i) {
+        const newt_body_desc& b = M->bodies[i];
+        if (!(b.mass > 0.0)) return cfail(err, NEWT_ERR_INVALID_ARG, "body mass must be positive");
+        BodyT t{};
+        t.mass = b.mass;
+        t.gravity_scale = b.gravity_scale;
+        t.has_max_lin = b.has_max_linear_velocity != 0;
+        t.has_max_ang = b.has_max_angular_velocity != 0;
+        t.max_lin = b.max_linear_velocity;
+        t.max_ang = b.max_angular_velocity;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) t.inertia[r * 3 + c] = b.inertia[c * 3 + r];
+        if (!body_inverse(b.inertia, t.inv_inertia))
+            return cfail(err, NEWT_ERR_INVALID_ARG, "body inertia tensor must be invertible");
+        t.q_off = 7 * i;
+        t.v_off = 6 * i;
+        M->hbodies.push_back(t);
+        for (int k = 0; k < 3; ++k) M->q_default.push_back(b.position[k]);
+        for (int k = 0; k < 4; ++k) M->q_default.push_back(b.orientation[k]);
+        for (int k = 0; k < 3; ++k) M->v_default.push_back(b.linear_velocity[k]);
+        for (int k = 0; k < 3; ++k) M->v_default.push_back(b.angular_velocity_body[k]);
+    }
+    M->links.clear();
+    M->limits.clear();
+    M->chain.clear();
+    M->hgeoms.clear();
+    for (const newt_geom_desc& g : M->geoms) {
+        if (g.shape < NEWT_SHAPE_PLANE || g.shape > NEWT_SHAPE_CAPSULE) return cfail(err, NEWT_ERR_INVALID_ARG, "unknown geom shape");
+        if (g.tree >= 0) return cfail(err, NEWT_ERR_INVALID_ARG, "geom attached to a missing tree");
+        if (g.body >= nb) return cfail(err, NEWT_ERR_INVALID_ARG, "geom attached to a missing body");
+        if (g.condim != 3 && g.condim != 1 && g.condim != 4 && g.condim != 6)
+            return cfail(err, NEWT_ERR_INVALID_ARG, "condim must be 1, 3, 4 or 6");
+        GeomT h{};
+        h.shape = g.shape;
+        h.body = g.body >= 0 ? g.body : -1;
+        h.link = h.body;  // pose slot n_links + body with no links
+        h.condim = g.condim;
+        for (int k = 0; k < 3; ++k) {
+            h.size[k] = g.size[k];
+            h.pos[k] = g.pos[k];
+        }
+        for (int k = 0; k < 4; ++k) h.quat[k] = g.quat[k];
+        h.friction = g.friction;
+        h.margin = g.margin;
+        h.gap = g.gap;
+        h.solref[0] = g.solref[0];
+        h.solref[1] = g.solref[1];
+        for (int k = 0; k < 4; ++k) h.solimp[k] = g.solimp[k];
+        h.power = g.solimp_power;
+        M->hgeoms.push_back(h);
+    }
+    // Static candidate pairs: group/mask and same attachment (Static ==
+    // Static, Body(i) == Body(i)) (broadphase.rs:16-18, world.rs:1219-1225).
+    M->pairs.clear();
+    long cap_c = 0;
+    for (size_t a = 0; a < M->geoms.size(); ++a)
+        for (size_t b = a + 1; b < M->geoms.size(); ++b) {
+            const GeomT& ga = M->hgeoms[a];
+            const GeomT& gb = M->hgeoms[b];
+            const newt_geom_desc& da = M->geoms[a];
+            const newt_geom_desc& db = M->geoms[b];
+            if (!((da.group & db.mask) != 0 && (db.group & da.mask) != 0)) continue;
+            if (ga.body == gb.body) continue;
+            const bool pa = ga.shape == SH_PLANE, pb = gb.shape == SH_PLANE;
+            const bool round_a = ga.shape == SH_SPHERE || ga.shape == SH_CAPSULE;
+            const bool round_b = gb.shape == SH_SPHERE || gb.shape == SH_CAPSULE;
+            const bool boxes = ga.shape == SH_BOX && gb.shape == SH_BOX;
+            if (!(pa != pb || (round_a && round_b) || boxes))
+                return cfail(err, NEWT_ERR_UNSUPPORTED_PAIR, "contact pair geom " + std::to_string(a) + " x geom " +
+                                                                 std::to_string(b) + " is an unsupported shape pair");
+            if (std::min(ga.condim, gb.condim) != 3)
+                return cfail(err, NEWT_ERR_UNSUPPORTED, "only condim 3 pyramidal contacts are built");
+            M->pairs.push_back({(int)a, (int)b});
+            cap_c += pair_max_contacts(ga.shape, gb.shape);
+        }
+    if (o.solver_mode == NEWT_SOLVER_PENALTY && !M->pairs.empty())
+        return cfail(err, NEWT_ERR_UNSUPPORTED, "penalty contact forces are not built; use the PGS solver");
+    // Every contact the pairs can emit fits up to 4096 per world; a world
+    // that exceeds a smaller compiled bound faults with NEWT_ERR_CAPACITY.
+    const int kMaxBodyContacts = 4096;
+    ModelT& d = M->dev;
+    d = ModelT{};
+    d.n_links = 0;
+    d.nq = 7 * nb;
+    d.nv = 6 * nb;
+    d.n_geoms = (int)M->geoms.size();
+    d.n_pairs = (int)M->pairs.size();
+    d.cap_c = (int)std::max<long>(std::min<long>(cap_c, kMaxBodyContacts), 1);
+    d.cap_r = 4 * d.cap_c;
+    d.integrator = o.integrator;
+    d.mode = o.solver_mode;
+    d.iterations = o.iterations;
+    d.cone = o.cone;
+    d.dt = o.dt;
+    d.slop = o.penetration_slop;
+    for (int k = 0; k < 3; ++k) d.gravity[k] = o.gravity[k];
+    d.geoms = M->hgeoms.data();
+    d.pairs = M->pairs.data();
+    d.n_bodies = nb;
+    d.bodies = M->hbodies.data();
+    return NEWT_OK;
+}
+
 inline newt_status compile_model(HostModel* M, std::string* err) {
+    if (!M->bodies.empty()) return compile_bodies(M, err);
     if (M->trees.size() != 1)
         return cfail(err, NEWT_ERR_UNSUPPORTED, "this build steps models with exactly one tree");
     const newt_options& o = M->opt;
     if (o.iterations < 1) return cfail(err, NEWT_ERR_INVALID_CONFIG, "solver iterations must be >= 1");
+    if (o.penetration_slop != 0.0) return cfail(err, NEWT_ERR_UNSUPPORTED, "penetration_slop is not built");
     if (o.solver_mode == NEWT_SOLVER_NEWTON && o.cone == NEWT_CONE_ELLIPTIC)
         return cfail(err, NEWT_ERR_INVALID_CONFIG, "solver=newton with cone=elliptic is not supported yet; use cone=pyramidal");
     // ImplicitFast differs from Euler only through actuator velocity terms;
@@ -218,11 +362,13 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
         if (g.shape < NEWT_SHAPE_PLANE || g.shape > NEWT_SHAPE_CAPSULE) return cfail(err, NEWT_ERR_INVALID_ARG, "unknown geom shape");
         if (g.tree >= 1) return cfail(err, NEWT_ERR_INVALID_ARG, "geom attached to a missing tree");
         if (g.tree == 0 && (g.link < 0 || g.link >= (int)T.size())) return cfail(err, NEWT_ERR_INVALID_ARG, "geom link out of range");
+        if (g.body >= 0) return cfail(err, NEWT_ERR_INVALID_ARG, "geom attached to a missing body");
         if (g.condim != 3 && g.condim != 1 && g.condim != 4 && g.condim != 6)
             return cfail(err, NEWT_ERR_INVALID_ARG, "condim must be 1, 3, 4 or 6");
         GeomT h{};
         h.shape = g.shape;
         h.link = g.tree == 0 ? g.link : -1;
+        h.body = -1;
         h.condim = g.condim;
         for (int k = 0; k < 3; ++k) {
             h.size[k] = g.size[k];
@@ -266,10 +412,13 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
         }
     if (o.solver_mode == NEWT_SOLVER_PENALTY && !M->pairs.empty())
         return cfail(err, NEWT_ERR_UNSUPPORTED, "penalty contact forces are not built; use the PGS solver");
+    M->v_default.assign(nv, 0.0);
     ModelT& d = M->dev;
     d.n_links = (int)T.size();
     d.nq = nq;
     d.nv = nv;
+    d.n_bodies = 0;
+    d.bodies = nullptr;
     d.n_geoms = (int)M->geoms.size();
     d.n_pairs = (int)M->pairs.size();
     d.cap_c = std::max(cap_c, 1);
@@ -362,5 +511,33 @@ inline Layout make_layout(const ModelT& m, int real_bytes) {
     return L;
 }
 
+
+}  // namespace newt
+
+namespace newt {
+
+// Per-world workspace of a free-body model: body poses, candidate pairs,
+// contact geometry and the free-body solve (rows 4 per pyramidal contact).
+inline Layout make_body_layout(const ModelT& m) {
+    Layout L{};
+    int dd = 0, ii = 0;
+    auto D = [&](int n) { int o = dd; dd += n; return o; };
+    auto I = [&](int n) { int o = ii; ii += n; return o; };
+    const int nb = m.n_bodies;
+    L.poses = D(7 * nb);
+    L.cgeo = D(9 * m.cap_c);
+    L.bwr = D(6 * nb);
+    L.biw = D(9 * nb);
+    L.bbd = D(6 * nb);
+    L.brow = D(7 * m.cap_r);
+    L.bblk = D(8 * m.cap_c);
+    L.total_d = std::max(dd, 1);
+    L.cids = I(3 * m.cap_c);
+    L.pairs = I(std::max(m.n_pairs, 1));
+    L.iwf = I(2);
+    L.bbi = I(2 * m.cap_c);
+    L.total_i = ii;
+    return L;
+}
 
 }  // namespace newt

@@ -14,6 +14,7 @@
 #include "newt_compile.hpp"
 #include "newt_step.hpp"
 #include "newt_team.hpp"
+#include "newt_bodies.hpp"
 
 namespace newt {
 // Defined in newt_team.hip (the persistent-step instantiations).
@@ -60,6 +61,7 @@ struct newt_model {
     int* d_level_nrank = nullptr;
     int* d_dof_link = nullptr;
     LimitT* d_limits = nullptr;
+    BodyT* d_bodies = nullptr;
     CacheLayout cache{};     // model tables as the persistent step caches them in LDS
     char* d_blob = nullptr;  // the same tables packed in that layout
 };
@@ -124,6 +126,33 @@ __global__ void __launch_bounds__(kBlock) k_step(ModelT m, Layout L, DataT d, R*
     d.ncontact[w] = iws[(size_t)(L.iwf + m.n_links + 1) * d.W + w];
 }
 
+// Free-body models: n_steps World::steps per lane in FP64 (every precision).
+__global__ void __launch_bounds__(kBlock) k_bodies(ModelT m, Layout L, DataT d, double* dws, int* iws, int n_steps,
+                                                   int init) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= d.W) return;
+    World<double> wd = world_of<double>(m, L, d, nullptr, dws, iws, w);
+    if (d.fault[w] != 0) return;
+    FreeBodies fb{wd};
+    fb.set_poses();
+    if (init) {
+        wd.populate(m.dt, true);
+        iws[(size_t)L.iwf * d.W + w] = 0;
+        iws[(size_t)(L.iwf + 1) * d.W + w] = 0;
+        return;
+    }
+    wd.populate(m.dt, false);
+    for (int s = 0; s < n_steps; ++s) {
+        int st = fb.step();
+        if (st != 0) {
+            d.fault[w] = st;
+            break;
+        }
+    }
+    d.npair[w] = iws[(size_t)L.iwf * d.W + w];
+    d.ncontact[w] = iws[(size_t)(L.iwf + 1) * d.W + w];
+}
+
 newt_status launch_persistent(newt_data* D, int n_steps, hipStream_t stream) {
     const newt_model* M = D->model;
     const int wpb = D->wpb;
@@ -137,8 +166,13 @@ newt_status launch_persistent(newt_data* D, int n_steps, hipStream_t stream) {
 newt_status launch(newt_data* D, int n_steps, hipStream_t stream, int init) {
     const newt_model* M = D->model;
     D->last_stream = stream;
-    if (!init && D->precision != NEWT_PRECISION_FP64_EXACT) return launch_persistent(D, n_steps, stream);
     dim3 grid((D->W + kBlock - 1) / kBlock), block(kBlock);
+    if (M->dev.n_bodies > 0) {
+        hipLaunchKernelGGL(k_bodies, grid, block, 0, stream, M->dev, D->L, D->d, D->dws, D->iws, n_steps, init);
+        HIP_TRY(hipGetLastError());
+        return NEWT_OK;
+    }
+    if (!init && D->precision != NEWT_PRECISION_FP64_EXACT) return launch_persistent(D, n_steps, stream);
     if (D->precision == NEWT_PRECISION_MIXED)
         hipLaunchKernelGGL(k_step<float>, grid, block, 0, stream, M->dev, D->L, D->d, (float*)D->ws, D->dws, D->iws,
                            n_steps, init);
@@ -206,6 +240,14 @@ newt_status newt_model_add_tree(newt_model* model, const newt_link_desc* links, 
     return NEWT_OK;
 }
 
+newt_status newt_model_add_body(newt_model* model, const newt_body_desc* body, int32_t* body_index) {
+    if (!model || !body) return fail(NEWT_ERR_INVALID_ARG, "bad body arguments");
+    if (model->finalized) return fail(NEWT_ERR_INVALID_ARG, "model is finalized");
+    model->h.bodies.push_back(*body);
+    if (body_index) *body_index = (int32_t)model->h.bodies.size() - 1;
+    return NEWT_OK;
+}
+
 newt_status newt_model_add_geom(newt_model* model, const newt_geom_desc* geom, int32_t* geom_index) {
     if (!model || !geom) return fail(NEWT_ERR_INVALID_ARG, "bad geom arguments");
     if (model->finalized) return fail(NEWT_ERR_INVALID_ARG, "model is finalized");
@@ -225,24 +267,33 @@ newt_status newt_model_finalize(newt_model* model) {
     HIP_TRY(hipMalloc(&model->d_geoms, sizeof(GeomT) * std::max<size_t>(1, H.hgeoms.size())));
     HIP_TRY(hipMalloc(&model->d_pairs, sizeof(PairT) * std::max<size_t>(1, H.pairs.size())));
     HIP_TRY(hipMalloc(&model->d_chain, sizeof(int) * std::max<size_t>(1, H.chain.size())));
-    HIP_TRY(hipMemcpy(model->d_links, H.links.data(), sizeof(LinkT) * H.links.size(), hipMemcpyHostToDevice));
+    if (!H.links.empty())
+        HIP_TRY(hipMemcpy(model->d_links, H.links.data(), sizeof(LinkT) * H.links.size(), hipMemcpyHostToDevice));
     if (!H.hgeoms.empty())
         HIP_TRY(hipMemcpy(model->d_geoms, H.hgeoms.data(), sizeof(GeomT) * H.hgeoms.size(), hipMemcpyHostToDevice));
     if (!H.pairs.empty())
         HIP_TRY(hipMemcpy(model->d_pairs, H.pairs.data(), sizeof(PairT) * H.pairs.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(model->d_chain, H.chain.data(), sizeof(int) * H.chain.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc(&model->d_level_off, sizeof(int) * H.level_off.size()));
-    HIP_TRY(hipMalloc(&model->d_level_links, sizeof(int) * H.level_links.size()));
-    HIP_TRY(hipMemcpy(model->d_level_off, H.level_off.data(), sizeof(int) * H.level_off.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(model->d_level_links, H.level_links.data(), sizeof(int) * H.level_links.size(),
-                      hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc(&model->d_level_nrank, sizeof(int) * H.level_nrank.size()));
+    if (!H.chain.empty())
+        HIP_TRY(hipMemcpy(model->d_chain, H.chain.data(), sizeof(int) * H.chain.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&model->d_level_off, sizeof(int) * std::max<size_t>(1, H.level_off.size())));
+    HIP_TRY(hipMalloc(&model->d_level_links, sizeof(int) * std::max<size_t>(1, H.level_links.size())));
+    if (!H.level_off.empty())
+        HIP_TRY(hipMemcpy(model->d_level_off, H.level_off.data(), sizeof(int) * H.level_off.size(),
+                          hipMemcpyHostToDevice));
+    if (!H.level_links.empty())
+        HIP_TRY(hipMemcpy(model->d_level_links, H.level_links.data(), sizeof(int) * H.level_links.size(),
+                          hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&model->d_level_nrank, sizeof(int) * std::max<size_t>(1, H.level_nrank.size())));
     HIP_TRY(hipMalloc(&model->d_dof_link, sizeof(int) * std::max<size_t>(1, H.dof_link.size())));
     if (!H.dof_link.empty())
         HIP_TRY(hipMemcpy(model->d_dof_link, H.dof_link.data(), sizeof(int) * H.dof_link.size(),
                           hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(model->d_level_nrank, H.level_nrank.data(), sizeof(int) * H.level_nrank.size(),
-                      hipMemcpyHostToDevice));
+    if (!H.level_nrank.empty())
+        HIP_TRY(hipMemcpy(model->d_level_nrank, H.level_nrank.data(), sizeof(int) * H.level_nrank.size(),
+                          hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&model->d_bodies, sizeof(BodyT) * std::max<size_t>(1, H.hbodies.size())));
+    if (!H.hbodies.empty())
+        HIP_TRY(hipMemcpy(model->d_bodies, H.hbodies.data(), sizeof(BodyT) * H.hbodies.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&model->d_limits, sizeof(LimitT) * std::max<size_t>(1, H.limits.size())));
     if (!H.limits.empty())
         HIP_TRY(hipMemcpy(model->d_limits, H.limits.data(), sizeof(LimitT) * H.limits.size(), hipMemcpyHostToDevice));
@@ -256,7 +307,8 @@ newt_status newt_model_finalize(newt_model* model) {
     model->dev.level_nrank = model->d_level_nrank;
     model->dev.dof_link = model->d_dof_link;
     model->dev.limits = model->d_limits;
-    {
+    model->dev.bodies = model->d_bodies;
+    if (H.dev.n_bodies == 0) {
         const ModelT& hd = H.dev;
         CacheLayout C = make_cache_layout(hd);
         std::vector<char> blob(C.bytes, 0);
@@ -299,6 +351,7 @@ void newt_model_destroy(newt_model* model) {
     (void)hipFree(model->d_level_nrank);
     (void)hipFree(model->d_dof_link);
     (void)hipFree(model->d_limits);
+    (void)hipFree(model->d_bodies);
     (void)hipFree(model->d_blob);
     delete model;
 }
@@ -309,24 +362,27 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
     if (precision != NEWT_PRECISION_FP64 && precision != NEWT_PRECISION_MIXED &&
         precision != NEWT_PRECISION_FP64_EXACT)
         return fail(NEWT_ERR_INVALID_ARG, "unknown precision");
-    if ((model->dev.nv > 64 || model->dev.n_links > 128) && precision != NEWT_PRECISION_FP64_EXACT)
+    // Free-body models run the lane-per-world FP64 step in every precision.
+    const bool bodies = model->dev.n_bodies > 0;
+    const bool team = !bodies && precision != NEWT_PRECISION_FP64_EXACT;
+    if ((model->dev.nv > 64 || model->dev.n_links > 128) && team)
         return fail(NEWT_ERR_UNSUPPORTED, "the wavefront-per-world step handles nv <= 64 and <= 128 links; "
                                           "use NEWT_PRECISION_FP64_EXACT for larger trees");
-    const TeamLayout TL = make_team_layout(model->dev, precision == NEWT_PRECISION_MIXED ? 4 : 8);
-    const int cache_bytes = model->cache.bytes;
+    const TeamLayout TL = bodies ? TeamLayout{} : make_team_layout(model->dev, precision == NEWT_PRECISION_MIXED ? 4 : 8);
+    const int cache_bytes = bodies ? 0 : model->cache.bytes;
     const int max_wpb = team_max_worlds_for(model->dev.nv);
-    const int wpb = std::min(max_wpb, (kMaxLds - cache_bytes) / std::max(TL.bytes, 1));
-    if (precision != NEWT_PRECISION_FP64_EXACT && wpb < 1)
+    const int wpb = bodies ? 1 : std::min(max_wpb, (kMaxLds - cache_bytes) / std::max(TL.bytes, 1));
+    if (team && wpb < 1)
         return fail(NEWT_ERR_UNSUPPORTED, "one world's step state exceeds the 160 KiB of LDS of a workgroup");
     newt_data* D = new newt_data();
     D->model = model;
     D->W = n_worlds;
     D->precision = precision;
-    D->L = make_layout(model->dev, precision == NEWT_PRECISION_MIXED ? 4 : 8);
+    D->L = bodies ? make_body_layout(model->dev) : make_layout(model->dev, precision == NEWT_PRECISION_MIXED ? 4 : 8);
     D->TL = TL;
     D->wpb = std::max(wpb, 1);
     D->lds_bytes = cache_bytes + D->wpb * TL.bytes;
-    if (precision != NEWT_PRECISION_FP64_EXACT && D->lds_bytes > 65536) {
+    if (team && D->lds_bytes > 65536) {
         // The attribute is per kernel, shared by every newt_data: allow the
         // whole workgroup LDS so that no data narrows another's launch.
         hipError_t e = team_set_lds(precision == NEWT_PRECISION_MIXED, model->dev.nv, model->dev.ext != 0, kMaxLds);
@@ -355,8 +411,10 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
     }
     D->d.W = n_worlds;
     std::vector<double> q((size_t)W * m.nq), qd((size_t)W * m.nv, 0.0);
-    for (size_t w = 0; w < W; ++w)
+    for (size_t w = 0; w < W; ++w) {
         for (int k = 0; k < m.nq; ++k) q[w * m.nq + k] = model->h.q_default[k];
+        for (int k = 0; k < m.nv; ++k) qd[w * m.nv + k] = model->h.v_default[k];
+    }
     HIP_TRY(hipMemset(D->d.qfrc, 0, W * m.nv * 8));
     HIP_TRY(hipMemset(D->d.qacc, 0, W * m.nv * 8));
     HIP_TRY(hipMemset(D->d.fault, 0, W * 4));

@@ -199,7 +199,22 @@ template <class R> struct World {
             geom_aabb(g, sp, so, &smn, &smx);
             double ang_speed = double(0.0);
             V3<double> pivot = sp;
-            if (g.link >= 0) {
+            if (g.body >= 0) {
+                // a free body's end pose (world.rs:1317-1326): its own
+                // linear velocity and body-frame angular velocity.
+                const BodyT& b = m.bodies[g.body];
+                V3<double> bp = mk3(qpos[b.q_off], qpos[b.q_off + 1], qpos[b.q_off + 2]);
+                Qt<double> bo{qpos[b.q_off + 3], qpos[b.q_off + 4], qpos[b.q_off + 5], qpos[b.q_off + 6]};
+                V3<double> bv = mk3(qvel[b.v_off], qvel[b.v_off + 1], qvel[b.v_off + 2]);
+                V3<double> bw = mk3(qvel[b.v_off + 3], qvel[b.v_off + 4], qvel[b.v_off + 5]);
+                V3<double> ep = bp + bv * dt;
+                Qt<double> eo = qintegrate(bo, bw, dt);
+                V3<double> gp = mk3((double)g.pos[0], (double)g.pos[1], (double)g.pos[2]);
+                Qt<double> gq{(double)g.quat[0], (double)g.quat[1], (double)g.quat[2], (double)g.quat[3]};
+                geom_aabb(g, ep + qrot(eo, gp), qmul(eo, gq), &emn, &emx);
+                ang_speed = len3(qrot(bo, bw));
+                pivot = bp;
+            } else if (g.link >= 0) {
                 V3<double> v = kld3(L.lvel + g.link * 6), w = kld3(L.lvel + g.link * 6 + 3);
                 V3<double> lp = pos(g.link);
                 Qt<double> lo = ori(g.link);

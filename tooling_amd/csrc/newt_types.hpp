@@ -39,12 +39,21 @@ struct LimitT {
 };
 
 struct GeomT {
-    int shape, link;           // link = -1 for a static geom
+    int shape, link;           // pose slot: the link, or n_links + body; -1 for a static geom
     int condim, max_contacts;
     double size[3], pos[3], quat[4];
     double friction, margin, gap;
     double solref[2], solimp[4];
-    int power, pad0;
+    int power, body;           // body = the free body it rides on, -1 for none
+};
+
+// A free Body (body.rs:22-47).  Inertia and its inverse row-major m[r*3+c];
+// its state is qpos[q_off..+7) = (position, orientation x y z w) and
+// qvel[v_off..+6) = (linear velocity in world, angular velocity in body).
+struct BodyT {
+    double mass, gravity_scale, max_lin, max_ang;
+    double inertia[9], inv_inertia[9];
+    int has_max_lin, has_max_ang, q_off, v_off;
 };
 
 struct PairT {
@@ -73,7 +82,8 @@ struct ModelT {
     int n_ranged;  // hinges/slides with a range
     const LimitT* limits;
     int ext;       // needs the extended persistent step: limits, RK4 or non-plane pairs
-    int pad1;
+    int n_bodies;  // free bodies (a model has a tree or free bodies, not both)
+    const BodyT* bodies;
 };
 
 // Offsets (in elements) of the per-world scratch regions.
@@ -88,10 +98,16 @@ struct Layout {
     // joint-limit rows (up to n_ranged): force delta [nv], M⁻¹ e_i, A, and
     // per-row bias, diagonal, impulse, coefficient, violation
     int lim, ldv, lA, lb, ldg, lf, lc, lvi;
+    // free-body solve: per body the wrench (6), world inverse inertia (9)
+    // and impulse accumulator (6); per row dir (3), reg, diag, bias,
+    // impulse; per contact block the two arms (6) and the two facet-pair
+    // cross responses
+    int bwr, biw, bbd, brow, bblk;
     int total_d;
     // integer workspace
     int cids, rinfo, pairs, iwf;
     int lsl, llm;  // limit rows: dof slot, limit index
+    int bbi;       // free-body contact blocks: body_a, body_b
     int total_i;
 };
 

@@ -61,14 +61,28 @@ def link(parent, joint, *, axis=(0.0, 0.0, 0.0), parent_pos=(0.0, 0.0, 0.0), par
                 limit_solimp=limit_solimp)
 
 
-def geom(shape, *, tree=-1, link=-1, size=(0.0, 0.0, 0.0), pos=(0.0, 0.0, 0.0), quat=IDENTITY_Q,
+def geom(shape, *, tree=-1, link=-1, body=-1, size=(0.0, 0.0, 0.0), pos=(0.0, 0.0, 0.0), quat=IDENTITY_Q,
          friction=1.0, margin=0.0, gap=0.0, solref=(0.02, 1.0), solimp=(0.9, 0.95, 0.001, 0.5), power=2,
          condim=3, group=0xFFFFFFFF, mask=0xFFFFFFFF):
-    """Geom defaults follow Geom::static_plane / *_on_link (geom.rs:465-700);
-    `group`/`mask` are collision_group / collision_mask (broadphase.rs:16-18)."""
-    return dict(shape=shape, tree=tree, link=link, size=size, pos=pos, quat=quat, friction=friction,
+    """Geom defaults follow Geom::static_plane / *_on_link / r#box (geom.rs:465-700);
+    `body` attaches it to a free body; `group`/`mask` are collision_group /
+    collision_mask (broadphase.rs:16-18)."""
+    return dict(shape=shape, tree=tree, link=link, body=body, size=size, pos=pos, quat=quat, friction=friction,
                 margin=margin, gap=gap, solref=solref, solimp=solimp, power=power, condim=condim, group=group,
                 mask=mask)
+
+
+def body(mass, inertia, position, orientation=IDENTITY_Q, linear_velocity=(0.0, 0.0, 0.0),
+         angular_velocity_body=(0.0, 0.0, 0.0), max_linear_velocity=None, max_angular_velocity=None,
+         gravity_scale=1.0):
+    """A free Body (body.rs:22-70); `inertia` is a column-major 3x3 about
+    the COM in body axes."""
+    return dict(mass=mass, inertia=tuple(inertia), position=tuple(position), orientation=tuple(orientation),
+                linear_velocity=tuple(linear_velocity), angular_velocity_body=tuple(angular_velocity_body),
+                max_linear_velocity=max_linear_velocity, max_angular_velocity=max_angular_velocity,
+                gravity_scale=gravity_scale)
+
+
 
 
 @dataclass
@@ -81,16 +95,17 @@ class Scene:
     sampler: Callable[[int, int], tuple] = None
     description: str = ""
     extra: dict = field(default_factory=dict)
+    bodies: List[dict] = field(default_factory=list)
 
     @property
     def nq(self):
         return sum(7 if l["joint"] == abi.JOINT_FREE else 0 if l["joint"] == abi.JOINT_FIXED
-                   else 4 if l["joint"] == abi.JOINT_BALL else 1 for l in self.links)
+                   else 4 if l["joint"] == abi.JOINT_BALL else 1 for l in self.links) + 7 * len(self.bodies)
 
     @property
     def nv(self):
         return sum(6 if l["joint"] == abi.JOINT_FREE else 0 if l["joint"] == abi.JOINT_FIXED
-                   else 3 if l["joint"] == abi.JOINT_BALL else 1 for l in self.links)
+                   else 3 if l["joint"] == abi.JOINT_BALL else 1 for l in self.links) + 6 * len(self.bodies)
 
     def initial_state(self, n_worlds: int, seed: int):
         return self.sampler(n_worlds, seed)
@@ -151,6 +166,26 @@ class Scene:
             d.solimp_power = g["power"]
             d.group = g.get("group", 0xFFFFFFFF)
             d.mask = g.get("mask", 0xFFFFFFFF)
+            d.body = g.get("body", -1)
+            out.append(d)
+        return out
+
+    def body_structs(self):
+        out = []
+        for b in self.bodies:
+            d = abi.BodyDesc()
+            d.mass = b["mass"]
+            d.inertia = (abi.C.c_double * 9)(*b["inertia"])
+            d.position = abi.D3(*b["position"])
+            d.orientation = abi.D4(*b["orientation"])
+            d.linear_velocity = abi.D3(*b["linear_velocity"])
+            d.angular_velocity_body = abi.D3(*b["angular_velocity_body"])
+            ml, ma = b.get("max_linear_velocity"), b.get("max_angular_velocity")
+            d.has_max_linear_velocity = 0 if ml is None else 1
+            d.has_max_angular_velocity = 0 if ma is None else 1
+            d.max_linear_velocity = 0.0 if ml is None else ml
+            d.max_angular_velocity = 0.0 if ma is None else ma
+            d.gravity_scale = b.get("gravity_scale", 1.0)
             out.append(d)
         return out
 
@@ -399,8 +434,89 @@ def with_options(sc: Scene, suffix: str, **options) -> Scene:
     o = dict(sc.options)
     o.update(options)
     return Scene(sc.name + suffix, o, sc.links, sc.geoms, sc.self_collide, sc.sampler,
-                 f"{sc.description}; options {options}")
+                 f"{sc.description}; options {options}", dict(sc.extra), list(sc.bodies))
+
+
+def _body_state(bodies):
+    q = np.array([[*b["position"], *b["orientation"]] for b in bodies], dtype=np.float64).reshape(-1)
+    v = np.array([[*b["linear_velocity"], *b["angular_velocity_body"]] for b in bodies], dtype=np.float64).reshape(-1)
+    return q, v
+
+
+def box_stack(n_boxes=3, dt=0.005, iterations=30, integrator=abi.INTEGRATOR_RK4) -> Scene:
+    """newt's solver golden stack (newt/tests/solver_golden.rs:27-46): boxes of
+    half extent 0.25 and 1 kg stacked on a plane with 1 cm x offsets per
+    level, friction 0.8, SolRef (0.05, 1.5); PGS, pyramidal cone.  Worlds
+    differ by a small random lateral velocity per box."""
+    half = (0.25, 0.25, 0.25)
+    bodies = [body(1.0, solid_box_inertia(1.0, half), (0.01 * k, 0.0, 0.25 + 0.5 * k)) for k in range(n_boxes)]
+    geoms = [geom(abi.SHAPE_PLANE, friction=0.8, solref=(0.05, 1.5))]
+    geoms += [geom(abi.SHAPE_BOX, body=k, size=half, friction=0.8, solref=(0.05, 1.5)) for k in range(n_boxes)]
+    q0, v0 = _body_state(bodies)
+
+    def sampler(n, seed):
+        r = _rng(seed)
+        q = np.tile(q0, (n, 1))
+        v = np.tile(v0, (n, 1))
+        for k in range(n_boxes):
+            v[:, 6 * k:6 * k + 2] = r.normal(0.0, 0.05, size=(n, 2))
+        return q, v
+
+    opts = dict(dt=dt, gravity=(0.0, 0.0, -9.81), integrator=integrator, solver=abi.SOLVER_PGS,
+                iterations=iterations)
+    return Scene(f"boxstack{n_boxes}", opts, [], geoms, 0, sampler,
+                 f"{n_boxes} stacked free boxes on a plane, PGS {iterations}", bodies=bodies)
+
+
+def random_quaternions(r, n):
+    """Uniform random unit quaternions (x, y, z, w), Shoemake's method."""
+    u1, u2, u3 = r.random(n), r.random(n), r.random(n)
+    a, b = np.sqrt(1.0 - u1), np.sqrt(u1)
+    return np.stack([a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2), b * np.sin(2 * np.pi * u3),
+                     b * np.cos(2 * np.pi * u3)], axis=1)
+
+
+def box_pile(n_side=4, dt=0.002, iterations=20, seed=2026, friction=0.8) -> Scene:
+    """n_side^3 free boxes dropped onto a plane: half extents ~ U[0.05, 0.2]^3
+    (one draw, shared by every world), density 1000 (mass = 8 hx hy hz 1000),
+    a jittered n_side^3 grid at 0.5 m spacing from 0.35 m up, uniform random
+    orientations per world; PGS, pyramidal cone, Euler.  At n_side 4: 64
+    boxes, 65 geoms, 2080 candidate pairs."""
+    r = _rng(seed)
+    nb = n_side ** 3
+    halves = r.uniform(0.05, 0.2, size=(nb, 3))
+    bodies, geoms = [], [geom(abi.SHAPE_PLANE, friction=friction)]
+    grid = []
+    for i in range(n_side):
+        for j in range(n_side):
+            for k in range(n_side):
+                grid.append(((i - (n_side - 1) / 2) * 0.5, (j - (n_side - 1) / 2) * 0.5, 0.35 + 0.5 * k))
+    for b in range(nb):
+        h = tuple(float(x) for x in halves[b])
+        mass = 8.0 * h[0] * h[1] * h[2] * 1000.0
+        bodies.append(body(mass, solid_box_inertia(mass, h), grid[b]))
+        geoms.append(geom(abi.SHAPE_BOX, body=b, size=h, friction=friction))
+    grid = np.array(grid)
+
+    def sampler(n, s):
+        rr = _rng(s)
+        q = np.zeros((n, 7 * nb))
+        v = np.zeros((n, 6 * nb))
+        for b in range(nb):
+            q[:, 7 * b:7 * b + 3] = grid[b] + rr.uniform(-0.04, 0.04, size=(n, 3))
+            q[:, 7 * b + 3:7 * b + 7] = random_quaternions(rr, n)
+        return q, v
+
+    q0, _ = sampler(1, seed)
+    for b in range(nb):
+        bodies[b]["position"] = tuple(q0[0, 7 * b:7 * b + 3])
+        bodies[b]["orientation"] = tuple(q0[0, 7 * b + 3:7 * b + 7])
+    opts = dict(dt=dt, gravity=(0.0, 0.0, -9.81), integrator=abi.INTEGRATOR_EULER, solver=abi.SOLVER_PGS,
+                iterations=iterations)
+    return Scene(f"boxpile{nb}", opts, [], geoms, 0, sampler,
+                 f"{nb} free boxes dropped on a plane, PGS {iterations}, Euler dt {dt}", bodies=bodies)
 
 
 SCENES = {"pendulum3": pendulum, "quadruped18": quadruped, "boxchain6": box_chain, "coil10": hinge_coil,
-          "chain40": lambda: hinge_chain(40), "chain64": lambda: hinge_chain(64)}
+          "chain40": lambda: hinge_chain(40), "chain64": lambda: hinge_chain(64), "boxstack3": box_stack,
+          "boxpile64": box_pile}
