@@ -8,6 +8,9 @@ bit for bit.  The oracle's float instantiation reproduces newt's own golden
 files for these paths byte for byte (test_oracle_golden.py: tumbling_3_body,
 integrator_*, solver_stack, solver_incline).
 
+FP64_EXACT runs the lane-per-world kernel, FP64 and MIXED the
+wavefront-per-world kernel.
+
 Scenes: three tumbling boxes with no contacts (newt/tests/golden.rs) with
 speed caps and a gravity scale; newt's golden stack of three boxes under
 RK4 and Euler; 64 boxes dropped on a plane (box-box and box-plane
@@ -111,10 +114,14 @@ def test_golden_stack(oracle, integrator, precision):
     assert np.all(out[0][:, 7 * 2 + 2] > 1.1)
 
 
-def test_box_pile(oracle):
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP64_EXACT, abi.PRECISION_FP64])
+def test_box_pile(oracle, precision):
+    """FP64_EXACT runs the lane-per-world step, FP64 the wavefront-per-world
+    step (rows in LDS, or in global memory for worlds with more active
+    contacts than fit): both bit for bit."""
     sc = scenes.box_pile()
-    say(f"{sc.name}: 32 worlds x 300 steps (drop and first impacts)")
-    out, ref = run_pair(oracle, sc, 32, 300, abi.PRECISION_FP64_EXACT, seed=5, chunks=3)
+    say(f"{sc.name} precision {precision}: 32 worlds x 400 steps (drop, impacts, piling)")
+    out, ref = run_pair(oracle, sc, 32, 400, precision, seed=5, chunks=4)
     check_exact(out, ref, sc.name)
 
 

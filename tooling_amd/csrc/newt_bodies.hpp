@@ -418,3 +418,693 @@ struct FreeBodies {
 };
 
 }  // namespace newt
+
+// ---------------------------------------------------------------------------
+// The same step with one wavefront per world.  The body state, world inverse
+// inertias, impulse accumulators, wrenches and fat bounds of the world live
+// in LDS; so do the contact rows when the world's active contacts fit the
+// LDS row capacity, else they go to a per-world region of global memory (the
+// same code through a flat pointer).  Candidate pairs, narrowphase, row
+// assembly, wrench accumulation, integration and the broadphase refresh run
+// one lane per pair / contact / body / geom; the Gauss-Seidel sweeps run on
+// lane 0 in the reference's order.  Every value is computed by the same
+// operations as FreeBodies above, so both match the FP64 oracle bit for bit.
+// ---------------------------------------------------------------------------
+namespace newt {
+
+// Offsets (elements) of one world's LDS block and its global fallback.
+struct BodyWaveLayout {
+    int st, iiw, bd, wr, fat, rows, blk;  // doubles
+    int ints_at;                          // doubles before the int region
+    int bbi, misc;                        // ints, after the doubles
+    int lds_bytes;                        // bytes per world
+    int lds_contacts;                     // active contacts whose rows fit in LDS
+    int g_rows, g_blk, g_stride_d;        // global fallback, doubles per world
+    int g_bbi, g_stride_i;                // ints per world
+};
+
+inline BodyWaveLayout make_body_wave_layout(const ModelT& m, int lds_budget_bytes) {
+    BodyWaveLayout B{};
+    const int nb = m.n_bodies, ng = m.n_geoms;
+    const int fixed_d = 34 * nb + 6 * ng;
+    int lc = (lds_budget_bytes - 8 * fixed_d - 4 * 8) / (8 * 36 + 4 * 2);
+    lc = std::max(0, std::min(lc, m.cap_c));
+    int d = 0;
+    auto D = [&](int n) { int o = d; d += n; return o; };
+    B.st = D(13 * nb);
+    B.iiw = D(9 * nb);
+    B.bd = D(6 * nb);
+    B.wr = D(6 * nb);
+    B.fat = D(6 * ng);
+    B.rows = D(28 * lc);
+    B.blk = D(8 * lc);
+    B.ints_at = d;
+    int i = 0;
+    auto I = [&](int n) { int o = i; i += n; return o; };
+    B.bbi = I(2 * lc);
+    B.misc = I(8);
+    B.lds_contacts = lc;
+    B.lds_bytes = 8 * d + 4 * i;
+    B.g_rows = 0;
+    B.g_blk = 28 * m.cap_c;
+    B.g_stride_d = 36 * m.cap_c;
+    B.g_bbi = 0;
+    B.g_stride_i = 2 * m.cap_c;
+    return B;
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// The Gauss-Seidel sweeps of solve_free_bodies (solver.rs:1066-1170) over
+// nblk pyramidal condim-3 blocks, on one lane: pgs_step_non_negative and two
+// pgs_step_pyramidal_pair per block, body impulse accumulators in bd.  Row
+// ri: dir, reg, diag, bias, impulse (7 doubles); block k: arm_a, arm_b, h01,
+// h23 (8 doubles) and its bodies in bbi.  Every pointer is a parameter so
+// the compiler keeps them in registers (the regions do not overlap).
+__device__ __noinline__ void body_pgs_sweeps(int iterations, int nblk, double* __restrict__ rows,
+                                             const double* __restrict__ blk, const int* __restrict__ bbi,
+                                             double* __restrict__ bd, const double* __restrict__ iiw,
+                                             const double* __restrict__ st, const BodyT* __restrict__ bodies) {
+    auto ld3 = [](const double* p) { return mk3(p[0], p[1], p[2]); };
+    auto ori = [&](int b) {
+        const double* p = st + 13 * b + 3;
+        return Qt<double>{p[0], p[1], p[2], p[3]};
+    };
+    auto residual = [&](int ri) {
+        const int k = ri >> 2;
+        const int a = bbi[2 * k], b = bbi[2 * k + 1];
+        const V3<double> dir = ld3(rows + 7 * ri);
+        double s = 0.0;
+        if (a >= 0) {
+            V3<double> wa = qrot(ori(a), ld3(bd + 6 * a + 3));
+            V3<double> vp = ld3(bd + 6 * a) + cross3(wa, ld3(blk + 8 * k));
+            s += dot3(vp, dir);
+        }
+        if (b >= 0) {
+            V3<double> wb = qrot(ori(b), ld3(bd + 6 * b + 3));
+            V3<double> vp = ld3(bd + 6 * b) + cross3(wb, ld3(blk + 8 * k + 3));
+            s -= dot3(vp, dir);
+        }
+        return s;
+    };
+    auto side = [&](int i, V3<double> dir, V3<double> arm, double df) {
+        double* p = bd + 6 * i;
+        V3<double> f = dir * df;
+        V3<double> tq = cross3(arm, f);
+        M33<double> w3;
+        for (int k = 0; k < 9; ++k) w3.m[k] = iiw[9 * i + k];
+        V3<double> dv = ld3(p) + f / bodies[i].mass;
+        V3<double> dw = ld3(p + 3) + qrot_inv(ori(i), mv(w3, tq));
+        p[0] = dv.x; p[1] = dv.y; p[2] = dv.z;
+        p[3] = dw.x; p[4] = dw.y; p[5] = dw.z;
+    };
+    auto apply = [&](int ri, double df) {
+        if (df == 0.0) return;
+        const int k = ri >> 2;
+        const int a = bbi[2 * k], b = bbi[2 * k + 1];
+        const V3<double> dir = ld3(rows + 7 * ri);
+        if (a >= 0) side(a, dir, ld3(blk + 8 * k), df);
+        if (b >= 0) side(b, -dir, ld3(blk + 8 * k + 3), df);
+    };
+    auto non_negative = [&](int ri) {
+        double* r = rows + 7 * ri;
+        const double f = r[6];
+        const double res = residual(ri) + r[3] * f + r[5];
+        double delta = -res / r[4];
+        const double nf = f + delta;
+        const double pr = nf < 0.0 ? 0.0 : nf;
+        delta = pr - f;
+        r[6] = pr;
+        apply(ri, delta);
+    };
+    auto pair = [&](int ri, double h01) {
+        const int rj = ri + 1;
+        double* r0 = rows + 7 * ri;
+        double* r1 = rows + 7 * rj;
+        const double fi = r0[6], fj = r1[6];
+        const double res_i = residual(ri) + r0[3] * fi + r0[5];
+        const double res_j = residual(rj) + r1[3] * fj + r1[5];
+        const double h00 = r0[4], h11 = r1[4];
+        const double det = h00 * h11 - h01 * h01;
+        if (det <= 0.0) {
+            non_negative(ri);
+            non_negative(rj);
+            return;
+        }
+        const double di = (-res_i * h11 + h01 * res_j) / det;
+        const double dj = (h01 * res_i - h00 * res_j) / det;
+        const double ni = fi + di, nj = fj + dj;
+        double a, b;
+        if (ni >= 0.0 && nj >= 0.0) {
+            a = ni;
+            b = nj;
+        } else if (ni < 0.0 && nj < 0.0) {
+            a = 0.0;
+            b = 0.0;
+        } else if (ni < 0.0) {
+            a = 0.0;
+            b = nmax(fj - res_j / h11, 0.0);
+        } else {
+            a = nmax(fi - res_i / h00, 0.0);
+            b = 0.0;
+        }
+        r0[6] = a;
+        r1[6] = b;
+        apply(ri, a - fi);
+        apply(rj, b - fj);
+    };
+    for (int it = 0; it < iterations; ++it)
+        for (int k = 0; k < nblk; ++k) {
+            non_negative(4 * k);
+            pair(4 * k, blk[8 * k + 6]);
+            pair(4 * k + 2, blk[8 * k + 7]);
+        }
+}
+
+struct BodyWave {
+    const ModelT& m;
+    const BodyWaveLayout& B;
+    const Layout& L;
+    const DataT& d;
+    double* dws;      // lane-interleaved contact geometry (Layout::cgeo)
+    int* iws;         // lane-interleaved contact ids and pair list
+    double* sd;       // this world's LDS doubles
+    int* si;          // this world's LDS ints
+    double* gd;       // this world's global fallback (doubles)
+    int* gi;          // and ints
+    int world, lane;
+
+    // rows / blocks: LDS or global, chosen per step
+    mutable double* rows;
+    mutable double* blk;
+    mutable int* bbi;
+
+    __device__ __forceinline__ double& cg(int k, int j) const { return dws[(size_t)(L.cgeo + 9 * k + j) * d.W + world]; }
+    __device__ __forceinline__ int& cid(int k, int j) const { return iws[(size_t)(L.cids + 3 * k + j) * d.W + world]; }
+
+    __device__ __forceinline__ const BodyT& body(int b) const { return m.bodies[b]; }
+    __device__ __forceinline__ V3<double> ld3(const double* p) const { return mk3(p[0], p[1], p[2]); }
+    __device__ __forceinline__ void st3(double* p, V3<double> v) const { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+    __device__ __forceinline__ V3<double> bpos(int b) const { return ld3(sd + B.st + 13 * b); }
+    __device__ __forceinline__ Qt<double> bori(int b) const {
+        const double* p = sd + B.st + 13 * b + 3;
+        return Qt<double>{p[0], p[1], p[2], p[3]};
+    }
+    __device__ __forceinline__ V3<double> bvel(int b) const { return ld3(sd + B.st + 13 * b + 7); }
+    __device__ __forceinline__ V3<double> bomega(int b) const { return ld3(sd + B.st + 13 * b + 10); }
+    __device__ __forceinline__ M33<double> iiw(int b) const {
+        M33<double> a;
+        for (int k = 0; k < 9; ++k) a.m[k] = sd[B.iiw + 9 * b + k];
+        return a;
+    }
+    __device__ __forceinline__ V3<double> gravity_of(int b) const {
+        const V3<double> g = mk3(m.gravity[0], m.gravity[1], m.gravity[2]);
+        const double s = body(b).gravity_scale;
+        return s == 1.0 ? g : g * s;
+    }
+
+    // ---------------- state in and out ------------------------------------
+    __device__ __forceinline__ void load() const {
+        const size_t W = d.W;
+        for (int b = lane; b < m.n_bodies; b += 64) {
+            const BodyT& bt = body(b);
+            double* s = sd + B.st + 13 * b;
+            for (int k = 0; k < 7; ++k) s[k] = d.qpos[(size_t)(bt.q_off + k) * W + world];
+            for (int k = 0; k < 6; ++k) s[7 + k] = d.qvel[(size_t)(bt.v_off + k) * W + world];
+        }
+        for (int k = lane; k < 6 * m.n_geoms; k += 64) sd[B.fat + k] = d.fat[(size_t)k * W + world];
+        gsync();
+    }
+    __device__ __forceinline__ void store() const {
+        const size_t W = d.W;
+        for (int b = lane; b < m.n_bodies; b += 64) {
+            const BodyT& bt = body(b);
+            const double* s = sd + B.st + 13 * b;
+            for (int k = 0; k < 7; ++k) d.qpos[(size_t)(bt.q_off + k) * W + world] = s[k];
+            for (int k = 0; k < 6; ++k) d.qvel[(size_t)(bt.v_off + k) * W + world] = s[7 + k];
+        }
+        for (int k = lane; k < 6 * m.n_geoms; k += 64) d.fat[(size_t)k * W + world] = sd[B.fat + k];
+    }
+
+    // ---------------- broadphase (world.rs:1254-1358) ----------------------
+    __device__ __forceinline__ void geom_pose(const GeomT& g, V3<double>* p, Qt<double>* o) const {
+        V3<double> gp = mk3(g.pos[0], g.pos[1], g.pos[2]);
+        Qt<double> gq{g.quat[0], g.quat[1], g.quat[2], g.quat[3]};
+        if (g.body < 0) {
+            *p = gp;
+            *o = gq;
+            return;
+        }
+        Qt<double> bo = bori(g.body);
+        *p = bpos(g.body) + qrot(bo, gp);
+        *o = qmul(bo, gq);
+    }
+    __device__ __forceinline__ void populate(bool insert_all) const {
+        const double dt = m.dt;
+        for (int gi0 = lane; gi0 < m.n_geoms; gi0 += 64) {
+            const GeomT& g = m.geoms[gi0];
+            V3<double> sp, smn, smx, emn, emx;
+            Qt<double> so;
+            geom_pose(g, &sp, &so);
+            World<double>::geom_aabb_of(g, sp, so, &smn, &smx);
+            double ang_speed = 0.0;
+            V3<double> pivot = sp;
+            if (g.body >= 0) {
+                const V3<double> bp = bpos(g.body), bv = bvel(g.body), bw = bomega(g.body);
+                const Qt<double> bo = bori(g.body);
+                V3<double> ep = bp + bv * dt;
+                Qt<double> eo = qintegrate(bo, bw, dt);
+                V3<double> gp = mk3(g.pos[0], g.pos[1], g.pos[2]);
+                Qt<double> gq{g.quat[0], g.quat[1], g.quat[2], g.quat[3]};
+                World<double>::geom_aabb_of(g, ep + qrot(eo, gp), qmul(eo, gq), &emn, &emx);
+                ang_speed = len3(qrot(bo, bw));
+                pivot = bp;
+            } else {
+                emn = smn;
+                emx = smx;
+            }
+            V3<double> bc = (smn + smx) * 0.5;
+            double br = len3((smx - smn) * 0.5);
+            double pr = len3(bc - pivot) + br;
+            double sweep = (ang_speed > 0.0 && nfinite(pr)) ? pr * ang_speed * dt : 0.0;
+            V3<double> mn = mk3(nmin(smn.x, emn.x), nmin(smn.y, emn.y), nmin(smn.z, emn.z));
+            V3<double> mx = mk3(nmax(smx.x, emx.x), nmax(smx.y, emx.y), nmax(smx.z, emx.z));
+            V3<double> s3 = mk3(sweep, sweep, sweep);
+            mn = mn - s3;
+            mx = mx + s3;
+            V3<double> m3 = mk3(g.margin, g.margin, g.margin);
+            mn = mn - m3;
+            mx = mx + m3;
+            double* f = sd + B.fat + 6 * gi0;
+            bool inside = f[0] <= mn.x && f[1] <= mn.y && f[2] <= mn.z && f[3] >= mx.x && f[4] >= mx.y && f[5] >= mx.z;
+            if (!inside || insert_all) {
+                V3<double> e = mx - mn;
+                V3<double> fm = mk3(nmax(e.x * 0.1, 1.0e-4), nmax(e.y * 0.1, 1.0e-4), nmax(e.z * 0.1, 1.0e-4));
+                V3<double> a = mn - fm, b = mx + fm;
+                f[0] = a.x; f[1] = a.y; f[2] = a.z;
+                f[3] = b.x; f[4] = b.y; f[5] = b.z;
+            }
+        }
+        gsync();
+    }
+
+    // ---------------- pairs + narrowphase (contact.rs:269-403, 569-1196) --
+    // Contacts of candidate pair k into (cp, nrm, pen); -1 when the fat
+    // bounds do not overlap.  Same operations as World::narrowphase.
+    __device__ __forceinline__ int pair_contacts(int k, V3<double>* cp, V3<double>* nrm, double* pen, double* fr_out,
+                                                 double* gap_out) const {
+        const PairT& pr = m.pairs[k];
+        const double* fa = sd + B.fat + 6 * pr.a;
+        const double* fb = sd + B.fat + 6 * pr.b;
+        bool ov = fa[0] <= fb[3] && fa[3] >= fb[0] && fa[1] <= fb[4] && fa[4] >= fb[1] && fa[2] <= fb[5] &&
+                  fa[5] >= fb[2];
+        if (!ov) return -1;
+        const GeomT& ga = m.geoms[pr.a];
+        const GeomT& gb = m.geoms[pr.b];
+        *fr_out = ga.friction < gb.friction ? ga.friction : gb.friction;
+        const double margin = ga.margin > gb.margin ? ga.margin : gb.margin;
+        *gap_out = ga.gap > gb.gap ? ga.gap : gb.gap;
+        if (ga.shape == SH_BOX && gb.shape == SH_BOX) {
+            V3<double> pa, pb, n;
+            Qt<double> qa, qb;
+            geom_pose(ga, &pa, &qa);
+            geom_pose(gb, &pb, &qb);
+            const int c = box_box(pa, qa, ga.size, pb, qb, gb.size, margin, cp, pen, &n);
+            for (int i = 0; i < c; ++i) nrm[i] = n;
+            return c;
+        }
+        if (ga.shape != SH_PLANE && gb.shape != SH_PLANE) {
+            V3<double> pa, pb;
+            Qt<double> qa, qb;
+            geom_pose(ga, &pa, &qa);
+            geom_pose(gb, &pb, &qb);
+            return round_pair(ga.shape == SH_CAPSULE, pa, qa, ga.size, gb.shape == SH_CAPSULE, pb, qb, gb.size, margin,
+                              &cp[0], &nrm[0], &pen[0])
+                       ? 1
+                       : 0;
+        }
+        const bool flip = gb.shape != SH_PLANE;
+        const GeomT& gp = flip ? gb : ga;
+        const GeomT& gq = flip ? ga : gb;
+        V3<double> pp, qp;
+        Qt<double> po, qo;
+        geom_pose(gq, &pp, &po);
+        geom_pose(gp, &qp, &qo);
+        const V3<double> n = qrot(po, mk3(0.0, 0.0, 1.0));
+        const V3<double> nn = flip ? -n : n;
+        int cnt = 0;
+        if (gp.shape == SH_SPHERE) {
+            const double radius = gp.size[0];
+            const double raw = dot3(qp - pp, n) - radius;
+            if (raw <= margin) {
+                cp[cnt] = qp - n * (radius + raw * 0.5);
+                nrm[cnt] = nn;
+                pen[cnt++] = margin - raw;
+            }
+        } else if (gp.shape == SH_BOX) {
+            const double dist = dot3(qp - pp, n);
+            for (int i = 0; i < 8; ++i) {
+                V3<double> loc = mk3((i & 1) == 0 ? -gp.size[0] : gp.size[0], (i & 2) == 0 ? -gp.size[1] : gp.size[1],
+                                     (i & 4) == 0 ? -gp.size[2] : gp.size[2]);
+                V3<double> co = qrot(qo, loc);
+                double ld = dot3(n, co);
+                double raw = dist + ld;
+                if (raw > margin || ld > 0.0) continue;
+                if (cnt >= 4) continue;
+                cp[cnt] = (qp + co) - n * (raw * 0.5);
+                nrm[cnt] = nn;
+                pen[cnt++] = margin - raw;
+            }
+        } else if (gp.shape == SH_CAPSULE) {
+            const double radius = gp.size[0], hh = gp.size[1];
+            const V3<double> aw = qrot(qo, mk3(0.0, 0.0, 1.0));
+            for (int e = 0; e < 2; ++e) {
+                V3<double> end = e == 0 ? qp + aw * hh : qp - aw * hh;
+                double raw = dot3(end - pp, n) - radius;
+                if (raw <= margin) {
+                    cp[cnt] = end - n * (radius + raw * 0.5);
+                    nrm[cnt] = nn;
+                    pen[cnt++] = margin - raw;
+                }
+            }
+        }
+        return cnt;
+    }
+    // Contacts of this step in pair order into Layout::cids/cgeo; active
+    // pair indices into Layout::pairs.  Returns the contact count (cap_c + 1
+    // when the world holds more than the compiled capacity).
+    __device__ __forceinline__ int contacts(int* np_out) const {
+        int nc = 0, np = 0;
+        bool over = false;
+        for (int base = 0; base < m.n_pairs; base += 64) {
+            const int k = base + lane;
+            V3<double> cp[4], nrm[4];
+            double pen[4], fr = 0.0, gap = 0.0;
+            int c = -1;
+            if (k < m.n_pairs) c = pair_contacts(k, cp, nrm, pen, &fr, &gap);
+            const unsigned long long act = __ballot(c >= 0);
+            if (c >= 0) iws[(size_t)(L.pairs + np + __popcll(act & lanes_below(lane))) * d.W + world] = k;
+            np += __popcll(act);
+            const int cc = c > 0 ? c : 0;
+            const int incl = wave_incl_scan(cc, lane);
+            const int total = __shfl(incl, 63, 64);
+            const int off = nc + incl - cc;
+            if (off + cc <= m.cap_c) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (i >= cc) break;
+                    const int o = off + i;
+                    cid(o, 0) = m.pairs[k].a;
+                    cid(o, 1) = m.pairs[k].b;
+                    cid(o, 2) = i;
+                    cg(o, 0) = cp[i].x; cg(o, 1) = cp[i].y; cg(o, 2) = cp[i].z;
+                    cg(o, 3) = nrm[i].x; cg(o, 4) = nrm[i].y; cg(o, 5) = nrm[i].z;
+                    cg(o, 6) = pen[i];
+                    cg(o, 7) = fr;
+                    cg(o, 8) = gap;
+                }
+            }
+            nc += total;
+            if (nc > m.cap_c) over = true;
+        }
+        gsync();
+        *np_out = np;
+        return over ? m.cap_c + 1 : nc;
+    }
+
+    // ---------------- rows (solver.rs:1892-2069, 2131-2154, 4012-4036) -----
+    __device__ __forceinline__ double residual_of(V3<double> dir, int a, int b, V3<double> arma, V3<double> armb,
+                                                  V3<double> dva, V3<double> dwa, V3<double> dvb, V3<double> dwb) const {
+        double s = 0.0;
+        if (a >= 0) {
+            V3<double> wa = qrot(bori(a), dwa);
+            V3<double> vp = dva + cross3(wa, arma);
+            s += dot3(vp, dir);
+        }
+        if (b >= 0) {
+            V3<double> wb = qrot(bori(b), dwb);
+            V3<double> vp = dvb + cross3(wb, armb);
+            s -= dot3(vp, dir);
+        }
+        return s;
+    }
+    __device__ __forceinline__ void side_delta(int i, V3<double> dir, V3<double> arm, double df, V3<double>* dv,
+                                               V3<double>* dwv) const {
+        V3<double> f = dir * df;
+        V3<double> tq = cross3(arm, f);
+        *dv = *dv + f / body(i).mass;
+        *dwv = *dwv + qrot_inv(bori(i), mv(iiw(i), tq));
+    }
+    __device__ __forceinline__ V3<double> rdir(int ri) const { return ld3(rows + 7 * ri); }
+
+    // Block k's four rows, arms, bodies and cross responses from contact c.
+    __device__ __forceinline__ void assemble(int k, int c) const {
+        const double dt = m.dt;
+        const GeomT& ga = m.geoms[cid(c, 0)];
+        const GeomT& gb = m.geoms[cid(c, 1)];
+        const int a = ga.body, b = gb.body;
+        const double pen = cg(c, 6) - cg(c, 8);
+        const double mu = ga.friction < gb.friction ? ga.friction : gb.friction;
+        const V3<double> cp = mk3(cg(c, 0), cg(c, 1), cg(c, 2)), n = mk3(cg(c, 3), cg(c, 4), cg(c, 5));
+        const V3<double> ref = nabs(n.z) < 0.9 ? mk3(0.0, 0.0, 1.0) : mk3(1.0, 0.0, 0.0);
+        const V3<double> t1 = unit3(cross3(ref, n));
+        const V3<double> t2 = cross3(n, t1);
+        const V3<double> arma = a >= 0 ? cp - bpos(a) : z3<double>();
+        const V3<double> armb = b >= 0 ? cp - bpos(b) : z3<double>();
+        bbi[2 * k] = a;
+        bbi[2 * k + 1] = b;
+        st3(blk + 8 * k, arma);
+        st3(blk + 8 * k + 3, armb);
+        const V3<double> dirs[4] = {n + t1 * mu, n - t1 * mu, n + t2 * mu, n - t2 * mu};
+        double sr0 = ga.solref[0] <= gb.solref[0] ? ga.solref[0] : gb.solref[0];
+        double sr1 = ga.solref[1] <= gb.solref[1] ? ga.solref[1] : gb.solref[1];
+        if (ga.solref[0] < 0.0 || gb.solref[0] < 0.0) {
+            if (!(ga.solref[0] < 0.0 && gb.solref[0] < 0.0)) {
+                const GeomT& gd = ga.solref[0] < 0.0 ? ga : gb;
+                sr0 = gd.solref[0];
+                sr1 = gd.solref[1];
+            }
+        }
+        const double si4[4] = {ga.solimp[0] < gb.solimp[0] ? ga.solimp[0] : gb.solimp[0],
+                               ga.solimp[1] < gb.solimp[1] ? ga.solimp[1] : gb.solimp[1],
+                               ga.solimp[2] < gb.solimp[2] ? ga.solimp[2] : gb.solimp[2], ga.solimp[3]};
+        const SolImpE se = solimp_effective(si4, ga.power);
+        const double imp = impedance_at(se, -pen);
+        for (int r = 0; r < 4; ++r) {
+            double* row = rows + 7 * (4 * k + r);
+            const V3<double> dir = dirs[r];
+            st3(row, dir);
+            double aii = 0.0;
+            if (a >= 0) {
+                V3<double> rc = cross3(arma, dir);
+                aii += dot3(rc, mv(iiw(a), rc)) + 1.0 / body(a).mass;
+            }
+            if (b >= 0) {
+                V3<double> rc = cross3(armb, dir);
+                aii += dot3(rc, mv(iiw(b), rc)) + 1.0 / body(b).mass;
+            }
+            double reg = 0.0;
+            if (imp > 0.0) {
+                double tran = 0.0;
+                if (a >= 0) tran += 1.0 / body(a).mass;
+                if (b >= 0) tran += 1.0 / body(b).mass;
+                const double base = (1.0 - imp) / imp * (tran * (1.0 + mu * mu));
+                reg = 2.0 * mu * mu * base;
+            }
+            row[3] = reg;
+            row[4] = aii + reg;
+            V3<double> va = z3<double>(), vb = z3<double>();
+            if (a >= 0) va = bvel(a) + cross3(qrot(bori(a), bomega(a)), arma);
+            if (b >= 0) vb = bvel(b) + cross3(qrot(bori(b), bomega(b)), armb);
+            const double vcur = dot3(va - vb, dir);
+            double dvf = 0.0;
+            if (a >= 0) {
+                V3<double> dwz = qrot(bori(a), z3<double>());
+                dvf += dot3(gravity_of(a) * dt + cross3(dwz, arma), dir);
+            }
+            if (b >= 0) {
+                V3<double> dwz = qrot(bori(b), z3<double>());
+                dvf -= dot3(gravity_of(b) * dt + cross3(dwz, armb), dir);
+            }
+            const double aref = reference_accel(-pen, vcur, sr0, sr1, se, dt);
+            row[5] = dvf - aref * dt;
+            row[6] = 0.0;
+        }
+        for (int p = 0; p < 2; ++p) {
+            const V3<double> applied = dirs[2 * p + 1];
+            V3<double> dva = z3<double>(), dwa = z3<double>(), dvb = z3<double>(), dwb = z3<double>();
+            if (a >= 0) side_delta(a, applied, arma, 1.0, &dva, &dwa);
+            if (b >= 0) side_delta(b, -applied, armb, 1.0, &dvb, &dwb);
+            blk[8 * k + 6 + p] = residual_of(dirs[2 * p], a, b, arma, armb, dva, dwa, dvb, dwb);
+        }
+    }
+
+    // solve_free_bodies (solver.rs:703-1257): wrenches into LDS.
+    __device__ __forceinline__ void solve(int nc) const {
+        const int nb = m.n_bodies;
+        for (int i = lane; i < 6 * nb; i += 64) sd[B.wr + i] = 0.0;
+        if (nc == 0 || m.dt <= 0.0) {
+            gsync();
+            return;
+        }
+        for (int i = lane; i < nb; i += 64) {
+            M33<double> r = qmat(bori(i));
+            M33<double> w3 = FreeBodies::mm(FreeBodies::mm(r, FreeBodies::tbl(body(i).inv_inertia)), FreeBodies::tr(r));
+            for (int k = 0; k < 9; ++k) sd[B.iiw + 9 * i + k] = w3.m[k];
+            for (int k = 0; k < 6; ++k) sd[B.bd + 6 * i + k] = 0.0;
+        }
+        // Active contacts (a body side, penetration beyond the gap), in order.
+        int nblk = 0;
+        for (int base = 0; base < nc; base += 64) {
+            const int c = base + lane;
+            bool act = false;
+            if (c < nc) {
+                const int a = m.geoms[cid(c, 0)].body, b = m.geoms[cid(c, 1)].body;
+                act = (a >= 0 || b >= 0) && cg(c, 6) - cg(c, 8) > 0.0;
+            }
+            nblk += __popcll(__ballot(act));
+        }
+        if (nblk <= B.lds_contacts) {
+            rows = sd + B.rows;
+            blk = sd + B.blk;
+            bbi = si + B.bbi;
+        } else {
+            rows = gd + B.g_rows;
+            blk = gd + B.g_blk;
+            bbi = gi + B.g_bbi;
+        }
+        gsync();
+        int k0 = 0;
+        for (int base = 0; base < nc; base += 64) {
+            const int c = base + lane;
+            bool act = false;
+            if (c < nc) {
+                const int a = m.geoms[cid(c, 0)].body, b = m.geoms[cid(c, 1)].body;
+                act = (a >= 0 || b >= 0) && cg(c, 6) - cg(c, 8) > 0.0;
+            }
+            const unsigned long long mask = __ballot(act);
+            if (act) assemble(k0 + __popcll(mask & lanes_below(lane)), c);
+            k0 += __popcll(mask);
+        }
+        gsync();
+        if (nblk == 0) return;
+        if (lane == 0)
+            body_pgs_sweeps(m.iterations, nblk, rows, blk, bbi, sd + B.bd, sd + B.iiw, sd + B.st, m.bodies);
+        gsync();
+        // Wrenches, one lane per body, rows in order (solver.rs:1198-1235).
+        const double dt = m.dt;
+        for (int b0 = lane; b0 < nb; b0 += 64) {
+            V3<double> f = z3<double>(), t = z3<double>();
+            for (int ri = 0; ri < 4 * nblk; ++ri) {
+                const int k = ri >> 2;
+                const int a = bbi[2 * k], b = bbi[2 * k + 1];
+                if (a != b0 && b != b0) continue;
+                const double fi = rows[7 * ri + 6];
+                if (fi == 0.0) continue;
+                const V3<double> fa = rdir(ri) * (fi / dt);
+                if (a == b0) {
+                    f = f + fa;
+                    t = t + cross3(ld3(blk + 8 * k), fa);
+                }
+                if (b == b0) {
+                    const V3<double> fb = -fa;
+                    f = f + fb;
+                    t = t + cross3(ld3(blk + 8 * k + 3), fb);
+                }
+            }
+            st3(sd + B.wr + 6 * b0, f);
+            st3(sd + B.wr + 6 * b0 + 3, t);
+        }
+        gsync();
+    }
+
+    // ---------------- integration (world.rs:1705-1817, 3223-3286) ---------
+    __device__ __forceinline__ FreeBodies::Dv evaluate(int b, const FreeBodies::St& s, V3<double> f, V3<double> tq) const {
+        const BodyT& bt = body(b);
+        FreeBodies::Dv dv;
+        dv.dv = gravity_of(b) + f / bt.mass;
+        const V3<double> iwv = mv(FreeBodies::tbl(bt.inertia), s.om);
+        const V3<double> gyro = -cross3(s.om, iwv);
+        const V3<double> tb = qrot_inv(s.q, tq);
+        dv.dw = mv(FreeBodies::tbl(bt.inv_inertia), tb + gyro);
+        dv.dq = qderiv(s.q, s.om);
+        dv.dp = s.v;
+        return dv;
+    }
+    __device__ __forceinline__ void integrate() const {
+        using FB = FreeBodies;
+        const double dt = m.dt;
+        const size_t W = d.W;
+        for (int b = lane; b < m.n_bodies; b += 64) {
+            const BodyT& bt = body(b);
+            const FB::St s0{bpos(b), bvel(b), bori(b), bomega(b)};
+            const V3<double> f = ld3(sd + B.wr + 6 * b), tq = ld3(sd + B.wr + 6 * b + 3);
+            const FB::Dv k1 = evaluate(b, s0, f, tq);
+            FB::St n;
+            if (m.integrator == NEWT_INTEGRATOR_RK4) {
+                const FB::Dv k2 = evaluate(b, FB::advance(s0, k1, dt * 0.5), f, tq);
+                const FB::Dv k3 = evaluate(b, FB::advance(s0, k2, dt * 0.5), f, tq);
+                const FB::Dv k4 = evaluate(b, FB::advance(s0, k3, dt), f, tq);
+                const double sixth = 1.0 / 6.0;
+                const V3<double> dp = (((k1.dp + k2.dp * 2.0) + k3.dp * 2.0) + k4.dp) * sixth;
+                const V3<double> dv = (((k1.dv + k2.dv * 2.0) + k3.dv * 2.0) + k4.dv) * sixth;
+                const Qt<double> dq =
+                    FB::qsc(FB::qadd(FB::qadd(FB::qadd(k1.dq, FB::qsc(k2.dq, 2.0)), FB::qsc(k3.dq, 2.0)), k4.dq), sixth);
+                const V3<double> dwv = (((k1.dw + k2.dw * 2.0) + k3.dw * 2.0) + k4.dw) * sixth;
+                n.p = s0.p + dp * dt;
+                n.v = FB::clampv(s0.v + dv * dt, bt.has_max_lin, bt.max_lin);
+                n.q = qnormalize(FB::qadd(s0.q, FB::qsc(dq, dt)));
+                n.om = FB::clampv(s0.om + dwv * dt, bt.has_max_ang, bt.max_ang);
+            } else {
+                const V3<double> v = s0.v + k1.dv * dt;
+                const V3<double> om = s0.om + k1.dw * dt;
+                n.p = s0.p + v * dt;
+                n.q = qintegrate(s0.q, om, dt);
+                n.v = FB::clampv(v, bt.has_max_lin, bt.max_lin);
+                n.om = FB::clampv(om, bt.has_max_ang, bt.max_ang);
+            }
+            double* s = sd + B.st + 13 * b;
+            st3(s, n.p);
+            s[3] = n.q.x; s[4] = n.q.y; s[5] = n.q.z; s[6] = n.q.w;
+            st3(s + 7, n.v);
+            st3(s + 10, n.om);
+            const int vo = bt.v_off;
+            d.qacc[(size_t)vo * W + world] = k1.dv.x;
+            d.qacc[(size_t)(vo + 1) * W + world] = k1.dv.y;
+            d.qacc[(size_t)(vo + 2) * W + world] = k1.dv.z;
+            d.qacc[(size_t)(vo + 3) * W + world] = k1.dw.x;
+            d.qacc[(size_t)(vo + 4) * W + world] = k1.dw.y;
+            d.qacc[(size_t)(vo + 5) * W + world] = k1.dw.z;
+        }
+        gsync();
+    }
+
+    // World::step; 0 or a newt_status fault code.  np / nc: pair and
+    // contact counts of the step.
+    __device__ __forceinline__ int step(int* np, int* nc) const {
+        *nc = 0;
+        *np = 0;
+        int c = 0;
+        if (m.mode == NEWT_SOLVER_PGS) {
+            c = contacts(np);
+            if (c > m.cap_c) return 8;  // NEWT_ERR_CAPACITY
+        } else {
+            // active pairs only (penalty models here have no candidate pairs)
+            *np = 0;
+        }
+        *nc = c;
+        solve(c);
+        integrate();
+        populate(false);
+        return 0;
+    }
+};
+
+}  // namespace newt

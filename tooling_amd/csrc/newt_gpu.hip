@@ -45,6 +45,12 @@ newt_status fail(newt_status s, const std::string& msg) {
 
 constexpr int kBlock = 64;  // one wavefront: 64 worlds
 constexpr int kMaxLds = 160 * 1024;  // LDS of one gfx950 workgroup
+constexpr int kBodyWaveLds = 40 * 1024;  // one free-body world: four per CU
+// Free-body batches up to this size run one wavefront per world (parallel
+// broadphase, narrowphase and row assembly; at most four worlds per CU by
+// LDS); larger batches run one lane per world, whose throughput grows with
+// the number of wavefronts.  Both give the same bits.
+constexpr int kBodyWaveMaxWorlds = 2048;
 
 }  // namespace
 
@@ -72,6 +78,9 @@ struct newt_data {
     int precision = 0;
     Layout L{};
     TeamLayout TL{};  // LDS layout of one world in the persistent step
+    BodyWaveLayout BL{};  // free-body models: one world's LDS block in the wavefront step
+    double* bws = nullptr;  // and its per-world global fallback for contact rows
+    int* bwi = nullptr;
     int wpb = 1;      // worlds (wavefronts) per workgroup of the persistent step
     int lds_bytes = 0;
     DataT d{};
@@ -153,6 +162,44 @@ __global__ void __launch_bounds__(kBlock) k_bodies(ModelT m, Layout L, DataT d, 
     d.ncontact[w] = iws[(size_t)(L.iwf + 1) * d.W + w];
 }
 
+// Free-body models, one wavefront (one workgroup) per world.
+__global__ void __launch_bounds__(64) k_bodies_wave(ModelT m, BodyWaveLayout B, Layout L, DataT d, double* dws,
+                                                    int* iws, double* gws, int* gwi, int n_steps, int init) {
+    extern __shared__ double lds_b[];
+    const int world = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (world >= d.W) return;
+    if (d.fault[world] != 0) return;
+    double* sd = lds_b;
+    int* si = reinterpret_cast<int*>(lds_b + B.ints_at);
+    BodyWave bw{m, B, L, d, dws, iws, sd, si, gws + (size_t)world * B.g_stride_d, gwi + (size_t)world * B.g_stride_i,
+                world, lane, nullptr, nullptr, nullptr};
+    bw.load();
+    if (init) {
+        bw.populate(true);
+        bw.store();
+        if (lane == 0) {
+            d.npair[world] = 0;
+            d.ncontact[world] = 0;
+        }
+        return;
+    }
+    bw.populate(false);
+    int np = 0, nc = 0;
+    for (int s = 0; s < n_steps; ++s) {
+        const int st = bw.step(&np, &nc);
+        if (st != 0) {
+            if (lane == 0) d.fault[world] = st;
+            break;
+        }
+    }
+    bw.store();
+    if (lane == 0) {
+        d.npair[world] = np;
+        d.ncontact[world] = nc;
+    }
+}
+
 newt_status launch_persistent(newt_data* D, int n_steps, hipStream_t stream) {
     const newt_model* M = D->model;
     const int wpb = D->wpb;
@@ -168,7 +215,11 @@ newt_status launch(newt_data* D, int n_steps, hipStream_t stream, int init) {
     D->last_stream = stream;
     dim3 grid((D->W + kBlock - 1) / kBlock), block(kBlock);
     if (M->dev.n_bodies > 0) {
-        hipLaunchKernelGGL(k_bodies, grid, block, 0, stream, M->dev, D->L, D->d, D->dws, D->iws, n_steps, init);
+        if (D->precision == NEWT_PRECISION_FP64_EXACT || !D->bws || D->W > kBodyWaveMaxWorlds)
+            hipLaunchKernelGGL(k_bodies, grid, block, 0, stream, M->dev, D->L, D->d, D->dws, D->iws, n_steps, init);
+        else
+            hipLaunchKernelGGL(k_bodies_wave, dim3(D->W), dim3(64), D->BL.lds_bytes, stream, M->dev, D->BL, D->L, D->d,
+                               D->dws, D->iws, D->bws, D->bwi, n_steps, init);
         HIP_TRY(hipGetLastError());
         return NEWT_OK;
     }
@@ -405,6 +456,18 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
               alloc(&D->ws, W * D->L.total_r * rsz) && alloc((void**)&D->dws, W * D->L.total_d * 8) &&
               alloc((void**)&D->iws, W * D->L.total_i * 4) &&
               alloc((void**)&D->ovf, W * (size_t)std::max(TL.ov_stride, 16));
+    if (ok && bodies && precision != NEWT_PRECISION_FP64_EXACT && n_worlds <= kBodyWaveMaxWorlds) {
+        // The wavefront step keeps a world's bodies and fat bounds in LDS;
+        // models whose bodies do not fit run the lane step.
+        D->BL = make_body_wave_layout(m, kBodyWaveLds);
+        if (8 * (34 * m.n_bodies + 6 * m.n_geoms) + 64 <= kBodyWaveLds) {
+            ok = alloc((void**)&D->bws, W * (size_t)D->BL.g_stride_d * 8) &&
+                 alloc((void**)&D->bwi, W * (size_t)D->BL.g_stride_i * 4);
+            if (ok && D->BL.lds_bytes > 65536)
+                ok = hipFuncSetAttribute((const void*)k_bodies_wave, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         D->BL.lds_bytes) == hipSuccess;
+        }
+    }
     if (!ok) {
         newt_data_destroy(D);
         return fail(NEWT_ERR_OOM, "device allocation failed");
@@ -448,6 +511,8 @@ void newt_data_destroy(newt_data* D) {
     (void)hipFree(D->dws);
     (void)hipFree(D->iws);
     (void)hipFree(D->ovf);
+    (void)hipFree(D->bws);
+    (void)hipFree(D->bwi);
     if (D->staged) {
         (void)hipEventSynchronize(D->staged);
         (void)hipEventDestroy(D->staged);

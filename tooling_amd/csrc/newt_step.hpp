@@ -164,6 +164,9 @@ template <class R> struct World {
 
     // geom_aabb (broadphase.rs:708-770) as (min, max).
     NEWT_HD void geom_aabb(const GeomT& g, V3<double> p, Qt<double> o, V3<double>* mn, V3<double>* mx) const {
+        geom_aabb_of(g, p, o, mn, mx);
+    }
+    static NEWT_HD void geom_aabb_of(const GeomT& g, V3<double> p, Qt<double> o, V3<double>* mn, V3<double>* mx) {
         if (g.shape == SH_PLANE) {
             double inf = double(__builtin_inf());
             *mn = mk3(-inf, -inf, -inf);
