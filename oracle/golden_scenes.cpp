@@ -291,11 +291,11 @@ std::vector<uint8_t> integrator_golden(int integrator) {
 }
 
 // newt/tests/solver_golden.rs:27-79 + record (170-190): PGS 30, pyramidal, RK4.
-std::vector<uint8_t> solver_stack() {
+std::vector<uint8_t> solver_stack(int mode = M_PGS) {
     World<F> w;
     w.dt = 0.005f;
     w.gravity = v3(0.0f, 0.0f, -9.81f);
-    w.mode = M_PGS;
+    w.mode = mode;
     w.iterations = 30;
     w.cone = C_PYRAMIDAL;
     SolRef<F> sr{0.05f, 1.5f};
@@ -312,13 +312,13 @@ std::vector<uint8_t> solver_stack() {
     }
     return record_at(w, {50, 200, 1000});
 }
-std::vector<uint8_t> solver_incline() {
+std::vector<uint8_t> solver_incline(int mode = M_PGS) {
     World<F> w;
     w.dt = 0.005f;
     F angle = 32.0f * 3.14159265358979323846f / 180.0f;
     F g = 9.81f;
     w.gravity = v3(g * nsin(angle), 0.0f, -g * ncos(angle));
-    w.mode = M_PGS;
+    w.mode = mode;
     w.iterations = 30;
     w.cone = C_PYRAMIDAL;
     SolRef<F> sr{0.05f, 1.5f};
@@ -348,6 +348,8 @@ extern "C" int oracle_golden_bytes(const char* name, uint8_t* out, int cap) {
     else if (n == "integrator_implicitfast") b = integrator_golden(I_IMPLICITFAST);
     else if (n == "solver_stack") b = solver_stack();
     else if (n == "solver_incline") b = solver_incline();
+    else if (n == "solver_newton_stack") b = solver_stack(M_NEWTON);
+    else if (n == "solver_newton_incline") b = solver_incline(M_NEWTON);
     else return -1;
     if ((int)b.size() > cap) return -2;
     std::memcpy(out, b.data(), b.size());

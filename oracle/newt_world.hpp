@@ -719,13 +719,117 @@ template <class T> struct FreeBodySolve {
     }
 };
 
-// solve_free_bodies (PGS; solver.rs:703-1257) for condim-3 contacts under the
-// pyramidal cone: per-body world wrenches (force, torque).
+// NewtonSystem::solve (newton.rs:275-340) for the free-body system with one
+// NonNegative projection per row (pyramidal condim 3): minimize
+// ½ xᵀ H x + linᵀ x over x >= 0 from x = 0 with Newton directions from one
+// Cholesky factor of H and the exact line search along the projected path
+// (exact_pyramidal_line_search, newton.rs:435-556).  Fails on a non-PD H.
+template <class T> struct NewtonNonNeg {
+    const std::vector<T>& H;
+    const std::vector<T>& lin;
+    int n;
+
+    std::vector<T> mat_vec(const std::vector<T>& v) const {  // newton.rs:603-612
+        std::vector<T> out(n, T(0.0));
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) out[i] += H[(size_t)i * n + j] * v[j];
+        return out;
+    }
+    static T dotv(const std::vector<T>& a, const std::vector<T>& b) {  // Iterator::sum
+        RSum<T> s;
+        for (size_t i = 0; i < a.size(); ++i) s.add(a[i] * b[i]);
+        return s.acc;
+    }
+    T cost(const std::vector<T>& x) const {  // quadratic_cost
+        return T(0.5) * dotv(x, mat_vec(x)) + dotv(lin, x);
+    }
+    std::vector<T> gradient(const std::vector<T>& x) const {
+        std::vector<T> g = mat_vec(x);
+        for (int i = 0; i < n; ++i) g[i] += lin[i];
+        return g;
+    }
+    static void project(std::vector<T>& x) {
+        for (T& v : x)
+            if (v < T(0.0)) v = T(0.0);
+    }
+    std::vector<T> affine(const std::vector<T>& x, const std::vector<T>& d, T alpha) const {
+        std::vector<T> p(n, T(0.0));
+        for (int i = 0; i < n; ++i) p[i] = x[i] + alpha * d[i];
+        project(p);
+        return p;
+    }
+    static void add_root(std::vector<T>& breaks, T value, T slope) {  // newton.rs:558-566
+        if (slope == T(0.0)) return;
+        T alpha = -value / slope;
+        if (alpha > T(0.0) && alpha < T(1.0)) breaks.push_back(alpha);
+    }
+    std::pair<T, T> line_search(const std::vector<T>& x, const std::vector<T>& d, T current) const {
+        std::vector<T> breaks{T(0.0), T(1.0)};
+        for (int i = 0; i < n; ++i) add_root(breaks, x[i], d[i]);
+        std::stable_sort(breaks.begin(), breaks.end(), [](T a, T b) { return a < b; });
+        std::vector<T> dd;  // Vec::dedup_by: drop an element within 1e-7 of the last kept one
+        for (T b : breaks)
+            if (dd.empty() || !(rabs(b - dd.back()) <= T(1e-7))) dd.push_back(b);
+        std::vector<T> cand = dd;
+        for (size_t k = 0; k + 1 < dd.size(); ++k) {
+            T lo = dd[k], hi = dd[k + 1];
+            if (hi - lo <= T(1e-7)) continue;
+            T mid = T(0.5) * (lo + hi);
+            T eps = (hi - lo) * T(1e-4);
+            T la = rmax(mid - eps, lo);
+            T ra = rmin(mid + eps, hi);
+            std::vector<T> L = affine(x, d, la), R = affine(x, d, ra);
+            T span = ra - la;
+            if (span <= T(0.0)) continue;
+            T inv = T(1.0) / span;
+            std::vector<T> slope(n);
+            for (int i = 0; i < n; ++i) slope[i] = (R[i] - L[i]) * inv;
+            std::vector<T> g = gradient(affine(x, d, mid));
+            T den = dotv(slope, mat_vec(slope));
+            T num = dotv(g, slope);
+            cand.push_back(den > T(0.0) ? rmin(rmax(mid - num / den, lo), hi) : mid);
+        }
+        T best_a = T(0.0), best_c = current;
+        for (T a : cand) {
+            T c = cost(affine(x, d, a));
+            if (c < best_c) {
+                best_c = c;
+                best_a = a;
+            }
+        }
+        return {best_a, best_c};
+    }
+    std::vector<T> solve(uint32_t max_iterations, T tol) const {
+        std::vector<T> x(n, T(0.0));
+        if (max_iterations == 0) return x;
+        std::vector<T> f;
+        if (!cholesky(H, n, f)) fail("Newton free-body solve failed: Newton Hessian is not positive definite");
+        T cur = cost(x);
+        for (uint32_t it = 0; it < max_iterations; ++it) {
+            std::vector<T> g = gradient(x);
+            std::vector<T> ng(n);
+            for (int i = 0; i < n; ++i) ng[i] = -g[i];
+            std::vector<T> d = cholesky_solve(f, n, ng);
+            if (dotv(g, d) >= T(0.0)) break;
+            auto [alpha, next] = line_search(x, d, cur);
+            if (alpha <= T(0.0) || cur - next <= tol * (T(1.0) + rabs(cur))) break;
+            for (int i = 0; i < n; ++i) x[i] += alpha * d[i];
+            project(x);
+            cur = next;
+        }
+        return x;
+    }
+};
+
+// solve_free_bodies (PGS, solver.rs:703-1257) and solve_free_bodies_newton
+// (Newton, solver.rs:1295-1407: A from unit impulses, H = A + diag(reg)) for
+// condim-3 contacts under the pyramidal cone: per-body world wrenches
+// (force, torque).
 template <class T>
 std::vector<std::pair<V3<T>, V3<T>>> solve_free_bodies(const std::vector<Body<T>>& bodies,
                                                       const std::vector<Geom<T>>& geoms,
                                                       const std::vector<Contact<T>>& contacts, V3<T> gravity,
-                                                      T dt, int cone, uint32_t iterations) {
+                                                      T dt, int cone, uint32_t iterations, bool newton = false) {
     const size_t nb = bodies.size();
     std::vector<std::pair<V3<T>, V3<T>>> wrench(nb, {vzero<T>(), vzero<T>()});
     if (contacts.empty() || dt <= T(0.0)) return wrench;
@@ -804,12 +908,28 @@ std::vector<std::pair<V3<T>, V3<T>>> solve_free_bodies(const std::vector<Body<T>
         b.h23 = S.cross_response(S.rows[b.start + 2], S.rows[b.start + 3]);
     }
     std::vector<T> f(nr, T(0.0));
-    std::vector<BodyDelta<T>> bd(nb);
-    for (uint32_t it = 0; it < iterations; ++it) {
-        for (const Block& b : blocks) {
-            S.step_non_negative(b.start, f, bd);
-            S.step_pair(b.start, b.h01, f, bd);
-            S.step_pair(b.start + 2, b.h23, f, bd);
+    if (newton) {
+        // solve_free_body_newton_impulses (solver.rs:1295-1351)
+        std::vector<T> A((size_t)nr * nr, T(0.0));
+        for (int j = 0; j < nr; ++j) {
+            std::vector<BodyDelta<T>> resp(nb);
+            S.apply(S.rows[j], T(1.0), resp);
+            for (int i = 0; i < nr; ++i) A[(size_t)i * nr + j] = S.row_residual(S.rows[i], resp);
+        }
+        std::vector<T> H = A;
+        for (int i = 0; i < nr; ++i) H[(size_t)i * nr + i] += S.rows[i].reg;
+        std::vector<T> lin(nr);
+        for (int i = 0; i < nr; ++i) lin[i] = S.rows[i].bias;
+        NewtonNonNeg<T> sys{H, lin, nr};
+        f = sys.solve(iterations < 1 ? 1u : iterations, T(1e-7));
+    } else {
+        std::vector<BodyDelta<T>> bd(nb);
+        for (uint32_t it = 0; it < iterations; ++it) {
+            for (const Block& b : blocks) {
+                S.step_non_negative(b.start, f, bd);
+                S.step_pair(b.start, b.h01, f, bd);
+                S.step_pair(b.start + 2, b.h23, f, bd);
+            }
         }
     }
     for (int ri = 0; ri < nr; ++ri) {
@@ -1308,8 +1428,7 @@ template <class T> struct World {
             last_contacts.insert(last_contacts.end(), fc.begin(), fc.end());
             int ti = integrator == I_RK4 ? -1 : integrator == I_IMPLICITFAST ? 1 : 0;
             tq = solve_tree_contacts(tc, ti);
-            if (mode == M_NEWTON && !fc.empty()) fail("Newton free-body solve is not restated");
-            bw = solve_free_bodies(bodies, geoms, fc, gravity, dt, cone, iterations);
+            bw = solve_free_bodies(bodies, geoms, fc, gravity, dt, cone, iterations, mode == M_NEWTON);
         } else {
             if (!pairs.empty()) fail("penalty contact forces are not restated");
         }

@@ -23,6 +23,10 @@ float instantiation rebuilds the same scene and must reproduce every byte:
                              (newt/tests/solver_golden.rs:27-46)
   solver_incline             free-body PGS friction under tilted gravity
                              (newt/tests/solver_golden.rs:48-79)
+  solver_newton_stack /      the same scenes under the Newton solver: dense
+  solver_newton_incline      Delassus from unit impulses, Cholesky, Newton
+                             directions and the exact projected line search
+                             (newton.rs:275-556; solver_golden.rs:81-91)
 """
 import os
 
@@ -33,7 +37,8 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 @pytest.mark.parametrize("name", ["joints_chain_and_floating", "joints_mixed", "dynamics_mass_bias", "solver_tree",
                                   "tumbling_3_body", "integrator_euler", "integrator_rk4",
-                                  "integrator_implicitfast", "solver_stack", "solver_incline"])
+                                  "integrator_implicitfast", "solver_stack", "solver_incline",
+                                  "solver_newton_stack", "solver_newton_incline"])
 def test_golden_bytes(oracle_lib, name):
     expected = open(os.path.join(GOLDEN, name + ".bin"), "rb").read()
     actual = oracle_lib.golden_bytes(name)
