@@ -98,9 +98,28 @@ def _status(sc):
     return e.value.status
 
 
-def test_newton_free_bodies_rejected():
+def test_newton_with_elliptic_cone_rejected():
+    """SolverConfig::validate (solver.rs:123-131): Newton needs the pyramidal cone."""
+    sc = scenes.with_options(scenes.box_stack(), "_newton_ell", solver=abi.SOLVER_NEWTON, cone=abi.CONE_ELLIPTIC)
+    assert _status(sc) == abi.ERR_INVALID_CONFIG
+
+
+def test_newton_stack_scene_is_newts_golden(oracle_lib):
+    """scenes.box_stack() under the Newton solver, through the C-ABI records,
+    writes newt's solver_newton_stack.bin byte for byte (oracle float)."""
     sc = scenes.with_options(scenes.box_stack(), "_newton", solver=abi.SOLVER_NEWTON)
-    assert _status(sc) == abi.ERR_UNSUPPORTED
+    ow = oracle_lib.OracleWorlds(sc, 1, 1)
+    expected = open(os.path.join(GOLDEN, "solver_newton_stack.bin"), "rb").read()
+    q, v = ow.state()
+    got = [_snapshot(q[0], v[0], 3)]
+    done = 0
+    for mark in (50, 200, 1000):
+        ow.step(mark - done)
+        done = mark
+        q, v = ow.state()
+        got.append(_snapshot(q[0], v[0], 3))
+    ow.close()
+    assert b"".join(got) == expected
 
 
 def test_trees_and_bodies_rejected():

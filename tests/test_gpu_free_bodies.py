@@ -149,3 +149,23 @@ def test_box_pile_contacts_match(oracle):
     d.close()
     m.close()
     ow.close()
+
+
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP64_EXACT, abi.PRECISION_FP64])
+def test_newton_golden_stack(oracle, precision):
+    """The Newton solver (solve_free_bodies_newton, solver.rs:1295-1351;
+    NewtonSystem::solve and the exact line search, newton.rs:275-556) on the
+    golden stack: dense response matrix, Cholesky, Newton directions, line
+    search, bit for bit."""
+    sc = scenes.with_options(scenes.box_stack(), "_newton", solver=abi.SOLVER_NEWTON)
+    say(f"{sc.name} precision {precision}: 32 worlds x 400 steps")
+    out, ref = run_pair(oracle, sc, 32, 400, precision, seed=7, chunks=2)
+    check_exact(out, ref, sc.name)
+
+
+def test_newton_box_pile(oracle):
+    """64 boxes under the Newton solver: a few hundred rows per world."""
+    sc = scenes.with_options(scenes.box_pile(), "_newton", solver=abi.SOLVER_NEWTON)
+    say(f"{sc.name}: 16 worlds x 300 steps")
+    out, ref = run_pair(oracle, sc, 16, 300, abi.PRECISION_FP64, seed=11, chunks=3)
+    check_exact(out, ref, sc.name)

@@ -441,7 +441,16 @@ struct BodyWaveLayout {
     int lds_contacts;                     // active contacts whose rows fit in LDS
     int g_rows, g_blk, g_stride_d;        // global fallback, doubles per world
     int g_bbi, g_stride_i;                // ints per world
+    // Newton solve (global, per world): rows cap nn, then H, L (nn x nn) and
+    // the vectors lin, x, g, d, y, p, pl, pr, slope, hv (nn each) and the
+    // break / candidate lists (2 nn + 4 each)
+    int nn, g_H, g_L, g_vec, g_brk, g_cand;
 };
+
+// Rows of one world's Newton system at most (1024 = 256 contacts): H and its
+// factor take 2 x 8 MiB per world.  A world with more active rows faults
+// with NEWT_ERR_CAPACITY.
+constexpr int kNewtonMaxRows = 1024;
 
 inline BodyWaveLayout make_body_wave_layout(const ModelT& m, int lds_budget_bytes) {
     BodyWaveLayout B{};
@@ -470,6 +479,16 @@ inline BodyWaveLayout make_body_wave_layout(const ModelT& m, int lds_budget_byte
     B.g_stride_d = 36 * m.cap_c;
     B.g_bbi = 0;
     B.g_stride_i = 2 * m.cap_c;
+    if (m.mode == NEWT_SOLVER_NEWTON) {
+        const int nn = std::min(4 * m.cap_c, kNewtonMaxRows);
+        B.nn = nn;
+        B.g_H = B.g_stride_d;
+        B.g_L = B.g_H + nn * nn;
+        B.g_vec = B.g_L + nn * nn;
+        B.g_brk = B.g_vec + 10 * nn;
+        B.g_cand = B.g_brk + 2 * nn + 4;
+        B.g_stride_d = B.g_cand + 2 * nn + 4;
+    }
     return B;
 }
 
@@ -946,13 +965,215 @@ struct BodyWave {
         }
     }
 
-    // solve_free_bodies (solver.rs:703-1257): wrenches into LDS.
-    __device__ __forceinline__ void solve(int nc) const {
+    // ---------------- Newton (solver.rs:1295-1351, newton.rs:275-556) ------
+    // Sequential sum a_0 b_0 + a_1 b_1 + ... from -0.0 (Iterator::sum), on
+    // every lane (same result everywhere).
+    __device__ __forceinline__ double dot_seq(const double* a, const double* b, int n) const {
+        double s = -0.0;
+        for (int i = 0; i < n; ++i) s = s + a[i] * b[i];
+        return s;
+    }
+    // out = H v, one lane per row, each row summed in column order from 0.0.
+    __device__ __forceinline__ void mat_vec(const double* H, const double* v, double* out, int n) const {
+        for (int i = lane; i < n; i += 64) {
+            const double* h = H + (size_t)i * n;
+            double s = 0.0;
+            for (int j = 0; j < n; ++j) s += h[j] * v[j];
+            out[i] = s;
+        }
+        gsync();
+    }
+    __device__ __forceinline__ void affine(const double* x, const double* d, double alpha, double* out, int n) const {
+        for (int i = lane; i < n; i += 64) {
+            const double v = x[i] + alpha * d[i];
+            out[i] = v < 0.0 ? 0.0 : v;
+        }
+        gsync();
+    }
+    // quadratic_cost 0.5 xᵀHx + linᵀx with Hx into hv.
+    __device__ __forceinline__ double cost(const double* H, const double* lin, const double* x, double* hv, int n) const {
+        mat_vec(H, x, hv, n);
+        return 0.5 * dot_seq(x, hv, n) + dot_seq(lin, x, n);
+    }
+    // Impulses of rows 0..4 nblk into rows[7 ri + 6]; 0 or a fault code.
+    __device__ __noinline__ int newton(int nblk) const {
+        const int n = 4 * nblk;
+        if (n > B.nn) return 8;  // NEWT_ERR_CAPACITY
+        const int nn = B.nn;
+        double* H = gd + B.g_H;
+        double* Lf = gd + B.g_L;
+        double* V = gd + B.g_vec;
+        double *lin = V, *x = V + nn, *g = V + 2 * nn, *dd = V + 3 * nn, *y = V + 4 * nn, *pt = V + 5 * nn;
+        double *pl = V + 6 * nn, *pr = V + 7 * nn, *sl = V + 8 * nn, *hv = V + 9 * nn;
+        double* brk = gd + B.g_brk;
+        double* cand = gd + B.g_cand;
+        // A[i][j] = response of row i to a unit impulse on row j; H = A + diag(reg).
+        for (int j = lane; j < n; j += 64) {
+            const int kj = j >> 2;
+            const int aj = bbi[2 * kj], bj = bbi[2 * kj + 1];
+            const V3<double> dir = rdir(j);
+            V3<double> dva = z3<double>(), dwa = z3<double>(), dvb = z3<double>(), dwb = z3<double>();
+            if (aj >= 0) side_delta(aj, dir, ld3(blk + 8 * kj), 1.0, &dva, &dwa);
+            if (bj >= 0) side_delta(bj, -dir, ld3(blk + 8 * kj + 3), 1.0, &dvb, &dwb);
+            for (int i = 0; i < n; ++i) {
+                const int ki = i >> 2;
+                const int ai = bbi[2 * ki], bi = bbi[2 * ki + 1];
+                V3<double> av = z3<double>(), aw = z3<double>(), bv = z3<double>(), bw = z3<double>();
+                bool any = false;
+                if (ai >= 0 && ai == aj) { av = dva; aw = dwa; any = true; }
+                if (ai >= 0 && ai == bj) { av = dvb; aw = dwb; any = true; }
+                if (bi >= 0 && bi == aj) { bv = dva; bw = dwa; any = true; }
+                if (bi >= 0 && bi == bj) { bv = dvb; bw = dwb; any = true; }
+                // untouched bodies keep zero accumulators: the residual is +0.0
+                double a = any ? residual_of(rdir(i), ai, bi, ld3(blk + 8 * ki), ld3(blk + 8 * ki + 3), av, aw, bv, bw) : 0.0;
+                if (i == j) a += rows[7 * i + 3];
+                H[(size_t)i * n + j] = a;
+            }
+        }
+        for (int i = lane; i < n; i += 64) {
+            lin[i] = rows[7 * i + 5];
+            x[i] = 0.0;
+        }
+        gsync();
+        // Cholesky-Banachiewicz (dynamics.rs:602-624) column by column; each
+        // entry's sum runs over k in order, as in the row-wise loop.
+        for (int j = 0; j < n; ++j) {
+            const double* lj = Lf + (size_t)j * n;
+            double sum = H[(size_t)j * n + j];
+            for (int k = 0; k < j; ++k) sum -= lj[k] * lj[k];
+            if (sum <= 0.0) return 4;  // NEWT_ERR_NEWTON_FAIL: Hessian not positive definite
+            const double ljj = __builtin_sqrt(sum);
+            if (lane == 0) Lf[(size_t)j * n + j] = ljj;
+            for (int i = j + 1 + lane; i < n; i += 64) {
+                const double* li = Lf + (size_t)i * n;
+                double s = H[(size_t)i * n + j];
+                for (int k = 0; k < j; ++k) s -= li[k] * lj[k];
+                Lf[(size_t)i * n + j] = s / ljj;
+            }
+            gsync();
+        }
+        const uint32_t max_it = m.iterations < 1 ? 1u : (uint32_t)m.iterations;
+        double cur = cost(H, lin, x, hv, n);
+        for (uint32_t it = 0; it < max_it; ++it) {
+            // gradient, direction = cholesky_solve(L, -g) (dynamics.rs:626-648)
+            mat_vec(H, x, g, n);
+            for (int i = lane; i < n; i += 64) {
+                g[i] = g[i] + lin[i];
+                y[i] = -g[i];
+            }
+            gsync();
+            // forward: y[i] -= L[i][k] y[k] for k = 0, 1, ... (the row sum's order)
+            for (int k = 0; k < n; ++k) {
+                const int own = k & 63;
+                double yk = 0.0;
+                if (lane == own) {
+                    yk = y[k] / Lf[(size_t)k * n + k];
+                    y[k] = yk;
+                }
+                yk = __shfl(yk, own, 64);
+                for (int i = k + 1 + lane - ((k + 1) & 63) + ((((k + 1) & 63) > lane) ? 64 : 0); i < n; i += 64)
+                    y[i] -= Lf[(size_t)i * n + k] * yk;
+            }
+            gsync();
+            // backward, one lane (sum over k > i in increasing k)
+            if (lane == 0) {
+                for (int i = n - 1; i >= 0; --i) {
+                    double sum = y[i];
+                    for (int k = i + 1; k < n; ++k) sum -= Lf[(size_t)k * n + i] * dd[k];
+                    dd[i] = sum / Lf[(size_t)i * n + i];
+                }
+            }
+            gsync();
+            if (dot_seq(g, dd, n) >= 0.0) break;
+            // exact_pyramidal_line_search with NonNegative projections
+            int nbk = 0;
+            if (lane == 0) {
+                brk[0] = 0.0;
+                brk[1] = 1.0;
+                int c = 2;
+                for (int i = 0; i < n; ++i) {
+                    const double slope = dd[i];
+                    if (slope == 0.0) continue;
+                    const double alpha = -x[i] / slope;
+                    if (alpha > 0.0 && alpha < 1.0) brk[c++] = alpha;
+                }
+                for (int a = 1; a < c; ++a) {  // stable insertion sort
+                    const double v = brk[a];
+                    int b = a - 1;
+                    while (b >= 0 && brk[b] > v) {
+                        brk[b + 1] = brk[b];
+                        --b;
+                    }
+                    brk[b + 1] = v;
+                }
+                int u = 0;
+                for (int a = 0; a < c; ++a) {
+                    const double v = brk[a];
+                    if (u > 0 && __builtin_fabs(v - brk[u - 1]) <= 1e-7) continue;
+                    brk[u++] = v;
+                }
+                nbk = u;
+            }
+            nbk = __shfl(nbk, 0, 64);
+            gsync();
+            int ncand = nbk;
+            for (int a = lane; a < nbk; a += 64) cand[a] = brk[a];
+            gsync();
+            for (int k = 0; k + 1 < nbk; ++k) {
+                const double lo = brk[k], hi = brk[k + 1];
+                if (hi - lo <= 1e-7) continue;
+                const double mid = 0.5 * (lo + hi);
+                const double eps = (hi - lo) * 1e-4;
+                const double la = nmax(mid - eps, lo);
+                const double ra = nmin(mid + eps, hi);
+                affine(x, dd, la, pl, n);
+                affine(x, dd, ra, pr, n);
+                const double span = ra - la;
+                if (span <= 0.0) continue;
+                const double inv = 1.0 / span;
+                for (int i = lane; i < n; i += 64) sl[i] = (pr[i] - pl[i]) * inv;
+                affine(x, dd, mid, pt, n);  // includes the fence for sl
+                mat_vec(H, pt, hv, n);
+                for (int i = lane; i < n; i += 64) pl[i] = hv[i] + lin[i];  // local gradient
+                mat_vec(H, sl, hv, n);  // fences pl too
+                const double den = dot_seq(sl, hv, n);
+                const double num = dot_seq(pl, sl, n);
+                const double opt = den > 0.0 ? nmin(nmax(mid - num / den, lo), hi) : mid;
+                if (lane == 0) cand[ncand] = opt;
+                ++ncand;
+                gsync();
+            }
+            double best_a = 0.0, best_c = cur;
+            for (int c = 0; c < ncand; ++c) {
+                const double a = cand[c];
+                affine(x, dd, a, pt, n);
+                const double cc = cost(H, lin, pt, hv, n);
+                if (cc < best_c) {
+                    best_c = cc;
+                    best_a = a;
+                }
+            }
+            if (best_a <= 0.0 || cur - best_c <= 1e-7 * (1.0 + __builtin_fabs(cur))) break;
+            for (int i = lane; i < n; i += 64) {
+                const double v = x[i] + best_a * dd[i];
+                x[i] = v < 0.0 ? 0.0 : v;
+            }
+            gsync();
+            cur = best_c;
+        }
+        for (int i = lane; i < n; i += 64) rows[7 * i + 6] = x[i];
+        gsync();
+        return 0;
+    }
+
+    // solve_free_bodies (solver.rs:703-1257): wrenches into LDS; 0 or a
+    // fault code.
+    __device__ __forceinline__ int solve(int nc) const {
         const int nb = m.n_bodies;
         for (int i = lane; i < 6 * nb; i += 64) sd[B.wr + i] = 0.0;
         if (nc == 0 || m.dt <= 0.0) {
             gsync();
-            return;
+            return 0;
         }
         for (int i = lane; i < nb; i += 64) {
             M33<double> r = qmat(bori(i));
@@ -994,10 +1215,15 @@ struct BodyWave {
             k0 += __popcll(mask);
         }
         gsync();
-        if (nblk == 0) return;
-        if (lane == 0)
-            body_pgs_sweeps(m.iterations, nblk, rows, blk, bbi, sd + B.bd, sd + B.iiw, sd + B.st, m.bodies);
-        gsync();
+        if (nblk == 0) return 0;
+        if (m.mode == NEWT_SOLVER_NEWTON) {
+            const int st = newton(nblk);
+            if (st != 0) return st;
+        } else {
+            if (lane == 0)
+                body_pgs_sweeps(m.iterations, nblk, rows, blk, bbi, sd + B.bd, sd + B.iiw, sd + B.st, m.bodies);
+            gsync();
+        }
         // Wrenches, one lane per body, rows in order (solver.rs:1198-1235).
         const double dt = m.dt;
         for (int b0 = lane; b0 < nb; b0 += 64) {
@@ -1023,6 +1249,7 @@ struct BodyWave {
             st3(sd + B.wr + 6 * b0 + 3, t);
         }
         gsync();
+        return 0;
     }
 
     // ---------------- integration (world.rs:1705-1817, 3223-3286) ---------
@@ -1092,7 +1319,7 @@ struct BodyWave {
         *nc = 0;
         *np = 0;
         int c = 0;
-        if (m.mode == NEWT_SOLVER_PGS) {
+        if (m.mode == NEWT_SOLVER_PGS || m.mode == NEWT_SOLVER_NEWTON) {
             c = contacts(np);
             if (c > m.cap_c) return 8;  // NEWT_ERR_CAPACITY
         } else {
@@ -1100,7 +1327,8 @@ struct BodyWave {
             *np = 0;
         }
         *nc = c;
-        solve(c);
+        const int st = solve(c);
+        if (st != 0) return st;
         integrate();
         populate(false);
         return 0;

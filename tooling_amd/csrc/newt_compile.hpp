@@ -115,7 +115,6 @@ inline newt_status compile_bodies(HostModel* M, std::string* err) {
     if (o.integrator != NEWT_INTEGRATOR_EULER && o.integrator != NEWT_INTEGRATOR_IMPLICITFAST &&
         o.integrator != NEWT_INTEGRATOR_RK4)
         return cfail(err, NEWT_ERR_INVALID_ARG, "unknown integrator");
-    if (o.solver_mode == NEWT_SOLVER_NEWTON) return cfail(err, NEWT_ERR_UNSUPPORTED, "the Newton free-body solve is not built");
     if (o.cone != NEWT_CONE_PYRAMIDAL) return cfail(err, NEWT_ERR_UNSUPPORTED, "elliptic cone is not built");
     const int nb = (int)M->bodies.size();
     M->hbodies.clear();

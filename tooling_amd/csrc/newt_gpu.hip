@@ -215,7 +215,8 @@ newt_status launch(newt_data* D, int n_steps, hipStream_t stream, int init) {
     D->last_stream = stream;
     dim3 grid((D->W + kBlock - 1) / kBlock), block(kBlock);
     if (M->dev.n_bodies > 0) {
-        if (D->precision == NEWT_PRECISION_FP64_EXACT || !D->bws || D->W > kBodyWaveMaxWorlds)
+        const bool newton = M->dev.mode == NEWT_SOLVER_NEWTON;  // the Newton solve is wavefront-only
+        if (!newton && (D->precision == NEWT_PRECISION_FP64_EXACT || !D->bws || D->W > kBodyWaveMaxWorlds))
             hipLaunchKernelGGL(k_bodies, grid, block, 0, stream, M->dev, D->L, D->d, D->dws, D->iws, n_steps, init);
         else
             hipLaunchKernelGGL(k_bodies_wave, dim3(D->W), dim3(64), D->BL.lds_bytes, stream, M->dev, D->BL, D->L, D->d,
@@ -456,11 +457,18 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
               alloc(&D->ws, W * D->L.total_r * rsz) && alloc((void**)&D->dws, W * D->L.total_d * 8) &&
               alloc((void**)&D->iws, W * D->L.total_i * 4) &&
               alloc((void**)&D->ovf, W * (size_t)std::max(TL.ov_stride, 16));
-    if (ok && bodies && precision != NEWT_PRECISION_FP64_EXACT && n_worlds <= kBodyWaveMaxWorlds) {
+    const bool newton = bodies && m.mode == NEWT_SOLVER_NEWTON;
+    if (ok && bodies && (newton || (precision != NEWT_PRECISION_FP64_EXACT && n_worlds <= kBodyWaveMaxWorlds))) {
         // The wavefront step keeps a world's bodies and fat bounds in LDS;
-        // models whose bodies do not fit run the lane step.
+        // models whose bodies do not fit run the lane step (PGS only).
         D->BL = make_body_wave_layout(m, kBodyWaveLds);
-        if (8 * (34 * m.n_bodies + 6 * m.n_geoms) + 64 <= kBodyWaveLds) {
+        const bool fits = 8 * (34 * m.n_bodies + 6 * m.n_geoms) + 64 <= kBodyWaveLds;
+        if (newton && !fits) {
+            newt_data_destroy(D);
+            return fail(NEWT_ERR_UNSUPPORTED, "the Newton free-body step keeps a world's bodies in 40 KiB of LDS; "
+                                              "this model has too many bodies or geoms");
+        }
+        if (fits) {
             ok = alloc((void**)&D->bws, W * (size_t)D->BL.g_stride_d * 8) &&
                  alloc((void**)&D->bwi, W * (size_t)D->BL.g_stride_i * 4);
             if (ok && D->BL.lds_bytes > 65536)
