@@ -262,24 +262,26 @@ def bench_chains(args):
 
 
 def bench_boxes(args):
-    """1024 worlds of 64 free boxes dropped on a plane (half extents ~
+    """C3: 1024 worlds of 64 free boxes dropped on a plane (half extents ~
     U[0.05, 0.2]^3, density 1000, jittered 4x4x4 grid, random orientations),
-    PGS 20, pyramidal, Euler dt 2 ms: the free-body step (World::add_body
-    models; newt/src/world.rs:1705-1817, solver.rs:703-1257), FP64 with
-    newt's operation order in every precision mode."""
+    Newton 20, pyramidal, Euler dt 2 ms: the free-body step (World::add_body
+    models; newt/src/world.rs:1705-1817, solver.rs:703-1407, newton.rs),
+    FP64 with newt's operation order.  The same batch under PGS 20 and a
+    16x larger PGS batch are reported beside it."""
     import numpy as np
     from oracle import pyoracle
     from tooling_amd import abi, scenes
     from tooling_amd.newt import Data, Model, step
 
     worlds = args.worlds if args.worlds != 8192 else 1024
-    scene = scenes.box_pile()
+    pgs_scene = scenes.box_pile()
+    scene = scenes.with_options(pgs_scene, "_newton", solver=abi.SOLVER_NEWTON)
     q0, v0 = scene.initial_state(worlds, args.seed)
     model = Model(scene)
     log(f"bench boxes: {scene.name} x {worlds} worlds, warmup {args.warmup}, steps {args.steps}, "
         f"{args.chunk} steps per launch")
     r = time_gpu(model, worlds, abi.PRECISION_FP64, args.steps, args.warmup, args.chunk, q0, v0, keep_state=True)
-    log(f"fp64: {r['value']:.4g} world-steps/s, {r['ms_per_step']:.3f} ms/step")
+    log(f"newton fp64: {r['value']:.4g} world-steps/s, {r['ms_per_step']:.3f} ms/step")
     qs, vs = r["warm_state"]
     # one step from the warm states on the GPU and on the FP64 oracle: bitwise
     k = 16
@@ -301,39 +303,54 @@ def bench_boxes(args):
     launch_ms = sum(r["launch_ms"]) / len(r["launch_ms"])
     B = bytes_per_world_step(model.nq, model.nv)
     achieved = B * worlds * r["steps_per_launch"] / (launch_ms / 1e3) / 1e9
-    # The same scene at 16x the batch (one lane per world at that size).
+    # PGS 20 on the same batch, and 16x the batch (one lane per world there)
+    pgs_model = Model(pgs_scene)
+    rp = time_gpu(pgs_model, worlds, abi.PRECISION_FP64, args.steps, args.warmup, args.chunk, q0, v0)
+    log(f"pgs fp64: {rp['value']:.4g} world-steps/s, {rp['ms_per_step']:.3f} ms/step")
     big_w = 16 * worlds
-    bq, bv = scene.initial_state(big_w, args.seed)
-    rb = time_gpu(model, big_w, abi.PRECISION_FP64, 20, args.warmup, 10, bq, bv)
-    log(f"{big_w} worlds: {rb['value']:.4g} world-steps/s, {rb['ms_per_step']:.3f} ms/step")
+    bq, bv = pgs_scene.initial_state(big_w, args.seed)
+    rb = time_gpu(pgs_model, big_w, abi.PRECISION_FP64, 20, args.warmup, 10, bq, bv)
+    log(f"pgs {big_w} worlds: {rb['value']:.4g} world-steps/s, {rb['ms_per_step']:.3f} ms/step")
     cpu = None
+    cpu_pgs = None
     if not args.no_cpu:
         threads = max(1, min(os.cpu_count() or 1, 16))
-        n = 4 * threads
+        n = 2 * threads
         ow = pyoracle.OracleWorlds(scene, n, 1)
+        ow.set_state(qs[:n], vs[:n])
+        t0 = time.perf_counter()
+        ow.step(10, threads=threads)
+        dt = time.perf_counter() - t0
+        ow.close()
+        cpu = dict(value=n * 10 / dt, unit="world-steps/s", cores=threads, kind="port",
+                   sample=f"{n} worlds of the same batch from the GPU's warm states, 10 steps, the oracle's "
+                          f"float instantiation (newt's f32 step, byte-identical to newt's solver_newton_stack "
+                          f"golden) on {threads} host threads ({dt:.1f} s)")
+        log(f"cpu newton: {cpu['value']:.4g} world-steps/s on {threads} threads")
+        n = 4 * threads
+        ow = pyoracle.OracleWorlds(pgs_scene, n, 1)
         ow.set_state(qs[:n], vs[:n])
         t0 = time.perf_counter()
         ow.step(50, threads=threads)
         dt = time.perf_counter() - t0
         ow.close()
-        cpu = dict(value=n * 50 / dt, unit="world-steps/s", cores=threads, kind="port",
-                   sample=f"{n} worlds of the same batch from the GPU's warm states, 50 steps, the oracle's "
-                          f"float instantiation (newt's f32 step, byte-identical to newt's solver_stack golden) "
-                          f"on {threads} host threads ({dt:.1f} s)")
-        log(f"cpu: {cpu['value']:.4g} world-steps/s on {threads} threads")
+        cpu_pgs = dict(value=n * 50 / dt, unit="world-steps/s", cores=threads, sample=f"{n} worlds x 50 steps")
+        log(f"cpu pgs: {cpu_pgs['value']:.4g} world-steps/s on {threads} threads")
     out = {
         "metric": "world-steps/s (free-body step, FP64)",
         "value": r["value"], "unit": "world-steps/s", "n_gpus": 1, "steps": r["steps"], "warmup": args.warmup,
         "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"boxes: {worlds} worlds x 64 free boxes on a plane (2080 candidate pairs), "
-                               f"PGS 20 pyramidal, Euler dt 2 ms, seed {args.seed}",
+        "config": {"workload": f"boxes (C3): {worlds} worlds x 64 free boxes on a plane (2080 candidate pairs), "
+                               f"Newton 20 pyramidal, Euler dt 2 ms, seed {args.seed}",
                    "worlds": worlds, "steps_per_launch": r["steps_per_launch"],
                    "parallelism": "1 GPU, one wavefront per world (k_bodies_wave)"},
         "exact_vs_fp64_oracle": {"bitwise": exact, "worlds": k, "contacts_min_max": [int(nc.min()), int(nc.max())]},
-        "larger_batch": {"worlds": big_w, "value": rb["value"], "ms_per_step": rb["ms_per_step"],
-                         "steps": rb["steps"], "warmup": args.warmup, "kernel": "k_bodies (one lane per world)",
-                         "faults": rb["faults"]},
+        "pgs": {"value": rp["value"], "ms_per_step": rp["ms_per_step"], "solver": "PGS 20", "faults": rp["faults"],
+                "cpu_baseline": cpu_pgs},
+        "pgs_larger_batch": {"worlds": big_w, "value": rb["value"], "ms_per_step": rb["ms_per_step"],
+                             "steps": rb["steps"], "warmup": args.warmup,
+                             "kernel": "k_bodies (one lane per world)", "faults": rb["faults"]},
         "faults": r["faults"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_bodies_wave",
@@ -427,7 +444,7 @@ def main():
     ap.add_argument("--workload", choices=["quadruped", "chains", "serve", "boxes"], default="quadruped",
                     help="quadruped: 8192 quadrupeds (the headline); chains: fixed-root hinge chains of "
                          "2..64 links, one model per length, batched over streams; serve: Poisson rollout "
-                         "requests served by continuous batching; boxes: 1024 worlds of 64 free boxes")
+                         "requests served by continuous batching; boxes: 1024 worlds of 64 free boxes, Newton 20 (C3)")
     ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
     ap.add_argument("--serve-chunk", type=int, default=400, help="serve: most steps per launch")

@@ -441,14 +441,14 @@ struct BodyWaveLayout {
     int lds_contacts;                     // active contacts whose rows fit in LDS
     int g_rows, g_blk, g_stride_d;        // global fallback, doubles per world
     int g_bbi, g_stride_i;                // ints per world
-    // Newton solve (global, per world): rows cap nn, then H, L (nn x nn) and
+    // Newton solve (global, per world): rows cap nn, then H, L, Lᵀ (nn x nn) and
     // the vectors lin, x, g, d, y, p, pl, pr, slope, hv (nn each) and the
     // break / candidate lists (2 nn + 4 each)
-    int nn, g_H, g_L, g_vec, g_brk, g_cand;
+    int nn, g_H, g_L, g_Lt, g_vec, g_brk, g_cand;  // Lt: the factor transposed
 };
 
-// Rows of one world's Newton system at most (1024 = 256 contacts): H and its
-// factor take 2 x 8 MiB per world.  A world with more active rows faults
+// Rows of one world's Newton system at most (1024 = 256 contacts): H, its
+// factor and the factor transposed take 3 x 8 MiB per world.  A world with more active rows faults
 // with NEWT_ERR_CAPACITY.
 constexpr int kNewtonMaxRows = 1024;
 
@@ -484,7 +484,8 @@ inline BodyWaveLayout make_body_wave_layout(const ModelT& m, int lds_budget_byte
         B.nn = nn;
         B.g_H = B.g_stride_d;
         B.g_L = B.g_H + nn * nn;
-        B.g_vec = B.g_L + nn * nn;
+        B.g_Lt = B.g_L + nn * nn;
+        B.g_vec = B.g_Lt + nn * nn;
         B.g_brk = B.g_vec + 10 * nn;
         B.g_cand = B.g_brk + 2 * nn + 4;
         B.g_stride_d = B.g_cand + 2 * nn + 4;
@@ -1002,6 +1003,7 @@ struct BodyWave {
         const int nn = B.nn;
         double* H = gd + B.g_H;
         double* Lf = gd + B.g_L;
+        double* Lt = gd + B.g_Lt;  // Lt[j n + i] = L[i][j]: rows for the substitutions
         double* V = gd + B.g_vec;
         double *lin = V, *x = V + nn, *g = V + 2 * nn, *dd = V + 3 * nn, *y = V + 4 * nn, *pt = V + 5 * nn;
         double *pl = V + 6 * nn, *pr = V + 7 * nn, *sl = V + 8 * nn, *hv = V + 9 * nn;
@@ -1043,12 +1045,17 @@ struct BodyWave {
             for (int k = 0; k < j; ++k) sum -= lj[k] * lj[k];
             if (sum <= 0.0) return 4;  // NEWT_ERR_NEWTON_FAIL: Hessian not positive definite
             const double ljj = __builtin_sqrt(sum);
-            if (lane == 0) Lf[(size_t)j * n + j] = ljj;
+            if (lane == 0) {
+                Lf[(size_t)j * n + j] = ljj;
+                Lt[(size_t)j * n + j] = ljj;
+            }
             for (int i = j + 1 + lane; i < n; i += 64) {
                 const double* li = Lf + (size_t)i * n;
                 double s = H[(size_t)i * n + j];
                 for (int k = 0; k < j; ++k) s -= li[k] * lj[k];
-                Lf[(size_t)i * n + j] = s / ljj;
+                const double v = s / ljj;
+                Lf[(size_t)i * n + j] = v;
+                Lt[(size_t)j * n + i] = v;
             }
             gsync();
         }
@@ -1067,20 +1074,22 @@ struct BodyWave {
                 const int own = k & 63;
                 double yk = 0.0;
                 if (lane == own) {
-                    yk = y[k] / Lf[(size_t)k * n + k];
+                    yk = y[k] / Lt[(size_t)k * n + k];
                     y[k] = yk;
                 }
                 yk = __shfl(yk, own, 64);
+                const double* ltk = Lt + (size_t)k * n;
                 for (int i = k + 1 + lane - ((k + 1) & 63) + ((((k + 1) & 63) > lane) ? 64 : 0); i < n; i += 64)
-                    y[i] -= Lf[(size_t)i * n + k] * yk;
+                    y[i] -= ltk[i] * yk;
             }
             gsync();
             // backward, one lane (sum over k > i in increasing k)
             if (lane == 0) {
                 for (int i = n - 1; i >= 0; --i) {
+                    const double* lti = Lt + (size_t)i * n;
                     double sum = y[i];
-                    for (int k = i + 1; k < n; ++k) sum -= Lf[(size_t)k * n + i] * dd[k];
-                    dd[i] = sum / Lf[(size_t)i * n + i];
+                    for (int k = i + 1; k < n; ++k) sum -= lti[k] * dd[k];
+                    dd[i] = sum / lti[i];
                 }
             }
             gsync();
