@@ -441,14 +441,14 @@ struct BodyWaveLayout {
     int lds_contacts;                     // active contacts whose rows fit in LDS
     int g_rows, g_blk, g_stride_d;        // global fallback, doubles per world
     int g_bbi, g_stride_i;                // ints per world
-    // Newton solve (global, per world): rows cap nn, then H, L, Lᵀ (nn x nn) and
+    // Newton solve (global, per world): rows cap nn, then H, L and Lᵀ (nn x nn) and
     // the vectors lin, x, g, d, y, p, pl, pr, slope, hv (nn each) and the
     // break / candidate lists (2 nn + 4 each)
-    int nn, g_H, g_L, g_Lt, g_vec, g_brk, g_cand;  // Lt: the factor transposed
+    int nn, g_H, g_L, g_Lt, g_vec, g_brk, g_cand;  // the factor row- and column-major
 };
 
-// Rows of one world's Newton system at most (1024 = 256 contacts): H, its
-// factor and the factor transposed take 3 x 8 MiB per world.  A world with more active rows faults
+// Rows of one world's Newton system at most (1024 = 256 contacts): H and its
+// factor in both orders take 3 x 8 MiB per world.  A world with more active rows faults
 // with NEWT_ERR_CAPACITY.
 constexpr int kNewtonMaxRows = 1024;
 
@@ -974,13 +974,25 @@ struct BodyWave {
         for (int i = 0; i < n; ++i) s = s + a[i] * b[i];
         return s;
     }
-    // out = H v, one lane per row, each row summed in column order from 0.0.
+    // out = H v, one lane per row, each row summed in column order from 0.0;
+    // a lane carries four rows at once (independent sums, same order each).
     __device__ __forceinline__ void mat_vec(const double* H, const double* v, double* out, int n) const {
-        for (int i = lane; i < n; i += 64) {
-            const double* h = H + (size_t)i * n;
-            double s = 0.0;
-            for (int j = 0; j < n; ++j) s += h[j] * v[j];
-            out[i] = s;
+        for (int i0 = lane; i0 < n; i0 += 256) {
+            const int i1 = i0 + 64 < n ? i0 + 64 : i0, i2 = i0 + 128 < n ? i0 + 128 : i0, i3 = i0 + 192 < n ? i0 + 192 : i0;
+            const double *h0 = H + (size_t)i0 * n, *h1 = H + (size_t)i1 * n, *h2 = H + (size_t)i2 * n,
+                         *h3 = H + (size_t)i3 * n;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            for (int j = 0; j < n; ++j) {
+                const double vj = v[j];
+                s0 += h0[j] * vj;
+                s1 += h1[j] * vj;
+                s2 += h2[j] * vj;
+                s3 += h3[j] * vj;
+            }
+            out[i0] = s0;
+            if (i0 + 64 < n) out[i0 + 64] = s1;
+            if (i0 + 128 < n) out[i0 + 128] = s2;
+            if (i0 + 192 < n) out[i0 + 192] = s3;
         }
         gsync();
     }
@@ -1002,14 +1014,17 @@ struct BodyWave {
         if (n > B.nn) return 8;  // NEWT_ERR_CAPACITY
         const int nn = B.nn;
         double* H = gd + B.g_H;
-        double* Lf = gd + B.g_L;
-        double* Lt = gd + B.g_Lt;  // Lt[j n + i] = L[i][j]: rows for the substitutions
+        double* Lf = gd + B.g_L;   // the factor, row-major
+        double* Lt = gd + B.g_Lt;  // Lt[j n + i] = L[i][j], column-major
         double* V = gd + B.g_vec;
         double *lin = V, *x = V + nn, *g = V + 2 * nn, *dd = V + 3 * nn, *y = V + 4 * nn, *pt = V + 5 * nn;
         double *pl = V + 6 * nn, *pr = V + 7 * nn, *sl = V + 8 * nn, *hv = V + 9 * nn;
         double* brk = gd + B.g_brk;
         double* cand = gd + B.g_cand;
         // A[i][j] = response of row i to a unit impulse on row j; H = A + diag(reg).
+        // One lane per column j; rows go by contact block (four rows share
+        // their bodies).  A block with neither of column j's bodies keeps
+        // zero accumulators, whose residual is +0.0.
         for (int j = lane; j < n; j += 64) {
             const int kj = j >> 2;
             const int aj = bbi[2 * kj], bj = bbi[2 * kj + 1];
@@ -1017,8 +1032,7 @@ struct BodyWave {
             V3<double> dva = z3<double>(), dwa = z3<double>(), dvb = z3<double>(), dwb = z3<double>();
             if (aj >= 0) side_delta(aj, dir, ld3(blk + 8 * kj), 1.0, &dva, &dwa);
             if (bj >= 0) side_delta(bj, -dir, ld3(blk + 8 * kj + 3), 1.0, &dvb, &dwb);
-            for (int i = 0; i < n; ++i) {
-                const int ki = i >> 2;
+            for (int ki = 0; ki < nblk; ++ki) {
                 const int ai = bbi[2 * ki], bi = bbi[2 * ki + 1];
                 V3<double> av = z3<double>(), aw = z3<double>(), bv = z3<double>(), bw = z3<double>();
                 bool any = false;
@@ -1026,10 +1040,19 @@ struct BodyWave {
                 if (ai >= 0 && ai == bj) { av = dvb; aw = dwb; any = true; }
                 if (bi >= 0 && bi == aj) { bv = dva; bw = dwa; any = true; }
                 if (bi >= 0 && bi == bj) { bv = dvb; bw = dwb; any = true; }
-                // untouched bodies keep zero accumulators: the residual is +0.0
-                double a = any ? residual_of(rdir(i), ai, bi, ld3(blk + 8 * ki), ld3(blk + 8 * ki + 3), av, aw, bv, bw) : 0.0;
-                if (i == j) a += rows[7 * i + 3];
-                H[(size_t)i * n + j] = a;
+                double a4[4] = {0.0, 0.0, 0.0, 0.0};
+                if (any) {
+                    const V3<double> arma = ld3(blk + 8 * ki), armb = ld3(blk + 8 * ki + 3);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) a4[r] = residual_of(rdir(4 * ki + r), ai, bi, arma, armb, av, aw, bv, bw);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = 4 * ki + r;
+                    double a = a4[r];
+                    if (i == j) a += rows[7 * i + 3];
+                    H[(size_t)i * n + j] = a;
+                }
             }
         }
         for (int i = lane; i < n; i += 64) {
@@ -1037,25 +1060,62 @@ struct BodyWave {
             x[i] = 0.0;
         }
         gsync();
-        // Cholesky-Banachiewicz (dynamics.rs:602-624) column by column; each
-        // entry's sum runs over k in order, as in the row-wise loop.
+        // Cholesky-Banachiewicz (dynamics.rs:602-624) column by column: every
+        // entry's sum runs over k in increasing order, as in the row-wise
+        // loop.  L is kept row-major (Lf, read by the factorization eight
+        // entries at a time so the loads overlap) and column-major (Lt, read
+        // by the substitutions).
         for (int j = 0; j < n; ++j) {
             const double* lj = Lf + (size_t)j * n;
             double sum = H[(size_t)j * n + j];
-            for (int k = 0; k < j; ++k) sum -= lj[k] * lj[k];
+            {
+                int k = 0;
+                for (; k + 8 <= j; k += 8) {
+                    double a[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) a[u] = lj[k + u];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) sum -= a[u] * a[u];
+                }
+                for (; k < j; ++k) sum -= lj[k] * lj[k];
+            }
             if (sum <= 0.0) return 4;  // NEWT_ERR_NEWTON_FAIL: Hessian not positive definite
             const double ljj = __builtin_sqrt(sum);
             if (lane == 0) {
                 Lf[(size_t)j * n + j] = ljj;
                 Lt[(size_t)j * n + j] = ljj;
             }
-            for (int i = j + 1 + lane; i < n; i += 64) {
-                const double* li = Lf + (size_t)i * n;
-                double s = H[(size_t)i * n + j];
-                for (int k = 0; k < j; ++k) s -= li[k] * lj[k];
-                const double v = s / ljj;
-                Lf[(size_t)i * n + j] = v;
-                Lt[(size_t)j * n + i] = v;
+            for (int i0 = j + 1 + lane; i0 < n; i0 += 128) {
+                const bool v1 = i0 + 64 < n;
+                const int i1 = v1 ? i0 + 64 : i0;
+                const double *r0 = Lf + (size_t)i0 * n, *r1 = Lf + (size_t)i1 * n;
+                double s0 = H[(size_t)i0 * n + j], s1 = H[(size_t)i1 * n + j];
+                int k = 0;
+                for (; k + 8 <= j; k += 8) {
+                    double b[8], c0[8], c1[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        b[u] = lj[k + u];
+                        c0[u] = r0[k + u];
+                        c1[u] = r1[k + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        s0 -= c0[u] * b[u];
+                        s1 -= c1[u] * b[u];
+                    }
+                }
+                for (; k < j; ++k) {
+                    s0 -= r0[k] * lj[k];
+                    s1 -= r1[k] * lj[k];
+                }
+                const double q0 = s0 / ljj, q1 = s1 / ljj;
+                Lf[(size_t)i0 * n + j] = q0;
+                Lt[(size_t)j * n + i0] = q0;
+                if (v1) {
+                    Lf[(size_t)i1 * n + j] = q1;
+                    Lt[(size_t)j * n + i1] = q1;
+                }
             }
             gsync();
         }
@@ -1088,7 +1148,18 @@ struct BodyWave {
                 for (int i = n - 1; i >= 0; --i) {
                     const double* lti = Lt + (size_t)i * n;
                     double sum = y[i];
-                    for (int k = i + 1; k < n; ++k) sum -= lti[k] * dd[k];
+                    int k = i + 1;
+                    for (; k + 8 <= n; k += 8) {
+                        double a[8], b[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            a[u] = lti[k + u];
+                            b[u] = dd[k + u];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) sum -= a[u] * b[u];
+                    }
+                    for (; k < n; ++k) sum -= lti[k] * dd[k];
                     dd[i] = sum / lti[i];
                 }
             }
