@@ -512,25 +512,50 @@ __device__ __noinline__ void body_pgs_sweeps(int iterations, int nblk, double* _
                                              const double* __restrict__ blk, const int* __restrict__ bbi,
                                              double* __restrict__ bd, const double* __restrict__ iiw,
                                              const double* __restrict__ st, const BodyT* __restrict__ bodies) {
+    // One block's rows, arms and bodies in registers: every load of a block
+    // is issued together (the rows may sit in global memory), and the four
+    // impulses stay in registers until the block is done.
+    struct Blk {
+        V3<double> dir[4];
+        double reg[4], diag[4], bias[4], f[4];
+        V3<double> arma, armb;
+        double h01, h23;
+        int a, b;
+    };
+    auto load = [&](int k, Blk& q) {
+        const double* r = rows + 28 * k;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            q.dir[u] = mk3(r[7 * u], r[7 * u + 1], r[7 * u + 2]);
+            q.reg[u] = r[7 * u + 3];
+            q.diag[u] = r[7 * u + 4];
+            q.bias[u] = r[7 * u + 5];
+            q.f[u] = r[7 * u + 6];
+        }
+        const double* p = blk + 8 * k;
+        q.arma = mk3(p[0], p[1], p[2]);
+        q.armb = mk3(p[3], p[4], p[5]);
+        q.h01 = p[6];
+        q.h23 = p[7];
+        q.a = bbi[2 * k];
+        q.b = bbi[2 * k + 1];
+    };
     auto ld3 = [](const double* p) { return mk3(p[0], p[1], p[2]); };
     auto ori = [&](int b) {
         const double* p = st + 13 * b + 3;
         return Qt<double>{p[0], p[1], p[2], p[3]};
     };
-    auto residual = [&](int ri) {
-        const int k = ri >> 2;
-        const int a = bbi[2 * k], b = bbi[2 * k + 1];
-        const V3<double> dir = ld3(rows + 7 * ri);
+    auto residual = [&](const Blk& q, int u) {
         double s = 0.0;
-        if (a >= 0) {
-            V3<double> wa = qrot(ori(a), ld3(bd + 6 * a + 3));
-            V3<double> vp = ld3(bd + 6 * a) + cross3(wa, ld3(blk + 8 * k));
-            s += dot3(vp, dir);
+        if (q.a >= 0) {
+            V3<double> wa = qrot(ori(q.a), ld3(bd + 6 * q.a + 3));
+            V3<double> vp = ld3(bd + 6 * q.a) + cross3(wa, q.arma);
+            s += dot3(vp, q.dir[u]);
         }
-        if (b >= 0) {
-            V3<double> wb = qrot(ori(b), ld3(bd + 6 * b + 3));
-            V3<double> vp = ld3(bd + 6 * b) + cross3(wb, ld3(blk + 8 * k + 3));
-            s -= dot3(vp, dir);
+        if (q.b >= 0) {
+            V3<double> wb = qrot(ori(q.b), ld3(bd + 6 * q.b + 3));
+            V3<double> vp = ld3(bd + 6 * q.b) + cross3(wb, q.armb);
+            s -= dot3(vp, q.dir[u]);
         }
         return s;
     };
@@ -545,37 +570,31 @@ __device__ __noinline__ void body_pgs_sweeps(int iterations, int nblk, double* _
         p[0] = dv.x; p[1] = dv.y; p[2] = dv.z;
         p[3] = dw.x; p[4] = dw.y; p[5] = dw.z;
     };
-    auto apply = [&](int ri, double df) {
+    auto apply = [&](const Blk& q, int u, double df) {
         if (df == 0.0) return;
-        const int k = ri >> 2;
-        const int a = bbi[2 * k], b = bbi[2 * k + 1];
-        const V3<double> dir = ld3(rows + 7 * ri);
-        if (a >= 0) side(a, dir, ld3(blk + 8 * k), df);
-        if (b >= 0) side(b, -dir, ld3(blk + 8 * k + 3), df);
+        if (q.a >= 0) side(q.a, q.dir[u], q.arma, df);
+        if (q.b >= 0) side(q.b, -q.dir[u], q.armb, df);
     };
-    auto non_negative = [&](int ri) {
-        double* r = rows + 7 * ri;
-        const double f = r[6];
-        const double res = residual(ri) + r[3] * f + r[5];
-        double delta = -res / r[4];
+    auto non_negative = [&](Blk& q, int u) {
+        const double f = q.f[u];
+        const double res = residual(q, u) + q.reg[u] * f + q.bias[u];
+        double delta = -res / q.diag[u];
         const double nf = f + delta;
         const double pr = nf < 0.0 ? 0.0 : nf;
         delta = pr - f;
-        r[6] = pr;
-        apply(ri, delta);
+        q.f[u] = pr;
+        apply(q, u, delta);
     };
-    auto pair = [&](int ri, double h01) {
-        const int rj = ri + 1;
-        double* r0 = rows + 7 * ri;
-        double* r1 = rows + 7 * rj;
-        const double fi = r0[6], fj = r1[6];
-        const double res_i = residual(ri) + r0[3] * fi + r0[5];
-        const double res_j = residual(rj) + r1[3] * fj + r1[5];
-        const double h00 = r0[4], h11 = r1[4];
+    auto pair = [&](Blk& q, int u, double h01) {
+        const int v = u + 1;
+        const double fi = q.f[u], fj = q.f[v];
+        const double res_i = residual(q, u) + q.reg[u] * fi + q.bias[u];
+        const double res_j = residual(q, v) + q.reg[v] * fj + q.bias[v];
+        const double h00 = q.diag[u], h11 = q.diag[v];
         const double det = h00 * h11 - h01 * h01;
         if (det <= 0.0) {
-            non_negative(ri);
-            non_negative(rj);
+            non_negative(q, u);
+            non_negative(q, v);
             return;
         }
         const double di = (-res_i * h11 + h01 * res_j) / det;
@@ -595,17 +614,26 @@ __device__ __noinline__ void body_pgs_sweeps(int iterations, int nblk, double* _
             a = nmax(fi - res_i / h00, 0.0);
             b = 0.0;
         }
-        r0[6] = a;
-        r1[6] = b;
-        apply(ri, a - fi);
-        apply(rj, b - fj);
+        q.f[u] = a;
+        q.f[v] = b;
+        apply(q, u, a - fi);
+        apply(q, v, b - fj);
     };
-    for (int it = 0; it < iterations; ++it)
+    for (int it = 0; it < iterations; ++it) {
+        Blk q;
+        if (nblk > 0) load(0, q);
         for (int k = 0; k < nblk; ++k) {
-            non_negative(4 * k);
-            pair(4 * k, blk[8 * k + 6]);
-            pair(4 * k + 2, blk[8 * k + 7]);
+            Blk nx;
+            if (k + 1 < nblk) load(k + 1, nx);  // issued before this block's chain
+            non_negative(q, 0);
+            pair(q, 0, q.h01);
+            pair(q, 2, q.h23);
+            double* r = rows + 28 * k;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) r[7 * u + 6] = q.f[u];
+            q = nx;
         }
+    }
 }
 
 struct BodyWave {
