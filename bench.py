@@ -303,6 +303,9 @@ def bench_boxes(args):
     launch_ms = sum(r["launch_ms"]) / len(r["launch_ms"])
     B = bytes_per_world_step(model.nq, model.nv)
     achieved = B * worlds * r["steps_per_launch"] / (launch_ms / 1e3) / 1e9
+    fpw, counts = flops_per_world_step(scene, qs, vs, 0, n=4, steps=5)
+    tflops = fpw * worlds * r["steps_per_launch"] / (launch_ms / 1e3) / 1e12
+    log(f"newt's operation count {fpw:.4g} flop/world-step -> {tflops:.3g} TFLOP/s")
     # PGS 20 on the same batch, and 16x the batch (one lane per world there)
     pgs_model = Model(pgs_scene)
     rp = time_gpu(pgs_model, worlds, abi.PRECISION_FP64, args.steps, args.warmup, args.chunk, q0, v0)
@@ -356,6 +359,10 @@ def bench_boxes(args):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_bodies_wave",
                      "bytes_per_world_step": B, "launch_ms": launch_ms,
                      "steps_per_launch": r["steps_per_launch"], "worlds": worlds},
+        "roofline_fp64": {"bound": "fp64", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                          "frac": tflops / FP64_PEAK_TFLOPS, "flop_per_world_step": fpw,
+                          "flop_counts_add_mul_div_sqrt": counts,
+                          "source": "oracle counting scalar, 4 worlds from the warm states, 5 steps"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
