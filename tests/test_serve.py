@@ -115,6 +115,36 @@ def test_served_equals_alone(precision):
 
 
 @pytest.mark.gpu
+def test_served_free_bodies_equal_alone():
+    """Free-body pools (the golden stack under PGS, the 64-box pile under
+    Newton) through the slot entry points: every served rollout equals the
+    same rollout run alone, bit for bit."""
+    spec = [("boxstack3", 20, 60, 0.0), ("boxpile64_newton", 6, 40, 0.0), ("boxstack3", 9, 45, 0.001),
+            ("boxpile64_newton", 5, 30, 0.002), ("boxstack3", 30, 20, 0.003)]
+    reqs = []
+    for i, (key, n, h, t) in enumerate(spec):
+        q, v = serve.scene_for(key).initial_state(n, 300 + i)
+        reqs.append(serve.Request(key, np.asarray(q, float), np.asarray(v, float), h, t))
+    pools = {"boxstack3": (serve.scene_for("boxstack3"), 40), "boxpile64_newton": (serve.scene_for("boxpile64_newton"), 8)}
+    say(f"serve free bodies: {len(reqs)} requests, chunk 16")
+    srv = serve.Server(pools, precision=abi.PRECISION_FP64, chunk=16, n_streams=2, hbm_budget=8 << 30)
+    try:
+        srv.run(reqs, time_limit=120.0)
+        models = {k: p.model for k, p in srv.pools.items()}
+        for r in reqs:
+            assert r.result is not None and (r.faults == 0).all()
+            d = Data(models[r.key], r.n_worlds, abi.PRECISION_FP64)
+            d.set_state(r.qpos, r.qvel)
+            step(models[r.key], d, r.horizon)
+            q, v = d.state()
+            d.close()
+            say(f"request {r.rid} {r.key} {r.n_worlds}x{r.horizon}: max |dq| {np.abs(q - r.result[0]).max():.3g}")
+            assert np.array_equal(q, r.result[0]) and np.array_equal(v, r.result[1])
+    finally:
+        srv.close()
+
+
+@pytest.mark.gpu
 def test_served_fp64_matches_oracle():
     from oracle import pyoracle
     reqs = _mix()[:3]

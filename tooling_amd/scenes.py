@@ -519,4 +519,5 @@ def box_pile(n_side=4, dt=0.002, iterations=20, seed=2026, friction=0.8) -> Scen
 
 SCENES = {"pendulum3": pendulum, "quadruped18": quadruped, "boxchain6": box_chain, "coil10": hinge_coil,
           "chain40": lambda: hinge_chain(40), "chain64": lambda: hinge_chain(64), "boxstack3": box_stack,
-          "boxpile64": box_pile}
+          "boxpile64": box_pile,
+          "boxpile64_newton": lambda: with_options(box_pile(), "_newton", solver=abi.SOLVER_NEWTON)}
