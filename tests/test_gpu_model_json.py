@@ -1,7 +1,9 @@
 """Models loaded from newt's JSON format, stepped on the GPU vs the CPU
 oracle on the same seeded states: newt's own models/pendulum.json (RK4,
 penalty mode, no contacts) and the quadruped written in the format
-(tests/golden/models/quadruped18.json).  EXACT bitwise, FP64 1e-9."""
+(tests/golden/models/quadruped18.json), and newt's models/stack.json (three
+free boxes) under PGS and Newton.  EXACT bitwise, FP64 1e-9 (free bodies:
+bitwise in every mode)."""
 import os
 import sys
 
@@ -66,3 +68,16 @@ def test_quadruped_json(oracle, precision):
     q0, v0 = scenes.quadruped().initial_state(64, 13)
     say(f"quadruped18.json precision {precision}: 64 worlds x 200 steps")
     _compare(oracle, sc, q0, v0, 200, precision)
+
+
+@pytest.mark.parametrize("solver", [abi.SOLVER_PGS, abi.SOLVER_NEWTON])
+def test_stack_json_free_bodies(oracle, solver):
+    """newt's stack.json loads as three free boxes; under PGS and Newton
+    (its own solver is penalty, not built) the GPU matches the oracle bit
+    for bit, with random velocity perturbations per world."""
+    sc = scenes.with_options(model_json.load(os.path.join(MODELS, "stack.json")), f"_s{solver}", solver=solver,
+                             iterations=30)
+    q0, v0 = sc.initial_state(32, 0)
+    v0 = v0 + np.random.default_rng(5).normal(0.0, 0.05, size=v0.shape)
+    say(f"stack.json solver {solver}: 32 worlds x 400 steps")
+    _compare(oracle, sc, q0, v0, 400, abi.PRECISION_FP64_EXACT)

@@ -29,7 +29,7 @@ def test_newt_pendulum_model_loads():
     assert sc.extra["sites"][0]["name"] == "tip"
 
 
-@pytest.mark.parametrize("name,path", [("arm", "$.actuators"), ("stack", "$.bodies")])
+@pytest.mark.parametrize("name,path", [("arm", "$.actuators")])
 def test_unbuilt_features_are_rejected_with_a_path(name, path):
     with pytest.raises(model_json.ModelError) as e:
         model_json.load(os.path.join(MODELS, f"{name}.json"))
@@ -98,3 +98,57 @@ def test_quadruped_json_matches_the_programmatic_scene():
         assert a["friction"] == pytest.approx(b["friction"], rel=1e-7)
     for k in ("integrator", "solver", "iterations"):
         assert js.options[k] == pr.options[k]
+
+
+def test_newt_stack_model_loads_free_bodies(oracle_lib):
+    """newt's stack.json (newt/tests/model_load.rs:202-221): three solid
+    boxes of half extent 0.35 with masses 1.2, 0.9, 1.5, the top one spinning
+    at 0.3 rad/s about body y, box geoms on the bodies, a static plane, all
+    in float32 as newt parses them.  Stepped by the oracle's float
+    instantiation under PGS, the loaded scene and the same scene built from
+    the reference's programmatic description give the same bytes."""
+    sc = model_json.load(os.path.join(MODELS, "stack.json"))
+    assert len(sc.links) == 0 and len(sc.bodies) == 3 and sc.extra["body_names"] == ["bottom", "middle", "top"]
+    f32 = lambda x: float(np.float32(x))
+    half = (0.35, 0.35, 0.35)
+    expect = [(1.2, (0.0, 0.0, 0.5), (0.0, 0.0, 0.0)), (0.9, (0.02, 0.0, 1.7), (0.0, 0.0, 0.0)),
+              (1.5, (0.0, 0.0, 2.9), (0.0, 0.3, 0.0))]
+    prog = []
+    for b, (m, p, w) in zip(sc.bodies, expect):
+        assert b["mass"] == f32(m)
+        assert b["position"] == tuple(f32(x) for x in p)
+        assert b["angular_velocity_body"] == tuple(f32(x) for x in w)
+        m32, h32 = np.float32(m), np.float32(0.35)
+        t = np.float32(m32 / np.float32(3.0))
+        i = float(np.float32(t * np.float32(h32 * h32 + h32 * h32)))
+        assert b["inertia"] == (i, 0, 0, 0, i, 0, 0, 0, i)
+        prog.append(scenes.body(f32(m), (i, 0, 0, 0, i, 0, 0, 0, i), tuple(f32(x) for x in p),
+                                angular_velocity_body=tuple(f32(x) for x in w)))
+    assert [g["body"] for g in sc.geoms] == [-1, 0, 1, 2]
+    assert sc.options["solver"] == abi.SOLVER_PENALTY
+    geoms = [scenes.geom(abi.SHAPE_PLANE, friction=f32(0.6))]
+    geoms += [scenes.geom(abi.SHAPE_BOX, body=k, size=tuple(f32(x) for x in half), friction=f32(0.6))
+              for k in range(3)]
+    pgs = dict(sc.options, solver=abi.SOLVER_PGS, iterations=30)
+    loaded = scenes.Scene("loaded", pgs, [], sc.geoms, 0, None, bodies=sc.bodies)
+    built = scenes.Scene("built", pgs, [], geoms, 0, None, bodies=prog)
+    out = []
+    for s2 in (loaded, built):
+        ow = oracle_lib.OracleWorlds(s2, 1, 1)
+        ow.step(500)
+        out.append(np.concatenate(ow.state(), axis=1).astype(np.float32).tobytes())
+        ow.close()
+    assert out[0] == out[1]
+
+
+def test_body_attachment_errors():
+    d = json.load(open(os.path.join(MODELS, "stack.json")))
+    d["geoms"][1]["attach"]["body"] = "nowhere"
+    with pytest.raises(model_json.ModelError) as e:
+        model_json.load(json.dumps(d))
+    assert e.value.path == "$.geoms[1].attach.body"
+    d = json.load(open(os.path.join(MODELS, "stack.json")))
+    d["bodies"][0]["rolling_friction"] = 0.1
+    with pytest.raises(model_json.ModelError) as e:
+        model_json.load(json.dumps(d))
+    assert e.value.path == "$.bodies[0].rolling_friction"

@@ -6,7 +6,8 @@ models the GPU step builds.
 strict-by-default rule: it rejects unknown fields at every scope, and every
 error names the JSON path of the offending value (``ModelError``).  It also
 rejects what the GPU step does not build, rather than dropping it:
-- free bodies, and more than one tree;
+- more than one tree, or a tree together with free bodies;
+- rolling friction on bodies;
 - actuators, equalities, tendons, meshes and heightfields;
 - ball joints, cylinders and ellipsoids;
 - explicit contact-pair lists.
@@ -174,7 +175,7 @@ def _solimp(v, path):
             _num(v.get("width", 0.001), f"{path}.width"), _num(v.get("midpoint", 0.5), f"{path}.midpoint"), power)
 
 
-UNBUILT = {"bodies": "free bodies", "actuators": "actuators", "equality": "equality constraints",
+UNBUILT = {"actuators": "actuators", "equality": "equality constraints",
            "tendons": "tendons", "meshes": "mesh assets", "hfields": "heightfields",
            "contact_pairs": "explicit contact-pair lists", "sensors": "sensors"}
 
@@ -218,22 +219,68 @@ def load(src) -> scenes.Scene:
         raise ModelError("$.solver.cone", "expected pyramidal or elliptic")
 
     trees = top.get("trees", [])
-    if len(trees) != 1:
-        raise ModelError("$.trees", "the GPU step builds models with exactly one tree")
-    tree = _obj(trees[0], "$.trees[0]", {"name", "self_collide", "links"})
-    tree_name = tree.get("name", "tree")
-    self_collide = tree.get("self_collide", False)
-    if not isinstance(self_collide, bool):
-        raise ModelError("$.trees[0].self_collide", "expected a bool")
-    links, index = _links(tree.get("links", []), "$.trees[0].links")
-    geoms = _geoms(top.get("geoms", []), tree_name, index)
+    bodies_v = top.get("bodies", [])
+    if not isinstance(bodies_v, list):
+        raise ModelError("$.bodies", "expected an array")
+    if bodies_v and trees:
+        raise ModelError("$.bodies", "the GPU step builds a tree or free bodies, not both")
+    bodies, body_index = _bodies(bodies_v)
+    tree_name, index, links, self_collide = None, {}, [], False
+    if not bodies:
+        if len(trees) != 1:
+            raise ModelError("$.trees", "the GPU step builds models with exactly one tree")
+        tree = _obj(trees[0], "$.trees[0]", {"name", "self_collide", "links"})
+        tree_name = tree.get("name", "tree")
+        self_collide = tree.get("self_collide", False)
+        if not isinstance(self_collide, bool):
+            raise ModelError("$.trees[0].self_collide", "expected a bool")
+        links, index = _links(tree.get("links", []), "$.trees[0].links")
+    geoms = _geoms(top.get("geoms", []), tree_name, index, body_index)
     sites = top.get("sites", [])
     opts = dict(dt=float(dt), gravity=tuple(float(g) for g in gravity), integrator=integ, solver=mode,
                 iterations=iterations, cone=cone)
-    sc = scenes.Scene(name, opts, links, geoms, 1 if self_collide else 0, None, f"loaded from JSON ({name})")
+    sc = scenes.Scene(name, opts, links, geoms, 1 if self_collide else 0, None, f"loaded from JSON ({name})",
+                      bodies=bodies)
     sc.extra["sites"] = sites
     sc.extra["link_names"] = list(index)
+    sc.extra["body_names"] = list(body_index)
+    if bodies:
+        q0 = np.array([x for b in bodies for x in (*b["position"], *b["orientation"])])
+        v0 = np.array([x for b in bodies for x in (*b["linear_velocity"], *b["angular_velocity_body"])])
+        sc.sampler = lambda n, seed: (np.tile(q0, (n, 1)), np.tile(v0, (n, 1)))
     return sc
+
+
+def _bodies(arr):
+    """parse_body (model.rs:1143-1186): mass, inertia, pose, velocity."""
+    out, index = [], {}
+    for i, b in enumerate(arr):
+        p = f"$.bodies[{i}]"
+        _obj(b, p, {"name", "mass", "inertia", "pose", "velocity", "rolling_friction"})
+        name = b.get("name")
+        if not isinstance(name, str) or not name or name in index:
+            raise ModelError(f"{p}.name", "expected a unique name")
+        if "mass" not in b:
+            raise ModelError(f"{p}.mass", "required")
+        mass = _num(b["mass"], f"{p}.mass", positive=True)
+        if "inertia" not in b:
+            raise ModelError(f"{p}.inertia", "required")
+        inertia = _inertia(b["inertia"], mass, f"{p}.inertia")
+        pos, quat = _pose(b.get("pose"), f"{p}.pose")
+        if b.get("rolling_friction") is not None:
+            raise ModelError(f"{p}.rolling_friction", "rolling friction is not built on the GPU path")
+        lin, ang = [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]
+        if "velocity" in b:
+            vel = _obj(b["velocity"], f"{p}.velocity", {"linear", "angular_body"})
+            if "linear" in vel:
+                lin = _vec(vel["linear"], 3, f"{p}.velocity.linear")
+            if "angular_body" in vel:
+                ang = _vec(vel["angular_body"], 3, f"{p}.velocity.angular_body")
+        index[name] = i
+        out.append(scenes.body(float(mass), tuple(float(x) for x in inertia), tuple(float(x) for x in pos),
+                               tuple(float(x) for x in quat), tuple(float(x) for x in lin),
+                               tuple(float(x) for x in ang)))
+    return out, index
 
 
 def _no_duplicates(pairs):
@@ -314,7 +361,8 @@ def _links(arr, path):
     return out, index
 
 
-def _geoms(arr, tree_name, index):
+def _geoms(arr, tree_name, index, body_index=None):
+    body_index = body_index or {}
     out = []
     shapes = {"plane": abi.SHAPE_PLANE, "sphere": abi.SHAPE_SPHERE, "box": abi.SHAPE_BOX, "capsule": abi.SHAPE_CAPSULE}
     for i, g in enumerate(arr):
@@ -336,16 +384,20 @@ def _geoms(arr, tree_name, index):
         else:
             size = (0.0, 0.0, 0.0)
         att = _obj(g.get("attach", {"kind": "static"}), f"{p}.attach", {"kind", "tree", "link", "body"})
+        tree, link, body = -1, -1, -1
         if att.get("kind") == "static":
-            tree, link = -1, -1
             if kind != "plane":
                 raise ModelError(f"{p}.attach", "only plane geoms may be static")
         elif att.get("kind") == "link":
-            if att.get("tree") != tree_name or att.get("link") not in index:
+            if tree_name is None or att.get("tree") != tree_name or att.get("link") not in index:
                 raise ModelError(f"{p}.attach", "must name the tree and one of its links")
             tree, link = 0, index[att["link"]]
+        elif att.get("kind") == "body":
+            if att.get("body") not in body_index:
+                raise ModelError(f"{p}.attach.body", "must name one of the bodies")
+            body = body_index[att["body"]]
         else:
-            raise ModelError(f"{p}.attach.kind", "expected static or link (free bodies are not built)")
+            raise ModelError(f"{p}.attach.kind", "expected static, link or body")
         pos = tuple(float(x) for x in _vec(g.get("local_offset", [0, 0, 0]), 3, f"{p}.local_offset"))
         quat = tuple(float(x) for x in _quat_normalized(_vec(g.get("local_orientation", [0, 0, 0, 1]), 4,
                                                              f"{p}.local_orientation")))
@@ -353,7 +405,7 @@ def _geoms(arr, tree_name, index):
         if condim not in (1, 3, 4, 6):
             raise ModelError(f"{p}.condim", "expected 1, 3, 4 or 6")
         si = _solimp(g.get("solimp"), f"{p}.solimp")
-        out.append(scenes.geom(shapes[kind], tree=tree, link=link, size=size, pos=pos, quat=quat,
+        out.append(scenes.geom(shapes[kind], tree=tree, link=link, body=body, size=size, pos=pos, quat=quat,
                                friction=float(_num(g.get("friction", 0.5), f"{p}.friction", nonneg=True)),
                                margin=float(_num(g.get("margin", 0.0), f"{p}.margin", nonneg=True)),
                                gap=float(_num(g.get("gap", 0.0), f"{p}.gap", nonneg=True)),
