@@ -435,6 +435,7 @@ namespace newt {
 // Offsets (elements) of one world's LDS block and its global fallback.
 struct BodyWaveLayout {
     int st, iiw, bd, wr, fat, rows, blk;  // doubles
+    int scr;                              // 64 doubles: one value per lane, for broadcasts
     int ints_at;                          // doubles before the int region
     int bbi, misc;                        // ints, after the doubles
     int lds_bytes;                        // bytes per world
@@ -455,7 +456,7 @@ constexpr int kNewtonMaxRows = 1024;
 inline BodyWaveLayout make_body_wave_layout(const ModelT& m, int lds_budget_bytes) {
     BodyWaveLayout B{};
     const int nb = m.n_bodies, ng = m.n_geoms;
-    const int fixed_d = 34 * nb + 6 * ng;
+    const int fixed_d = 34 * nb + 6 * ng + 64;
     int lc = (lds_budget_bytes - 8 * fixed_d - 4 * 8) / (8 * 36 + 4 * 2);
     lc = std::max(0, std::min(lc, m.cap_c));
     int d = 0;
@@ -465,6 +466,7 @@ inline BodyWaveLayout make_body_wave_layout(const ModelT& m, int lds_budget_byte
     B.bd = D(6 * nb);
     B.wr = D(6 * nb);
     B.fat = D(6 * ng);
+    B.scr = D(64);
     B.rows = D(28 * lc);
     B.blk = D(8 * lc);
     B.ints_at = d;
@@ -1093,20 +1095,14 @@ struct BodyWave {
         // loop.  L is kept row-major (Lf, read by the factorization eight
         // entries at a time so the loads overlap) and column-major (Lt, read
         // by the substitutions).
+        // The diagonal sums H[i][i] - L[i][0]^2 - L[i][1]^2 - ... are carried
+        // in `pt`, each term subtracted right after its L[i][k] is computed
+        // (k increasing, the row-wise order).
+        for (int i = lane; i < n; i += 64) pt[i] = H[(size_t)i * n + i];
+        gsync();
         for (int j = 0; j < n; ++j) {
             const double* lj = Lf + (size_t)j * n;
-            double sum = H[(size_t)j * n + j];
-            {
-                int k = 0;
-                for (; k + 8 <= j; k += 8) {
-                    double a[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) a[u] = lj[k + u];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) sum -= a[u] * a[u];
-                }
-                for (; k < j; ++k) sum -= lj[k] * lj[k];
-            }
+            const double sum = pt[j];
             if (sum <= 0.0) return 4;  // NEWT_ERR_NEWTON_FAIL: Hessian not positive definite
             const double ljj = __builtin_sqrt(sum);
             if (lane == 0) {
@@ -1140,9 +1136,11 @@ struct BodyWave {
                 const double q0 = s0 / ljj, q1 = s1 / ljj;
                 Lf[(size_t)i0 * n + j] = q0;
                 Lt[(size_t)j * n + i0] = q0;
+                pt[i0] -= q0 * q0;
                 if (v1) {
                     Lf[(size_t)i1 * n + j] = q1;
                     Lt[(size_t)j * n + i1] = q1;
+                    pt[i1] -= q1 * q1;
                 }
             }
             gsync();
@@ -1171,24 +1169,70 @@ struct BodyWave {
                     y[i] -= ltk[i] * yk;
             }
             gsync();
-            // backward, one lane (sum over k > i in increasing k)
-            if (lane == 0) {
-                for (int i = n - 1; i >= 0; --i) {
+            // backward: x[i] = (y[i] - sum_{k>i} L[k][i] x[k]) / L[i][i], the sum
+            // in increasing k.  Lane k mod 64 keeps x[k] in a register (chunk
+            // k / 64 of at most 16) and forms the product L[k][i] x[k]; every
+            // lane runs the subtraction chain on the broadcast products in k
+            // order.  Row i - 1's factor entries, y[i - 1] and L[i-1][i-1] load
+            // while row i is summed, so no load sits on the chain.
+            {
+                double xr[16], lr[16], ln[16];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    xr[c] = 0.0;
+                    lr[c] = 0.0;
+                }
+                auto load_row = [&](int i, double* out) {
                     const double* lti = Lt + (size_t)i * n;
-                    double sum = y[i];
-                    int k = i + 1;
-                    for (; k + 8 <= n; k += 8) {
-                        double a[8], b[8];
 #pragma unroll
-                        for (int u = 0; u < 8; ++u) {
-                            a[u] = lti[k + u];
-                            b[u] = dd[k + u];
-                        }
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) sum -= a[u] * b[u];
+                    for (int c = 0; c < 16; ++c) {
+                        const int k = 64 * c + lane;
+                        out[c] = (k > i && k < n) ? lti[k] : 0.0;
                     }
-                    for (; k < n; ++k) sum -= lti[k] * dd[k];
-                    dd[i] = sum / lti[i];
+                };
+                double yi = 0.0, di = 1.0;
+                if (n > 0) {
+                    yi = y[n - 1];
+                    di = Lt[(size_t)(n - 1) * n + (n - 1)];
+                    load_row(n - 1, lr);
+                }
+                for (int i = n - 1; i >= 0; --i) {
+                    double ny = 0.0, nd = 1.0;
+                    if (i > 0) {
+                        ny = y[i - 1];
+                        nd = Lt[(size_t)(i - 1) * n + (i - 1)];
+                        load_row(i - 1, ln);
+                    }
+                    double sum = yi;
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) {
+                        const int b0 = 64 * c;
+                        if (b0 + 63 <= i || b0 >= n) continue;
+                        const double p = lr[c] * xr[c];
+                        const int lo = (i + 1 > b0 ? i + 1 : b0) - b0;
+                        const int hi = (n < b0 + 64 ? n : b0 + 64) - b0;
+                        if (lo == 0 && hi == 64) {
+#pragma unroll
+                            for (int u = 0; u < 64; ++u) sum -= readlane_f64(p, u);
+                        } else {
+#pragma unroll
+                            for (int u = 0; u < 64; ++u) {
+                                const double pu = readlane_f64(p, u);
+                                if (u >= lo && u < hi) sum -= pu;
+                            }
+                        }
+                    }
+                    const double xi = sum / di;
+#pragma unroll
+                    for (int c = 0; c < 16; ++c)
+                        if (64 * c + lane == i) xr[c] = xi;
+                    if (lane == (i & 63)) dd[i] = xi;
+                    if (i > 0) {
+                        yi = ny;
+                        di = nd;
+#pragma unroll
+                        for (int c = 0; c < 16; ++c) lr[c] = ln[c];
+                    }
                 }
             }
             gsync();
