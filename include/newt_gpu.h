@@ -126,12 +126,16 @@ typedef struct newt_link_desc {
     double child_quat[4];    /* joint_offset_in_child.1 */
     double mass;
     double inertia[9];       /* inertia_body, column-major */
-    /* JointLimit::solref / solimp of the PGS limit row (joint.rs:124-141);
-     * limit_custom = 0 takes SolRef::DEFAULT and SolImp::DEFAULT. */
+    /* JointLimit (joint.rs:124-165): solref / solimp of the PGS limit row
+     * and the penalty spring (stiffness, damping) of the penalty solver;
+     * limit_custom = 0 takes JointLimit::DEFAULT (1000, 60, SolRef::DEFAULT,
+     * SolImp::DEFAULT). */
     int32_t limit_custom;
     int32_t limit_solimp_power;
     double limit_solref[2];
     double limit_solimp[4];
+    double limit_stiffness;
+    double limit_damping;
 } newt_link_desc;
 
 /* One free Body (newt/src/body.rs:22-47): inertia about the COM in body

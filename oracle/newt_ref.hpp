@@ -494,6 +494,7 @@ template <class T> struct Link {
     T limit_solref[2] = {T(0.02), T(1.0)};                    // JointLimit::solref or SolRef::DEFAULT
     T limit_solimp[4] = {T(0.9), T(0.95), T(0.001), T(0.5)};  // JointLimit::solimp or SolImp::DEFAULT
     int limit_power = 2;
+    T limit_stiffness = T(1000.0), limit_damping = T(60.0);  // JointLimit penalty spring (JointLimit::DEFAULT)
     V3<T> pj_pos = vzero<T>();
     Q<T> pj_quat = qidentity<T>();
     V3<T> cj_pos = vzero<T>();
@@ -899,6 +900,24 @@ void aba_run(const Tree<T>& t, const std::vector<T>& q, const std::vector<T>& qd
 
 enum VelImplicit { VI_EXPLICIT = 0, VI_JOINT_DAMPING = 1, VI_JOINT_DAMPING_ACT = 2 };
 
+// joint_limit_scalar_force (tree.rs:1170-1207): the one-sided penalty
+// spring-damper outside a hinge/slide range.
+template <class T> T joint_limit_scalar_force(T q, T qdot, T lo, T hi, T k, T c) {
+    if (q < lo) {
+        T violation = lo - q;
+        T spring = k * violation;
+        T damp = qdot < T(0.0) ? -c * qdot : T(0.0);
+        return spring + damp;
+    }
+    if (q > hi) {
+        T violation = q - hi;
+        T spring = -k * violation;
+        T damp = qdot > T(0.0) ? -c * qdot : T(0.0);
+        return spring + damp;
+    }
+    return T(0.0);
+}
+
 // assemble_joint_forces (forces.rs:6-85) for trees without tendons or
 // actuators; joint_limit_scalar_force (tree.rs:1170-1207).
 template <class T>
@@ -924,10 +943,9 @@ Force<T> assemble_joint_forces(const Tree<T>& t, std::vector<T>& scalar, std::ve
                 T qq = t.q[t.q_offset[i]];
                 T qd = t.qdot[t.v_offset[i]];
                 T tau_limit = T(0.0);
-                if (!t.disable_penalty_limits && l.has_range) {
-                    fail("penalty joint limits are not restated (no ranged joints on this path)");
-                }
-                (void)qq;
+                if (!t.disable_penalty_limits && l.has_range)
+                    tau_limit = joint_limit_scalar_force(qq, qd, l.range_lo, l.range_hi, l.limit_stiffness,
+                                                         l.limit_damping);
                 T tau_act = T(-0.0);  // empty Sum
                 scalar[i] = (((t.qfrc_applied[t.v_offset[i]] + tendon) - l.damping * qd) + tau_limit) + tau_act;
                 break;

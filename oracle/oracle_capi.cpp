@@ -72,6 +72,9 @@ template <class T> struct Impl : Base {
             for (int k = 0; k < 2; ++k) l.limit_solref[k] = T(d.limit_custom ? d.limit_solref[k] : sr[k]);
             for (int k = 0; k < 4; ++k) l.limit_solimp[k] = T(d.limit_custom ? d.limit_solimp[k] : si[k]);
             l.limit_power = d.limit_custom ? d.limit_solimp_power : 2;
+            // JointLimit::DEFAULT (joint.rs:152-157)
+            l.limit_stiffness = T(d.limit_custom ? d.limit_stiffness : 1000.0);
+            l.limit_damping = T(d.limit_custom ? d.limit_damping : 60.0);
             l.pj_pos = v3(T(d.parent_pos[0]), T(d.parent_pos[1]), T(d.parent_pos[2]));
             l.pj_quat = Q<T>{T(d.parent_quat[0]), T(d.parent_quat[1]), T(d.parent_quat[2]), T(d.parent_quat[3])};
             l.cj_pos = v3(T(d.child_pos[0]), T(d.child_pos[1]), T(d.child_pos[2]));

@@ -155,3 +155,82 @@ def test_reference_anchor_on_gpu(oracle):
         assert abs(q - ow.state()[0][0, 0]) <= TOL_FP64
     say(f"anchor peak {peak:.4f}, final {q:.4f}")
     assert peak < math.pi / 3 + 0.2
+
+
+def penalty(sc, lo, hi, stiffness=None, damping=None):
+    """Hinge ranges under the penalty solver: joint_limit_scalar_force
+    (tree.rs:1170-1207) enters the ABA bias force instead of limit rows."""
+    links = [dict(l, range=(lo, hi) if l["joint"] == abi.JOINT_HINGE else None, limit_stiffness=stiffness,
+                  limit_damping=damping) for l in sc.links]
+    return scenes.Scene(sc.name + "_penalty_ranged", dict(sc.options, solver=abi.SOLVER_PENALTY), links, sc.geoms,
+                        sc.self_collide, sc.sampler, sc.description + f"; penalty hinge ranges ({lo}, {hi})")
+
+
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP64_EXACT, abi.PRECISION_FP64, abi.PRECISION_MIXED])
+@pytest.mark.parametrize("integrator", [abi.INTEGRATOR_EULER, abi.INTEGRATOR_RK4])
+@pytest.mark.parametrize("custom", [False, True])
+def test_pendulum_penalty_limits(oracle, precision, integrator, custom):
+    """Ranged joints under the penalty solver run the lane-per-world FP64
+    step in every precision mode: state, qacc and qfrc_applied bit for bit
+    with the FP64 oracle (JointLimit::DEFAULT 1000 / 60, or 2500 / 15)."""
+    k, c = (2500.0, 15.0) if custom else (None, None)
+    sc = scenes.with_options(penalty(scenes.pendulum(dt=0.0005), -1.0, 1.2, k, c), f"_i{integrator}",
+                             integrator=integrator)
+    say(f"{sc.name} custom {custom} precision {precision}: 128 worlds x 200 steps, qfrc_applied ~ N(0, 2)")
+    out, ref = _run(oracle, sc, 128, 63, 200, precision, qfrc_scale=2.0)
+    _check(out, ref, abi.PRECISION_FP64_EXACT)
+    q = out[0]
+    assert np.isfinite(q).all()
+    assert ((q < -1.0) | (q > 1.2)).any()   # springs engaged at the end
+
+
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP64_EXACT, abi.PRECISION_MIXED])
+def test_chain_penalty_limits(oracle, precision):
+    """A 16-link hinge chain (no geoms) with every hinge ranged: many springs
+    active at once across the tree, bit for bit."""
+    base = scenes.hinge_chain(16)
+    sc = penalty(scenes.Scene(base.name, dict(base.options), base.links, [], 0, base.sampler, base.description),
+                 -0.3, 0.3, 400.0, 2.0)
+    for warm, steps in ((0, 200), (400, 100)):
+        say(f"{sc.name} precision {precision}: 64 worlds x {steps} steps from step {warm}")
+        out, ref = _run(oracle, sc, 64, 75, steps, precision, qfrc_scale=0.2, warm=warm)
+        _check(out, ref, abi.PRECISION_FP64_EXACT)
+        assert np.isfinite(out[0]).all()
+
+
+def test_penalty_reference_anchor_on_gpu(oracle):
+    """newt/tests/joints_damping_armature_limits.rs:147-211 in FP64 on the
+    GPU: a unit rod released 0.3 rad above its range (-0.5, 0.5) with limit
+    (2000, 40) under RK4, dt 1 ms, ends within 0.05 of the range after 6 s,
+    its late violations no larger than its early ones, bit for bit with the
+    FP64 oracle all the way."""
+    lo, hi = -0.5, 0.5
+    links = [scenes.link(-1, abi.JOINT_FIXED),
+             scenes.link(0, abi.JOINT_HINGE, axis=(1.0, 0.0, 0.0), child_pos=(0.0, 0.0, 0.5), mass=1.0,
+                         inertia=(1.0 / 12.0, 0, 0, 0, 1.0 / 12.0, 0, 0, 0, 1e-6), damping=0.05, range=(lo, hi),
+                         limit_stiffness=2000.0, limit_damping=40.0)]
+    opts = dict(dt=0.001, gravity=(0.0, 0.0, -9.81), integrator=abi.INTEGRATOR_RK4, solver=abi.SOLVER_PENALTY,
+                iterations=1)
+    sc = scenes.Scene("penalty_anchor", opts, links, [], 0, None)
+    m = Model(sc)
+    d = Data(m, 1, abi.PRECISION_FP64)
+    d.set_state(np.array([[hi + 0.3]]), np.zeros((1, 1)))
+    ow = oracle.OracleWorlds(sc, 1, 0)
+    ow.set_state(np.array([[hi + 0.3]]), np.zeros((1, 1)))
+    early = late = 0.0
+    for i in range(60):
+        step(m, d, 100)
+        ow.step(100)
+        q = d.state()[0][0, 0]
+        assert q == ow.state()[0][0, 0]
+        v = max(q - hi, lo - q, 0.0)
+        if i < 15:
+            early = max(early, v)
+        if i > 45:
+            late = max(late, v)
+    d.close()
+    m.close()
+    ow.close()
+    say(f"penalty anchor early violation {early:.4f}, late {late:.4f}, final {q:.4f}")
+    assert lo - 0.05 <= q <= hi + 0.05
+    assert late <= early

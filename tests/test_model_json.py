@@ -71,12 +71,13 @@ def test_limits_and_solver_fields():
     d = _pendulum_doc()
     d["solver"] = {"mode": "pgs", "iterations": 30}
     d["trees"][0]["links"][1]["joint"].update(range=[-0.5, 0.5], damping=0.25, armature=0.01,
-                                              limit={"stiffness": 0, "damping": 0,
+                                              limit={"stiffness": 2000, "damping": 40,
                                                      "solref": {"timeconst": 0.05, "dampratio": 0.9}})
     sc = model_json.load(json.dumps(d))
     l1 = sc.links[1]
     assert l1["range"] == (-0.5, 0.5) and l1["damping"] == float(np.float32(0.25))
     assert l1["limit_solref"] == pytest.approx((0.05, 0.9)) and l1["limit_solimp"] is None
+    assert l1["limit_stiffness"] == 2000.0 and l1["limit_damping"] == 40.0
     assert sc.options["solver"] == abi.SOLVER_PGS and sc.options["iterations"] == 30
 
 

@@ -121,15 +121,78 @@ def test_reference_anchor_solref_override(oracle_lib):
     assert ps < 0.05 and pf < 0.5 and pf > ps + 0.001
 
 
-def test_limits_without_pgs_are_rejected():
-    """Penalty limit torques are not built: such a model fails at finalize
+def test_limits_with_newton_are_rejected():
+    """Newton limit rows are not built: such a model fails at finalize
     instead of running with the limit silently missing."""
     from tooling_amd.newt import Model
     sc = _hinge(1.0, 1.0, 0.01, (-1.0, 1.0))
-    sc = scenes.with_options(sc, "_penalty", solver=abi.SOLVER_PENALTY)
+    sc = scenes.with_options(sc, "_newton", solver=abi.SOLVER_NEWTON)
     try:
         abi.load_library()
     except RuntimeError:
         pytest.skip("HIP library not built")
     with pytest.raises(abi.NewtError):
         Model(sc)
+
+
+def _penalty_hinge(q, qd, k, c, lo=-1.2, hi=0.9, l=0.6, m=1.4, ixx=0.03, dt=0.001):
+    links = [scenes.link(-1, abi.JOINT_FIXED, mass=1.0, inertia=(1, 0, 0, 0, 1, 0, 0, 0, 1)),
+             scenes.link(0, abi.JOINT_HINGE, axis=(1.0, 0.0, 0.0), child_pos=(0.0, 0.0, l), mass=m,
+                         inertia=(ixx, 0, 0, 0, ixx, 0, 0, 0, ixx), range=(lo, hi), limit_stiffness=k,
+                         limit_damping=c)]
+    opts = dict(dt=dt, gravity=(0.0, 0.0, -9.81), integrator=abi.INTEGRATOR_EULER, solver=abi.SOLVER_PENALTY,
+                iterations=1)
+    return scenes.Scene("hinge_penalty", opts, links, [], 0, None)
+
+
+@pytest.mark.parametrize("q,qd", [(0.95, 0.3), (0.95, -0.2), (-1.3, -0.4), (-1.3, 0.5), (0.2, 1.0)])
+def test_penalty_limit_closed_form(oracle_lib, q, qd):
+    """joint_limit_scalar_force (tree.rs:1170-1207) under the penalty solver:
+    outside the range a one-sided spring k * violation toward the range plus
+    a damper -c * qdot only while moving further out; zero inside.  One Euler
+    step's qacc is (tau_limit + gravity torque) / M."""
+    l, m, ixx, k, c = 0.6, 1.4, 0.03, 700.0, 25.0
+    lo, hi = -1.2, 0.9
+    ow = oracle_lib.OracleWorlds(_penalty_hinge(q, qd, k, c), 1, 0)
+    ow.set_state(np.array([[q]]), np.array([[qd]]))
+    ow.step(1)
+    a = ow.qacc()[0, 0]
+    L = oracle_lib.lib()
+    sin_q = 2.0 * L.oracle_nsin64(q * 0.5) * L.oracle_ncos64(q * 0.5)
+    tau_g = -m * 9.81 * l * sin_q
+    if q < lo:
+        tau = k * (lo - q) + (-c * qd if qd < 0 else 0.0)
+    elif q > hi:
+        tau = -k * (q - hi) + (-c * qd if qd > 0 else 0.0)
+    else:
+        tau = 0.0
+    assert a == pytest.approx((tau + tau_g) / (m * l * l + ixx), rel=1e-12, abs=1e-12)
+
+
+def test_penalty_limit_reference_anchor(oracle_lib):
+    """newt/tests/joints_damping_armature_limits.rs:147-211 on the oracle's
+    float instantiation: a unit rod (COM 0.5 m out, I = m l^2 / 12) on a
+    hinge with range (-0.5, 0.5), damping 0.05 and limit (2000, 40), released
+    at 0.8 under RK4 with dt 1 ms, ends within 0.05 of the range after 6 s
+    and its late violations do not exceed its early ones."""
+    lo, hi = -0.5, 0.5
+    links = [scenes.link(-1, abi.JOINT_FIXED, mass=1.0, inertia=(1, 0, 0, 0, 1, 0, 0, 0, 1)),
+             scenes.link(0, abi.JOINT_HINGE, axis=(1.0, 0.0, 0.0), child_pos=(0.0, 0.0, 0.5), mass=1.0,
+                         inertia=(1.0 / 12.0, 0, 0, 0, 1.0 / 12.0, 0, 0, 0, 1e-6), damping=0.05, range=(lo, hi),
+                         limit_stiffness=2000.0, limit_damping=40.0)]
+    opts = dict(dt=0.001, gravity=(0.0, 0.0, -9.81), integrator=abi.INTEGRATOR_RK4, solver=abi.SOLVER_PENALTY,
+                iterations=1)
+    ow = oracle_lib.OracleWorlds(scenes.Scene("anchor", opts, links, [], 0, None), 1, 1)
+    ow.set_state(np.array([[hi + 0.3]]), np.zeros((1, 1)))
+    early = late = 0.0
+    for step_ in range(60):
+        ow.step(100)
+        qq = ow.state()[0][0, 0]
+        v = max(qq - hi, lo - qq, 0.0)
+        if step_ < 15:
+            early = max(early, v)
+        if step_ > 45:
+            late = max(late, v)
+    ow.close()
+    assert lo - 0.05 <= qq <= hi + 0.05
+    assert late <= early

@@ -62,6 +62,8 @@ class LinkDesc(C.Structure):
         ("limit_solimp_power", C.c_int32),
         ("limit_solref", C.c_double * 2),
         ("limit_solimp", C.c_double * 4),
+        ("limit_stiffness", C.c_double),
+        ("limit_damping", C.c_double),
     ]
 
 

@@ -255,10 +255,8 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
         if (d.joint == NEWT_JOINT_BALL) return cfail(err, NEWT_ERR_UNSUPPORTED, "ball joints are not built on the GPU path");
         if (d.has_range && !(d.joint == NEWT_JOINT_HINGE || d.joint == NEWT_JOINT_SLIDE))
             return cfail(err, NEWT_ERR_INVALID_ARG, "a range needs a hinge or slide joint");
-        if (d.has_range && o.solver_mode != NEWT_SOLVER_PGS)
-            return cfail(err, NEWT_ERR_UNSUPPORTED,
-                         "joint ranges are built as PGS limit rows only (penalty limit torques and Newton "
-                         "limit rows are not built)");
+        if (d.has_range && o.solver_mode == NEWT_SOLVER_NEWTON)
+            return cfail(err, NEWT_ERR_UNSUPPORTED, "Newton joint-limit rows are not built");
         LinkT l{};
         l.lim = -1;
         if (d.has_range) {
@@ -270,6 +268,9 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
             for (int k = 0; k < 2; ++k) lt.solref[k] = d.limit_custom ? d.limit_solref[k] : sr[k];
             for (int k = 0; k < 4; ++k) lt.solimp[k] = d.limit_custom ? d.limit_solimp[k] : sim[k];
             lt.power = d.limit_custom ? d.limit_solimp_power : 2;
+            // JointLimit::DEFAULT stiffness / damping (joint.rs:152-157)
+            lt.stiffness = d.limit_custom ? d.limit_stiffness : 1000.0;
+            lt.damping = d.limit_custom ? d.limit_damping : 60.0;
             lt.link = (int)i;
             l.lim = (int)M->limits.size();
             M->limits.push_back(lt);
@@ -418,6 +419,7 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     d.nv = nv;
     d.n_bodies = 0;
     d.bodies = nullptr;
+    d.penalty_limits = !M->limits.empty() && o.solver_mode == NEWT_SOLVER_PENALTY;
     d.n_geoms = (int)M->geoms.size();
     d.n_pairs = (int)M->pairs.size();
     d.cap_c = std::max(cap_c, 1);

@@ -224,8 +224,9 @@ newt_status launch(newt_data* D, int n_steps, hipStream_t stream, int init) {
         HIP_TRY(hipGetLastError());
         return NEWT_OK;
     }
-    if (!init && D->precision != NEWT_PRECISION_FP64_EXACT) return launch_persistent(D, n_steps, stream);
-    if (D->precision == NEWT_PRECISION_MIXED)
+    if (!init && D->precision != NEWT_PRECISION_FP64_EXACT && !M->dev.penalty_limits)
+        return launch_persistent(D, n_steps, stream);
+    if (D->precision == NEWT_PRECISION_MIXED && !M->dev.penalty_limits)
         hipLaunchKernelGGL(k_step<float>, grid, block, 0, stream, M->dev, D->L, D->d, (float*)D->ws, D->dws, D->iws,
                            n_steps, init);
     else
@@ -416,7 +417,9 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
         return fail(NEWT_ERR_INVALID_ARG, "unknown precision");
     // Free-body models run the lane-per-world FP64 step in every precision.
     const bool bodies = model->dev.n_bodies > 0;
-    const bool team = !bodies && precision != NEWT_PRECISION_FP64_EXACT;
+    // Ranged joints under the penalty solver step on the lane-per-world
+    // FP64 kernel in every precision.
+    const bool team = !bodies && !model->dev.penalty_limits && precision != NEWT_PRECISION_FP64_EXACT;
     if ((model->dev.nv > 64 || model->dev.n_links > 128) && team)
         return fail(NEWT_ERR_UNSUPPORTED, "the wavefront-per-world step handles nv <= 64 and <= 128 links; "
                                           "use NEWT_PRECISION_FP64_EXACT for larger trees");
@@ -430,7 +433,9 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
     D->model = model;
     D->W = n_worlds;
     D->precision = precision;
-    D->L = bodies ? make_body_layout(model->dev) : make_layout(model->dev, precision == NEWT_PRECISION_MIXED ? 4 : 8);
+    // The FP32 workspace serves the mixed tree kernels only.
+    const bool ws32 = precision == NEWT_PRECISION_MIXED && !bodies && !model->dev.penalty_limits;
+    D->L = bodies ? make_body_layout(model->dev) : make_layout(model->dev, ws32 ? 4 : 8);
     D->TL = TL;
     D->wpb = std::max(wpb, 1);
     D->lds_bytes = cache_bytes + D->wpb * TL.bytes;
@@ -445,7 +450,7 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
     }
     const ModelT& m = model->dev;
     size_t W = (size_t)n_worlds;
-    size_t rsz = precision == NEWT_PRECISION_MIXED ? sizeof(float) : sizeof(double);
+    size_t rsz = ws32 ? sizeof(float) : sizeof(double);
     auto alloc = [&](void** p, size_t bytes) -> bool {
         D->device_bytes += std::max<size_t>(bytes, 8);
         return hipMalloc(p, std::max<size_t>(bytes, 8)) == hipSuccess;

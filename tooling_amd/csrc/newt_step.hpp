@@ -545,7 +545,19 @@ template <class R> struct World {
                 double d = (sdot(s, ias) + l.armature) + dm;
                 SF<double> ps = pa + m6mul(a, c);
                 double tauv = dw[L.tau + l.v_off];
-                double fs = (((tauv + double(0.0)) - l.damping * qvel[l.v_off]) + double(0.0)) + double(-0.0);
+                // joint_limit_scalar_force (tree.rs:1170-1207): the penalty
+                // spring, while the constraint solver has not disabled it.
+                double tl = 0.0;
+                if (!free_applied && l.lim >= 0) {
+                    const LimitT& lm = m.limits[l.lim];
+                    const double q = qpos[l.q_off], qd = qvel[l.v_off];
+                    if (q < lm.lo) {
+                        tl = lm.stiffness * (lm.lo - q) + (qd < 0.0 ? -lm.damping * qd : 0.0);
+                    } else if (q > lm.hi) {
+                        tl = -lm.stiffness * (q - lm.hi) + (qd > 0.0 ? -lm.damping * qd : 0.0);
+                    }
+                }
+                double fs = (((tauv + double(0.0)) - l.damping * qvel[l.v_off]) + tl) + double(-0.0);
                 dw[L.fs + i] = fs;
                 double u = fs - sdot(s, ps);
                 double qdd = u / d;

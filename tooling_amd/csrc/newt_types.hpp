@@ -36,6 +36,7 @@ struct LimitT {
     double lo, hi;
     double solref[2], solimp[4];
     int power, link;
+    double stiffness, damping;  // penalty spring (penalty solver)
 };
 
 struct GeomT {
@@ -83,6 +84,7 @@ struct ModelT {
     const LimitT* limits;
     int ext;       // needs the extended persistent step: limits, RK4 or non-plane pairs
     int n_bodies;  // free bodies (a model has a tree or free bodies, not both)
+    int penalty_limits;  // ranged joints under the penalty solver: the lane-per-world step only
     const BodyT* bodies;
 };
 

@@ -335,9 +335,14 @@ def _links(arr, path):
             rng = (float(lo), float(hi))
         damping = _num(j.get("damping", 0.0), f"{p}.joint.damping", nonneg=True)
         armature = _num(j.get("armature", 0.0), f"{p}.joint.armature", nonneg=True)
-        lsr = lsi = None
+        lsr = lsi = lk = lc = None
         if "limit" in j:
+            # model.rs:679-735: stiffness and damping required, solref/solimp optional
             lim = _obj(j["limit"], f"{p}.joint.limit", {"stiffness", "damping", "solref", "solimp"})
+            if "stiffness" not in lim or "damping" not in lim:
+                raise ModelError(f"{p}.joint.limit", "stiffness and damping are required")
+            lk = float(_num(lim["stiffness"], f"{p}.joint.limit.stiffness", nonneg=True))
+            lc = float(_num(lim["damping"], f"{p}.joint.limit.damping", nonneg=True))
             if "solref" in lim:
                 lsr = tuple(float(x) for x in _solref(lim["solref"], f"{p}.joint.limit.solref"))
             if "solimp" in lim:
@@ -355,7 +360,7 @@ def _links(arr, path):
                         inertia=tuple(float(x) for x in inertia),
                         damping=float(damping) if kind != "free" else 0.0, armature=float(armature),
                         free_damping=float(damping) if kind == "free" else 0.0, range=rng, limit_solref=lsr,
-                        limit_solimp=lsi)
+                        limit_solimp=lsi, limit_stiffness=lk, limit_damping=lc)
         out.append(d)
         index[name] = i
     return out, index

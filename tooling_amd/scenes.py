@@ -51,14 +51,18 @@ def quat_axis_angle(axis, angle):
 
 def link(parent, joint, *, axis=(0.0, 0.0, 0.0), parent_pos=(0.0, 0.0, 0.0), parent_quat=IDENTITY_Q,
          child_pos=(0.0, 0.0, 0.0), mass=1.0, inertia=_diag(1.0, 1.0, 1.0), damping=0.0, armature=0.0,
-         free_damping=0.0, range=None, limit_solref=None, limit_solimp=None):
-    """`range` (lo, hi) adds PGS limit rows; `limit_solref` (timeconst,
-    dampratio) and `limit_solimp` (dmin, dmax, width, midpoint, power)
-    override SolRef::DEFAULT / SolImp::DEFAULT for them (joint.rs:124-141)."""
+         free_damping=0.0, range=None, limit_solref=None, limit_solimp=None, limit_stiffness=None,
+         limit_damping=None):
+    """`range` (lo, hi) adds PGS limit rows under PGS and a penalty spring
+    under the penalty solver; `limit_solref` (timeconst, dampratio) and
+    `limit_solimp` (dmin, dmax, width, midpoint, power) override
+    SolRef::DEFAULT / SolImp::DEFAULT for the rows, `limit_stiffness` /
+    `limit_damping` the spring's JointLimit::DEFAULT 1000 / 60
+    (joint.rs:124-165)."""
     return dict(parent=parent, joint=joint, axis=axis, parent_pos=parent_pos, parent_quat=parent_quat,
                 child_pos=child_pos, child_quat=IDENTITY_Q, mass=mass, inertia=inertia, damping=damping,
                 armature=armature, free_damping=free_damping, range=range, limit_solref=limit_solref,
-                limit_solimp=limit_solimp)
+                limit_solimp=limit_solimp, limit_stiffness=limit_stiffness, limit_damping=limit_damping)
 
 
 def geom(shape, *, tree=-1, link=-1, body=-1, size=(0.0, 0.0, 0.0), pos=(0.0, 0.0, 0.0), quat=IDENTITY_Q,
@@ -132,13 +136,16 @@ class Scene:
             if rng is not None:
                 d.range = (abi.C.c_double * 2)(*rng)
             sr, si = l.get("limit_solref"), l.get("limit_solimp")
-            if sr is not None or si is not None:
+            lk, lc = l.get("limit_stiffness"), l.get("limit_damping")
+            if sr is not None or si is not None or lk is not None or lc is not None:
                 sr = sr if sr is not None else (0.02, 1.0)
                 si = si if si is not None else (0.9, 0.95, 0.001, 0.5, 2)
                 d.limit_custom = 1
                 d.limit_solref = (abi.C.c_double * 2)(*sr)
                 d.limit_solimp = (abi.C.c_double * 4)(*si[:4])
                 d.limit_solimp_power = int(si[4])
+                d.limit_stiffness = 1000.0 if lk is None else lk   # JointLimit::DEFAULT
+                d.limit_damping = 60.0 if lc is None else lc
             d.parent_pos = abi.D3(*l["parent_pos"])
             d.parent_quat = abi.D4(*l["parent_quat"])
             d.child_pos = abi.D3(*l["child_pos"])
