@@ -284,7 +284,9 @@ newt_status newt_data_get_faults(const newt_data* data, int32_t* faults);
  * (newt/src/world.rs:2476-2528): for world w, counts[w] contacts; entry
  * [w*cap + k] holds (geom_a, geom_b, slot) in `ids` (3 int32 each) and
  * (position xyz, normal xyz, penetration) in `geo` (7 doubles each).
- * `cap` is the per-world capacity reported by newt_data_contact_capacity. */
+ * `cap` is the per-world capacity reported by newt_data_contact_capacity.
+ * Free bodies under the penalty solver: the contacts of the step's last
+ * RK4 stage (collect_contacts, world.rs:2434-2474, at that stage's state). */
 newt_status newt_data_contact_capacity(const newt_data* data, int32_t* cap);
 newt_status newt_data_get_contacts(const newt_data* data, int32_t* counts,
                                    int32_t* ids, double* geo);

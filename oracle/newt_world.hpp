@@ -431,11 +431,93 @@ void box_box(int ia, const Pose<T>& pa, V3<T> ha3, int ib, const Pose<T>& pb, V3
         out.push_back({ia, ib, (int)out.size(), cpos[order[i]], nw, cpen[order[i]], fr, gap});
 }
 
+// face_along_direction (contact.rs:1209-1245): a vertex inside the box on
+// every axis, assigned to the face along the dominant component of `dir`.
+template <class T> bool face_along_direction(V3<T> lp, V3<T> h, V3<T> dir, T* pen, V3<T>* nl) {
+    const T dx = h.x - rabs(lp.x), dy = h.y - rabs(lp.y), dz = h.z - rabs(lp.z);
+    if (dx < T(0.0) || dy < T(0.0) || dz < T(0.0)) return false;
+    const T ax = rabs(dir.x), ay = rabs(dir.y), az = rabs(dir.z);
+    T p;
+    V3<T> n;
+    if (ax >= ay && ax >= az) {
+        const T sg = dir.x >= T(0.0) ? T(1.0) : T(-1.0);
+        p = h.x - sg * lp.x;
+        n = v3(sg, T(0.0), T(0.0));
+    } else if (ay >= az) {
+        const T sg = dir.y >= T(0.0) ? T(1.0) : T(-1.0);
+        p = h.y - sg * lp.y;
+        n = v3(T(0.0), sg, T(0.0));
+    } else {
+        const T sg = dir.z >= T(0.0) ? T(1.0) : T(-1.0);
+        p = h.z - sg * lp.z;
+        n = v3(T(0.0), T(0.0), sg);
+    }
+    if (p <= T(0.0)) return false;
+    *pen = p;
+    *nl = n;
+    return true;
+}
+
+// box_box_vertex_face (contact.rs:587-714): A's corners inside B (normal
+// B's face along B->A, point pushed onto B's surface), then B's corners
+// inside A (normal A's face flipped, point the corner itself); the four
+// deepest, stable on ties.  The penalty pipeline's box_box (534-553) uses
+// it first and falls back to the SAT manifold when it is empty.
+template <class T>
+void box_box_vertex_face(int ia, const Pose<T>& pa, V3<T> ha, int ib, const Pose<T>& pb, V3<T> hb, T fr, T margin,
+                         T gap, std::vector<Contact<T>>& out) {
+    static const int sgn[8][3] = {{-1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {-1, 1, 1},
+                                  {1, -1, -1},  {1, -1, 1},  {1, 1, -1},  {1, 1, 1}};
+    const V3<T> dw = pa.p - pb.p;
+    const V3<T> din_b = inverse_rotate(pb.q, dw);
+    const V3<T> din_a = inverse_rotate(pa.q, -dw);
+    const V3<T> ez = v3(T(0.0), T(0.0), T(1.0));
+    const V3<T> dir_b = dot(din_b, din_b) > T(0.0) ? din_b : ez;
+    const V3<T> dir_a = dot(din_a, din_a) > T(0.0) ? din_a : -ez;
+    T cpen[16];
+    V3<T> cpos[16], cnrm[16];
+    int count = 0;
+    for (int c = 0; c < 8; ++c) {
+        const V3<T> la = v3(T(sgn[c][0]) * ha.x, T(sgn[c][1]) * ha.y, T(sgn[c][2]) * ha.z);
+        const V3<T> wv = pa.p + rotate(pa.q, la);
+        const V3<T> lb = inverse_rotate(pb.q, wv - pb.p);
+        T pen;
+        V3<T> nl;
+        if (face_along_direction(lb, hb, dir_b, &pen, &nl)) {
+            const V3<T> nw = rotate(pb.q, nl);
+            cpen[count] = pen;
+            cpos[count] = wv + nw * pen;
+            cnrm[count] = nw;
+            ++count;
+        }
+    }
+    for (int c = 0; c < 8; ++c) {
+        const V3<T> lb = v3(T(sgn[c][0]) * hb.x, T(sgn[c][1]) * hb.y, T(sgn[c][2]) * hb.z);
+        const V3<T> wv = pb.p + rotate(pb.q, lb);
+        const V3<T> la = inverse_rotate(pa.q, wv - pa.p);
+        T pen;
+        V3<T> nl;
+        if (face_along_direction(la, ha, dir_a, &pen, &nl)) {
+            cpen[count] = pen;
+            cpos[count] = wv;
+            cnrm[count] = -rotate(pa.q, nl);
+            ++count;
+        }
+    }
+    int order[16];
+    for (int i = 0; i < 16; ++i) order[i] = i;
+    for (int i = 1; i < count; ++i)
+        for (int j = i; j > 0 && cpen[order[j]] > cpen[order[j - 1]]; --j) std::swap(order[j - 1], order[j]);
+    const int take = count > 4 ? 4 : count;
+    for (int i = 0; i < take; ++i)
+        out.push_back({ia, ib, i, cpos[order[i]], cnrm[order[i]], cpen[order[i]] + margin, fr, gap});
+}
+
 // One-direction primitive (try_narrow_phase, contact.rs:2592-2806).  Returns
 // false when (a, b) is not a primitive in this order.
 template <class T>
 bool try_narrow(int ia, const Geom<T>& ga, const Pose<T>& pa, int ib, const Geom<T>& gb, const Pose<T>& pb,
-                T fr, T margin, T gap, std::vector<Contact<T>>& out) {
+                T fr, T margin, T gap, std::vector<Contact<T>>& out, bool legacy = false) {
     if (ga.shape == S_SPHERE && gb.shape == S_PLANE) { sphere_plane(ia, pa, ga.size.x, fr, margin, gap, ib, pb, out); return true; }
     if (ga.shape == S_BOX && gb.shape == S_PLANE) { box_plane(ia, pa, ga.size, fr, margin, gap, ib, pb, out); return true; }
     if (ga.shape == S_CAPSULE && gb.shape == S_PLANE) { capsule_plane(ia, pa, ga.size.x, ga.size.y, fr, margin, gap, ib, pb, out); return true; }
@@ -461,22 +543,27 @@ bool try_narrow(int ia, const Geom<T>& ga, const Pose<T>& pa, int ib, const Geom
         return true;
     }
     if (ga.shape == S_BOX && gb.shape == S_BOX) {
+        if (legacy) {  // NarrowPhaseMode::LegacyPenalty (contact.rs:2767-2769)
+            box_box_vertex_face(ia, pa, ga.size, ib, pb, gb.size, fr, margin, gap, out);
+            if (!out.empty()) return true;
+        }
         box_box(ia, pa, ga.size, ib, pb, gb.size, fr, margin, gap, out);
         return true;
     }
     return false;
 }
 
-// dispatch_narrow_phase (contact.rs:2509-2590), FullManifold mode.
+// dispatch_narrow_phase (contact.rs:2509-2590), FullManifold mode, or
+// LegacyPenalty (the penalty pipeline's box-box) with `legacy`.
 template <class T>
 void narrow_phase(int ia, const Geom<T>& ga, const Pose<T>& pa, int ib, const Geom<T>& gb, const Pose<T>& pb,
-                  std::vector<Contact<T>>& out) {
+                  std::vector<Contact<T>>& out, bool legacy = false) {
     out.clear();
     T fr = combine_friction(ga.friction, gb.friction);
     T margin = combine_max(ga.margin, gb.margin);
     T gap = combine_max(ga.gap, gb.gap);
-    if (try_narrow(ia, ga, pa, ib, gb, pb, fr, margin, gap, out)) return;
-    if (try_narrow(ib, gb, pb, ia, ga, pa, fr, margin, gap, out)) {
+    if (try_narrow(ia, ga, pa, ib, gb, pb, fr, margin, gap, out, legacy)) return;
+    if (try_narrow(ib, gb, pb, ia, ga, pa, fr, margin, gap, out, legacy)) {
         for (auto& c : out) {
             std::swap(c.geom_a, c.geom_b);
             c.normal = -c.normal;
@@ -1358,6 +1445,170 @@ template <class T> struct World {
         return qfrc;
     }
 
+    // apply_contact_wrench (world.rs:2868-2949) for geoms with no friction
+    // anisotropy and bodies with no rolling friction: the normal force
+    // max(0, k pen_eff - c v_n) with (k, c) = solref_to_kc (geom.rs:137-145)
+    // of the combined SolRef on the reduced mass; per tangent_basis axis the
+    // damping force -c v_t clamped to +-mu f_n (clamp_symmetric,
+    // world.rs:3195-3203); force and r x force on each body, Newton's third
+    // law on the second.
+    void apply_contact_wrench(std::vector<std::pair<V3<T>, V3<T>>>& ext, const std::vector<Body<T>>& st,
+                              const Contact<T>& c) const {
+        const Geom<T>& ga = geoms[c.geom_a];
+        const Geom<T>& gb = geoms[c.geom_b];
+        const V3<T> n = c.normal;
+        T m_eff;
+        if (ga.is_body() && gb.is_body()) {
+            const T ma = st[ga.body].mass, mb = st[gb.body].mass;
+            m_eff = ma * mb / (ma + mb);
+        } else if (ga.is_body()) {
+            m_eff = st[ga.body].mass;
+        } else if (gb.is_body()) {
+            m_eff = st[gb.body].mass;
+        } else {
+            return;
+        }
+        const SolRef<T> sr = combine_solref(ga.solref, gb.solref);
+        T k, cd;
+        if (sr.is_direct()) {
+            k = -sr.timeconst;
+            cd = -sr.dampratio;
+        } else {
+            const T tc = sr.timeconst;
+            k = m_eff / (tc * tc);
+            cd = T(2.0) * sr.dampratio * m_eff / tc;
+        }
+        // point_velocity (world.rs:3038-3048)
+        auto point_velocity = [&](const Geom<T>& g, V3<T>* r) {
+            if (!g.is_body()) {
+                *r = vzero<T>();
+                return vzero<T>();
+            }
+            const Body<T>& b = st[g.body];
+            *r = c.pos - b.p;
+            return b.v + cross(rotate(b.q, b.w), *r);
+        };
+        V3<T> ra, rb;
+        const V3<T> va = point_velocity(ga, &ra);
+        const V3<T> vb = point_velocity(gb, &rb);
+        const V3<T> vrel = va - vb;
+        const T vn = dot(vrel, n);
+        const T pen_eff = c.penetration - c.gap;
+        if (pen_eff <= T(0.0)) return;
+        const T f_raw = k * pen_eff - cd * vn;
+        const T fn = f_raw > T(0.0) ? f_raw : T(0.0);
+        if (fn <= T(0.0)) return;
+        const V3<T> t1 = tangent_basis(n).first;
+        const V3<T> t2 = cross(n, t1);  // contact_friction_axes, solver.rs:3441-3465
+        const V3<T> vt = vrel - n * vn;
+        const T vt1 = dot(vt, t1), vt2 = dot(vt, t2);
+        auto clamp_sym = [](T x, T cap) { return x > cap ? cap : (x < -cap ? -cap : x); };
+        const T ft1 = clamp_sym(-cd * vt1, c.friction * fn);
+        const T ft2 = clamp_sym(-cd * vt2, c.friction * fn);
+        const V3<T> fa = (n * fn + t1 * ft1) + t2 * ft2;
+        if (ga.is_body()) {
+            auto& e = ext[ga.body];
+            e.first = e.first + fa;
+            e.second = e.second + cross(ra, fa);
+            e.second = e.second + vzero<T>();  // rolling_drag_torque: no rolling friction
+        }
+        if (gb.is_body()) {
+            const V3<T> fb = -fa;
+            auto& e = ext[gb.body];
+            e.first = e.first + fb;
+            e.second = e.second + cross(rb, fb);
+            e.second = e.second + vzero<T>();
+        }
+    }
+
+    // compute_penalty_wrenches (world.rs:2281-2301) at body state `st`:
+    // collect_contacts over the step's pairs with the legacy manifold
+    // (world.rs:2434-2474), then apply_contact_wrench contact by contact.
+    std::vector<std::pair<V3<T>, V3<T>>> penalty_wrenches(const std::vector<Body<T>>& st,
+                                                          const std::vector<std::pair<int, int>>& pairs,
+                                                          std::vector<Contact<T>>* keep) const {
+        std::vector<Pose<T>> gp(geoms.size());
+        for (size_t gi = 0; gi < geoms.size(); ++gi) {
+            const Geom<T>& g = geoms[gi];
+            gp[gi] = g.is_body() ? geom_world_pose(g, st[g.body].p, st[g.body].q)
+                                 : geom_world_pose(g, vzero<T>(), qidentity<T>());
+        }
+        std::vector<std::pair<V3<T>, V3<T>>> out(st.size(), {vzero<T>(), vzero<T>()});
+        std::vector<Contact<T>> buf;
+        if (keep) keep->clear();
+        for (auto [a, b] : pairs) {
+            if (geoms[a].is_link() || geoms[b].is_link()) continue;
+            narrow_phase(a, geoms[a], gp[a], b, geoms[b], gp[b], buf, true);
+            for (auto c : buf) {
+                if (penetration_slop > T(0.0) && c.gap < penetration_slop) c.gap = penetration_slop;
+                if (keep) keep->push_back(c);
+                apply_contact_wrench(out, st, c);
+            }
+        }
+        return out;
+    }
+
+    // step_bodies / step_bodies_euler under SolverMode::Penalty
+    // (world.rs:1705-1785, 1790-1817): the wrenches are sampled again at
+    // every RK4 stage state.  last_contacts keeps the last stage's contacts.
+    void step_bodies_penalty(const std::vector<std::pair<int, int>>& pairs) {
+        const size_t n = bodies.size();
+        const std::vector<Body<T>> s0 = bodies;
+        std::vector<BodyDeriv<T>> k1(n), k2(n), k3(n), k4(n);
+        auto stage = [&](const std::vector<Body<T>>& st, std::vector<BodyDeriv<T>>& kk, std::vector<Contact<T>>* keep) {
+            const auto ext = penalty_wrenches(st, pairs, keep);
+            for (size_t i = 0; i < n; ++i) kk[i] = evaluate_body(st[i], gravity, ext[i].first, ext[i].second);
+        };
+        stage(s0, k1, &last_contacts);
+        for (size_t i = 0; i < n; ++i) {
+            const V3<T> a1[2] = {k1[i].dv, k1[i].dw};
+            for (int k = 0; k < 2; ++k) {
+                last_qacc.push_back(a1[k].x);
+                last_qacc.push_back(a1[k].y);
+                last_qacc.push_back(a1[k].z);
+            }
+        }
+        if (integrator == I_RK4) {
+            std::vector<Body<T>> s(n);
+            for (size_t i = 0; i < n; ++i) s[i] = advance_body(s0[i], k1[i], dt * T(0.5));
+            stage(s, k2, &last_contacts);
+            for (size_t i = 0; i < n; ++i) s[i] = advance_body(s0[i], k2[i], dt * T(0.5));
+            stage(s, k3, &last_contacts);
+            for (size_t i = 0; i < n; ++i) s[i] = advance_body(s0[i], k3[i], dt);
+            stage(s, k4, &last_contacts);
+            for (size_t i = 0; i < n; ++i) bodies[i] = rk4_combine(s0[i], k1[i], k2[i], k3[i], k4[i]);
+        } else {
+            for (size_t i = 0; i < n; ++i) bodies[i] = euler_combine(s0[i], k1[i]);
+        }
+    }
+
+    // The RK4 combination and the semi-implicit Euler update of one body
+    // (world.rs:1745-1784, 1806-1816).
+    Body<T> rk4_combine(const Body<T>& s0, const BodyDeriv<T>& k1, const BodyDeriv<T>& k2, const BodyDeriv<T>& k3,
+                        const BodyDeriv<T>& k4) const {
+        Body<T> b = s0;
+        const T sixth = T(1.0) / T(6.0);
+        V3<T> dp = (k1.dp + k2.dp * T(2.0) + k3.dp * T(2.0) + k4.dp) * sixth;
+        V3<T> dv = (k1.dv + k2.dv * T(2.0) + k3.dv * T(2.0) + k4.dv) * sixth;
+        Q<T> dq = qscale(qadd(qadd(qadd(k1.dq, qscale(k2.dq, T(2.0))), qscale(k3.dq, T(2.0))), k4.dq), sixth);
+        V3<T> dw = (k1.dw + k2.dw * T(2.0) + k3.dw * T(2.0) + k4.dw) * sixth;
+        b.p = s0.p + dp * dt;
+        b.v = clamp_velocity(s0.v + dv * dt, s0.has_max_lin, s0.max_lin);
+        b.q = renormalize(qadd(s0.q, qscale(dq, dt)));
+        b.w = clamp_velocity(s0.w + dw * dt, s0.has_max_ang, s0.max_ang);
+        return b;
+    }
+    Body<T> euler_combine(const Body<T>& s0, const BodyDeriv<T>& k1) const {
+        Body<T> b = s0;
+        V3<T> v = s0.v + k1.dv * dt;
+        V3<T> w = s0.w + k1.dw * dt;
+        b.p = s0.p + v * dt;
+        b.q = integrate_body_angular_velocity(s0.q, w, dt);
+        b.v = clamp_velocity(v, s0.has_max_lin, s0.max_lin);
+        b.w = clamp_velocity(w, s0.has_max_ang, s0.max_ang);
+        return b;
+    }
+
     // step_bodies (RK4, world.rs:1705-1785) and step_bodies_euler (Euler and
     // ImplicitFast, 1790-1817) with the solver wrenches held over the step.
     // last_qacc gets (dlinear_velocity, dangular_velocity_body) of the
@@ -1430,7 +1681,7 @@ template <class T> struct World {
             tq = solve_tree_contacts(tc, ti);
             bw = solve_free_bodies(bodies, geoms, fc, gravity, dt, cone, iterations, mode == M_NEWTON);
         } else {
-            if (!pairs.empty()) fail("penalty contact forces are not restated");
+            if (!pairs.empty() && !trees.empty()) fail("penalty tree contact forces are not restated");
         }
         last_qacc.clear();
         for (size_t t = 0; t < trees.size(); ++t) {
@@ -1457,7 +1708,8 @@ template <class T> struct World {
             for (size_t s = 0; s < delta.size(); ++s) tr.qfrc_applied[s] -= delta[s];
             tr.disable_penalty_limits = prior;
         }
-        step_bodies(bw);
+        if (!constraint && !pairs.empty()) step_bodies_penalty(pairs);
+        else step_bodies(bw);
         refresh_broadphase();
     }
 };

@@ -41,6 +41,44 @@ def test_box_stack_scene_is_newts_golden_stack(oracle_lib):
     assert b"".join(got) == expected
 
 
+def _golden_rollout(ow, nb, marks):
+    q, v = ow.state()
+    got = [_snapshot(q[0], v[0], nb)]
+    done = 0
+    for mark in marks:
+        ow.step(mark - done)
+        done = mark
+        q, v = ow.state()
+        got.append(_snapshot(q[0], v[0], nb))
+    ow.close()
+    return b"".join(got)
+
+
+def test_penalty_stack_scene_is_newts_contacts_golden(oracle_lib):
+    """scenes.penalty_stack() through the C-ABI records, stepped by the
+    oracle's float instantiation under the penalty solver (spring-damper
+    contact wrenches re-sampled at every RK4 stage, legacy vertex-face
+    box-box manifold with the SAT fallback), writes newt's
+    stacking_3_boxes.bin (newt/tests/contacts_golden.rs:82-97) byte for
+    byte."""
+    ow = oracle_lib.OracleWorlds(scenes.penalty_stack(), 1, 1)
+    expected = open(os.path.join(GOLDEN, "stacking_3_boxes.bin"), "rb").read()
+    assert _golden_rollout(ow, 3, (100, 1000)) == expected
+
+
+def test_stack_json_penalty_is_newts_model_golden(oracle_lib):
+    """newt's stack.json under its own (default penalty) solver, loaded by
+    tooling_amd.model_json and stepped by the oracle's float instantiation,
+    writes newt's model_stack.bin (newt/tests/model_load.rs:315-330) byte
+    for byte."""
+    from tooling_amd import model_json
+    sc = model_json.load(os.path.join(GOLDEN, "models", "stack.json"))
+    assert sc.options["solver"] == abi.SOLVER_PENALTY
+    ow = oracle_lib.OracleWorlds(sc, 1, 1)
+    expected = open(os.path.join(GOLDEN, "model_stack.bin"), "rb").read()
+    assert _golden_rollout(ow, 3, (200, 500, 1000)) == expected
+
+
 def test_free_fall_closed_form(oracle_lib):
     """Semi-implicit Euler with no contacts: v_n = v0 + n g dt and
     p_n = p0 + dt sum_{k=1..n} v_k = p0 + n dt v0 + g dt^2 n(n+1)/2; a body

@@ -169,3 +169,49 @@ def test_newton_box_pile(oracle):
     say(f"{sc.name}: 16 worlds x 300 steps")
     out, ref = run_pair(oracle, sc, 16, 300, abi.PRECISION_FP64, seed=11, chunks=3)
     check_exact(out, ref, sc.name)
+
+
+@pytest.mark.parametrize("integrator", [abi.INTEGRATOR_RK4, abi.INTEGRATOR_EULER])
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP64_EXACT, abi.PRECISION_FP64, abi.PRECISION_MIXED])
+def test_penalty_stack(oracle, integrator, precision):
+    """Penalty contacts (compute_penalty_wrenches / apply_contact_wrench,
+    newt/src/world.rs:2281-2301, 2868-2949; the legacy vertex-face box-box
+    manifold, contact.rs:534-714) on newt's penalty golden scene: contacts
+    and wrenches re-sampled at every RK4 stage, bit for bit in every
+    precision mode (the lane-per-world step runs them all)."""
+    sc = scenes.penalty_stack(integrator=integrator)
+    say(f"{sc.name} integrator {integrator} precision {precision}: 64 worlds x 600 steps")
+    out, ref = run_pair(oracle, sc, 64, 600, precision, seed=13, chunks=3)
+    check_exact(out, ref, sc.name)
+
+
+@pytest.mark.parametrize("integrator", [abi.INTEGRATOR_RK4, abi.INTEGRATOR_EULER])
+def test_penalty_box_pile(oracle, integrator):
+    """64 boxes under penalty contacts: box-plane and box-box (vertex-face
+    with the SAT fallback), over a hundred contacts per world, bit for bit;
+    the last stage's contacts match world by world."""
+    sc = scenes.with_options(scenes.box_pile(), f"_pen_i{integrator}", solver=abi.SOLVER_PENALTY,
+                             integrator=integrator)
+    n = 16
+    say(f"{sc.name}: {n} worlds x 300 steps")
+    q0, v0 = sc.initial_state(n, 17)
+    m = Model(sc)
+    d = Data(m, n, abi.PRECISION_FP64)
+    d.set_state(q0, v0)
+    ow = oracle.OracleWorlds(sc, n, 0)
+    ow.set_state(q0, v0)
+    for _ in range(3):
+        step(m, d, 100)
+        ow.step(100, threads=8)
+    check_exact(d.state() + (d.qacc(), d.faults()), ow.state() + (ow.qacc(),), sc.name)
+    nc, ids_g, geo_g = d.contacts()
+    for w in range(n):
+        ids, geo = ow.contacts(w)
+        assert nc[w] == len(ids), (w, nc[w], len(ids))
+        assert np.array_equal(ids_g[w, :nc[w]], ids), w
+        assert np.array_equal(geo_g[w, :nc[w]], geo), w
+    say(f"penalty pile contacts per world: {nc.min()}..{nc.max()}")
+    assert nc.max() > 50
+    d.close()
+    m.close()
+    ow.close()

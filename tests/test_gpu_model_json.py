@@ -70,13 +70,14 @@ def test_quadruped_json(oracle, precision):
     _compare(oracle, sc, q0, v0, 200, precision)
 
 
-@pytest.mark.parametrize("solver", [abi.SOLVER_PGS, abi.SOLVER_NEWTON])
+@pytest.mark.parametrize("solver", [abi.SOLVER_PENALTY, abi.SOLVER_PGS, abi.SOLVER_NEWTON])
 def test_stack_json_free_bodies(oracle, solver):
-    """newt's stack.json loads as three free boxes; under PGS and Newton
-    (its own solver is penalty, not built) the GPU matches the oracle bit
-    for bit, with random velocity perturbations per world."""
-    sc = scenes.with_options(model_json.load(os.path.join(MODELS, "stack.json")), f"_s{solver}", solver=solver,
-                             iterations=30)
+    """newt's stack.json loads as three free boxes; under its own penalty
+    solver, PGS and Newton the GPU matches the oracle bit for bit, with
+    random velocity perturbations per world."""
+    sc = model_json.load(os.path.join(MODELS, "stack.json"))
+    if solver != abi.SOLVER_PENALTY:
+        sc = scenes.with_options(sc, f"_s{solver}", solver=solver, iterations=30)
     q0, v0 = sc.initial_state(32, 0)
     v0 = v0 + np.random.default_rng(5).normal(0.0, 0.05, size=v0.shape)
     say(f"stack.json solver {solver}: 32 worlds x 400 steps")

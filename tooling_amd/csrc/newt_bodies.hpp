@@ -399,9 +399,187 @@ struct FreeBodies {
         }
     }
 
+    // ---------------- penalty contacts (world.rs:2281-2301, 2868-2949) -------
+    // apply_contact_wrench of stored contact c at the state in qpos/qvel,
+    // into bwr: the normal force max(0, k pen_eff - c v_n) with (k, c) =
+    // solref_to_kc (geom.rs:137-145) of the combined SolRef on the reduced
+    // mass; per tangent_basis axis the damping force -c v_t clamped to
+    // +-mu f_n; force and r x force on each body, opposite on the second.
+    NEWT_HD void penalty_wrench(int c) const {
+        const int o = w.L.cgeo + 9 * c;
+        const GeomT& ga = w.m.geoms[w.iw[w.L.cids + 3 * c]];
+        const GeomT& gb = w.m.geoms[w.iw[w.L.cids + 3 * c + 1]];
+        const int a = ga.body, b = gb.body;
+        double m_eff;
+        if (a >= 0 && b >= 0) {
+            const double ma = body(a).mass, mb = body(b).mass;
+            m_eff = ma * mb / (ma + mb);
+        } else if (a >= 0) {
+            m_eff = body(a).mass;
+        } else if (b >= 0) {
+            m_eff = body(b).mass;
+        } else {
+            return;
+        }
+        // combine_solref (geom.rs:156-185)
+        double sr0 = ga.solref[0] <= gb.solref[0] ? ga.solref[0] : gb.solref[0];
+        double sr1 = ga.solref[1] <= gb.solref[1] ? ga.solref[1] : gb.solref[1];
+        if ((ga.solref[0] < 0.0) != (gb.solref[0] < 0.0)) {
+            const GeomT& gd = ga.solref[0] < 0.0 ? ga : gb;
+            sr0 = gd.solref[0];
+            sr1 = gd.solref[1];
+        }
+        double k, cd;
+        if (sr0 < 0.0) {
+            k = -sr0;
+            cd = -sr1;
+        } else {
+            k = m_eff / (sr0 * sr0);
+            cd = 2.0 * sr1 * m_eff / sr0;
+        }
+        const V3<double> cp = w.kld3(o), n = w.kld3(o + 3);
+        // point_velocity (world.rs:3038-3048)
+        V3<double> ra = z3<double>(), rb = z3<double>(), va = z3<double>(), vb = z3<double>();
+        if (a >= 0) {
+            ra = cp - bpos(a);
+            va = bvel(a) + cross3(qrot(bori(a), bomega(a)), ra);
+        }
+        if (b >= 0) {
+            rb = cp - bpos(b);
+            vb = bvel(b) + cross3(qrot(bori(b), bomega(b)), rb);
+        }
+        const V3<double> vrel = va - vb;
+        const double vn = dot3(vrel, n);
+        const double pen_eff = w.dw[o + 6] - w.dw[o + 8];
+        if (pen_eff <= 0.0) return;
+        const double f_raw = k * pen_eff - cd * vn;
+        const double fn = f_raw > 0.0 ? f_raw : 0.0;
+        if (fn <= 0.0) return;
+        const V3<double> ref = nabs(n.z) < 0.9 ? mk3(0.0, 0.0, 1.0) : mk3(1.0, 0.0, 0.0);
+        const V3<double> t1 = unit3(cross3(ref, n));
+        const V3<double> t2 = cross3(n, t1);
+        const V3<double> vt = vrel - n * vn;
+        const double vt1 = dot3(vt, t1), vt2 = dot3(vt, t2);
+        const double cap = w.dw[o + 7] * fn;
+        const double x1 = -cd * vt1, x2 = -cd * vt2;
+        const double ft1 = x1 > cap ? cap : (x1 < -cap ? -cap : x1);
+        const double ft2 = x2 > cap ? cap : (x2 < -cap ? -cap : x2);
+        const V3<double> fa = (n * fn + t1 * ft1) + t2 * ft2;
+        if (a >= 0) {
+            const int q = w.L.bwr + 6 * a;
+            w.kst3(q, w.kld3(q) + fa);
+            w.kst3(q + 3, (w.kld3(q + 3) + cross3(ra, fa)) + z3<double>());  // no rolling friction
+        }
+        if (b >= 0) {
+            const V3<double> fb = -fa;
+            const int q = w.L.bwr + 6 * b;
+            w.kst3(q, w.kld3(q) + fb);
+            w.kst3(q + 3, (w.kld3(q + 3) + cross3(rb, fb)) + z3<double>());
+        }
+    }
+
+    // compute_penalty_wrenches at the state in qpos/qvel: geom poses, the
+    // legacy narrowphase over the step's pairs, wrenches into bwr.  Returns
+    // the contact count (above cap_c: a capacity fault, nothing applied).
+    NEWT_HD int penalty_wrenches(int np) const {
+        set_poses();
+        const int nc = w.narrowphase(np, true);
+        for (int i = 0; i < 6 * w.m.n_bodies; ++i) w.dw[w.L.bwr + i] = 0.0;
+        if (nc > w.m.cap_c) return nc;
+        for (int c = 0; c < nc; ++c) penalty_wrench(c);
+        return nc;
+    }
+
+    // step_bodies / step_bodies_euler under the penalty solver
+    // (world.rs:1705-1785, 1790-1817): contacts and wrenches are sampled
+    // again at every RK4 stage.  Stage states are written into qpos/qvel;
+    // s0 and the running RK4 sum (k1 + 2 k2 + 2 k3 + k4, summed in that
+    // order) sit in the row scratch, 26 doubles per body.
+    NEWT_HD int step_penalty(int np) const {
+        const int nb = w.m.n_bodies;
+        const double dt = w.m.dt;
+        int nc = penalty_wrenches(np);
+        if (nc > w.m.cap_c) return 8;  // NEWT_ERR_CAPACITY
+        w.iw[w.L.iwf] = np;
+        const bool rk4 = w.m.integrator == NEWT_INTEGRATOR_RK4;
+        const int S = w.L.brow;
+        auto put = [&](int b, const St& s) {
+            const BodyT& bt = body(b);
+            const int qo = bt.q_off, vo = bt.v_off;
+            w.qpos[qo] = s.p.x; w.qpos[qo + 1] = s.p.y; w.qpos[qo + 2] = s.p.z;
+            w.qpos[qo + 3] = s.q.x; w.qpos[qo + 4] = s.q.y; w.qpos[qo + 5] = s.q.z; w.qpos[qo + 6] = s.q.w;
+            w.qvel[vo] = s.v.x; w.qvel[vo + 1] = s.v.y; w.qvel[vo + 2] = s.v.z;
+            w.qvel[vo + 3] = s.om.x; w.qvel[vo + 4] = s.om.y; w.qvel[vo + 5] = s.om.z;
+        };
+        auto save = [&](int off, V3<double> a, V3<double> b2, Qt<double> q, V3<double> c) {
+            w.kst3(off, a);
+            w.kst3(off + 3, b2);
+            w.dw[off + 6] = q.x; w.dw[off + 7] = q.y; w.dw[off + 8] = q.z; w.dw[off + 9] = q.w;
+            w.kst3(off + 10, c);
+        };
+        auto ldq = [&](int off) { return Qt<double>{w.dw[off], w.dw[off + 1], w.dw[off + 2], w.dw[off + 3]}; };
+        for (int b = 0; b < nb; ++b) {
+            const BodyT& bt = body(b);
+            const St s0{bpos(b), bvel(b), bori(b), bomega(b)};
+            const Dv k1 = evaluate(b, s0, w.kld3(w.L.bwr + 6 * b), w.kld3(w.L.bwr + 6 * b + 3));
+            const int vo = bt.v_off;
+            w.qacc[vo] = k1.dv.x; w.qacc[vo + 1] = k1.dv.y; w.qacc[vo + 2] = k1.dv.z;
+            w.qacc[vo + 3] = k1.dw.x; w.qacc[vo + 4] = k1.dw.y; w.qacc[vo + 5] = k1.dw.z;
+            if (rk4) {
+                save(S + 26 * b, s0.p, s0.v, s0.q, s0.om);
+                save(S + 26 * b + 13, k1.dp, k1.dv, k1.dq, k1.dw);
+                put(b, advance(s0, k1, dt * 0.5));
+            } else {
+                const V3<double> v = s0.v + k1.dv * dt;
+                const V3<double> om = s0.om + k1.dw * dt;
+                St n;
+                n.p = s0.p + v * dt;
+                n.q = qintegrate(s0.q, om, dt);
+                n.v = clampv(v, bt.has_max_lin, bt.max_lin);
+                n.om = clampv(om, bt.has_max_ang, bt.max_ang);
+                put(b, n);
+            }
+        }
+        if (rk4) {
+            for (int stg = 2; stg <= 4; ++stg) {
+                nc = penalty_wrenches(np);
+                if (nc > w.m.cap_c) return 8;
+                const double wgt = stg == 4 ? 1.0 : 2.0;
+                for (int b = 0; b < nb; ++b) {
+                    const BodyT& bt = body(b);
+                    const int o0 = S + 26 * b, oa = o0 + 13;
+                    const St s0{w.kld3(o0), w.kld3(o0 + 3), ldq(o0 + 6), w.kld3(o0 + 10)};
+                    const St sc{bpos(b), bvel(b), bori(b), bomega(b)};
+                    const Dv kk = evaluate(b, sc, w.kld3(w.L.bwr + 6 * b), w.kld3(w.L.bwr + 6 * b + 3));
+                    const V3<double> sdp = stg == 4 ? w.kld3(oa) + kk.dp : w.kld3(oa) + kk.dp * wgt;
+                    const V3<double> sdv = stg == 4 ? w.kld3(oa + 3) + kk.dv : w.kld3(oa + 3) + kk.dv * wgt;
+                    const Qt<double> sdq = stg == 4 ? qadd(ldq(oa + 6), kk.dq) : qadd(ldq(oa + 6), qsc(kk.dq, wgt));
+                    const V3<double> sdw = stg == 4 ? w.kld3(oa + 10) + kk.dw : w.kld3(oa + 10) + kk.dw * wgt;
+                    if (stg < 4) {
+                        save(oa, sdp, sdv, sdq, sdw);
+                        put(b, advance(s0, kk, stg == 2 ? dt * 0.5 : dt));
+                    } else {
+                        const double sixth = 1.0 / 6.0;
+                        St n;
+                        n.p = s0.p + (sdp * sixth) * dt;
+                        n.v = clampv(s0.v + (sdv * sixth) * dt, bt.has_max_lin, bt.max_lin);
+                        n.q = qnormalize(qadd(s0.q, qsc(qsc(sdq, sixth), dt)));
+                        n.om = clampv(s0.om + (sdw * sixth) * dt, bt.has_max_ang, bt.max_ang);
+                        put(b, n);
+                    }
+                }
+            }
+        }
+        w.iw[w.L.iwf + 1] = nc;
+        set_poses();
+        w.populate(w.m.dt, false);
+        return 0;
+    }
+
     // World::step for a free-body model; 0 or a newt_status fault code.
     NEWT_HD int step() const {
         const int np = w.active_pairs();
+        if (w.m.mode == NEWT_SOLVER_PENALTY && w.m.n_pairs > 0) return step_penalty(np);
         int nc = 0;
         if (w.m.mode == NEWT_SOLVER_PGS) {
             nc = w.narrowphase(np);

@@ -195,8 +195,6 @@ inline newt_status compile_bodies(HostModel* M, std::string* err) {
             M->pairs.push_back({(int)a, (int)b});
             cap_c += pair_max_contacts(ga.shape, gb.shape);
         }
-    if (o.solver_mode == NEWT_SOLVER_PENALTY && !M->pairs.empty())
-        return cfail(err, NEWT_ERR_UNSUPPORTED, "penalty contact forces are not built; use the PGS solver");
     // Every contact the pairs can emit fits up to 4096 per world; a world
     // that exceeds a smaller compiled bound faults with NEWT_ERR_CAPACITY.
     const int kMaxBodyContacts = 4096;
@@ -530,7 +528,9 @@ inline Layout make_body_layout(const ModelT& m) {
     L.bwr = D(6 * nb);
     L.biw = D(9 * nb);
     L.bbd = D(6 * nb);
-    L.brow = D(7 * m.cap_r);
+    // rows; under the penalty solver the RK4 stage scratch (s0 and the
+    // running stage sum, 26 per body) instead
+    L.brow = D(std::max(7 * m.cap_r, m.mode == NEWT_SOLVER_PENALTY ? 26 * nb : 0));
     L.bblk = D(8 * m.cap_c);
     L.total_d = std::max(dd, 1);
     L.cids = I(3 * m.cap_c);

@@ -463,7 +463,10 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
               alloc((void**)&D->iws, W * D->L.total_i * 4) &&
               alloc((void**)&D->ovf, W * (size_t)std::max(TL.ov_stride, 16));
     const bool newton = bodies && m.mode == NEWT_SOLVER_NEWTON;
-    if (ok && bodies && (newton || (precision != NEWT_PRECISION_FP64_EXACT && n_worlds <= kBodyWaveMaxWorlds))) {
+    // Penalty contacts run on the lane step only.
+    const bool penalty_contacts = m.mode == NEWT_SOLVER_PENALTY && m.n_pairs > 0;
+    if (ok && bodies &&
+        (newton || (precision != NEWT_PRECISION_FP64_EXACT && n_worlds <= kBodyWaveMaxWorlds && !penalty_contacts))) {
         // The wavefront step keeps a world's bodies and fat bounds in LDS;
         // models whose bodies do not fit run the lane step (PGS only).
         D->BL = make_body_wave_layout(m, kBodyWaveLds);

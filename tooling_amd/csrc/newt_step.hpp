@@ -35,6 +35,92 @@ template <class R> struct Sum {
     NEWT_HD void add(R x) { acc = acc + x; }
 };
 
+// face_along_direction (contact.rs:1209-1245): a vertex inside the box on
+// every axis, assigned to the face along the dominant component of `dir`.
+NEWT_HD inline bool face_along_direction(V3<double> lp, const double* h, V3<double> dir, double* pen,
+                                         V3<double>* nl) {
+    const double dx = h[0] - nabs(lp.x), dy = h[1] - nabs(lp.y), dz = h[2] - nabs(lp.z);
+    if (dx < 0.0 || dy < 0.0 || dz < 0.0) return false;
+    const double ax = nabs(dir.x), ay = nabs(dir.y), az = nabs(dir.z);
+    double p;
+    V3<double> n;
+    if (ax >= ay && ax >= az) {
+        const double sg = dir.x >= 0.0 ? 1.0 : -1.0;
+        p = h[0] - sg * lp.x;
+        n = mk3(sg, 0.0, 0.0);
+    } else if (ay >= az) {
+        const double sg = dir.y >= 0.0 ? 1.0 : -1.0;
+        p = h[1] - sg * lp.y;
+        n = mk3(0.0, sg, 0.0);
+    } else {
+        const double sg = dir.z >= 0.0 ? 1.0 : -1.0;
+        p = h[2] - sg * lp.z;
+        n = mk3(0.0, 0.0, sg);
+    }
+    if (p <= 0.0) return false;
+    *pen = p;
+    *nl = n;
+    return true;
+}
+
+// Box-box vertex-vs-face manifold of the penalty pipeline
+// (box_box_vertex_face, contact.rs:587-714): corners of A inside B (normal
+// B's face along B->A, point moved onto B's surface), then corners of B
+// inside A (A's face normal flipped, the corner itself); the four deepest,
+// stable on ties, with the margin added.  Per-contact normals.
+NEWT_HD inline int box_box_vertex_face(V3<double> pa, Qt<double> qa, const double* ha, V3<double> pb, Qt<double> qb,
+                                       const double* hb, double margin, V3<double>* cp, double* pen,
+                                       V3<double>* nrm) {
+    const V3<double> dw = pa - pb;
+    const V3<double> dib = qrot_inv(qb, dw);
+    const V3<double> dia = qrot_inv(qa, -dw);
+    const V3<double> dir_b = dot3(dib, dib) > 0.0 ? dib : mk3(0.0, 0.0, 1.0);
+    const V3<double> dir_a = dot3(dia, dia) > 0.0 ? dia : mk3(-0.0, -0.0, -1.0);
+    double cpen[16];
+    V3<double> cpos[16], cn[16];
+    int count = 0;
+    for (int c = 0; c < 8; ++c) {  // corner signs in the order (-,-,-), (-,-,+), (-,+,-), ...
+        const V3<double> la = mk3((c & 4) ? ha[0] : -ha[0], (c & 2) ? ha[1] : -ha[1], (c & 1) ? ha[2] : -ha[2]);
+        const V3<double> wv = pa + qrot(qa, la);
+        double p;
+        V3<double> nl;
+        if (face_along_direction(qrot_inv(qb, wv - pb), hb, dir_b, &p, &nl)) {
+            const V3<double> nw = qrot(qb, nl);
+            cpen[count] = p;
+            cpos[count] = wv + nw * p;
+            cn[count] = nw;
+            ++count;
+        }
+    }
+    for (int c = 0; c < 8; ++c) {
+        const V3<double> lb = mk3((c & 4) ? hb[0] : -hb[0], (c & 2) ? hb[1] : -hb[1], (c & 1) ? hb[2] : -hb[2]);
+        const V3<double> wv = pb + qrot(qb, lb);
+        double p;
+        V3<double> nl;
+        if (face_along_direction(qrot_inv(qa, wv - pa), ha, dir_a, &p, &nl)) {
+            cpen[count] = p;
+            cpos[count] = wv;
+            cn[count] = -qrot(qa, nl);
+            ++count;
+        }
+    }
+    int order[16];
+    for (int i = 0; i < 16; ++i) order[i] = i;
+    for (int i = 1; i < count; ++i)
+        for (int j = i; j > 0 && cpen[order[j]] > cpen[order[j - 1]]; --j) {
+            const int t = order[j - 1];
+            order[j - 1] = order[j];
+            order[j] = t;
+        }
+    const int take = count > 4 ? 4 : count;
+    for (int i = 0; i < take; ++i) {
+        cp[i] = cpos[order[i]];
+        pen[i] = cpen[order[i]] + margin;
+        nrm[i] = cn[order[i]];
+    }
+    return take;
+}
+
 template <class R> struct World {
     const ModelT& m;
     const Layout& L;
@@ -328,7 +414,9 @@ template <class R> struct World {
         }
     }
 
-    NEWT_HD int narrowphase(int npairs) const {
+    // legacy: the penalty pipeline's box-box (box_box, contact.rs:534-553),
+    // vertex-vs-face first and the SAT manifold only when that is empty.
+    NEWT_HD int narrowphase(int npairs, bool legacy = false) const {
         int nc = 0;
         for (int k = 0; k < npairs; ++k) {
             const PairT& pr = m.pairs[iw[L.pairs + k]];
@@ -344,6 +432,12 @@ template <class R> struct World {
                 double pen[4];
                 geom_pose(ga, &pa, &qa);
                 geom_pose(gb, &pb, &qb);
+                if (legacy) {
+                    V3<double> nv[4];
+                    const int c = box_box_vertex_face(pa, qa, ga.size, pb, qb, gb.size, margin, cp, pen, nv);
+                    for (int i = 0; i < c; ++i) emit(&nc, pr.a, pr.b, i, cp[i], nv[i], pen[i], fr, gap);
+                    if (c > 0) continue;
+                }
                 const int c = box_box(pa, qa, ga.size, pb, qb, gb.size, margin, cp, pen, &nrm);
                 for (int i = 0; i < c; ++i) emit(&nc, pr.a, pr.b, i, cp[i], nrm, pen[i], fr, gap);
             } else if (ga.shape != SH_PLANE && gb.shape != SH_PLANE) {

@@ -475,6 +475,34 @@ def box_stack(n_boxes=3, dt=0.005, iterations=30, integrator=abi.INTEGRATOR_RK4)
                  f"{n_boxes} stacked free boxes on a plane, PGS {iterations}", bodies=bodies)
 
 
+def penalty_stack(dt=0.005, integrator=abi.INTEGRATOR_RK4) -> Scene:
+    """newt's penalty contact golden scene (newt/tests/contacts_golden.rs:31-59):
+    three 1 kg boxes of half extent 0.35 dropped onto a plane (friction 0.6,
+    default SolRef), the middle one 2 cm off in x, the top one spinning at
+    0.3 rad/s about its y axis; the penalty solver (spring-damper contacts,
+    legacy box-box manifold).  Worlds differ by a small random lateral
+    velocity per box."""
+    half = (0.35, 0.35, 0.35)
+    drops = [((0.0, 0.0, 0.5), (0.0, 0.0, 0.0)), ((0.02, 0.0, 1.7), (0.0, 0.0, 0.0)),
+             ((0.0, 0.0, 2.9), (0.0, 0.3, 0.0))]
+    bodies = [body(1.0, solid_box_inertia(1.0, half), p, angular_velocity_body=w) for p, w in drops]
+    geoms = [geom(abi.SHAPE_PLANE, friction=0.6)]
+    geoms += [geom(abi.SHAPE_BOX, body=k, size=half, friction=0.6) for k in range(3)]
+    q0, v0 = _body_state(bodies)
+
+    def sampler(n, seed):
+        r = _rng(seed)
+        q = np.tile(q0, (n, 1))
+        v = np.tile(v0, (n, 1))
+        for k in range(3):
+            v[:, 6 * k:6 * k + 2] = r.normal(0.0, 0.05, size=(n, 2))
+        return q, v
+
+    opts = dict(dt=dt, gravity=(0.0, 0.0, -9.81), integrator=integrator, solver=abi.SOLVER_PENALTY, iterations=20)
+    return Scene("penaltystack3", opts, [], geoms, 0, sampler, "3 boxes dropped on a plane, penalty contacts",
+                 bodies=bodies)
+
+
 def random_quaternions(r, n):
     """Uniform random unit quaternions (x, y, z, w), Shoemake's method."""
     u1, u2, u3 = r.random(n), r.random(n), r.random(n)
