@@ -266,8 +266,9 @@ def bench_boxes(args):
     U[0.05, 0.2]^3, density 1000, jittered 4x4x4 grid, random orientations),
     Newton 20, pyramidal, Euler dt 2 ms: the free-body step (World::add_body
     models; newt/src/world.rs:1705-1817, solver.rs:703-1407, newton.rs),
-    FP64 with newt's operation order.  The same batch under PGS 20 and a
-    16x larger PGS batch are reported beside it."""
+    FP64 with newt's operation order.  The same batch under PGS 20, a 16x
+    larger PGS batch and newt's default penalty solver are reported beside
+    it."""
     import numpy as np
     from oracle import pyoracle
     from tooling_amd import abi, scenes
@@ -314,8 +315,18 @@ def bench_boxes(args):
     bq, bv = pgs_scene.initial_state(big_w, args.seed)
     rb = time_gpu(pgs_model, big_w, abi.PRECISION_FP64, 20, args.warmup, 10, bq, bv)
     log(f"pgs {big_w} worlds: {rb['value']:.4g} world-steps/s, {rb['ms_per_step']:.3f} ms/step")
+    # newt's default solver (penalty contacts, one lane per world) on the same batch
+    pen_scene = scenes.with_options(pgs_scene, "_penalty", solver=abi.SOLVER_PENALTY)
+    pen_model = Model(pen_scene)
+    rpen = time_gpu(pen_model, worlds, abi.PRECISION_FP64, args.steps, args.warmup, args.chunk, q0, v0,
+                    keep_state=True)
+    log(f"penalty fp64: {rpen['value']:.4g} world-steps/s, {rpen['ms_per_step']:.3f} ms/step")
+    pq, pv = rpen["warm_state"]
+    rpb = time_gpu(pen_model, big_w, abi.PRECISION_FP64, 20, args.warmup, 10, bq, bv)
+    log(f"penalty {big_w} worlds: {rpb['value']:.4g} world-steps/s, {rpb['ms_per_step']:.3f} ms/step")
     cpu = None
     cpu_pgs = None
+    cpu_pen = None
     if not args.no_cpu:
         threads = max(1, min(os.cpu_count() or 1, 16))
         n = 2 * threads
@@ -339,6 +350,16 @@ def bench_boxes(args):
         ow.close()
         cpu_pgs = dict(value=n * 50 / dt, unit="world-steps/s", cores=threads, sample=f"{n} worlds x 50 steps")
         log(f"cpu pgs: {cpu_pgs['value']:.4g} world-steps/s on {threads} threads")
+        n = 4 * threads
+        ow = pyoracle.OracleWorlds(pen_scene, n, 1)
+        ow.set_state(pq[:n], pv[:n])
+        t0 = time.perf_counter()
+        ow.step(100, threads=threads)
+        dt = time.perf_counter() - t0
+        ow.close()
+        cpu_pen = dict(value=n * 100 / dt, unit="world-steps/s", cores=threads,
+                       sample=f"{n} worlds x 100 steps from the GPU's warm states")
+        log(f"cpu penalty: {cpu_pen['value']:.4g} world-steps/s on {threads} threads")
     out = {
         "metric": "world-steps/s (free-body step, FP64)",
         "value": r["value"], "unit": "world-steps/s", "n_gpus": 1, "steps": r["steps"], "warmup": args.warmup,
@@ -351,6 +372,11 @@ def bench_boxes(args):
         "exact_vs_fp64_oracle": {"bitwise": exact, "worlds": k, "contacts_min_max": [int(nc.min()), int(nc.max())]},
         "pgs": {"value": rp["value"], "ms_per_step": rp["ms_per_step"], "solver": "PGS 20", "faults": rp["faults"],
                 "cpu_baseline": cpu_pgs},
+        "penalty": {"value": rpen["value"], "ms_per_step": rpen["ms_per_step"],
+                    "solver": "penalty (newt's default): spring-damper contacts, legacy box-box manifold",
+                    "kernel": "k_bodies (one lane per world)", "faults": rpen["faults"], "cpu_baseline": cpu_pen,
+                    "larger_batch": {"worlds": big_w, "value": rpb["value"], "ms_per_step": rpb["ms_per_step"],
+                                     "faults": rpb["faults"]}},
         "pgs_larger_batch": {"worlds": big_w, "value": rb["value"], "ms_per_step": rb["ms_per_step"],
                              "steps": rb["steps"], "warmup": args.warmup,
                              "kernel": "k_bodies (one lane per world)", "faults": rb["faults"]},

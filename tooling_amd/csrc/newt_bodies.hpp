@@ -483,7 +483,7 @@ struct FreeBodies {
     // the contact count (above cap_c: a capacity fault, nothing applied).
     NEWT_HD int penalty_wrenches(int np) const {
         set_poses();
-        const int nc = w.narrowphase(np, true);
+        const int nc = w.narrowphase<true>(np);
         for (int i = 0; i < 6 * w.m.n_bodies; ++i) w.dw[w.L.bwr + i] = 0.0;
         if (nc > w.m.cap_c) return nc;
         for (int c = 0; c < nc; ++c) penalty_wrench(c);
@@ -577,9 +577,11 @@ struct FreeBodies {
     }
 
     // World::step for a free-body model; 0 or a newt_status fault code.
-    NEWT_HD int step() const {
+    // Penalty selects the penalty-contact step (its own kernel instance, so
+    // its registers do not weigh on the constraint-solver step).
+    template <bool Penalty> NEWT_HD int step() const {
         const int np = w.active_pairs();
-        if (w.m.mode == NEWT_SOLVER_PENALTY && w.m.n_pairs > 0) return step_penalty(np);
+        if constexpr (Penalty) return step_penalty(np);
         int nc = 0;
         if (w.m.mode == NEWT_SOLVER_PGS) {
             nc = w.narrowphase(np);

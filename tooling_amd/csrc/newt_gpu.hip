@@ -136,6 +136,7 @@ __global__ void __launch_bounds__(kBlock) k_step(ModelT m, Layout L, DataT d, R*
 }
 
 // Free-body models: n_steps World::steps per lane in FP64 (every precision).
+template <bool Penalty>
 __global__ void __launch_bounds__(kBlock) k_bodies(ModelT m, Layout L, DataT d, double* dws, int* iws, int n_steps,
                                                    int init) {
     int w = blockIdx.x * blockDim.x + threadIdx.x;
@@ -152,7 +153,7 @@ __global__ void __launch_bounds__(kBlock) k_bodies(ModelT m, Layout L, DataT d, 
     }
     wd.populate(m.dt, false);
     for (int s = 0; s < n_steps; ++s) {
-        int st = fb.step();
+        int st = fb.step<Penalty>();
         if (st != 0) {
             d.fault[w] = st;
             break;
@@ -217,7 +218,14 @@ newt_status launch(newt_data* D, int n_steps, hipStream_t stream, int init) {
     if (M->dev.n_bodies > 0) {
         const bool newton = M->dev.mode == NEWT_SOLVER_NEWTON;  // the Newton solve is wavefront-only
         if (!newton && (D->precision == NEWT_PRECISION_FP64_EXACT || !D->bws || D->W > kBodyWaveMaxWorlds))
-            hipLaunchKernelGGL(k_bodies, grid, block, 0, stream, M->dev, D->L, D->d, D->dws, D->iws, n_steps, init);
+        {
+            if (M->dev.mode == NEWT_SOLVER_PENALTY && M->dev.n_pairs > 0)
+                hipLaunchKernelGGL(k_bodies<true>, grid, block, 0, stream, M->dev, D->L, D->d, D->dws, D->iws, n_steps,
+                                   init);
+            else
+                hipLaunchKernelGGL(k_bodies<false>, grid, block, 0, stream, M->dev, D->L, D->d, D->dws, D->iws, n_steps,
+                                   init);
+        }
         else
             hipLaunchKernelGGL(k_bodies_wave, dim3(D->W), dim3(64), D->BL.lds_bytes, stream, M->dev, D->BL, D->L, D->d,
                                D->dws, D->iws, D->bws, D->bwi, n_steps, init);

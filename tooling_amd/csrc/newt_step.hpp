@@ -416,7 +416,7 @@ template <class R> struct World {
 
     // legacy: the penalty pipeline's box-box (box_box, contact.rs:534-553),
     // vertex-vs-face first and the SAT manifold only when that is empty.
-    NEWT_HD int narrowphase(int npairs, bool legacy = false) const {
+    template <bool legacy = false> NEWT_HD int narrowphase(int npairs) const {
         int nc = 0;
         for (int k = 0; k < npairs; ++k) {
             const PairT& pr = m.pairs[iw[L.pairs + k]];
@@ -432,7 +432,7 @@ template <class R> struct World {
                 double pen[4];
                 geom_pose(ga, &pa, &qa);
                 geom_pose(gb, &pb, &qb);
-                if (legacy) {
+                if constexpr (legacy) {
                     V3<double> nv[4];
                     const int c = box_box_vertex_face(pa, qa, ga.size, pb, qb, gb.size, margin, cp, pen, nv);
                     for (int i = 0; i < c; ++i) emit(&nc, pr.a, pr.b, i, cp[i], nv[i], pen[i], fr, gap);
