@@ -23,6 +23,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <cstdio>
+#include <functional>
 #include <vector>
 #include <set>
 #include <limits>
@@ -1039,11 +1040,18 @@ template <class T> void integrate_tree_positions(Tree<T>& t, T dt) {
 
 // euler_step_with_workspace (tree.rs:1393-1420) with zero external wrenches
 // (the constraint-mode callback, world.rs:2027-2029).  Returns qddot.
+// `wrench` (penalty contacts): the per-link world (force, torque) callback
+// of step_trees / step_trees_euler, called with the tree and its FK poses.
 template <class T>
-void euler_step(Tree<T>& t, V3<T> gravity, T dt, bool implicit_fast, AbaWs<T>& ws, std::vector<T>& qacc) {
+using WrenchFn = std::function<std::vector<std::pair<V3<T>, V3<T>>>(const Tree<T>&, const std::vector<Pose<T>>&)>;
+
+template <class T>
+void euler_step(Tree<T>& t, V3<T> gravity, T dt, bool implicit_fast, AbaWs<T>& ws, std::vector<T>& qacc,
+                const WrenchFn<T>* wrench = nullptr) {
     std::vector<Pose<T>> poses;
     forward_kinematics(t, poses);
     std::vector<std::pair<V3<T>, V3<T>>> ext(t.links.size(), {vzero<T>(), vzero<T>()});
+    if (wrench) ext = (*wrench)(t, poses);
     aba_tree(t, poses, gravity, ext, implicit_fast ? VI_JOINT_DAMPING_ACT : VI_JOINT_DAMPING, dt, ws, qacc);
     for (int i = 0; i < t.nv(); ++i) t.qdot[i] += qacc[i] * dt;
     integrate_tree_positions(t, dt);
@@ -1083,7 +1091,9 @@ template <class T> void tree_deriv(const Tree<T>& t, const std::vector<T>& qdd, 
 
 // rk4_step_with_workspace (tree.rs:1234-1364), zero external wrenches, no actuators.
 // k1_out (test instrumentation, not in newt): the first stage's qddot.
-template <class T> void rk4_step(Tree<T>& t, V3<T> gravity, T dt, AbaWs<T>& ws, std::vector<T>* k1_out = nullptr) {
+template <class T>
+void rk4_step(Tree<T>& t, V3<T> gravity, T dt, AbaWs<T>& ws, std::vector<T>* k1_out = nullptr,
+              const WrenchFn<T>* wrench = nullptr) {
     std::vector<T> q0 = t.q, qd0 = t.qdot;
     std::vector<std::pair<V3<T>, V3<T>>> ext(t.links.size(), {vzero<T>(), vzero<T>()});
     std::vector<Pose<T>> poses;
@@ -1091,6 +1101,7 @@ template <class T> void rk4_step(Tree<T>& t, V3<T> gravity, T dt, AbaWs<T>& ws, 
     T stage_dt[3] = {dt * T(0.5), dt * T(0.5), dt};
     for (int s = 0; s < 4; ++s) {
         forward_kinematics(t, poses);
+        if (wrench) ext = (*wrench)(t, poses);
         aba_tree(t, poses, gravity, ext, VI_EXPLICIT, dt, ws, k);
         if (s == 0 && k1_out) *k1_out = k;
         tree_deriv(t, k, dq[s], dv[s]);

@@ -1521,6 +1521,98 @@ template <class T> struct World {
         }
     }
 
+    // tree_wrenches_from_pairs (world.rs:2570-2624) with
+    // apply_tree_contact_wrench (2633-2741): penalty wrenches on the links of
+    // tree `ti` at FK poses `lp`, from the step's pairs touching the tree
+    // (legacy manifold); the other side contributes only its point velocity
+    // and no reaction.  last_contacts keeps the contacts of the last call.
+    std::vector<std::pair<V3<T>, V3<T>>> tree_penalty_wrenches(const Tree<T>& tr, int ti,
+                                                               const std::vector<Pose<T>>& lp,
+                                                               const std::vector<std::pair<int, int>>& tp) {
+        std::vector<std::pair<V3<T>, V3<T>>> out(tr.links.size(), {vzero<T>(), vzero<T>()});
+        last_contacts.clear();
+        if (tp.empty()) return out;
+        std::vector<std::pair<V3<T>, V3<T>>> vel;
+        link_world_velocities(tr, lp, vel);
+        auto pose_of = [&](const Geom<T>& g) {
+            if (g.is_body()) return geom_world_pose(g, bodies[g.body].p, bodies[g.body].q);
+            if (g.is_link() && g.tree == ti) return geom_world_pose(g, lp[g.link].p, lp[g.link].q);
+            return geom_world_pose(g, vzero<T>(), qidentity<T>());
+        };
+        auto mass_of = [&](const Geom<T>& g) {
+            if (g.is_link()) return tr.links[g.link].mass * (g.tree == ti ? T(1.0) : T(0.0));
+            if (g.is_body()) return bodies[g.body].mass;
+            return T(0.0);
+        };
+        auto point_velocity = [&](const Geom<T>& g, V3<T> at, V3<T>* r) {  // point_velocity_generic (2748-2775)
+            *r = vzero<T>();
+            if (g.is_body()) {
+                const Body<T>& b = bodies[g.body];
+                *r = at - b.p;
+                return b.v + cross(rotate(b.q, b.w), *r);
+            }
+            if (g.is_link() && g.tree == ti) {
+                *r = at - lp[g.link].p;
+                return vel[g.link].first + cross(vel[g.link].second, *r);
+            }
+            return vzero<T>();
+        };
+        std::vector<Contact<T>> buf;
+        for (auto [a, b] : tp) {
+            narrow_phase(a, geoms[a], pose_of(geoms[a]), b, geoms[b], pose_of(geoms[b]), buf, true);
+            for (const auto& c : buf) {
+                last_contacts.push_back(c);
+                const Geom<T>& ga = geoms[c.geom_a];
+                const Geom<T>& gb = geoms[c.geom_b];
+                const T ma = mass_of(ga), mb = mass_of(gb);
+                T m_eff;
+                if (ma > T(0.0) && mb > T(0.0)) m_eff = ma * mb / (ma + mb);
+                else if (ma > T(0.0)) m_eff = ma;
+                else if (mb > T(0.0)) m_eff = mb;
+                else continue;
+                const SolRef<T> sr = combine_solref(ga.solref, gb.solref);
+                T k, cd;
+                if (sr.is_direct()) {
+                    k = -sr.timeconst;
+                    cd = -sr.dampratio;
+                } else {
+                    const T tc = sr.timeconst;
+                    k = m_eff / (tc * tc);
+                    cd = T(2.0) * sr.dampratio * m_eff / tc;
+                }
+                V3<T> ra, rb;
+                const V3<T> va = point_velocity(ga, c.pos, &ra);
+                const V3<T> vb = point_velocity(gb, c.pos, &rb);
+                const V3<T> n = c.normal;
+                const V3<T> vrel = va - vb;
+                const T vn = dot(vrel, n);
+                const T pen_eff = penetration_slop > c.gap ? c.penetration - penetration_slop : c.penetration - c.gap;
+                if (pen_eff <= T(0.0)) continue;
+                const T f_raw = k * pen_eff - cd * vn;
+                const T fn = f_raw > T(0.0) ? f_raw : T(0.0);
+                if (fn <= T(0.0)) continue;
+                const V3<T> t1 = tangent_basis(n).first;
+                const V3<T> t2 = cross(n, t1);
+                const V3<T> vt = vrel - n * vn;
+                const T vt1 = dot(vt, t1), vt2 = dot(vt, t2);
+                auto clamp_sym = [](T x, T cap) { return x > cap ? cap : (x < -cap ? -cap : x); };
+                const T ft1 = clamp_sym(-cd * vt1, c.friction * fn);
+                const T ft2 = clamp_sym(-cd * vt2, c.friction * fn);
+                const V3<T> fa = (n * fn + t1 * ft1) + t2 * ft2;
+                if (ga.is_link() && ga.tree == ti) {
+                    out[ga.link].first = out[ga.link].first + fa;
+                    out[ga.link].second = out[ga.link].second + cross(ra, fa);
+                }
+                if (gb.is_link() && gb.tree == ti) {
+                    const V3<T> fb = -fa;
+                    out[gb.link].first = out[gb.link].first + fb;
+                    out[gb.link].second = out[gb.link].second + cross(rb, fb);
+                }
+            }
+        }
+        return out;
+    }
+
     // compute_penalty_wrenches (world.rs:2281-2301) at body state `st`:
     // collect_contacts over the step's pairs with the legacy manifold
     // (world.rs:2434-2474), then apply_contact_wrench contact by contact.
@@ -1681,7 +1773,8 @@ template <class T> struct World {
             tq = solve_tree_contacts(tc, ti);
             bw = solve_free_bodies(bodies, geoms, fc, gravity, dt, cone, iterations, mode == M_NEWTON);
         } else {
-            if (!pairs.empty() && !trees.empty()) fail("penalty tree contact forces are not restated");
+            if (!pairs.empty() && !trees.empty() && !bodies.empty())
+                fail("penalty contacts between trees and bodies are not restated");
         }
         last_qacc.clear();
         for (size_t t = 0; t < trees.size(); ++t) {
@@ -1697,18 +1790,27 @@ template <class T> struct World {
                 for (int s = 0; s < tr.nv(); ++s) tr.qfrc_applied[s] += delta[s];
                 tr.disable_penalty_limits = true;
             }
+            std::vector<std::pair<int, int>> tp;  // step_trees: the pairs touching this tree (penalty)
+            if (!constraint)
+                for (auto [a, b] : pairs)
+                    if ((geoms[a].is_link() && geoms[a].tree == (int)t) || (geoms[b].is_link() && geoms[b].tree == (int)t))
+                        tp.push_back({a, b});
+            WrenchFn<T> wf = [&](const Tree<T>& cur, const std::vector<Pose<T>>& lp) {
+                return tree_penalty_wrenches(cur, (int)t, lp, tp);
+            };
+            const WrenchFn<T>* wfp = tp.empty() ? nullptr : &wf;
             std::vector<T> qacc;
             if (integrator == I_RK4) {
-                rk4_step(tr, gravity, dt, ws[t], &qacc);
+                rk4_step(tr, gravity, dt, ws[t], &qacc, wfp);
                 last_qacc.insert(last_qacc.end(), qacc.begin(), qacc.end());
             } else {
-                euler_step(tr, gravity, dt, integrator == I_IMPLICITFAST, ws[t], qacc);
+                euler_step(tr, gravity, dt, integrator == I_IMPLICITFAST, ws[t], qacc, wfp);
                 last_qacc.insert(last_qacc.end(), qacc.begin(), qacc.end());
             }
             for (size_t s = 0; s < delta.size(); ++s) tr.qfrc_applied[s] -= delta[s];
             tr.disable_penalty_limits = prior;
         }
-        if (!constraint && !pairs.empty()) step_bodies_penalty(pairs);
+        if (!constraint && !pairs.empty() && trees.empty()) step_bodies_penalty(pairs);
         else step_bodies(bw);
         refresh_broadphase();
     }

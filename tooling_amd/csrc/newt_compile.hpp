@@ -408,8 +408,6 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
             M->pairs.push_back({(int)a, (int)b});
             cap_c += pair_max_contacts(ga.shape, gb.shape);
         }
-    if (o.solver_mode == NEWT_SOLVER_PENALTY && !M->pairs.empty())
-        return cfail(err, NEWT_ERR_UNSUPPORTED, "penalty contact forces are not built; use the PGS solver");
     M->v_default.assign(nv, 0.0);
     ModelT& d = M->dev;
     d.n_links = (int)T.size();
@@ -417,7 +415,8 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     d.nv = nv;
     d.n_bodies = 0;
     d.bodies = nullptr;
-    d.penalty_limits = !M->limits.empty() && o.solver_mode == NEWT_SOLVER_PENALTY;
+    // Penalty limits and penalty contacts run on the lane-per-world FP64 step.
+    d.penalty_limits = (!M->limits.empty() || !M->pairs.empty()) && o.solver_mode == NEWT_SOLVER_PENALTY;
     d.n_geoms = (int)M->geoms.size();
     d.n_pairs = (int)M->pairs.size();
     d.cap_c = std::max(cap_c, 1);
@@ -475,7 +474,8 @@ inline Layout make_layout(const ModelT& m, int real_bytes) {
     L.iwc = D(n);
     L.rreg = D(m.cap_r);
     L.rdiag = D(m.cap_r);
-    L.rbias = D(m.cap_r);
+    // row biases; under penalty contacts the per-link contact wrenches instead
+    L.rbias = D(std::max(m.cap_r, m.mode == NEWT_SOLVER_PENALTY ? 6 * m.n_links : 0));
     L.f = D(m.cap_r);
     L.M = D(nv * nv);
     L.Li = D(nv * nv);

@@ -583,6 +583,87 @@ template <class R> struct World {
     // tau: dw[L.tau] generalized applied force; free_applied: whether the
     // free root reads it (disable_penalty_limits).  implicit: joint damping
     // folded into the pivots (Euler).  Result into dw[out].
+    // Penalty contacts on the tree: the penalty solver with contact pairs.
+    NEWT_HD bool penalty_contacts() const { return m.mode == 0 && m.n_pairs > 0; }
+
+    // tree_wrenches_from_pairs (world.rs:2570-2624) with
+    // apply_tree_contact_wrench (2633-2741) at the state whose poses and link
+    // velocities are current: the legacy narrowphase over the step's pairs,
+    // then per contact the spring-damper force on the tree's side(s) (the
+    // static side only lends its zero velocity), world force and torque per
+    // link at dw[L.rbias + 6 link].  Returns the contact count (above cap_c:
+    // a capacity fault, nothing applied).
+    NEWT_HD int tree_penalty_wrenches(int np) const {
+        const int nc = narrowphase<true>(np);
+        for (int k = 0; k < 6 * m.n_links; ++k) dw[L.rbias + k] = 0.0;
+        if (nc > m.cap_c) return nc;
+        for (int c = 0; c < nc; ++c) {
+            const int o = L.cgeo + 9 * c;
+            const GeomT& ga = m.geoms[iw[L.cids + 3 * c]];
+            const GeomT& gb = m.geoms[iw[L.cids + 3 * c + 1]];
+            const int la = ga.link, lb = gb.link;  // -1: static
+            const double ma = la >= 0 ? m.links[la].mass * 1.0 : 0.0, mb = lb >= 0 ? m.links[lb].mass * 1.0 : 0.0;
+            double m_eff;
+            if (ma > 0.0 && mb > 0.0) m_eff = ma * mb / (ma + mb);
+            else if (ma > 0.0) m_eff = ma;
+            else if (mb > 0.0) m_eff = mb;
+            else continue;
+            double sr0 = ga.solref[0] <= gb.solref[0] ? ga.solref[0] : gb.solref[0];
+            double sr1 = ga.solref[1] <= gb.solref[1] ? ga.solref[1] : gb.solref[1];
+            if ((ga.solref[0] < 0.0) != (gb.solref[0] < 0.0)) {
+                const GeomT& gd = ga.solref[0] < 0.0 ? ga : gb;
+                sr0 = gd.solref[0];
+                sr1 = gd.solref[1];
+            }
+            double k, cd;
+            if (sr0 < 0.0) {
+                k = -sr0;
+                cd = -sr1;
+            } else {
+                k = m_eff / (sr0 * sr0);
+                cd = 2.0 * sr1 * m_eff / sr0;
+            }
+            const V3<double> cp = kld3(o), n = kld3(o + 3);
+            V3<double> ra = z3<double>(), rb = z3<double>(), va = z3<double>(), vb = z3<double>();
+            if (la >= 0) {  // point_velocity_generic (world.rs:2748-2775)
+                ra = cp - pos(la);
+                va = kld3(L.lvel + la * 6) + cross3(kld3(L.lvel + la * 6 + 3), ra);
+            }
+            if (lb >= 0) {
+                rb = cp - pos(lb);
+                vb = kld3(L.lvel + lb * 6) + cross3(kld3(L.lvel + lb * 6 + 3), rb);
+            }
+            const V3<double> vrel = va - vb;
+            const double vn = dot3(vrel, n);
+            const double pen = dw[o + 6], gap = dw[o + 8];
+            const double pen_eff = m.slop > gap ? pen - m.slop : pen - gap;
+            if (pen_eff <= 0.0) continue;
+            const double f_raw = k * pen_eff - cd * vn;
+            const double fn = f_raw > 0.0 ? f_raw : 0.0;
+            if (fn <= 0.0) continue;
+            const V3<double> ref = nabs(n.z) < 0.9 ? mk3(0.0, 0.0, 1.0) : mk3(1.0, 0.0, 0.0);
+            const V3<double> t1 = unit3(cross3(ref, n));
+            const V3<double> t2 = cross3(n, t1);
+            const V3<double> vt = vrel - n * vn;
+            const double vt1 = dot3(vt, t1), vt2 = dot3(vt, t2);
+            const double cap = dw[o + 7] * fn;
+            const double x1 = -cd * vt1, x2 = -cd * vt2;
+            const double ft1 = x1 > cap ? cap : (x1 < -cap ? -cap : x1);
+            const double ft2 = x2 > cap ? cap : (x2 < -cap ? -cap : x2);
+            const V3<double> fa = (n * fn + t1 * ft1) + t2 * ft2;
+            if (la >= 0) {
+                kst3(L.rbias + 6 * la, kld3(L.rbias + 6 * la) + fa);
+                kst3(L.rbias + 6 * la + 3, kld3(L.rbias + 6 * la + 3) + cross3(ra, fa));
+            }
+            if (lb >= 0) {
+                const V3<double> fb = -fa;
+                kst3(L.rbias + 6 * lb, kld3(L.rbias + 6 * lb) + fb);
+                kst3(L.rbias + 6 * lb + 3, kld3(L.rbias + 6 * lb + 3) + cross3(rb, fb));
+            }
+        }
+        return nc;
+    }
+
     NEWT_HD void aba(bool free_applied, bool implicit, double dt, int out) const {
         const int n = m.n_links;
         for (int i = 0; i < n; ++i) {
@@ -607,10 +688,16 @@ template <class R> struct World {
             }
             kstm(L.v + i * 6, v);
             kstm(L.c + i * 6, c);
-            // body-frame gravity + zero external wrench (tree.rs:1049-1063)
+            // body-frame gravity + the external wrench (tree.rs:1049-1063):
+            // the penalty contact wrench, else zero
             V3<double> gd = mk3(m.gravity[0], m.gravity[1], m.gravity[2]);
-            V3<double> fbd = qrot_inv(ori(i), z3<double>() + z3<double>() + gd * l.mass);
-            V3<double> tbd = qrot_inv(ori(i), z3<double>() + z3<double>());
+            V3<double> ef = z3<double>(), et = z3<double>();
+            if (penalty_contacts()) {
+                ef = kld3(L.rbias + 6 * i);
+                et = kld3(L.rbias + 6 * i + 3);
+            }
+            V3<double> fbd = qrot_inv(ori(i), ef + z3<double>() + gd * l.mass);
+            V3<double> tbd = qrot_inv(ori(i), et + z3<double>());
             V3<double> fb = mk3((double)fbd.x, (double)fbd.y, (double)fbd.z);
             V3<double> tb = mk3((double)tbd.x, (double)tbd.y, (double)tbd.z);
             double a[36];
@@ -1092,7 +1179,9 @@ template <class R> struct World {
     // rk4_step_with_workspace (tree.rs:1234-1364): the contact force held
     // constant over four explicit ABA stages; the first stage's
     // acceleration is reported as qacc.
-    NEWT_HD void rk4(bool constraint) const {
+    // Penalty contacts are sampled again at every stage (world.rs:1913-1931);
+    // returns a capacity fault code or 0.
+    NEWT_HD int rk4(bool constraint) const {
         const int nv = m.nv, nq = m.nq;
         const double dt = m.dt;
         for (int k = 0; k < nq; ++k) dw[L.rq0 + k] = qpos[k];
@@ -1100,6 +1189,12 @@ template <class R> struct World {
         const double h[3] = {dt * 0.5, dt * 0.5, dt};
         for (int st = 0; st < 4; ++st) {
             forward_kinematics();
+            if (penalty_contacts()) {
+                link_velocities();
+                const int nc = tree_penalty_wrenches(iw[L.iwf + m.n_links]);
+                iw[L.iwf + m.n_links + 1] = nc;
+                if (nc > m.cap_c) return 8;  // NEWT_ERR_CAPACITY
+            }
             aba(constraint, false, dt, L.delta);
             if (st == 0)
                 for (int s = 0; s < nv; ++s) qacc[s] = dw[L.delta + s];
@@ -1128,9 +1223,10 @@ template <class R> struct World {
             qpos[5] = o.z;
             qpos[6] = o.w;
         }
+        return 0;
     }
 
-    NEWT_HD void post() const {
+    NEWT_HD int post() const {
         const R dt = (R)m.dt;
         const int nv = m.nv;
         bool constraint = m.mode == 1;
@@ -1143,12 +1239,18 @@ template <class R> struct World {
             dw[L.tau + s] = qfrc[s] + dl;
         }
         if (m.integrator == 2) {
-            rk4(constraint);
+            const int f = rk4(constraint);
+            if (f != 0) return f;
             for (int s = 0; s < nv; ++s) qfrc[s] = dw[L.tau + s] - dw[L.lim + s];
             forward_kinematics();
             link_velocities();
             populate(dt, false);
-            return;
+            return 0;
+        }
+        if (penalty_contacts()) {  // step_trees_euler: sampled once at the current state
+            const int nc = tree_penalty_wrenches(iw[L.iwf + m.n_links]);
+            iw[L.iwf + m.n_links + 1] = nc;
+            if (nc > m.cap_c) return 8;  // NEWT_ERR_CAPACITY
         }
         aba(constraint, true, m.dt, L.delta);
         for (int s = 0; s < nv; ++s) qfrc[s] = dw[L.tau + s] - dw[L.lim + s];
@@ -1180,13 +1282,13 @@ template <class R> struct World {
         forward_kinematics();
         link_velocities();
         populate(dt, false);
+        return 0;
     }
 
     NEWT_HD int step() const {
         int st = pre();
         if (st != 0) return st;
-        post();
-        return 0;
+        return post();
     }
 };
 
