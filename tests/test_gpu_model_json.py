@@ -2,8 +2,8 @@
 oracle on the same seeded states: newt's own models/pendulum.json (RK4,
 penalty mode, no contacts) and the quadruped written in the format
 (tests/golden/models/quadruped18.json), and newt's models/stack.json (three
-free boxes) under PGS and Newton.  EXACT bitwise, FP64 1e-9 (free bodies:
-bitwise in every mode)."""
+free boxes) under its own penalty solver, PGS and Newton.  EXACT bitwise,
+FP64 1e-9 (free bodies: bitwise in every mode)."""
 import os
 import sys
 
