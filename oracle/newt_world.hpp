@@ -1056,6 +1056,11 @@ template <class T> struct World {
     // Broadphase proxies: fat AABB per geom (DynamicAabbTree leaf bounds).
     std::vector<Aabb<T>> fat;
     std::vector<bool> present;
+    // The bound each proxy was last updated with (swept, margin included),
+    // and newt's reinsert counter (world.rs:1209-1212; a first insertion
+    // counts, as update_with_filter_and_tree returns true for it).
+    std::vector<Aabb<T>> bound;
+    uint64_t reinserts = 0;
     // Outputs of the last step.
     std::vector<std::pair<int, int>> last_pairs;
     std::vector<Contact<T>> last_contacts;
@@ -1097,6 +1102,7 @@ template <class T> struct World {
         std::vector<std::vector<std::pair<V3<T>, V3<T>>>> vel(trees.size());
         for (size_t t = 0; t < trees.size(); ++t) link_world_velocities(trees[t], poses[t], vel[t]);
         fat.resize(geoms.size());
+        bound.resize(geoms.size());
         present.resize(geoms.size(), false);
         for (size_t gi = 0; gi < geoms.size(); ++gi) {
             const Geom<T>& g = geoms[gi];
@@ -1122,11 +1128,13 @@ template <class T> struct World {
             T br = length((sb.mx - sb.mn) * T(0.5));
             T pr = length(bc - pivot) + br;
             T sweep = (ang_speed > T(0.0) && std::isfinite(pr)) ? pr * ang_speed * dt : T(0.0);
-            Aabb<T> bound = expanded(expanded(aunion(sb, eb), sweep), g.margin);
+            const Aabb<T> bd = expanded(expanded(aunion(sb, eb), sweep), g.margin);
+            bound[gi] = bd;
             // update_with_filter_and_tree (broadphase.rs:320-346).
-            if (!present[gi] || !contains(fat[gi], bound)) {
-                fat[gi] = fatten(bound);
+            if (!present[gi] || !contains(fat[gi], bd)) {
+                fat[gi] = fatten(bd);
                 present[gi] = true;
+                ++reinserts;
             }
         }
     }

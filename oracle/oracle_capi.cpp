@@ -38,6 +38,8 @@ struct Base {
     virtual int pairs(int world, int cap, int* out) const = 0;
     virtual void mass_matrix(int world, double* out) const = 0;
     virtual void dynamics(int world, double* qacc, double* M, double* bias) const = 0;
+    virtual int broadphase(int world, double* fat, double* bound) const = 0;
+    virtual long long reinserts(int world, int reset) = 0;
 };
 
 template <class T> struct Impl : Base {
@@ -267,6 +269,25 @@ template <class T> struct Impl : Base {
         for (int k = 0; k < t.nv(); ++k) { qacc[k] = (double)a[k]; bias[k] = (double)h[k]; }
         for (size_t k = 0; k < m.size(); ++k) Mout[k] = (double)m[k];
     }
+    // Each geom's fat proxy bound and the swept bound it was last updated
+    // with, [geom][min xyz, max xyz].
+    int broadphase(int world, double* fat, double* bound) const override {
+        const World<T>& w = worlds[world];
+        for (size_t g = 0; g < w.fat.size(); ++g) {
+            const Aabb<T>* src[2] = {&w.fat[g], &w.bound[g]};
+            double* dst[2] = {fat + 6 * g, bound + 6 * g};
+            for (int k = 0; k < 2; ++k) {
+                dst[k][0] = (double)src[k]->mn.x; dst[k][1] = (double)src[k]->mn.y; dst[k][2] = (double)src[k]->mn.z;
+                dst[k][3] = (double)src[k]->mx.x; dst[k][4] = (double)src[k]->mx.y; dst[k][5] = (double)src[k]->mx.z;
+            }
+        }
+        return (int)w.fat.size();
+    }
+    long long reinserts(int world, int reset) override {
+        const long long n = (long long)worlds[world].reinserts;
+        if (reset) worlds[world].reinserts = 0;
+        return n;
+    }
     void mass_matrix(int world, double* out) const override {
         const Tree<T>& t = worlds[world].trees[0];
         std::vector<T> m = oracle::mass_matrix(t);
@@ -315,6 +336,32 @@ void oracle_flops(unsigned long long* out, int reset) {
 }
 
 // Deterministic sin/cos of the oracle, for the kernel's polynomial check.
+int oracle_broadphase(void* h, int world, double* fat, double* bound) {
+    return static_cast<Base*>(h)->broadphase(world, fat, bound);
+}
+long long oracle_reinserts(void* h, int world, int reset) { return static_cast<Base*>(h)->reinserts(world, reset); }
+// The dynamic tree's pair set for standalone boxes (DynamicAabbTree::insert
+// + compute_pairs, broadphase.rs:46-54, 384-455), in f32: every box is
+// fattened at insertion and the pairs are the sorted (a, b), a < b, whose
+// fat bounds overlap.  boxes: [n][min xyz, max xyz].
+int oracle_box_pairs_f32(int n, const float* boxes, int cap, int* out) {
+    std::vector<Aabb<float>> fat((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const float* b = boxes + 6 * i;
+        fat[i] = fatten(Aabb<float>{{b[0], b[1], b[2]}, {b[3], b[4], b[5]}});
+    }
+    int k = 0;
+    for (int a = 0; a < n; ++a)
+        for (int b = a + 1; b < n; ++b)
+            if (overlaps(fat[a], fat[b])) {
+                if (k < cap) {
+                    out[2 * k] = a;
+                    out[2 * k + 1] = b;
+                }
+                ++k;
+            }
+    return k;
+}
 double oracle_nsin64(double x) { return nsin(x); }
 double oracle_ncos64(double x) { return ncos(x); }
 float oracle_nsin32(float x) { return nsin(x); }

@@ -49,6 +49,9 @@ def lib():
             "oracle_dynamics": (None, [P, I, PD, PD, PD]),
             "oracle_golden_bytes": (I, [C.c_char_p, C.POINTER(C.c_uint8), I]),
             "oracle_flops": (None, [C.POINTER(C.c_ulonglong), I]),
+            "oracle_broadphase": (I, [P, I, PD, PD]),
+            "oracle_reinserts": (C.c_longlong, [P, I, I]),
+            "oracle_box_pairs_f32": (I, [I, C.POINTER(C.c_float), I, PI]),
             "oracle_nsin64": (C.c_double, [C.c_double]),
             "oracle_ncos64": (C.c_double, [C.c_double]),
             "oracle_nsin32": (C.c_float, [C.c_float]),
@@ -68,6 +71,18 @@ def golden_bytes(name: str) -> bytes:
     if n < 0:
         raise KeyError(name)
     return bytes(buf[:n])
+
+
+def box_pairs_f32(boxes):
+    """The oracle's f32 dynamic-tree pair set of standalone boxes
+    [n, 6] = (min xyz, max xyz): fattened at insertion, sorted (a, b)."""
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32)
+    n = boxes.shape[0]
+    cap = n * (n - 1) // 2 + 1
+    out = np.zeros((cap, 2), dtype=np.int32)
+    k = lib().oracle_box_pairs_f32(n, boxes.ctypes.data_as(C.POINTER(C.c_float)), cap,
+                                   out.ctypes.data_as(C.POINTER(C.c_int)))
+    return out[:k].copy()
 
 
 def _dp(a):
@@ -90,7 +105,8 @@ class OracleWorlds:
             L.oracle_add_tree(self.h, C.cast(links, C.c_void_p), len(scene.links), scene.self_collide)
         for b in scene.body_structs():
             L.oracle_add_body(self.h, C.cast(C.pointer(b), C.c_void_p))
-        for g in scene.geom_structs():
+        self._geoms = scene.geom_structs()
+        for g in self._geoms:
             L.oracle_add_geom(self.h, C.cast(C.pointer(g), C.c_void_p))
         L.oracle_finalize(self.h, n_worlds)
         self.nq = L.oracle_nq(self.h)
@@ -135,6 +151,19 @@ class OracleWorlds:
         out = np.zeros((cap, 2), dtype=np.int32)
         n = self.L.oracle_pairs(self.h, world, cap, out.ctypes.data_as(C.POINTER(C.c_int)))
         return out[:n].copy()
+
+    def broadphase(self, world):
+        """(fat proxy bounds, swept bounds they were last updated with),
+        each [n_geoms, 6] = (min xyz, max xyz)."""
+        n = len(self._geoms) if hasattr(self, "_geoms") else 4096
+        fat = np.zeros((n, 6))
+        bound = np.zeros((n, 6))
+        k = self.L.oracle_broadphase(self.h, world, _dp(fat), _dp(bound))
+        return fat[:k].copy(), bound[:k].copy()
+
+    def reinserts(self, world, reset=False):
+        """Broadphase proxy (re)insertions since the last reset."""
+        return int(self.L.oracle_reinserts(self.h, world, 1 if reset else 0))
 
     def mass_matrix(self, world):
         out = np.zeros((self.nv, self.nv))
