@@ -1,28 +1,36 @@
 """Throughput of the batched newt step on one MI355X.
 
-A step is one World::step of every world in the batch.  The workload is the
-18-DoF floating-base quadruped on a plane: 8192 worlds, PGS 50 iterations,
-dt 2 ms, Euler, seeded states, timed after a 300-step warmup that brings the
-batch into contact (newt/src/world.rs:1459-1537 per world and step).
+A step is one World::step of every world in a batch (newt/src/world.rs:
+1459-1537 per world).  The headline workload is the 18-DoF floating-base
+quadruped on a plane: 8192 worlds, PGS 50 iterations, Euler dt 2 ms, seeded
+states dropped from 0.3-0.5 m.  Every run first steps the batch 300 steps
+(untimed, on top of --warmup): the landing, which brings every quadruped
+into contact.  The timed window is then the contact steady state, whatever
+--warmup says.
 
-`value` is the mixed step's world-steps/s with the state resident in HBM;
-the FP64 step (the drop-in) is timed in the same run and reported beside it.
-Each launch advances every world `--chunk` steps; HIP events on the stream
-the kernel runs on bracket every launch, so launch durations are the
-dominant kernel's durations (one kernel per launch).
+`value` is the mixed step's world-steps/s over the --steps timed steps with
+the state resident in HBM; the FP64 step (the drop-in) is timed the same way
+and reported beside it.  Each launch advances every world `--chunk` steps;
+HIP events recorded on the stream the kernel runs on bracket every launch,
+so launch durations are the dominant kernel's durations.  The whole rollout
+from the drop (settle + warmup + timed steps) is timed as well.
 
 Reported beside the throughput:
   roofline        HBM fraction of the algorithmic state traffic (736 B per
                   world-step: qpos/qvel read and written, qacc written, FP64)
-  roofline_fp64   FP64 fraction of the algorithmic FLOPs (newt's own
-                  operation count, from the oracle's counting scalar on a
-                  sample of the same worlds and steps)
+  roofline_fp64   FP64 fraction of newt's own operation count (the oracle's
+                  counting scalar over 50 steps from the timed states)
   qacc_err        mixed step vs the FP64 oracle, one step from the timed
-                  states, relative (max |Δ| / max(max |ref|, 1) per world)
+                  states; `err_100` the same per quantity after 100 steps
   cpu_baseline    the oracle's float instantiation (newt's f32 arithmetic)
                   stepping a sample of the same worlds on the host cores
+  c1 .. c5        one summary per further workload: the 3-link pendulum
+                  (256 worlds x 1000 steps, FP64), the 64-box pile (Newton
+                  20), the uneven hinge-chain batch, and the serving mix,
+                  whose request latency p50/p99 is `rollout_latency_ms`
 
-    python bench.py --gpus 1 --steps 200 --warmup 300
+    python bench.py                      # everything, a few minutes
+    python bench.py --workload boxes     # one workload in detail
 """
 import argparse
 import json
@@ -34,8 +42,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
-FP64_PEAK_TFLOPS = 78.6    # MI355X dense FP64
+FP64_PEAK_TFLOPS = 78.6    # MI355X dense FP64 (vector)
+QUADRUPED_SETTLE = 300     # steps from the drop to contact steady state
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_team_quadruped18_traffic.json")
+METRIC = "world-steps/s mixed-prec vs FP64 GPU at qacc err<=1e-5; p99 rollout latency"
 
 
 def bytes_per_world_step(nq, nv):
@@ -47,75 +57,102 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def threads():
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
 def percentile(xs, p):
     s = sorted(xs)
     k = min(len(s) - 1, max(0, int(round(p / 100.0 * (len(s) - 1)))))
     return s[k]
 
 
-def time_gpu(model, worlds, precision, steps, warmup, chunk, q0, v0, keep_state=False):
-    import torch
+def time_gpu(model, worlds, precision, steps, chunk, q0, v0, settle=0, keep_state=False):
+    """Steps `settle` untimed steps, then `steps` timed steps in launches of
+    `chunk`; each launch is bracketed by HIP events on its stream and the
+    window by device synchronizations.  Also times the whole rollout."""
+    from tooling_amd import hip
     from tooling_amd.newt import Data, step
 
     d = Data(model, worlds, precision)
     d.set_state(q0, v0)
-    stream = torch.cuda.current_stream()
-    sp = stream.cuda_stream
-    torch.cuda.synchronize()
-    if warmup:
-        step(model, d, warmup, sp)
-    torch.cuda.synchronize()
-    warm_state = d.state() if keep_state else None
+    stream = hip.Stream()
+    sp = stream.handle
     n_launch = max(1, steps // chunk)
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(n_launch)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(n_launch)]
-    torch.cuda.synchronize()
+    n_settle = (settle + chunk - 1) // chunk if settle else 0
+    ev = [(hip.Event(), hip.Event()) for _ in range(n_settle + n_launch)]
+    hip.synchronize()
+    for i in range(n_settle):
+        ev[i][0].record(stream)
+        step(model, d, min(chunk, settle - i * chunk), sp)
+        ev[i][1].record(stream)
+    hip.synchronize()
+    warm_state = d.state() if keep_state else None
     t0 = time.perf_counter()
-    for i in range(n_launch):
-        starts[i].record(stream)
+    for i in range(n_settle, n_settle + n_launch):
+        ev[i][0].record(stream)
         step(model, d, chunk, sp)
-        ends[i].record(stream)
-    torch.cuda.synchronize()
+        ev[i][1].record(stream)
+    hip.synchronize()
     wall = time.perf_counter() - t0
-    launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
-    faults = int(d.faults().sum())
+    ms = [a.elapsed_ms(b) for a, b in ev]
+    launch_ms = ms[n_settle:]
+    faults = int((d.faults() != 0).sum())
     d.close()
-    total_steps = n_launch * chunk
+    stream.close()
+    total = n_launch * chunk
     gpu_ms = sum(launch_ms)
-    return dict(steps=total_steps, ms_per_step=gpu_ms / total_steps, wall_s=wall, launch_ms=launch_ms,
-                steps_per_launch=chunk, value=worlds * total_steps / (gpu_ms / 1e3), faults=faults,
-                warm_state=warm_state)
+    rollout_steps = settle + total
+    rollout_ms = sum(ms)
+    return dict(steps=total, ms_per_step=gpu_ms / total, wall_s=wall, launch_ms=launch_ms, steps_per_launch=chunk,
+                value=worlds * total / (gpu_ms / 1e3), faults=faults, warm_state=warm_state,
+                rollout={"steps": rollout_steps, "ms": rollout_ms,
+                         "value": worlds * rollout_steps / (rollout_ms / 1e3)})
 
 
-def qacc_error(model, scene, qs, vs, n):
-    """Mixed step vs the FP64 oracle, one step from the given states."""
+def rel_err(a, r):
+    """Per world: max |a - r| / max(max |r|, 1)."""
     import numpy as np
+    return np.abs(a - r).max(axis=1) / np.maximum(np.abs(r).max(axis=1), 1.0)
+
+
+def error_summary(err):
+    import numpy as np
+    return dict(max=float(err.max()), p99=float(np.percentile(err, 99)), median=float(np.median(err)),
+                worlds=int(err.size), above_1e5=int((err > 1e-5).sum()))
+
+
+def precision_errors(model, scene, qs, vs, n, precision, horizon=100):
+    """One step's qacc from the given states, and qpos/qvel/qacc after
+    `horizon` steps, against the FP64 oracle."""
     from oracle import pyoracle
-    from tooling_amd import abi
     from tooling_amd.newt import Data, step
 
-    d = Data(model, n, abi.PRECISION_MIXED)
+    d = Data(model, n, precision)
     d.set_state(qs[:n], vs[:n])
-    step(model, d, 1)
-    a = d.qacc()
-    d.close()
     ow = pyoracle.OracleWorlds(scene, n, 0)
     ow.set_state(qs[:n], vs[:n])
-    ow.step(1, threads=max(1, min(os.cpu_count() or 1, 16)))
-    r = ow.qacc()
+    step(model, d, 1)
+    ow.step(1, threads=threads())
+    out = {"qacc": error_summary(rel_err(d.qacc(), ow.qacc()))}
+    step(model, d, horizon - 1)
+    ow.step(horizon - 1, threads=threads())
+    gq, gv = d.state()
+    oq, ov = ow.state()
+    out[f"after_{horizon}_steps"] = {
+        "qpos": error_summary(rel_err(gq, oq)), "qvel": error_summary(rel_err(gv, ov)),
+        "qacc": error_summary(rel_err(d.qacc(), ow.qacc()))}
+    d.close()
     ow.close()
-    err = np.abs(a - r).max(axis=1) / np.maximum(np.abs(r).max(axis=1), 1.0)
-    return dict(max=float(err.max()), p99=float(np.percentile(err, 99)), median=float(np.median(err)),
-                worlds=n, above_1e5=int((err > 1e-5).sum()))
+    return out
 
 
-def flops_per_world_step(scene, q0, v0, warmup, n=16, steps=50):
+def flops_per_world_step(scene, qs, vs, n=16, steps=50):
     """newt's operation count per world-step (oracle counting scalar) over
-    steps warmup..warmup+steps of the first n worlds."""
+    `steps` steps of the first n worlds from the given states."""
     from oracle import pyoracle
     ow = pyoracle.OracleWorlds(scene, n, 2)
-    ow.set_state(q0[:n], v0[:n])
-    ow.step(warmup, threads=1)
+    ow.set_state(qs[:n], vs[:n])
     ow.flops(reset=True)
     ow.step(steps, threads=1)
     c = ow.flops(reset=True)
@@ -123,168 +160,148 @@ def flops_per_world_step(scene, q0, v0, warmup, n=16, steps=50):
     return sum(c) / (n * steps), c
 
 
-def cpu_baseline(scene, q0, v0, warmup, steps):
+def cpu_rate(scene, qs, vs, n, steps, warmup=0):
+    """The oracle's float instantiation (newt's f32 step) on the host cores,
+    one world per task."""
     from oracle import pyoracle
-    threads = max(1, min(os.cpu_count() or 1, 16))
-    n = 64 * threads
-    ow = pyoracle.OracleWorlds(scene, n, 1)  # float: newt's arithmetic
-    ow.set_state(q0[:n], v0[:n])
-    ow.step(warmup, threads=threads)
+    th = threads()
+    ow = pyoracle.OracleWorlds(scene, n, 1)
+    ow.set_state(qs[:n], vs[:n])
+    if warmup:
+        ow.step(warmup, threads=th)
     t0 = time.perf_counter()
-    ow.step(steps, threads=threads)
+    ow.step(steps, threads=th)
     dt = time.perf_counter() - t0
     ow.close()
-    return dict(value=n * steps / dt, unit="world-steps/s", cores=threads, kind="port",
-                sample=f"{n} worlds of the same seeded quadruped batch, steps {warmup}..{warmup + steps} after "
-                       f"the same untimed warmup as the GPU; the oracle's float instantiation (newt's f32 "
-                       f"step, byte-identical to newt's goldens) on {threads} host threads, one world per "
-                       f"task ({dt:.1f} s)")
+    return n * steps / dt, dt, th
 
 
-def measured_traffic(worlds, chunk):
-    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE passes, when they were taken on this workload."""
+def measured_traffic(key):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes, when they were taken with the
+    same workload key."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("worlds") != worlds or t.get("steps_per_launch") != chunk:
+    if t.get("key") != key:
         return None
     return t.get("bytes_per_launch")
 
 
-def bench_chains(args):
-    """16384 worlds of fixed-root hinge chains, n ~ U{2..64} links (one model
-    per length), PGS 30, dt 1 ms; every model's batch advances `chunk` steps
-    per launch, launches spread over four streams.  Timed with the device
-    synchronized on both sides of the K steps."""
-    import numpy as np
-    import torch
-    from oracle import pyoracle
+# ---------------------------------------------------------------------------
+def bench_quadruped(args):
     from tooling_amd import abi, scenes
-    from tooling_amd.newt import Data, Model, step
+    from tooling_amd.newt import Model
 
-    worlds = args.worlds if args.worlds != 8192 else 16384
-    rng = np.random.Generator(np.random.PCG64(args.seed))
-    lengths = rng.integers(2, 65, size=worlds)
-    groups = [(n, int((lengths == n).sum())) for n in range(2, 65) if (lengths == n).sum() > 0]
-    streams = [torch.cuda.Stream() for _ in range(4)]
-    log(f"bench chains: {worlds} worlds in {len(groups)} models, warmup {args.warmup}, steps {args.steps}")
-
-    def run(precision):
-        batches = []
-        for n, cnt in groups:
-            sc = scenes.hinge_chain(n)
-            m = Model(sc)
-            d = Data(m, cnt, precision)
-            q0, v0 = sc.initial_state(cnt, args.seed + n)
-            d.set_state(q0, v0)
-            batches.append((n, sc, m, d, q0, v0))
-        torch.cuda.synchronize()
-        for i, (_, _, m, d, _, _) in enumerate(batches):
-            step(m, d, args.warmup, streams[i % 4].cuda_stream)
-        torch.cuda.synchronize()
-        warm = [d.state() for (_, _, _, d, _, _) in batches]
-        t0 = time.perf_counter()
-        for _ in range(args.steps // args.chunk):
-            for i, (_, _, m, d, _, _) in enumerate(batches):
-                step(m, d, args.chunk, streams[i % 4].cuda_stream)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        faults = sum(int(d.faults().sum()) for (_, _, _, d, _, _) in batches)
-        return batches, warm, dt, faults
-
-    mixed, warm, t_mixed, f_mixed = run(abi.PRECISION_MIXED)
-    total = worlds * (args.steps // args.chunk) * args.chunk
-    log(f"mixed: {total / t_mixed:.4g} world-steps/s")
-    errs = []
-    for (n, sc, m, d, _, _), (qs, vs) in zip(mixed, warm):
-        k = min(4, len(qs))
-        dd = Data(m, k, abi.PRECISION_MIXED)
-        dd.set_state(qs[:k], vs[:k])
-        step(m, dd, 1)
-        a = dd.qacc()
-        dd.close()
-        ow = pyoracle.OracleWorlds(sc, k, 0)
-        ow.set_state(qs[:k], vs[:k])
-        ow.step(1, threads=k)
-        r = ow.qacc()
-        ow.close()
-        errs.extend((np.abs(a - r).max(axis=1) / np.maximum(np.abs(r).max(axis=1), 1.0)).tolist())
-    for b in mixed:
-        b[3].close()
-    fp64, _, t_fp64, f_fp64 = run(abi.PRECISION_FP64)
-    for b in fp64:
-        b[3].close()
-    log(f"fp64: {total / t_fp64:.4g} world-steps/s")
-    mean_n = float(lengths.mean())
-    B = 40.0 * mean_n  # 2*(n*8 + n*8) + n*8 bytes per world-step, FP64 state
-    achieved = B * total / t_mixed / 1e9
+    scene = scenes.quadruped()
+    q0, v0 = scene.initial_state(args.worlds, args.seed)
+    model = Model(scene)
+    settle = QUADRUPED_SETTLE + args.warmup
+    log(f"quadruped: {args.worlds} worlds, settle {QUADRUPED_SETTLE} + warmup {args.warmup}, steps {args.steps}, "
+        f"{args.chunk} steps per launch")
+    mixed = time_gpu(model, args.worlds, abi.PRECISION_MIXED, args.steps, args.chunk, q0, v0, settle, True)
+    log(f"  mixed: {mixed['value']:.4g} world-steps/s ({mixed['ms_per_step']:.4f} ms/step); from the drop "
+        f"{mixed['rollout']['value']:.4g}")
+    fp64 = time_gpu(model, args.worlds, abi.PRECISION_FP64, args.steps, args.chunk, q0, v0, settle)
+    log(f"  fp64:  {fp64['value']:.4g} world-steps/s ({fp64['ms_per_step']:.4f} ms/step); from the drop "
+        f"{fp64['rollout']['value']:.4g}")
+    qs, vs = mixed["warm_state"]
+    n_err = min(512, args.worlds)
+    err = precision_errors(model, scene, qs, vs, n_err, abi.PRECISION_MIXED)
+    log(f"  mixed vs FP64 oracle: qacc max {err['qacc']['max']:.3g} over {n_err} worlds")
+    launch_ms = sum(mixed["launch_ms"]) / len(mixed["launch_ms"])
+    spl = mixed["steps_per_launch"]
+    B = bytes_per_world_step(model.nq, model.nv)
+    achieved = B * args.worlds * spl / (launch_ms / 1e3) / 1e9
+    key = f"quadruped18/{args.worlds}/chunk{spl}/settle{settle}"
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(key),
+                "kernel": "newt::k_team<float, 24, false>", "bytes_per_world_step": B,
+                "launch_ms": launch_ms, "steps_per_launch": spl, "worlds": args.worlds, "traffic_key": key}
+    fpw, counts = flops_per_world_step(scene, qs, vs)
+    tflops = fpw * args.worlds * spl / (launch_ms / 1e3) / 1e12
+    roofline_fp64 = {"bound": "fp64", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / FP64_PEAK_TFLOPS, "flop_per_world_step": fpw,
+                     "flop_counts_add_mul_div_sqrt": counts,
+                     "source": "oracle counting scalar, 16 worlds, 50 steps from the timed (settled) states"}
     cpu = None
     if not args.no_cpu:
-        t_cpu, ws = 0.0, 0
-        threads = max(1, min(os.cpu_count() or 1, 16))
-        for n, cnt in groups[::4]:
-            sc = scenes.hinge_chain(n)
-            k = min(cnt, threads)
-            q0, v0 = sc.initial_state(k, args.seed + n)
-            ow = pyoracle.OracleWorlds(sc, k, 1)
-            ow.set_state(q0, v0)
-            ow.step(args.warmup, threads=k)
-            t0 = time.perf_counter()
-            ow.step(100, threads=k)
-            t_cpu += time.perf_counter() - t0
-            ws += k * 100
-            ow.close()
-        cpu = dict(value=ws / t_cpu, unit="world-steps/s", cores=threads, kind="port",
-                   sample=f"up to {threads} worlds of every 4th chain length, 100 steps after the warmup, oracle "
-                          f"float (newt's f32 step), one world per host thread ({t_cpu:.1f} s)")
-    out = {
-        "metric": "world-steps/s mixed-prec vs FP64 GPU at qacc err<=1e-5; p99 rollout latency",
-        "value": total / t_mixed, "unit": "world-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": t_mixed / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (f32 Delassus rows)", "data": "synthetic",
-        "config": {"workload": f"chains: {worlds} worlds, fixed-root hinge chains n ~ U{{2..64}} (mean "
-                               f"{mean_n:.1f}), {len(groups)} models, PGS 30, Euler dt 1 ms, seed {args.seed}",
-                   "worlds": worlds, "steps_per_launch": args.chunk, "streams": 4,
-                   "parallelism": "1 GPU, one wavefront per world, one launch per model per chunk"},
-        "fp64": {"value": total / t_fp64, "ms_per_step": t_fp64 / args.steps * 1e3, "dtype": "f64"},
-        "speedup_vs_fp64": t_fp64 / t_mixed,
-        "qacc_err": {"max": float(max(errs)), "median": float(np.median(errs)), "worlds": len(errs)},
-        "faults": {"mixed": f_mixed, "fp64": f_fp64},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_world_step": B, "note": "wall-clock over all models' launches"},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out))
+        n = 64 * threads()
+        rate, dt, th = cpu_rate(scene, qs, vs, n, args.cpu_steps)
+        cpu = dict(value=rate, unit="world-steps/s", cores=th, kind="port",
+                   sample=f"{n} worlds of the same batch from the GPU's settled states, {args.cpu_steps} steps; "
+                          f"the oracle's float instantiation (newt's f32 step, byte-identical to newt's goldens) "
+                          f"on {th} host threads, one world per task ({dt:.1f} s)")
+        log(f"  cpu: {rate:.4g} world-steps/s on {th} threads")
+    return dict(scene=scene, model=model, mixed=mixed, fp64=fp64, err=err, roofline=roofline,
+                roofline_fp64=roofline_fp64, cpu=cpu, settle=settle)
 
 
-def bench_boxes(args):
+def bench_pendulum(args, worlds=256, steps=1000):
+    """C1: 256 worlds of the 3-link pendulum, 1000 FP64 steps in one launch."""
+    from tooling_amd import abi, scenes
+    from tooling_amd.newt import Model
+
+    scene = scenes.pendulum()
+    q0, v0 = scene.initial_state(worlds, args.seed)
+    model = Model(scene)
+    out = {"workload": f"pendulum3 (C1): {worlds} worlds, 3-link planar revolute pendulum, no contacts, "
+                       f"Euler dt 2 ms, {steps} steps in one launch, seed {args.seed}"}
+    for name, prec in (("fp64", abi.PRECISION_FP64), ("mixed", abi.PRECISION_MIXED)):
+        r = time_gpu(model, worlds, prec, steps, steps, q0, v0)
+        out[name] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "faults": r["faults"]}
+        log(f"pendulum {name}: {r['value']:.4g} world-steps/s")
+    B = bytes_per_world_step(model.nq, model.nv)
+    out["bytes_per_world_step"] = B
+    out["hbm_frac"] = B * out["fp64"]["value"] / 1e9 / HBM_PEAK_GBS
+    from oracle import pyoracle
+    from tooling_amd.newt import Data, step
+    d = Data(model, worlds, abi.PRECISION_FP64)
+    d.set_state(q0, v0)
+    step(model, d, steps)
+    gq, gv = d.state()
+    d.close()
+    ow = pyoracle.OracleWorlds(scene, worlds, 0)
+    ow.set_state(q0, v0)
+    ow.step(steps, threads=threads())
+    oq, ov = ow.state()
+    ow.close()
+    out["fp64_vs_oracle_after_1000_steps"] = {"qpos": float(rel_err(gq, oq).max()),
+                                              "qvel": float(rel_err(gv, ov).max())}
+    if not args.no_cpu:
+        rate, dt, th = cpu_rate(scene, q0, v0, worlds, steps)
+        out["cpu_baseline"] = dict(value=rate, unit="world-steps/s", cores=th, kind="port",
+                                   sample=f"all {worlds} worlds x {steps} steps, oracle float ({dt:.2f} s)")
+    model.close()
+    return out
+
+
+def bench_boxes(args, worlds=1024, settle=300, steps=20, chunk=10):
     """C3: 1024 worlds of 64 free boxes dropped on a plane (half extents ~
     U[0.05, 0.2]^3, density 1000, jittered 4x4x4 grid, random orientations),
-    Newton 20, pyramidal, Euler dt 2 ms: the free-body step (World::add_body
-    models; newt/src/world.rs:1705-1817, solver.rs:703-1407, newton.rs),
-    FP64 with newt's operation order.  The same batch under PGS 20, a 16x
-    larger PGS batch and newt's default penalty solver are reported beside
-    it."""
+    Newton 20, pyramidal, Euler dt 2 ms (World::add_body models;
+    newt/src/world.rs:1705-1817, solver.rs:703-1407, newton.rs)."""
     import numpy as np
     from oracle import pyoracle
     from tooling_amd import abi, scenes
     from tooling_amd.newt import Data, Model, step
 
-    worlds = args.worlds if args.worlds != 8192 else 1024
     pgs_scene = scenes.box_pile()
     scene = scenes.with_options(pgs_scene, "_newton", solver=abi.SOLVER_NEWTON)
     q0, v0 = scene.initial_state(worlds, args.seed)
     model = Model(scene)
-    log(f"bench boxes: {scene.name} x {worlds} worlds, warmup {args.warmup}, steps {args.steps}, "
-        f"{args.chunk} steps per launch")
-    r = time_gpu(model, worlds, abi.PRECISION_FP64, args.steps, args.warmup, args.chunk, q0, v0, keep_state=True)
-    log(f"newton fp64: {r['value']:.4g} world-steps/s, {r['ms_per_step']:.3f} ms/step")
-    qs, vs = r["warm_state"]
-    # one step from the warm states on the GPU and on the FP64 oracle: bitwise
+    out = {"workload": f"boxpile64 (C3): {worlds} worlds x 64 free boxes on a plane, Newton 20 pyramidal, Euler "
+                       f"dt 2 ms, timed after {settle} steps, seed {args.seed}"}
+    runs = {}
+    for name, prec in (("fp64", abi.PRECISION_FP64), ("mixed", abi.PRECISION_MIXED)):
+        r = time_gpu(model, worlds, prec, steps, chunk, q0, v0, settle, keep_state=name == "fp64")
+        runs[name] = r
+        out[name] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "faults": r["faults"],
+                     "launch_ms": sum(r["launch_ms"]) / len(r["launch_ms"])}
+        log(f"boxes {name}: {r['value']:.4g} world-steps/s ({r['ms_per_step']:.3f} ms/step)")
+    qs, vs = runs["fp64"]["warm_state"]
     k = 16
     d = Data(model, k, abi.PRECISION_FP64)
     d.set_state(qs[:k], vs[:k])
@@ -299,261 +316,161 @@ def bench_boxes(args):
     oq, ov = ow.state()
     oa = ow.qacc()
     ow.close()
-    exact = bool(np.array_equal(gq, oq) and np.array_equal(gv, ov) and np.array_equal(ga, oa))
-    log(f"one step from the warm states vs the FP64 oracle: bit-exact {exact}; contacts {nc.min()}..{nc.max()}")
-    launch_ms = sum(r["launch_ms"]) / len(r["launch_ms"])
+    out["fp64_vs_oracle_one_step"] = {"qpos": float(rel_err(gq, oq).max()), "qvel": float(rel_err(gv, ov).max()),
+                                      "qacc": float(rel_err(ga, oa).max()), "worlds": k,
+                                      "contacts_min_max": [int(nc.min()), int(nc.max())]}
+    fpw, counts = flops_per_world_step(scene, qs, vs, n=4, steps=5)
+    launch_ms = out["fp64"]["launch_ms"]
+    tflops = fpw * worlds * chunk / (launch_ms / 1e3) / 1e12
+    out["roofline_fp64"] = {"achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                            "frac": tflops / FP64_PEAK_TFLOPS, "flop_per_world_step": fpw,
+                            "source": "oracle counting scalar, 4 worlds from the settled states, 5 steps"}
     B = bytes_per_world_step(model.nq, model.nv)
-    achieved = B * worlds * r["steps_per_launch"] / (launch_ms / 1e3) / 1e9
-    fpw, counts = flops_per_world_step(scene, qs, vs, 0, n=4, steps=5)
-    tflops = fpw * worlds * r["steps_per_launch"] / (launch_ms / 1e3) / 1e12
-    log(f"newt's operation count {fpw:.4g} flop/world-step -> {tflops:.3g} TFLOP/s")
-    # PGS 20 on the same batch, and 16x the batch (one lane per world there)
-    pgs_model = Model(pgs_scene)
-    rp = time_gpu(pgs_model, worlds, abi.PRECISION_FP64, args.steps, args.warmup, args.chunk, q0, v0)
-    log(f"pgs fp64: {rp['value']:.4g} world-steps/s, {rp['ms_per_step']:.3f} ms/step")
-    big_w = 16 * worlds
-    bq, bv = pgs_scene.initial_state(big_w, args.seed)
-    rb = time_gpu(pgs_model, big_w, abi.PRECISION_FP64, 20, args.warmup, 10, bq, bv)
-    log(f"pgs {big_w} worlds: {rb['value']:.4g} world-steps/s, {rb['ms_per_step']:.3f} ms/step")
-    # newt's default solver (penalty contacts, one lane per world) on the same batch
-    pen_scene = scenes.with_options(pgs_scene, "_penalty", solver=abi.SOLVER_PENALTY)
-    pen_model = Model(pen_scene)
-    rpen = time_gpu(pen_model, worlds, abi.PRECISION_FP64, args.steps, args.warmup, args.chunk, q0, v0,
-                    keep_state=True)
-    log(f"penalty fp64: {rpen['value']:.4g} world-steps/s, {rpen['ms_per_step']:.3f} ms/step")
-    pq, pv = rpen["warm_state"]
-    rpb = time_gpu(pen_model, big_w, abi.PRECISION_FP64, 20, args.warmup, 10, bq, bv)
-    log(f"penalty {big_w} worlds: {rpb['value']:.4g} world-steps/s, {rpb['ms_per_step']:.3f} ms/step")
-    cpu = None
-    cpu_pgs = None
-    cpu_pen = None
+    out["hbm_frac"] = B * out["fp64"]["value"] / 1e9 / HBM_PEAK_GBS
     if not args.no_cpu:
-        threads = max(1, min(os.cpu_count() or 1, 16))
-        n = 2 * threads
-        ow = pyoracle.OracleWorlds(scene, n, 1)
-        ow.set_state(qs[:n], vs[:n])
-        t0 = time.perf_counter()
-        ow.step(10, threads=threads)
-        dt = time.perf_counter() - t0
-        ow.close()
-        cpu = dict(value=n * 10 / dt, unit="world-steps/s", cores=threads, kind="port",
-                   sample=f"{n} worlds of the same batch from the GPU's warm states, 10 steps, the oracle's "
-                          f"float instantiation (newt's f32 step, byte-identical to newt's solver_newton_stack "
-                          f"golden) on {threads} host threads ({dt:.1f} s)")
-        log(f"cpu newton: {cpu['value']:.4g} world-steps/s on {threads} threads")
-        n = 4 * threads
-        ow = pyoracle.OracleWorlds(pgs_scene, n, 1)
-        ow.set_state(qs[:n], vs[:n])
-        t0 = time.perf_counter()
-        ow.step(50, threads=threads)
-        dt = time.perf_counter() - t0
-        ow.close()
-        cpu_pgs = dict(value=n * 50 / dt, unit="world-steps/s", cores=threads, sample=f"{n} worlds x 50 steps")
-        log(f"cpu pgs: {cpu_pgs['value']:.4g} world-steps/s on {threads} threads")
-        n = 4 * threads
-        ow = pyoracle.OracleWorlds(pen_scene, n, 1)
-        ow.set_state(pq[:n], pv[:n])
-        t0 = time.perf_counter()
-        ow.step(100, threads=threads)
-        dt = time.perf_counter() - t0
-        ow.close()
-        cpu_pen = dict(value=n * 100 / dt, unit="world-steps/s", cores=threads,
-                       sample=f"{n} worlds x 100 steps from the GPU's warm states")
-        log(f"cpu penalty: {cpu_pen['value']:.4g} world-steps/s on {threads} threads")
-    out = {
-        "metric": "world-steps/s (free-body step, FP64)",
-        "value": r["value"], "unit": "world-steps/s", "n_gpus": 1, "steps": r["steps"], "warmup": args.warmup,
-        "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"boxes (C3): {worlds} worlds x 64 free boxes on a plane (2080 candidate pairs), "
-                               f"Newton 20 pyramidal, Euler dt 2 ms, seed {args.seed}",
-                   "worlds": worlds, "steps_per_launch": r["steps_per_launch"],
-                   "parallelism": "1 GPU, one wavefront per world (k_bodies_wave)"},
-        "exact_vs_fp64_oracle": {"bitwise": exact, "worlds": k, "contacts_min_max": [int(nc.min()), int(nc.max())]},
-        "pgs": {"value": rp["value"], "ms_per_step": rp["ms_per_step"], "solver": "PGS 20", "faults": rp["faults"],
-                "cpu_baseline": cpu_pgs},
-        "penalty": {"value": rpen["value"], "ms_per_step": rpen["ms_per_step"],
-                    "solver": "penalty (newt's default): spring-damper contacts, legacy box-box manifold",
-                    "kernel": "k_bodies (one lane per world)", "faults": rpen["faults"], "cpu_baseline": cpu_pen,
-                    "larger_batch": {"worlds": big_w, "value": rpb["value"], "ms_per_step": rpb["ms_per_step"],
-                                     "faults": rpb["faults"]}},
-        "pgs_larger_batch": {"worlds": big_w, "value": rb["value"], "ms_per_step": rb["ms_per_step"],
-                             "steps": rb["steps"], "warmup": args.warmup,
-                             "kernel": "k_bodies (one lane per world)", "faults": rb["faults"]},
-        "faults": r["faults"],
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_bodies_wave",
-                     "bytes_per_world_step": B, "launch_ms": launch_ms,
-                     "steps_per_launch": r["steps_per_launch"], "worlds": worlds},
-        "roofline_fp64": {"bound": "fp64", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                          "frac": tflops / FP64_PEAK_TFLOPS, "flop_per_world_step": fpw,
-                          "flop_counts_add_mul_div_sqrt": counts,
-                          "source": "oracle counting scalar, 4 worlds from the warm states, 5 steps"},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out))
+        n = 2 * threads()
+        rate, dt, th = cpu_rate(scene, qs, vs, n, 10)
+        out["cpu_baseline"] = dict(value=rate, unit="world-steps/s", cores=th, kind="port",
+                                   sample=f"{n} worlds from the GPU's settled states x 10 steps, oracle float "
+                                          f"(newt's f32 step) ({dt:.1f} s)")
+        log(f"boxes cpu: {rate:.4g} world-steps/s")
+    model.close()
+    return out
 
 
-def bench_serve(args):
-    """Rollout requests arriving as a Poisson process (`--rate` per second):
-    model uniform over the 3-link pendulum, the quadruped and fixed-root
-    hinge chains of U{2..64} links, worlds ~ U{1..512}, horizon
-    ~ U{100..2000} steps.  Served by continuous batching (one resident pool
-    per model, four streams) in the mixed step, then in the FP64 step; latency
-    is arrival to results on the host."""
+def bench_chains(args, worlds=16384, settle=100, steps=100, chunk=20):
+    """C4: 16384 worlds of fixed-root hinge chains, n ~ U{2..64} links (one
+    model per length), PGS 30, dt 1 ms, contacts against the plane.  Every
+    model's batch advances `chunk` steps per launch; the launches of one
+    chunk are enqueued over four streams.  Timed with the device
+    synchronized on both sides of the timed steps."""
+    import numpy as np
+    from oracle import pyoracle
+    from tooling_amd import abi, hip, scenes
+    from tooling_amd.newt import Data, Model, step
+
+    rng = np.random.Generator(np.random.PCG64(args.seed))
+    lengths = rng.integers(2, 65, size=worlds)
+    groups = [(n, int((lengths == n).sum())) for n in range(2, 65) if (lengths == n).sum() > 0]
+    streams = [hip.Stream() for _ in range(4)]
+    log(f"chains: {worlds} worlds in {len(groups)} models, settle {settle}, steps {steps}")
+
+    def run(precision):
+        batches = []
+        for n, cnt in groups:
+            sc = scenes.hinge_chain(n)
+            m = Model(sc)
+            d = Data(m, cnt, precision)
+            q0, v0 = sc.initial_state(cnt, args.seed + n)
+            d.set_state(q0, v0)
+            batches.append((n, sc, m, d))
+        hip.synchronize()
+        for i, (_, _, m, d) in enumerate(batches):
+            step(m, d, settle, streams[i % 4].handle)
+        hip.synchronize()
+        warm = [d.state() for (_, _, _, d) in batches]
+        t0 = time.perf_counter()
+        for _ in range(steps // chunk):
+            for i, (_, _, m, d) in enumerate(batches):
+                step(m, d, chunk, streams[i % 4].handle)
+        hip.synchronize()
+        dt = time.perf_counter() - t0
+        faults = sum(int((d.faults() != 0).sum()) for (_, _, _, d) in batches)
+        return batches, warm, dt, faults
+
+    total = worlds * (steps // chunk) * chunk
+    out = {"workload": f"chains (C4): {worlds} worlds, fixed-root hinge chains n ~ U{{2..64}} (mean "
+                       f"{float(lengths.mean()):.1f}), {len(groups)} models, PGS 30, Euler dt 1 ms, "
+                       f"timed after {settle} steps, seed {args.seed}"}
+    mixed, warm, t_mixed, f_mixed = run(abi.PRECISION_MIXED)
+    errs = []
+    for (n, sc, m, d), (qs, vs) in zip(mixed, warm):
+        if n % 8:
+            continue
+        k = min(4, len(qs))
+        dd = Data(m, k, abi.PRECISION_MIXED)
+        dd.set_state(qs[:k], vs[:k])
+        step(m, dd, 1)
+        a = dd.qacc()
+        dd.close()
+        ow = pyoracle.OracleWorlds(sc, k, 0)
+        ow.set_state(qs[:k], vs[:k])
+        ow.step(1, threads=k)
+        errs.extend(rel_err(a, ow.qacc()).tolist())
+        ow.close()
+    for b in mixed:
+        b[3].close()
+        b[2].close()
+    fp64, _, t_fp64, f_fp64 = run(abi.PRECISION_FP64)
+    for b in fp64:
+        b[3].close()
+        b[2].close()
+    for s in streams:
+        s.close()
+    out["mixed"] = {"value": total / t_mixed, "ms_per_step": t_mixed / steps * 1e3, "faults": f_mixed}
+    out["fp64"] = {"value": total / t_fp64, "ms_per_step": t_fp64 / steps * 1e3, "faults": f_fp64}
+    out["qacc_err_mixed"] = {"max": float(max(errs)), "worlds": len(errs)}
+    B = 40.0 * float(lengths.mean())
+    out["hbm_frac"] = B * out["mixed"]["value"] / 1e9 / HBM_PEAK_GBS
+    log(f"chains mixed {out['mixed']['value']:.4g}, fp64 {out['fp64']['value']:.4g} world-steps/s")
+    return out
+
+
+def bench_serve(args, n_requests=32):
+    """C5: rollout requests arriving as a Poisson process (`--rate` per
+    second): model uniform over the 3-link pendulum, the quadruped and
+    fixed-root hinge chains of U{2..64} links, worlds ~ U{1..512}, horizon
+    ~ U{100..2000} steps; continuous batching over resident pools, mixed
+    then FP64.  Latency is arrival to results on the host."""
     import numpy as np
     from tooling_amd import abi, serve
 
     def requests():
-        return serve.rollout_mix(args.requests, args.seed, rate=args.rate)
+        return serve.rollout_mix(n_requests, args.seed, rate=args.rate)
 
     reqs = requests()
     keys = sorted({r.key for r in reqs})
     total = sum(r.n_worlds * r.horizon for r in reqs)
-    log(f"bench serve: {len(reqs)} requests over {len(keys)} models at {args.rate}/s, {total} world-steps, "
-        f"chunk <= {args.serve_chunk}, quantum {args.serve_quantum_ms} ms")
+    log(f"serve: {len(reqs)} requests over {len(keys)} models at {args.rate}/s, {total} world-steps")
     pools = {k: (serve.scene_for(k), 2048 if k.startswith("chain") else 8192) for k in keys}
-    runs = {}
+    out = {"workload": f"serve (C5): {len(reqs)} rollout requests, Poisson {args.rate}/s, model ~ U{{pendulum3, "
+                       f"quadruped18, chain2..64}}, worlds ~ U{{1..512}}, horizon ~ U{{100..2000}}, seed "
+                       f"{args.seed}", "world_steps": total, "models": len(keys)}
     for name, precision in (("mixed", abi.PRECISION_MIXED), ("fp64", abi.PRECISION_FP64)):
-        srv = serve.Server(pools, precision=precision, chunk=args.serve_chunk, n_streams=4,
-                           quantum=args.serve_quantum_ms / 1e3 if args.serve_quantum_ms > 0 else None)
+        srv = serve.Server(pools, precision=precision, chunk=400, n_streams=4, quantum=0.05)
         try:
-            # Load every kernel once before the clock starts.
             srv.run([serve.Request(k, r.qpos[:1], r.qvel[:1], 1) for k in keys
                      for r in [next(x for x in reqs if x.key == k)]])
             srv.launches = srv.world_steps = 0
             rs = requests()
-            last = [time.perf_counter()]
-
-            def progress(left, t):
-                if time.perf_counter() - last[0] > 20.0:
-                    last[0] = time.perf_counter()
-                    log(f"  {name}: {left} requests left at {t:.1f} s")
-
-            elapsed = srv.run(rs, time_limit=600.0, progress=progress)
+            elapsed = srv.run(rs, time_limit=600.0)
             faults = int(sum(int((r.faults != 0).sum()) for r in rs))
             by_family = {}
             for r in rs:
                 fam = "chain" if r.key.startswith("chain") else r.key
                 by_family.setdefault(fam, []).append(r.latency * 1e3)
-            runs[name] = {"value": total / elapsed, "elapsed_s": elapsed, "latency_ms": serve.latency_summary(rs),
-                          "latency_p99_ms_by_family": {f: float(np.percentile(v, 99)) for f, v in by_family.items()},
-                          "launches": srv.launches, "device_bytes": srv.device_bytes, "faults": faults}
+            out[name] = {"value": total / elapsed, "elapsed_s": elapsed, "latency_ms": serve.latency_summary(rs),
+                         "latency_p99_ms_by_family": {f: float(np.percentile(v, 99)) for f, v in by_family.items()},
+                         "launches": srv.launches, "device_bytes": srv.device_bytes, "faults": faults}
         finally:
             srv.close()
-        log(f"{name}: {runs[name]['value']:.4g} world-steps/s over {runs[name]['elapsed_s']:.2f} s, "
-            f"p50 {runs[name]['latency_ms']['p50_ms']:.1f} ms, p99 {runs[name]['latency_ms']['p99_ms']:.1f} ms")
-    m, f = runs["mixed"], runs["fp64"]
+        log(f"serve {name}: {out[name]['value']:.4g} world-steps/s, p50 {out[name]['latency_ms']['p50_ms']:.1f} ms, "
+            f"p99 {out[name]['latency_ms']['p99_ms']:.1f} ms")
+    out["speedup_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    return out
+
+
+def headline_line(args, q, extras):
+    mixed, fp64 = q["mixed"], q["fp64"]
+    scene = q["scene"]
+    serve = extras.get("c5_serve")
+    lat = None
+    if serve:
+        l = serve["mixed"]["latency_ms"]
+        lat = {"p50": l["p50_ms"], "p99": l["p99_ms"], "max": l["max_ms"], "requests": l["n"],
+               "source": "c5_serve: arrival to results on the host, mixed step"}
     out = {
-        "metric": "world-steps/s mixed-prec vs FP64 GPU at qacc err<=1e-5; p99 rollout latency",
-        "value": m["value"], "unit": "world-steps/s", "n_gpus": 1, "steps": None, "warmup": None,
-        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 (f32 Delassus rows)", "data": "synthetic",
-        "config": {"workload": f"serve: {len(reqs)} rollout requests, Poisson {args.rate}/s, model ~ U{{pendulum3, "
-                               f"quadruped18, chain2..64}}, worlds ~ U{{1..512}}, horizon ~ U{{100..2000}}, "
-                               f"seed {args.seed}",
-                   "models": len(keys), "world_steps": total, "max_steps_per_launch": args.serve_chunk,
-                   "launch_quantum_ms": args.serve_quantum_ms,
-                   "streams": 4, "parallelism": "1 GPU, continuous batching: one resident pool per model"},
-        "latency_ms": m["latency_ms"], "latency_p99_ms_by_family": m["latency_p99_ms_by_family"],
-        "launches": m["launches"], "device_bytes": m["device_bytes"], "faults": {"mixed": m["faults"],
-                                                                                 "fp64": f["faults"]},
-        "fp64": f, "speedup_vs_fp64": m["value"] / f["value"],
-    }
-    print(json.dumps(out))
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=300,
-                    help="untimed steps; 300 steps take the batch through landing into contact")
-    ap.add_argument("--worlds", type=int, default=8192)
-    ap.add_argument("--chunk", type=int, default=10, help="steps per kernel launch (one rollout)")
-    ap.add_argument("--seed", type=int, default=2026)
-    ap.add_argument("--cpu-steps", type=int, default=1000)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["quadruped", "chains", "serve", "boxes"], default="quadruped",
-                    help="quadruped: 8192 quadrupeds (the headline); chains: fixed-root hinge chains of "
-                         "2..64 links, one model per length, batched over streams; serve: Poisson rollout "
-                         "requests served by continuous batching; boxes: 1024 worlds of 64 free boxes, Newton 20 (C3)")
-    ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
-    ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
-    ap.add_argument("--serve-chunk", type=int, default=400, help="serve: most steps per launch")
-    ap.add_argument("--serve-quantum-ms", type=float, default=50.0,
-                    help="serve: target launch length; 0 runs every launch at --serve-chunk steps")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only copy a MIB-MiB device buffer 5 times (reference kernel for the HBM counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: worlds are independent and shard by world index with no "
-              "per-step exchange, so runs on more than one GPU are not part of this build.", file=sys.stderr)
-        sys.exit(2)
-    if args.steps % args.chunk:
-        args.chunk = 1
-    import torch
-    if not torch.cuda.is_available():
-        print("no GPU visible", file=sys.stderr)
-        sys.exit(1)
-    if args.copy_calibration:
-        n = args.copy_calibration * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_mib": args.copy_calibration, "copies": 5}))
-        return
-    if args.workload == "chains":
-        bench_chains(args)
-        return
-    if args.workload == "serve":
-        bench_serve(args)
-        return
-    if args.workload == "boxes":
-        bench_boxes(args)
-        return
-    from tooling_amd import abi, scenes
-    from tooling_amd.newt import Model
-
-    scene = scenes.quadruped()
-    q0, v0 = scene.initial_state(args.worlds, args.seed)
-    model = Model(scene)
-    log(f"bench: {scene.name} x {args.worlds} worlds, warmup {args.warmup}, steps {args.steps}, "
-        f"{args.chunk} steps per launch")
-    mixed = time_gpu(model, args.worlds, abi.PRECISION_MIXED, args.steps, args.warmup, args.chunk, q0, v0,
-                     keep_state=True)
-    log(f"mixed: {mixed['value']:.4g} world-steps/s, {mixed['ms_per_step']:.4f} ms/step")
-    fp64 = time_gpu(model, args.worlds, abi.PRECISION_FP64, args.steps, args.warmup, args.chunk, q0, v0)
-    log(f"fp64: {fp64['value']:.4g} world-steps/s, {fp64['ms_per_step']:.4f} ms/step")
-
-    qs, vs = mixed["warm_state"]
-    err = qacc_error(model, scene, qs, vs, min(512, args.worlds))
-    log(f"mixed qacc error vs FP64 oracle: max {err['max']:.3g}, p99 {err['p99']:.3g} over {err['worlds']} worlds")
-
-    launch_ms = sum(mixed["launch_ms"]) / len(mixed["launch_ms"])
-    steps_per_launch = mixed["steps_per_launch"]
-    B = bytes_per_world_step(model.nq, model.nv)
-    achieved = B * args.worlds * steps_per_launch / (launch_ms / 1e3) / 1e9
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.worlds, args.chunk),
-                "kernel": "newt::k_team<float, 24>", "bytes_per_world_step": B,
-                "launch_ms": launch_ms, "steps_per_launch": steps_per_launch, "worlds": args.worlds}
-    fpw, counts = flops_per_world_step(scene, q0, v0, args.warmup)
-    tflops = fpw * args.worlds * steps_per_launch / (launch_ms / 1e3) / 1e12
-    roofline_fp64 = {"bound": "fp64", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": tflops / FP64_PEAK_TFLOPS, "flop_per_world_step": fpw,
-                     "flop_counts_add_mul_div_sqrt": counts,
-                     "source": f"oracle counting scalar, 16 worlds, steps {args.warmup}..{args.warmup + 50}"}
-    cpu = None
-    if not args.no_cpu:
-        log("cpu baseline ...")
-        cpu = cpu_baseline(scene, q0, v0, args.warmup, args.cpu_steps)
-        log(f"cpu: {cpu['value']:.4g} world-steps/s on {cpu['cores']} threads")
-    lat = mixed["launch_ms"]
-    out = {
-        "metric": "world-steps/s mixed-prec vs FP64 GPU at qacc err<=1e-5; p99 rollout latency",
+        "metric": METRIC,
         "value": mixed["value"],
         "unit": "world-steps/s",
         "n_gpus": 1,
@@ -565,27 +482,66 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 (f32 Delassus rows)",
         "data": "synthetic",
-        "config": {"workload": f"{scene.name}: 18-DoF floating-base quadruped on a plane, "
-                               f"{args.worlds} worlds, PGS {scene.options['iterations']} iterations, "
-                               f"Euler dt {scene.options['dt']} s, seed {args.seed}, timed after {args.warmup} "
-                               f"steps",
-                   "worlds": args.worlds, "steps_per_launch": args.chunk,
-                   "precision": "mixed: FP64 state, kinematics, dynamics, factor and PGS; rows of "
-                                "Y = L^-1 J^T stored in FP32",
-                   "parallelism": "1 GPU, one wavefront per world, 8 worlds per workgroup"},
+        "config": {"workload": f"{scene.name}: 18-DoF floating-base quadruped on a plane, {args.worlds} worlds, "
+                               f"PGS {scene.options['iterations']} iterations, Euler dt {scene.options['dt']} s, "
+                               f"seed {args.seed}; timed in contact steady state after {QUADRUPED_SETTLE} "
+                               f"settle steps (the drop and landing) + {args.warmup} warmup steps, untimed",
+                   "worlds": args.worlds, "steps_per_launch": args.chunk, "settle_steps": QUADRUPED_SETTLE,
+                   "parallelism": "1 GPU, one wavefront per world"},
         "fp64": {"value": fp64["value"], "ms_per_step": fp64["ms_per_step"], "dtype": "f64",
-                 "launch_ms": sum(fp64["launch_ms"]) / len(fp64["launch_ms"])},
+                 "launch_ms": sum(fp64["launch_ms"]) / len(fp64["launch_ms"]), "rollout_from_drop": fp64["rollout"]},
         "speedup_vs_fp64": mixed["value"] / fp64["value"],
-        "qacc_err": err,
-        "rollout_latency_ms": {"steps_per_rollout": args.chunk, "worlds": args.worlds,
-                               "p50": percentile(lat, 50), "p99": percentile(lat, 99), "max": max(lat),
-                               "samples": len(lat)},
+        "rollout_from_drop": mixed["rollout"],
+        "qacc_err": q["err"]["qacc"],
+        "err_per_quantity": q["err"],
+        "rollout_latency_ms": lat,
         "faults": {"mixed": mixed["faults"], "fp64": fp64["faults"]},
-        "roofline": roofline,
-        "roofline_fp64": roofline_fp64,
-        "cpu_baseline": cpu,
+        "roofline": q["roofline"],
+        "roofline_fp64": q["roofline_fp64"],
+        "cpu_baseline": q["cpu"],
     }
-    print(json.dumps(out))
+    out.update(extras)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=0,
+                    help="untimed steps after the intrinsic 300-step settle")
+    ap.add_argument("--worlds", type=int, default=8192)
+    ap.add_argument("--chunk", type=int, default=10, help="steps per kernel launch")
+    ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--cpu-steps", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="the quadruped line only")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
+    ap.add_argument("--workload", choices=["all", "quadruped", "pendulum", "boxes", "chains", "serve"],
+                    default="all")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: worlds are independent and shard by world index with no "
+              "per-step exchange, so runs on more than one GPU are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+    if args.steps % args.chunk:
+        args.chunk = 1
+    from tooling_amd import hip
+    if hip.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    single = {"pendulum": bench_pendulum, "boxes": bench_boxes, "chains": bench_chains, "serve": bench_serve}
+    if args.workload in single:
+        print(json.dumps(single[args.workload](args)))
+        return
+    q = bench_quadruped(args)
+    extras = {}
+    if args.workload == "all" and not args.no_extras:
+        extras["c1_pendulum"] = bench_pendulum(args)
+        extras["c3_boxes"] = bench_boxes(args)
+        extras["c4_chains"] = bench_chains(args)
+        extras["c5_serve"] = bench_serve(args)
+    print(json.dumps(headline_line(args, q, extras)))
 
 
 if __name__ == "__main__":

@@ -285,8 +285,10 @@ newt_status newt_data_get_faults(const newt_data* data, int32_t* faults);
  * [w*cap + k] holds (geom_a, geom_b, slot) in `ids` (3 int32 each) and
  * (position xyz, normal xyz, penetration) in `geo` (7 doubles each).
  * `cap` is the per-world capacity reported by newt_data_contact_capacity.
- * Free bodies under the penalty solver: the contacts of the step's last
- * RK4 stage (collect_contacts, world.rs:2434-2474, at that stage's state). */
+ * Under the penalty solver (trees or free bodies), RK4 reports the contacts
+ * of the step's last stage (collect_contacts, world.rs:2434-2474, at that
+ * stage's state); Euler and ImplicitFast report the start-of-step contacts.
+ * A world frozen by a fault keeps its last completed state. */
 newt_status newt_data_contact_capacity(const newt_data* data, int32_t* cap);
 newt_status newt_data_get_contacts(const newt_data* data, int32_t* counts,
                                    int32_t* ids, double* geo);

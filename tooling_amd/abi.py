@@ -150,11 +150,10 @@ def load_library(path: str = LIB_PATH):
     if not os.path.exists(path):
         raise RuntimeError(
             f"{path} is missing: build the HIP extension first (python -m tooling_amd.build)")
-    # torch ships its own HIP runtime under the soname of the system one
-    # (libamdhip64.so.7).  Loaded after this library, torch would map a second
-    # runtime and report no device; loaded first, its runtime serves both, so
-    # torch streams and events are valid handles for this library.
-    import torch  # noqa: F401
+    # The library links the system HIP runtime (libamdhip64.so.7);
+    # tooling_amd.hip drives streams and events through that same runtime.
+    # A process that also uses torch must import torch first: torch ships its
+    # own runtime under that soname, which then serves both.
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

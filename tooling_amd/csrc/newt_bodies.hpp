@@ -543,7 +543,15 @@ struct FreeBodies {
         if (rk4) {
             for (int stg = 2; stg <= 4; ++stg) {
                 nc = penalty_wrenches(np);
-                if (nc > w.m.cap_c) return 8;
+                if (nc > w.m.cap_c) {
+                    // A world frozen here reports its start-of-step state,
+                    // not the stage state written for this evaluation.
+                    for (int b = 0; b < nb; ++b) {
+                        const int o0 = S + 26 * b;
+                        put(b, St{w.kld3(o0), w.kld3(o0 + 3), ldq(o0 + 6), w.kld3(o0 + 10)});
+                    }
+                    return 8;  // NEWT_ERR_CAPACITY
+                }
                 const double wgt = stg == 4 ? 1.0 : 2.0;
                 for (int b = 0; b < nb; ++b) {
                     const BodyT& bt = body(b);

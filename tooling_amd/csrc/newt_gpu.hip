@@ -16,35 +16,26 @@
 #include "newt_team.hpp"
 #include "newt_bodies.hpp"
 
-namespace newt {
-// Defined in newt_team.hip (the persistent-step instantiations).
-hipError_t team_launch(bool mixed, int nv, bool ext, dim3 grid, dim3 block, int lds_bytes, hipStream_t stream,
-                       const ModelT& m, const TeamLayout& T, const CacheLayout& C, const int4* blob, const Layout& L,
-                       const DataT& d, double* dws, int* iws, char* ovf, int n_steps, int wpb);
-hipError_t team_set_lds(bool mixed, int nv, bool ext, int bytes);
-int team_max_worlds_for(int nv);
-}  // namespace newt
+#include "newt_handles.hpp"
 
 using namespace newt;
 
-namespace {
+namespace newt {
 
+namespace {
 thread_local std::string g_err;
+}  // namespace
 
 newt_status fail(newt_status s, const std::string& msg) {
     g_err = msg;
     return s;
 }
 
-#define HIP_TRY(expr)                                                                 \
-    do {                                                                              \
-        hipError_t e_ = (expr);                                                       \
-        if (e_ != hipSuccess)                                                         \
-            return fail(NEWT_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
-    } while (0)
+}  // namespace newt
+
+namespace {
 
 constexpr int kBlock = 64;  // one wavefront: 64 worlds
-constexpr int kMaxLds = 160 * 1024;  // LDS of one gfx950 workgroup
 constexpr int kBodyWaveLds = 40 * 1024;  // one free-body world: four per CU
 // Free-body batches up to this size run one wavefront per world (parallel
 // broadphase, narrowphase and row assembly; at most four worlds per CU by
@@ -53,47 +44,6 @@ constexpr int kBodyWaveLds = 40 * 1024;  // one free-body world: four per CU
 constexpr int kBodyWaveMaxWorlds = 2048;
 
 }  // namespace
-
-struct newt_model {
-    HostModel h;
-    bool finalized = false;
-    ModelT dev{};  // device pointers
-    LinkT* d_links = nullptr;
-    GeomT* d_geoms = nullptr;
-    PairT* d_pairs = nullptr;
-    int* d_chain = nullptr;
-    int* d_level_off = nullptr;
-    int* d_level_links = nullptr;
-    int* d_level_nrank = nullptr;
-    int* d_dof_link = nullptr;
-    LimitT* d_limits = nullptr;
-    BodyT* d_bodies = nullptr;
-    CacheLayout cache{};     // model tables as the persistent step caches them in LDS
-    char* d_blob = nullptr;  // the same tables packed in that layout
-};
-
-struct newt_data {
-    const newt_model* model = nullptr;
-    int W = 0;
-    int precision = 0;
-    Layout L{};
-    TeamLayout TL{};  // LDS layout of one world in the persistent step
-    BodyWaveLayout BL{};  // free-body models: one world's LDS block in the wavefront step
-    double* bws = nullptr;  // and its per-world global fallback for contact rows
-    int* bwi = nullptr;
-    int wpb = 1;      // worlds (wavefronts) per workgroup of the persistent step
-    int lds_bytes = 0;
-    DataT d{};
-    void* ws = nullptr;      // real workspace (double or float)
-    double* dws = nullptr;   // factorization workspace
-    int* iws = nullptr;      // integer workspace
-    char* ovf = nullptr;     // per-world blocks for contact sets over 64 rows (persistent step)
-    uint64_t device_bytes = 0;     // device memory this data allocated
-    std::vector<double> fat_default;  // fat bounds of one world at the default state
-    double* staging = nullptr;     // pinned host copy of the last asynchronous state upload
-    hipEvent_t staged = nullptr;   // completes when that upload has left the staging buffer
-    hipStream_t last_stream = nullptr;
-};
 
 namespace {
 
@@ -206,8 +156,7 @@ newt_status launch_persistent(newt_data* D, int n_steps, hipStream_t stream) {
     const int wpb = D->wpb;
     dim3 grid((D->W + wpb - 1) / wpb), block(64 * wpb);
     HIP_TRY(team_launch(D->precision == NEWT_PRECISION_MIXED, M->dev.nv, M->dev.ext != 0, grid, block, D->lds_bytes,
-                        stream, M->dev, D->TL, M->cache, (const int4*)M->d_blob, D->L, D->d, D->dws, D->iws, D->ovf,
-                        n_steps, wpb));
+                        stream, D->d_job, 1, n_steps));
     return NEWT_OK;
 }
 
@@ -515,6 +464,12 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
     s = launch(D, 0, nullptr, 1);
     if (s != NEWT_OK) return s;
     HIP_TRY(hipDeviceSynchronize());
+    if (team) {
+        const TeamJob job = team_job(D);
+        HIP_TRY(hipMalloc(&D->d_job, sizeof(TeamJob)));
+        D->device_bytes += sizeof(TeamJob);
+        HIP_TRY(hipMemcpy(D->d_job, &job, sizeof(TeamJob), hipMemcpyHostToDevice));
+    }
     D->fat_default.resize((size_t)6 * m.n_geoms);
     if (m.n_geoms)
         HIP_TRY(hipMemcpy2D(D->fat_default.data(), 8, D->d.fat, W * 8, 8, 6 * m.n_geoms, hipMemcpyDeviceToHost));
@@ -537,6 +492,7 @@ void newt_data_destroy(newt_data* D) {
     (void)hipFree(D->ovf);
     (void)hipFree(D->bws);
     (void)hipFree(D->bwi);
+    (void)hipFree(D->d_job);
     if (D->staged) {
         (void)hipEventSynchronize(D->staged);
         (void)hipEventDestroy(D->staged);

@@ -1193,7 +1193,12 @@ template <class R> struct World {
                 link_velocities();
                 const int nc = tree_penalty_wrenches(iw[L.iwf + m.n_links]);
                 iw[L.iwf + m.n_links + 1] = nc;
-                if (nc > m.cap_c) return 8;  // NEWT_ERR_CAPACITY
+                if (nc > m.cap_c) {
+                    // Frozen at the start-of-step state, not a stage state.
+                    for (int k = 0; k < nq; ++k) qpos[k] = dw[L.rq0 + k];
+                    for (int s = 0; s < nv; ++s) qvel[s] = dw[L.rv0 + s];
+                    return 8;  // NEWT_ERR_CAPACITY
+                }
             }
             aba(constraint, false, dt, L.delta);
             if (st == 0)

@@ -29,12 +29,11 @@ int team_max_worlds_for(int nv) {
 
 namespace {
 template <class R, bool X>
-hipError_t launch_r(int nv, dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const ModelT& m,
-                    const TeamLayout& T, const CacheLayout& C, const int4* blob, const Layout& L, const DataT& d,
-                    double* dws, int* iws, char* ovf, int n_steps, int wpb) {
-    if (nv <= 24) return launch_team<R, 24, X>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb);
-    if (nv <= 32) return launch_team<R, 32, X>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb);
-    return launch_team<R, 64, X>(grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb);
+hipError_t launch_r(int nv, dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const TeamJob* jobs, int n_jobs,
+                    int n_steps) {
+    if (nv <= 24) return launch_team<R, 24, X>(grid, block, lds_bytes, stream, jobs, n_jobs, n_steps);
+    if (nv <= 32) return launch_team<R, 32, X>(grid, block, lds_bytes, stream, jobs, n_jobs, n_steps);
+    return launch_team<R, 64, X>(grid, block, lds_bytes, stream, jobs, n_jobs, n_steps);
 }
 template <class R, bool X>
 hipError_t set_lds_r(int nv, int bytes) {
@@ -45,13 +44,12 @@ hipError_t set_lds_r(int nv, int bytes) {
 }  // namespace
 
 hipError_t team_launch(bool mixed, int nv, bool ext, dim3 grid, dim3 block, int lds_bytes, hipStream_t stream,
-                       const ModelT& m, const TeamLayout& T, const CacheLayout& C, const int4* blob, const Layout& L,
-                       const DataT& d, double* dws, int* iws, char* ovf, int n_steps, int wpb) {
+                       const TeamJob* jobs, int n_jobs, int n_steps) {
     if (mixed)
-        return ext ? launch_r<float, true>(nv, grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb)
-                   : launch_r<float, false>(nv, grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb);
-    return ext ? launch_r<double, true>(nv, grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb)
-               : launch_r<double, false>(nv, grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps, wpb);
+        return ext ? launch_r<float, true>(nv, grid, block, lds_bytes, stream, jobs, n_jobs, n_steps)
+                   : launch_r<float, false>(nv, grid, block, lds_bytes, stream, jobs, n_jobs, n_steps);
+    return ext ? launch_r<double, true>(nv, grid, block, lds_bytes, stream, jobs, n_jobs, n_steps)
+               : launch_r<double, false>(nv, grid, block, lds_bytes, stream, jobs, n_jobs, n_steps);
 }
 
 hipError_t team_set_lds(bool mixed, int nv, bool ext, int bytes) {

@@ -1687,20 +1687,56 @@ struct Team {
 // Largest workgroup of the persistent step, in worlds (wavefronts).
 template <int NV> constexpr int team_max_worlds() { return NV <= 24 ? 8 : 4; }
 
-// Persistent step kernel.  A workgroup of `wpb` wavefronts steps worlds
-// blockIdx.x*wpb .. +wpb-1, one wavefront each, n_steps times.  The model
-// tables are copied once into the workgroup's LDS (shared by its worlds);
-// each world's state follows at C.bytes + wave*T.bytes.  Global state is
-// lane-interleaved ([element][W]).  At NV <= 24 the register budget is held
-// to 256 so two wavefronts fit a SIMD (eight per CU).
+// One batch of worlds sharing a model, as the persistent step reads it: the
+// model's scalars and LDS table layout, the per-world LDS layout, the state
+// and output arrays, and the workgroups it occupies in the grid.
+struct TeamJob {
+    ModelT m;
+    TeamLayout T;
+    CacheLayout C;
+    const int4* blob;  // model tables in the layout of C
+    Layout L;          // offsets of the contact and pair outputs in dws / iws
+    DataT d;
+    double* dws;
+    int* iws;
+    char* ovf;         // per-world blocks for contact sets over 64 rows
+    int wpb;           // worlds (wavefronts) per workgroup
+    int first_block;   // first workgroup of this job in the grid
+};
+
+// Persistent step kernel over a table of jobs (sorted by first_block; one
+// job for a newt_data, several for a batch of datas of different models).
+// Workgroup b runs the last job whose first_block <= b: its `wpb`
+// wavefronts step worlds (b - first_block)*wpb .. +wpb-1, one wavefront
+// each, n_steps times.  The model tables are copied once into the
+// workgroup's LDS (shared by its worlds); each world's state follows at
+// C.bytes + wave*T.bytes.  Global state is lane-interleaved ([element][W]).
+// At NV <= 24 the register budget is held to 256 so two wavefronts fit a
+// SIMD (eight per CU).
 template <class R, int NV, bool X>
 __global__ void __launch_bounds__(64 * team_max_worlds<NV>())
     __attribute__((amdgpu_waves_per_eu(NV <= 24 ? 2 : 1)))
-    k_team(ModelT m, TeamLayout T, CacheLayout C, const int4* __restrict__ blob, Layout L, DataT d,
-           double* __restrict__ dws, int* __restrict__ iws, char* __restrict__ ovf, int n_steps, int wpb) {
+    k_team(const TeamJob* __restrict__ jobs, int n_jobs, int n_steps) {
     extern __shared__ __align__(16) double smem[];
+    // The job of this workgroup: uniform, found by bisection.
+    int lo = 0, hi = n_jobs;
+    const int b = (int)blockIdx.x;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (jobs[mid].first_block <= b) lo = mid;
+        else hi = mid;
+    }
+    const TeamJob& J = jobs[lo];
+    const ModelT& m = J.m;
+    const TeamLayout& T = J.T;
+    const CacheLayout& C = J.C;
+    const Layout& L = J.L;
+    const DataT& d = J.d;
+    double* __restrict__ dws = J.dws;
+    int* __restrict__ iws = J.iws;
+    const int wpb = J.wpb;
     char* lds = (char*)smem;
-    for (int k = threadIdx.x; k < C.bytes / 16; k += blockDim.x) ((int4*)lds)[k] = blob[k];
+    for (int k = threadIdx.x; k < C.bytes / 16; k += blockDim.x) ((int4*)lds)[k] = J.blob[k];
     __syncthreads();
     const Tables tb{(const LinkT*)(lds + C.links),     (const GeomT*)(lds + C.geoms),
                     (const PairT*)(lds + C.pairs),     (const int*)(lds + C.level_off),
@@ -1709,14 +1745,14 @@ __global__ void __launch_bounds__(64 * team_max_worlds<NV>())
     // The wavefront index is uniform within a wavefront; readfirstlane lets
     // the compiler treat everything derived from it as scalar.
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63, W = d.W;
-    const int w = blockIdx.x * wpb + wave;
-    if (w >= W || d.fault[w] != 0) return;
+    const int w = (b - J.first_block) * wpb + wave;
+    if (wave >= wpb || w >= W || d.fault[w] != 0) return;
     char* base = lds + C.bytes + wave * T.bytes;
     double* sd = (double*)base;
     int* si = (int*)(base + T.i_off);
     R* sJ = (R*)(base + T.j_off);
     const int cr = m.cap_r;
-    const RowStore<R> ov{(R*)(ovf + (size_t)w * T.ov_stride), sd + T.lg, sd + T.lg + cr, sd + T.lg + 2 * cr,
+    const RowStore<R> ov{(R*)(J.ovf + (size_t)w * T.ov_stride), sd + T.lg, sd + T.lg + cr, sd + T.lg + 2 * cr,
                          sd + T.lg + 3 * cr};
     Team<R, NV, X> t{m, T, tb, lane, sd, si, sJ, ov};
     for (int k = lane; k < m.nq; k += 64) sd[T.q + k] = d.qpos[(size_t)k * W + w];
@@ -1736,9 +1772,13 @@ __global__ void __launch_bounds__(64 * team_max_worlds<NV>())
     wsync();
     t.kinematics();
     // update_broadphase at step start (world.rs:1190-1226): a no-op unless
-    // the state was set since the last step.
-    t.populate(false);
-    int fault = 0, np = d.npair[w], nc = d.ncontact[w];
+    // the state was set since the last step; with init_bounds the proxies
+    // are rebuilt from the state, as after a state setter.
+    t.populate(d.init_bounds != 0);
+    // The previous launch's counts, clamped to the compiled capacities (they
+    // bound the copy-out below when no step runs).
+    int fault = 0, np = d.init_bounds ? 0 : min(max(d.npair[w], 0), m.n_pairs),
+        nc = d.init_bounds ? 0 : min(max(d.ncontact[w], 0), m.cap_c);
     for (int s = 0; s < n_steps && fault == 0; ++s) fault = t.step(&np, &nc);
     for (int k = lane; k < m.nq; k += 64) d.qpos[(size_t)k * W + w] = sd[T.q + k];
     for (int k = lane; k < m.nv; k += 64) {
@@ -1760,11 +1800,9 @@ __global__ void __launch_bounds__(64 * team_max_worlds<NV>())
 // one of the newt_team*.hip units (so the units compile in parallel) and
 // declared extern below for the dispatcher.
 template <class R, int NV, bool X>
-hipError_t launch_team(dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const ModelT& m, const TeamLayout& T,
-                       const CacheLayout& C, const int4* blob, const Layout& L, const DataT& d, double* dws, int* iws,
-                       char* ovf, int n_steps, int wpb) {
-    hipLaunchKernelGGL((k_team<R, NV, X>), grid, block, lds_bytes, stream, m, T, C, blob, L, d, dws, iws, ovf, n_steps,
-                       wpb);
+hipError_t launch_team(dim3 grid, dim3 block, int lds_bytes, hipStream_t stream, const TeamJob* jobs, int n_jobs,
+                       int n_steps) {
+    hipLaunchKernelGGL((k_team<R, NV, X>), grid, block, lds_bytes, stream, jobs, n_jobs, n_steps);
     return hipGetLastError();
 }
 template <class R, int NV, bool X>
@@ -1772,15 +1810,11 @@ hipError_t set_team_lds(int bytes) {
     return hipFuncSetAttribute((const void*)k_team<R, NV, X>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
-#define NEWT_TEAM_EXTERN(R, NV, X)                                                                                   \
-    extern template hipError_t launch_team<R, NV, X>(dim3, dim3, int, hipStream_t, const ModelT&, const TeamLayout&,  \
-                                                     const CacheLayout&, const int4*, const Layout&, const DataT&,    \
-                                                     double*, int*, char*, int, int);                                 \
+#define NEWT_TEAM_EXTERN(R, NV, X)                                                                           \
+    extern template hipError_t launch_team<R, NV, X>(dim3, dim3, int, hipStream_t, const TeamJob*, int, int); \
     extern template hipError_t set_team_lds<R, NV, X>(int);
-#define NEWT_TEAM_INSTANCE(R, NV, X)                                                                          \
-    template hipError_t launch_team<R, NV, X>(dim3, dim3, int, hipStream_t, const ModelT&, const TeamLayout&, \
-                                              const CacheLayout&, const int4*, const Layout&, const DataT&,   \
-                                              double*, int*, char*, int, int);                                \
+#define NEWT_TEAM_INSTANCE(R, NV, X)                                                                  \
+    template hipError_t launch_team<R, NV, X>(dim3, dim3, int, hipStream_t, const TeamJob*, int, int); \
     template hipError_t set_team_lds<R, NV, X>(int);
 
 }  // namespace newt

@@ -123,6 +123,10 @@ struct DataT {
     int* fault;       // [W]
     int* ncontact;    // [W] contacts of the last step
     int* npair;       // [W] pairs of the last step
+    // Nonzero: rebuild every proxy's fat bound from the state before the
+    // first step (the refresh the state setters make, world.rs:1119-1136)
+    // instead of continuing from `fat`.  Read by the persistent step.
+    int init_bounds;
 };
 
 }  // namespace newt

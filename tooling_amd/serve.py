@@ -43,7 +43,7 @@ from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
-from . import abi
+from . import abi, hip
 from .newt import Data, Model, step
 from .scenes import Scene
 
@@ -165,8 +165,7 @@ class Server:
 
     def __init__(self, pools: Dict[str, tuple], precision: int = abi.PRECISION_MIXED, chunk: int = 50,
                  n_streams: int = 4, hbm_budget: int = 200 << 30, quantum: Optional[float] = None):
-        import torch
-        if not torch.cuda.is_available():
+        if hip.device_count() < 1:
             raise RuntimeError("the serving loop needs a GPU")
         if chunk < 1:
             raise ValueError("chunk must be at least 1")
@@ -187,7 +186,7 @@ class Server:
         except BaseException:
             self.close()
             raise
-        self.lanes = [_Lane(torch.cuda.Stream(), torch.cuda.Event()) for _ in range(n_streams)]
+        self.lanes = [_Lane(hip.Stream(), hip.Event(timing=False)) for _ in range(n_streams)]
         self.world_steps = 0
         self.launches = 0
 
@@ -236,7 +235,7 @@ class Server:
                 best = p
         if best is None:
             return False
-        p, s = best, lane.stream.cuda_stream
+        p, s = best, lane.stream.handle
         for first, n in p.to_park:
             p.data.park_worlds(first, n, s)
         p.to_park.clear()
