@@ -312,6 +312,42 @@ newt_status newt_step(const newt_model* model, newt_data* data,
  * stream of its last step). */
 newt_status newt_data_synchronize(newt_data* data);
 
+/* ---- step without allocation --------------------------------------------- */
+
+/* Device state of n_worlds worlds owned by the caller, lane-interleaved:
+ * element k of world w at [k * n_worlds + w].  Same meaning as a newt_data's
+ * state: qpos [nq], qvel/qacc/qfrc [nv] (qfrc = Tree::qfrc_applied), fat
+ * [6 * n_geoms] broadphase proxy bounds (min xyz, max xyz per geom), and per
+ * world the fault code, the last step's contact and pair counts. */
+typedef struct newt_device_state {
+    double* qpos;
+    double* qvel;
+    double* qacc;
+    double* qfrc;
+    double* fat;
+    int32_t* fault;
+    int32_t* ncontact;
+    int32_t* npair;
+} newt_device_state;
+
+/* newt_step on caller-owned device state, allocating nothing: no hipMalloc,
+ * hipFree or other allocating call; its scratch (the last step's contacts
+ * and pairs, and the Jacobian rows of contact sets over 64 rows) is a
+ * static device array of 160 MiB per GPU, held until the process ends.
+ * No counterpart in the reference (newt steps one World in host memory);
+ * it is for callers that cannot allocate: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation.
+ *   reset_bounds = 1 rebuilds every proxy bound from qpos/qvel before the
+ *   first step, as newt_data_set_state does; 0 continues from `fat`.
+ * Tree models, NEWT_PRECISION_FP64 or _MIXED; other models return
+ * NEWT_ERR_UNSUPPORTED.  A batch over the scratch size returns
+ * NEWT_ERR_CAPACITY (newt_noalloc_max_worlds gives the largest).  The
+ * scratch keeps the last call's contents; calls that overlap on one GPU
+ * race. */
+newt_status newt_step_noalloc(const newt_model* model, const newt_device_state* state, int32_t n_worlds,
+                              int32_t precision, int32_t reset_bounds, int32_t n_steps, void* stream);
+newt_status newt_noalloc_max_worlds(const newt_model* model, int32_t precision, int32_t* max_worlds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -104,6 +104,19 @@ class BodyDesc(C.Structure):
     ]
 
 
+class DeviceState(C.Structure):
+    _fields_ = [
+        ("qpos", C.c_void_p),
+        ("qvel", C.c_void_p),
+        ("qacc", C.c_void_p),
+        ("qfrc", C.c_void_p),
+        ("fat", C.c_void_p),
+        ("fault", C.c_void_p),
+        ("ncontact", C.c_void_p),
+        ("npair", C.c_void_p),
+    ]
+
+
 # Every symbol include/newt_gpu.h declares, with (restype, argtypes).
 _P = C.c_void_p
 _I = C.c_int32
@@ -137,6 +150,8 @@ SIGNATURES = {
     "newt_data_get_pairs": (_I, [_P, _PI, _PI]),
     "newt_step": (_I, [_P, _P, _I, _P]),
     "newt_data_synchronize": (_I, [_P]),
+    "newt_step_noalloc": (_I, [_P, C.POINTER(DeviceState), _I, _I, _I, _I, _P]),
+    "newt_noalloc_max_worlds": (_I, [_P, _I, _PI]),
 }
 
 _lib = None
