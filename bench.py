@@ -338,51 +338,46 @@ def bench_boxes(args, worlds=1024, settle=300, steps=20, chunk=10):
     return out
 
 
-def bench_chains(args, worlds=16384, settle=100, steps=100, chunk=20):
+def bench_chains(args, worlds=16384, settle=100, steps=100):
     """C4: 16384 worlds of fixed-root hinge chains, n ~ U{2..64} links (one
-    model per length), PGS 30, dt 1 ms, contacts against the plane.  Every
-    model's batch advances `chunk` steps per launch; the launches of one
-    chunk are enqueued over four streams.  Timed with the device
-    synchronized on both sides of the timed steps."""
+    model per length), PGS 30, dt 1 ms, contacts against the plane, stepped
+    as one uneven batch (newt_batch): one persistent launch per kernel
+    instantiation, replayed from a HIP graph, in the launch shape the
+    autotuner chose (tooling_amd/tuning/chains.json).  Timed with HIP events
+    on the replay stream, the device synchronized on both sides."""
     import numpy as np
     from oracle import pyoracle
-    from tooling_amd import abi, hip, scenes
-    from tooling_amd.newt import Data, Model, step
+    from tooling_amd import abi, autotune, hip
+    from tooling_amd.newt import Batch, Data, step
 
-    rng = np.random.Generator(np.random.PCG64(args.seed))
-    lengths = rng.integers(2, 65, size=worlds)
-    groups = [(n, int((lengths == n).sum())) for n in range(2, 65) if (lengths == n).sum() > 0]
-    streams = [hip.Stream() for _ in range(4)]
-    log(f"chains: {worlds} worlds in {len(groups)} models, settle {settle}, steps {steps}")
+    cfg = autotune.load()
+    chunk = cfg["chunk"]
+    log(f"chains: {worlds} worlds, settle {settle}, steps {steps}, tuned shape {cfg}")
 
     def run(precision):
-        batches = []
-        for n, cnt in groups:
-            sc = scenes.hinge_chain(n)
-            m = Model(sc)
-            d = Data(m, cnt, precision)
-            q0, v0 = sc.initial_state(cnt, args.seed + n)
-            d.set_state(q0, v0)
-            batches.append((n, sc, m, d))
+        lengths, datas = autotune.chain_batch(worlds, args.seed, precision)
+        stream = hip.Stream()
+        b = Batch([d for (_, _, _, d) in datas], order=cfg["order"], wpb_cap=tuple(cfg["wpb_cap"]),
+                  use_graph=cfg["use_graph"], branches=cfg["branches"])
+        b.step(settle, stream.handle)
+        b.step(chunk, stream.handle)  # captures the timed graph
         hip.synchronize()
-        for i, (_, _, m, d) in enumerate(batches):
-            step(m, d, settle, streams[i % 4].handle)
+        warm = [d.state() for (_, _, _, d) in datas]
+        n = max(1, steps // chunk)
+        t0, t1 = hip.Event(), hip.Event()
+        t0.record(stream)
+        for _ in range(n):
+            b.step(chunk, stream.handle)
+        t1.record(stream)
         hip.synchronize()
-        warm = [d.state() for (_, _, _, d) in batches]
-        t0 = time.perf_counter()
-        for _ in range(steps // chunk):
-            for i, (_, _, m, d) in enumerate(batches):
-                step(m, d, chunk, streams[i % 4].handle)
-        hip.synchronize()
-        dt = time.perf_counter() - t0
-        faults = sum(int((d.faults() != 0).sum()) for (_, _, _, d) in batches)
-        return batches, warm, dt, faults
+        ms = t0.elapsed_ms(t1)
+        faults = sum(int((d.faults() != 0).sum()) for (_, _, _, d) in datas)
+        launches = b.launches()
+        b.close()
+        stream.close()
+        return lengths, datas, warm, ms, n * chunk, faults, launches
 
-    total = worlds * (steps // chunk) * chunk
-    out = {"workload": f"chains (C4): {worlds} worlds, fixed-root hinge chains n ~ U{{2..64}} (mean "
-                       f"{float(lengths.mean()):.1f}), {len(groups)} models, PGS 30, Euler dt 1 ms, "
-                       f"timed after {settle} steps, seed {args.seed}"}
-    mixed, warm, t_mixed, f_mixed = run(abi.PRECISION_MIXED)
+    lengths, mixed, warm, ms_m, total_steps, f_mixed, launches = run(abi.PRECISION_MIXED)
     errs = []
     for (n, sc, m, d), (qs, vs) in zip(mixed, warm):
         if n % 8:
@@ -398,17 +393,23 @@ def bench_chains(args, worlds=16384, settle=100, steps=100, chunk=20):
         ow.step(1, threads=k)
         errs.extend(rel_err(a, ow.qacc()).tolist())
         ow.close()
-    for b in mixed:
-        b[3].close()
-        b[2].close()
-    fp64, _, t_fp64, f_fp64 = run(abi.PRECISION_FP64)
-    for b in fp64:
-        b[3].close()
-        b[2].close()
-    for s in streams:
-        s.close()
-    out["mixed"] = {"value": total / t_mixed, "ms_per_step": t_mixed / steps * 1e3, "faults": f_mixed}
-    out["fp64"] = {"value": total / t_fp64, "ms_per_step": t_fp64 / steps * 1e3, "faults": f_fp64}
+    for (_, _, m, d) in mixed:
+        d.close()
+        m.close()
+    _, fp64, _, ms_f, _, f_fp64, _ = run(abi.PRECISION_FP64)
+    for (_, _, m, d) in fp64:
+        d.close()
+        m.close()
+    out = {"workload": f"chains (C4): {worlds} worlds, fixed-root hinge chains n ~ U{{2..64}} (mean "
+                       f"{float(lengths.mean()):.1f}), {len(mixed)} models in one uneven batch, PGS 30, Euler "
+                       f"dt 1 ms, timed after {settle} steps, seed {args.seed}",
+           "launch_shape": dict(cfg, launches_per_replay=launches,
+                                source="tooling_amd/tuning/chains.json (tooling_amd.autotune)")}
+    out["mixed"] = {"value": worlds * total_steps / (ms_m / 1e3), "ms_per_step": ms_m / total_steps,
+                    "faults": f_mixed}
+    out["fp64"] = {"value": worlds * total_steps / (ms_f / 1e3), "ms_per_step": ms_f / total_steps,
+                   "faults": f_fp64}
+    out["speedup_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
     out["qacc_err_mixed"] = {"max": float(max(errs)), "worlds": len(errs)}
     B = 40.0 * float(lengths.mean())
     out["hbm_frac"] = B * out["mixed"]["value"] / 1e9 / HBM_PEAK_GBS

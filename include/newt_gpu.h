@@ -312,6 +312,40 @@ newt_status newt_step(const newt_model* model, newt_data* data,
  * stream of its last step). */
 newt_status newt_data_synchronize(newt_data* data);
 
+/* ---- uneven batches: several datas in one launch ------------------------- */
+
+/* A batch of newt_datas of different models (e.g. trees of different dof
+ * counts) stepped together.  No counterpart in newt, which steps one World
+ * per call (newt/src/world.rs:1459-1537); each world's step is the same as
+ * newt_step's, bit for bit.  The datas are grouped by kernel instantiation
+ * (precision, register width for nv <= 24 / 32 / 64, extended step); each
+ * group is one launch of the persistent step over a table of jobs, one job
+ * (a run of workgroups) per data.  The datas must run the persistent step:
+ * tree models, NEWT_PRECISION_FP64 or _MIXED, no penalty joint limits. */
+typedef struct newt_batch newt_batch;
+
+typedef struct newt_batch_options {
+    int32_t order;        /* 1: datas with the most dofs first (they bound the
+                             launch); 0: as given */
+    int32_t wpb_cap[3];   /* most worlds per workgroup for register widths 24,
+                             32, 64 (0: as many as the LDS holds) */
+    int32_t use_graph;    /* 1: newt_batch_step replays a HIP graph of the
+                             group launches, captured once per step count */
+    int32_t branches;     /* 1: the groups run on parallel graph branches */
+} newt_batch_options;
+
+/* `options` NULL: order 1, no caps, use_graph 1, branches 1.  The datas stay
+ * owned by the caller and must outlive the batch. */
+newt_status newt_batch_create(newt_data* const* datas, int32_t n_datas, const newt_batch_options* options,
+                              newt_batch** out);
+/* n_steps World::steps on every world of every data of the batch, enqueued on
+ * `stream` (a hipStream_t; NULL = the default stream).  The datas' getters
+ * are ordered after it. */
+newt_status newt_batch_step(newt_batch* batch, int32_t n_steps, void* stream);
+/* Kernel launches per newt_batch_step (the number of groups). */
+newt_status newt_batch_launches(const newt_batch* batch, int32_t* n_launches);
+void newt_batch_destroy(newt_batch* batch);
+
 /* ---- step without allocation --------------------------------------------- */
 
 /* Device state of n_worlds worlds owned by the caller, lane-interleaved:

@@ -117,6 +117,15 @@ class DeviceState(C.Structure):
     ]
 
 
+class BatchOptions(C.Structure):
+    _fields_ = [
+        ("order", C.c_int32),
+        ("wpb_cap", C.c_int32 * 3),
+        ("use_graph", C.c_int32),
+        ("branches", C.c_int32),
+    ]
+
+
 # Every symbol include/newt_gpu.h declares, with (restype, argtypes).
 _P = C.c_void_p
 _I = C.c_int32
@@ -150,6 +159,10 @@ SIGNATURES = {
     "newt_data_get_pairs": (_I, [_P, _PI, _PI]),
     "newt_step": (_I, [_P, _P, _I, _P]),
     "newt_data_synchronize": (_I, [_P]),
+    "newt_batch_create": (_I, [C.POINTER(_P), _I, C.POINTER(BatchOptions), C.POINTER(_P)]),
+    "newt_batch_step": (_I, [_P, _I, _P]),
+    "newt_batch_launches": (_I, [_P, _PI]),
+    "newt_batch_destroy": (None, [_P]),
     "newt_step_noalloc": (_I, [_P, C.POINTER(DeviceState), _I, _I, _I, _I, _P]),
     "newt_noalloc_max_worlds": (_I, [_P, _I, _PI]),
 }

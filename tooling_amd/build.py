@@ -26,7 +26,8 @@ HIPCC_FLAGS = [
 ]
 
 # Per translation unit: FP contraction.
-SOURCES = {"newt_gpu.hip": ["-ffp-contract=off"], "newt_noalloc.hip": ["-ffp-contract=off"]}
+SOURCES = {"newt_gpu.hip": ["-ffp-contract=off"], "newt_noalloc.hip": ["-ffp-contract=off"],
+           "newt_batch.hip": ["-ffp-contract=off"]}
 for _unit in ("newt_team.hip", "newt_team24x.hip", "newt_team32.hip", "newt_team64f.hip", "newt_team64fx.hip",
               "newt_team64d.hip", "newt_team64dx.hip"):
     SOURCES[_unit] = ["-ffp-contract=fast"]

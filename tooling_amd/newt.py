@@ -173,3 +173,38 @@ def step(model: Model, data: Data, n_steps: int = 1, stream=None):
     """Enqueue n_steps World::step calls for every world of `data`."""
     s = C.c_void_p(stream) if stream is not None else None
     abi.check(model.lib.newt_step(model.handle, data.handle, int(n_steps), s))
+
+
+class Batch:
+    """Several Data of different models stepped together (newt_batch): one
+    launch per kernel instantiation, replayed from a HIP graph."""
+
+    def __init__(self, datas, order=1, wpb_cap=(0, 0, 0), use_graph=1, branches=1):
+        self.datas = list(datas)
+        self.lib = self.datas[0].lib
+        arr = (C.c_void_p * len(self.datas))(*[d.handle.value for d in self.datas])
+        opt = abi.BatchOptions(order=order, wpb_cap=(C.c_int32 * 3)(*wpb_cap), use_graph=use_graph,
+                               branches=branches)
+        h = C.c_void_p()
+        abi.check(self.lib.newt_batch_create(arr, len(self.datas), C.byref(opt), C.byref(h)))
+        self.handle = h
+
+    def step(self, n_steps=1, stream=None):
+        s = C.c_void_p(stream) if stream is not None else None
+        abi.check(self.lib.newt_batch_step(self.handle, int(n_steps), s))
+
+    def launches(self):
+        n = C.c_int32()
+        abi.check(self.lib.newt_batch_launches(self.handle, C.byref(n)))
+        return n.value
+
+    def close(self):
+        if self.handle:
+            self.lib.newt_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
