@@ -206,6 +206,33 @@ newt_status newt_model_add_geom(newt_model* model, const newt_geom_desc* geom,
  * model into device tables.  No further add_* calls are accepted. */
 newt_status newt_model_finalize(newt_model* model);
 
+/* Most contacts per world (a configuration cap such as "up to 512 contacts
+ * per world"; newt itself has none): a world whose step finds
+ * more is frozen with NEWT_ERR_CAPACITY at its start-of-step state while the
+ * other worlds go on.  0 (the default) = the largest count the model's
+ * candidate pairs can produce.  Before newt_model_finalize. */
+newt_status newt_model_set_contact_capacity(newt_model* model, int32_t max_contacts);
+
+/* A model from newt's JSON scene format (model::load_str,
+ * newt/src/model.rs:179-195; docs/model-format.md): strict like newt's
+ * loader (unknown fields rejected; newt_last_error names the JSON path of
+ * the offending value), numbers rounded to f32 as newt parses them.  What
+ * this build does not step (several trees, trees with bodies, actuators,
+ * equalities, tendons, meshes, heightfields, sensors, explicit pair lists,
+ * ball joints) returns NEWT_ERR_INVALID_CONFIG.  finalize = 1 also runs
+ * newt_model_finalize; 0 returns the model open for further add_* calls. */
+newt_status newt_model_from_json(const char* json, int32_t finalize, newt_model** out);
+
+/* A model's descriptions, as built (before or after finalize): counts,
+ * options, one tree's links (up to `cap`, with its link count and
+ * self-collision flag), one geom, one body. */
+newt_status newt_model_counts(const newt_model* model, int32_t* n_trees, int32_t* n_bodies, int32_t* n_geoms);
+newt_status newt_model_get_options(const newt_model* model, newt_options* out);
+newt_status newt_model_get_tree(const newt_model* model, int32_t tree, newt_link_desc* links, int32_t cap,
+                                int32_t* n_links, int32_t* self_collide);
+newt_status newt_model_get_geom(const newt_model* model, int32_t index, newt_geom_desc* out);
+newt_status newt_model_get_body(const newt_model* model, int32_t index, newt_body_desc* out);
+
 /* Sizes of one world: nq, nv, link count, geom count.  Free body i holds
  * qpos[7i..7i+7) = (position, orientation x y z w) and qvel[6i..6i+6) =
  * (linear velocity in world, angular velocity in body frame), after the
@@ -268,6 +295,31 @@ newt_status newt_data_set_worlds_async(newt_data* data, int32_t first, int32_t c
 newt_status newt_data_park_worlds(newt_data* data, int32_t first, int32_t count, void* stream);
 newt_status newt_data_get_worlds(const newt_data* data, int32_t first, int32_t count,
                                  double* qpos, double* qvel);
+
+/* Worlds with world_mask[w] != 0 (every world when world_mask is NULL) go
+ * back to the model's default state, as a fresh World (World::new +
+ * add_tree/add_body/add_geom, newt/src/world.rs:326-361, 950-979): default
+ * qpos/qvel, zero applied force, proxies at the default state, faults
+ * cleared.  Stream-ordered like newt_data_set_state_async. */
+newt_status newt_data_reset(newt_data* data, const uint8_t* world_mask, void* stream);
+
+/* World::reset_to_keyframe (newt/src/world.rs:492-519) on the masked worlds:
+ * qpos [nq] and qvel [nv] (one keyframe, every masked world) replace the
+ * state, the applied force is kept, and the proxies are refit from the new
+ * state at the next step (refresh_broadphase_for_query); faults cleared. */
+newt_status newt_data_reset_to_keyframe(newt_data* data, const uint8_t* world_mask, const double* qpos,
+                                        const double* qvel, void* stream);
+
+/* A device copy of a data's state (qpos, qvel, qacc, applied force, proxy
+ * bounds, faults, contact and pair counts): the batched form of World's
+ * Clone (newt/src/world.rs:107).  newt_data_snapshot waits for the data's
+ * last stream; restore is enqueued on `stream` and needs a snapshot of a
+ * data of the same model and world count.  The last step's contact and
+ * pair lists are not part of it. */
+typedef struct newt_snapshot newt_snapshot;
+newt_status newt_data_snapshot(const newt_data* data, newt_snapshot** out);
+newt_status newt_data_restore(newt_data* data, const newt_snapshot* snapshot, void* stream);
+void newt_snapshot_destroy(newt_snapshot* snapshot);
 
 /* Device memory allocated by this data, in bytes (admission control). */
 newt_status newt_data_device_bytes(const newt_data* data, uint64_t* bytes);

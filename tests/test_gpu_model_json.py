@@ -82,3 +82,45 @@ def test_stack_json_free_bodies(oracle, solver):
     v0 = v0 + np.random.default_rng(5).normal(0.0, 0.05, size=v0.shape)
     say(f"stack.json solver {solver}: 32 worlds x 400 steps")
     _compare(oracle, sc, q0, v0, 400, abi.PRECISION_FP64_EXACT)
+
+
+@pytest.mark.parametrize("name", ["pendulum", "quadruped18", "stack"])
+def test_c_abi_json_loader_steps_like_the_python_loader(name):
+    """newt_model_from_json (the C++ loader) and the Python loader give
+    models whose FP64_EXACT steps are bitwise equal."""
+    import ctypes as C
+    path = os.path.join(MODELS, f"{name}.json")
+    sc = model_json.load(path)
+    m = Model(sc)
+    lib = m.lib
+    h = C.c_void_p()
+    abi.check(lib.newt_model_from_json(open(path).read().encode(), 1, C.byref(h)))
+    n = 16
+    if name == "quadruped18":
+        q0, v0 = scenes.quadruped().initial_state(n, 4)
+    elif name == "stack":
+        q0, v0 = sc.initial_state(n, 0)
+        v0 = v0 + np.random.default_rng(6).normal(0.0, 0.05, size=v0.shape)
+    else:
+        rng = np.random.default_rng(4)
+        q0, v0 = rng.uniform(-1.0, 1.0, size=(n, m.nq)), rng.normal(0.0, 0.5, size=(n, m.nv))
+    say(f"C-ABI JSON {name}: {n} worlds x 200 steps")
+    d1 = Data(m, n, abi.PRECISION_FP64_EXACT)
+    d1.set_state(q0, v0)
+    step(m, d1, 200)
+    d2 = C.c_void_p()
+    abi.check(lib.newt_data_create(h, n, abi.PRECISION_FP64_EXACT, C.byref(d2)))
+    qa = np.ascontiguousarray(q0, dtype=np.float64)
+    va = np.ascontiguousarray(v0, dtype=np.float64)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    abi.check(lib.newt_data_set_state(d2, dp(qa), dp(va)))
+    abi.check(lib.newt_step(h, d2, 200, None))
+    q2, v2 = np.zeros_like(qa), np.zeros_like(va)
+    abi.check(lib.newt_data_get_state(d2, dp(q2), dp(v2)))
+    q1, v1 = d1.state()
+    np.testing.assert_array_equal(q1, q2)
+    np.testing.assert_array_equal(v1, v2)
+    lib.newt_data_destroy(d2)
+    lib.newt_model_destroy(h)
+    d1.close()
+    m.close()

@@ -159,6 +159,18 @@ SIGNATURES = {
     "newt_data_get_pairs": (_I, [_P, _PI, _PI]),
     "newt_step": (_I, [_P, _P, _I, _P]),
     "newt_data_synchronize": (_I, [_P]),
+    "newt_model_set_contact_capacity": (_I, [_P, _I]),
+    "newt_model_from_json": (_I, [C.c_char_p, _I, C.POINTER(_P)]),
+    "newt_model_counts": (_I, [_P, _PI, _PI, _PI]),
+    "newt_model_get_options": (_I, [_P, C.POINTER(Options)]),
+    "newt_model_get_tree": (_I, [_P, _I, C.POINTER(LinkDesc), _I, _PI, _PI]),
+    "newt_model_get_geom": (_I, [_P, _I, C.POINTER(GeomDesc)]),
+    "newt_model_get_body": (_I, [_P, _I, C.POINTER(BodyDesc)]),
+    "newt_data_reset": (_I, [_P, C.POINTER(C.c_uint8), _P]),
+    "newt_data_reset_to_keyframe": (_I, [_P, C.POINTER(C.c_uint8), _PD, _PD, _P]),
+    "newt_data_snapshot": (_I, [_P, C.POINTER(_P)]),
+    "newt_data_restore": (_I, [_P, _P, _P]),
+    "newt_snapshot_destroy": (None, [_P]),
     "newt_batch_create": (_I, [C.POINTER(_P), _I, C.POINTER(BatchOptions), C.POINTER(_P)]),
     "newt_batch_step": (_I, [_P, _I, _P]),
     "newt_batch_launches": (_I, [_P, _PI]),

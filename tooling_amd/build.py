@@ -27,22 +27,51 @@ HIPCC_FLAGS = [
 
 # Per translation unit: FP contraction.
 SOURCES = {"newt_gpu.hip": ["-ffp-contract=off"], "newt_noalloc.hip": ["-ffp-contract=off"],
-           "newt_batch.hip": ["-ffp-contract=off"]}
+           "newt_batch.hip": ["-ffp-contract=off"], "newt_model_json.hip": ["-ffp-contract=off"]}
 for _unit in ("newt_team.hip", "newt_team24x.hip", "newt_team32.hip", "newt_team64f.hip", "newt_team64fx.hip",
               "newt_team64d.hip", "newt_team64dx.hip"):
     SOURCES[_unit] = ["-ffp-contract=fast"]
 
 
+def _includes(path, seen=None):
+    """Every file `path` pulls in through quoted #include lines, recursively."""
+    seen = set() if seen is None else seen
+    base = os.path.dirname(path)
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return seen
+    for line in lines:
+        line = line.strip()
+        if line.startswith("#include \""):
+            dep = os.path.normpath(os.path.join(base, line.split('"')[1]))
+            if dep not in seen and os.path.exists(dep):
+                seen.add(dep)
+                _includes(dep, seen)
+    return seen
+
+
+def _stale(src_path, obj):
+    """The object is missing or older than its source or any header it includes."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(f) > t for f in [src_path, *_includes(src_path)])
+
+
 def build(verbose: bool = False, only=None) -> str:
     """Compile each translation unit (in parallel) and link the library.
-    ``only``: names of the units to recompile; the others' objects are reused."""
+    A unit is recompiled when its object is older than the unit or any header
+    it includes; ``only`` names units to recompile in any case (``only`` given:
+    the other up-to-date objects are reused; not given: every unit is
+    recompiled)."""
     os.makedirs(OUT_DIR, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
     jobs = []
     for src, extra in SOURCES.items():
         obj = os.path.join(OUT_DIR, src.replace(".hip", ".o"))
-        if only and src not in only and os.path.exists(obj):
+        if only and src not in only and not _stale(os.path.join(CSRC, src), obj):
             jobs.append((src, obj, None))
             continue
         cmd = [hipcc, *compile_flags, *extra, "-c", "-o", obj, os.path.join(CSRC, src)]

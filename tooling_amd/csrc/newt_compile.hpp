@@ -30,6 +30,7 @@ struct HostModel {
     std::vector<double> v_default;  // zero for trees; the bodies' initial velocities
     std::vector<newt_body_desc> bodies;
     std::vector<BodyT> hbodies;
+    int contact_cap = 0;  // most contacts per world (0: the sum of the pairs' maxima)
 };
 
 inline newt_status cfail(std::string* err, newt_status s, const std::string& msg) {

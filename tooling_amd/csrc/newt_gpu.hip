@@ -218,8 +218,10 @@ hipError_t d2h(const newt_data* D, void* dst, const void* src, size_t bytes) {
 hipError_t stage(newt_data* D) {
     if (D->staging) return hipEventSynchronize(D->staged);
     const ModelT& m = D->model->dev;
+    // Every world's qpos, qvel and bounds; or a world mask and one state.
     hipError_t e = hipHostMalloc((void**)&D->staging,
-                                 std::max<size_t>((size_t)D->W * (m.nq + m.nv + 6 * m.n_geoms) * 8, 8),
+                                 (size_t)D->W * (m.nq + m.nv + 6 * m.n_geoms) * 8 + D->W + 256 +
+                                     (size_t)(m.nq + m.nv + 6 * m.n_geoms) * 8,
                                  hipHostMallocDefault);
     if (e != hipSuccess) return e;
     return hipEventCreateWithFlags(&D->staged, hipEventDisableTiming);
@@ -273,6 +275,10 @@ newt_status newt_model_finalize(newt_model* model) {
     newt_status s = compile_model(&model->h, &err);
     if (s != NEWT_OK) return fail(s, err);
     HostModel& H = model->h;
+    if (H.contact_cap > 0 && H.contact_cap < H.dev.cap_c) {
+        H.dev.cap_c = H.contact_cap;
+        H.dev.cap_r = 4 * H.contact_cap;
+    }
     HIP_TRY(hipMalloc(&model->d_links, sizeof(LinkT) * std::max<size_t>(1, H.links.size())));
     HIP_TRY(hipMalloc(&model->d_geoms, sizeof(GeomT) * std::max<size_t>(1, H.hgeoms.size())));
     HIP_TRY(hipMalloc(&model->d_pairs, sizeof(PairT) * std::max<size_t>(1, H.pairs.size())));
@@ -338,6 +344,13 @@ newt_status newt_model_finalize(newt_model* model) {
         model->cache = C;
     }
     model->finalized = true;
+    return NEWT_OK;
+}
+
+newt_status newt_model_set_contact_capacity(newt_model* model, int32_t max_contacts) {
+    if (!model || max_contacts < 0) return fail(NEWT_ERR_INVALID_ARG, "bad capacity arguments");
+    if (model->finalized) return fail(NEWT_ERR_INVALID_ARG, "model is finalized");
+    model->h.contact_cap = max_contacts;
     return NEWT_OK;
 }
 
@@ -706,6 +719,150 @@ newt_status newt_data_get_pairs(const newt_data* D, int32_t* counts, int32_t* pa
             pairs[(w * cap + k) * 2 + 1] = k < counts[w] ? M->h.pairs[idx].b : -1;
         }
     return NEWT_OK;
+}
+
+// ---- reset, keyframes, snapshots -------------------------------------------
+
+}  // extern "C"
+
+namespace {
+
+// Worlds with mask[w] != 0 take the state in `src` ([nq + nv + 6 n_geoms]):
+// qpos, qvel and, with `bounds`, the proxy bounds; with `clear_force` their
+// applied force and qacc are zeroed.  Faults and counts are cleared.  The
+// mask and source sit in pinned host memory, read by the kernel directly.
+__global__ void k_reset(DataT d, int nq, int nv, int ng, const unsigned char* mask, const double* src, int bounds,
+                        int clear_force) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= d.W || !mask[w]) return;
+    const size_t W = (size_t)d.W;
+    for (int k = 0; k < nq; ++k) d.qpos[k * W + w] = src[k];
+    for (int k = 0; k < nv; ++k) {
+        d.qvel[k * W + w] = src[nq + k];
+        if (clear_force) {
+            d.qfrc[k * W + w] = 0.0;
+            d.qacc[k * W + w] = 0.0;
+        }
+    }
+    if (bounds)
+        for (int k = 0; k < 6 * ng; ++k) d.fat[k * W + w] = src[nq + nv + k];
+    d.fault[w] = 0;
+    d.ncontact[w] = 0;
+    d.npair[w] = 0;
+}
+
+newt_status reset_worlds(newt_data* D, const uint8_t* mask, const double* q, const double* v, bool defaults,
+                         void* stream) {
+    const ModelT& m = D->model->dev;
+    HIP_TRY(stage(D));
+    const size_t W = (size_t)D->W;
+    unsigned char* hm = reinterpret_cast<unsigned char*>(D->staging);
+    double* src = D->staging + (W + 7) / 8;
+    for (size_t w = 0; w < W; ++w) hm[w] = mask ? (mask[w] != 0) : 1;
+    for (int k = 0; k < m.nq; ++k) src[k] = q[k];
+    for (int k = 0; k < m.nv; ++k) src[m.nq + k] = v[k];
+    if (defaults)
+        for (int k = 0; k < 6 * m.n_geoms; ++k) src[m.nq + m.nv + k] = D->fat_default[k];
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_reset, dim3((D->W + 63) / 64), dim3(64), 0, s, D->d, m.nq, m.nv, m.n_geoms, hm, src,
+                       defaults ? 1 : 0, defaults ? 1 : 0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(D->staged, s));
+    D->last_stream = s;
+    return NEWT_OK;
+}
+
+}  // namespace
+
+struct newt_snapshot {
+    const newt_model* model = nullptr;
+    int W = 0;
+    DataT d{};
+    uint64_t bytes = 0;
+};
+
+extern "C" {
+
+newt_status newt_data_reset(newt_data* D, const uint8_t* world_mask, void* stream) {
+    if (!D) return fail(NEWT_ERR_INVALID_ARG, "null data");
+    const HostModel& H = D->model->h;
+    return reset_worlds(D, world_mask, H.q_default.data(), H.v_default.data(), true, stream);
+}
+
+newt_status newt_data_reset_to_keyframe(newt_data* D, const uint8_t* world_mask, const double* qpos,
+                                        const double* qvel, void* stream) {
+    if (!D || !qpos || !qvel) return fail(NEWT_ERR_INVALID_ARG, "null argument");
+    return reset_worlds(D, world_mask, qpos, qvel, false, stream);
+}
+
+newt_status newt_data_snapshot(const newt_data* D, newt_snapshot** out) {
+    if (!D || !out) return fail(NEWT_ERR_INVALID_ARG, "null argument");
+    const ModelT& m = D->model->dev;
+    newt_snapshot* S = new newt_snapshot();
+    S->model = D->model;
+    S->W = D->W;
+    S->d.W = D->W;
+    const size_t W = (size_t)D->W;
+    auto dup = [&](void** dst, const void* src, size_t bytes) -> hipError_t {
+        bytes = std::max<size_t>(bytes, 8);
+        hipError_t e = hipMalloc(dst, bytes);
+        if (e != hipSuccess) return e;
+        S->bytes += bytes;
+        return hipMemcpyAsync(*dst, src, bytes, hipMemcpyDeviceToDevice, D->last_stream);
+    };
+    struct Item {
+        void** dst;
+        const void* src;
+        size_t bytes;
+    };
+    const Item items[] = {{(void**)&S->d.qpos, D->d.qpos, W * m.nq * 8},
+                          {(void**)&S->d.qvel, D->d.qvel, W * m.nv * 8},
+                          {(void**)&S->d.qacc, D->d.qacc, W * m.nv * 8},
+                          {(void**)&S->d.qfrc, D->d.qfrc, W * m.nv * 8},
+                          {(void**)&S->d.fat, D->d.fat, W * m.n_geoms * 6 * 8},
+                          {(void**)&S->d.fault, D->d.fault, W * 4},
+                          {(void**)&S->d.ncontact, D->d.ncontact, W * 4},
+                          {(void**)&S->d.npair, D->d.npair, W * 4}};
+    hipError_t e = hipSuccess;
+    for (const Item& it : items)
+        if (e == hipSuccess) e = dup(it.dst, it.src, it.bytes);
+    if (e == hipSuccess) e = hipStreamSynchronize(D->last_stream);
+    if (e != hipSuccess) {
+        newt_snapshot_destroy(S);
+        return fail(NEWT_ERR_DEVICE, std::string("snapshot: ") + hipGetErrorString(e));
+    }
+    *out = S;
+    return NEWT_OK;
+}
+
+newt_status newt_data_restore(newt_data* D, const newt_snapshot* S, void* stream) {
+    if (!D || !S) return fail(NEWT_ERR_INVALID_ARG, "null argument");
+    if (S->model != D->model || S->W != D->W)
+        return fail(NEWT_ERR_INVALID_ARG, "the snapshot was taken of a data of another model or size");
+    const ModelT& m = D->model->dev;
+    const size_t W = (size_t)D->W;
+    hipStream_t s = (hipStream_t)stream;
+    auto cp = [&](void* dst, const void* src, size_t bytes) {
+        return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s) : hipSuccess;
+    };
+    HIP_TRY(cp(D->d.qpos, S->d.qpos, W * m.nq * 8));
+    HIP_TRY(cp(D->d.qvel, S->d.qvel, W * m.nv * 8));
+    HIP_TRY(cp(D->d.qacc, S->d.qacc, W * m.nv * 8));
+    HIP_TRY(cp(D->d.qfrc, S->d.qfrc, W * m.nv * 8));
+    HIP_TRY(cp(D->d.fat, S->d.fat, W * m.n_geoms * 6 * 8));
+    HIP_TRY(cp(D->d.fault, S->d.fault, W * 4));
+    HIP_TRY(cp(D->d.ncontact, S->d.ncontact, W * 4));
+    HIP_TRY(cp(D->d.npair, S->d.npair, W * 4));
+    D->last_stream = s;
+    return NEWT_OK;
+}
+
+void newt_snapshot_destroy(newt_snapshot* S) {
+    if (!S) return;
+    for (void* p : {(void*)S->d.qpos, (void*)S->d.qvel, (void*)S->d.qacc, (void*)S->d.qfrc, (void*)S->d.fat,
+                    (void*)S->d.fault, (void*)S->d.ncontact, (void*)S->d.npair})
+        (void)hipFree(p);
+    delete S;
 }
 
 newt_status newt_step(const newt_model* model, newt_data* data, int32_t n_steps, void* stream) {

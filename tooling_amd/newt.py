@@ -18,7 +18,10 @@ from .scenes import Scene
 
 
 class Model:
-    def __init__(self, scene: Scene):
+    """`contact_capacity`: most contacts per world (newt_model_set_contact_capacity);
+    a world that finds more freezes with abi.ERR_CAPACITY."""
+
+    def __init__(self, scene: Scene, contact_capacity: int | None = None):
         self.lib = abi.load_library()
         self.scene = scene
         h = C.c_void_p()
@@ -32,6 +35,8 @@ class Model:
             abi.check(self.lib.newt_model_add_body(h, C.byref(b), None))
         for g in scene.geom_structs():
             abi.check(self.lib.newt_model_add_geom(h, C.byref(g), None))
+        if contact_capacity is not None:
+            abi.check(self.lib.newt_model_set_contact_capacity(h, int(contact_capacity)))
         abi.check(self.lib.newt_model_finalize(h))
         nq, nv, nl, ng = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
         abi.check(self.lib.newt_model_dims(h, C.byref(nq), C.byref(nv), C.byref(nl), C.byref(ng)))
@@ -157,9 +162,58 @@ class Data:
     def synchronize(self):
         abi.check(self.lib.newt_data_synchronize(self.handle))
 
+    def _mask(self, mask):
+        if mask is None:
+            return None, None
+        m = np.ascontiguousarray(np.asarray(mask, dtype=bool).astype(np.uint8).reshape(self.n_worlds))
+        return m, m.ctypes.data_as(C.POINTER(C.c_uint8))
+
+    def reset(self, mask=None, stream=None):
+        """Masked worlds (all when None) back to the model's default state."""
+        keep, mp = self._mask(mask)
+        abi.check(self.lib.newt_data_reset(self.handle, mp, C.c_void_p(stream) if stream else None))
+        self.synchronize()
+
+    def reset_to_keyframe(self, qpos, qvel, mask=None, stream=None):
+        """Masked worlds take the keyframe (qpos [nq], qvel [nv])."""
+        keep, mp = self._mask(mask)
+        q = np.ascontiguousarray(qpos, dtype=np.float64).reshape(self.model.nq)
+        v = np.ascontiguousarray(qvel, dtype=np.float64).reshape(self.model.nv)
+        abi.check(self.lib.newt_data_reset_to_keyframe(self.handle, mp, self._dp(q), self._dp(v),
+                                                       C.c_void_p(stream) if stream else None))
+        self.synchronize()
+
+    def snapshot(self):
+        return Snapshot(self)
+
+    def restore(self, snap, stream=None):
+        abi.check(self.lib.newt_data_restore(self.handle, snap.handle, C.c_void_p(stream) if stream else None))
+        self.synchronize()
+
     def close(self):
         if self.handle:
             self.lib.newt_data_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Snapshot:
+    """A device copy of a Data's state (newt_data_snapshot)."""
+
+    def __init__(self, data: Data):
+        self.lib = data.lib
+        h = C.c_void_p()
+        abi.check(self.lib.newt_data_snapshot(data.handle, C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self.lib.newt_snapshot_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
