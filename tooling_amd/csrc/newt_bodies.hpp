@@ -1186,21 +1186,53 @@ struct BodyWave {
 
     // ---------------- Newton (solver.rs:1295-1351, newton.rs:275-556) ------
     // Sequential sum a_0 b_0 + a_1 b_1 + ... from -0.0 (Iterator::sum), on
-    // every lane (same result everywhere).
+    // every lane (same result everywhere).  Each lane forms one product of a
+    // chunk of 64 (all loads in flight at once); the additions then run in
+    // index order on the broadcast products.
     __device__ __forceinline__ double dot_seq(const double* a, const double* b, int n) const {
         double s = -0.0;
-        for (int i = 0; i < n; ++i) s = s + a[i] * b[i];
+        for (int i0 = 0; i0 < n; i0 += 64) {
+            const int i = i0 + lane;
+            const double p = i < n ? a[i] * b[i] : 0.0;
+            const int cnt = n - i0 < 64 ? n - i0 : 64;
+            if (cnt == 64) {
+#pragma unroll
+                for (int u = 0; u < 64; ++u) s = s + readlane_f64(p, u);
+            } else {
+                for (int u = 0; u < cnt; ++u) s = s + readlane_f64(p, u);
+            }
+        }
         return s;
     }
     // out = H v, one lane per row, each row summed in column order from 0.0;
-    // a lane carries four rows at once (independent sums, same order each).
+    // a lane carries four rows at once (independent sums, same order each),
+    // and eight columns' loads are issued before their products are summed.
     __device__ __forceinline__ void mat_vec(const double* H, const double* v, double* out, int n) const {
         for (int i0 = lane; i0 < n; i0 += 256) {
             const int i1 = i0 + 64 < n ? i0 + 64 : i0, i2 = i0 + 128 < n ? i0 + 128 : i0, i3 = i0 + 192 < n ? i0 + 192 : i0;
             const double *h0 = H + (size_t)i0 * n, *h1 = H + (size_t)i1 * n, *h2 = H + (size_t)i2 * n,
                          *h3 = H + (size_t)i3 * n;
             double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-            for (int j = 0; j < n; ++j) {
+            int j = 0;
+            for (; j + 8 <= n; j += 8) {
+                double vv[8], a0[8], a1[8], a2[8], a3[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    vv[u] = v[j + u];
+                    a0[u] = h0[j + u];
+                    a1[u] = h1[j + u];
+                    a2[u] = h2[j + u];
+                    a3[u] = h3[j + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    s0 += a0[u] * vv[u];
+                    s1 += a1[u] * vv[u];
+                    s2 += a2[u] * vv[u];
+                    s3 += a3[u] * vv[u];
+                }
+            }
+            for (; j < n; ++j) {
                 const double vj = v[j];
                 s0 += h0[j] * vj;
                 s1 += h1[j] * vj;
@@ -1278,60 +1310,96 @@ struct BodyWave {
             x[i] = 0.0;
         }
         gsync();
-        // Cholesky-Banachiewicz (dynamics.rs:602-624) column by column: every
-        // entry's sum runs over k in increasing order, as in the row-wise
-        // loop.  L is kept row-major (Lf, read by the factorization eight
-        // entries at a time so the loads overlap) and column-major (Lt, read
-        // by the substitutions).
-        // The diagonal sums H[i][i] - L[i][0]^2 - L[i][1]^2 - ... are carried
-        // in `pt`, each term subtracted right after its L[i][k] is computed
-        // (k increasing, the row-wise order).
+        // Cholesky-Banachiewicz (dynamics.rs:602-624), by panels of eight
+        // columns.  Every entry L[i][j] receives H[i][j] minus L[i][k] L[j][k]
+        // for k = 0, 1, ..., j-1 in that order, as in the row-wise loop: first
+        // the terms k < j0 (left of the panel, eight columns accumulated at
+        // once per row, so each L[i][k] load serves eight entries), then the
+        // in-panel terms k = j0 .. j-1, column by column.  The diagonal sums
+        // H[i][i] - L[i][0]^2 - L[i][1]^2 - ... are carried in `pt`, each term
+        // subtracted right after its L[i][k] is computed (k increasing).  L is
+        // kept column-major (Lt, read by the factorization and the forward
+        // substitution) and row-major (Lf, read by the back substitution).
         for (int i = lane; i < n; i += 64) pt[i] = H[(size_t)i * n + i];
         gsync();
-        for (int j = 0; j < n; ++j) {
-            const double* lj = Lf + (size_t)j * n;
-            const double sum = pt[j];
-            if (sum <= 0.0) return 4;  // NEWT_ERR_NEWTON_FAIL: Hessian not positive definite
-            const double ljj = __builtin_sqrt(sum);
-            if (lane == 0) {
-                Lf[(size_t)j * n + j] = ljj;
-                Lt[(size_t)j * n + j] = ljj;
-            }
-            for (int i0 = j + 1 + lane; i0 < n; i0 += 128) {
-                const bool v1 = i0 + 64 < n;
-                const int i1 = v1 ? i0 + 64 : i0;
-                const double *r0 = Lf + (size_t)i0 * n, *r1 = Lf + (size_t)i1 * n;
-                double s0 = H[(size_t)i0 * n + j], s1 = H[(size_t)i1 * n + j];
+        for (int j0 = 0; j0 < n; j0 += 8) {
+            const int pw = n - j0 < 8 ? n - j0 : 8;  // panel width
+            // Rows j0 .. n-1, one per lane, in chunks of 64.
+            for (int r0 = j0; r0 < n; r0 += 64) {
+                const int i = r0 + lane;
+                const bool on = i < n;
+                const int ii = on ? i : n - 1;
+                double acc[8];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) acc[c] = (c < pw && on) ? H[(size_t)ii * n + j0 + c] : 0.0;
                 int k = 0;
-                for (; k + 8 <= j; k += 8) {
-                    double b[8], c0[8], c1[8];
+                for (; k + 4 <= j0; k += 4) {
+                    double li[4], lj[4][8];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        b[u] = lj[k + u];
-                        c0[u] = r0[k + u];
-                        c1[u] = r1[k + u];
+                    for (int u = 0; u < 4; ++u) {
+                        li[u] = Lt[(size_t)(k + u) * n + ii];
+#pragma unroll
+                        for (int c = 0; c < 8; ++c) lj[u][c] = c < pw ? Lt[(size_t)(k + u) * n + j0 + c] : 0.0;
                     }
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        s0 -= c0[u] * b[u];
-                        s1 -= c1[u] * b[u];
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int c = 0; c < 8; ++c) acc[c] -= li[u] * lj[u][c];
+                }
+                for (; k < j0; ++k) {
+                    const double li = Lt[(size_t)k * n + ii];
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) acc[c] -= li * (c < pw ? Lt[(size_t)k * n + j0 + c] : 0.0);
+                }
+                // In-panel columns j = j0 + c, in order.  Rows of the first
+                // chunk include the panel's own rows, whose entries are the
+                // pivots' and the later columns' multipliers.  A row's
+                // diagonal sum stays in a register while the panel runs.
+                double pr = on ? pt[ii] : 0.0;
+                double q[8];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    q[c] = 0.0;
+                    if (c < pw) {
+                        const int j = j0 + c;
+                        double sum = acc[c];
+#pragma unroll
+                        for (int c2 = 0; c2 < 8; ++c2)
+                            if (c2 < c) {
+                                // L[j][j0 + c2]: computed earlier in this panel (row j is
+                                // in the first chunk: its lane's q[c2], else from Lt).
+                                const double ljk = r0 == j0 ? readlane_f64(q[c2], j - j0)
+                                                            : Lt[(size_t)(j0 + c2) * n + j];
+                                sum -= q[c2] * ljk;
+                            }
+                        // Pivot L[j][j] = sqrt(pt[j]) once every k < j is subtracted.
+                        double ljj;
+                        if (r0 == j0) {
+                            const double ptj = readlane_f64(pr, j - j0);
+                            if (ptj <= 0.0) return 4;  // NEWT_ERR_NEWTON_FAIL: Hessian not positive definite
+                            ljj = __builtin_sqrt(ptj);
+                        } else {
+                            ljj = Lt[(size_t)j * n + j];
+                        }
+                        if (on && i > j) {
+                            q[c] = sum / ljj;
+                            pr -= q[c] * q[c];
+                        } else if (on && i == j) {
+                            q[c] = ljj;
+                        }
                     }
                 }
-                for (; k < j; ++k) {
-                    s0 -= r0[k] * lj[k];
-                    s1 -= r1[k] * lj[k];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const int j = j0 + c;
+                    if (c < pw && on && i >= j) {
+                        Lt[(size_t)j * n + i] = q[c];
+                        Lf[(size_t)i * n + j] = q[c];
+                    }
                 }
-                const double q0 = s0 / ljj, q1 = s1 / ljj;
-                Lf[(size_t)i0 * n + j] = q0;
-                Lt[(size_t)j * n + i0] = q0;
-                pt[i0] -= q0 * q0;
-                if (v1) {
-                    Lf[(size_t)i1 * n + j] = q1;
-                    Lt[(size_t)j * n + i1] = q1;
-                    pt[i1] -= q1 * q1;
-                }
+                if (on) pt[i] = pr;
+                gsync();
             }
-            gsync();
         }
         const uint32_t max_it = m.iterations < 1 ? 1u : (uint32_t)m.iterations;
         double cur = cost(H, lin, x, hv, n);
@@ -1343,18 +1411,38 @@ struct BodyWave {
                 y[i] = -g[i];
             }
             gsync();
-            // forward: y[i] -= L[i][k] y[k] for k = 0, 1, ... (the row sum's order)
-            for (int k = 0; k < n; ++k) {
-                const int own = k & 63;
-                double yk = 0.0;
-                if (lane == own) {
-                    yk = y[k] / Lt[(size_t)k * n + k];
-                    y[k] = yk;
+            // forward: y[i] -= L[i][k] y[k] for k = 0, 1, ... (the row sum's
+            // order), y[64 c + lane] held in register yr[c]; column k of the
+            // factor is one coalesced load, the pivot's owner divides and its
+            // value is broadcast.
+            {
+                double yr[16];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    const int i = 64 * c + lane;
+                    yr[c] = i < n ? y[i] : 0.0;
                 }
-                yk = __shfl(yk, own, 64);
-                const double* ltk = Lt + (size_t)k * n;
-                for (int i = k + 1 + lane - ((k + 1) & 63) + ((((k + 1) & 63) > lane) ? 64 : 0); i < n; i += 64)
-                    y[i] -= ltk[i] * yk;
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    if (64 * c >= n) break;
+                    const int kend = n - 64 * c < 64 ? n - 64 * c : 64;
+                    for (int kk = 0; kk < kend; ++kk) {
+                        const int k = 64 * c + kk;
+                        const double* ltk = Lt + (size_t)k * n;
+                        const double yk = readlane_f64(yr[c] / ltk[k], kk);
+                        if (lane == kk) yr[c] = yk;
+#pragma unroll
+                        for (int c2 = c; c2 < 16; ++c2) {
+                            const int i = 64 * c2 + lane;
+                            if (i > k && i < n) yr[c2] -= ltk[i] * yk;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    const int i = 64 * c + lane;
+                    if (i < n) y[i] = yr[c];
+                }
             }
             gsync();
             // backward: x[i] = (y[i] - sum_{k>i} L[k][i] x[k]) / L[i][i], the sum
