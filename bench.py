@@ -520,6 +520,8 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
     ap.add_argument("--workload", choices=["all", "quadruped", "pendulum", "boxes", "chains", "serve"],
                     default="all")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB device buffer 5 times (the reference kernel for the HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: worlds are independent and shard by world index with no "
@@ -527,6 +529,16 @@ def main():
         sys.exit(2)
     if args.steps % args.chunk:
         args.chunk = 1
+    if args.copy_calibration:
+        import torch  # a plain device copy kernel of known size
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_mib": args.copy_calibration, "copies": 5}))
+        return
     from tooling_amd import hip
     if hip.device_count() < 1:
         print("no GPU visible", file=sys.stderr)

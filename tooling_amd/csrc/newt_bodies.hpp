@@ -843,6 +843,9 @@ struct BodyWave {
     mutable double* rows;
     mutable double* blk;
     mutable int* bbi;
+    // 1: newt's operation order everywhere (FP64_EXACT); 0: the Newton back
+    // substitution runs column by column (the drop-in's 1e-9 contract).
+    int exact;
 
     __device__ __forceinline__ double& cg(int k, int j) const { return dws[(size_t)(L.cgeo + 9 * k + j) * d.W + world]; }
     __device__ __forceinline__ int& cid(int k, int j) const { return iws[(size_t)(L.cids + 3 * k + j) * d.W + world]; }
@@ -1310,61 +1313,63 @@ struct BodyWave {
             x[i] = 0.0;
         }
         gsync();
-        // Cholesky-Banachiewicz (dynamics.rs:602-624), by panels of eight
+        // Cholesky-Banachiewicz (dynamics.rs:602-624), by panels of sixteen
         // columns.  Every entry L[i][j] receives H[i][j] minus L[i][k] L[j][k]
         // for k = 0, 1, ..., j-1 in that order, as in the row-wise loop: first
-        // the terms k < j0 (left of the panel, eight columns accumulated at
-        // once per row, so each L[i][k] load serves eight entries), then the
-        // in-panel terms k = j0 .. j-1, column by column.  The diagonal sums
+        // the terms k < j0 (left of the panel, sixteen columns accumulated at
+        // once per row, so each L[i][k] load serves sixteen entries, four k
+        // at a time with every load in flight together), then the in-panel
+        // terms k = j0 .. j-1, column by column.  The diagonal sums
         // H[i][i] - L[i][0]^2 - L[i][1]^2 - ... are carried in `pt`, each term
         // subtracted right after its L[i][k] is computed (k increasing).  L is
         // kept column-major (Lt, read by the factorization and the forward
         // substitution) and row-major (Lf, read by the back substitution).
+        constexpr int P = 16;
         for (int i = lane; i < n; i += 64) pt[i] = H[(size_t)i * n + i];
         gsync();
-        for (int j0 = 0; j0 < n; j0 += 8) {
-            const int pw = n - j0 < 8 ? n - j0 : 8;  // panel width
+        for (int j0 = 0; j0 < n; j0 += P) {
+            const int pw = n - j0 < P ? n - j0 : P;  // panel width
             // Rows j0 .. n-1, one per lane, in chunks of 64.
             for (int r0 = j0; r0 < n; r0 += 64) {
                 const int i = r0 + lane;
                 const bool on = i < n;
                 const int ii = on ? i : n - 1;
-                double acc[8];
+                double acc[P];
 #pragma unroll
-                for (int c = 0; c < 8; ++c) acc[c] = (c < pw && on) ? H[(size_t)ii * n + j0 + c] : 0.0;
+                for (int c = 0; c < P; ++c) acc[c] = (c < pw && on) ? H[(size_t)ii * n + j0 + c] : 0.0;
                 int k = 0;
                 for (; k + 4 <= j0; k += 4) {
-                    double li[4], lj[4][8];
+                    double li[4], lj[4][P];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         li[u] = Lt[(size_t)(k + u) * n + ii];
 #pragma unroll
-                        for (int c = 0; c < 8; ++c) lj[u][c] = c < pw ? Lt[(size_t)(k + u) * n + j0 + c] : 0.0;
+                        for (int c = 0; c < P; ++c) lj[u][c] = c < pw ? Lt[(size_t)(k + u) * n + j0 + c] : 0.0;
                     }
 #pragma unroll
                     for (int u = 0; u < 4; ++u)
 #pragma unroll
-                        for (int c = 0; c < 8; ++c) acc[c] -= li[u] * lj[u][c];
+                        for (int c = 0; c < P; ++c) acc[c] -= li[u] * lj[u][c];
                 }
                 for (; k < j0; ++k) {
                     const double li = Lt[(size_t)k * n + ii];
 #pragma unroll
-                    for (int c = 0; c < 8; ++c) acc[c] -= li * (c < pw ? Lt[(size_t)k * n + j0 + c] : 0.0);
+                    for (int c = 0; c < P; ++c) acc[c] -= li * (c < pw ? Lt[(size_t)k * n + j0 + c] : 0.0);
                 }
                 // In-panel columns j = j0 + c, in order.  Rows of the first
                 // chunk include the panel's own rows, whose entries are the
                 // pivots' and the later columns' multipliers.  A row's
                 // diagonal sum stays in a register while the panel runs.
                 double pr = on ? pt[ii] : 0.0;
-                double q[8];
+                double q[P];
 #pragma unroll
-                for (int c = 0; c < 8; ++c) {
+                for (int c = 0; c < P; ++c) {
                     q[c] = 0.0;
                     if (c < pw) {
                         const int j = j0 + c;
                         double sum = acc[c];
 #pragma unroll
-                        for (int c2 = 0; c2 < 8; ++c2)
+                        for (int c2 = 0; c2 < P; ++c2)
                             if (c2 < c) {
                                 // L[j][j0 + c2]: computed earlier in this panel (row j is
                                 // in the first chunk: its lane's q[c2], else from Lt).
@@ -1390,7 +1395,7 @@ struct BodyWave {
                     }
                 }
 #pragma unroll
-                for (int c = 0; c < 8; ++c) {
+                for (int c = 0; c < P; ++c) {
                     const int j = j0 + c;
                     if (c < pw && on && i >= j) {
                         Lt[(size_t)j * n + i] = q[c];
@@ -1413,30 +1418,43 @@ struct BodyWave {
             gsync();
             // forward: y[i] -= L[i][k] y[k] for k = 0, 1, ... (the row sum's
             // order), y[64 c + lane] held in register yr[c]; column k of the
-            // factor is one coalesced load, the pivot's owner divides and its
-            // value is broadcast.
+            // factor is one coalesced load, issued while column k - 1 is
+            // applied; the pivot's owner divides and its value is broadcast.
             {
-                double yr[16];
+                double yr[16], lc[16], ln[16];
 #pragma unroll
                 for (int c = 0; c < 16; ++c) {
                     const int i = 64 * c + lane;
                     yr[c] = i < n ? y[i] : 0.0;
+                    lc[c] = i < n ? Lt[i] : 0.0;  // column 0
                 }
+                double piv = Lt[0];
+                for (int k = 0; k < n; ++k) {
+                    const int kc = k >> 6, kk = k & 63;
+                    double pn = 1.0;
+                    if (k + 1 < n) {
+                        const double* nxt = Lt + (size_t)(k + 1) * n;
+                        pn = nxt[k + 1];
 #pragma unroll
-                for (int c = 0; c < 16; ++c) {
-                    if (64 * c >= n) break;
-                    const int kend = n - 64 * c < 64 ? n - 64 * c : 64;
-                    for (int kk = 0; kk < kend; ++kk) {
-                        const int k = 64 * c + kk;
-                        const double* ltk = Lt + (size_t)k * n;
-                        const double yk = readlane_f64(yr[c] / ltk[k], kk);
-                        if (lane == kk) yr[c] = yk;
-#pragma unroll
-                        for (int c2 = c; c2 < 16; ++c2) {
-                            const int i = 64 * c2 + lane;
-                            if (i > k && i < n) yr[c2] -= ltk[i] * yk;
+                        for (int c = 0; c < 16; ++c) {
+                            const int i = 64 * c + lane;
+                            ln[c] = (i > k + 1 && i < n) ? nxt[i] : 0.0;
                         }
                     }
+                    double own = 0.0;
+#pragma unroll
+                    for (int c = 0; c < 16; ++c)
+                        if (c == kc) own = yr[c];
+                    const double yk = readlane_f64(own / piv, kk);
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) {
+                        const int i = 64 * c + lane;
+                        if (i == k) yr[c] = yk;
+                        else if (i > k && i < n) yr[c] -= lc[c] * yk;
+                    }
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) lc[c] = ln[c];
+                    piv = pn;
                 }
 #pragma unroll
                 for (int c = 0; c < 16; ++c) {
@@ -1445,6 +1463,62 @@ struct BodyWave {
                 }
             }
             gsync();
+            if (!exact) {
+                // backward, column by column: x[i] = y[i] / L[i][i], then
+                // y[k] -= L[i][k] x[i] for every k < i (row i of the factor is
+                // one coalesced load, issued while row i + 1 is applied).
+                // Each sum takes its terms in decreasing k, not newt's
+                // increasing order: within the drop-in's tolerance.
+                double yr[16], lr[16], ln[16];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    const int k = 64 * c + lane;
+                    yr[c] = k < n ? y[k] : 0.0;
+                    lr[c] = 0.0;
+                }
+                double piv = 1.0;
+                if (n > 0) {
+                    const double* row = Lf + (size_t)(n - 1) * n;
+                    piv = row[n - 1];
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) {
+                        const int k = 64 * c + lane;
+                        lr[c] = k < n - 1 ? row[k] : 0.0;
+                    }
+                }
+                for (int i = n - 1; i >= 0; --i) {
+                    double pn = 1.0;
+                    if (i > 0) {
+                        const double* row = Lf + (size_t)(i - 1) * n;
+                        pn = row[i - 1];
+#pragma unroll
+                        for (int c = 0; c < 16; ++c) {
+                            const int k = 64 * c + lane;
+                            ln[c] = k < i - 1 ? row[k] : 0.0;
+                        }
+                    }
+                    const int ic = i >> 6, ii = i & 63;
+                    double own = 0.0;
+#pragma unroll
+                    for (int c = 0; c < 16; ++c)
+                        if (c == ic) own = yr[c];
+                    const double xi = readlane_f64(own / piv, ii);
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) {
+                        const int k = 64 * c + lane;
+                        if (k == i) yr[c] = xi;
+                        else if (k < i) yr[c] -= lr[c] * xi;
+                    }
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) lr[c] = ln[c];
+                    piv = pn;
+                }
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    const int k = 64 * c + lane;
+                    if (k < n) dd[k] = yr[c];
+                }
+            } else
             // backward: x[i] = (y[i] - sum_{k>i} L[k][i] x[k]) / L[i][i], the sum
             // in increasing k.  Lane k mod 64 keeps x[k] in a register (chunk
             // k / 64 of at most 16) and forms the product L[k][i] x[k]; every

@@ -115,7 +115,8 @@ __global__ void __launch_bounds__(kBlock) k_bodies(ModelT m, Layout L, DataT d, 
 
 // Free-body models, one wavefront (one workgroup) per world.
 __global__ void __launch_bounds__(64) k_bodies_wave(ModelT m, BodyWaveLayout B, Layout L, DataT d, double* dws,
-                                                    int* iws, double* gws, int* gwi, int n_steps, int init) {
+                                                    int* iws, double* gws, int* gwi, int n_steps, int init,
+                                                    int exact) {
     extern __shared__ double lds_b[];
     const int world = blockIdx.x;
     const int lane = threadIdx.x;
@@ -124,7 +125,7 @@ __global__ void __launch_bounds__(64) k_bodies_wave(ModelT m, BodyWaveLayout B, 
     double* sd = lds_b;
     int* si = reinterpret_cast<int*>(lds_b + B.ints_at);
     BodyWave bw{m, B, L, d, dws, iws, sd, si, gws + (size_t)world * B.g_stride_d, gwi + (size_t)world * B.g_stride_i,
-                world, lane, nullptr, nullptr, nullptr};
+                world, lane, nullptr, nullptr, nullptr, exact};
     bw.load();
     if (init) {
         bw.populate(true);
@@ -177,7 +178,8 @@ newt_status launch(newt_data* D, int n_steps, hipStream_t stream, int init) {
         }
         else
             hipLaunchKernelGGL(k_bodies_wave, dim3(D->W), dim3(64), D->BL.lds_bytes, stream, M->dev, D->BL, D->L, D->d,
-                               D->dws, D->iws, D->bws, D->bwi, n_steps, init);
+                               D->dws, D->iws, D->bws, D->bwi, n_steps, init,
+                               D->precision == NEWT_PRECISION_FP64_EXACT ? 1 : 0);
         HIP_TRY(hipGetLastError());
         return NEWT_OK;
     }
