@@ -641,6 +641,8 @@ struct BodyWaveLayout {
 // with NEWT_ERR_CAPACITY.
 constexpr int kNewtonMaxRows = 1024;
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
 inline BodyWaveLayout make_body_wave_layout(const ModelT& m, int lds_budget_bytes) {
     BodyWaveLayout B{};
     const int nb = m.n_bodies, ng = m.n_geoms;
@@ -1325,6 +1327,10 @@ struct BodyWave {
         // kept column-major (Lt, read by the factorization and the forward
         // substitution) and row-major (Lf, read by the back substitution).
         constexpr int P = 16;
+        // The matrix-core left part needs a 64 x 16 scratch: the vectors
+        // pl, pr, sl and hv (unused until the line search), 4 nn doubles.
+        const bool mfma_left = !exact && 4 * nn >= 64 * P;
+        double* xb = pl;
         for (int i = lane; i < n; i += 64) pt[i] = H[(size_t)i * n + i];
         gsync();
         for (int j0 = 0; j0 < n; j0 += P) {
@@ -1338,6 +1344,42 @@ struct BodyWave {
 #pragma unroll
                 for (int c = 0; c < P; ++c) acc[c] = (c < pw && on) ? H[(size_t)ii * n + j0 + c] : 0.0;
                 int k = 0;
+                if (mfma_left && j0 > 0) {
+                    // The left part on the matrix cores (FP64/mixed only): the
+                    // 64 x 16 block sum_k L[i][k] L[j0 + c][k] as four 16 x 16
+                    // tiles of v_mfma_f64_16x16x4 over k, four k per
+                    // instruction.  Lane l feeds A[l % 16][l / 16] (rows of the
+                    // chunk, column k0 + l / 16 of Lt) and B[l / 16][l % 16] (the
+                    // panel's rows); tile entry (4 r + l / 16, l % 16) comes back
+                    // in register r of lane l, and goes through the scratch
+                    // vector `xb` to the lane of its row.  j0 is a multiple of 16.
+                    f64x4 t0 = {0.0, 0.0, 0.0, 0.0}, t1 = t0, t2 = t0, t3 = t0;
+                    const int lr = lane & 15, lk = lane >> 4;
+                    const int a0r = r0 + lr < n ? r0 + lr : n - 1, a1r = r0 + 16 + lr < n ? r0 + 16 + lr : n - 1;
+                    const int a2r = r0 + 32 + lr < n ? r0 + 32 + lr : n - 1, a3r = r0 + 48 + lr < n ? r0 + 48 + lr : n - 1;
+                    const bool bon = lr < pw;
+#pragma unroll 4
+                    for (int k0 = 0; k0 < j0; k0 += 4) {
+                        const double* col = Lt + (size_t)(k0 + lk) * n;
+                        const double b = bon ? col[j0 + lr] : 0.0;
+                        t0 = __builtin_amdgcn_mfma_f64_16x16x4f64(col[a0r], b, t0, 0, 0, 0);
+                        t1 = __builtin_amdgcn_mfma_f64_16x16x4f64(col[a1r], b, t1, 0, 0, 0);
+                        t2 = __builtin_amdgcn_mfma_f64_16x16x4f64(col[a2r], b, t2, 0, 0, 0);
+                        t3 = __builtin_amdgcn_mfma_f64_16x16x4f64(col[a3r], b, t3, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        xb[(4 * r + lk) * P + lr] = t0[r];
+                        xb[(16 + 4 * r + lk) * P + lr] = t1[r];
+                        xb[(32 + 4 * r + lk) * P + lr] = t2[r];
+                        xb[(48 + 4 * r + lk) * P + lr] = t3[r];
+                    }
+                    gsync();
+#pragma unroll
+                    for (int c = 0; c < P; ++c) acc[c] -= xb[lane * P + c];
+                    gsync();
+                    k = j0;
+                }
                 for (; k + 4 <= j0; k += 4) {
                     double li[4], lj[4][P];
 #pragma unroll
