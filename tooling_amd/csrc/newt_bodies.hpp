@@ -1335,6 +1335,9 @@ struct BodyWave {
         gsync();
         for (int j0 = 0; j0 < n; j0 += P) {
             const int pw = n - j0 < P ? n - j0 : P;  // panel width
+            // The panel rows' own entries L[j0 + t][j0 + c] (lane t of the
+            // first chunk), for the later chunks.
+            double q0[P];
             // Rows j0 .. n-1, one per lane, in chunks of 64.
             for (int r0 = j0; r0 < n; r0 += 64) {
                 const int i = r0 + lane;
@@ -1415,8 +1418,7 @@ struct BodyWave {
                             if (c2 < c) {
                                 // L[j][j0 + c2]: computed earlier in this panel (row j is
                                 // in the first chunk: its lane's q[c2], else from Lt).
-                                const double ljk = r0 == j0 ? readlane_f64(q[c2], j - j0)
-                                                            : Lt[(size_t)(j0 + c2) * n + j];
+                                const double ljk = readlane_f64(r0 == j0 ? q[c2] : q0[c2], j - j0);
                                 sum -= q[c2] * ljk;
                             }
                         // Pivot L[j][j] = sqrt(pt[j]) once every k < j is subtracted.
@@ -1426,7 +1428,7 @@ struct BodyWave {
                             if (ptj <= 0.0) return 4;  // NEWT_ERR_NEWTON_FAIL: Hessian not positive definite
                             ljj = __builtin_sqrt(ptj);
                         } else {
-                            ljj = Lt[(size_t)j * n + j];
+                            ljj = readlane_f64(q0[c], j - j0);
                         }
                         if (on && i > j) {
                             q[c] = sum / ljj;
@@ -1444,6 +1446,10 @@ struct BodyWave {
                         Lf[(size_t)i * n + j] = q[c];
                     }
                 }
+                if (r0 == j0) {
+#pragma unroll
+                    for (int c = 0; c < P; ++c) q0[c] = q[c];
+                }
                 if (on) pt[i] = pr;
                 gsync();
             }
@@ -1459,44 +1465,59 @@ struct BodyWave {
             }
             gsync();
             // forward: y[i] -= L[i][k] y[k] for k = 0, 1, ... (the row sum's
-            // order), y[64 c + lane] held in register yr[c]; column k of the
-            // factor is one coalesced load, issued while column k - 1 is
-            // applied; the pivot's owner divides and its value is broadcast.
+            // order), y[64 c + lane] held in register yr[c], sixteen columns
+            // k0 .. k0+15 at a time: the panel's own rows (all in one chunk)
+            // are solved in order with the pivot's owner dividing and its
+            // value broadcast, then every row below takes the panel's sixteen
+            // terms in k order, all sixteen factor loads of a row in flight
+            // together.
             {
-                double yr[16], lc[16], ln[16];
+                double yr[16];
 #pragma unroll
                 for (int c = 0; c < 16; ++c) {
                     const int i = 64 * c + lane;
                     yr[c] = i < n ? y[i] : 0.0;
-                    lc[c] = i < n ? Lt[i] : 0.0;  // column 0
                 }
-                double piv = Lt[0];
-                for (int k = 0; k < n; ++k) {
-                    const int kc = k >> 6, kk = k & 63;
-                    double pn = 1.0;
-                    if (k + 1 < n) {
-                        const double* nxt = Lt + (size_t)(k + 1) * n;
-                        pn = nxt[k + 1];
+                for (int k0 = 0; k0 < n; k0 += 16) {
+                    const int kw = n - k0 < 16 ? n - k0 : 16, cp = k0 >> 6, base = k0 & 63;
+                    const int rp = 64 * cp + lane;  // this lane's row in the panel's chunk
+                    const int rpc = rp < n ? rp : n - 1;
+                    double lp[16], yp[16];
 #pragma unroll
-                        for (int c = 0; c < 16; ++c) {
-                            const int i = 64 * c + lane;
-                            ln[c] = (i > k + 1 && i < n) ? nxt[i] : 0.0;
-                        }
-                    }
-                    double own = 0.0;
+                    for (int u = 0; u < 16; ++u) lp[u] = u < kw ? Lt[(size_t)(k0 + u) * n + rpc] : 0.0;
+                    double ycp = 0.0;
 #pragma unroll
                     for (int c = 0; c < 16; ++c)
-                        if (c == kc) own = yr[c];
-                    const double yk = readlane_f64(own / piv, kk);
+                        if (c == cp) ycp = yr[c];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) {
+                        yp[u] = 0.0;
+                        if (u < kw) {
+                            const int k = k0 + u, own = base + u;
+                            const double yk = readlane_f64(ycp / lp[u], own);
+                            if (lane == own) ycp = yk;
+                            else if (rp > k && rp < n) ycp -= lp[u] * yk;
+                            yp[u] = yk;
+                        }
+                    }
+#pragma unroll
+                    for (int c = 0; c < 16; ++c)
+                        if (c == cp) yr[c] = ycp;
 #pragma unroll
                     for (int c = 0; c < 16; ++c) {
                         const int i = 64 * c + lane;
-                        if (i == k) yr[c] = yk;
-                        else if (i > k && i < n) yr[c] -= lc[c] * yk;
-                    }
+                        if (c > cp && 64 * c < n) {
+                            const int ic = i < n ? i : n - 1;
+                            double l[16];
 #pragma unroll
-                    for (int c = 0; c < 16; ++c) lc[c] = ln[c];
-                    piv = pn;
+                            for (int u = 0; u < 16; ++u) l[u] = u < kw ? Lt[(size_t)(k0 + u) * n + ic] : 0.0;
+                            if (i < n) {
+#pragma unroll
+                                for (int u = 0; u < 16; ++u)
+                                    if (u < kw) yr[c] -= l[u] * yp[u];
+                            }
+                        }
+                    }
                 }
 #pragma unroll
                 for (int c = 0; c < 16; ++c) {
@@ -1506,54 +1527,56 @@ struct BodyWave {
             }
             gsync();
             if (!exact) {
-                // backward, column by column: x[i] = y[i] / L[i][i], then
-                // y[k] -= L[i][k] x[i] for every k < i (row i of the factor is
-                // one coalesced load, issued while row i + 1 is applied).
-                // Each sum takes its terms in decreasing k, not newt's
-                // increasing order: within the drop-in's tolerance.
-                double yr[16], lr[16], ln[16];
+                // backward, sixteen rows i0 .. i0+15 at a time from the bottom:
+                // x[i] = y[i] / L[i][i] for the panel's rows in decreasing i,
+                // each then subtracted from the panel rows above it, then
+                // y[k] -= L[i][k] x[i] for every k < i0, the sixteen factor
+                // rows' loads in flight together.  Each sum takes its terms in
+                // decreasing i, not newt's increasing order: within the
+                // drop-in's tolerance.
+                double yr[16];
 #pragma unroll
                 for (int c = 0; c < 16; ++c) {
                     const int k = 64 * c + lane;
                     yr[c] = k < n ? y[k] : 0.0;
-                    lr[c] = 0.0;
                 }
-                double piv = 1.0;
-                if (n > 0) {
-                    const double* row = Lf + (size_t)(n - 1) * n;
-                    piv = row[n - 1];
+                for (int i0 = (n - 1) & ~15; i0 >= 0; i0 -= 16) {
+                    const int iw = n - i0 < 16 ? n - i0 : 16, cp = i0 >> 6, base = i0 & 63;
+                    const int kp = 64 * cp + lane;  // this lane's index in the panel's chunk
+                    const int kpc = kp < n ? kp : n - 1;
+                    double lp[16], xp[16];
 #pragma unroll
-                    for (int c = 0; c < 16; ++c) {
-                        const int k = 64 * c + lane;
-                        lr[c] = k < n - 1 ? row[k] : 0.0;
-                    }
-                }
-                for (int i = n - 1; i >= 0; --i) {
-                    double pn = 1.0;
-                    if (i > 0) {
-                        const double* row = Lf + (size_t)(i - 1) * n;
-                        pn = row[i - 1];
-#pragma unroll
-                        for (int c = 0; c < 16; ++c) {
-                            const int k = 64 * c + lane;
-                            ln[c] = k < i - 1 ? row[k] : 0.0;
-                        }
-                    }
-                    const int ic = i >> 6, ii = i & 63;
-                    double own = 0.0;
+                    for (int u = 0; u < 16; ++u) lp[u] = u < iw ? Lf[(size_t)(i0 + u) * n + kpc] : 0.0;
+                    double ycp = 0.0;
 #pragma unroll
                     for (int c = 0; c < 16; ++c)
-                        if (c == ic) own = yr[c];
-                    const double xi = readlane_f64(own / piv, ii);
+                        if (c == cp) ycp = yr[c];
+#pragma unroll
+                    for (int u = 15; u >= 0; --u) {
+                        xp[u] = 0.0;
+                        if (u < iw) {
+                            const int i = i0 + u, own = base + u;
+                            const double xi = readlane_f64(ycp / lp[u], own);
+                            if (lane == own) ycp = xi;
+                            else if (kp < i) ycp -= lp[u] * xi;
+                            xp[u] = xi;
+                        }
+                    }
+#pragma unroll
+                    for (int c = 0; c < 16; ++c)
+                        if (c == cp) yr[c] = ycp;
 #pragma unroll
                     for (int c = 0; c < 16; ++c) {
                         const int k = 64 * c + lane;
-                        if (k == i) yr[c] = xi;
-                        else if (k < i) yr[c] -= lr[c] * xi;
-                    }
+                        if (c < cp) {
+                            double l[16];
 #pragma unroll
-                    for (int c = 0; c < 16; ++c) lr[c] = ln[c];
-                    piv = pn;
+                            for (int u = 0; u < 16; ++u) l[u] = u < iw ? Lf[(size_t)(i0 + u) * n + k] : 0.0;
+#pragma unroll
+                            for (int u = 15; u >= 0; --u)
+                                if (u < iw) yr[c] -= l[u] * xp[u];
+                        }
+                    }
                 }
 #pragma unroll
                 for (int c = 0; c < 16; ++c) {
