@@ -642,6 +642,10 @@ struct BodyWaveLayout {
 constexpr int kNewtonMaxRows = 1024;
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+// Doubles in global memory.  The Newton workspace is addressed through these:
+// from the workspace's generic pointers the compiler emits flat loads, whose
+// completion waits also cover LDS traffic and serialize every load.
+typedef __attribute__((address_space(1))) double gf64;
 
 inline BodyWaveLayout make_body_wave_layout(const ModelT& m, int lds_budget_bytes) {
     BodyWaveLayout B{};
@@ -1194,7 +1198,8 @@ struct BodyWave {
     // every lane (same result everywhere).  Each lane forms one product of a
     // chunk of 64 (all loads in flight at once); the additions then run in
     // index order on the broadcast products.
-    __device__ __forceinline__ double dot_seq(const double* a, const double* b, int n) const {
+    template <class PA, class PB>
+    __device__ __forceinline__ double dot_seq(PA a, PB b, int n) const {
         double s = -0.0;
         for (int i0 = 0; i0 < n; i0 += 64) {
             const int i = i0 + lane;
@@ -1212,11 +1217,11 @@ struct BodyWave {
     // out = H v, one lane per row, each row summed in column order from 0.0;
     // a lane carries four rows at once (independent sums, same order each),
     // and eight columns' loads are issued before their products are summed.
-    __device__ __forceinline__ void mat_vec(const double* H, const double* v, double* out, int n) const {
+    template <class PH, class PV, class PO>
+    __device__ __forceinline__ void mat_vec(PH H, PV v, PO out, int n) const {
         for (int i0 = lane; i0 < n; i0 += 256) {
             const int i1 = i0 + 64 < n ? i0 + 64 : i0, i2 = i0 + 128 < n ? i0 + 128 : i0, i3 = i0 + 192 < n ? i0 + 192 : i0;
-            const double *h0 = H + (size_t)i0 * n, *h1 = H + (size_t)i1 * n, *h2 = H + (size_t)i2 * n,
-                         *h3 = H + (size_t)i3 * n;
+            const auto h0 = H + (size_t)i0 * n, h1 = H + (size_t)i1 * n, h2 = H + (size_t)i2 * n, h3 = H + (size_t)i3 * n;
             double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
             int j = 0;
             for (; j + 8 <= n; j += 8) {
@@ -1251,7 +1256,8 @@ struct BodyWave {
         }
         gsync();
     }
-    __device__ __forceinline__ void affine(const double* x, const double* d, double alpha, double* out, int n) const {
+    template <class PX, class PD, class PO>
+    __device__ __forceinline__ void affine(PX x, PD d, double alpha, PO out, int n) const {
         for (int i = lane; i < n; i += 64) {
             const double v = x[i] + alpha * d[i];
             out[i] = v < 0.0 ? 0.0 : v;
@@ -1259,7 +1265,8 @@ struct BodyWave {
         gsync();
     }
     // quadratic_cost 0.5 xᵀHx + linᵀx with Hx into hv.
-    __device__ __forceinline__ double cost(const double* H, const double* lin, const double* x, double* hv, int n) const {
+    template <class P>
+    __device__ __forceinline__ double cost(P H, P lin, P x, P hv, int n) const {
         mat_vec(H, x, hv, n);
         return 0.5 * dot_seq(x, hv, n) + dot_seq(lin, x, n);
     }
@@ -1268,14 +1275,14 @@ struct BodyWave {
         const int n = 4 * nblk;
         if (n > B.nn) return 8;  // NEWT_ERR_CAPACITY
         const int nn = B.nn;
-        double* H = gd + B.g_H;
-        double* Lf = gd + B.g_L;   // the factor, row-major
-        double* Lt = gd + B.g_Lt;  // Lt[j n + i] = L[i][j], column-major
-        double* V = gd + B.g_vec;
-        double *lin = V, *x = V + nn, *g = V + 2 * nn, *dd = V + 3 * nn, *y = V + 4 * nn, *pt = V + 5 * nn;
-        double *pl = V + 6 * nn, *pr = V + 7 * nn, *sl = V + 8 * nn, *hv = V + 9 * nn;
-        double* brk = gd + B.g_brk;
-        double* cand = gd + B.g_cand;
+        gf64* H = (gf64*)(gd + B.g_H);
+        gf64* Lf = (gf64*)(gd + B.g_L);   // the factor, row-major
+        gf64* Lt = (gf64*)(gd + B.g_Lt);  // Lt[j n + i] = L[i][j], column-major
+        gf64* V = (gf64*)(gd + B.g_vec);
+        gf64 *lin = V, *x = V + nn, *g = V + 2 * nn, *dd = V + 3 * nn, *y = V + 4 * nn, *pt = V + 5 * nn;
+        gf64 *pl = V + 6 * nn, *pr = V + 7 * nn, *sl = V + 8 * nn, *hv = V + 9 * nn;
+        gf64* brk = (gf64*)(gd + B.g_brk);
+        gf64* cand = (gf64*)(gd + B.g_cand);
         // A[i][j] = response of row i to a unit impulse on row j; H = A + diag(reg).
         // One lane per column j; rows go by contact block (four rows share
         // their bodies).  A block with neither of column j's bodies keeps
@@ -1327,10 +1334,7 @@ struct BodyWave {
         // kept column-major (Lt, read by the factorization and the forward
         // substitution) and row-major (Lf, read by the back substitution).
         constexpr int P = 16;
-        // The matrix-core left part needs a 64 x 16 scratch: the vectors
-        // pl, pr, sl and hv (unused until the line search), 4 nn doubles.
-        const bool mfma_left = !exact && 4 * nn >= 64 * P;
-        double* xb = pl;
+        const bool mfma_left = !exact;
         for (int i = lane; i < n; i += 64) pt[i] = H[(size_t)i * n + i];
         gsync();
         for (int j0 = 0; j0 < n; j0 += P) {
@@ -1349,38 +1353,74 @@ struct BodyWave {
                 int k = 0;
                 if (mfma_left && j0 > 0) {
                     // The left part on the matrix cores (FP64/mixed only): the
-                    // 64 x 16 block sum_k L[i][k] L[j0 + c][k] as four 16 x 16
-                    // tiles of v_mfma_f64_16x16x4 over k, four k per
-                    // instruction.  Lane l feeds A[l % 16][l / 16] (rows of the
-                    // chunk, column k0 + l / 16 of Lt) and B[l / 16][l % 16] (the
-                    // panel's rows); tile entry (4 r + l / 16, l % 16) comes back
-                    // in register r of lane l, and goes through the scratch
-                    // vector `xb` to the lane of its row.  j0 is a multiple of 16.
-                    f64x4 t0 = {0.0, 0.0, 0.0, 0.0}, t1 = t0, t2 = t0, t3 = t0;
+                    // 16 x 64 block sum_k L[j0 + c][k] L[i][k] as four 16 x 16
+                    // tiles T of v_mfma_f64_16x16x4 (rows i = r0 + 16 T + 0..15),
+                    // four k per instruction.  Lane l feeds A[l % 16][l / 16] =
+                    // L[j0 + l % 16][k0 + l / 16] and B[l / 16][l % 16] =
+                    // L[r0 + 16 T + l % 16][k0 + l / 16]; register r of lane l
+                    // returns column c = 4 r + l / 16 of row r0 + 16 T + l % 16.
+                    // Sixteen k go per group, the next group's 20 loads in
+                    // flight while the current group's 16 products run.  j0 is
+                    // a multiple of 16.
+                    f64x4 t[4];
+#pragma unroll
+                    for (int T = 0; T < 4; ++T) t[T] = f64x4{0.0, 0.0, 0.0, 0.0};
                     const int lr = lane & 15, lk = lane >> 4;
-                    const int a0r = r0 + lr < n ? r0 + lr : n - 1, a1r = r0 + 16 + lr < n ? r0 + 16 + lr : n - 1;
-                    const int a2r = r0 + 32 + lr < n ? r0 + 32 + lr : n - 1, a3r = r0 + 48 + lr < n ? r0 + 48 + lr : n - 1;
+                    int ar[4];
+#pragma unroll
+                    for (int T = 0; T < 4; ++T) ar[T] = r0 + 16 * T + lr < n ? r0 + 16 * T + lr : n - 1;
+                    const int jb = j0 + lr < n ? j0 + lr : n - 1;
                     const bool bon = lr < pw;
-#pragma unroll 4
-                    for (int k0 = 0; k0 < j0; k0 += 4) {
-                        const double* col = Lt + (size_t)(k0 + lk) * n;
-                        const double b = bon ? col[j0 + lr] : 0.0;
-                        t0 = __builtin_amdgcn_mfma_f64_16x16x4f64(col[a0r], b, t0, 0, 0, 0);
-                        t1 = __builtin_amdgcn_mfma_f64_16x16x4f64(col[a1r], b, t1, 0, 0, 0);
-                        t2 = __builtin_amdgcn_mfma_f64_16x16x4f64(col[a2r], b, t2, 0, 0, 0);
-                        t3 = __builtin_amdgcn_mfma_f64_16x16x4f64(col[a3r], b, t3, 0, 0, 0);
+                    double ca[4][4], cb[4], na[4][4], nb[4];
+                    auto load_group = [&](int k0, double (&a)[4][4], double (&b)[4]) {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const auto col = Lt + (size_t)(k0 + 4 * u + lk) * n;
+                            b[u] = col[jb];
+#pragma unroll
+                            for (int T = 0; T < 4; ++T) a[u][T] = col[ar[T]];
+                        }
+                    };
+                    load_group(0, ca, cb);
+                    for (int k0 = 0; k0 < j0; k0 += 16) {
+                        // The next group (the last group again at the end: every
+                        // load stays in bounds and unconditional).
+                        load_group(k0 + 16 < j0 ? k0 + 16 : k0, na, nb);
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const double bu = bon ? cb[u] : 0.0;
+#pragma unroll
+                            for (int T = 0; T < 4; ++T) t[T] = __builtin_amdgcn_mfma_f64_16x16x4f64(bu, ca[u][T], t[T], 0, 0, 0);
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            cb[u] = nb[u];
+#pragma unroll
+                            for (int T = 0; T < 4; ++T) ca[u][T] = na[u][T];
+                        }
+                    }
+                    // To one row per lane: lane 16 q + m holds rows 16 T + m
+                    // (all T) at columns 4 r + q; lane 16 q + m wants row
+                    // 16 q + m at every column.  Round s swaps with lane
+                    // l ^ 16 s: each lane sends tile q ^ s and receives row
+                    // 16 q + m at columns 4 r + (q ^ s).
+                    const int q = lane >> 4;
+                    double rv[4][4];
+#pragma unroll
+                    for (int sx = 0; sx < 4; ++sx) {
+                        const int tq = q ^ sx;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const double v = tq == 0 ? t[0][r] : tq == 1 ? t[1][r] : tq == 2 ? t[2][r] : t[3][r];
+                            rv[sx][r] = sx == 0 ? v : __shfl_xor(v, 16 * sx, 64);
+                        }
                     }
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        xb[(4 * r + lk) * P + lr] = t0[r];
-                        xb[(16 + 4 * r + lk) * P + lr] = t1[r];
-                        xb[(32 + 4 * r + lk) * P + lr] = t2[r];
-                        xb[(48 + 4 * r + lk) * P + lr] = t3[r];
+                    for (int c = 0; c < P; ++c) {
+                        const int sx = (c & 3) ^ q, r = c >> 2;
+                        const double v = sx == 0 ? rv[0][r] : sx == 1 ? rv[1][r] : sx == 2 ? rv[2][r] : rv[3][r];
+                        acc[c] -= v;
                     }
-                    gsync();
-#pragma unroll
-                    for (int c = 0; c < P; ++c) acc[c] -= xb[lane * P + c];
-                    gsync();
                     k = j0;
                 }
                 for (; k + 4 <= j0; k += 4) {
@@ -1598,7 +1638,7 @@ struct BodyWave {
                     lr[c] = 0.0;
                 }
                 auto load_row = [&](int i, double* out) {
-                    const double* lti = Lt + (size_t)i * n;
+                    const auto lti = Lt + (size_t)i * n;
 #pragma unroll
                     for (int c = 0; c < 16; ++c) {
                         const int k = 64 * c + lane;
