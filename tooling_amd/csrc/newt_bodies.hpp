@@ -630,9 +630,10 @@ struct BodyWaveLayout {
     int lds_contacts;                     // active contacts whose rows fit in LDS
     int g_rows, g_blk, g_stride_d;        // global fallback, doubles per world
     int g_bbi, g_stride_i;                // ints per world
-    // Newton solve (global, per world): rows cap nn, then H, L and Lᵀ (nn x nn) and
-    // the vectors lin, x, g, d, y, p, pl, pr, slope, hv (nn each) and the
-    // break / candidate lists (2 nn + 4 each)
+    // Newton solve (global, per world): rows cap nn, then H, L and Lᵀ (nn x nn), the
+    // vectors lin, x, g, d, y, p, pl, pr, slope, hv, four probe points and their
+    // products with H, and H times the best probe (nn each), and the break /
+    // candidate lists (2 nn + 4 each)
     int nn, g_H, g_L, g_Lt, g_vec, g_brk, g_cand;  // the factor row- and column-major
 };
 
@@ -682,7 +683,7 @@ inline BodyWaveLayout make_body_wave_layout(const ModelT& m, int lds_budget_byte
         B.g_L = B.g_H + nn * nn;
         B.g_Lt = B.g_L + nn * nn;
         B.g_vec = B.g_Lt + nn * nn;
-        B.g_brk = B.g_vec + 10 * nn;
+        B.g_brk = B.g_vec + 19 * nn;
         B.g_cand = B.g_brk + 2 * nn + 4;
         B.g_stride_d = B.g_cand + 2 * nn + 4;
     }
@@ -1256,6 +1257,65 @@ struct BodyWave {
         }
         gsync();
     }
+    // out_k = H v_k for the K <= 4 vectors v_k = vb + k nn (out_k = ob + k nn):
+    // each row summed in column order from 0.0 exactly as mat_vec, so every
+    // out_k equals mat_vec(H, v_k) bit for bit; one pass over H serves all K.
+    template <class PH, class PV>
+    __device__ __forceinline__ void mat_vec4(PH H, PV vb, PV ob, int K, int n, int nn) const {
+        const int k1 = K > 1 ? 1 : 0, k2 = K > 2 ? 2 : k1, k3 = K > 3 ? 3 : k2;
+        const auto v0 = vb, v1 = vb + (size_t)k1 * nn, v2 = vb + (size_t)k2 * nn, v3 = vb + (size_t)k3 * nn;
+        for (int base = 0; base < n; base += 256) {
+            const int i0 = base + lane;
+            int ir[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) ir[r] = i0 + 64 * r < n ? i0 + 64 * r : n - 1;
+            double s[4][4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) s[r][k] = 0.0;
+            int j = 0;
+            for (; j + 8 <= n; j += 8) {
+                double h[4][8], vv[4][8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    vv[0][u] = v0[j + u];
+                    vv[1][u] = v1[j + u];
+                    vv[2][u] = v2[j + u];
+                    vv[3][u] = v3[j + u];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) h[r][u] = H[(size_t)ir[r] * n + j + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) s[r][k] += h[r][u] * vv[k][u];
+            }
+            for (; j < n; ++j) {
+                const double w0 = v0[j], w1 = v1[j], w2 = v2[j], w3 = v3[j];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const double hr = H[(size_t)ir[r] * n + j];
+                    s[r][0] += hr * w0;
+                    s[r][1] += hr * w1;
+                    s[r][2] += hr * w2;
+                    s[r][3] += hr * w3;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + 64 * r;
+                if (i < n) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        if (k < K) ob[(size_t)k * nn + i] = s[r][k];
+                }
+            }
+        }
+        gsync();
+    }
     template <class PX, class PD, class PO>
     __device__ __forceinline__ void affine(PX x, PD d, double alpha, PO out, int n) const {
         for (int i = lane; i < n; i += 64) {
@@ -1495,12 +1555,15 @@ struct BodyWave {
             }
         }
         const uint32_t max_it = m.iterations < 1 ? 1u : (uint32_t)m.iterations;
+        // Probe points P(x + a d) and their products with H, four per pass
+        // over H; hcur holds H x for the current x (the gradient's product,
+        // bitwise what mat_vec(H, x) gives), hbest H times the best probe.
+        gf64 *vb = V + 10 * nn, *ob = V + 14 * nn, *hcur = hv, *hbest = V + 18 * nn;
         double cur = cost(H, lin, x, hv, n);
         for (uint32_t it = 0; it < max_it; ++it) {
             // gradient, direction = cholesky_solve(L, -g) (dynamics.rs:626-648)
-            mat_vec(H, x, g, n);
             for (int i = lane; i < n; i += 64) {
-                g[i] = g[i] + lin[i];
+                g[i] = hcur[i] + lin[i];
                 y[i] = -g[i];
             }
             gsync();
@@ -1693,77 +1756,128 @@ struct BodyWave {
             gsync();
             if (dot_seq(g, dd, n) >= 0.0) break;
             // exact_pyramidal_line_search with NonNegative projections
+            // (newton.rs:400-556).  Breakpoints: 0, 1, then -x[i] / d[i] in
+            // (0, 1) in index order (compacted across the wave), sorted and
+            // deduplicated on lane 0.
             int nbk = 0;
-            if (lane == 0) {
-                brk[0] = 0.0;
-                brk[1] = 1.0;
+            {
                 int c = 2;
-                for (int i = 0; i < n; ++i) {
-                    const double slope = dd[i];
-                    if (slope == 0.0) continue;
-                    const double alpha = -x[i] / slope;
-                    if (alpha > 0.0 && alpha < 1.0) brk[c++] = alpha;
-                }
-                for (int a = 1; a < c; ++a) {  // stable insertion sort
-                    const double v = brk[a];
-                    int b = a - 1;
-                    while (b >= 0 && brk[b] > v) {
-                        brk[b + 1] = brk[b];
-                        --b;
+                for (int i0 = 0; i0 < n; i0 += 64) {
+                    const int i = i0 + lane;
+                    double alpha = 0.0;
+                    bool ok = false;
+                    if (i < n) {
+                        const double slope = dd[i];
+                        if (slope != 0.0) {
+                            alpha = -x[i] / slope;
+                            ok = alpha > 0.0 && alpha < 1.0;
+                        }
                     }
-                    brk[b + 1] = v;
+                    const unsigned long long mk = __ballot(ok);
+                    if (ok) brk[c + __popcll(mk & lanes_below(lane))] = alpha;
+                    c += __popcll(mk);
                 }
-                int u = 0;
-                for (int a = 0; a < c; ++a) {
-                    const double v = brk[a];
-                    if (u > 0 && __builtin_fabs(v - brk[u - 1]) <= 1e-7) continue;
-                    brk[u++] = v;
+                if (lane == 0) {
+                    brk[0] = 0.0;
+                    brk[1] = 1.0;
                 }
-                nbk = u;
+                gsync();
+                if (lane == 0) {
+                    for (int a = 1; a < c; ++a) {  // stable insertion sort
+                        const double v = brk[a];
+                        int b = a - 1;
+                        while (b >= 0 && brk[b] > v) {
+                            brk[b + 1] = brk[b];
+                            --b;
+                        }
+                        brk[b + 1] = v;
+                    }
+                    int u = 0;
+                    for (int a = 0; a < c; ++a) {
+                        const double v = brk[a];
+                        if (u > 0 && __builtin_fabs(v - brk[u - 1]) <= 1e-7) continue;
+                        brk[u++] = v;
+                    }
+                    nbk = u;
+                }
+                nbk = __shfl(nbk, 0, 64);
+                gsync();
             }
-            nbk = __shfl(nbk, 0, 64);
-            gsync();
             int ncand = nbk;
             for (int a = lane; a < nbk; a += 64) cand[a] = brk[a];
             gsync();
-            for (int k = 0; k + 1 < nbk; ++k) {
-                const double lo = brk[k], hi = brk[k + 1];
-                if (hi - lo <= 1e-7) continue;
-                const double mid = 0.5 * (lo + hi);
-                const double eps = (hi - lo) * 1e-4;
-                const double la = nmax(mid - eps, lo);
-                const double ra = nmin(mid + eps, hi);
-                affine(x, dd, la, pl, n);
-                affine(x, dd, ra, pr, n);
-                const double span = ra - la;
-                if (span <= 0.0) continue;
-                const double inv = 1.0 / span;
-                for (int i = lane; i < n; i += 64) sl[i] = (pr[i] - pl[i]) * inv;
-                affine(x, dd, mid, pt, n);  // includes the fence for sl
-                mat_vec(H, pt, hv, n);
-                for (int i = lane; i < n; i += 64) pl[i] = hv[i] + lin[i];  // local gradient
-                mat_vec(H, sl, hv, n);  // fences pl too
-                const double den = dot_seq(sl, hv, n);
-                const double num = dot_seq(pl, sl, n);
-                const double opt = den > 0.0 ? nmin(nmax(mid - num / den, lo), hi) : mid;
-                if (lane == 0) cand[ncand] = opt;
-                ++ncand;
+            // Each interval's stationary point from the slope of the projected
+            // path (mid point and secant slope), two intervals per pass over H.
+            for (int k = 0; k + 1 < nbk;) {
+                int nk = 0;
+                double mids[2], los[2], his[2];
+                while (k + 1 < nbk && nk < 2) {
+                    const double lo = brk[k], hi = brk[k + 1];
+                    ++k;
+                    if (hi - lo <= 1e-7) continue;
+                    const double mid = 0.5 * (lo + hi);
+                    const double eps = (hi - lo) * 1e-4;
+                    const double la = nmax(mid - eps, lo);
+                    const double ra = nmin(mid + eps, hi);
+                    const double span = ra - la;
+                    if (span <= 0.0) continue;
+                    const double inv = 1.0 / span;
+                    for (int i = lane; i < n; i += 64) {
+                        const double xi = x[i], di = dd[i];
+                        const double vl = xi + la * di, vr = xi + ra * di, vm = xi + mid * di;
+                        const double pli = vl < 0.0 ? 0.0 : vl, pri = vr < 0.0 ? 0.0 : vr;
+                        vb[(size_t)(2 * nk) * nn + i] = vm < 0.0 ? 0.0 : vm;
+                        vb[(size_t)(2 * nk + 1) * nn + i] = (pri - pli) * inv;
+                    }
+                    mids[nk] = mid;
+                    los[nk] = lo;
+                    his[nk] = hi;
+                    ++nk;
+                }
+                if (nk == 0) continue;
+                gsync();
+                mat_vec4(H, vb, ob, 2 * nk, n, nn);
+                for (int t = 0; t < nk; ++t) {
+                    gf64 *pm = vb + (size_t)(2 * t) * nn, *sl_t = vb + (size_t)(2 * t + 1) * nn;
+                    gf64 *hm = ob + (size_t)(2 * t) * nn, *hs = ob + (size_t)(2 * t + 1) * nn;
+                    for (int i = lane; i < n; i += 64) pm[i] = hm[i] + lin[i];  // local gradient
+                    gsync();
+                    const double den = dot_seq(sl_t, hs, n);
+                    const double num = dot_seq(pm, sl_t, n);
+                    const double mid = mids[t], lo = los[t], hi = his[t];
+                    const double opt = den > 0.0 ? nmin(nmax(mid - num / den, lo), hi) : mid;
+                    if (lane == 0) cand[ncand] = opt;
+                    ++ncand;
+                }
                 gsync();
             }
+            // Every candidate's cost 0.5 pᵀHp + linᵀp in candidate order (the
+            // first strict minimum wins), four probes per pass over H.
             double best_a = 0.0, best_c = cur;
-            for (int c = 0; c < ncand; ++c) {
-                const double a = cand[c];
-                affine(x, dd, a, pt, n);
-                const double cc = cost(H, lin, pt, hv, n);
-                if (cc < best_c) {
-                    best_c = cc;
-                    best_a = a;
+            for (int c0 = 0; c0 < ncand; c0 += 4) {
+                const int K = ncand - c0 < 4 ? ncand - c0 : 4;
+                for (int t = 0; t < K; ++t) affine(x, dd, cand[c0 + t], vb + (size_t)t * nn, n);
+                mat_vec4(H, vb, ob, K, n, nn);
+                for (int t = 0; t < K; ++t) {
+                    gf64 *pt_t = vb + (size_t)t * nn, *hp = ob + (size_t)t * nn;
+                    const double cc = 0.5 * dot_seq(pt_t, hp, n) + dot_seq(lin, pt_t, n);
+                    if (cc < best_c) {
+                        best_c = cc;
+                        best_a = cand[c0 + t];
+                        for (int i = lane; i < n; i += 64) hbest[i] = hp[i];
+                    }
                 }
+                gsync();
             }
             if (best_a <= 0.0 || cur - best_c <= 1e-7 * (1.0 + __builtin_fabs(cur))) break;
             for (int i = lane; i < n; i += 64) {
                 const double v = x[i] + best_a * dd[i];
                 x[i] = v < 0.0 ? 0.0 : v;
+            }
+            {
+                gf64* t = hcur;
+                hcur = hbest;
+                hbest = t;
             }
             gsync();
             cur = best_c;
