@@ -1346,8 +1346,22 @@ struct BodyWave {
         // A[i][j] = response of row i to a unit impulse on row j; H = A + diag(reg).
         // One lane per column j; rows go by contact block (four rows share
         // their bodies).  A block with neither of column j's bodies keeps
-        // zero accumulators, whose residual is +0.0.
-        for (int j = lane; j < n; j += 64) {
+        // zero accumulators, whose residual is +0.0.  Every block's two body
+        // indices are loaded once (lane k % 64 holds block k's in slot k / 64,
+        // nblk <= 256) and broadcast in the loop, so no load waits behind the
+        // stores of H.  The column loop keeps every lane active (lanes past
+        // the last column work on column n - 1 and store nothing), so each
+        // broadcast reads a register every lane has written.
+        int ba[4], bb[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int k = 64 * t + lane;
+            ba[t] = k < nblk ? bbi[2 * k] : -1;
+            bb[t] = k < nblk ? bbi[2 * k + 1] : -1;
+        }
+        for (int j0 = 0; j0 < n; j0 += 64) {
+            const bool on = j0 + lane < n;
+            const int j = on ? j0 + lane : n - 1;
             const int kj = j >> 2;
             const int aj = bbi[2 * kj], bj = bbi[2 * kj + 1];
             const V3<double> dir = rdir(j);
@@ -1355,7 +1369,10 @@ struct BodyWave {
             if (aj >= 0) side_delta(aj, dir, ld3(blk + 8 * kj), 1.0, &dva, &dwa);
             if (bj >= 0) side_delta(bj, -dir, ld3(blk + 8 * kj + 3), 1.0, &dvb, &dwb);
             for (int ki = 0; ki < nblk; ++ki) {
-                const int ai = bbi[2 * ki], bi = bbi[2 * ki + 1];
+                const int t = ki >> 6;
+                const int va = t == 0 ? ba[0] : t == 1 ? ba[1] : t == 2 ? ba[2] : ba[3];
+                const int vb = t == 0 ? bb[0] : t == 1 ? bb[1] : t == 2 ? bb[2] : bb[3];
+                const int ai = __builtin_amdgcn_readlane(va, ki & 63), bi = __builtin_amdgcn_readlane(vb, ki & 63);
                 V3<double> av = z3<double>(), aw = z3<double>(), bv = z3<double>(), bw = z3<double>();
                 bool any = false;
                 if (ai >= 0 && ai == aj) { av = dva; aw = dwa; any = true; }
@@ -1373,7 +1390,7 @@ struct BodyWave {
                     const int i = 4 * ki + r;
                     double a = a4[r];
                     if (i == j) a += rows[7 * i + 3];
-                    H[(size_t)i * n + j] = a;
+                    if (on) H[(size_t)i * n + j] = a;
                 }
             }
         }
