@@ -648,6 +648,13 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 // completion waits also cover LDS traffic and serialize every load.
 typedef __attribute__((address_space(1))) double gf64;
 
+// Lane r's 64-bit value (0 for r >= 64).
+__device__ __forceinline__ unsigned long long row_mask(unsigned long long v, int r) {
+    if (r >= 64) return 0ull;
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)v, r), hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), r);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 inline BodyWaveLayout make_body_wave_layout(const ModelT& m, int lds_budget_bytes) {
     BodyWaveLayout B{};
     const int nb = m.n_bodies, ng = m.n_geoms;
@@ -1412,6 +1419,12 @@ struct BodyWave {
         // substitution) and row-major (Lf, read by the back substitution).
         constexpr int P = 16;
         const bool mfma_left = !exact;
+        // Lane R holds row block R's mask of 16-column blocks of L with a
+        // nonzero entry (rows 16 R .. 16 R + 15; n <= 1024 gives R < 64), set
+        // as each panel is written.  The matrix-core left part visits only
+        // the column blocks where both the panel's rows and the chunk's rows
+        // have one: every skipped product is an exact zero.
+        unsigned long long rmask = 0;
         for (int i = lane; i < n; i += 64) pt[i] = H[(size_t)i * n + i];
         gsync();
         for (int j0 = 0; j0 < n; j0 += P) {
@@ -1442,6 +1455,10 @@ struct BodyWave {
                     f64x4 t[4];
 #pragma unroll
                     for (int T = 0; T < 4; ++T) t[T] = f64x4{0.0, 0.0, 0.0, 0.0};
+                    const int kj = j0 >> 4, r16 = r0 >> 4;
+                    unsigned long long km = row_mask(rmask, kj) & (row_mask(rmask, r16) | row_mask(rmask, r16 + 1) |
+                                                                   row_mask(rmask, r16 + 2) | row_mask(rmask, r16 + 3));
+                    km &= (1ull << kj) - 1;  // column blocks left of the panel (kj <= 63)
                     const int lr = lane & 15, lk = lane >> 4;
                     int ar[4];
 #pragma unroll
@@ -1458,11 +1475,14 @@ struct BodyWave {
                             for (int T = 0; T < 4; ++T) a[u][T] = col[ar[T]];
                         }
                     };
-                    load_group(0, ca, cb);
-                    for (int k0 = 0; k0 < j0; k0 += 16) {
-                        // The next group (the last group again at the end: every
-                        // load stays in bounds and unconditional).
-                        load_group(k0 + 16 < j0 ? k0 + 16 : k0, na, nb);
+                    int kb = km ? __builtin_ctzll(km) : 0;
+                    if (km) load_group(16 * kb, ca, cb);
+                    while (km) {
+                        // The next visited group (the same group again at the
+                        // end: every load stays in bounds and unconditional).
+                        km &= km - 1;
+                        kb = km ? __builtin_ctzll(km) : kb;
+                        load_group(16 * kb, na, nb);
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
                             const double bu = bon ? cb[u] : 0.0;
@@ -1554,6 +1574,14 @@ struct BodyWave {
                             q[c] = ljj;
                         }
                     }
+                }
+                if (mfma_left) {
+                    bool nz = false;
+#pragma unroll
+                    for (int c = 0; c < P; ++c) nz |= q[c] != 0.0;
+                    const unsigned long long bm = __ballot(nz);
+                    const int g = lane - (r0 >> 4);
+                    if (g >= 0 && g < 4 && ((bm >> (16 * g)) & 0xffffull) != 0) rmask |= 1ull << (j0 >> 4);
                 }
 #pragma unroll
                 for (int c = 0; c < P; ++c) {
