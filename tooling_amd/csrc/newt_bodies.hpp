@@ -1440,6 +1440,7 @@ struct BodyWave {
                 double acc[P];
 #pragma unroll
                 for (int c = 0; c < P; ++c) acc[c] = (c < pw && on) ? H[(size_t)ii * n + j0 + c] : 0.0;
+                double pr = on ? pt[ii] : 0.0;  // the row's diagonal sum, with its H entries
                 int k = 0;
                 if (mfma_left && j0 > 0) {
                     // The left part on the matrix cores (FP64/mixed only): the
@@ -1542,7 +1543,6 @@ struct BodyWave {
                 // chunk include the panel's own rows, whose entries are the
                 // pivots' and the later columns' multipliers.  A row's
                 // diagonal sum stays in a register while the panel runs.
-                double pr = on ? pt[ii] : 0.0;
                 double q[P];
 #pragma unroll
                 for (int c = 0; c < P; ++c) {
@@ -1596,7 +1596,9 @@ struct BodyWave {
                     for (int c = 0; c < P; ++c) q0[c] = q[c];
                 }
                 if (on) pt[i] = pr;
-                gsync();
+                // The panel's chunks read nothing their siblings write (other
+                // rows' pt, columns left of the panel): one fence per panel.
+                if (r0 + 64 >= n) gsync();
             }
         }
         const uint32_t max_it = m.iterations < 1 ? 1u : (uint32_t)m.iterations;
