@@ -1110,6 +1110,23 @@ struct BodyWave {
         }
         return s;
     }
+    // residual_of with the two bodies' orientations given.
+    __device__ __forceinline__ double residual_q(V3<double> dir, int a, int b, Qt<double> qa, Qt<double> qb,
+                                                 V3<double> arma, V3<double> armb, V3<double> dva, V3<double> dwa,
+                                                 V3<double> dvb, V3<double> dwb) const {
+        double s = 0.0;
+        if (a >= 0) {
+            V3<double> wa = qrot(qa, dwa);
+            V3<double> vp = dva + cross3(wa, arma);
+            s += dot3(vp, dir);
+        }
+        if (b >= 0) {
+            V3<double> wb = qrot(qb, dwb);
+            V3<double> vp = dvb + cross3(wb, armb);
+            s -= dot3(vp, dir);
+        }
+        return s;
+    }
     __device__ __forceinline__ void side_delta(int i, V3<double> dir, V3<double> arm, double df, V3<double>* dv,
                                                V3<double>* dwv) const {
         V3<double> f = dir * df;
@@ -1353,51 +1370,68 @@ struct BodyWave {
         // A[i][j] = response of row i to a unit impulse on row j; H = A + diag(reg).
         // One lane per column j; rows go by contact block (four rows share
         // their bodies).  A block with neither of column j's bodies keeps
-        // zero accumulators, whose residual is +0.0.  Every block's two body
-        // indices are loaded once (lane k % 64 holds block k's in slot k / 64,
-        // nblk <= 256) and broadcast in the loop, so no load waits behind the
-        // stores of H.  The column loop keeps every lane active (lanes past
-        // the last column work on column n - 1 and store nothing), so each
-        // broadcast reads a register every lane has written.
-        int ba[4], bb[4];
+        // zero accumulators, whose residual is +0.0.  Blocks go in chunks of
+        // 64: lane u loads block kc + u's row directions, arms, bodies and
+        // their orientations, and the column lanes take each block from
+        // broadcasts, so no load waits behind the stores of H.  The loops
+        // keep every lane active (lanes past the last column work on column
+        // n - 1 and store nothing), so each broadcast reads a register every
+        // lane has written.
+        for (int kc = 0; kc < nblk; kc += 64) {
+            const int kl = kc + lane < nblk ? kc + lane : nblk - 1;
+            const int la = bbi[2 * kl], lb = bbi[2 * kl + 1];
+            V3<double> ldir[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int k = 64 * t + lane;
-            ba[t] = k < nblk ? bbi[2 * k] : -1;
-            bb[t] = k < nblk ? bbi[2 * k + 1] : -1;
-        }
-        for (int j0 = 0; j0 < n; j0 += 64) {
-            const bool on = j0 + lane < n;
-            const int j = on ? j0 + lane : n - 1;
-            const int kj = j >> 2;
-            const int aj = bbi[2 * kj], bj = bbi[2 * kj + 1];
-            const V3<double> dir = rdir(j);
-            V3<double> dva = z3<double>(), dwa = z3<double>(), dvb = z3<double>(), dwb = z3<double>();
-            if (aj >= 0) side_delta(aj, dir, ld3(blk + 8 * kj), 1.0, &dva, &dwa);
-            if (bj >= 0) side_delta(bj, -dir, ld3(blk + 8 * kj + 3), 1.0, &dvb, &dwb);
-            for (int ki = 0; ki < nblk; ++ki) {
-                const int t = ki >> 6;
-                const int va = t == 0 ? ba[0] : t == 1 ? ba[1] : t == 2 ? ba[2] : ba[3];
-                const int vb = t == 0 ? bb[0] : t == 1 ? bb[1] : t == 2 ? bb[2] : bb[3];
-                const int ai = __builtin_amdgcn_readlane(va, ki & 63), bi = __builtin_amdgcn_readlane(vb, ki & 63);
-                V3<double> av = z3<double>(), aw = z3<double>(), bv = z3<double>(), bw = z3<double>();
-                bool any = false;
-                if (ai >= 0 && ai == aj) { av = dva; aw = dwa; any = true; }
-                if (ai >= 0 && ai == bj) { av = dvb; aw = dwb; any = true; }
-                if (bi >= 0 && bi == aj) { bv = dva; bw = dwa; any = true; }
-                if (bi >= 0 && bi == bj) { bv = dvb; bw = dwb; any = true; }
-                double a4[4] = {0.0, 0.0, 0.0, 0.0};
-                if (any) {
-                    const V3<double> arma = ld3(blk + 8 * ki), armb = ld3(blk + 8 * ki + 3);
+            for (int r = 0; r < 4; ++r) ldir[r] = rdir(4 * kl + r);
+            const V3<double> lam = ld3(blk + 8 * kl), lbm = ld3(blk + 8 * kl + 3);
+            const Qt<double> qid{0.0, 0.0, 0.0, 1.0};
+            const Qt<double> lqa = la >= 0 ? bori(la) : qid, lqb = lb >= 0 ? bori(lb) : qid;
+            const int kcnt = nblk - kc < 64 ? nblk - kc : 64;
+            for (int j0 = 0; j0 < n; j0 += 64) {
+                const bool on = j0 + lane < n;
+                const int j = on ? j0 + lane : n - 1;
+                const int kj = j >> 2;
+                const int aj = bbi[2 * kj], bj = bbi[2 * kj + 1];
+                const V3<double> dir = rdir(j);
+                V3<double> dva = z3<double>(), dwa = z3<double>(), dvb = z3<double>(), dwb = z3<double>();
+                if (aj >= 0) side_delta(aj, dir, ld3(blk + 8 * kj), 1.0, &dva, &dwa);
+                if (bj >= 0) side_delta(bj, -dir, ld3(blk + 8 * kj + 3), 1.0, &dvb, &dwb);
+                for (int u = 0; u < kcnt; ++u) {
+                    const int ki = kc + u;
+                    const int ai = __builtin_amdgcn_readlane(la, u), bi = __builtin_amdgcn_readlane(lb, u);
+                    V3<double> av = z3<double>(), aw = z3<double>(), bv = z3<double>(), bw = z3<double>();
+                    bool any = false;
+                    if (ai >= 0 && ai == aj) { av = dva; aw = dwa; any = true; }
+                    if (ai >= 0 && ai == bj) { av = dvb; aw = dwb; any = true; }
+                    if (bi >= 0 && bi == aj) { bv = dva; bw = dwa; any = true; }
+                    if (bi >= 0 && bi == bj) { bv = dvb; bw = dwb; any = true; }
+                    double a4[4] = {0.0, 0.0, 0.0, 0.0};
+                    if (__ballot(any) != 0) {
+                        auto bc3 = [&](V3<double> v) {
+                            return mk3(readlane_f64(v.x, u), readlane_f64(v.y, u), readlane_f64(v.z, u));
+                        };
+                        auto bcq = [&](Qt<double> q) {
+                            return Qt<double>{readlane_f64(q.x, u), readlane_f64(q.y, u), readlane_f64(q.z, u),
+                                              readlane_f64(q.w, u)};
+                        };
+                        const V3<double> arma = bc3(lam), armb = bc3(lbm);
+                        const Qt<double> qa = bcq(lqa), qb = bcq(lqb);
+                        V3<double> dr[4];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) a4[r] = residual_of(rdir(4 * ki + r), ai, bi, arma, armb, av, aw, bv, bw);
-                }
+                        for (int r = 0; r < 4; ++r) dr[r] = bc3(ldir[r]);
+                        if (any) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int i = 4 * ki + r;
-                    double a = a4[r];
-                    if (i == j) a += rows[7 * i + 3];
-                    if (on) H[(size_t)i * n + j] = a;
+                            for (int r = 0; r < 4; ++r)
+                                a4[r] = residual_q(dr[r], ai, bi, qa, qb, arma, armb, av, aw, bv, bw);
+                        }
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int i = 4 * ki + r;
+                        double a = a4[r];
+                        if (i == j) a += rows[7 * i + 3];
+                        if (on) H[(size_t)i * n + j] = a;
+                    }
                 }
             }
         }
@@ -1993,28 +2027,44 @@ struct BodyWave {
             gsync();
         }
         // Wrenches, one lane per body, rows in order (solver.rs:1198-1235).
+        // Rows go in chunks of 64: lane u forms row r0 + u's force and its
+        // torques about both contact points, then every body lane takes the
+        // chunk's rows in order from broadcasts and adds those on its body.
         const double dt = m.dt;
-        for (int b0 = lane; b0 < nb; b0 += 64) {
+        const int nr = 4 * nblk;
+        for (int b00 = 0; b00 < nb; b00 += 64) {
+            const int b0 = b00 + lane;
             V3<double> f = z3<double>(), t = z3<double>();
-            for (int ri = 0; ri < 4 * nblk; ++ri) {
+            for (int r0 = 0; r0 < nr; r0 += 64) {
+                const int ri = r0 + lane < nr ? r0 + lane : nr - 1;
                 const int k = ri >> 2;
-                const int a = bbi[2 * k], b = bbi[2 * k + 1];
-                if (a != b0 && b != b0) continue;
                 const double fi = rows[7 * ri + 6];
-                if (fi == 0.0) continue;
-                const V3<double> fa = rdir(ri) * (fi / dt);
-                if (a == b0) {
-                    f = f + fa;
-                    t = t + cross3(ld3(blk + 8 * k), fa);
-                }
-                if (b == b0) {
-                    const V3<double> fb = -fa;
-                    f = f + fb;
-                    t = t + cross3(ld3(blk + 8 * k + 3), fb);
+                const bool live = r0 + lane < nr && fi != 0.0;
+                const int ra = live ? bbi[2 * k] : -2, rb = live ? bbi[2 * k + 1] : -2;  // -2: no body
+                const V3<double> fa = rdir(ri) * (fi / dt), fb = -fa;
+                const V3<double> ta = cross3(ld3(blk + 8 * k), fa), tb = cross3(ld3(blk + 8 * k + 3), fb);
+                const int cnt = nr - r0 < 64 ? nr - r0 : 64;
+                for (int u = 0; u < cnt; ++u) {
+                    const int au = __builtin_amdgcn_readlane(ra, u), bu = __builtin_amdgcn_readlane(rb, u);
+                    if (au == -2) continue;  // uniform
+                    const V3<double> fau = mk3(readlane_f64(fa.x, u), readlane_f64(fa.y, u), readlane_f64(fa.z, u));
+                    const V3<double> tau = mk3(readlane_f64(ta.x, u), readlane_f64(ta.y, u), readlane_f64(ta.z, u));
+                    const V3<double> fbu = mk3(readlane_f64(fb.x, u), readlane_f64(fb.y, u), readlane_f64(fb.z, u));
+                    const V3<double> tbu = mk3(readlane_f64(tb.x, u), readlane_f64(tb.y, u), readlane_f64(tb.z, u));
+                    if (au == b0) {
+                        f = f + fau;
+                        t = t + tau;
+                    }
+                    if (bu == b0) {
+                        f = f + fbu;
+                        t = t + tbu;
+                    }
                 }
             }
-            st3(sd + B.wr + 6 * b0, f);
-            st3(sd + B.wr + 6 * b0 + 3, t);
+            if (b0 < nb) {
+                st3(sd + B.wr + 6 * b0, f);
+                st3(sd + B.wr + 6 * b0 + 3, t);
+            }
         }
         gsync();
         return 0;
