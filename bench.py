@@ -208,6 +208,11 @@ def bench_quadruped(args):
     log(f"  fp64:  {fp64['value']:.4g} world-steps/s ({fp64['ms_per_step']:.4f} ms/step); from the drop "
         f"{fp64['rollout']['value']:.4g}")
     qs, vs = mixed["warm_state"]
+    # The plain FP64 step: newt's operation order, one lane per world
+    # (bit-identical to the FP64 oracle), from the same settled states.
+    plain = time_gpu(model, args.worlds, abi.PRECISION_FP64_EXACT, args.steps, args.chunk, qs, vs)
+    log(f"  fp64 plain (newt's order, lane per world): {plain['value']:.4g} world-steps/s "
+        f"({plain['ms_per_step']:.4f} ms/step)")
     n_err = min(512, args.worlds)
     err = precision_errors(model, scene, qs, vs, n_err, abi.PRECISION_MIXED)
     log(f"  mixed vs FP64 oracle: qacc max {err['qacc']['max']:.3g} over {n_err} worlds")
@@ -235,7 +240,7 @@ def bench_quadruped(args):
                           f"the oracle's float instantiation (newt's f32 step, byte-identical to newt's goldens) "
                           f"on {th} host threads, one world per task ({dt:.1f} s)")
         log(f"  cpu: {rate:.4g} world-steps/s on {th} threads")
-    return dict(scene=scene, model=model, mixed=mixed, fp64=fp64, err=err, roofline=roofline,
+    return dict(scene=scene, model=model, mixed=mixed, fp64=fp64, plain=plain, err=err, roofline=roofline,
                 roofline_fp64=roofline_fp64, cpu=cpu, settle=settle)
 
 
@@ -481,7 +486,7 @@ def headline_line(args, q, extras):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (f32 Delassus rows)",
+        "dtype": "f64 state and factor; f32 Delassus rows, A = Y Y^T on MFMA and PGS sweeps",
         "data": "synthetic",
         "config": {"workload": f"{scene.name}: 18-DoF floating-base quadruped on a plane, {args.worlds} worlds, "
                                f"PGS {scene.options['iterations']} iterations, Euler dt {scene.options['dt']} s, "
@@ -491,12 +496,17 @@ def headline_line(args, q, extras):
                    "parallelism": "1 GPU, one wavefront per world"},
         "fp64": {"value": fp64["value"], "ms_per_step": fp64["ms_per_step"], "dtype": "f64",
                  "launch_ms": sum(fp64["launch_ms"]) / len(fp64["launch_ms"]), "rollout_from_drop": fp64["rollout"]},
+        "fp64_plain": {"value": q["plain"]["value"], "ms_per_step": q["plain"]["ms_per_step"], "dtype": "f64",
+                       "faults": q["plain"]["faults"],
+                       "what": "the plain FP64 step: newt's operation order (bit-identical to the FP64 oracle), "
+                               "one lane per world, timed from the same settled states"},
         "speedup_vs_fp64": mixed["value"] / fp64["value"],
+        "speedup_vs_fp64_plain": mixed["value"] / q["plain"]["value"],
         "rollout_from_drop": mixed["rollout"],
         "qacc_err": q["err"]["qacc"],
         "err_per_quantity": q["err"],
         "rollout_latency_ms": lat,
-        "faults": {"mixed": mixed["faults"], "fp64": fp64["faults"]},
+        "faults": {"mixed": mixed["faults"], "fp64": fp64["faults"], "fp64_plain": q["plain"]["faults"]},
         "roofline": q["roofline"],
         "roofline_fp64": q["roofline_fp64"],
         "cpu_baseline": q["cpu"],

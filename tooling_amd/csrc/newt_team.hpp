@@ -172,10 +172,17 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
     return __builtin_bit_cast(double, p);
 }
 
+__device__ __forceinline__ float readlane_f32(float v, int lane) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+__device__ __forceinline__ double readlane_r(double v, int lane) { return readlane_f64(v, lane); }
+__device__ __forceinline__ float readlane_r(float v, int lane) { return readlane_f32(v, lane); }
+
 // v, with x in lane i.  `lane` passes through an empty asm so the
 // compare is evaluated here: otherwise the compiler hoists one lane mask per
 // unrolled i out of the enclosing loops and spills them.
-__device__ __forceinline__ double put_lane(double v, double x, int lane, int i) {
+template <class P>
+__device__ __forceinline__ P put_lane(P v, P x, int lane, int i) {
     asm volatile("" : "+v"(lane));
     return lane == i ? x : v;
 }
@@ -1292,16 +1299,17 @@ struct Team {
     // every lane evaluates it for its own row, lane i's result is broadcast,
     // kept by lane i, and every residual moves by A_ji Δf_i.  Rows
     // past the real count carry c = d = 0 and a zero row of A, so they stay
-    // at zero and move nothing.
-    template <int NR>
-    __device__ __forceinline__ void sweeps(const double* a, double cf, double invd, double& r, double& f) const {
+    // at zero and move nothing.  FP64 in every mode (FP32 residuals put the
+    // mixed step's qacc over 1e-5 on box-box chains).
+    template <int NR, class P>
+    __device__ __forceinline__ void sweeps(const P* a, P cf, P invd, P& r, P& f) const {
         for (int it = 0; it < m.iterations; ++it) {
-            const double g = cf * f;  // f_i changes only at row i
+            const P g = cf * f;  // f_i changes only at row i
 #pragma unroll
             for (int i = 0; i < NR; ++i) {
-                double fn = g - invd * r;
-                fn = fn > 0.0 ? fn : 0.0;
-                const double dl = readlane_f64(fn - f, i);
+                P fn = g - invd * r;
+                fn = fn > P(0) ? fn : P(0);
+                const P dl = readlane_r(fn - f, i);
                 f = put_lane(f, fn, lane, i);
                 r = r + a[i] * dl;
             }
@@ -1309,13 +1317,74 @@ struct Team {
     }
 
     // Delassus + PGS (solver.rs:3045-3065, 3269-3331) for nr <= 64 rows; on
-    // return sd[u] = Y f/dt = L⁻¹ Jᵀ f/dt.
+    // return sd[u] = Y f/dt = L⁻¹ Jᵀ f/dt.  A = Y Yᵀ in FP64 sums: on the
+    // matrix cores up to 32 rows, on the vector units above.  The diagonal
+    // comes from the FP64 row; the sweeps and u = Y f/dt run in FP64.
+    // Row `lane` of A = Y Yᵀ for nr <= 16 NB rows, on the matrix cores:
+    // v_mfma_f64_16x16x4f64 on 16 x 16 tiles (I, J) over the dofs four at a
+    // time, FP64 sums of the stored rows (products of FP32 rows are exact).
+    // Lane l feeds Y[16 I + l % 16][k0 + l / 16] as the first operand and
+    // Y[16 J + l % 16][k0 + l / 16] as the second; register r of lane l then
+    // holds A[16 I + 4 r + l / 16][16 J + l % 16].  A is symmetric, so lane
+    // j = 16 J + m collects column j from the four lanes 16 p + m (p = 0..3):
+    // one shuffle per (J, p, I, r), kept by the lanes of row block J.
+    template <int NB>
+    __device__ __forceinline__ void delassus_mfma(int nr, double (&a)[64]) const {
+        typedef double f64x4 __attribute__((ext_vector_type(4)));
+        f64x4 t[NB][NB];
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int J = 0; J < NB; ++J) t[I][J] = f64x4{0.0, 0.0, 0.0, 0.0};
+        const int nv = m.nv, lr = lane & 15, lk = lane >> 4;
+        for (int k0 = 0; k0 < nv; k0 += 4) {
+            const int k = k0 + lk;
+            double y[NB];
+#pragma unroll
+            for (int I = 0; I < NB; ++I) {
+                const int row = 16 * I + lr;
+                y[I] = (row < nr && k < nv) ? (double)sJ[row * nv + k] : 0.0;
+            }
+#pragma unroll
+            for (int I = 0; I < NB; ++I)
+#pragma unroll
+                for (int J = 0; J < NB; ++J) t[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(y[I], y[J], t[I][J], 0, 0, 0);
+        }
+#pragma unroll
+        for (int c = 0; c < 64; ++c) a[c] = 0.0;
+#pragma unroll
+        for (int J = 0; J < NB; ++J) {
+            const bool keep = lk == J;
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                const int src = 16 * p + lr;
+#pragma unroll
+                for (int I = 0; I < NB; ++I) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const double x = __shfl(t[I][J][r], src);
+                        a[16 * I + 4 * r + p] = keep ? x : a[16 * I + 4 * r + p];
+                    }
+                }
+            }
+        }
+    }
+
     __device__ __forceinline__ void pgs_small(int nr) const {
         const int nv = m.nv;
         const bool active = lane < nr;
-        // Row `lane` of A = Y Yᵀ (FP64 sums of the stored R values), eight
-        // rows at a time.
         double a[64];
+        if (nr <= 16) {
+            delassus_mfma<1>(nr, a);
+            pgs_rows(nr, a);
+            return;
+        }
+        if (nr <= 32) {
+            delassus_mfma<2>(nr, a);
+            pgs_rows(nr, a);
+            return;
+        }
+        // Row `lane` of A = Y Yᵀ, eight rows at a time.
         const R* yl = sJ + (active ? lane : 0) * nv;
 #pragma unroll
         for (int ib = 0; ib < 64; ib += 8) {
@@ -1333,6 +1402,13 @@ struct Team {
 #pragma unroll
             for (int ii = 0; ii < 8; ++ii) a[ib + ii] = (active && ib + ii < nr) ? acc[ii] : 0.0;
         }
+        pgs_rows(nr, a);
+    }
+
+    // The sweeps on row `lane` of A, then u = Y f/dt.
+    __device__ __forceinline__ void pgs_rows(int nr, const double (&a)[64]) const {
+        const int nv = m.nv;
+        const bool active = lane < nr;
         const double diag = active ? sd[T.rdg + lane] : 0.0;
         const double reg = active ? sd[T.rreg + lane] : 0.0;
         const double invd = active ? 1.0 / (diag + reg) : 0.0;
