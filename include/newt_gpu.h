@@ -352,6 +352,62 @@ newt_status newt_data_pair_capacity(const newt_data* data, int32_t* cap);
 newt_status newt_data_get_pairs(const newt_data* data, int32_t* counts,
                                 int32_t* pairs);
 
+/* ---- sensors (newt/src/sensor.rs) ---------------------------------------- */
+
+/* SensorKind (newt/src/sensor.rs:153-196), numbered in newt's declaration
+ * order; the kinds this build evaluates. */
+typedef enum newt_sensor_kind {
+    NEWT_SENSOR_JOINT_POS = 0,      /* hinge angle / slide displacement, 1 */
+    NEWT_SENSOR_JOINT_VEL = 1,      /* hinge / slide rate, 1 */
+    NEWT_SENSOR_FRAME_POS = 4,      /* site position in world, 3 */
+    NEWT_SENSOR_FRAME_QUAT = 5,     /* site orientation in world (x, y, z, w), 4 */
+    NEWT_SENSOR_GYRO = 6,           /* parent angular velocity in the site frame, 3 */
+    NEWT_SENSOR_ACCELEROMETER = 7,  /* specific force at the site, site frame, 3 */
+    NEWT_SENSOR_TOUCH = 8,          /* sum of contact normal-force magnitudes on a geom, 1 */
+    NEWT_SENSOR_FORCE = 9,          /* parent-joint force on a link, child frame, 3 */
+    NEWT_SENSOR_TORQUE = 10,        /* parent-joint torque at the joint anchor, child frame, 3 */
+    NEWT_SENSOR_VELOCIMETER = 13,   /* site linear velocity in the site frame, 3 */
+    NEWT_SENSOR_MAGNETOMETER = 14,  /* World::magnetic_field in the site frame, 3 */
+    NEWT_SENSOR_SUBTREE_COM = 16,   /* centre of mass of a link's subtree, world, 3 */
+    NEWT_SENSOR_FRAME_LINVEL = 17,  /* site linear velocity in world, 3 */
+    NEWT_SENSOR_FRAME_ANGVEL = 18   /* parent angular velocity in world, 3 */
+} newt_sensor_kind;
+
+/* One Sensor (newt/src/sensor.rs:226-229) with its SiteFrame (sensor.rs:118-122).
+ * Joint, force, torque and subtree sensors name a link of the tree; site
+ * sensors name the link or the free body the site rides on (the other
+ * index -1), with the site's offset in that body's COM frame and the
+ * orientation mapping site vectors into it (x, y, z, w); touch names a geom. */
+typedef struct newt_sensor_desc {
+    int32_t kind;     /* newt_sensor_kind */
+    int32_t link;     /* tree link, or -1 */
+    int32_t body;     /* free body, or -1 */
+    int32_t geom;     /* touch: the geom, else -1 */
+    double offset[3];
+    double quat[4];
+} newt_sensor_desc;
+
+/* World::add_sensor (newt/src/world.rs:441; SensorBank::push, sensor.rs:395-401):
+ * the sensor's values follow the earlier sensors' in sensordata, in
+ * declaration order.  Accelerometer, force, torque and touch need the
+ * post-step accelerations and contact forces; this build evaluates them on
+ * tree models under the PGS solver (the step's start-of-step contacts
+ * re-solved at the post-step state, World::build_sensor_inputs,
+ * world.rs:1578-1696) or on tree models without contact pairs; elsewhere
+ * finalize returns NEWT_ERR_UNSUPPORTED.  Before newt_model_finalize. */
+newt_status newt_model_add_sensor(newt_model* model, const newt_sensor_desc* sensor, int32_t* sensor_index);
+/* World::magnetic_field (newt/src/world.rs:119), read by magnetometers.
+ * Before finalize. */
+newt_status newt_model_set_magnetic_field(newt_model* model, const double field[3]);
+/* Values per world (SensorBank::data length). */
+newt_status newt_model_sensor_dim(const newt_model* model, int32_t* dim);
+/* World::sensordata after the last newt_step (sensor::evaluate on the
+ * post-step state, sensor.rs:446-702), [n_worlds * dim]; zeros before the
+ * first step.  newt evaluates after every step and keeps the last values;
+ * here they are evaluated once after each newt_step call, which gives the
+ * same values.  Worlds frozen by a fault keep their previous values. */
+newt_status newt_data_get_sensordata(const newt_data* data, double* sensordata);
+
 /* ---- step --------------------------------------------------------------- */
 
 /* n_steps × World::step (newt/src/world.rs:1459) on every world of `data`,

@@ -335,6 +335,41 @@ std::vector<uint8_t> solver_incline(int mode = M_PGS) {
 
 }  // namespace
 
+// newt/tests/sensors_battery.rs:1056-1152: a pendulum tree and a tumbling
+// free body in one world (World::new: RK4, penalty solver, no geoms), one
+// sensor of each of six kinds; sensordata after steps 1, 100 and 500.
+std::vector<uint8_t> sensors_battery() {
+    World<F> w;
+    w.dt = 0.005f;
+    w.gravity = v3(0.0f, 0.0f, -9.81f);
+    Tree<F> tree;  // pendulum_tree (sensors_battery.rs:376-395)
+    tree.push_link(mk_link(-1, J_FIXED, vzero<F>(), qidentity<F>(), vzero<F>(), 1.0f, m3diag(1.0f, 1.0f, 1.0f)));
+    tree.push_link(hinge_link(0, v3(1.0f, 0.0f, 0.0f), vzero<F>(), v3(0.0f, 0.0f, 1.0f), 1.0f,
+                              m3diag(1e-3f, 1e-3f, 1e-3f)));
+    tree.q[tree.q_offset[1]] = 0.4f;      // set_hinge_angle(1, 0.4)
+    tree.qdot[tree.v_offset[1]] = -0.2f;  // set_hinge_rate(1, -0.2)
+    w.add_tree(tree, true);
+    Body<F> b = make_body(1.0f, m3diag(1.0f, 2.0f, 3.0f), v3(2.0f, 0.0f, 3.0f),
+                          from_axis_angle(v3(1.0f, 0.4f, -0.3f), 0.7f));
+    b.w = v3(0.6f, 0.9f, -0.4f);
+    b.v = v3(-0.1f, 0.2f, 0.05f);
+    w.add_body(b);
+    using W = World<F>;
+    w.add_sensor(W::SK_JPOS, 1, -1, -1, vzero<F>(), qidentity<F>());
+    w.add_sensor(W::SK_JVEL, 1, -1, -1, vzero<F>(), qidentity<F>());
+    w.add_sensor(W::SK_FPOS, 1, -1, -1, v3(0.0f, 0.0f, -0.5f), qidentity<F>());
+    w.add_sensor(W::SK_FQUAT, 1, -1, -1, vzero<F>(), qidentity<F>());
+    w.add_sensor(W::SK_GYRO, -1, 0, -1, v3(0.1f, 0.2f, 0.3f), qidentity<F>());
+    w.add_sensor(W::SK_ACCEL, -1, 0, -1, v3(0.1f, 0.2f, 0.3f), qidentity<F>());
+    std::vector<uint8_t> out;
+    int done = 0;
+    for (int at : {1, 100, 500}) {
+        for (; done < at; ++done) w.step();
+        for (F v : w.sensordata) put(out, v);
+    }
+    return out;
+}
+
 extern "C" int oracle_golden_bytes(const char* name, uint8_t* out, int cap) {
     std::vector<uint8_t> b;
     std::string n(name);
@@ -350,6 +385,7 @@ extern "C" int oracle_golden_bytes(const char* name, uint8_t* out, int cap) {
     else if (n == "solver_incline") b = solver_incline();
     else if (n == "solver_newton_stack") b = solver_stack(M_NEWTON);
     else if (n == "solver_newton_incline") b = solver_incline(M_NEWTON);
+    else if (n == "sensors_battery") b = sensors_battery();
     else return -1;
     if ((int)b.size() > cap) return -2;
     std::memcpy(out, b.data(), b.size());

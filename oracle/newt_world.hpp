@@ -1066,6 +1066,15 @@ template <class T> struct World {
     std::vector<Contact<T>> last_contacts;
     std::vector<T> last_qacc;  // flattened over trees
     std::vector<AbaWs<T>> ws;
+    // Sensors (World::sensors, world.rs:441; SensorBank, sensor.rs:380-410).
+    struct SensorSpec {
+        int kind, link, body, geom, out;  // link/body: the site's parent (one of them), tree 0
+        V3<T> off;
+        Q<T> quat;
+    };
+    std::vector<SensorSpec> sensors;
+    std::vector<T> sensordata;
+    V3<T> magnetic_field{T(0.0), T(0.0), T(0.0)};  // World::magnetic_field (world.rs:119)
 
     int add_tree(const Tree<T>& t, bool collide) {
         trees.push_back(t);
@@ -1189,7 +1198,7 @@ template <class T> struct World {
     struct Comp {  // WorldTreeJacobian (solver.rs:2768-2774)
         int tree, link;
         std::vector<T> coeff;
-        V3<T> wrench_linear;
+        V3<T> wrench_linear, wrench_angular;
     };
     struct Row {  // WorldContactRow (solver.rs:2746-2752)
         std::vector<Comp> comps;
@@ -1221,9 +1230,19 @@ template <class T> struct World {
 
     // solve_tree_contacts (solver.rs:2805-3366), PGS, trees only.
     // Returns tree_qfrc per tree.
-    std::vector<std::vector<T>> solve_tree_contacts(const std::vector<Contact<T>>& contacts, int tree_implicit /*-1 none,0,1*/) const {
+    // With `wrenches` / `normals`: also TreeContactSolution::tree_wrenches
+    // (per tree and link, world force and torque at the COM) and
+    // contact_normal_forces (per input contact), solver.rs:3334-3364.
+    std::vector<std::vector<T>> solve_tree_contacts(const std::vector<Contact<T>>& contacts, int tree_implicit /*-1 none,0,1*/,
+                                                    std::vector<std::vector<std::pair<V3<T>, V3<T>>>>* wrenches = nullptr,
+                                                    std::vector<T>* normals = nullptr) const {
         std::vector<std::vector<T>> qfrc(trees.size());
         for (size_t t = 0; t < trees.size(); ++t) qfrc[t].assign(trees[t].nv(), T(0.0));
+        if (wrenches) {
+            wrenches->assign(trees.size(), {});
+            for (size_t t = 0; t < trees.size(); ++t) (*wrenches)[t].assign(trees[t].links.size(), {vzero<T>(), vzero<T>()});
+        }
+        if (normals) normals->assign(contacts.size(), T(0.0));
         if (contacts.empty() || dt <= T(0.0) || trees.empty()) return qfrc;
         if (mode == M_NEWTON) fail("Newton tree-contact solve is not restated");
         if (cone != C_PYRAMIDAL) fail("elliptic cone is not restated");
@@ -1253,6 +1272,7 @@ template <class T> struct World {
             SolRef<T> solref;
             SolImp<T> solimp;
             T penetration, mu;
+            int contact;
         };
         std::vector<Row> rows;
         std::vector<Block> blocks;
@@ -1291,11 +1311,12 @@ template <class T> struct World {
                     cp.coeff.resize(jt.size());
                     for (size_t s = 0; s < jt.size(); ++s) cp.coeff[s] = dot(jt[s], dirs[k]) * sign;
                     cp.wrench_linear = dirs[k] * sign;
+                    cp.wrench_angular = cross(c.pos - lp.p, dirs[k] * sign);
                     r.comps.push_back(cp);
                 }
                 rows.push_back(r);
             }
-            blocks.push_back({start, sr, si, pen, mu});
+            blocks.push_back({start, sr, si, pen, mu, (int)ci});
         }
         if (rows.empty()) return qfrc;
         int nr = (int)rows.size();
@@ -1363,11 +1384,23 @@ template <class T> struct World {
                     T residual = (rows[i].bias + s.acc) + rows[i].reg * f[i];
                     f[i] = rmax(f[i] - residual / rows[i].diag, T(0.0));
                 }
-        // Force recovery (solver.rs:3345-3364).
+        // Force recovery (solver.rs:3334-3364).
+        if (normals)
+            for (const Block& b : blocks) {
+                T sum = T(-0.0);  // Iterator::sum over the block's four rows
+                for (int off = 0; off < 4; ++off) sum = sum + f[b.start + off];
+                (*normals)[b.contact] = sum / dt;
+            }
         for (int i = 0; i < nr; ++i) {
             T force = f[i] / dt;
-            for (const Comp& cp : rows[i].comps)
+            for (const Comp& cp : rows[i].comps) {
                 for (int s = 0; s < trees[cp.tree].nv(); ++s) qfrc[cp.tree][s] += cp.coeff[s] * force;
+                if (wrenches) {
+                    auto& w = (*wrenches)[cp.tree][cp.link];
+                    w.first = w.first + cp.wrench_linear * force;
+                    w.second = w.second + cp.wrench_angular * force;
+                }
+            }
         }
         return qfrc;
     }
@@ -1750,6 +1783,235 @@ template <class T> struct World {
     }
 
     // World::step (world.rs:1459-1537) for trees.
+    // ---------------- sensors (sensor.rs:446-702) ---------------------------
+    enum { SK_JPOS = 0, SK_JVEL = 1, SK_FPOS = 4, SK_FQUAT = 5, SK_GYRO = 6, SK_ACCEL = 7, SK_TOUCH = 8, SK_FORCE = 9,
+           SK_TORQUE = 10, SK_VELO = 13, SK_MAG = 14, SK_SUBCOM = 16, SK_FLINVEL = 17, SK_FANGVEL = 18 };
+    static int sensor_dim(int k) {  // SensorKind::dim (sensor.rs:200-221)
+        if (k == SK_JPOS || k == SK_JVEL || k == SK_TOUCH) return 1;
+        if (k == SK_FQUAT) return 4;
+        return 3;
+    }
+    void add_sensor(int kind, int link, int body, int geom, V3<T> off, Q<T> quat) {
+        int o = (int)sensordata.size();
+        sensors.push_back({kind, link, body, geom, o, off, quat});
+        for (int k = 0; k < sensor_dim(kind); ++k) sensordata.push_back(T(0.0));
+    }
+    // compute_link_va (sensor.rs:834-916): per link (v, a), body frame at the
+    // COM; hinge/slide by propagate_single_dof (sensor.rs:935-962).
+    std::vector<std::pair<Motion<T>, Motion<T>>> link_va(const Tree<T>& t, const std::vector<T>& qdd) const {
+        size_t n = t.links.size();
+        std::vector<std::pair<Motion<T>, Motion<T>>> out(n, {mzero<T>(), mzero<T>()});
+        for (size_t i = 0; i < n; ++i) {
+            const Link<T>& l = t.links[i];
+            if (l.joint == J_FREE) {
+                int o = t.v_offset[i];
+                out[i] = {Motion<T>{{t.qdot[o], t.qdot[o + 1], t.qdot[o + 2]}, {t.qdot[o + 3], t.qdot[o + 4], t.qdot[o + 5]}},
+                          Motion<T>{{qdd[o], qdd[o + 1], qdd[o + 2]}, {qdd[o + 3], qdd[o + 4], qdd[o + 5]}}};
+            } else if (l.joint == J_FIXED) {
+                GX<T> x = xup_fixed(l);
+                auto p = l.parent >= 0 ? out[l.parent] : std::make_pair(mzero<T>(), mzero<T>());
+                out[i] = {gx_motion(x, p.first), gx_motion(x, p.second)};
+            } else if (l.joint == J_HINGE || l.joint == J_SLIDE) {
+                GX<T> x = xup_joint(l, t.q[t.q_offset[i]]);
+                Motion<T> sub = l.joint == J_HINGE ? Motion<T>{l.axis, cross(l.cj_pos, l.axis)} : Motion<T>{vzero<T>(), l.axis};
+                auto p = out[l.parent];
+                Motion<T> sq = mscale(sub, t.qdot[t.v_offset[i]]);
+                Motion<T> sqq = mscale(sub, qdd[t.v_offset[i]]);
+                Motion<T> v = madd(gx_motion(x, p.first), sq);
+                Motion<T> c = cross_motion(v, sq);
+                Motion<T> a = madd(madd(gx_motion(x, p.second), sqq), c);
+                out[i] = {v, a};
+            } else {
+                fail("ball-joint sensors are not restated");
+            }
+        }
+        return out;
+    }
+    // compute_link_wrenches (sensor.rs:964-1028).
+    std::vector<Force<T>> link_wrenches(const Tree<T>& t, const std::vector<Pose<T>>& poses,
+                                        const std::vector<std::pair<Motion<T>, Motion<T>>>& va,
+                                        const std::vector<std::pair<V3<T>, V3<T>>>& ext) const {
+        size_t n = t.links.size();
+        std::vector<Force<T>> f(n, fzero<T>());
+        for (size_t i = 0; i < n; ++i) {
+            const Link<T>& l = t.links[i];
+            G6<T> im = spatial_inertia6(l.mass, l.inertia);
+            Force<T> ia = g6times(im, va[i].second), iv = g6times(im, va[i].first);
+            Force<T> bias = cross_force(va[i].first, iv);
+            V3<T> fw = ext[i].first + gravity * l.mass;
+            Force<T> fe{inverse_rotate(poses[i].q, ext[i].second), inverse_rotate(poses[i].q, fw)};
+            f[i] = fsub(fadd(ia, bias), fe);
+        }
+        for (size_t i = n; i-- > 1;) {
+            const Link<T>& l = t.links[i];
+            GX<T> x = l.joint == J_FREE ? gxidentity<T>() : l.joint == J_FIXED ? xup_fixed(l) : xup_joint(l, t.q[t.q_offset[i]]);
+            f[l.parent] = fadd(f[l.parent], gx_tforce(x, f[i]));
+        }
+        return f;
+    }
+    // World::build_sensor_inputs (world.rs:1578-1696) + sensor::evaluate, for
+    // the cases restated: the contact list empty, or (PGS) every contact on a
+    // tree; the contacts are the step's start-of-step set under PGS/Newton and
+    // the post-step detection under penalty (world.rs:1516-1522).
+    void evaluate_sensors(const std::vector<Contact<T>>& contacts) {
+        bool needs_qddot = false, needs_va = false, needs_f = false, touch = false;
+        for (const SensorSpec& s : sensors) {
+            touch = touch || s.kind == SK_TOUCH;
+            needs_qddot = needs_qddot || s.kind == SK_ACCEL || s.kind == SK_FORCE || s.kind == SK_TORQUE;
+            needs_va = needs_va || s.kind == SK_GYRO || s.kind == SK_VELO || s.kind == SK_FLINVEL || s.kind == SK_FANGVEL;
+            needs_f = needs_f || s.kind == SK_FORCE || s.kind == SK_TORQUE;
+        }
+        needs_va = needs_va || needs_qddot;
+        std::vector<std::vector<Pose<T>>> poses;
+        tree_poses(poses);
+        std::vector<std::vector<std::pair<V3<T>, V3<T>>>> twr(trees.size());
+        for (size_t t = 0; t < trees.size(); ++t) twr[t].assign(trees[t].links.size(), {vzero<T>(), vzero<T>()});
+        std::vector<T> normals(contacts.size(), T(0.0));
+        // The contact solve only feeds the acceleration, force and touch
+        // sensors (their inputs; the other kinds read the state alone).
+        if (!contacts.empty() && (needs_qddot || touch)) {
+            for (const Contact<T>& c : contacts)
+                if (!(geoms[c.geom_a].is_link() || geoms[c.geom_b].is_link()) || geoms[c.geom_a].is_body() ||
+                    geoms[c.geom_b].is_body())
+                    fail("sensors with free-body contacts are not restated");
+            if (mode != M_PGS) fail("sensors with penalty or Newton contacts are not restated");
+            int ti = integrator == I_RK4 ? -1 : integrator == I_IMPLICITFAST ? 1 : 0;
+            solve_tree_contacts(contacts, ti, &twr, &normals);
+        }
+        std::vector<std::vector<T>> qdd(trees.size());
+        std::vector<std::vector<std::pair<Motion<T>, Motion<T>>>> va(trees.size());
+        for (size_t t = 0; t < trees.size(); ++t) {
+            if (needs_qddot) {
+                AbaWs<T> tmp;
+                aba_tree(trees[t], poses[t], gravity, twr[t], VI_EXPLICIT, dt, tmp, qdd[t]);
+            } else {
+                qdd[t].assign(trees[t].nv(), T(0.0));
+            }
+            if (needs_va) va[t] = link_va(trees[t], qdd[t]);
+        }
+        std::vector<std::vector<Force<T>>> lf(trees.size());
+        if (needs_f)
+            for (size_t t = 0; t < trees.size(); ++t) lf[t] = link_wrenches(trees[t], poses[t], va[t], twr[t]);
+        // body accelerations from the (zero) body wrenches (sensor.rs:527-546)
+        auto body_accel = [&](const Body<T>& b, V3<T>* acom, V3<T>* alpha) {
+            const V3<T> fext = vzero<T>(), text = vzero<T>();
+            *acom = gravity + fext / b.mass;
+            V3<T> tau_body = inverse_rotate(b.q, text);
+            V3<T> iw = b.inertia * b.w;
+            V3<T> gyro = -cross(b.w, iw);
+            *alpha = b.inv_inertia * (tau_body + gyro);
+        };
+        auto site_pose = [&](const SensorSpec& s, V3<T>* pp, Q<T>* pq) {  // sensor.rs:794-808
+            if (s.body >= 0) { *pp = bodies[s.body].p; *pq = bodies[s.body].q; }
+            else { *pp = poses[0][s.link].p; *pq = poses[0][s.link].q; }
+        };
+        auto site_vel = [&](const SensorSpec& s, V3<T>* lin, V3<T>* ang) {  // sensor.rs:705-726
+            if (s.body >= 0) {
+                const Body<T>& b = bodies[s.body];
+                V3<T> angular = rotate(b.q, b.w);
+                V3<T> offw = rotate(b.q, s.off);
+                *lin = b.v + cross(angular, offw);
+                *ang = angular;
+            } else {
+                const Motion<T>& v = va[0][s.link].first;
+                const Q<T>& o = poses[0][s.link].q;
+                *ang = rotate(o, v.ang);
+                *lin = rotate(o, v.lin + cross(v.ang, s.off));
+            }
+        };
+        auto put3 = [&](int o, V3<T> v) { sensordata[o] = v.x; sensordata[o + 1] = v.y; sensordata[o + 2] = v.z; };
+        for (const SensorSpec& s : sensors) {
+            const int o = s.out;
+            switch (s.kind) {
+                case SK_JPOS: sensordata[o] = trees[0].q[trees[0].q_offset[s.link]]; break;
+                case SK_JVEL: sensordata[o] = trees[0].qdot[trees[0].v_offset[s.link]]; break;
+                case SK_FPOS: { V3<T> p; Q<T> q; site_pose(s, &p, &q); put3(o, p + rotate(q, s.off)); break; }
+                case SK_FQUAT: {
+                    V3<T> p; Q<T> q; site_pose(s, &p, &q);
+                    Q<T> r = q * s.quat;
+                    sensordata[o] = r.x; sensordata[o + 1] = r.y; sensordata[o + 2] = r.z; sensordata[o + 3] = r.w;
+                    break;
+                }
+                case SK_GYRO: {  // parent_body_velocities (sensor.rs:814-832)
+                    V3<T> w = s.body >= 0 ? bodies[s.body].w : va[0][s.link].first.ang;
+                    put3(o, inverse_rotate(s.quat, w));
+                    break;
+                }
+                case SK_ACCEL: {  // accelerometer_reading (sensor.rs:1030-1088)
+                    V3<T> pp, omega_world, alpha_world, a_com_world;
+                    Q<T> pq;
+                    site_pose(s, &pp, &pq);
+                    if (s.body >= 0) {
+                        const Body<T>& b = bodies[s.body];
+                        V3<T> ac, alpha_body;
+                        body_accel(b, &ac, &alpha_body);
+                        omega_world = rotate(b.q, b.w);
+                        alpha_world = rotate(b.q, alpha_body);
+                        a_com_world = ac;
+                    } else {
+                        const Motion<T>& v = va[0][s.link].first;
+                        const Motion<T>& a = va[0][s.link].second;
+                        omega_world = rotate(pq, v.ang);
+                        alpha_world = rotate(pq, a.ang);
+                        a_com_world = rotate(pq, a.lin + cross(v.ang, v.lin));
+                    }
+                    V3<T> r = rotate(pq, s.off);
+                    V3<T> a_site = (a_com_world + cross(alpha_world, r)) + cross(omega_world, cross(omega_world, r));
+                    V3<T> a_proper = a_site - gravity;
+                    put3(o, inverse_rotate(pq * s.quat, a_proper));
+                    break;
+                }
+                case SK_TOUCH: {  // touch_reading (sensor.rs:1103-1112)
+                    T sum = T(0.0);
+                    for (size_t c = 0; c < contacts.size(); ++c)
+                        if (contacts[c].geom_a == s.geom || contacts[c].geom_b == s.geom) sum += rabs(normals[c]);
+                    sensordata[o] = sum;
+                    break;
+                }
+                case SK_FORCE: put3(o, lf[0][s.link].lin); break;
+                case SK_TORQUE: {
+                    const Force<T>& f = lf[0][s.link];
+                    put3(o, f.tq - cross(trees[0].links[s.link].cj_pos, f.lin));
+                    break;
+                }
+                case SK_VELO: {
+                    V3<T> lin, ang, pp; Q<T> pq;
+                    site_vel(s, &lin, &ang);
+                    site_pose(s, &pp, &pq);
+                    put3(o, inverse_rotate(pq * s.quat, lin));
+                    break;
+                }
+                case SK_MAG: { V3<T> pp; Q<T> pq; site_pose(s, &pp, &pq); put3(o, inverse_rotate(pq * s.quat, magnetic_field)); break; }
+                case SK_SUBCOM: {  // subtree_com (sensor.rs:728-757)
+                    const Tree<T>& t = trees[0];
+                    V3<T> weighted = vzero<T>();
+                    T mass = T(0.0);
+                    for (size_t li = 0; li < t.links.size(); ++li) {
+                        int cur = (int)li;
+                        bool inc = false;
+                        while (cur >= 0) {
+                            if (cur == s.link) { inc = true; break; }
+                            cur = t.links[cur].parent;
+                        }
+                        if (inc) {
+                            weighted = weighted + poses[0][li].p * t.links[li].mass;
+                            mass += t.links[li].mass;
+                        }
+                    }
+                    put3(o, mass == T(0.0) ? vzero<T>() : weighted / mass);
+                    break;
+                }
+                case SK_FLINVEL: case SK_FANGVEL: {
+                    V3<T> lin, ang;
+                    site_vel(s, &lin, &ang);
+                    put3(o, s.kind == SK_FLINVEL ? lin : ang);
+                    break;
+                }
+                default: fail("sensor kind not restated");
+            }
+        }
+    }
+
     void step() {
         if (iterations == 0) fail("solver iterations must be >= 1");
         if (mode == M_NEWTON && cone == C_ELLIPTIC) fail("newton + elliptic");
@@ -1820,6 +2082,10 @@ template <class T> struct World {
         }
         if (!constraint && !pairs.empty() && trees.empty()) step_bodies_penalty(pairs);
         else step_bodies(bw);
+        // sensors on the post-step state (world.rs:1516-1522): the
+        // start-of-step contacts under PGS/Newton, a fresh detection under
+        // penalty.
+        if (!sensors.empty()) evaluate_sensors(constraint ? last_contacts : detect_contacts(pairs));
         refresh_broadphase();
     }
 };

@@ -40,6 +40,10 @@ struct Base {
     virtual void dynamics(int world, double* qacc, double* M, double* bias) const = 0;
     virtual int broadphase(int world, double* fat, double* bound) const = 0;
     virtual long long reinserts(int world, int reset) = 0;
+    virtual void add_sensor(const newt_sensor_desc* d) = 0;
+    virtual void set_magnetic(const double* f) = 0;
+    virtual int sensor_dim() const = 0;
+    virtual void get_sensordata(double* out) const = 0;
 };
 
 template <class T> struct Impl : Base {
@@ -125,6 +129,17 @@ template <class T> struct Impl : Base {
         proto.add_body(b);
     }
     void finalize(int n) override { worlds.assign(n, proto); }
+    void add_sensor(const newt_sensor_desc* d) override {
+        proto.add_sensor(d->kind, d->link, d->body, d->geom, v3(T(d->offset[0]), T(d->offset[1]), T(d->offset[2])),
+                         Q<T>{T(d->quat[0]), T(d->quat[1]), T(d->quat[2]), T(d->quat[3])});
+    }
+    void set_magnetic(const double* f) override { proto.magnetic_field = v3(T(f[0]), T(f[1]), T(f[2])); }
+    int sensor_dim() const override { return (int)proto.sensordata.size(); }
+    void get_sensordata(double* out) const override {
+        const int n = sensor_dim();
+        for (size_t w = 0; w < worlds.size(); ++w)
+            for (int k = 0; k < n; ++k) out[w * n + k] = (double)worlds[w].sensordata[k];
+    }
     int tree_nq() const {
         int s = 0;
         for (auto& t : proto.trees) s += t.nq();
@@ -340,6 +355,10 @@ int oracle_broadphase(void* h, int world, double* fat, double* bound) {
     return static_cast<Base*>(h)->broadphase(world, fat, bound);
 }
 long long oracle_reinserts(void* h, int world, int reset) { return static_cast<Base*>(h)->reinserts(world, reset); }
+void oracle_add_sensor(void* h, const newt_sensor_desc* d) { static_cast<Base*>(h)->add_sensor(d); }
+void oracle_set_magnetic(void* h, const double* f) { static_cast<Base*>(h)->set_magnetic(f); }
+int oracle_sensor_dim(void* h) { return static_cast<Base*>(h)->sensor_dim(); }
+void oracle_get_sensordata(void* h, double* out) { static_cast<Base*>(h)->get_sensordata(out); }
 // The dynamic tree's pair set for standalone boxes (DynamicAabbTree::insert
 // + compute_pairs, broadphase.rs:46-54, 384-455), in f32: every box is
 // fattened at insertion and the pairs are the sorted (a, b), a < b, whose

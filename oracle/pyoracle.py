@@ -56,6 +56,10 @@ def lib():
             "oracle_ncos64": (C.c_double, [C.c_double]),
             "oracle_nsin32": (C.c_float, [C.c_float]),
             "oracle_ncos32": (C.c_float, [C.c_float]),
+            "oracle_add_sensor": (None, [P, P]),
+            "oracle_set_magnetic": (None, [P, PD]),
+            "oracle_sensor_dim": (I, [P]),
+            "oracle_get_sensordata": (None, [P, PD]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -108,7 +112,13 @@ class OracleWorlds:
         self._geoms = scene.geom_structs()
         for g in self._geoms:
             L.oracle_add_geom(self.h, C.cast(C.pointer(g), C.c_void_p))
+        for sd in getattr(scene, "sensor_structs", lambda: [])():
+            L.oracle_add_sensor(self.h, C.cast(C.pointer(sd), C.c_void_p))
+        mf = tuple(getattr(scene, "magnetic_field", (0.0, 0.0, 0.0)))
+        if any(mf):
+            L.oracle_set_magnetic(self.h, (C.c_double * 3)(*mf))
         L.oracle_finalize(self.h, n_worlds)
+        self.sensor_dim = L.oracle_sensor_dim(self.h)
         self.nq = L.oracle_nq(self.h)
         self.nv = L.oracle_nv(self.h)
 
@@ -136,6 +146,12 @@ class OracleWorlds:
     def qfrc(self):
         out = np.zeros((self.n, self.nv))
         self.L.oracle_get_qfrc(self.h, _dp(out))
+        return out
+
+    def sensordata(self):
+        out = np.zeros((self.n, self.sensor_dim))
+        if self.sensor_dim:
+            self.L.oracle_get_sensordata(self.h, _dp(out))
         return out
 
     def step(self, n_steps=1, threads=1):

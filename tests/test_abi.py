@@ -38,7 +38,7 @@ def test_struct_layouts_match_the_header():
         pytest.skip("no C compiler")
     structs = {"newt_options": abi.Options, "newt_link_desc": abi.LinkDesc, "newt_geom_desc": abi.GeomDesc,
                "newt_body_desc": abi.BodyDesc, "newt_device_state": abi.DeviceState,
-               "newt_batch_options": abi.BatchOptions}
+               "newt_batch_options": abi.BatchOptions, "newt_sensor_desc": abi.SensorDesc}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} sizeof() %zu\\n", sizeof({cname}));')

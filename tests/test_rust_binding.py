@@ -31,7 +31,7 @@ def test_sys_rs_records_match_the_header_fields():
     text = open(SYS).read()
     records = {"NewtOptions": abi.Options, "NewtLinkDesc": abi.LinkDesc, "NewtGeomDesc": abi.GeomDesc,
                "NewtBodyDesc": abi.BodyDesc, "NewtDeviceState": abi.DeviceState,
-               "NewtBatchOptions": abi.BatchOptions}
+               "NewtBatchOptions": abi.BatchOptions, "NewtSensorDesc": abi.SensorDesc}
     for name, py in records.items():
         m = re.search(r"pub struct " + name + r" \{(.*?)\n\}", text, re.S)
         assert m, name

@@ -104,6 +104,28 @@ class BodyDesc(C.Structure):
     ]
 
 
+class SensorDesc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("link", C.c_int32),
+        ("body", C.c_int32),
+        ("geom", C.c_int32),
+        ("offset", D3),
+        ("quat", D4),
+    ]
+
+
+# newt_sensor_kind
+(SENSOR_JOINT_POS, SENSOR_JOINT_VEL, SENSOR_FRAME_POS, SENSOR_FRAME_QUAT, SENSOR_GYRO, SENSOR_ACCELEROMETER,
+ SENSOR_TOUCH, SENSOR_FORCE, SENSOR_TORQUE, SENSOR_VELOCIMETER, SENSOR_MAGNETOMETER, SENSOR_SUBTREE_COM,
+ SENSOR_FRAME_LINVEL, SENSOR_FRAME_ANGVEL) = (0, 1, 4, 5, 6, 7, 8, 9, 10, 13, 14, 16, 17, 18)
+SENSOR_DIMS = {SENSOR_JOINT_POS: 1, SENSOR_JOINT_VEL: 1, SENSOR_TOUCH: 1, SENSOR_FRAME_QUAT: 4}
+
+
+def sensor_dim(kind: int) -> int:
+    return SENSOR_DIMS.get(kind, 3)
+
+
 class DeviceState(C.Structure):
     _fields_ = [
         ("qpos", C.c_void_p),
@@ -177,6 +199,10 @@ SIGNATURES = {
     "newt_batch_destroy": (None, [_P]),
     "newt_step_noalloc": (_I, [_P, C.POINTER(DeviceState), _I, _I, _I, _I, _P]),
     "newt_noalloc_max_worlds": (_I, [_P, _I, _PI]),
+    "newt_model_add_sensor": (_I, [_P, C.POINTER(SensorDesc), _PI]),
+    "newt_model_set_magnetic_field": (_I, [_P, _PD]),
+    "newt_model_sensor_dim": (_I, [_P, _PI]),
+    "newt_data_get_sensordata": (_I, [_P, _PD]),
 }
 
 _lib = None

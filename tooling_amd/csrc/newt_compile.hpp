@@ -31,6 +31,9 @@ struct HostModel {
     std::vector<newt_body_desc> bodies;
     std::vector<BodyT> hbodies;
     int contact_cap = 0;  // most contacts per world (0: the sum of the pairs' maxima)
+    std::vector<newt_sensor_desc> sensors;
+    std::vector<SensorT> hsensors;
+    double magnetic[3] = {0.0, 0.0, 0.0};
 };
 
 inline newt_status cfail(std::string* err, newt_status s, const std::string& msg) {
@@ -219,6 +222,60 @@ inline newt_status compile_bodies(HostModel* M, std::string* err) {
     d.pairs = M->pairs.data();
     d.n_bodies = nb;
     d.bodies = M->hbodies.data();
+    return NEWT_OK;
+}
+
+// Sensors (World::add_sensor, world.rs:441; Sensor::validate, sensor.rs:248-323)
+// into the device table, with each sensor's first value in sensordata.
+// Called after the model's trees or bodies are compiled.
+inline newt_status compile_sensors(HostModel* M, std::string* err) {
+    ModelT& d = M->dev;
+    M->hsensors.clear();
+    int out = 0;
+    bool dyn = false;
+    for (size_t k = 0; k < M->sensors.size(); ++k) {
+        const newt_sensor_desc& s = M->sensors[k];
+        const std::string who = "sensor " + std::to_string(k) + ": ";
+        const int dim = sensor_kind_dim(s.kind);
+        if (dim == 0) return cfail(err, NEWT_ERR_UNSUPPORTED, who + "this sensor kind is not built");
+        const bool link_ok = s.link >= 0 && s.link < d.n_links;
+        const bool body_ok = s.body >= 0 && s.body < d.n_bodies;
+        if (s.kind == SK_JOINT_POS || s.kind == SK_JOINT_VEL) {
+            if (!link_ok) return cfail(err, NEWT_ERR_INVALID_CONFIG, who + "link out of range");
+            const int j = M->links[s.link].joint;
+            if (j != JF_HINGE && j != JF_SLIDE)
+                return cfail(err, NEWT_ERR_INVALID_CONFIG, who + "joint sensors need a hinge or slide joint");
+        } else if (s.kind == SK_FORCE || s.kind == SK_TORQUE || s.kind == SK_SUBTREE_COM) {
+            if (!link_ok) return cfail(err, NEWT_ERR_INVALID_CONFIG, who + "link out of range");
+        } else if (s.kind == SK_TOUCH) {
+            if (s.geom < 0 || s.geom >= d.n_geoms) return cfail(err, NEWT_ERR_INVALID_CONFIG, who + "geom out of range");
+        } else if (sensor_kind_site(s.kind)) {
+            if (link_ok == body_ok || (s.link >= 0 && !link_ok) || (s.body >= 0 && !body_ok))
+                return cfail(err, NEWT_ERR_INVALID_CONFIG, who + "a site rides on exactly one link or free body");
+        }
+        if (sensor_kind_dynamic(s.kind)) {
+            dyn = true;
+            if (d.n_bodies > 0)
+                return cfail(err, NEWT_ERR_UNSUPPORTED, who + "acceleration, force and touch sensors are built for trees only");
+            if (d.n_pairs > 0 && d.mode != NEWT_SOLVER_PGS)
+                return cfail(err, NEWT_ERR_UNSUPPORTED,
+                             who + "acceleration, force and touch sensors with contacts are built for the PGS solver only");
+        }
+        SensorT t{};
+        t.kind = s.kind;
+        t.link = s.link;
+        t.body = s.body;
+        t.geom = s.geom;
+        t.out = out;
+        for (int i = 0; i < 3; ++i) t.off[i] = s.offset[i];
+        for (int i = 0; i < 4; ++i) t.quat[i] = s.quat[i];
+        M->hsensors.push_back(t);
+        out += dim;
+    }
+    d.n_sensors = (int)M->hsensors.size();
+    d.sensor_dim = out;
+    d.sensor_dyn = dyn ? 1 : 0;
+    for (int i = 0; i < 3; ++i) d.magnetic[i] = M->magnetic[i];
     return NEWT_OK;
 }
 
@@ -499,6 +556,12 @@ inline Layout make_layout(const ModelT& m, int real_bytes) {
     L.lf = D(std::max(nr, 1));
     L.lc = D(std::max(nr, 1));
     L.lvi = D(std::max(nr, 1));
+    const bool sens = m.n_sensors > 0;
+    L.swr = sens ? D(6 * n) : 0;
+    L.sva = sens ? D(12 * n) : 0;
+    L.sfl = sens ? D(6 * n) : 0;
+    L.sqdd = sens ? D(nv) : 0;
+    L.sfn = sens ? D(std::max(m.cap_c, 1)) : 0;
     L.total_d = dd;
     L.cids = I(3 * m.cap_c);
     L.rinfo = I(3 * m.cap_r);

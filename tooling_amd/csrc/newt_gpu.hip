@@ -152,6 +152,32 @@ __global__ void __launch_bounds__(64) k_bodies_wave(ModelT m, BodyWaveLayout B, 
     }
 }
 
+// Sensors on the post-step state, one lane per world (newt_sensors.hpp).
+// Parked and faulted worlds keep their previous values.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_sensors(ModelT m, Layout L, DataT d, R* ws, double* dws, int* iws) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= d.W || d.fault[w] != 0) return;
+    World<R> wd = world_of<R>(m, L, d, ws, dws, iws, w);
+    const int nc = m.n_bodies == 0 ? min(max(d.ncontact[w], 0), m.cap_c) : 0;
+    SensorEval<R> se{wd, m, L, {d.sensordata, d.W, w}};
+    se.evaluate(nc);
+}
+
+newt_status launch_sensors(newt_data* D, hipStream_t stream) {
+    const newt_model* M = D->model;
+    if (M->dev.n_sensors == 0) return NEWT_OK;
+    dim3 grid((D->W + kBlock - 1) / kBlock), block(kBlock);
+    const bool ws32 = D->precision == NEWT_PRECISION_MIXED && M->dev.n_bodies == 0 && !M->dev.penalty_limits;
+    if (ws32)
+        hipLaunchKernelGGL(k_sensors<float>, grid, block, 0, stream, M->dev, D->L, D->d, (float*)D->ws, D->dws, D->iws);
+    else
+        hipLaunchKernelGGL(k_sensors<double>, grid, block, 0, stream, M->dev, D->L, D->d, (double*)D->ws, D->dws,
+                           D->iws);
+    HIP_TRY(hipGetLastError());
+    return NEWT_OK;
+}
+
 newt_status launch_persistent(newt_data* D, int n_steps, hipStream_t stream) {
     const newt_model* M = D->model;
     const int wpb = D->wpb;
@@ -270,6 +296,28 @@ newt_status newt_model_add_geom(newt_model* model, const newt_geom_desc* geom, i
     return NEWT_OK;
 }
 
+newt_status newt_model_add_sensor(newt_model* model, const newt_sensor_desc* sensor, int32_t* sensor_index) {
+    if (!model || !sensor) return fail(NEWT_ERR_INVALID_ARG, "null argument");
+    if (model->finalized) return fail(NEWT_ERR_INVALID_ARG, "model already finalized");
+    if (sensor_index) *sensor_index = (int32_t)model->h.sensors.size();
+    model->h.sensors.push_back(*sensor);
+    return NEWT_OK;
+}
+
+newt_status newt_model_set_magnetic_field(newt_model* model, const double field[3]) {
+    if (!model || !field) return fail(NEWT_ERR_INVALID_ARG, "null argument");
+    if (model->finalized) return fail(NEWT_ERR_INVALID_ARG, "model already finalized");
+    for (int i = 0; i < 3; ++i) model->h.magnetic[i] = field[i];
+    return NEWT_OK;
+}
+
+newt_status newt_model_sensor_dim(const newt_model* model, int32_t* dim) {
+    if (!model || !dim) return fail(NEWT_ERR_INVALID_ARG, "null argument");
+    if (!model->finalized) return fail(NEWT_ERR_INVALID_ARG, "model is not finalized");
+    *dim = model->dev.sensor_dim;
+    return NEWT_OK;
+}
+
 newt_status newt_model_finalize(newt_model* model) {
     if (!model) return fail(NEWT_ERR_INVALID_ARG, "null model");
     if (model->finalized) return NEWT_OK;
@@ -281,6 +329,12 @@ newt_status newt_model_finalize(newt_model* model) {
         H.dev.cap_c = H.contact_cap;
         H.dev.cap_r = 4 * H.contact_cap;
     }
+    s = compile_sensors(&H, &err);
+    if (s != NEWT_OK) return fail(s, err);
+    HIP_TRY(hipMalloc(&model->d_sensors, sizeof(SensorT) * std::max<size_t>(1, H.hsensors.size())));
+    if (!H.hsensors.empty())
+        HIP_TRY(hipMemcpy(model->d_sensors, H.hsensors.data(), sizeof(SensorT) * H.hsensors.size(),
+                          hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&model->d_links, sizeof(LinkT) * std::max<size_t>(1, H.links.size())));
     HIP_TRY(hipMalloc(&model->d_geoms, sizeof(GeomT) * std::max<size_t>(1, H.hgeoms.size())));
     HIP_TRY(hipMalloc(&model->d_pairs, sizeof(PairT) * std::max<size_t>(1, H.pairs.size())));
@@ -326,6 +380,7 @@ newt_status newt_model_finalize(newt_model* model) {
     model->dev.dof_link = model->d_dof_link;
     model->dev.limits = model->d_limits;
     model->dev.bodies = model->d_bodies;
+    model->dev.sensors = model->d_sensors;
     if (H.dev.n_bodies == 0) {
         const ModelT& hd = H.dev;
         CacheLayout C = make_cache_layout(hd);
@@ -378,6 +433,7 @@ void newt_model_destroy(newt_model* model) {
     (void)hipFree(model->d_limits);
     (void)hipFree(model->d_bodies);
     (void)hipFree(model->d_blob);
+    (void)hipFree(model->d_sensors);
     delete model;
 }
 
@@ -433,7 +489,8 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
               alloc((void**)&D->d.ncontact, W * 4) && alloc((void**)&D->d.npair, W * 4) &&
               alloc(&D->ws, W * D->L.total_r * rsz) && alloc((void**)&D->dws, W * D->L.total_d * 8) &&
               alloc((void**)&D->iws, W * D->L.total_i * 4) &&
-              alloc((void**)&D->ovf, W * (size_t)std::max(TL.ov_stride, 16));
+              alloc((void**)&D->ovf, W * (size_t)std::max(TL.ov_stride, 16)) &&
+              alloc((void**)&D->d.sensordata, W * (size_t)std::max(m.sensor_dim, 1) * 8);
     const bool newton = bodies && m.mode == NEWT_SOLVER_NEWTON;
     // Penalty contacts run on the lane step only.
     const bool penalty_contacts = m.mode == NEWT_SOLVER_PENALTY && m.n_pairs > 0;
@@ -472,6 +529,7 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
     HIP_TRY(hipMemset(D->d.ncontact, 0, W * 4));
     HIP_TRY(hipMemset(D->d.npair, 0, W * 4));
     HIP_TRY(hipMemset(D->iws, 0, W * D->L.total_i * 4));
+    HIP_TRY(hipMemset(D->d.sensordata, 0, W * (size_t)std::max(m.sensor_dim, 1) * 8));
     *out = D;
     // Default state, then the fat bounds World::add_geom builds from it.
     newt_status s = newt_data_set_state(D, q.data(), qd.data());
@@ -501,6 +559,7 @@ void newt_data_destroy(newt_data* D) {
     (void)hipFree(D->d.fault);
     (void)hipFree(D->d.ncontact);
     (void)hipFree(D->d.npair);
+    (void)hipFree(D->d.sensordata);
     (void)hipFree(D->ws);
     (void)hipFree(D->dws);
     (void)hipFree(D->iws);
@@ -672,6 +731,16 @@ newt_status newt_data_get_qacc(const newt_data* D, double* qacc) {
 newt_status newt_data_get_faults(const newt_data* D, int32_t* faults) {
     if (!D || !faults) return fail(NEWT_ERR_INVALID_ARG, "null argument");
     HIP_TRY(d2h(D, faults, D->d.fault, (size_t)D->W * 4));
+    return NEWT_OK;
+}
+
+newt_status newt_data_get_sensordata(const newt_data* D, double* sensordata) {
+    if (!D || !sensordata) return fail(NEWT_ERR_INVALID_ARG, "null argument");
+    const int dim = D->model->dev.sensor_dim;
+    if (dim == 0) return NEWT_OK;
+    std::vector<double> tmp((size_t)D->W * dim);
+    HIP_TRY(d2h(D, tmp.data(), D->d.sensordata, tmp.size() * 8));
+    from_lanes(tmp, sensordata, D->W, dim);
     return NEWT_OK;
 }
 
@@ -871,7 +940,9 @@ newt_status newt_step(const newt_model* model, newt_data* data, int32_t n_steps,
     if (!model || !data || n_steps < 0) return fail(NEWT_ERR_INVALID_ARG, "bad step arguments");
     if (data->model != model) return fail(NEWT_ERR_INVALID_ARG, "data was created for another model");
     if (n_steps == 0) return NEWT_OK;
-    return launch(data, n_steps, (hipStream_t)stream, 0);
+    newt_status st = launch(data, n_steps, (hipStream_t)stream, 0);
+    if (st != NEWT_OK) return st;
+    return launch_sensors(data, (hipStream_t)stream);
 }
 
 newt_status newt_data_synchronize(newt_data* data) {

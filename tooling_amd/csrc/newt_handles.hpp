@@ -12,6 +12,7 @@
 #include "newt_step.hpp"
 #include "newt_team.hpp"
 #include "newt_bodies.hpp"
+#include "newt_sensors.hpp"
 
 namespace newt {
 // Defined in newt_team.hip (the persistent-step instantiations).
@@ -48,6 +49,7 @@ struct newt_model {
     int* d_dof_link = nullptr;
     newt::LimitT* d_limits = nullptr;
     newt::BodyT* d_bodies = nullptr;
+    newt::SensorT* d_sensors = nullptr;
     newt::CacheLayout cache{};  // model tables as the persistent step caches them in LDS
     char* d_blob = nullptr;     // the same tables packed in that layout
 };

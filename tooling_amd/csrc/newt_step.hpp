@@ -128,6 +128,10 @@ template <class R> struct World {
     Lane<double> dw;
     Lane<int> iw;
     Lane<double> qpos, qvel, qacc, qfrc, fat;
+    // Workspace offset of per-link external world wrenches (force, torque;
+    // 6 per link) that aba applies, -1 for none; the sensors' post-step ABA
+    // sets it.  Penalty contacts use their own wrenches (L.rbias).
+    int ext_wr = -1;
 
     // ---------------- state access -----------------------------------------
     NEWT_HD R q(int k) const { return (R)qpos[k]; }
@@ -695,6 +699,9 @@ template <class R> struct World {
             if (penalty_contacts()) {
                 ef = kld3(L.rbias + 6 * i);
                 et = kld3(L.rbias + 6 * i + 3);
+            } else if (ext_wr >= 0) {
+                ef = kld3(ext_wr + 6 * i);
+                et = kld3(ext_wr + 6 * i + 3);
             }
             V3<double> fbd = qrot_inv(ori(i), ef + z3<double>() + gd * l.mass);
             V3<double> tbd = qrot_inv(ori(i), et + z3<double>());

@@ -61,6 +61,44 @@ struct PairT {
     int a, b;
 };
 
+enum : int {
+    SK_JOINT_POS = 0, SK_JOINT_VEL = 1, SK_FRAME_POS = 4, SK_FRAME_QUAT = 5, SK_GYRO = 6, SK_ACCEL = 7,
+    SK_TOUCH = 8, SK_FORCE = 9, SK_TORQUE = 10, SK_VELOCIMETER = 13, SK_MAGNETOMETER = 14, SK_SUBTREE_COM = 16,
+    SK_FRAME_LINVEL = 17, SK_FRAME_ANGVEL = 18
+};
+
+// SensorKind::dim (sensor.rs:200-221); 0 for kinds this build does not
+// evaluate.
+inline __host__ __device__ int sensor_kind_dim(int kind) {
+    switch (kind) {
+        case SK_JOINT_POS: case SK_JOINT_VEL: case SK_TOUCH: return 1;
+        case SK_FRAME_QUAT: return 4;
+        case SK_FRAME_POS: case SK_GYRO: case SK_ACCEL: case SK_FORCE: case SK_TORQUE: case SK_VELOCIMETER:
+        case SK_MAGNETOMETER: case SK_SUBTREE_COM: case SK_FRAME_LINVEL: case SK_FRAME_ANGVEL: return 3;
+        default: return 0;
+    }
+}
+// Kinds that read the post-step accelerations or contact forces.
+inline __host__ __device__ bool sensor_kind_dynamic(int kind) {
+    return kind == SK_ACCEL || kind == SK_FORCE || kind == SK_TORQUE || kind == SK_TOUCH;
+}
+inline __host__ __device__ bool sensor_kind_site(int kind) {
+    return kind == SK_FRAME_POS || kind == SK_FRAME_QUAT || kind == SK_GYRO || kind == SK_ACCEL ||
+           kind == SK_VELOCIMETER || kind == SK_MAGNETOMETER || kind == SK_FRAME_LINVEL || kind == SK_FRAME_ANGVEL;
+}
+// Kinds that read the links' spatial velocities (sensor.rs:463-474).
+inline __host__ __device__ bool sensor_kind_va(int kind) {
+    return kind == SK_GYRO || kind == SK_ACCEL || kind == SK_FORCE || kind == SK_TORQUE || kind == SK_VELOCIMETER ||
+           kind == SK_FRAME_LINVEL || kind == SK_FRAME_ANGVEL;
+}
+
+// A sensor (sensor.rs:226-229) and its site frame (sensor.rs:118-122):
+// `out` is its first value in a world's sensordata.
+struct SensorT {
+    int kind, link, body, geom, out, pad;
+    double off[3], quat[4];
+};
+
 struct ModelT {
     int n_links, nq, nv, n_geoms;
     int n_pairs, cap_c, cap_r, n_chain;
@@ -86,6 +124,12 @@ struct ModelT {
     int n_bodies;  // free bodies (a model has a tree or free bodies, not both)
     int penalty_limits;  // ranged joints under the penalty solver: the lane-per-world step only
     const BodyT* bodies;
+    // Sensors, evaluated after the last step of a launch (sensor.rs:446-702).
+    const SensorT* sensors;
+    int n_sensors, sensor_dim;
+    int sensor_dyn;  // some sensor needs the post-step accelerations or contact forces
+    int pad2;
+    double magnetic[3];  // World::magnetic_field
 };
 
 // Offsets (in elements) of the per-world scratch regions.
@@ -105,6 +149,9 @@ struct Layout {
     // impulse; per contact block the two arms (6) and the two facet-pair
     // cross responses
     int bwr, biw, bbd, brow, bblk;
+    // sensors: per-link contact wrenches (6), v and a (12) and joint
+    // wrenches (6); qddot [nv]; per-contact normal forces
+    int swr, sva, sfl, sqdd, sfn;
     int total_d;
     // integer workspace
     int cids, rinfo, pairs, iwf;
@@ -127,6 +174,7 @@ struct DataT {
     // first step (the refresh the state setters make, world.rs:1119-1136)
     // instead of continuing from `fat`.  Read by the persistent step.
     int init_bounds;
+    double* sensordata;  // [sensor_dim][W]
 };
 
 }  // namespace newt

@@ -37,7 +37,15 @@ class Model:
             abi.check(self.lib.newt_model_add_geom(h, C.byref(g), None))
         if contact_capacity is not None:
             abi.check(self.lib.newt_model_set_contact_capacity(h, int(contact_capacity)))
+        for sd in scene.sensor_structs():
+            abi.check(self.lib.newt_model_add_sensor(h, C.byref(sd), None))
+        if any(scene.magnetic_field):
+            mf = (C.c_double * 3)(*scene.magnetic_field)
+            abi.check(self.lib.newt_model_set_magnetic_field(h, mf))
         abi.check(self.lib.newt_model_finalize(h))
+        dim = C.c_int32()
+        abi.check(self.lib.newt_model_sensor_dim(h, C.byref(dim)))
+        self.sensor_dim = dim.value
         nq, nv, nl, ng = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
         abi.check(self.lib.newt_model_dims(h, C.byref(nq), C.byref(nv), C.byref(nl), C.byref(ng)))
         self.nq, self.nv, self.n_links, self.n_geoms = nq.value, nv.value, nl.value, ng.value
@@ -127,6 +135,13 @@ class Data:
     def qacc(self):
         out = np.zeros((self.n_worlds, self.model.nv))
         abi.check(self.lib.newt_data_get_qacc(self.handle, self._dp(out)))
+        return out
+
+    def sensordata(self):
+        """World::sensordata after the last step, [n_worlds, sensor_dim]."""
+        out = np.zeros((self.n_worlds, max(self.model.sensor_dim, 0)))
+        if self.model.sensor_dim:
+            abi.check(self.lib.newt_data_get_sensordata(self.handle, self._dp(out)))
         return out
 
     def faults(self):

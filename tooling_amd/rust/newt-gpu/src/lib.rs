@@ -216,6 +216,15 @@ impl Model {
         for g in world.geoms.iter() {
             check(unsafe { newt_model_add_geom(m.raw, &lower_geom(g), std::ptr::null_mut()) })?;
         }
+        for s in world.sensors.sensors.iter() {
+            let d = lower_sensor(&s.kind).ok_or_else(|| NewtError {
+                status: NEWT_ERR_UNSUPPORTED,
+                message: format!("sensor {:?} is not built", s.name),
+            })?;
+            check(unsafe { newt_model_add_sensor(m.raw, &d, std::ptr::null_mut()) })?;
+        }
+        let mf = v3(world.magnetic_field);
+        check(unsafe { newt_model_set_magnetic_field(m.raw, mf.as_ptr()) })?;
         m.finalize()
     }
 
@@ -235,6 +244,37 @@ impl Model {
         self.nv = nv as usize;
         Ok(self)
     }
+}
+
+/// newt's Sensor (sensor.rs:153-196, tree 0) as the C record; None for
+/// kinds this build does not evaluate.
+fn lower_sensor(kind: &newt::sensor::SensorKind) -> Option<NewtSensorDesc> {
+    use newt::sensor::{SensorAttach, SensorKind as K, SiteFrame};
+    let joint = |k: i32, link: usize| NewtSensorDesc { kind: k, link: link as i32, body: -1, geom: -1, ..Default::default() };
+    let site = |k: i32, s: &SiteFrame| {
+        let (link, body) = match s.attach {
+            SensorAttach::Link(_, l) => (l as i32, -1),
+            SensorAttach::Body(b) => (-1, b as i32),
+        };
+        NewtSensorDesc { kind: k, link, body, geom: -1, offset: v3(s.local_offset), quat: q4(s.local_orientation) }
+    };
+    Some(match kind {
+        K::JointPos { link, .. } => joint(NEWT_SENSOR_JOINT_POS, *link),
+        K::JointVel { link, .. } => joint(NEWT_SENSOR_JOINT_VEL, *link),
+        K::Force { link, .. } => joint(NEWT_SENSOR_FORCE, *link),
+        K::Torque { link, .. } => joint(NEWT_SENSOR_TORQUE, *link),
+        K::SubtreeCom { link, .. } => joint(NEWT_SENSOR_SUBTREE_COM, *link),
+        K::Touch { geom } => NewtSensorDesc { kind: NEWT_SENSOR_TOUCH, link: -1, body: -1, geom: *geom as i32, ..Default::default() },
+        K::FramePos(s) => site(NEWT_SENSOR_FRAME_POS, s),
+        K::FrameQuat(s) => site(NEWT_SENSOR_FRAME_QUAT, s),
+        K::Gyro(s) => site(NEWT_SENSOR_GYRO, s),
+        K::Accelerometer(s) => site(NEWT_SENSOR_ACCELEROMETER, s),
+        K::Velocimeter(s) => site(NEWT_SENSOR_VELOCIMETER, s),
+        K::Magnetometer(s) => site(NEWT_SENSOR_MAGNETOMETER, s),
+        K::FrameLinVel(s) => site(NEWT_SENSOR_FRAME_LINVEL, s),
+        K::FrameAngVel(s) => site(NEWT_SENSOR_FRAME_ANGVEL, s),
+        _ => return None,
+    })
 }
 
 impl Drop for Model {
@@ -303,6 +343,15 @@ impl Data {
     pub fn set_qfrc_applied(&mut self, qfrc: &[f64]) -> Result<(), NewtError> {
         assert_eq!(qfrc.len(), self.n_worlds * self.nv);
         check(unsafe { newt_data_set_qfrc_applied(self.raw, qfrc.as_ptr()) })
+    }
+
+    /// World::sensordata of every world after the last step, [world][dim].
+    pub fn sensordata(&self, model: &Model) -> Result<Vec<f64>, NewtError> {
+        let mut dim = 0;
+        check(unsafe { newt_model_sensor_dim(model.raw, &mut dim) })?;
+        let mut s = vec![0.0; self.n_worlds * dim as usize];
+        check(unsafe { newt_data_get_sensordata(self.raw, s.as_mut_ptr()) })?;
+        Ok(s)
     }
 
     /// Per-world fault codes (0 = healthy, else a newt_status).

@@ -93,6 +93,33 @@ pub struct NewtBodyDesc {
 
 #[repr(C)]
 #[derive(Clone, Copy, Default)]
+pub struct NewtSensorDesc {
+    pub kind: i32,
+    pub link: i32,
+    pub body: i32,
+    pub geom: i32,
+    pub offset: [f64; 3],
+    pub quat: [f64; 4],
+}
+
+// newt_sensor_kind: newt's SensorKind declaration order.
+pub const NEWT_SENSOR_JOINT_POS: i32 = 0;
+pub const NEWT_SENSOR_JOINT_VEL: i32 = 1;
+pub const NEWT_SENSOR_FRAME_POS: i32 = 4;
+pub const NEWT_SENSOR_FRAME_QUAT: i32 = 5;
+pub const NEWT_SENSOR_GYRO: i32 = 6;
+pub const NEWT_SENSOR_ACCELEROMETER: i32 = 7;
+pub const NEWT_SENSOR_TOUCH: i32 = 8;
+pub const NEWT_SENSOR_FORCE: i32 = 9;
+pub const NEWT_SENSOR_TORQUE: i32 = 10;
+pub const NEWT_SENSOR_VELOCIMETER: i32 = 13;
+pub const NEWT_SENSOR_MAGNETOMETER: i32 = 14;
+pub const NEWT_SENSOR_SUBTREE_COM: i32 = 16;
+pub const NEWT_SENSOR_FRAME_LINVEL: i32 = 17;
+pub const NEWT_SENSOR_FRAME_ANGVEL: i32 = 18;
+
+#[repr(C)]
+#[derive(Clone, Copy, Default)]
 pub struct NewtGeomDesc {
     pub shape: i32,
     pub tree: i32,
@@ -190,6 +217,10 @@ extern "C" {
     pub fn newt_data_pair_capacity(d: *const NewtData, cap: *mut i32) -> NewtStatus;
     pub fn newt_data_get_pairs(d: *const NewtData, counts: *mut i32, pairs: *mut i32) -> NewtStatus;
 
+    pub fn newt_model_add_sensor(m: *mut NewtModel, s: *const NewtSensorDesc, index: *mut i32) -> NewtStatus;
+    pub fn newt_model_set_magnetic_field(m: *mut NewtModel, field: *const f64) -> NewtStatus;
+    pub fn newt_model_sensor_dim(m: *const NewtModel, dim: *mut i32) -> NewtStatus;
+    pub fn newt_data_get_sensordata(d: *const NewtData, out: *mut f64) -> NewtStatus;
     pub fn newt_step(m: *const NewtModel, d: *mut NewtData, n_steps: i32, stream: *mut c_void) -> NewtStatus;
     pub fn newt_data_synchronize(d: *mut NewtData) -> NewtStatus;
 

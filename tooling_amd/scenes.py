@@ -100,6 +100,10 @@ class Scene:
     description: str = ""
     extra: dict = field(default_factory=dict)
     bodies: List[dict] = field(default_factory=list)
+    # Sensors (World::add_sensor): dicts made by `sensor(...)`, in
+    # declaration order; `magnetic_field` is World::magnetic_field.
+    sensors: List[dict] = field(default_factory=list)
+    magnetic_field: tuple = (0.0, 0.0, 0.0)
 
     @property
     def nq(self):
@@ -176,6 +180,23 @@ class Scene:
             d.body = g.get("body", -1)
             out.append(d)
         return out
+
+    def sensor_structs(self):
+        out = []
+        for t in self.sensors:
+            d = abi.SensorDesc()
+            d.kind = t["kind"]
+            d.link = t.get("link", -1)
+            d.body = t.get("body", -1)
+            d.geom = t.get("geom", -1)
+            d.offset = abi.D3(*t.get("offset", (0.0, 0.0, 0.0)))
+            d.quat = abi.D4(*t.get("quat", (0.0, 0.0, 0.0, 1.0)))
+            out.append(d)
+        return out
+
+    @property
+    def sensor_dim(self):
+        return sum(abi.sensor_dim(t["kind"]) for t in self.sensors)
 
     def body_structs(self):
         out = []
@@ -556,3 +577,10 @@ SCENES = {"pendulum3": pendulum, "quadruped18": quadruped, "boxchain6": box_chai
           "chain40": lambda: hinge_chain(40), "chain64": lambda: hinge_chain(64), "boxstack3": box_stack,
           "boxpile64": box_pile,
           "boxpile64_newton": lambda: with_options(box_pile(), "_newton", solver=abi.SOLVER_NEWTON)}
+
+
+def sensor(kind, link=-1, body=-1, geom=-1, offset=(0.0, 0.0, 0.0), quat=(0.0, 0.0, 0.0, 1.0)):
+    """One sensor (newt's Sensor with its SiteFrame, sensor.rs:118-122,
+    226-229): `link` or `body` is the site's parent (or the joint/force
+    sensor's link), `geom` the touch sensor's geom."""
+    return dict(kind=kind, link=link, body=body, geom=geom, offset=tuple(offset), quat=tuple(quat))
