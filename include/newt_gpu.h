@@ -188,7 +188,12 @@ const char* newt_last_error(void);
 newt_status newt_model_create(const newt_options* options, newt_model** out);
 
 /* World::add_tree (newt/src/world.rs:965).  `self_collide` = 0 adds the
- * tree to World::disabled_self_collision (newt/src/world.rs:130). */
+ * tree to World::disabled_self_collision (newt/src/world.rs:130).  Several
+ * trees share one world: their coordinates follow each other in qpos/qvel
+ * in the order added, geoms name (tree, link), and one contact system
+ * couples them; such models step on the lane-per-world FP64 step in every
+ * precision, and under the penalty solver pairs between two trees are
+ * rejected at finalize. */
 newt_status newt_model_add_tree(newt_model* model, const newt_link_desc* links,
                                 int32_t n_links, int32_t self_collide,
                                 int32_t* tree_index);
@@ -217,7 +222,7 @@ newt_status newt_model_set_contact_capacity(newt_model* model, int32_t max_conta
  * newt/src/model.rs:179-195; docs/model-format.md): strict like newt's
  * loader (unknown fields rejected; newt_last_error names the JSON path of
  * the offending value), numbers rounded to f32 as newt parses them.  What
- * this build does not step (several trees, trees with bodies, actuators,
+ * this build does not step (trees with bodies, actuators,
  * equalities, tendons, meshes, heightfields, sensors, explicit pair lists,
  * ball joints) returns NEWT_ERR_INVALID_CONFIG.  finalize = 1 also runs
  * newt_model_finalize; 0 returns the model open for further add_* calls. */
@@ -374,13 +379,14 @@ typedef enum newt_sensor_kind {
 } newt_sensor_kind;
 
 /* One Sensor (newt/src/sensor.rs:226-229) with its SiteFrame (sensor.rs:118-122).
- * Joint, force, torque and subtree sensors name a link of the tree; site
+ * Joint, force, torque and subtree sensors name a link (tree, link); site
  * sensors name the link or the free body the site rides on (the other
  * index -1), with the site's offset in that body's COM frame and the
  * orientation mapping site vectors into it (x, y, z, w); touch names a geom. */
 typedef struct newt_sensor_desc {
     int32_t kind;     /* newt_sensor_kind */
-    int32_t link;     /* tree link, or -1 */
+    int32_t tree;     /* the link's tree (SensorKind's `tree`) */
+    int32_t link;     /* link of that tree, or -1 */
     int32_t body;     /* free body, or -1 */
     int32_t geom;     /* touch: the geom, else -1 */
     double offset[3];

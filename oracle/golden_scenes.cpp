@@ -355,12 +355,12 @@ std::vector<uint8_t> sensors_battery() {
     b.v = v3(-0.1f, 0.2f, 0.05f);
     w.add_body(b);
     using W = World<F>;
-    w.add_sensor(W::SK_JPOS, 1, -1, -1, vzero<F>(), qidentity<F>());
-    w.add_sensor(W::SK_JVEL, 1, -1, -1, vzero<F>(), qidentity<F>());
-    w.add_sensor(W::SK_FPOS, 1, -1, -1, v3(0.0f, 0.0f, -0.5f), qidentity<F>());
-    w.add_sensor(W::SK_FQUAT, 1, -1, -1, vzero<F>(), qidentity<F>());
-    w.add_sensor(W::SK_GYRO, -1, 0, -1, v3(0.1f, 0.2f, 0.3f), qidentity<F>());
-    w.add_sensor(W::SK_ACCEL, -1, 0, -1, v3(0.1f, 0.2f, 0.3f), qidentity<F>());
+    w.add_sensor(W::SK_JPOS, 0, 1, -1, -1, vzero<F>(), qidentity<F>());
+    w.add_sensor(W::SK_JVEL, 0, 1, -1, -1, vzero<F>(), qidentity<F>());
+    w.add_sensor(W::SK_FPOS, 0, 1, -1, -1, v3(0.0f, 0.0f, -0.5f), qidentity<F>());
+    w.add_sensor(W::SK_FQUAT, 0, 1, -1, -1, vzero<F>(), qidentity<F>());
+    w.add_sensor(W::SK_GYRO, 0, -1, 0, -1, v3(0.1f, 0.2f, 0.3f), qidentity<F>());
+    w.add_sensor(W::SK_ACCEL, 0, -1, 0, -1, v3(0.1f, 0.2f, 0.3f), qidentity<F>());
     std::vector<uint8_t> out;
     int done = 0;
     for (int at : {1, 100, 500}) {

@@ -1068,7 +1068,7 @@ template <class T> struct World {
     std::vector<AbaWs<T>> ws;
     // Sensors (World::sensors, world.rs:441; SensorBank, sensor.rs:380-410).
     struct SensorSpec {
-        int kind, link, body, geom, out;  // link/body: the site's parent (one of them), tree 0
+        int kind, tree, link, body, geom, out;  // (tree, link) or body: the site's parent
         V3<T> off;
         Q<T> quat;
     };
@@ -1791,9 +1791,9 @@ template <class T> struct World {
         if (k == SK_FQUAT) return 4;
         return 3;
     }
-    void add_sensor(int kind, int link, int body, int geom, V3<T> off, Q<T> quat) {
+    void add_sensor(int kind, int tree, int link, int body, int geom, V3<T> off, Q<T> quat) {
         int o = (int)sensordata.size();
-        sensors.push_back({kind, link, body, geom, o, off, quat});
+        sensors.push_back({kind, tree, link, body, geom, o, off, quat});
         for (int k = 0; k < sensor_dim(kind); ++k) sensordata.push_back(T(0.0));
     }
     // compute_link_va (sensor.rs:834-916): per link (v, a), body frame at the
@@ -1903,7 +1903,7 @@ template <class T> struct World {
         };
         auto site_pose = [&](const SensorSpec& s, V3<T>* pp, Q<T>* pq) {  // sensor.rs:794-808
             if (s.body >= 0) { *pp = bodies[s.body].p; *pq = bodies[s.body].q; }
-            else { *pp = poses[0][s.link].p; *pq = poses[0][s.link].q; }
+            else { *pp = poses[s.tree][s.link].p; *pq = poses[s.tree][s.link].q; }
         };
         auto site_vel = [&](const SensorSpec& s, V3<T>* lin, V3<T>* ang) {  // sensor.rs:705-726
             if (s.body >= 0) {
@@ -1913,8 +1913,8 @@ template <class T> struct World {
                 *lin = b.v + cross(angular, offw);
                 *ang = angular;
             } else {
-                const Motion<T>& v = va[0][s.link].first;
-                const Q<T>& o = poses[0][s.link].q;
+                const Motion<T>& v = va[s.tree][s.link].first;
+                const Q<T>& o = poses[s.tree][s.link].q;
                 *ang = rotate(o, v.ang);
                 *lin = rotate(o, v.lin + cross(v.ang, s.off));
             }
@@ -1923,8 +1923,8 @@ template <class T> struct World {
         for (const SensorSpec& s : sensors) {
             const int o = s.out;
             switch (s.kind) {
-                case SK_JPOS: sensordata[o] = trees[0].q[trees[0].q_offset[s.link]]; break;
-                case SK_JVEL: sensordata[o] = trees[0].qdot[trees[0].v_offset[s.link]]; break;
+                case SK_JPOS: sensordata[o] = trees[s.tree].q[trees[s.tree].q_offset[s.link]]; break;
+                case SK_JVEL: sensordata[o] = trees[s.tree].qdot[trees[s.tree].v_offset[s.link]]; break;
                 case SK_FPOS: { V3<T> p; Q<T> q; site_pose(s, &p, &q); put3(o, p + rotate(q, s.off)); break; }
                 case SK_FQUAT: {
                     V3<T> p; Q<T> q; site_pose(s, &p, &q);
@@ -1933,7 +1933,7 @@ template <class T> struct World {
                     break;
                 }
                 case SK_GYRO: {  // parent_body_velocities (sensor.rs:814-832)
-                    V3<T> w = s.body >= 0 ? bodies[s.body].w : va[0][s.link].first.ang;
+                    V3<T> w = s.body >= 0 ? bodies[s.body].w : va[s.tree][s.link].first.ang;
                     put3(o, inverse_rotate(s.quat, w));
                     break;
                 }
@@ -1949,8 +1949,8 @@ template <class T> struct World {
                         alpha_world = rotate(b.q, alpha_body);
                         a_com_world = ac;
                     } else {
-                        const Motion<T>& v = va[0][s.link].first;
-                        const Motion<T>& a = va[0][s.link].second;
+                        const Motion<T>& v = va[s.tree][s.link].first;
+                        const Motion<T>& a = va[s.tree][s.link].second;
                         omega_world = rotate(pq, v.ang);
                         alpha_world = rotate(pq, a.ang);
                         a_com_world = rotate(pq, a.lin + cross(v.ang, v.lin));
@@ -1968,10 +1968,10 @@ template <class T> struct World {
                     sensordata[o] = sum;
                     break;
                 }
-                case SK_FORCE: put3(o, lf[0][s.link].lin); break;
+                case SK_FORCE: put3(o, lf[s.tree][s.link].lin); break;
                 case SK_TORQUE: {
-                    const Force<T>& f = lf[0][s.link];
-                    put3(o, f.tq - cross(trees[0].links[s.link].cj_pos, f.lin));
+                    const Force<T>& f = lf[s.tree][s.link];
+                    put3(o, f.tq - cross(trees[s.tree].links[s.link].cj_pos, f.lin));
                     break;
                 }
                 case SK_VELO: {
@@ -1983,7 +1983,7 @@ template <class T> struct World {
                 }
                 case SK_MAG: { V3<T> pp; Q<T> pq; site_pose(s, &pp, &pq); put3(o, inverse_rotate(pq * s.quat, magnetic_field)); break; }
                 case SK_SUBCOM: {  // subtree_com (sensor.rs:728-757)
-                    const Tree<T>& t = trees[0];
+                    const Tree<T>& t = trees[s.tree];
                     V3<T> weighted = vzero<T>();
                     T mass = T(0.0);
                     for (size_t li = 0; li < t.links.size(); ++li) {
@@ -1994,7 +1994,7 @@ template <class T> struct World {
                             cur = t.links[cur].parent;
                         }
                         if (inc) {
-                            weighted = weighted + poses[0][li].p * t.links[li].mass;
+                            weighted = weighted + poses[s.tree][li].p * t.links[li].mass;
                             mass += t.links[li].mass;
                         }
                     }

@@ -130,7 +130,7 @@ template <class T> struct Impl : Base {
     }
     void finalize(int n) override { worlds.assign(n, proto); }
     void add_sensor(const newt_sensor_desc* d) override {
-        proto.add_sensor(d->kind, d->link, d->body, d->geom, v3(T(d->offset[0]), T(d->offset[1]), T(d->offset[2])),
+        proto.add_sensor(d->kind, d->tree, d->link, d->body, d->geom, v3(T(d->offset[0]), T(d->offset[1]), T(d->offset[2])),
                          Q<T>{T(d->quat[0]), T(d->quat[1]), T(d->quat[2]), T(d->quat[3])});
     }
     void set_magnetic(const double* f) override { proto.magnetic_field = v3(T(f[0]), T(f[1]), T(f[2])); }

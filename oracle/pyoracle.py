@@ -104,9 +104,11 @@ class OracleWorlds:
         self.n = n_worlds
         opts = scene.options_struct()
         self.h = L.oracle_create(C.byref(opts), precision)
-        if scene.links:
-            links = scene.link_array()
-            L.oracle_add_tree(self.h, C.cast(links, C.c_void_p), len(scene.links), scene.self_collide)
+        trees = scene.tree_arrays() if hasattr(scene, "tree_arrays") else (
+            [(scene.link_array(), len(scene.links), scene.self_collide)] if scene.links else [])
+        self._trees = trees
+        for links, count, collide in trees:
+            L.oracle_add_tree(self.h, C.cast(links, C.c_void_p), count, collide)
         for b in scene.body_structs():
             L.oracle_add_body(self.h, C.cast(C.pointer(b), C.c_void_p))
         self._geoms = scene.geom_structs()

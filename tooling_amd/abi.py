@@ -107,6 +107,7 @@ class BodyDesc(C.Structure):
 class SensorDesc(C.Structure):
     _fields_ = [
         ("kind", C.c_int32),
+        ("tree", C.c_int32),
         ("link", C.c_int32),
         ("body", C.c_int32),
         ("geom", C.c_int32),

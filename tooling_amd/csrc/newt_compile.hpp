@@ -238,11 +238,17 @@ inline newt_status compile_sensors(HostModel* M, std::string* err) {
         const std::string who = "sensor " + std::to_string(k) + ": ";
         const int dim = sensor_kind_dim(s.kind);
         if (dim == 0) return cfail(err, NEWT_ERR_UNSUPPORTED, who + "this sensor kind is not built");
-        const bool link_ok = s.link >= 0 && s.link < d.n_links;
+        // (tree, link) -> the link's index in the forest
+        int gl = -1;
+        if (s.tree >= 0 && s.tree < (int)M->trees.size() && s.link >= 0 && s.link < (int)M->trees[s.tree].size()) {
+            gl = s.link;
+            for (int t = 0; t < s.tree; ++t) gl += (int)M->trees[t].size();
+        }
+        const bool link_ok = gl >= 0;
         const bool body_ok = s.body >= 0 && s.body < d.n_bodies;
         if (s.kind == SK_JOINT_POS || s.kind == SK_JOINT_VEL) {
             if (!link_ok) return cfail(err, NEWT_ERR_INVALID_CONFIG, who + "link out of range");
-            const int j = M->links[s.link].joint;
+            const int j = M->links[gl].joint;
             if (j != JF_HINGE && j != JF_SLIDE)
                 return cfail(err, NEWT_ERR_INVALID_CONFIG, who + "joint sensors need a hinge or slide joint");
         } else if (s.kind == SK_FORCE || s.kind == SK_TORQUE || s.kind == SK_SUBTREE_COM) {
@@ -263,7 +269,7 @@ inline newt_status compile_sensors(HostModel* M, std::string* err) {
         }
         SensorT t{};
         t.kind = s.kind;
-        t.link = s.link;
+        t.link = link_ok ? gl : -1;
         t.body = s.body;
         t.geom = s.geom;
         t.out = out;
@@ -281,8 +287,7 @@ inline newt_status compile_sensors(HostModel* M, std::string* err) {
 
 inline newt_status compile_model(HostModel* M, std::string* err) {
     if (!M->bodies.empty()) return compile_bodies(M, err);
-    if (M->trees.size() != 1)
-        return cfail(err, NEWT_ERR_UNSUPPORTED, "this build steps models with exactly one tree");
+    if (M->trees.empty()) return cfail(err, NEWT_ERR_INVALID_CONFIG, "a model needs a tree or free bodies");
     const newt_options& o = M->opt;
     if (o.iterations < 1) return cfail(err, NEWT_ERR_INVALID_CONFIG, "solver iterations must be >= 1");
     if (o.penetration_slop != 0.0) return cfail(err, NEWT_ERR_UNSUPPORTED, "penetration_slop is not built");
@@ -295,7 +300,23 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
         return cfail(err, NEWT_ERR_INVALID_ARG, "unknown integrator");
     if (o.solver_mode == NEWT_SOLVER_NEWTON) return cfail(err, NEWT_ERR_UNSUPPORTED, "Newton solver is not built for trees");
     if (o.cone != NEWT_CONE_PYRAMIDAL) return cfail(err, NEWT_ERR_UNSUPPORTED, "elliptic cone is not built");
-    const auto& T = M->trees[0];
+    // Several trees (World::trees) compile into one forest: tree t's links
+    // follow tree t-1's, parents offset by the links before them, so each
+    // tree keeps its own root; qpos/qvel concatenate the trees in order.
+    std::vector<newt_link_desc> T;
+    std::vector<int> link_tree, tree_base;
+    for (size_t t = 0; t < M->trees.size(); ++t) {
+        tree_base.push_back((int)T.size());
+        for (size_t i = 0; i < M->trees[t].size(); ++i) {
+            newt_link_desc d = M->trees[t][i];
+            if (i == 0 && d.parent >= 0) return cfail(err, NEWT_ERR_INVALID_ARG, "a tree's first link is its root");
+            if (i > 0 && !(d.parent >= 0 && d.parent < (int)i))
+                return cfail(err, NEWT_ERR_INVALID_ARG, "parent index must precede its link");
+            if (i > 0) d.parent += tree_base[t];
+            T.push_back(d);
+            link_tree.push_back((int)t);
+        }
+    }
     int nq = 0, nv = 0;
     M->links.clear();
     M->limits.clear();
@@ -303,11 +324,10 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     M->q_default.clear();
     for (size_t i = 0; i < T.size(); ++i) {
         const newt_link_desc& d = T[i];
-        if (i == 0 && !(d.joint == NEWT_JOINT_FREE || d.joint == NEWT_JOINT_FIXED))
+        const bool root = (int)i == tree_base[link_tree[i]];
+        if (root && !(d.joint == NEWT_JOINT_FREE || d.joint == NEWT_JOINT_FIXED))
             return cfail(err, NEWT_ERR_INVALID_ARG, "root joint must be Free or Fixed");
-        if (i > 0 && !(d.parent >= 0 && d.parent < (int)i))
-            return cfail(err, NEWT_ERR_INVALID_ARG, "parent index must precede its link");
-        if (i > 0 && d.joint == NEWT_JOINT_FREE) return cfail(err, NEWT_ERR_INVALID_ARG, "free joint only at the root");
+        if (!root && d.joint == NEWT_JOINT_FREE) return cfail(err, NEWT_ERR_INVALID_ARG, "free joint only at the root");
         if (d.joint == NEWT_JOINT_BALL) return cfail(err, NEWT_ERR_UNSUPPORTED, "ball joints are not built on the GPU path");
         if (d.has_range && !(d.joint == NEWT_JOINT_HINGE || d.joint == NEWT_JOINT_SLIDE))
             return cfail(err, NEWT_ERR_INVALID_ARG, "a range needs a hinge or slide joint");
@@ -331,7 +351,7 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
             l.lim = (int)M->limits.size();
             M->limits.push_back(lt);
         }
-        l.parent = d.parent;
+        l.parent = root ? -1 : d.parent;
         l.joint = d.joint;
         l.q_off = nq;
         l.v_off = nv;
@@ -352,7 +372,7 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
         l.free_damping = d.free_damping;
         // ancestor chain root..self
         std::vector<int> ch;
-        for (int j = (int)i; j >= 0; j = T[j].parent) ch.push_back(j);
+        for (int j = (int)i; j >= 0; j = j == tree_base[link_tree[j]] ? -1 : T[j].parent) ch.push_back(j);
         std::reverse(ch.begin(), ch.end());
         l.chain_off = (int)M->chain.size();
         l.chain_len = (int)ch.size();
@@ -416,14 +436,15 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     M->hgeoms.clear();
     for (const newt_geom_desc& g : M->geoms) {
         if (g.shape < NEWT_SHAPE_PLANE || g.shape > NEWT_SHAPE_CAPSULE) return cfail(err, NEWT_ERR_INVALID_ARG, "unknown geom shape");
-        if (g.tree >= 1) return cfail(err, NEWT_ERR_INVALID_ARG, "geom attached to a missing tree");
-        if (g.tree == 0 && (g.link < 0 || g.link >= (int)T.size())) return cfail(err, NEWT_ERR_INVALID_ARG, "geom link out of range");
+        if (g.tree >= (int)M->trees.size()) return cfail(err, NEWT_ERR_INVALID_ARG, "geom attached to a missing tree");
+        if (g.tree >= 0 && (g.link < 0 || g.link >= (int)M->trees[g.tree].size()))
+            return cfail(err, NEWT_ERR_INVALID_ARG, "geom link out of range");
         if (g.body >= 0) return cfail(err, NEWT_ERR_INVALID_ARG, "geom attached to a missing body");
         if (g.condim != 3 && g.condim != 1 && g.condim != 4 && g.condim != 6)
             return cfail(err, NEWT_ERR_INVALID_ARG, "condim must be 1, 3, 4 or 6");
         GeomT h{};
         h.shape = g.shape;
-        h.link = g.tree == 0 ? g.link : -1;
+        h.link = g.tree >= 0 ? tree_base[g.tree] + g.link : -1;
         h.body = -1;
         h.condim = g.condim;
         for (int k = 0; k < 3; ++k) {
@@ -444,15 +465,22 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     // same attachment (broadphase.rs:16-18, 426-437; world.rs:1219-1225).
     M->pairs.clear();
     int cap_c = 0;
-    bool self_collide = M->tree_collide[0] != 0;
     for (size_t a = 0; a < M->geoms.size(); ++a)
         for (size_t b = a + 1; b < M->geoms.size(); ++b) {
             const newt_geom_desc& ga = M->geoms[a];
             const newt_geom_desc& gb = M->geoms[b];
             if (!((ga.group & gb.mask) != 0 && (gb.group & ga.mask) != 0)) continue;
             bool la = ga.tree >= 0, lb = gb.tree >= 0;
-            if (la && lb && !self_collide) continue;
-            if (la == lb && (!la || ga.link == gb.link)) continue;
+            // disabled_self_collision holds a tree whose links skip each other
+            if (la && lb && ga.tree == gb.tree && M->tree_collide[ga.tree] == 0) continue;
+            if (la == lb && (!la || (ga.tree == gb.tree && ga.link == gb.link))) continue;
+            // newt steps trees one after another under the penalty solver
+            // (step_trees, world.rs:1826-1946), each against the others'
+            // states of that moment: not built.
+            if (la && lb && ga.tree != gb.tree && o.solver_mode == NEWT_SOLVER_PENALTY)
+                return cfail(err, NEWT_ERR_UNSUPPORTED,
+                             "penalty contacts between two trees are not built (geom " + std::to_string(a) +
+                                 " x geom " + std::to_string(b) + ")");
             const bool pa = ga.shape == NEWT_SHAPE_PLANE, pb = gb.shape == NEWT_SHAPE_PLANE;
             const bool round_a = ga.shape == NEWT_SHAPE_SPHERE || ga.shape == NEWT_SHAPE_CAPSULE;
             const bool round_b = gb.shape == NEWT_SHAPE_SPHERE || gb.shape == NEWT_SHAPE_CAPSULE;
@@ -471,6 +499,7 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     d.n_links = (int)T.size();
     d.nq = nq;
     d.nv = nv;
+    d.forest = M->trees.size() > 1 ? 1 : 0;
     d.n_bodies = 0;
     d.bodies = nullptr;
     // Penalty limits and penalty contacts run on the lane-per-world FP64 step.

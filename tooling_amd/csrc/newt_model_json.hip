@@ -8,8 +8,8 @@
 // rounded to f32 as newt parses them, and derived quantities (solid-shape
 // inertias, normalized axes and quaternions) are computed in f32 in newt's
 // operation order, so the model holds newt's values exactly.  What the GPU
-// step does not build is rejected, not dropped: more than one tree, trees
-// together with bodies, actuators, equalities, tendons, meshes,
+// step does not build is rejected, not dropped: trees together with
+// bodies, actuators, equalities, tendons, meshes,
 // heightfields, sensors, explicit pair lists, ball joints, cylinders,
 // ellipsoids and rolling friction.  Sites are accepted and ignored.
 #include <hip/hip_runtime.h>
@@ -241,13 +241,14 @@ void set3(double* d, const std::vector<float>& v) {
 
 struct Loaded {
     newt_options opt{};
-    std::vector<newt_link_desc> links;
-    bool has_tree = false, self_collide = false;
+    std::vector<std::vector<newt_link_desc>> trees;  // World::trees in order
+    std::vector<int> self_collide;
     std::vector<newt_body_desc> bodies;
     std::vector<newt_geom_desc> geoms;
 };
 
-void load_links(const Value* arr, const std::string& path, Loaded* L, std::map<std::string, int>* index) {
+void load_links(const Value* arr, const std::string& path, std::vector<newt_link_desc>* out,
+                std::map<std::string, int>* index) {
     if (!arr || arr->kind != Value::Array || arr->arr.empty()) err(path, "expected a non-empty array of links");
     for (size_t i = 0; i < arr->arr.size(); ++i) {
         const Value& l = arr->arr[i];
@@ -321,7 +322,7 @@ void load_links(const Value* arr, const std::string& path, Loaded* L, std::map<s
         d.damping = kind != "free" ? damping : 0.0;
         d.free_damping = kind == "free" ? damping : 0.0;
         d.armature = armature;
-        L->links.push_back(d);
+        out->push_back(d);
         (*index)[nm->str] = (int)i;
     }
 }
@@ -360,7 +361,8 @@ void load_bodies(const Value* arr, Loaded* L, std::map<std::string, int>* index)
     }
 }
 
-void load_geoms(const Value* arr, const std::string& tree_name, bool has_tree, const std::map<std::string, int>& links,
+void load_geoms(const Value* arr, const std::map<std::string, int>& tree_index,
+                const std::vector<std::map<std::string, int>>& links,
                 const std::map<std::string, int>& bodies, Loaded* L) {
     if (!arr) return;
     if (arr->kind != Value::Array) err("$.geoms", "expected an array");
@@ -408,11 +410,11 @@ void load_geoms(const Value* arr, const std::string& tree_name, bool has_tree, c
         } else if (akind == "link") {
             const Value* t = att->get("tree");
             const Value* lk = att->get("link");
-            if (!has_tree || !t || t->kind != Value::String || t->str != tree_name || !lk ||
-                lk->kind != Value::String || !links.count(lk->str))
-                err(p + ".attach", "must name the tree and one of its links");
-            d.tree = 0;
-            d.link = links.at(lk->str);
+            if (!t || t->kind != Value::String || !tree_index.count(t->str) || !lk || lk->kind != Value::String ||
+                !links[tree_index.at(t->str)].count(lk->str))
+                err(p + ".attach", "must name a tree and one of its links");
+            d.tree = tree_index.at(t->str);
+            d.link = links[d.tree].at(lk->str);
         } else if (akind == "body") {
             const Value* bd = att->get("body");
             if (!bd || bd->kind != Value::String || !bodies.count(bd->str))
@@ -492,25 +494,32 @@ Loaded load(const Value& root) {
     const Value* bodies = root.get("bodies");
     const bool n_trees = trees && trees->kind == Value::Array && !trees->arr.empty();
     if (bodies && bodies->kind == Value::Array && !bodies->arr.empty() && n_trees)
-        err("$.bodies", "the GPU step builds a tree or free bodies, not both");
-    std::map<std::string, int> body_index, link_index;
+        err("$.bodies", "the GPU step builds trees or free bodies, not both");
+    std::map<std::string, int> body_index, tree_index;
+    std::vector<std::map<std::string, int>> link_index;
     load_bodies(bodies, &L, &body_index);
-    std::string tree_name;
     if (L.bodies.empty()) {
-        if (!trees || trees->kind != Value::Array || trees->arr.size() != 1)
-            err("$.trees", "the GPU step builds models with exactly one tree");
-        const Value& t = trees->arr[0];
-        obj(&t, "$.trees[0]", {"name", "self_collide", "links"});
-        const Value* nm = t.get("name");
-        tree_name = nm && nm->kind == Value::String ? nm->str : "tree";
-        if (const Value* sc = t.get("self_collide")) {
-            if (sc->kind != Value::Bool) err("$.trees[0].self_collide", "expected a bool");
-            L.self_collide = sc->b;
+        if (!n_trees) err("$.trees", "expected a non-empty array of trees");
+        for (size_t k = 0; k < trees->arr.size(); ++k) {
+            const std::string tp = "$.trees[" + std::to_string(k) + "]";
+            const Value& t = trees->arr[k];
+            obj(&t, tp, {"name", "self_collide", "links"});
+            const Value* nm = t.get("name");
+            const std::string name = nm && nm->kind == Value::String ? nm->str : "tree";
+            if (tree_index.count(name)) err(tp + ".name", "expected a unique name");
+            tree_index[name] = (int)k;
+            int collide = 0;
+            if (const Value* sc = t.get("self_collide")) {
+                if (sc->kind != Value::Bool) err(tp + ".self_collide", "expected a bool");
+                collide = sc->b ? 1 : 0;
+            }
+            L.trees.emplace_back();
+            link_index.emplace_back();
+            L.self_collide.push_back(collide);
+            load_links(t.get("links"), tp + ".links", &L.trees.back(), &link_index.back());
         }
-        load_links(t.get("links"), "$.trees[0].links", &L, &link_index);
-        L.has_tree = true;
     }
-    load_geoms(root.get("geoms"), tree_name, L.has_tree, link_index, body_index, &L);
+    load_geoms(root.get("geoms"), tree_index, link_index, body_index, &L);
     return L;
 }
 
@@ -533,7 +542,8 @@ newt_status newt_model_from_json(const char* text, int32_t finalize, newt_model*
     newt_model* m = nullptr;
     newt_status s = newt_model_create(&L.opt, &m);
     if (s != NEWT_OK) return s;
-    if (L.has_tree) s = newt_model_add_tree(m, L.links.data(), (int32_t)L.links.size(), L.self_collide ? 1 : 0, nullptr);
+    for (size_t t = 0; s == NEWT_OK && t < L.trees.size(); ++t)
+        s = newt_model_add_tree(m, L.trees[t].data(), (int32_t)L.trees[t].size(), L.self_collide[t], nullptr);
     for (size_t i = 0; s == NEWT_OK && i < L.bodies.size(); ++i) s = newt_model_add_body(m, &L.bodies[i], nullptr);
     for (size_t i = 0; s == NEWT_OK && i < L.geoms.size(); ++i) s = newt_model_add_geom(m, &L.geoms[i], nullptr);
     if (s == NEWT_OK && finalize) s = newt_model_finalize(m);

@@ -505,21 +505,25 @@ template <class R> struct World {
             for (int k = 0; k < 36; ++k) dw[L.ia + i * 36 + k] = m.links[i].si[k];
             kstx(i, xup(i, true));
         }
+        // Several trees are one forest: every root (parent < 0) starts its
+        // own tree, and M is block diagonal over the trees.
         for (int i = n - 1; i >= 1; --i) {
             int p = m.links[i].parent;
+            if (p < 0) continue;
             double a[36], out[36];
             for (int k = 0; k < 36; ++k) a[k] = dw[L.ia + i * 36 + k];
             m6pullback(a, kldx(i), out);
             for (int k = 0; k < 36; ++k) dw[L.ia + p * 36 + k] = dw[L.ia + p * 36 + k] + out[k];
         }
-        if (m.links[0].joint == JF_FREE) {
-            int o = m.links[0].v_off;
+        for (int i = 0; i < n; ++i) {
+            if (m.links[i].parent >= 0 || m.links[i].joint != JF_FREE) continue;
+            int o = m.links[i].v_off;
             for (int r = 0; r < 6; ++r)
-                for (int c = 0; c < 6; ++c) dw[L.M + (o + r) * nv + (o + c)] = dw[L.ia + r * 6 + c];
+                for (int c = 0; c < 6; ++c) dw[L.M + (o + r) * nv + (o + c)] = dw[L.ia + i * 36 + r * 6 + c];
         }
         for (int i = 1; i < n; ++i) {
             const LinkT& l = m.links[i];
-            if (l.joint == JF_FIXED) continue;
+            if (l.joint == JF_FIXED || l.parent < 0) continue;
             int vi = l.v_off;
             SM<double> s = subspace(i);
             double a[36];
@@ -718,6 +722,7 @@ template <class R> struct World {
         for (int i = n - 1; i >= 1; --i) {
             const LinkT& l = m.links[i];
             int p = l.parent;
+            if (p < 0) continue;  // another tree's root
             double a[36], out6[36];
             for (int k = 0; k < 36; ++k) a[k] = dw[L.ia + i * 36 + k];
             Xf<double> x = kldx(i);
@@ -761,17 +766,21 @@ template <class R> struct World {
             kstf(L.pa + p * 6, kldf(L.pa + p * 6) + xtforce(x, paf));
         }
         for (int k = 0; k < m.nv; ++k) dw[out + k] = double(0);
+        // Each root: a free root solves its 6x6 articulated system, a fixed
+        // root does not accelerate.
+        for (int ri = 0; ri < n; ++ri) {
+        if (m.links[ri].parent >= 0) continue;
         SM<double> a0 = smz<double>();
-        if (m.links[0].joint == JF_FREE) {
-            const LinkT& l = m.links[0];
+        if (m.links[ri].joint == JF_FREE) {
+            const LinkT& l = m.links[ri];
             int o = l.v_off;
             double fr[6];
             for (int k = 0; k < 6; ++k) fr[k] = double(0.0) + (free_applied ? dw[L.tau + o + k] : double(0.0));
-            SF<double> pa0 = kldf(L.pa);
+            SF<double> pa0 = kldf(L.pa + ri * 6);
             SF<double> rhs = SF<double>{mk3(fr[0], fr[1], fr[2]), mk3(fr[3], fr[4], fr[5])} - pa0;
             double A[6][7];
             for (int r = 0; r < 6; ++r)
-                for (int cc = 0; cc < 6; ++cc) A[r][cc] = dw[L.ia + r * 6 + cc];
+                for (int cc = 0; cc < 6; ++cc) A[r][cc] = dw[L.ia + ri * 36 + r * 6 + cc];
             double fd = l.free_damping;
             if (fd != double(0)) {
                 SM<double> qr{mk3(qvel[o], qvel[o + 1], qvel[o + 2]), mk3(qvel[o + 3], qvel[o + 4], qvel[o + 5])};
@@ -803,9 +812,11 @@ template <class R> struct World {
             a0 = {mk3(x6[0], x6[1], x6[2]), mk3(x6[3], x6[4], x6[5])};
             for (int k = 0; k < 6; ++k) dw[out + o + k] = x6[k];
         }
-        kstm(L.a, a0);
+        kstm(L.a + ri * 6, a0);
+        }
         for (int i = 1; i < n; ++i) {
             const LinkT& l = m.links[i];
+            if (l.parent < 0) continue;
             Xf<double> x = kldx(i);
             SM<double> ap = xmotion(x, kldm(L.a + l.parent * 6));
             if (l.joint == JF_FIXED) {
@@ -1228,12 +1239,14 @@ template <class R> struct World {
         const double w6 = dt * (1.0 / 6.0);
         for (int k = 0; k < nq; ++k) qpos[k] = dw[L.rq0 + k] + dw[L.raq + k] * w6;
         for (int s = 0; s < nv; ++s) qvel[s] = dw[L.rv0 + s] + dw[L.rav + s] * w6;
-        if (m.links[0].joint == JF_FREE) {
-            Qt<double> o = qnormalize(Qt<double>{qpos[3], qpos[4], qpos[5], qpos[6]});
-            qpos[3] = o.x;
-            qpos[4] = o.y;
-            qpos[5] = o.z;
-            qpos[6] = o.w;
+        for (int i = 0; i < m.n_links; ++i) {
+            if (m.links[i].joint != JF_FREE) continue;
+            const int qo = m.links[i].q_off;
+            Qt<double> o = qnormalize(Qt<double>{qpos[qo + 3], qpos[qo + 4], qpos[qo + 5], qpos[qo + 6]});
+            qpos[qo + 3] = o.x;
+            qpos[qo + 4] = o.y;
+            qpos[qo + 5] = o.z;
+            qpos[qo + 6] = o.w;
         }
         return 0;
     }

@@ -128,7 +128,7 @@ struct ModelT {
     const SensorT* sensors;
     int n_sensors, sensor_dim;
     int sensor_dyn;  // some sensor needs the post-step accelerations or contact forces
-    int pad2;
+    int forest;      // several trees (several roots): the lane-per-world step only
     double magnetic[3];  // World::magnetic_field
 };
 

@@ -6,7 +6,7 @@ models the GPU step builds.
 strict-by-default rule: it rejects unknown fields at every scope, and every
 error names the JSON path of the offending value (``ModelError``).  It also
 rejects what the GPU step does not build, rather than dropping it:
-- more than one tree, or a tree together with free bodies;
+- trees together with free bodies;
 - rolling friction on bodies;
 - actuators, equalities, tendons, meshes and heightfields;
 - ball joints, cylinders and ellipsoids;
@@ -223,24 +223,33 @@ def load(src) -> scenes.Scene:
     if not isinstance(bodies_v, list):
         raise ModelError("$.bodies", "expected an array")
     if bodies_v and trees:
-        raise ModelError("$.bodies", "the GPU step builds a tree or free bodies, not both")
+        raise ModelError("$.bodies", "the GPU step builds trees or free bodies, not both")
     bodies, body_index = _bodies(bodies_v)
-    tree_name, index, links, self_collide = None, {}, [], False
+    tree_index, indexes, loaded = {}, [], []
     if not bodies:
-        if len(trees) != 1:
-            raise ModelError("$.trees", "the GPU step builds models with exactly one tree")
-        tree = _obj(trees[0], "$.trees[0]", {"name", "self_collide", "links"})
-        tree_name = tree.get("name", "tree")
-        self_collide = tree.get("self_collide", False)
-        if not isinstance(self_collide, bool):
-            raise ModelError("$.trees[0].self_collide", "expected a bool")
-        links, index = _links(tree.get("links", []), "$.trees[0].links")
-    geoms = _geoms(top.get("geoms", []), tree_name, index, body_index)
+        if not isinstance(trees, list) or not trees:
+            raise ModelError("$.trees", "expected a non-empty array of trees")
+        for k, t in enumerate(trees):
+            tp = f"$.trees[{k}]"
+            tree = _obj(t, tp, {"name", "self_collide", "links"})
+            tname = tree.get("name", "tree")
+            if tname in tree_index:
+                raise ModelError(f"{tp}.name", "expected a unique name")
+            collide = tree.get("self_collide", False)
+            if not isinstance(collide, bool):
+                raise ModelError(f"{tp}.self_collide", "expected a bool")
+            tlinks, tidx = _links(tree.get("links", []), f"{tp}.links")
+            tree_index[tname] = k
+            indexes.append(tidx)
+            loaded.append((tlinks, 1 if collide else 0))
+    links, self_collide = (loaded[0] if loaded else ([], 0))
+    index = indexes[0] if indexes else {}
+    geoms = _geoms(top.get("geoms", []), tree_index, indexes, body_index)
     sites = top.get("sites", [])
     opts = dict(dt=float(dt), gravity=tuple(float(g) for g in gravity), integrator=integ, solver=mode,
                 iterations=iterations, cone=cone)
-    sc = scenes.Scene(name, opts, links, geoms, 1 if self_collide else 0, None, f"loaded from JSON ({name})",
-                      bodies=bodies)
+    sc = scenes.Scene(name, opts, links, geoms, self_collide, None, f"loaded from JSON ({name})",
+                      bodies=bodies, more_trees=loaded[1:])
     sc.extra["sites"] = sites
     sc.extra["link_names"] = list(index)
     sc.extra["body_names"] = list(body_index)
@@ -366,7 +375,7 @@ def _links(arr, path):
     return out, index
 
 
-def _geoms(arr, tree_name, index, body_index=None):
+def _geoms(arr, tree_index, indexes, body_index=None):
     body_index = body_index or {}
     out = []
     shapes = {"plane": abi.SHAPE_PLANE, "sphere": abi.SHAPE_SPHERE, "box": abi.SHAPE_BOX, "capsule": abi.SHAPE_CAPSULE}
@@ -394,9 +403,10 @@ def _geoms(arr, tree_name, index, body_index=None):
             if kind != "plane":
                 raise ModelError(f"{p}.attach", "only plane geoms may be static")
         elif att.get("kind") == "link":
-            if tree_name is None or att.get("tree") != tree_name or att.get("link") not in index:
-                raise ModelError(f"{p}.attach", "must name the tree and one of its links")
-            tree, link = 0, index[att["link"]]
+            t = tree_index.get(att.get("tree")) if isinstance(att.get("tree"), str) else None
+            if t is None or att.get("link") not in indexes[t]:
+                raise ModelError(f"{p}.attach", "must name a tree and one of its links")
+            tree, link = t, indexes[t][att["link"]]
         elif att.get("kind") == "body":
             if att.get("body") not in body_index:
                 raise ModelError(f"{p}.attach.body", "must name one of the bodies")

@@ -28,9 +28,8 @@ class Model:
         opts = scene.options_struct()
         abi.check(self.lib.newt_model_create(C.byref(opts), C.byref(h)))
         self.handle = h
-        if scene.links:
-            links = scene.link_array()
-            abi.check(self.lib.newt_model_add_tree(h, links, len(scene.links), scene.self_collide, None))
+        for links, count, collide in scene.tree_arrays():
+            abi.check(self.lib.newt_model_add_tree(h, links, count, collide, None))
         for b in scene.body_structs():
             abi.check(self.lib.newt_model_add_body(h, C.byref(b), None))
         for g in scene.geom_structs():

@@ -246,25 +246,27 @@ impl Model {
     }
 }
 
-/// newt's Sensor (sensor.rs:153-196, tree 0) as the C record; None for
+/// newt's Sensor (sensor.rs:153-196) as the C record; None for
 /// kinds this build does not evaluate.
 fn lower_sensor(kind: &newt::sensor::SensorKind) -> Option<NewtSensorDesc> {
     use newt::sensor::{SensorAttach, SensorKind as K, SiteFrame};
-    let joint = |k: i32, link: usize| NewtSensorDesc { kind: k, link: link as i32, body: -1, geom: -1, ..Default::default() };
+    let joint = |k: i32, tree: usize, link: usize| NewtSensorDesc {
+        kind: k, tree: tree as i32, link: link as i32, body: -1, geom: -1, ..Default::default()
+    };
     let site = |k: i32, s: &SiteFrame| {
-        let (link, body) = match s.attach {
-            SensorAttach::Link(_, l) => (l as i32, -1),
-            SensorAttach::Body(b) => (-1, b as i32),
+        let (tree, link, body) = match s.attach {
+            SensorAttach::Link(t, l) => (t as i32, l as i32, -1),
+            SensorAttach::Body(b) => (0, -1, b as i32),
         };
-        NewtSensorDesc { kind: k, link, body, geom: -1, offset: v3(s.local_offset), quat: q4(s.local_orientation) }
+        NewtSensorDesc { kind: k, tree, link, body, geom: -1, offset: v3(s.local_offset), quat: q4(s.local_orientation) }
     };
     Some(match kind {
-        K::JointPos { link, .. } => joint(NEWT_SENSOR_JOINT_POS, *link),
-        K::JointVel { link, .. } => joint(NEWT_SENSOR_JOINT_VEL, *link),
-        K::Force { link, .. } => joint(NEWT_SENSOR_FORCE, *link),
-        K::Torque { link, .. } => joint(NEWT_SENSOR_TORQUE, *link),
-        K::SubtreeCom { link, .. } => joint(NEWT_SENSOR_SUBTREE_COM, *link),
-        K::Touch { geom } => NewtSensorDesc { kind: NEWT_SENSOR_TOUCH, link: -1, body: -1, geom: *geom as i32, ..Default::default() },
+        K::JointPos { tree, link } => joint(NEWT_SENSOR_JOINT_POS, *tree, *link),
+        K::JointVel { tree, link } => joint(NEWT_SENSOR_JOINT_VEL, *tree, *link),
+        K::Force { tree, link } => joint(NEWT_SENSOR_FORCE, *tree, *link),
+        K::Torque { tree, link } => joint(NEWT_SENSOR_TORQUE, *tree, *link),
+        K::SubtreeCom { tree, link } => joint(NEWT_SENSOR_SUBTREE_COM, *tree, *link),
+        K::Touch { geom } => NewtSensorDesc { kind: NEWT_SENSOR_TOUCH, tree: 0, link: -1, body: -1, geom: *geom as i32, ..Default::default() },
         K::FramePos(s) => site(NEWT_SENSOR_FRAME_POS, s),
         K::FrameQuat(s) => site(NEWT_SENSOR_FRAME_QUAT, s),
         K::Gyro(s) => site(NEWT_SENSOR_GYRO, s),

@@ -95,6 +95,7 @@ pub struct NewtBodyDesc {
 #[derive(Clone, Copy, Default)]
 pub struct NewtSensorDesc {
     pub kind: i32,
+    pub tree: i32,
     pub link: i32,
     pub body: i32,
     pub geom: i32,

@@ -104,16 +104,32 @@ class Scene:
     # declaration order; `magnetic_field` is World::magnetic_field.
     sensors: List[dict] = field(default_factory=list)
     magnetic_field: tuple = (0.0, 0.0, 0.0)
+    # Further trees of the world (World::trees[1..]): (links, self_collide)
+    # each; geoms name their tree by index.
+    more_trees: List[tuple] = field(default_factory=list)
+
+    @property
+    def all_links(self):
+        return list(self.links) + [l for t, _ in self.more_trees for l in t]
 
     @property
     def nq(self):
         return sum(7 if l["joint"] == abi.JOINT_FREE else 0 if l["joint"] == abi.JOINT_FIXED
-                   else 4 if l["joint"] == abi.JOINT_BALL else 1 for l in self.links) + 7 * len(self.bodies)
+                   else 4 if l["joint"] == abi.JOINT_BALL else 1 for l in self.all_links) + 7 * len(self.bodies)
 
     @property
     def nv(self):
         return sum(6 if l["joint"] == abi.JOINT_FREE else 0 if l["joint"] == abi.JOINT_FIXED
-                   else 3 if l["joint"] == abi.JOINT_BALL else 1 for l in self.links) + 6 * len(self.bodies)
+                   else 3 if l["joint"] == abi.JOINT_BALL else 1 for l in self.all_links) + 6 * len(self.bodies)
+
+    def tree_arrays(self):
+        """(LinkDesc array, count, self_collide) per tree, in World order."""
+        out = []
+        if self.links:
+            out.append((self.link_array(), len(self.links), self.self_collide))
+        for links, collide in self.more_trees:
+            out.append((self.link_array(links), len(links), collide))
+        return out
 
     def initial_state(self, n_worlds: int, seed: int):
         return self.sampler(n_worlds, seed)
@@ -125,9 +141,10 @@ class Scene:
                            integrator=o["integrator"], solver_mode=o["solver"], iterations=o["iterations"],
                            cone=o.get("cone", abi.CONE_PYRAMIDAL))
 
-    def link_array(self):
-        arr = (abi.LinkDesc * len(self.links))()
-        for i, l in enumerate(self.links):
+    def link_array(self, links=None):
+        links = self.links if links is None else links
+        arr = (abi.LinkDesc * len(links))()
+        for i, l in enumerate(links):
             d = arr[i]
             d.parent = l["parent"]
             d.joint = l["joint"]
@@ -186,6 +203,7 @@ class Scene:
         for t in self.sensors:
             d = abi.SensorDesc()
             d.kind = t["kind"]
+            d.tree = t.get("tree", 0)
             d.link = t.get("link", -1)
             d.body = t.get("body", -1)
             d.geom = t.get("geom", -1)
@@ -579,8 +597,9 @@ SCENES = {"pendulum3": pendulum, "quadruped18": quadruped, "boxchain6": box_chai
           "boxpile64_newton": lambda: with_options(box_pile(), "_newton", solver=abi.SOLVER_NEWTON)}
 
 
-def sensor(kind, link=-1, body=-1, geom=-1, offset=(0.0, 0.0, 0.0), quat=(0.0, 0.0, 0.0, 1.0)):
+def sensor(kind, link=-1, body=-1, geom=-1, offset=(0.0, 0.0, 0.0), quat=(0.0, 0.0, 0.0, 1.0), tree=0):
     """One sensor (newt's Sensor with its SiteFrame, sensor.rs:118-122,
-    226-229): `link` or `body` is the site's parent (or the joint/force
-    sensor's link), `geom` the touch sensor's geom."""
-    return dict(kind=kind, link=link, body=body, geom=geom, offset=tuple(offset), quat=tuple(quat))
+    226-229): link `link` of tree `tree` or free body `body` is the site's
+    parent (or the joint/force sensor's link), `geom` the touch sensor's
+    geom."""
+    return dict(kind=kind, tree=tree, link=link, body=body, geom=geom, offset=tuple(offset), quat=tuple(quat))
