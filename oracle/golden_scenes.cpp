@@ -196,11 +196,11 @@ Geom<F> box_on_link(int tree, int link, V3<F> he, V3<F> off, Q<F> q, F friction)
 }
 
 // newt/tests/solver_golden.rs:93-144 + record_tree (272-297), PGS, RK4.
-std::vector<uint8_t> solver_tree() {
+std::vector<uint8_t> solver_tree(int mode = M_PGS) {
     World<F> w;
     w.dt = 0.005f;
     w.gravity = v3(0.0f, 0.0f, -9.81f);
-    w.mode = M_PGS;
+    w.mode = mode;
     w.iterations = 30;
     w.cone = C_PYRAMIDAL;
     w.integrator = I_RK4;
@@ -377,6 +377,7 @@ extern "C" int oracle_golden_bytes(const char* name, uint8_t* out, int cap) {
     else if (n == "joints_mixed") b = joints_mixed();
     else if (n == "dynamics_mass_bias") b = dynamics_mass_bias();
     else if (n == "solver_tree") b = solver_tree();
+    else if (n == "solver_newton_tree") b = solver_tree(M_NEWTON);
     else if (n == "tumbling_3_body") b = tumbling_3_body();
     else if (n == "integrator_euler") b = integrator_golden(I_EULER);
     else if (n == "integrator_rk4") b = integrator_golden(I_RK4);

@@ -1244,7 +1244,6 @@ template <class T> struct World {
         }
         if (normals) normals->assign(contacts.size(), T(0.0));
         if (contacts.empty() || dt <= T(0.0) || trees.empty()) return qfrc;
-        if (mode == M_NEWTON) fail("Newton tree-contact solve is not restated");
         if (cone != C_PYRAMIDAL) fail("elliptic cone is not restated");
         std::vector<std::vector<Pose<T>>> poses;
         tree_poses(poses);
@@ -1373,17 +1372,38 @@ template <class T> struct World {
                 r.bias = fvel.acc - ref * dt;
             }
         }
-        // PGS sweeps (solver.rs:3269-3331; world_pgs_non_negative 3851-3861).
         std::vector<T> f(nr, T(0.0));
-        for (uint32_t it = 0; it < iterations; ++it)
-            for (const Block& b : blocks)
-                for (int off = 0; off < 4; ++off) {
-                    int i = b.start + off;
-                    RSum<T> s;
-                    for (int col = 0; col < nr; ++col) s.add(A[(size_t)i * nr + col] * f[col]);
-                    T residual = (rows[i].bias + s.acc) + rows[i].reg * f[i];
-                    f[i] = rmax(f[i] - residual / rows[i].diag, T(0.0));
-                }
+        if (mode == M_NEWTON) {
+            // Newton on H = A + diag(R) with every pyramidal row projected
+            // onto f >= 0 (solver.rs:3161-3241): a zero-friction set whose
+            // H is only semidefinite gets a solver-only diagonal pivot.
+            std::vector<T> H = A, lin(nr), tmp;
+            for (int i = 0; i < nr; ++i) {
+                H[(size_t)i * nr + i] += rows[i].reg;
+                lin[i] = rows[i].bias;
+            }
+            bool frictionless = false;
+            for (const Block& b : blocks) frictionless = frictionless || b.mu == T(0.0);
+            if (frictionless && !cholesky(H, nr, tmp)) {
+                T scale = T(0.0);
+                for (T v : H) scale = rmax(scale, rabs(v));
+                T pivot = rmax(scale * T(1.0e-6), T(1.0e-7));
+                for (int i = 0; i < nr; ++i) H[(size_t)i * nr + i] += pivot;
+            }
+            NewtonNonNeg<T> ns{H, lin, nr};
+            f = ns.solve(iterations > 0 ? iterations : 1, T(1e-7));
+        } else {
+            // PGS sweeps (solver.rs:3269-3331; world_pgs_non_negative 3851-3861).
+            for (uint32_t it = 0; it < iterations; ++it)
+                for (const Block& b : blocks)
+                    for (int off = 0; off < 4; ++off) {
+                        int i = b.start + off;
+                        RSum<T> s;
+                        for (int col = 0; col < nr; ++col) s.add(A[(size_t)i * nr + col] * f[col]);
+                        T residual = (rows[i].bias + s.acc) + rows[i].reg * f[i];
+                        f[i] = rmax(f[i] - residual / rows[i].diag, T(0.0));
+                    }
+        }
         // Force recovery (solver.rs:3334-3364).
         if (normals)
             for (const Block& b : blocks) {

@@ -27,6 +27,10 @@ float instantiation rebuilds the same scene and must reproduce every byte:
   solver_newton_incline      Delassus from unit impulses, Cholesky, Newton
                              directions and the exact projected line search
                              (newton.rs:275-556; solver_golden.rs:81-91)
+  solver_newton_tree         the tree-contact scene of solver_tree under the
+                             Newton solver: H = A + diag(R) of the tree rows,
+                             every row projected onto f >= 0
+                             (solver.rs:3161-3241; solver_golden.rs:265-269)
   sensors_battery            sensordata of a pendulum tree and a tumbling
                              free body in one world after steps 1, 100 and
                              500: joint position/velocity, frame position and
@@ -43,7 +47,7 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 @pytest.mark.parametrize("name", ["joints_chain_and_floating", "joints_mixed", "dynamics_mass_bias", "solver_tree",
                                   "tumbling_3_body", "integrator_euler", "integrator_rk4",
                                   "integrator_implicitfast", "solver_stack", "solver_incline",
-                                  "solver_newton_stack", "solver_newton_incline", "sensors_battery"])
+                                  "solver_newton_stack", "solver_newton_incline", "solver_newton_tree", "sensors_battery"])
 def test_golden_bytes(oracle_lib, name):
     expected = open(os.path.join(GOLDEN, name + ".bin"), "rb").read()
     actual = oracle_lib.golden_bytes(name)
