@@ -298,7 +298,6 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     if (o.integrator != NEWT_INTEGRATOR_EULER && o.integrator != NEWT_INTEGRATOR_IMPLICITFAST &&
         o.integrator != NEWT_INTEGRATOR_RK4)
         return cfail(err, NEWT_ERR_INVALID_ARG, "unknown integrator");
-    if (o.solver_mode == NEWT_SOLVER_NEWTON) return cfail(err, NEWT_ERR_UNSUPPORTED, "Newton solver is not built for trees");
     if (o.cone != NEWT_CONE_PYRAMIDAL) return cfail(err, NEWT_ERR_UNSUPPORTED, "elliptic cone is not built");
     // Several trees (World::trees) compile into one forest: tree t's links
     // follow tree t-1's, parents offset by the links before them, so each
@@ -531,6 +530,13 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     return NEWT_OK;
 }
 
+// Tree models the persistent wavefront step does not run: ranged joints
+// under the penalty solver, several trees and the Newton contact solve step
+// on the lane-per-world FP64 kernel in every precision.
+inline bool lane_only(const ModelT& m) {
+    return m.n_bodies == 0 && (m.penalty_limits || m.forest || m.mode == NEWT_SOLVER_NEWTON);
+}
+
 inline Layout make_layout(const ModelT& m, int real_bytes) {
     Layout L{};
     int r = 0, dd = 0, ii = 0;
@@ -591,6 +597,10 @@ inline Layout make_layout(const ModelT& m, int real_bytes) {
     L.sfl = sens ? D(6 * n) : 0;
     L.sqdd = sens ? D(nv) : 0;
     L.sfn = sens ? D(std::max(m.cap_c, 1)) : 0;
+    const bool newton = m.mode == NEWT_SOLVER_NEWTON;
+    L.nfac = newton ? D(m.cap_r * m.cap_r) : 0;
+    L.nvec = newton ? D(9 * m.cap_r) : 0;
+    L.nbrk = newton ? D(m.cap_r + 2) : 0;
     L.total_d = dd;
     L.cids = I(3 * m.cap_c);
     L.rinfo = I(3 * m.cap_r);

@@ -168,8 +168,7 @@ newt_status launch_sensors(newt_data* D, hipStream_t stream) {
     const newt_model* M = D->model;
     if (M->dev.n_sensors == 0) return NEWT_OK;
     dim3 grid((D->W + kBlock - 1) / kBlock), block(kBlock);
-    const bool ws32 = D->precision == NEWT_PRECISION_MIXED && M->dev.n_bodies == 0 && !M->dev.penalty_limits &&
-                      !M->dev.forest;
+    const bool ws32 = D->precision == NEWT_PRECISION_MIXED && M->dev.n_bodies == 0 && !lane_only(M->dev);
     if (ws32)
         hipLaunchKernelGGL(k_sensors<float>, grid, block, 0, stream, M->dev, D->L, D->d, (float*)D->ws, D->dws, D->iws);
     else
@@ -210,8 +209,7 @@ newt_status launch(newt_data* D, int n_steps, hipStream_t stream, int init) {
         HIP_TRY(hipGetLastError());
         return NEWT_OK;
     }
-    // Penalty limits and several trees step on the lane-per-world FP64 kernel.
-    const bool lane_fp64 = M->dev.penalty_limits || M->dev.forest;
+    const bool lane_fp64 = lane_only(M->dev);
     if (!init && D->precision != NEWT_PRECISION_FP64_EXACT && !lane_fp64)
         return launch_persistent(D, n_steps, stream);
     if (D->precision == NEWT_PRECISION_MIXED && !lane_fp64)
@@ -448,11 +446,7 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
         return fail(NEWT_ERR_INVALID_ARG, "unknown precision");
     // Free-body models run the lane-per-world FP64 step in every precision.
     const bool bodies = model->dev.n_bodies > 0;
-    // Ranged joints under the penalty solver step on the lane-per-world
-    // FP64 kernel in every precision.
-    // Several trees run the lane-per-world FP64 step in every precision too.
-    const bool team = !bodies && !model->dev.penalty_limits && !model->dev.forest &&
-                      precision != NEWT_PRECISION_FP64_EXACT;
+    const bool team = !bodies && !lane_only(model->dev) && precision != NEWT_PRECISION_FP64_EXACT;
     if ((model->dev.nv > 64 || model->dev.n_links > 128) && team)
         return fail(NEWT_ERR_UNSUPPORTED, "the wavefront-per-world step handles nv <= 64 and <= 128 links; "
                                           "use NEWT_PRECISION_FP64_EXACT for larger trees");
@@ -467,7 +461,7 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
     D->W = n_worlds;
     D->precision = precision;
     // The FP32 workspace serves the mixed tree kernels only.
-    const bool ws32 = precision == NEWT_PRECISION_MIXED && !bodies && !model->dev.penalty_limits && !model->dev.forest;
+    const bool ws32 = precision == NEWT_PRECISION_MIXED && !bodies && !lane_only(model->dev);
     D->L = bodies ? make_body_layout(model->dev) : make_layout(model->dev, ws32 ? 4 : 8);
     D->TL = TL;
     D->wpb = std::max(wpb, 1);

@@ -80,9 +80,9 @@ newt_status check(const newt_model* model, int32_t n_worlds, int32_t precision, 
     if (precision != NEWT_PRECISION_FP64 && precision != NEWT_PRECISION_MIXED)
         return fail(NEWT_ERR_UNSUPPORTED, "newt_step_noalloc runs the FP64 and mixed persistent step only");
     const ModelT& m = model->dev;
-    if (m.n_bodies > 0 || m.penalty_limits || m.forest)
+    if (m.n_bodies > 0 || lane_only(m))
         return fail(NEWT_ERR_UNSUPPORTED, "newt_step_noalloc steps tree models on the persistent step only "
-                                          "(one tree, no free bodies, no penalty joint limits)");
+                                          "(one tree, no free bodies, no penalty joint limits, PGS or penalty)");
     if (m.nv > 64 || m.n_links > 128)
         return fail(NEWT_ERR_UNSUPPORTED, "the persistent step handles nv <= 64 and <= 128 links");
     *T = make_team_layout(m, precision == NEWT_PRECISION_MIXED ? 4 : 8);

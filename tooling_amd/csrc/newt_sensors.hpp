@@ -76,7 +76,7 @@ template <class R> struct SensorEval {
         for (int k = 0; k < nc; ++k) wd.dw[L.sfn + k] = 0.0;
         if (nc == 0 || m.mode != 1) return true;  // PGS only (the other modes have no pairs here)
         int nr = 0;
-        if (!wd.solve_contacts(nc, (R)m.dt, &nr)) return false;
+        if (wd.solve_contacts(nc, (R)m.dt, &nr) != 0) return false;
         const double dt = m.dt;
         int row = 0;
         for (int k = 0; k < nc; ++k) {

@@ -887,7 +887,9 @@ template <class R> struct World {
 
     // Builds rows, Delassus, PGS; leaves the generalized contact force in
     // ws[L.delta].  Returns false if a mass matrix is not positive definite.
-    NEWT_HD bool solve_contacts(int nc, R dt, int* nrows_out) const {
+    // Returns 0, or the fault: 3 a factor not positive definite, 4 a failed
+    // Newton solve.
+    NEWT_HD int solve_contacts(int nc, R dt, int* nrows_out) const {
         const int nv = m.nv;
         const double dtd = m.dt;
         for (int s = 0; s < nv; ++s) dw[L.delta + s] = 0.0;
@@ -896,7 +898,7 @@ template <class R> struct World {
         bool any = false;
         for (int k = 0; k < nc; ++k)
             if (dw[L.cgeo + 9 * k + 6] - dw[L.cgeo + 9 * k + 8] > 0.0) any = true;
-        if (!any) return true;
+        if (!any) return 0;
         // Response factor (M + dt*B for Euler) and the diagApprox factor (M).
         mass_matrix();
         bool implicit = m.integrator != 2;
@@ -905,13 +907,14 @@ template <class R> struct World {
             if (l.joint == JF_FREE) for (int k = 0; k < 6; ++k) dw[L.dvec + l.v_off + k] = l.free_damping;
             else if (l.joint == JF_HINGE || l.joint == JF_SLIDE) dw[L.dvec + l.v_off] = l.damping + 0.0;
         }
-        if (!cholesky(L.M, L.Li, implicit && dtd != 0.0, dtd)) return false;
-        if (!cholesky(L.M, L.Ld, false, 0.0)) return false;
+        if (!cholesky(L.M, L.Li, implicit && dtd != 0.0, dtd)) return 3;
+        if (!cholesky(L.M, L.Ld, false, 0.0)) return 3;
         // Free velocity: ABA at the current state, user forces only.
         for (int s = 0; s < nv; ++s) dw[L.tau + s] = qfrc[s];
         aba(true, implicit, dtd, L.qacc_free);
         for (int i = 0; i < m.n_links; ++i) iw[L.iwf + i] = 0;
         int nr = 0;
+        bool frictionless = false;
         for (int k = 0; k < nc; ++k) {
             int o = L.cgeo + 9 * k;
             double pen = dw[o + 6] - dw[o + 8];
@@ -920,6 +923,7 @@ template <class R> struct World {
             const GeomT& gb = m.geoms[iw[L.cids + 3 * k + 1]];
             V3<double> cp = kld3(o), n = kld3(o + 3);
             double mu = ga.friction < gb.friction ? ga.friction : gb.friction;
+            frictionless = frictionless || mu == 0.0;
             // tangent_basis (world.rs:3104-3111)
             V3<double> ref = nabs(n.z) < 0.9 ? mk3(0.0, 0.0, 1.0) : mk3(1.0, 0.0, 0.0);
             V3<double> t1 = unit3(cross3(ref, n));
@@ -1017,7 +1021,7 @@ template <class R> struct World {
             nr += 4;
         }
         *nrows_out = nr;
-        if (nr == 0) return true;
+        if (nr == 0) return 0;
         // Delassus A = J (M+dtB)^-1 Jᵀ by columns (solver.rs:3045-3065): FP64
         // solves and dot products, stored in R.
         const int cap = m.cap_r;
@@ -1046,6 +1050,9 @@ template <class R> struct World {
             dw[L.rdiag + row] = (double)ws[L.A + row * cap + row] + dw[L.rreg + row];
             dw[L.f + row] = 0.0;
         }
+        if (m.mode == 2) {
+            if (!newton_contacts(nr, frictionless)) return 4;
+        } else
         // Projected Gauss-Seidel (solver.rs:3269-3331, 3838-3861): Delassus in
         // R, impulses and residual accumulation in FP64.
         for (int it = 0; it < m.iterations; ++it)
@@ -1065,6 +1072,164 @@ template <class R> struct World {
                 for (int s = 0; s < nv; ++s) dw[L.delta + s] = dw[L.delta + s] + (double)ws[co + s] * force;
             }
         }
+        return 0;
+    }
+
+    // ---------------- Newton on the tree contact rows ----------------------
+    // The nonnegative Newton of newton.rs:275-556 over H = A + diag(reg),
+    // lin = the row biases (solver.rs:3161-3241), every pyramidal row
+    // projected onto f >= 0.  H[i][j] is A in R off the diagonal and
+    // dw[rdiag] on it (A_ii + reg_i, plus the pivot below); the factor has
+    // stride nr.  Vectors: x = dw[f], then g, d, Hv, a point, the left and
+    // right points and slope of a line-search interval, a gradient.
+    NEWT_HD double hval(int i, int j) const {
+        return i == j ? dw[L.rdiag + i] : (double)ws[L.A + i * m.cap_r + j];
+    }
+    // cholesky (dynamics.rs:602-622) of H into dw[nfac]; false when a pivot
+    // is not positive.
+    NEWT_HD bool dense_cholesky(int n) const {
+        const int fac = L.nfac;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j <= i; ++j) {
+                double sum = hval(i, j);
+                for (int k = 0; k < j; ++k) sum -= dw[fac + i * n + k] * dw[fac + j * n + k];
+                if (i == j) {
+                    if (sum <= 0.0) return false;
+                    dw[fac + i * n + i] = __builtin_sqrt(sum);
+                } else {
+                    dw[fac + i * n + j] = sum / dw[fac + j * n + j];
+                }
+            }
+        return true;
+    }
+    // out = H v, each row summed in column order from 0.0 (newton.rs:603-612).
+    NEWT_HD void hmul(int n, int v, int out) const {
+        for (int i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < n; ++j) s += hval(i, j) * dw[v + j];
+            dw[out + i] = s;
+        }
+    }
+    NEWT_HD double ndot(int n, int a, int b) const {  // Iterator::sum from -0.0
+        Sum<double> s;
+        for (int i = 0; i < n; ++i) s.add(dw[a + i] * dw[b + i]);
+        return s.acc;
+    }
+    // dw[out] = max(x + alpha d, 0)
+    NEWT_HD void naffine(int n, int x, int d, double alpha, int out) const {
+        for (int i = 0; i < n; ++i) {
+            double v = dw[x + i] + alpha * dw[d + i];
+            dw[out + i] = v < 0.0 ? 0.0 : v;
+        }
+    }
+    // quadratic_cost 0.5 pᵀHp + linᵀp, with Hp into hv
+    NEWT_HD double ncost(int n, int p, int hv) const {
+        hmul(n, p, hv);
+        return 0.5 * ndot(n, p, hv) + ndot(n, L.rbias, p);
+    }
+    NEWT_HD bool newton_contacts(int n, bool frictionless) const {
+        const int x = L.f, g = L.nvec, d = g + m.cap_r, hv = d + m.cap_r, pt = hv + m.cap_r;
+        const int lp = pt + m.cap_r, rp = lp + m.cap_r, sl = rp + m.cap_r, gm = sl + m.cap_r;
+        const int brk = L.nbrk;
+        // A zero-friction contact repeats its normal row four times, so H is
+        // only semidefinite: a solver-only diagonal pivot (solver.rs:3188-3203).
+        if (!dense_cholesky(n)) {
+            if (!frictionless) return false;
+            double scale = 0.0;
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) scale = nmax(scale, nabs(hval(i, j)));
+            const double pivot = nmax(scale * 1.0e-6, 1.0e-7);
+            for (int i = 0; i < n; ++i) dw[L.rdiag + i] = dw[L.rdiag + i] + pivot;
+            if (!dense_cholesky(n)) return false;
+        }
+        const double tol = 1e-7;
+        double cur = ncost(n, x, hv);
+        for (int it = 0; it < m.iterations; ++it) {
+            // g = H x + lin; d = -H⁻¹ g (cholesky_solve, dynamics.rs:626-648)
+            hmul(n, x, g);
+            for (int i = 0; i < n; ++i) dw[g + i] = dw[g + i] + dw[L.rbias + i];
+            for (int i = 0; i < n; ++i) {
+                double s = -dw[g + i];
+                for (int k = 0; k < i; ++k) s -= dw[L.nfac + i * n + k] * dw[d + k];
+                dw[d + i] = s / dw[L.nfac + i * n + i];
+            }
+            for (int i = n - 1; i >= 0; --i) {
+                double s = dw[d + i];
+                for (int k = i + 1; k < n; ++k) s -= dw[L.nfac + k * n + i] * dw[d + k];
+                dw[d + i] = s / dw[L.nfac + i * n + i];
+            }
+            if (ndot(n, g, d) >= 0.0) break;
+            // exact_pyramidal_line_search (newton.rs:400-556): breakpoints 0,
+            // 1 and each -x_i / d_i inside (0, 1), sorted (stable) and
+            // deduplicated within 1e-7; candidates are the breakpoints, then
+            // per interval the minimiser of the quadratic along the
+            // interval's slope; the first strictly cheapest wins.
+            int nb = 2;
+            dw[brk] = 0.0;
+            dw[brk + 1] = 1.0;
+            for (int i = 0; i < n; ++i) {
+                const double di = dw[d + i];
+                if (di == 0.0) continue;
+                const double a = -dw[x + i] / di;
+                if (a > 0.0 && a < 1.0) dw[brk + nb++] = a;
+            }
+            for (int i = 1; i < nb; ++i) {  // insertion sort: stable
+                const double v = dw[brk + i];
+                int j = i;
+                while (j > 0 && v < dw[brk + j - 1]) {
+                    dw[brk + j] = dw[brk + j - 1];
+                    --j;
+                }
+                dw[brk + j] = v;
+            }
+            int nd = 0;
+            for (int i = 0; i < nb; ++i) {
+                const double b = dw[brk + i];
+                if (nd == 0 || !(nabs(b - dw[brk + nd - 1]) <= 1e-7)) dw[brk + nd++] = b;
+            }
+            double best_a = 0.0, best_c = cur;
+            for (int k = 0; k < nd; ++k) {
+                const double a = dw[brk + k];
+                naffine(n, x, d, a, pt);
+                const double c = ncost(n, pt, hv);
+                if (c < best_c) {
+                    best_c = c;
+                    best_a = a;
+                }
+            }
+            for (int k = 0; k + 1 < nd; ++k) {
+                const double lo = dw[brk + k], hi = dw[brk + k + 1];
+                if (hi - lo <= 1e-7) continue;
+                const double mid = 0.5 * (lo + hi);
+                const double eps = (hi - lo) * 1e-4;
+                const double la = nmax(mid - eps, lo), ra = nmin(mid + eps, hi);
+                naffine(n, x, d, la, lp);
+                naffine(n, x, d, ra, rp);
+                const double span = ra - la;
+                if (span <= 0.0) continue;
+                const double inv = 1.0 / span;
+                for (int i = 0; i < n; ++i) dw[sl + i] = (dw[rp + i] - dw[lp + i]) * inv;
+                naffine(n, x, d, mid, pt);
+                hmul(n, pt, gm);
+                for (int i = 0; i < n; ++i) dw[gm + i] = dw[gm + i] + dw[L.rbias + i];
+                hmul(n, sl, hv);
+                const double den = ndot(n, sl, hv);
+                const double num = ndot(n, gm, sl);
+                const double a = den > 0.0 ? nmin(nmax(mid - num / den, lo), hi) : mid;
+                naffine(n, x, d, a, pt);
+                const double c = ncost(n, pt, hv);
+                if (c < best_c) {
+                    best_c = c;
+                    best_a = a;
+                }
+            }
+            if (best_a <= 0.0 || cur - best_c <= tol * (1.0 + nabs(cur))) break;
+            for (int i = 0; i < n; ++i) {
+                const double v = dw[x + i] + best_a * dw[d + i];
+                dw[x + i] = v < 0.0 ? 0.0 : v;
+            }
+            cur = best_c;
+        }
         return true;
     }
 
@@ -1077,13 +1242,14 @@ template <class R> struct World {
         const int nv = m.nv;
         int np = active_pairs();
         int nc = 0;
-        bool constraint = m.mode == 1;
+        bool constraint = m.mode == 1 || m.mode == 2;
         for (int s = 0; s < nv; ++s) dw[L.delta + s] = 0.0;
         if (constraint) {
             nc = narrowphase(np);
             if (nc > m.cap_c) return 8;  // NEWT_ERR_CAPACITY
             int nr;
-            if (!solve_contacts(nc, dt, &nr)) return 3;  // NEWT_ERR_NOT_PD
+            const int st = solve_contacts(nc, dt, &nr);  // NEWT_ERR_NOT_PD, NEWT_ERR_NEWTON_FAIL
+            if (st != 0) return st;
         }
         iw[L.iwf + m.n_links] = np;
         iw[L.iwf + m.n_links + 1] = nc;
@@ -1254,7 +1420,7 @@ template <class R> struct World {
     NEWT_HD int post() const {
         const R dt = (R)m.dt;
         const int nv = m.nv;
-        bool constraint = m.mode == 1;
+        bool constraint = m.mode == 1 || m.mode == 2;
         // step_trees_euler / step_trees (world.rs:1826-2051): the limit force
         // plus the contact force is added to qfrc_applied for the step and
         // taken back off afterwards.  dw[L.lim] keeps that delta.

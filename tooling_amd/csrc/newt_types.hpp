@@ -152,6 +152,9 @@ struct Layout {
     // sensors: per-link contact wrenches (6), v and a (12) and joint
     // wrenches (6); qddot [nv]; per-contact normal forces
     int swr, sva, sfl, sqdd, sfn;
+    // Newton contact solve on a tree: the factor of H = A + diag(reg)
+    // [cap_r^2], vectors [9 cap_r] and line-search breakpoints [cap_r + 2]
+    int nfac, nvec, nbrk;
     int total_d;
     // integer workspace
     int cids, rinfo, pairs, iwf;
