@@ -56,7 +56,8 @@ typedef enum newt_shape_kind {
     NEWT_SHAPE_PLANE = 0,
     NEWT_SHAPE_SPHERE = 1,
     NEWT_SHAPE_BOX = 2,
-    NEWT_SHAPE_CAPSULE = 3
+    NEWT_SHAPE_CAPSULE = 3,
+    NEWT_SHAPE_MESH = 4           /* GeomShape::Mesh (geom.rs:220): a convex mesh */
 } newt_shape_kind;
 
 /* Integrator (newt/src/world.rs:76-94). */
@@ -174,6 +175,7 @@ typedef struct newt_geom_desc {
     uint32_t group;          /* collision_group */
     uint32_t mask;           /* collision_mask */
     int32_t body;            /* Geom::body: attached free body, -1 for none */
+    int32_t mesh;            /* NEWT_SHAPE_MESH: the newt_model_add_mesh index */
 } newt_geom_desc;
 
 typedef struct newt_model newt_model;
@@ -202,6 +204,19 @@ newt_status newt_model_add_tree(newt_model* model, const newt_link_desc* links,
  * free bodies, not both. */
 newt_status newt_model_add_body(newt_model* model, const newt_body_desc* body,
                                 int32_t* body_index);
+
+/* World::meshes (newt/src/world.rs; ConvexMesh, newt/src/geom.rs:305-340):
+ * a convex mesh of n_vertices mesh-local points (x, y, z) and n_faces
+ * triangles (three vertex indices each, counter-clockwise seen from
+ * outside).  ConvexMesh::validate's checks apply (at least 4 vertices and 4
+ * faces, indices in range, finite coordinates): NEWT_ERR_INVALID_ARG
+ * otherwise.  Geoms refer to it by index (newt_geom_desc::mesh).  Mesh geoms
+ * meet planes (mesh_plane, contact.rs:1424-1510), spheres (sphere_mesh,
+ * 1720-1776) and meshes (GJK + EPA and the face/edge manifold,
+ * contact/ccd.rs:1594-2092); models with meshes step on the lane-per-world
+ * FP64 kernels in every precision. */
+newt_status newt_model_add_mesh(newt_model* model, const double* vertices, int32_t n_vertices,
+                                const int32_t* faces, int32_t n_faces, int32_t* mesh_index);
 
 /* World::add_geom (newt/src/world.rs:974). */
 newt_status newt_model_add_geom(newt_model* model, const newt_geom_desc* geom,

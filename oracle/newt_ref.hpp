@@ -30,6 +30,7 @@
 #include <utility>
 #include <algorithm>
 #include <array>
+#include <memory>
 
 namespace oracle {
 
@@ -477,7 +478,14 @@ template <class T> inline G6<T> spatial_inertia6(T mass, const M3<T>& inertia) {
 // ---------------------------------------------------------------------------
 
 enum JointKind { J_FREE = 0, J_FIXED = 1, J_HINGE = 2, J_SLIDE = 3, J_BALL = 4 };
-enum ShapeKind { S_PLANE = 0, S_SPHERE = 1, S_BOX = 2, S_CAPSULE = 3 };
+enum ShapeKind { S_PLANE = 0, S_SPHERE = 1, S_BOX = 2, S_CAPSULE = 3, S_MESH = 4 };
+
+// ConvexMesh (geom.rs:305-312): mesh-local vertices and outward-wound
+// triangles.
+template <class T> struct ConvexMesh {
+    std::vector<V3<T>> v;
+    std::vector<std::array<int, 3>> f;
+};
 enum Integrator { I_EULER = 0, I_IMPLICITFAST = 1, I_RK4 = 2 };
 enum SolverMode { M_PENALTY = 0, M_PGS = 1, M_NEWTON = 2 };
 enum Cone { C_PYRAMIDAL = 0, C_ELLIPTIC = 1 };
@@ -558,6 +566,8 @@ template <class T> struct Geom {
     SolImp<T> solimp;
     int condim = 3;
     uint32_t group = 0xffffffffu, mask = 0xffffffffu;
+    int mesh = -1;  // GeomShape::Mesh::mesh_id
+    std::shared_ptr<const ConvexMesh<T>> mesh_data;
     bool is_link() const { return tree >= 0; }
     bool is_body() const { return body >= 0; }
 };

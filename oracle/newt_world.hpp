@@ -11,6 +11,7 @@
 //   newt/src/solver.rs       SolImp/SolRef math, solve_tree_contacts (PGS)
 #pragma once
 
+#include "newt_ccd.hpp"
 #include "newt_ref.hpp"
 
 namespace oracle {
@@ -65,6 +66,16 @@ template <class T> Aabb<T> geom_aabb(const Geom<T>& g, const Pose<T>& p) {
             V3<T> axis = rotate(p.q, v3(T(0), T(0), T(1)));
             T r = g.size.x;
             return aabb_center_ext(p.p, vabs(axis) * g.size.y + V3<T>{r, r, r});
+        }
+        case S_MESH: {  // transformed_vertices_aabb (broadphase.rs:780-791)
+            const std::vector<V3<T>>& v = g.mesh_data->v;
+            V3<T> first = point_to_world(p, v[0]);
+            Aabb<T> b{first, first};
+            for (size_t i = 1; i < v.size(); ++i) {
+                V3<T> w = point_to_world(p, v[i]);
+                b = aunion(b, Aabb<T>{w, w});
+            }
+            return b;
         }
     }
     fail("geom_aabb: unknown shape");
@@ -224,8 +235,7 @@ void capsule_plane(int ia, const Pose<T>& pc, T radius, T hh, T fr, T margin, T 
     }
 }
 
-// closest_points_on_segments (contact.rs:2342-2385), clamp01 (2324-2334).
-template <class T> inline T clamp01(T x) { return x < T(0.0) ? T(0.0) : (x > T(1.0) ? T(1.0) : x); }
+// closest_points_on_segments (contact.rs:2342-2385); clamp01 is in newt_ccd.hpp.
 template <class T>
 std::pair<V3<T>, V3<T>> closest_points_on_segments(V3<T> a0, V3<T> a1, V3<T> b0, V3<T> b1) {
     V3<T> d1 = a1 - a0, d2 = b1 - b0, r = a0 - b0;
@@ -521,6 +531,16 @@ bool try_narrow(int ia, const Geom<T>& ga, const Pose<T>& pa, int ib, const Geom
     if (ga.shape == S_SPHERE && gb.shape == S_PLANE) { sphere_plane(ia, pa, ga.size.x, fr, margin, gap, ib, pb, out); return true; }
     if (ga.shape == S_BOX && gb.shape == S_PLANE) { box_plane(ia, pa, ga.size, fr, margin, gap, ib, pb, out); return true; }
     if (ga.shape == S_CAPSULE && gb.shape == S_PLANE) { capsule_plane(ia, pa, ga.size.x, ga.size.y, fr, margin, gap, ib, pb, out); return true; }
+    if (ga.shape == S_MESH && gb.shape == S_PLANE) {
+        for (const CcdContact<T>& c : mesh_plane(pa, *ga.mesh_data, margin, pb))
+            out.push_back({ia, ib, (int)out.size(), c.pos, c.normal, c.pen, fr, gap});
+        return true;
+    }
+    if (ga.shape == S_SPHERE && gb.shape == S_MESH) {
+        for (const CcdContact<T>& c : sphere_mesh(pa, ga.size.x, pb, *gb.mesh_data, margin))
+            out.push_back({ia, ib, (int)out.size(), c.pos, c.normal, c.pen, fr, gap});
+        return true;
+    }
     // sphere_sphere (contact.rs:407-427), sphere_capsule (473-499),
     // capsule_capsule (1249-1272).
     if (ga.shape == S_SPHERE && gb.shape == S_SPHERE) {
@@ -562,6 +582,11 @@ void narrow_phase(int ia, const Geom<T>& ga, const Pose<T>& pa, int ib, const Ge
     T fr = combine_friction(ga.friction, gb.friction);
     T margin = combine_max(ga.margin, gb.margin);
     T gap = combine_max(ga.gap, gb.gap);
+    if (ga.shape == S_MESH && gb.shape == S_MESH) {  // ccd::route_pair (contact/ccd.rs:6-8)
+        for (const CcdContact<T>& c : mesh_mesh(MeshShape<T>{pa, ga.mesh_data.get()}, MeshShape<T>{pb, gb.mesh_data.get()}, margin))
+            out.push_back({ia, ib, (int)out.size(), c.pos, c.normal, c.pen, fr, gap});
+        return;
+    }
     if (try_narrow(ia, ga, pa, ib, gb, pb, fr, margin, gap, out, legacy)) return;
     if (try_narrow(ib, gb, pb, ia, ga, pa, fr, margin, gap, out, legacy)) {
         for (auto& c : out) {
@@ -578,7 +603,8 @@ inline bool pair_supported(int a, int b) {
     auto one = [](int x, int y) {
         return (x == S_SPHERE && y == S_PLANE) || (x == S_BOX && y == S_PLANE) || (x == S_CAPSULE && y == S_PLANE) ||
                (x == S_SPHERE && y == S_SPHERE) || (x == S_SPHERE && y == S_CAPSULE) ||
-               (x == S_CAPSULE && y == S_CAPSULE) || (x == S_BOX && y == S_BOX);
+               (x == S_CAPSULE && y == S_CAPSULE) || (x == S_BOX && y == S_BOX) || (x == S_MESH && y == S_PLANE) ||
+               (x == S_SPHERE && y == S_MESH) || (x == S_MESH && y == S_MESH);
     };
     return one(a, b) || one(b, a);
 }
@@ -1052,6 +1078,7 @@ template <class T> struct World {
     std::vector<Tree<T>> trees;
     std::vector<Body<T>> bodies;
     std::vector<Geom<T>> geoms;
+    std::vector<std::shared_ptr<const ConvexMesh<T>>> meshes;
     std::vector<bool> self_collide;  // per tree; false => in disabled_self_collision
     // Broadphase proxies: fat AABB per geom (DynamicAabbTree leaf bounds).
     std::vector<Aabb<T>> fat;
@@ -1086,7 +1113,15 @@ template <class T> struct World {
         bodies.push_back(b);
         return (int)bodies.size() - 1;
     }
-    int add_geom(const Geom<T>& g) {  // world.rs:974-979
+    int add_mesh(const ConvexMesh<T>& m) {  // World::meshes
+        meshes.push_back(std::make_shared<const ConvexMesh<T>>(m));
+        return (int)meshes.size() - 1;
+    }
+    int add_geom(Geom<T> g) {  // world.rs:974-979
+        if (g.shape == S_MESH) {
+            if (g.mesh < 0 || g.mesh >= (int)meshes.size()) fail("mesh geom names no mesh");
+            g.mesh_data = meshes[g.mesh];
+        }
         geoms.push_back(g);
         refresh_broadphase();
         return (int)geoms.size() - 1;

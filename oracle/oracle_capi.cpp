@@ -24,6 +24,7 @@ struct Base {
     virtual ~Base() = default;
     virtual void add_tree(const newt_link_desc* l, int n, int collide) = 0;
     virtual void add_geom(const newt_geom_desc* g) = 0;
+    virtual int add_mesh(const double* v, int nv, const int* f, int nf) = 0;
     virtual void add_body(const newt_body_desc* b) = 0;
     virtual void finalize(int n_worlds) = 0;
     virtual int nq() const = 0;
@@ -36,6 +37,7 @@ struct Base {
     virtual void step(int n_steps, int threads) = 0;
     virtual int contacts(int world, int cap, int* ids, double* geo) const = 0;
     virtual int pairs(int world, int cap, int* out) const = 0;
+    virtual int detect(int world) const = 0;
     virtual void mass_matrix(int world, double* out) const = 0;
     virtual void dynamics(int world, double* qacc, double* M, double* bias) const = 0;
     virtual int broadphase(int world, double* fat, double* bound) const = 0;
@@ -112,7 +114,14 @@ template <class T> struct Impl : Base {
         g.group = d->group;
         g.mask = d->mask;
         g.body = d->body;
+        g.mesh = d->shape == S_MESH ? d->mesh : -1;
         proto.add_geom(g);
+    }
+    int add_mesh(const double* v, int nv, const int* f, int nf) override {
+        ConvexMesh<T> m;
+        for (int i = 0; i < nv; ++i) m.v.push_back(v3(T(v[3 * i]), T(v[3 * i + 1]), T(v[3 * i + 2])));
+        for (int i = 0; i < nf; ++i) m.f.push_back({f[3 * i], f[3 * i + 1], f[3 * i + 2]});
+        return proto.add_mesh(m);
     }
     void add_body(const newt_body_desc* d) override {
         M3<T> inertia;
@@ -245,6 +254,12 @@ template <class T> struct Impl : Base {
             });
         for (auto& th : pool) th.join();
     }
+    // World::detect_contacts (world.rs:2055-2062) at the current state: a
+    // fresh broadphase over a copy of the world, then the narrow phase.
+    int detect(int world) const override {
+        World<T> w = worlds[world];
+        return (int)w.detect_contacts(w.active_pairs()).size();
+    }
     int contacts(int world, int cap, int* ids, double* geo) const override {
         const auto& cs = worlds[world].last_contacts;
         int n = (int)cs.size();
@@ -312,6 +327,44 @@ template <class T> struct Impl : Base {
 
 }  // namespace
 
+// One narrow_phase call (dispatch_narrow_phase, contact.rs:2509-2590) on
+// two geoms at the given world poses (x y z, qx qy qz qw), both mesh geoms
+// naming the one mesh passed; contacts into out[k] = (position, normal,
+// penetration).  precision 1 = float (newt's arithmetic), 0 = double.
+namespace {
+template <class T>
+int narrow_one(const newt_geom_desc* da, const double* pa, const newt_geom_desc* db, const double* pb, const double* mv,
+               int nv, const int* mf, int nf, int cap, double* out) {
+    auto mesh = std::make_shared<ConvexMesh<T>>();
+    for (int i = 0; i < nv; ++i) mesh->v.push_back(v3(T(mv[3 * i]), T(mv[3 * i + 1]), T(mv[3 * i + 2])));
+    for (int i = 0; i < nf; ++i) mesh->f.push_back({mf[3 * i], mf[3 * i + 1], mf[3 * i + 2]});
+    auto geom = [&](const newt_geom_desc* d) {
+        Geom<T> g;
+        g.shape = d->shape;
+        g.size = v3(T(d->size[0]), T(d->size[1]), T(d->size[2]));
+        g.friction = T(d->friction);
+        g.margin = T(d->margin);
+        g.gap = T(d->gap);
+        if (g.shape == S_MESH) g.mesh_data = mesh;
+        return g;
+    };
+    auto pose = [](const double* p) {
+        return Pose<T>{v3(T(p[0]), T(p[1]), T(p[2])), Q<T>{T(p[3]), T(p[4]), T(p[5]), T(p[6])}};
+    };
+    std::vector<Contact<T>> buf;
+    narrow_phase(0, geom(da), pose(pa), 1, geom(db), pose(pb), buf);
+    int n = 0;
+    for (const Contact<T>& c : buf) {
+        if (n == cap) break;
+        const double v[7] = {(double)c.pos.x, (double)c.pos.y, (double)c.pos.z, (double)c.normal.x,
+                             (double)c.normal.y, (double)c.normal.z, (double)c.penetration};
+        for (int k = 0; k < 7; ++k) out[7 * n + k] = v[k];
+        ++n;
+    }
+    return n;
+}
+}  // namespace
+
 extern "C" {
 
 void* oracle_create(const newt_options* o, int precision) {
@@ -323,6 +376,14 @@ void oracle_destroy(void* h) { delete static_cast<Base*>(h); }
 void oracle_add_tree(void* h, const newt_link_desc* l, int n, int collide) { static_cast<Base*>(h)->add_tree(l, n, collide); }
 void oracle_add_geom(void* h, const newt_geom_desc* g) { static_cast<Base*>(h)->add_geom(g); }
 void oracle_add_body(void* h, const newt_body_desc* b) { static_cast<Base*>(h)->add_body(b); }
+int oracle_narrow(int precision, const newt_geom_desc* da, const double* pa, const newt_geom_desc* db, const double* pb,
+                  const double* mv, int nv, const int* mf, int nf, int cap, double* out) {
+    if (precision == 1) return narrow_one<float>(da, pa, db, pb, mv, nv, mf, nf, cap, out);
+    return narrow_one<double>(da, pa, db, pb, mv, nv, mf, nf, cap, out);
+}
+int oracle_add_mesh(void* h, const double* v, int nv, const int* f, int nf) {
+    return static_cast<Base*>(h)->add_mesh(v, nv, f, nf);
+}
 void oracle_finalize(void* h, int n_worlds) { static_cast<Base*>(h)->finalize(n_worlds); }
 int oracle_nq(void* h) { return static_cast<Base*>(h)->nq(); }
 int oracle_nv(void* h) { return static_cast<Base*>(h)->nv(); }
@@ -333,6 +394,7 @@ void oracle_set_qfrc(void* h, const double* qfrc) { static_cast<Base*>(h)->set_q
 void oracle_get_qfrc(void* h, double* qfrc) { static_cast<Base*>(h)->get_qfrc(qfrc); }
 void oracle_step(void* h, int n_steps, int threads) { static_cast<Base*>(h)->step(n_steps, threads); }
 int oracle_contacts(void* h, int world, int cap, int* ids, double* geo) { return static_cast<Base*>(h)->contacts(world, cap, ids, geo); }
+int oracle_detect(void* h, int world) { return static_cast<Base*>(h)->detect(world); }
 int oracle_pairs(void* h, int world, int cap, int* out) { return static_cast<Base*>(h)->pairs(world, cap, out); }
 void oracle_mass_matrix(void* h, int world, double* out) { static_cast<Base*>(h)->mass_matrix(world, out); }
 void oracle_dynamics(void* h, int world, double* qacc, double* M, double* bias) {

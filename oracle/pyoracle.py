@@ -34,6 +34,8 @@ def lib():
             "oracle_add_tree": (None, [P, P, I, I]),
             "oracle_add_geom": (None, [P, P]),
             "oracle_add_body": (None, [P, P]),
+            "oracle_add_mesh": (I, [P, PD, I, PI, I]),
+            "oracle_narrow": (I, [I, P, PD, P, PD, PD, I, PI, I, I, PD]),
             "oracle_finalize": (None, [P, I]),
             "oracle_nq": (I, [P]),
             "oracle_nv": (I, [P]),
@@ -45,6 +47,7 @@ def lib():
             "oracle_step": (None, [P, I, I]),
             "oracle_contacts": (I, [P, I, I, PI, PD]),
             "oracle_pairs": (I, [P, I, I, PI]),
+            "oracle_detect": (I, [P, I]),
             "oracle_mass_matrix": (None, [P, I, PD]),
             "oracle_dynamics": (None, [P, I, PD, PD, PD]),
             "oracle_golden_bytes": (I, [C.c_char_p, C.POINTER(C.c_uint8), I]),
@@ -93,6 +96,22 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def narrow_phase(geom_a, pose_a, geom_b, pose_b, mesh=None, precision=1):
+    """One narrow-phase call on two geoms (abi.GeomDesc) at world poses
+    (x, y, z, qx, qy, qz, qw); mesh geoms use `mesh` = (vertices, faces).
+    Returns [n, 7] = (position, normal from B into A, penetration)."""
+    verts, faces = mesh if mesh is not None else (np.zeros((0, 3)), np.zeros((0, 3), dtype=np.int32))
+    v = np.ascontiguousarray(verts, dtype=np.float64)
+    f = np.ascontiguousarray(faces, dtype=np.int32)
+    pa = np.ascontiguousarray(pose_a, dtype=np.float64)
+    pb = np.ascontiguousarray(pose_b, dtype=np.float64)
+    out = np.zeros((8, 7))
+    n = lib().oracle_narrow(precision, C.cast(C.pointer(geom_a), C.c_void_p), _dp(pa),
+                            C.cast(C.pointer(geom_b), C.c_void_p), _dp(pb), _dp(v), len(v),
+                            f.ctypes.data_as(C.POINTER(C.c_int)), len(f), 8, _dp(out))
+    return out[:n].copy()
+
+
 class OracleWorlds:
     """N oracle worlds built from a tooling_amd.scenes.Scene.
     precision: 0 = double (FP64 oracle), 1 = float (newt's arithmetic),
@@ -111,6 +130,10 @@ class OracleWorlds:
             L.oracle_add_tree(self.h, C.cast(links, C.c_void_p), count, collide)
         for b in scene.body_structs():
             L.oracle_add_body(self.h, C.cast(C.pointer(b), C.c_void_p))
+        for verts, faces in getattr(scene, "meshes", []):
+            v = np.ascontiguousarray(verts, dtype=np.float64)
+            f = np.ascontiguousarray(faces, dtype=np.int32)
+            L.oracle_add_mesh(self.h, _dp(v), len(v), f.ctypes.data_as(C.POINTER(C.c_int)), len(f))
         self._geoms = scene.geom_structs()
         for g in self._geoms:
             L.oracle_add_geom(self.h, C.cast(C.pointer(g), C.c_void_p))
@@ -158,6 +181,10 @@ class OracleWorlds:
 
     def step(self, n_steps=1, threads=1):
         self.L.oracle_step(self.h, n_steps, threads)
+
+    def detect(self, world):
+        """Contacts World::detect_contacts finds at world's current state."""
+        return self.L.oracle_detect(self.h, world)
 
     def contacts(self, world, cap=4096):
         ids = np.zeros((cap, 3), dtype=np.int32)

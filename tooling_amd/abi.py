@@ -16,7 +16,7 @@ STATUS_NAMES = {
     5: "OOM", 6: "DEVICE", 7: "INVALID_ARG", 8: "CAPACITY", 9: "UNSUPPORTED",
 }
 JOINT_FREE, JOINT_FIXED, JOINT_HINGE, JOINT_SLIDE, JOINT_BALL = range(5)
-SHAPE_PLANE, SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE = range(4)
+SHAPE_PLANE, SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_MESH = range(5)
 INTEGRATOR_EULER, INTEGRATOR_IMPLICITFAST, INTEGRATOR_RK4 = range(3)
 SOLVER_PENALTY, SOLVER_PGS, SOLVER_NEWTON = range(3)
 CONE_PYRAMIDAL, CONE_ELLIPTIC = range(2)
@@ -85,6 +85,7 @@ class GeomDesc(C.Structure):
         ("group", C.c_uint32),
         ("mask", C.c_uint32),
         ("body", C.c_int32),
+        ("mesh", C.c_int32),
     ]
 
 
@@ -159,6 +160,7 @@ SIGNATURES = {
     "newt_model_create": (_I, [C.POINTER(Options), C.POINTER(_P)]),
     "newt_model_add_tree": (_I, [_P, C.POINTER(LinkDesc), _I, _I, _PI]),
     "newt_model_add_geom": (_I, [_P, C.POINTER(GeomDesc), _PI]),
+    "newt_model_add_mesh": (_I, [_P, C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_int32), C.c_int32, _PI]),
     "newt_model_add_body": (_I, [_P, C.POINTER(BodyDesc), _PI]),
     "newt_model_finalize": (_I, [_P]),
     "newt_model_dims": (_I, [_P, _PI, _PI, _PI, _PI]),

@@ -34,15 +34,59 @@ struct HostModel {
     std::vector<newt_sensor_desc> sensors;
     std::vector<SensorT> hsensors;
     double magnetic[3] = {0.0, 0.0, 0.0};
+    std::vector<MeshT> meshes;  // World::meshes, as added
+    std::vector<double> mesh_v;
+    std::vector<int> mesh_f;
 };
+
+
 
 inline newt_status cfail(std::string* err, newt_status s, const std::string& msg) {
     if (err) *err = msg;
     return s;
 }
 
+// ConvexMesh::validate (geom.rs:318-340).
+inline newt_status check_mesh(const double* v, int nv, const int* f, int nf, std::string* err) {
+    if (nv < 4) return cfail(err, NEWT_ERR_INVALID_ARG, "a convex mesh needs at least 4 vertices");
+    if (nf < 4) return cfail(err, NEWT_ERR_INVALID_ARG, "a convex mesh needs at least 4 triangular faces");
+    for (int k = 0; k < 3 * nf; ++k)
+        if (f[k] < 0 || f[k] >= nv) return cfail(err, NEWT_ERR_INVALID_ARG, "a mesh face index is out of range");
+    for (int k = 0; k < 3 * nv; ++k)
+        if (!(v[k] == v[k] && v[k] - v[k] == 0.0))
+            return cfail(err, NEWT_ERR_INVALID_ARG, "a mesh vertex has a non-finite coordinate");
+    return NEWT_OK;
+}
+
+// is_pair_supported (contact.rs:232-257) for the shapes built here.
+inline bool pair_supported(int a, int b) {
+    auto one = [](int x, int y) {
+        return (y == SH_PLANE && (x == SH_SPHERE || x == SH_BOX || x == SH_CAPSULE || x == SH_MESH)) ||
+               (x == SH_SPHERE && (y == SH_SPHERE || y == SH_CAPSULE || y == SH_MESH)) ||
+               (x == SH_CAPSULE && y == SH_CAPSULE) || (x == SH_BOX && y == SH_BOX) || (x == SH_MESH && y == SH_MESH);
+    };
+    return one(a, b) || one(b, a);
+}
+
+// The mesh tables and the mesh flags of the compiled model.
+inline void set_mesh_tables(HostModel* M) {
+    ModelT& d = M->dev;
+    d.meshes = M->meshes.data();
+    d.mesh_v = M->mesh_v.data();
+    d.mesh_f = M->mesh_f.data();
+    d.n_meshes = (int)M->meshes.size();
+    d.has_mesh = 0;
+    d.mesh_mesh = 0;
+    for (const GeomT& g : M->hgeoms)
+        if (g.shape == SH_MESH) d.has_mesh = 1;
+    for (const PairT& p : M->pairs)
+        if (M->hgeoms[p.a].shape == SH_MESH && M->hgeoms[p.b].shape == SH_MESH) d.mesh_mesh = 1;
+}
+
 // Largest contact count one candidate pair can emit (ContactBuf, contact.rs:108-124).
 inline int pair_max_contacts(int sa, int sb) {
+    if (sa == SH_MESH && sb == SH_MESH) return 4;    // the mesh-mesh manifold
+    if (sa == SH_MESH || sb == SH_MESH) return (sa == SH_PLANE || sb == SH_PLANE) ? 2 : 1;  // mesh_plane, sphere_mesh
     if (sa == SH_BOX && sb == SH_BOX) return 4;      // box_box keeps the four deepest
     if (sa != SH_PLANE && sb != SH_PLANE) return 1;  // sphere_vs_sphere_at
     int p = sa == SH_PLANE ? sb : sa;
@@ -151,7 +195,9 @@ inline newt_status compile_bodies(HostModel* M, std::string* err) {
     M->chain.clear();
     M->hgeoms.clear();
     for (const newt_geom_desc& g : M->geoms) {
-        if (g.shape < NEWT_SHAPE_PLANE || g.shape > NEWT_SHAPE_CAPSULE) return cfail(err, NEWT_ERR_INVALID_ARG, "unknown geom shape");
+        if (g.shape < NEWT_SHAPE_PLANE || g.shape > NEWT_SHAPE_MESH) return cfail(err, NEWT_ERR_INVALID_ARG, "unknown geom shape");
+        if (g.shape == NEWT_SHAPE_MESH && (g.mesh < 0 || g.mesh >= (int)M->meshes.size()))
+            return cfail(err, NEWT_ERR_INVALID_ARG, "a mesh geom names a mesh that was not added");
         if (g.tree >= 0) return cfail(err, NEWT_ERR_INVALID_ARG, "geom attached to a missing tree");
         if (g.body >= nb) return cfail(err, NEWT_ERR_INVALID_ARG, "geom attached to a missing body");
         if (g.condim != 3 && g.condim != 1 && g.condim != 4 && g.condim != 6)
@@ -173,6 +219,7 @@ inline newt_status compile_bodies(HostModel* M, std::string* err) {
         h.solref[1] = g.solref[1];
         for (int k = 0; k < 4; ++k) h.solimp[k] = g.solimp[k];
         h.power = g.solimp_power;
+        h.mesh = g.shape == NEWT_SHAPE_MESH ? g.mesh : -1;
         M->hgeoms.push_back(h);
     }
     // Static candidate pairs: group/mask and same attachment (Static ==
@@ -187,11 +234,7 @@ inline newt_status compile_bodies(HostModel* M, std::string* err) {
             const newt_geom_desc& db = M->geoms[b];
             if (!((da.group & db.mask) != 0 && (db.group & da.mask) != 0)) continue;
             if (ga.body == gb.body) continue;
-            const bool pa = ga.shape == SH_PLANE, pb = gb.shape == SH_PLANE;
-            const bool round_a = ga.shape == SH_SPHERE || ga.shape == SH_CAPSULE;
-            const bool round_b = gb.shape == SH_SPHERE || gb.shape == SH_CAPSULE;
-            const bool boxes = ga.shape == SH_BOX && gb.shape == SH_BOX;
-            if (!(pa != pb || (round_a && round_b) || boxes))
+            if (!pair_supported(ga.shape, gb.shape))
                 return cfail(err, NEWT_ERR_UNSUPPORTED_PAIR, "contact pair geom " + std::to_string(a) + " x geom " +
                                                                  std::to_string(b) + " is an unsupported shape pair");
             if (std::min(ga.condim, gb.condim) != 3)
@@ -222,6 +265,10 @@ inline newt_status compile_bodies(HostModel* M, std::string* err) {
     d.pairs = M->pairs.data();
     d.n_bodies = nb;
     d.bodies = M->hbodies.data();
+    set_mesh_tables(M);
+    if (d.has_mesh && o.solver_mode == NEWT_SOLVER_NEWTON)
+        return cfail(err, NEWT_ERR_UNSUPPORTED, "the Newton free-body step runs one wavefront per world and has no "
+                                                "mesh contacts; mesh models on free bodies run under PGS or penalty");
     return NEWT_OK;
 }
 
@@ -434,7 +481,9 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     // geoms
     M->hgeoms.clear();
     for (const newt_geom_desc& g : M->geoms) {
-        if (g.shape < NEWT_SHAPE_PLANE || g.shape > NEWT_SHAPE_CAPSULE) return cfail(err, NEWT_ERR_INVALID_ARG, "unknown geom shape");
+        if (g.shape < NEWT_SHAPE_PLANE || g.shape > NEWT_SHAPE_MESH) return cfail(err, NEWT_ERR_INVALID_ARG, "unknown geom shape");
+        if (g.shape == NEWT_SHAPE_MESH && (g.mesh < 0 || g.mesh >= (int)M->meshes.size()))
+            return cfail(err, NEWT_ERR_INVALID_ARG, "a mesh geom names a mesh that was not added");
         if (g.tree >= (int)M->trees.size()) return cfail(err, NEWT_ERR_INVALID_ARG, "geom attached to a missing tree");
         if (g.tree >= 0 && (g.link < 0 || g.link >= (int)M->trees[g.tree].size()))
             return cfail(err, NEWT_ERR_INVALID_ARG, "geom link out of range");
@@ -458,6 +507,7 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
         h.solref[1] = g.solref[1];
         for (int k = 0; k < 4; ++k) h.solimp[k] = g.solimp[k];
         h.power = g.solimp_power;
+        h.mesh = g.shape == NEWT_SHAPE_MESH ? g.mesh : -1;
         M->hgeoms.push_back(h);
     }
     // Static candidate pairs: group/mask filter, disabled self collision,
@@ -480,13 +530,8 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
                 return cfail(err, NEWT_ERR_UNSUPPORTED,
                              "penalty contacts between two trees are not built (geom " + std::to_string(a) +
                                  " x geom " + std::to_string(b) + ")");
-            const bool pa = ga.shape == NEWT_SHAPE_PLANE, pb = gb.shape == NEWT_SHAPE_PLANE;
-            const bool round_a = ga.shape == NEWT_SHAPE_SPHERE || ga.shape == NEWT_SHAPE_CAPSULE;
-            const bool round_b = gb.shape == NEWT_SHAPE_SPHERE || gb.shape == NEWT_SHAPE_CAPSULE;
             const std::string which = "contact pair geom " + std::to_string(a) + " x geom " + std::to_string(b);
-            const bool boxes = ga.shape == NEWT_SHAPE_BOX && gb.shape == NEWT_SHAPE_BOX;
-            // is_pair_supported (contact.rs:232-257) for plane, sphere, box, capsule
-            if (!(pa != pb || (round_a && round_b) || boxes))
+            if (!pair_supported(ga.shape, gb.shape))
                 return cfail(err, NEWT_ERR_UNSUPPORTED_PAIR, which + " is an unsupported shape pair");
             if (std::min(ga.condim, gb.condim) != 3)
                 return cfail(err, NEWT_ERR_UNSUPPORTED, "only condim 3 pyramidal contacts are built");
@@ -527,14 +572,33 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     d.level_links = M->level_links.data();
     d.level_nrank = M->level_nrank.data();
     d.dof_link = M->dof_link.data();
+    set_mesh_tables(M);
     return NEWT_OK;
 }
 
 // Tree models the persistent wavefront step does not run: ranged joints
-// under the penalty solver, several trees and the Newton contact solve step
-// on the lane-per-world FP64 kernel in every precision.
+// under the penalty solver, several trees, the Newton contact solve and
+// convex meshes step on the lane-per-world FP64 kernel in every precision.
 inline bool lane_only(const ModelT& m) {
-    return m.n_bodies == 0 && (m.penalty_limits || m.forest || m.mode == NEWT_SOLVER_NEWTON);
+    return m.n_bodies == 0 && (m.penalty_limits || m.forest || m.mode == NEWT_SOLVER_NEWTON || m.has_mesh);
+}
+
+// Per-world mesh-mesh workspace (newt_ccd.hpp, Work): the EPA polytope
+// (128 vertices, two buffers of 512 faces, 1024 horizon edges) and the
+// manifold's five 32-point polygons; only when some pair is mesh-mesh.
+template <class F> void add_ccd_layout(const ModelT& m, Layout& L, F D) {
+    const bool on = m.mesh_mesh != 0;
+    L.cv = on ? D(9 * 128) : 0;
+    L.cf0 = on ? D(4 * 512) : 0;
+    L.cf1 = on ? D(4 * 512) : 0;
+    L.cpoly = on ? D(5 * 96) : 0;
+}
+template <class F> void add_ccd_layout_i(const ModelT& m, Layout& L, F I) {
+    const bool on = m.mesh_mesh != 0;
+    L.cvi = on ? I(2 * 128) : 0;
+    L.cfi0 = on ? I(3 * 512) : 0;
+    L.cfi1 = on ? I(3 * 512) : 0;
+    L.ce = on ? I(2 * 1024) : 0;
 }
 
 inline Layout make_layout(const ModelT& m, int real_bytes) {
@@ -601,6 +665,7 @@ inline Layout make_layout(const ModelT& m, int real_bytes) {
     L.nfac = newton ? D(m.cap_r * m.cap_r) : 0;
     L.nvec = newton ? D(9 * m.cap_r) : 0;
     L.nbrk = newton ? D(m.cap_r + 2) : 0;
+    add_ccd_layout(m, L, D);
     L.total_d = dd;
     L.cids = I(3 * m.cap_c);
     L.rinfo = I(3 * m.cap_r);
@@ -608,6 +673,7 @@ inline Layout make_layout(const ModelT& m, int real_bytes) {
     L.iwf = I(n + 2);
     L.lsl = I(std::max(m.n_ranged, 1));
     L.llm = I(std::max(m.n_ranged, 1));
+    add_ccd_layout_i(m, L, I);
     L.total_i = ii;
     (void)real_bytes;
     return L;
@@ -635,11 +701,13 @@ inline Layout make_body_layout(const ModelT& m) {
     // running stage sum, 26 per body) instead
     L.brow = D(std::max(7 * m.cap_r, m.mode == NEWT_SOLVER_PENALTY ? 26 * nb : 0));
     L.bblk = D(8 * m.cap_c);
+    add_ccd_layout(m, L, D);
     L.total_d = std::max(dd, 1);
     L.cids = I(3 * m.cap_c);
     L.pairs = I(std::max(m.n_pairs, 1));
     L.iwf = I(2);
     L.bbi = I(2 * m.cap_c);
+    add_ccd_layout_i(m, L, I);
     L.total_i = ii;
     return L;
 }

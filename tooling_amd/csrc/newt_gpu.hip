@@ -193,7 +193,9 @@ newt_status launch(newt_data* D, int n_steps, hipStream_t stream, int init) {
     dim3 grid((D->W + kBlock - 1) / kBlock), block(kBlock);
     if (M->dev.n_bodies > 0) {
         const bool newton = M->dev.mode == NEWT_SOLVER_NEWTON;  // the Newton solve is wavefront-only
-        if (!newton && (D->precision == NEWT_PRECISION_FP64_EXACT || !D->bws || D->W > kBodyWaveMaxWorlds))
+        // Meshes run on the lane step only.
+        if (!newton && (D->precision == NEWT_PRECISION_FP64_EXACT || !D->bws || D->W > kBodyWaveMaxWorlds ||
+                        M->dev.has_mesh))
         {
             if (M->dev.mode == NEWT_SOLVER_PENALTY && M->dev.n_pairs > 0)
                 hipLaunchKernelGGL(k_bodies<true>, grid, block, 0, stream, M->dev, D->L, D->d, D->dws, D->iws, n_steps,
@@ -297,6 +299,22 @@ newt_status newt_model_add_geom(newt_model* model, const newt_geom_desc* geom, i
     return NEWT_OK;
 }
 
+newt_status newt_model_add_mesh(newt_model* model, const double* vertices, int32_t n_vertices, const int32_t* faces,
+                                int32_t n_faces, int32_t* mesh_index) {
+    if (!model || !vertices || !faces) return fail(NEWT_ERR_INVALID_ARG, "null argument");
+    if (model->finalized) return fail(NEWT_ERR_INVALID_ARG, "model is finalized");
+    std::string err;
+    const newt_status s = check_mesh(vertices, n_vertices, faces, n_faces, &err);
+    if (s != NEWT_OK) return fail(s, err);
+    HostModel& H = model->h;
+    MeshT t{(int)H.mesh_v.size() / 3, n_vertices, (int)H.mesh_f.size() / 3, n_faces};
+    H.mesh_v.insert(H.mesh_v.end(), vertices, vertices + 3 * (size_t)n_vertices);
+    H.mesh_f.insert(H.mesh_f.end(), faces, faces + 3 * (size_t)n_faces);
+    H.meshes.push_back(t);
+    if (mesh_index) *mesh_index = (int32_t)H.meshes.size() - 1;
+    return NEWT_OK;
+}
+
 newt_status newt_model_add_sensor(newt_model* model, const newt_sensor_desc* sensor, int32_t* sensor_index) {
     if (!model || !sensor) return fail(NEWT_ERR_INVALID_ARG, "null argument");
     if (model->finalized) return fail(NEWT_ERR_INVALID_ARG, "model already finalized");
@@ -367,6 +385,14 @@ newt_status newt_model_finalize(newt_model* model) {
     HIP_TRY(hipMalloc(&model->d_bodies, sizeof(BodyT) * std::max<size_t>(1, H.hbodies.size())));
     if (!H.hbodies.empty())
         HIP_TRY(hipMemcpy(model->d_bodies, H.hbodies.data(), sizeof(BodyT) * H.hbodies.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&model->d_meshes, sizeof(MeshT) * std::max<size_t>(1, H.meshes.size())));
+    HIP_TRY(hipMalloc(&model->d_mesh_v, sizeof(double) * std::max<size_t>(1, H.mesh_v.size())));
+    HIP_TRY(hipMalloc(&model->d_mesh_f, sizeof(int) * std::max<size_t>(1, H.mesh_f.size())));
+    if (!H.meshes.empty()) {
+        HIP_TRY(hipMemcpy(model->d_meshes, H.meshes.data(), sizeof(MeshT) * H.meshes.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(model->d_mesh_v, H.mesh_v.data(), sizeof(double) * H.mesh_v.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(model->d_mesh_f, H.mesh_f.data(), sizeof(int) * H.mesh_f.size(), hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMalloc(&model->d_limits, sizeof(LimitT) * std::max<size_t>(1, H.limits.size())));
     if (!H.limits.empty())
         HIP_TRY(hipMemcpy(model->d_limits, H.limits.data(), sizeof(LimitT) * H.limits.size(), hipMemcpyHostToDevice));
@@ -382,6 +408,9 @@ newt_status newt_model_finalize(newt_model* model) {
     model->dev.limits = model->d_limits;
     model->dev.bodies = model->d_bodies;
     model->dev.sensors = model->d_sensors;
+    model->dev.meshes = model->d_meshes;
+    model->dev.mesh_v = model->d_mesh_v;
+    model->dev.mesh_f = model->d_mesh_f;
     if (H.dev.n_bodies == 0) {
         const ModelT& hd = H.dev;
         CacheLayout C = make_cache_layout(hd);
@@ -435,6 +464,9 @@ void newt_model_destroy(newt_model* model) {
     (void)hipFree(model->d_bodies);
     (void)hipFree(model->d_blob);
     (void)hipFree(model->d_sensors);
+    (void)hipFree(model->d_meshes);
+    (void)hipFree(model->d_mesh_v);
+    (void)hipFree(model->d_mesh_f);
     delete model;
 }
 
@@ -491,9 +523,9 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
               alloc((void**)&D->ovf, W * (size_t)std::max(TL.ov_stride, 16)) &&
               alloc((void**)&D->d.sensordata, W * (size_t)std::max(m.sensor_dim, 1) * 8);
     const bool newton = bodies && m.mode == NEWT_SOLVER_NEWTON;
-    // Penalty contacts run on the lane step only.
+    // Penalty contacts and meshes run on the lane step only.
     const bool penalty_contacts = m.mode == NEWT_SOLVER_PENALTY && m.n_pairs > 0;
-    if (ok && bodies &&
+    if (ok && bodies && !m.has_mesh &&
         (newton || (precision != NEWT_PRECISION_FP64_EXACT && n_worlds <= kBodyWaveMaxWorlds && !penalty_contacts))) {
         // The wavefront step keeps a world's bodies and fat bounds in LDS;
         // models whose bodies do not fit run the lane step (PGS only).

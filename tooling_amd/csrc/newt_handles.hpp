@@ -50,6 +50,9 @@ struct newt_model {
     newt::LimitT* d_limits = nullptr;
     newt::BodyT* d_bodies = nullptr;
     newt::SensorT* d_sensors = nullptr;
+    newt::MeshT* d_meshes = nullptr;
+    double* d_mesh_v = nullptr;
+    int* d_mesh_f = nullptr;
     newt::CacheLayout cache{};  // model tables as the persistent step caches them in LDS
     char* d_blob = nullptr;     // the same tables packed in that layout
 };

@@ -9,7 +9,7 @@
 // inertias, normalized axes and quaternions) are computed in f32 in newt's
 // operation order, so the model holds newt's values exactly.  What the GPU
 // step does not build is rejected, not dropped: trees together with
-// bodies, actuators, equalities, tendons, meshes,
+// bodies, actuators, equalities, tendons,
 // heightfields, sensors, explicit pair lists, ball joints, cylinders,
 // ellipsoids and rolling friction.  Sites are accepted and ignored.
 #include <hip/hip_runtime.h>
@@ -245,7 +245,51 @@ struct Loaded {
     std::vector<int> self_collide;
     std::vector<newt_body_desc> bodies;
     std::vector<newt_geom_desc> geoms;
+    std::vector<std::vector<double>> mesh_v;  // World::meshes: vertices (x, y, z) and faces (i, j, k)
+    std::vector<std::vector<int>> mesh_f;
+    std::map<std::string, int> mesh_index;
 };
+
+// parse_mesh_asset (model.rs:1640-1699): {name, vertices, faces}, checked as
+// ConvexMesh::validate does (geom.rs:318-340).
+void load_meshes(const Value* arr, Loaded* L) {
+    if (!arr) return;
+    if (arr->kind != Value::Array) err("$.meshes", "expected an array");
+    for (size_t i = 0; i < arr->arr.size(); ++i) {
+        const Value& m = arr->arr[i];
+        const std::string p = "$.meshes[" + std::to_string(i) + "]";
+        obj(&m, p, {"name", "vertices", "faces"});
+        const Value* nm = m.get("name");
+        if (!nm || nm->kind != Value::String || nm->str.empty()) err(p + ".name", "mesh name must not be empty");
+        if (L->mesh_index.count(nm->str)) err(p, "duplicate mesh name \"" + nm->str + "\"");
+        const Value* vs = m.get("vertices");
+        if (!vs || vs->kind != Value::Array) err(p + ".vertices", "expected an array");
+        std::vector<double> v;
+        for (size_t k = 0; k < vs->arr.size(); ++k)
+            for (float x : vec(&vs->arr[k], 3, p + ".vertices[" + std::to_string(k) + "]")) v.push_back(x);
+        const Value* fs = m.get("faces");
+        if (!fs || fs->kind != Value::Array) err(p + ".faces", "expected an array");
+        std::vector<int> f;
+        for (size_t k = 0; k < fs->arr.size(); ++k) {
+            const std::string fp = p + ".faces[" + std::to_string(k) + "]";
+            const Value& t = fs->arr[k];
+            if (t.kind != Value::Array || t.arr.size() != 3) err(fp, "face must have exactly 3 vertex indices");
+            for (size_t c = 0; c < 3; ++c) {
+                const float n = num(&t.arr[c], fp + "[" + std::to_string(c) + "]");
+                if (n < 0.0f || n != std::floor(n)) err(fp + "[" + std::to_string(c) + "]", "face vertex index must be a non-negative integer");
+                f.push_back((int)n);
+            }
+        }
+        const int nv = (int)v.size() / 3, nf = (int)f.size() / 3;
+        if (nv < 4) err(p, "convex mesh needs >= 4 vertices");
+        if (nf < 4) err(p, "convex mesh needs >= 4 triangular faces");
+        for (int x : f)
+            if (x >= nv) err(p, "mesh face index out of range");
+        L->mesh_index[nm->str] = (int)L->mesh_v.size();
+        L->mesh_v.push_back(v);
+        L->mesh_f.push_back(f);
+    }
+}
 
 void load_links(const Value* arr, const std::string& path, std::vector<newt_link_desc>* out,
                 std::map<std::string, int>* index) {
@@ -388,10 +432,18 @@ void load_geoms(const Value* arr, const std::map<std::string, int>& tree_index,
             d.shape = NEWT_SHAPE_CAPSULE;
             d.size[0] = num(sh->get("radius"), p + ".shape.radius", true);
             d.size[1] = num(sh->get("half_height"), p + ".shape.half_height", false, true);
+        } else if (kind == "mesh") {
+            obj(sh, p + ".shape", {"kind", "mesh"});
+            const Value* mn = sh->get("mesh");
+            if (!mn || mn->kind != Value::String || !L->mesh_index.count(mn->str))
+                err(p + ".shape.mesh", "unknown mesh name; declare it in the top-level meshes array");
+            d.shape = NEWT_SHAPE_MESH;
+            d.mesh = L->mesh_index.at(mn->str);
         } else {
             err(p + ".shape.kind", "'" + kind + "' geoms are not built on the GPU path");
         }
         d.tree = d.link = d.body = -1;
+        if (d.shape != NEWT_SHAPE_MESH) d.mesh = -1;
         const Value* att = g.get("attach");
         Value dflt;
         if (!att) {
@@ -450,7 +502,7 @@ Loaded load(const Value& root) {
         if (ver->kind != Value::String || ver->str != "1") err("$.version", "must be \"1\"");
     const std::pair<const char*, const char*> unbuilt[] = {
         {"actuators", "actuators"}, {"equality", "equality constraints"}, {"tendons", "tendons"},
-        {"meshes", "mesh assets"}, {"hfields", "heightfields"}, {"contact_pairs", "explicit contact-pair lists"},
+        {"hfields", "heightfields"}, {"contact_pairs", "explicit contact-pair lists"},
         {"sensors", "sensors"}};
     for (const auto& u : unbuilt)
         if (truthy(root.get(u.first))) err(std::string("$.") + u.first, std::string(u.second) + " are not built on the GPU path");
@@ -519,6 +571,7 @@ Loaded load(const Value& root) {
             load_links(t.get("links"), tp + ".links", &L.trees.back(), &link_index.back());
         }
     }
+    load_meshes(root.get("meshes"), &L);
     load_geoms(root.get("geoms"), tree_index, link_index, body_index, &L);
     return L;
 }
@@ -545,6 +598,9 @@ newt_status newt_model_from_json(const char* text, int32_t finalize, newt_model*
     for (size_t t = 0; s == NEWT_OK && t < L.trees.size(); ++t)
         s = newt_model_add_tree(m, L.trees[t].data(), (int32_t)L.trees[t].size(), L.self_collide[t], nullptr);
     for (size_t i = 0; s == NEWT_OK && i < L.bodies.size(); ++i) s = newt_model_add_body(m, &L.bodies[i], nullptr);
+    for (size_t i = 0; s == NEWT_OK && i < L.mesh_v.size(); ++i)
+        s = newt_model_add_mesh(m, L.mesh_v[i].data(), (int32_t)L.mesh_v[i].size() / 3, L.mesh_f[i].data(),
+                                (int32_t)L.mesh_f[i].size() / 3, nullptr);
     for (size_t i = 0; s == NEWT_OK && i < L.geoms.size(); ++i) s = newt_model_add_geom(m, &L.geoms[i], nullptr);
     if (s == NEWT_OK && finalize) s = newt_model_finalize(m);
     if (s != NEWT_OK) {

@@ -17,16 +17,11 @@
 // The FP64 step instantiates R = double; the mixed step R = float.
 #pragma once
 
+#include "newt_ccd.hpp"
 #include "newt_math.hpp"
 #include "newt_types.hpp"
 
 namespace newt {
-
-template <class T> struct Lane {
-    T* p;
-    int W, w;
-    NEWT_HD T& operator[](int i) const { return p[(size_t)i * W + w]; }
-};
 
 // Rust's float `Sum`: a fold that starts at -0.0.
 template <class R> struct Sum {
@@ -254,7 +249,51 @@ template <class R> struct World {
 
     // geom_aabb (broadphase.rs:708-770) as (min, max).
     NEWT_HD void geom_aabb(const GeomT& g, V3<double> p, Qt<double> o, V3<double>* mn, V3<double>* mx) const {
+        if (g.shape == SH_MESH) {  // transformed_vertices_aabb (broadphase.rs:780-791)
+            const ccd::Mesh ms = mesh_of(g, p, o);
+            *mn = ms.world(0);
+            *mx = *mn;
+            for (int k = 1; k < ms.nv; ++k) {
+                const V3<double> w = ms.world(k);
+                *mn = mk3(nmin(mn->x, w.x), nmin(mn->y, w.y), nmin(mn->z, w.z));
+                *mx = mk3(nmax(mx->x, w.x), nmax(mx->y, w.y), nmax(mx->z, w.z));
+            }
+            return;
+        }
         geom_aabb_of(g, p, o, mn, mx);
+    }
+    NEWT_HD ccd::Mesh mesh_of(const GeomT& g, V3<double> p, Qt<double> o) const {
+        const MeshT& mt = m.meshes[g.mesh];
+        return ccd::Mesh{m.mesh_v + 3 * mt.v_off, m.mesh_f + 3 * mt.f_off, mt.nv, mt.nf, p, o};
+    }
+    // Mesh pairs (dispatch_narrow_phase, contact.rs:2509-2590): mesh-mesh
+    // through GJK/EPA (ccd::route_pair), mesh-plane and sphere-mesh in either
+    // order; a swapped primitive's contacts keep the pair's geom order with
+    // the normal negated.
+    __host__ __device__ __noinline__ void mesh_pair(int* nc, int ia, const GeomT& ga, int ib, const GeomT& gb, double fr,
+                                                    double margin, double gap) const {
+        V3<double> pa, pb;
+        Qt<double> qa, qb;
+        geom_pose(ga, &pa, &qa);
+        geom_pose(gb, &pb, &qb);
+        ccd::Buf out;
+        bool flip = false;
+        if (ga.shape == SH_MESH && gb.shape == SH_MESH) {
+            const ccd::Work wk{dw, iw, L.cv, L.cvi, {L.cf0, L.cf1}, {L.cfi0, L.cfi1}, L.ce, L.cpoly};
+            ccd::mesh_mesh(mesh_of(ga, pa, qa), mesh_of(gb, pb, qb), margin, wk, out);
+        } else if (ga.shape == SH_MESH && gb.shape == SH_PLANE) {
+            ccd::mesh_plane(mesh_of(ga, pa, qa), pb, qb, margin, out);
+        } else if (ga.shape == SH_PLANE) {
+            ccd::mesh_plane(mesh_of(gb, pb, qb), pa, qa, margin, out);
+            flip = true;
+        } else if (gb.shape == SH_MESH) {
+            ccd::sphere_mesh(pa, (double)ga.size[0], mesh_of(gb, pb, qb), margin, out);
+        } else {
+            ccd::sphere_mesh(pb, (double)gb.size[0], mesh_of(ga, pa, qa), margin, out);
+            flip = true;
+        }
+        for (int k = 0; k < out.n; ++k)
+            emit(nc, ia, ib, k, out.c[k].pos, flip ? -out.c[k].normal : out.c[k].normal, out.c[k].pen, fr, gap);
     }
     static NEWT_HD void geom_aabb_of(const GeomT& g, V3<double> p, Qt<double> o, V3<double>* mn, V3<double>* mx) {
         if (g.shape == SH_PLANE) {
@@ -430,7 +469,9 @@ template <class R> struct World {
             double margin = (double)ga.margin > (double)gb.margin ? (double)ga.margin : (double)gb.margin;
             double gap = (double)ga.gap > (double)gb.gap ? (double)ga.gap : (double)gb.gap;
             // (primitive, plane) in the given order, else swapped.
-            if (ga.shape == SH_BOX && gb.shape == SH_BOX) {
+            if (ga.shape == SH_MESH || gb.shape == SH_MESH) {
+                mesh_pair(&nc, pr.a, ga, pr.b, gb, fr, margin, gap);
+            } else if (ga.shape == SH_BOX && gb.shape == SH_BOX) {
                 V3<double> pa, pb, nrm, cp[4];
                 Qt<double> qa, qb;
                 double pen[4];

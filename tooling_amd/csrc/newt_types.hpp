@@ -14,7 +14,7 @@
 namespace newt {
 
 enum : int { JF_FREE = 0, JF_FIXED = 1, JF_HINGE = 2, JF_SLIDE = 3, JF_BALL = 4 };
-enum : int { SH_PLANE = 0, SH_SPHERE = 1, SH_BOX = 2, SH_CAPSULE = 3 };
+enum : int { SH_PLANE = 0, SH_SPHERE = 1, SH_BOX = 2, SH_CAPSULE = 3, SH_MESH = 4 };
 
 struct LinkT {
     int parent, joint, q_off, v_off;
@@ -46,6 +46,13 @@ struct GeomT {
     double friction, margin, gap;
     double solref[2], solimp[4];
     int power, body;           // body = the free body it rides on, -1 for none
+    int mesh, pad;             // SH_MESH: index into ModelT::meshes
+};
+
+// A convex mesh (ConvexMesh, geom.rs:305-312): vertices mesh_v[3 v_off ..
+// 3 (v_off + nv)) (x, y, z), faces mesh_f[3 f_off .. 3 (f_off + nf)).
+struct MeshT {
+    int v_off, nv, f_off, nf;
 };
 
 // A free Body (body.rs:22-47).  Inertia and its inverse row-major m[r*3+c];
@@ -130,6 +137,12 @@ struct ModelT {
     int sensor_dyn;  // some sensor needs the post-step accelerations or contact forces
     int forest;      // several trees (several roots): the lane-per-world step only
     double magnetic[3];  // World::magnetic_field
+    // Convex meshes (World::meshes) and whether some geom is a mesh (the
+    // lane-per-world steps only) or some pair is mesh-mesh (EPA workspace).
+    const MeshT* meshes;
+    const double* mesh_v;
+    const int* mesh_f;
+    int n_meshes, has_mesh, mesh_mesh;
 };
 
 // Offsets (in elements) of the per-world scratch regions.
@@ -155,11 +168,15 @@ struct Layout {
     // Newton contact solve on a tree: the factor of H = A + diag(reg)
     // [cap_r^2], vectors [9 cap_r] and line-search breakpoints [cap_r + 2]
     int nfac, nvec, nbrk;
+    // mesh-mesh contacts: EPA vertices (9 each, 128), two face buffers (4
+    // each, 512), the manifold's polygons (5 x 32 points)
+    int cv, cf0, cf1, cpoly;
     int total_d;
     // integer workspace
     int cids, rinfo, pairs, iwf;
     int lsl, llm;  // limit rows: dof slot, limit index
     int bbi;       // free-body contact blocks: body_a, body_b
+    int cvi, cfi0, cfi1, ce;  // EPA tie flags (2 each), face indices (3 each), horizon edges (2 each, 1024)
     int total_i;
 };
 

@@ -8,11 +8,12 @@ error names the JSON path of the offending value (``ModelError``).  It also
 rejects what the GPU step does not build, rather than dropping it:
 - trees together with free bodies;
 - rolling friction on bodies;
-- actuators, equalities, tendons, meshes and heightfields;
+- actuators, equalities, tendons and heightfields;
 - ball joints, cylinders and ellipsoids;
 - explicit contact-pair lists.
 
-Sites are parsed and kept (``scene.extra["sites"]``); they do not take part
+Convex mesh assets (top-level ``meshes``, referenced by mesh geoms by name)
+load into ``scene.meshes``.  Sites are parsed and kept (``scene.extra["sites"]``); they do not take part
 in the step.
 
 Numbers are rounded to float32 as newt parses them (``get_f32``), and derived
@@ -176,7 +177,7 @@ def _solimp(v, path):
 
 
 UNBUILT = {"actuators": "actuators", "equality": "equality constraints",
-           "tendons": "tendons", "meshes": "mesh assets", "hfields": "heightfields",
+           "tendons": "tendons", "hfields": "heightfields",
            "contact_pairs": "explicit contact-pair lists", "sensors": "sensors"}
 
 
@@ -244,12 +245,13 @@ def load(src) -> scenes.Scene:
             loaded.append((tlinks, 1 if collide else 0))
     links, self_collide = (loaded[0] if loaded else ([], 0))
     index = indexes[0] if indexes else {}
-    geoms = _geoms(top.get("geoms", []), tree_index, indexes, body_index)
+    meshes, mesh_index = _meshes(top.get("meshes", []))
+    geoms = _geoms(top.get("geoms", []), tree_index, indexes, body_index, mesh_index)
     sites = top.get("sites", [])
     opts = dict(dt=float(dt), gravity=tuple(float(g) for g in gravity), integrator=integ, solver=mode,
                 iterations=iterations, cone=cone)
     sc = scenes.Scene(name, opts, links, geoms, self_collide, None, f"loaded from JSON ({name})",
-                      bodies=bodies, more_trees=loaded[1:])
+                      bodies=bodies, more_trees=loaded[1:], meshes=meshes)
     sc.extra["sites"] = sites
     sc.extra["link_names"] = list(index)
     sc.extra["body_names"] = list(body_index)
@@ -375,10 +377,55 @@ def _links(arr, path):
     return out, index
 
 
-def _geoms(arr, tree_index, indexes, body_index=None):
+def _meshes(arr):
+    """parse_mesh_asset (model.rs:1640-1699): named convex meshes, checked as
+    ConvexMesh::validate does (geom.rs:318-340)."""
+    if not isinstance(arr, list):
+        raise ModelError("$.meshes", "expected an array")
+    out, index = [], {}
+    for i, m in enumerate(arr):
+        p = f"$.meshes[{i}]"
+        _obj(m, p, {"name", "vertices", "faces"})
+        name = m.get("name")
+        if not isinstance(name, str) or not name:
+            raise ModelError(f"{p}.name", "mesh name must not be empty")
+        if name in index:
+            raise ModelError(p, f'duplicate mesh name "{name}"')
+        verts = m.get("vertices")
+        if not isinstance(verts, list):
+            raise ModelError(f"{p}.vertices", "expected an array")
+        v = [_vec(x, 3, f"{p}.vertices[{k}]") for k, x in enumerate(verts)]
+        faces = m.get("faces")
+        if not isinstance(faces, list):
+            raise ModelError(f"{p}.faces", "expected an array")
+        f = []
+        for k, t in enumerate(faces):
+            if not isinstance(t, list) or len(t) != 3:
+                raise ModelError(f"{p}.faces[{k}]", "face must have exactly 3 vertex indices")
+            tri = []
+            for c, e in enumerate(t):
+                n = _num(e, f"{p}.faces[{k}][{c}]")
+                if n < 0 or n != math.floor(n):
+                    raise ModelError(f"{p}.faces[{k}][{c}]", "face vertex index must be a non-negative integer")
+                tri.append(int(n))
+            f.append(tri)
+        if len(v) < 4:
+            raise ModelError(p, "convex mesh needs >= 4 vertices")
+        if len(f) < 4:
+            raise ModelError(p, "convex mesh needs >= 4 triangular faces")
+        if any(c >= len(v) for tri in f for c in tri):
+            raise ModelError(p, "mesh face index out of range")
+        index[name] = i
+        out.append((np.array(v, dtype=np.float64), np.array(f, dtype=np.int32)))
+    return out, index
+
+
+def _geoms(arr, tree_index, indexes, body_index=None, mesh_index=None):
     body_index = body_index or {}
+    mesh_index = mesh_index or {}
     out = []
-    shapes = {"plane": abi.SHAPE_PLANE, "sphere": abi.SHAPE_SPHERE, "box": abi.SHAPE_BOX, "capsule": abi.SHAPE_CAPSULE}
+    shapes = {"plane": abi.SHAPE_PLANE, "sphere": abi.SHAPE_SPHERE, "box": abi.SHAPE_BOX, "capsule": abi.SHAPE_CAPSULE,
+              "mesh": abi.SHAPE_MESH}
     for i, g in enumerate(arr):
         p = f"$.geoms[{i}]"
         _obj(g, p, {"name", "shape", "attach", "local_offset", "local_orientation", "friction", "solref", "solimp",
@@ -397,6 +444,12 @@ def _geoms(arr, tree_index, indexes, body_index=None):
                     float(_num(sh.get("half_height"), f"{p}.shape.half_height", nonneg=True)), 0.0)
         else:
             size = (0.0, 0.0, 0.0)
+        mesh = -1
+        if kind == "mesh":
+            _obj(sh, f"{p}.shape", {"kind", "mesh"})
+            if sh.get("mesh") not in mesh_index:
+                raise ModelError(f"{p}.shape.mesh", "unknown mesh name; declare it in the top-level meshes array")
+            mesh = mesh_index[sh["mesh"]]
         att = _obj(g.get("attach", {"kind": "static"}), f"{p}.attach", {"kind", "tree", "link", "body"})
         tree, link, body = -1, -1, -1
         if att.get("kind") == "static":
@@ -425,5 +478,5 @@ def _geoms(arr, tree_index, indexes, body_index=None):
                                margin=float(_num(g.get("margin", 0.0), f"{p}.margin", nonneg=True)),
                                gap=float(_num(g.get("gap", 0.0), f"{p}.gap", nonneg=True)),
                                solref=tuple(float(x) for x in _solref(g.get("solref"), f"{p}.solref")),
-                               solimp=tuple(float(x) for x in si[:4]), power=si[4], condim=condim))
+                               solimp=tuple(float(x) for x in si[:4]), power=si[4], condim=condim, mesh=mesh))
     return out

@@ -32,6 +32,11 @@ class Model:
             abi.check(self.lib.newt_model_add_tree(h, links, count, collide, None))
         for b in scene.body_structs():
             abi.check(self.lib.newt_model_add_body(h, C.byref(b), None))
+        for verts, faces in getattr(scene, "meshes", []):
+            v = np.ascontiguousarray(verts, dtype=np.float64)
+            f = np.ascontiguousarray(faces, dtype=np.int32)
+            abi.check(self.lib.newt_model_add_mesh(h, v.ctypes.data_as(C.POINTER(C.c_double)), len(v),
+                                                   f.ctypes.data_as(C.POINTER(C.c_int32)), len(f), None))
         for g in scene.geom_structs():
             abi.check(self.lib.newt_model_add_geom(h, C.byref(g), None))
         if contact_capacity is not None:

@@ -138,6 +138,10 @@ fn lower_geom(g: &Geom) -> NewtGeomDesc {
             d.shape = NEWT_SHAPE_CAPSULE;
             d.size = [*radius as f64, *half_height as f64, 0.0];
         }
+        GeomShape::Mesh { mesh_id } => {
+            d.shape = NEWT_SHAPE_MESH;
+            d.mesh = *mesh_id as i32;
+        }
         _ => d.shape = -1,
     }
     if let Some((t, l)) = g.link {
@@ -212,6 +216,14 @@ impl Model {
         }
         for b in world.bodies.iter() {
             check(unsafe { newt_model_add_body(m.raw, &lower_body(b), std::ptr::null_mut()) })?;
+        }
+        for mesh in world.meshes.iter() {
+            let v: Vec<f64> = mesh.vertices.iter().flat_map(|p| [p.x as f64, p.y as f64, p.z as f64]).collect();
+            let f: Vec<i32> = mesh.faces.iter().flat_map(|t| [t[0] as i32, t[1] as i32, t[2] as i32]).collect();
+            check(unsafe {
+                newt_model_add_mesh(m.raw, v.as_ptr(), mesh.vertices.len() as i32, f.as_ptr(), mesh.faces.len() as i32,
+                                    std::ptr::null_mut())
+            })?;
         }
         for g in world.geoms.iter() {
             check(unsafe { newt_model_add_geom(m.raw, &lower_geom(g), std::ptr::null_mut()) })?;

@@ -25,6 +25,7 @@ pub const NEWT_SHAPE_PLANE: i32 = 0;
 pub const NEWT_SHAPE_SPHERE: i32 = 1;
 pub const NEWT_SHAPE_BOX: i32 = 2;
 pub const NEWT_SHAPE_CAPSULE: i32 = 3;
+pub const NEWT_SHAPE_MESH: i32 = 4;
 pub const NEWT_INTEGRATOR_EULER: i32 = 0;
 pub const NEWT_INTEGRATOR_IMPLICITFAST: i32 = 1;
 pub const NEWT_INTEGRATOR_RK4: i32 = 2;
@@ -138,6 +139,7 @@ pub struct NewtGeomDesc {
     pub group: u32,
     pub mask: u32,
     pub body: i32,
+    pub mesh: i32,
 }
 
 #[repr(C)]
@@ -175,6 +177,8 @@ extern "C" {
                                idx: *mut i32) -> NewtStatus;
     pub fn newt_model_add_body(m: *mut NewtModel, b: *const NewtBodyDesc, idx: *mut i32) -> NewtStatus;
     pub fn newt_model_add_geom(m: *mut NewtModel, g: *const NewtGeomDesc, idx: *mut i32) -> NewtStatus;
+    pub fn newt_model_add_mesh(m: *mut NewtModel, vertices: *const f64, n_vertices: i32, faces: *const i32,
+                               n_faces: i32, idx: *mut i32) -> NewtStatus;
     pub fn newt_model_set_contact_capacity(m: *mut NewtModel, max_contacts: i32) -> NewtStatus;
     pub fn newt_model_finalize(m: *mut NewtModel) -> NewtStatus;
     pub fn newt_model_from_json(json: *const c_char, finalize: i32, out: *mut *mut NewtModel) -> NewtStatus;

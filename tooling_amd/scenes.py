@@ -67,13 +67,14 @@ def link(parent, joint, *, axis=(0.0, 0.0, 0.0), parent_pos=(0.0, 0.0, 0.0), par
 
 def geom(shape, *, tree=-1, link=-1, body=-1, size=(0.0, 0.0, 0.0), pos=(0.0, 0.0, 0.0), quat=IDENTITY_Q,
          friction=1.0, margin=0.0, gap=0.0, solref=(0.02, 1.0), solimp=(0.9, 0.95, 0.001, 0.5), power=2,
-         condim=3, group=0xFFFFFFFF, mask=0xFFFFFFFF):
-    """Geom defaults follow Geom::static_plane / *_on_link / r#box (geom.rs:465-700);
-    `body` attaches it to a free body; `group`/`mask` are collision_group /
-    collision_mask (broadphase.rs:16-18)."""
+         condim=3, group=0xFFFFFFFF, mask=0xFFFFFFFF, mesh=-1):
+    """Geom defaults follow Geom::static_plane / *_on_link / r#box / mesh
+    (geom.rs:465-790); `body` attaches it to a free body; `group`/`mask` are
+    collision_group / collision_mask (broadphase.rs:16-18); `mesh` indexes
+    Scene.meshes for SHAPE_MESH."""
     return dict(shape=shape, tree=tree, link=link, body=body, size=size, pos=pos, quat=quat, friction=friction,
                 margin=margin, gap=gap, solref=solref, solimp=solimp, power=power, condim=condim, group=group,
-                mask=mask)
+                mask=mask, mesh=mesh)
 
 
 def body(mass, inertia, position, orientation=IDENTITY_Q, linear_velocity=(0.0, 0.0, 0.0),
@@ -107,6 +108,9 @@ class Scene:
     # Further trees of the world (World::trees[1..]): (links, self_collide)
     # each; geoms name their tree by index.
     more_trees: List[tuple] = field(default_factory=list)
+    # Convex meshes (World::meshes): (vertices [n, 3], faces [m, 3]) each;
+    # mesh geoms name theirs by index.
+    meshes: List[tuple] = field(default_factory=list)
 
     @property
     def all_links(self):
@@ -195,6 +199,7 @@ class Scene:
             d.group = g.get("group", 0xFFFFFFFF)
             d.mask = g.get("mask", 0xFFFFFFFF)
             d.body = g.get("body", -1)
+            d.mesh = g.get("mesh", -1)
             out.append(d)
         return out
 
@@ -480,7 +485,8 @@ def with_options(sc: Scene, suffix: str, **options) -> Scene:
     o = dict(sc.options)
     o.update(options)
     return Scene(sc.name + suffix, o, sc.links, sc.geoms, sc.self_collide, sc.sampler,
-                 f"{sc.description}; options {options}", dict(sc.extra), list(sc.bodies))
+                 f"{sc.description}; options {options}", dict(sc.extra), list(sc.bodies), list(sc.sensors),
+                 sc.magnetic_field, list(sc.more_trees), list(sc.meshes))
 
 
 def _body_state(bodies):
@@ -589,6 +595,120 @@ def box_pile(n_side=4, dt=0.002, iterations=20, seed=2026, friction=0.8) -> Scen
                 iterations=iterations)
     return Scene(f"boxpile{nb}", opts, [], geoms, 0, sampler,
                  f"{nb} free boxes dropped on a plane, PGS {iterations}, Euler dt {dt}", bodies=bodies)
+
+
+# ---------------------------------------------------------------------------
+# Convex meshes (GeomShape::Mesh): GJK/EPA mesh-mesh, mesh-plane and
+# sphere-mesh contacts.
+# ---------------------------------------------------------------------------
+TETRA_MESH = (np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]]),
+              np.array([[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]], dtype=np.int32))
+
+
+def cube_mesh(h):
+    """A cube of half extent h as 8 vertices and 12 outward-wound triangles."""
+    v = np.array([[sx * h, sy * h, sz * h] for sz in (-1, 1) for sy in (-1, 1) for sx in (-1, 1)], dtype=np.float64)
+    f = np.array([[0, 3, 1], [0, 2, 3], [4, 5, 7], [4, 7, 6], [0, 1, 5], [0, 5, 4],
+                  [2, 6, 7], [2, 7, 3], [0, 4, 6], [0, 6, 2], [1, 3, 7], [1, 7, 5]], dtype=np.int32)
+    return v, f
+
+
+def mesh_tumble(rotated=False) -> Scene:
+    """newt's dynamic mesh anchor world (newt/tests/contacts_geoms_v1.rs:
+    2194-2229): a static unit tetrahedron and a free one (mass 1, principal
+    inertia 0.166667) overlapping it and spinning, zero gravity, dt 5 ms,
+    World::new's RK4 and penalty contacts."""
+    if rotated:
+        pos, q, w = (0.25, 0.1, 0.25), quat_axis_angle((0.0, 0.0, 1.0), math.radians(0.4)), (-0.6, 0.9, 0.5)
+    else:
+        pos, q, w = (0.2, 0.2, 0.2), IDENTITY_Q, (1.0, 0.7, -0.4)
+    i = 0.166667
+    b = body(1.0, _diag(i, i, i), pos, q, angular_velocity_body=w)
+    geoms = [geom(abi.SHAPE_MESH, mesh=0, friction=0.5), geom(abi.SHAPE_MESH, body=0, mesh=0, friction=0.5)]
+    qpos = np.array([list(pos) + list(q)])
+    qvel = np.array([[0.0, 0.0, 0.0] + list(w)])
+    opts = dict(dt=0.005, gravity=(0.0, 0.0, 0.0), integrator=abi.INTEGRATOR_RK4, solver=abi.SOLVER_PENALTY,
+                iterations=20)
+    return Scene("mesh_tumble" + ("_rotated" if rotated else ""), opts, [], geoms, 0,
+                 lambda n, seed: (np.repeat(qpos, n, 0), np.repeat(qvel, n, 0)),
+                 "a tetrahedron tumbling inside another (penalty, RK4)", bodies=[b], meshes=[TETRA_MESH])
+
+
+def mesh_pile(dt=0.002, iterations=30, seed=7, friction=0.8) -> Scene:
+    """Free convex meshes dropped onto a plane: three cubes (half extent 0.1),
+    three tetrahedra (edge 0.3) and a sphere (r 0.08) in a jittered stack
+    0.33-0.4 m apart, uniform random orientations per world; PGS, pyramidal cone,
+    Euler.  Mesh-mesh (GJK/EPA and the face manifold), mesh-plane and
+    sphere-mesh contacts."""
+    cube = cube_mesh(0.1)
+    tet = (TETRA_MESH[0] * 0.3 - 0.075, TETRA_MESH[1])
+    kinds = ["cube", "tet", "cube", "tet", "sphere", "cube", "tet"]
+    bodies, geoms = [], [geom(abi.SHAPE_PLANE, friction=friction)]
+    spots = [(0.0, 0.0, 0.15), (0.05, 0.02, 0.55), (-0.04, 0.06, 0.95), (0.02, -0.05, 1.35), (0.0, 0.03, 1.72),
+             (0.06, 0.0, 2.05), (-0.03, -0.02, 2.45)]
+    for b, k in enumerate(kinds):
+        if k == "cube":
+            m = 8.0 * 0.1 ** 3 * 500.0
+            bodies.append(body(m, solid_box_inertia(m, (0.1, 0.1, 0.1)), spots[b]))
+            geoms.append(geom(abi.SHAPE_MESH, body=b, mesh=0, friction=friction))
+        elif k == "tet":
+            m = 2.0
+            bodies.append(body(m, solid_box_inertia(m, (0.08, 0.08, 0.08)), spots[b]))
+            geoms.append(geom(abi.SHAPE_MESH, body=b, mesh=1, friction=friction))
+        else:
+            m = 1.0
+            ii = 0.4 * m * 0.08 ** 2
+            bodies.append(body(m, _diag(ii, ii, ii), spots[b]))
+            geoms.append(geom(abi.SHAPE_SPHERE, body=b, size=(0.08, 0.0, 0.0), friction=friction))
+    nb = len(kinds)
+    grid = np.array(spots)
+
+    def sampler(n, s):
+        rr = _rng(s)
+        q = np.zeros((n, 7 * nb))
+        v = np.zeros((n, 6 * nb))
+        for b in range(nb):
+            q[:, 7 * b:7 * b + 3] = grid[b] + rr.uniform(-0.03, 0.03, size=(n, 3))
+            q[:, 7 * b + 3:7 * b + 7] = random_quaternions(rr, n)
+        return q, v
+
+    opts = dict(dt=dt, gravity=(0.0, 0.0, -9.81), integrator=abi.INTEGRATOR_EULER, solver=abi.SOLVER_PGS,
+                iterations=iterations)
+    return Scene("meshpile7", opts, [], geoms, 0, sampler,
+                 "cube and tetrahedron meshes and a sphere dropped on a plane, PGS, Euler", bodies=bodies,
+                 meshes=[cube, tet])
+
+
+def mesh_tree(dt=0.002, iterations=30) -> Scene:
+    """A free-root tree whose root carries a cube mesh (half extent 0.12) and a
+    hinged child a tetrahedron mesh, dropped onto a plane beside a static cube
+    mesh obstacle; self collision on.  PGS, RK4.  Tree mesh-plane and
+    link-static and link-link mesh-mesh contacts."""
+    cube = cube_mesh(0.12)
+    tet = (TETRA_MESH[0] * 0.25, TETRA_MESH[1])
+    links = [link(-1, abi.JOINT_FREE, mass=3.0, inertia=solid_box_inertia(3.0, (0.12, 0.12, 0.12))),
+             link(0, abi.JOINT_HINGE, axis=(0.0, 1.0, 0.0), parent_pos=(0.15, 0.0, -0.05), child_pos=(0.0, 0.0, 0.1),
+                  mass=0.8, inertia=solid_box_inertia(0.8, (0.08, 0.08, 0.08)), damping=0.05)]
+    geoms = [geom(abi.SHAPE_PLANE, friction=0.9),
+             geom(abi.SHAPE_MESH, tree=0, link=0, mesh=0, friction=0.9),
+             geom(abi.SHAPE_MESH, tree=0, link=1, mesh=1, pos=(0.0, 0.0, -0.1), friction=0.9),
+             geom(abi.SHAPE_MESH, mesh=0, pos=(0.35, 0.0, 0.12), friction=0.9)]
+
+    def sampler(n, s):
+        rr = _rng(s)
+        q = np.zeros((n, 8))
+        q[:, :3] = np.array([0.05, 0.0, 0.5]) + rr.uniform(-0.05, 0.05, size=(n, 3))
+        q[:, 3:7] = random_quaternions(rr, n)
+        q[:, 7] = rr.uniform(-0.6, 0.6, size=n)
+        v = np.zeros((n, 7))
+        v[:, :3] = rr.normal(0.0, 0.3, size=(n, 3))
+        return q, v
+
+    opts = dict(dt=dt, gravity=(0.0, 0.0, -9.81), integrator=abi.INTEGRATOR_RK4, solver=abi.SOLVER_PGS,
+                iterations=iterations)
+    return Scene("meshtree", opts, links, geoms, 1, sampler,
+                 "a cube-mesh root with a hinged tetrahedron mesh dropped beside a static cube mesh, PGS, RK4",
+                 meshes=[cube, tet])
 
 
 SCENES = {"pendulum3": pendulum, "quadruped18": quadruped, "boxchain6": box_chain, "coil10": hinge_coil,
