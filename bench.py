@@ -424,26 +424,32 @@ def bench_chains(args, worlds=16384, settle=100, steps=100):
 
 def bench_serve(args, n_requests=32):
     """C5: rollout requests arriving as a Poisson process (`--rate` per
-    second): model uniform over the 3-link pendulum, the quadruped and
-    fixed-root hinge chains of U{2..64} links, worlds ~ U{1..512}, horizon
-    ~ U{100..2000} steps; continuous batching over resident pools, mixed
-    then FP64.  Latency is arrival to results on the host."""
+    second): model uniform over the C1-C4 families (the 3-link pendulum, the
+    quadruped, the 64-box pile under Newton, fixed-root hinge chains of
+    U{2..64} links), worlds ~ U{1..512}, horizon ~ U{100..2000} steps;
+    continuous batching over pools allocated on demand under a 200 GB
+    device budget (idle pools freed when the budget is short), mixed then
+    FP64.  Latency is arrival to results on the host."""
     import numpy as np
     from tooling_amd import abi, serve
 
     def requests():
-        return serve.rollout_mix(n_requests, args.seed, rate=args.rate)
+        return serve.rollout_mix(n_requests, args.seed, rate=args.rate,
+                                 families=("pendulum3", "quadruped18", "boxes", "chain"), box_worlds=(1, 512))
 
     reqs = requests()
     keys = sorted({r.key for r in reqs})
     total = sum(r.n_worlds * r.horizon for r in reqs)
     log(f"serve: {len(reqs)} requests over {len(keys)} models at {args.rate}/s, {total} world-steps")
-    pools = {k: (serve.scene_for(k), 2048 if k.startswith("chain") else 8192) for k in keys}
+    pools = {k: (serve.scene_for(k), 2048 if k.startswith("chain") else 1024 if k.startswith("boxpile") else 8192)
+             for k in keys}
     out = {"workload": f"serve (C5): {len(reqs)} rollout requests, Poisson {args.rate}/s, model ~ U{{pendulum3, "
-                       f"quadruped18, chain2..64}}, worlds ~ U{{1..512}}, horizon ~ U{{100..2000}}, seed "
-                       f"{args.seed}", "world_steps": total, "models": len(keys)}
+                       f"quadruped18, boxpile64 (Newton), chain2..64}}, worlds ~ U{{1..512}}, horizon ~ "
+                       f"U{{100..2000}}, pools on demand under a 200 GB budget, seed {args.seed}",
+           "world_steps": total, "models": len(keys)}
     for name, precision in (("mixed", abi.PRECISION_MIXED), ("fp64", abi.PRECISION_FP64)):
-        srv = serve.Server(pools, precision=precision, chunk=400, n_streams=4, quantum=0.05)
+        srv = serve.Server(pools, precision=precision, chunk=400, n_streams=4, quantum=0.05, lazy=True,
+                           hbm_budget=200 << 30)
         try:
             srv.run([serve.Request(k, r.qpos[:1], r.qvel[:1], 1) for k in keys
                      for r in [next(x for x in reqs if x.key == k)]])
@@ -451,13 +457,16 @@ def bench_serve(args, n_requests=32):
             rs = requests()
             elapsed = srv.run(rs, time_limit=600.0)
             faults = int(sum(int((r.faults != 0).sum()) for r in rs))
+            codes = np.concatenate([r.faults[r.faults != 0] for r in rs] + [np.zeros(0, dtype=np.int32)])
             by_family = {}
             for r in rs:
                 fam = "chain" if r.key.startswith("chain") else r.key
                 by_family.setdefault(fam, []).append(r.latency * 1e3)
             out[name] = {"value": total / elapsed, "elapsed_s": elapsed, "latency_ms": serve.latency_summary(rs),
                          "latency_p99_ms_by_family": {f: float(np.percentile(v, 99)) for f, v in by_family.items()},
-                         "launches": srv.launches, "device_bytes": srv.device_bytes, "faults": faults}
+                         "launches": srv.launches, "device_bytes": srv.device_bytes, "faults": faults,
+                         "fault_codes": {int(c): int((codes == c).sum()) for c in np.unique(codes)},
+                         "pools_created": srv.created, "pools_evicted": srv.evicted}
         finally:
             srv.close()
         log(f"serve {name}: {out[name]['value']:.4g} world-steps/s, p50 {out[name]['latency_ms']['p50_ms']:.1f} ms, "

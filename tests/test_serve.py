@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from tooling_amd import abi, serve
-from tooling_amd.newt import Data, step
+from tooling_amd.newt import Data, Model, step
 
 
 def say(msg):
@@ -182,3 +182,37 @@ def test_server_rejects_what_cannot_be_served():
         srv.close()
     with pytest.raises(MemoryError):
         serve.Server({"pendulum3": (sc, 1024)}, precision=abi.PRECISION_FP64, hbm_budget=1 << 10)
+
+
+@pytest.mark.gpu
+def test_lazy_pools_admit_as_memory_frees():
+    """Pools allocated on demand under a device budget that holds only one of
+    them at a time: the quadruped's requests wait until the pendulum pool is
+    idle and freed, then run; every result still equals the request stepped
+    alone, and pools were created and evicted along the way."""
+    reqs = _mix()
+    keys = ["quadruped18", "pendulum3", "chain8"]
+    specs = {k: (serve.scene_for(k), 48 if k == "quadruped18" else 64 if k == "pendulum3" else 32) for k in keys}
+    probe = serve.Server(specs, precision=abi.PRECISION_MIXED, chunk=16, n_streams=2, lazy=True)
+    need = {k: probe._probe(k)[1] * specs[k][1] for k in keys}
+    probe.close()
+    budget = max(need.values()) + min(need.values()) // 2   # one pool at a time
+    say(f"lazy pools: bytes {need}, budget {budget}")
+    srv = serve.Server(specs, precision=abi.PRECISION_MIXED, chunk=16, n_streams=2, hbm_budget=budget, lazy=True)
+    try:
+        srv.run(reqs, time_limit=120.0)
+        say(f"created {srv.created}, evicted {srv.evicted}, launches {srv.launches}")
+        assert srv.created >= len(keys) and srv.evicted >= 1
+        assert srv.device_bytes <= budget
+        for r in reqs:
+            assert r.result is not None and (r.faults == 0).all()
+            m = Model(serve.scene_for(r.key))
+            d = Data(m, r.n_worlds, abi.PRECISION_MIXED)
+            d.set_state(r.qpos, r.qvel)
+            step(m, d, r.horizon)
+            q, v = d.state()
+            d.close()
+            m.close()
+            assert np.array_equal(q, r.result[0]) and np.array_equal(v, r.result[1])
+    finally:
+        srv.close()

@@ -637,10 +637,10 @@ struct BodyWaveLayout {
     int nn, g_H, g_L, g_Lt, g_vec, g_brk, g_cand;  // the factor row- and column-major
 };
 
-// Rows of one world's Newton system at most (1024 = 256 contacts): H and its
-// factor in both orders take 3 x 8 MiB per world.  A world with more active rows faults
-// with NEWT_ERR_CAPACITY.
-constexpr int kNewtonMaxRows = 1024;
+// Rows of one world's Newton system at most (2048 = 512 contacts): H and its
+// factor in both orders take 3 x 32 MiB per world at that size (3 x (4 cap_c)^2
+// doubles below it).  A world with more active rows faults with NEWT_ERR_CAPACITY.
+constexpr int kNewtonMaxRows = 2048;
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 // Doubles in global memory.  The Newton workspace is addressed through these:
@@ -1355,9 +1355,14 @@ struct BodyWave {
         return 0.5 * dot_seq(x, hv, n) + dot_seq(lin, x, n);
     }
     // Impulses of rows 0..4 nblk into rows[7 ri + 6]; 0 or a fault code.
+    // The substitutions keep a lane's share of a vector (rows lane + 64 c) in
+    // C registers, so n <= 64 C: C = 16 up to 1024 rows, C = 32 (whose
+    // arrays the compiler spills) up to 2048.
+    template <int C>
     __device__ __noinline__ int newton(int nblk) const {
+        static_assert(C == 16 || C == 32, "16 or 32 row chunks");
         const int n = 4 * nblk;
-        if (n > B.nn) return 8;  // NEWT_ERR_CAPACITY
+        if (n > B.nn || n > 64 * C) return 8;  // NEWT_ERR_CAPACITY
         const int nn = B.nn;
         gf64* H = (gf64*)(gd + B.g_H);
         gf64* Lf = (gf64*)(gd + B.g_L);   // the factor, row-major
@@ -1453,11 +1458,13 @@ struct BodyWave {
         // substitution) and row-major (Lf, read by the back substitution).
         constexpr int P = 16;
         const bool mfma_left = !exact;
-        // Lane R holds row block R's mask of 16-column blocks of L with a
-        // nonzero entry (rows 16 R .. 16 R + 15; n <= 1024 gives R < 64), set
-        // as each panel is written.  The matrix-core left part visits only
-        // the column blocks where both the panel's rows and the chunk's rows
-        // have one: every skipped product is an exact zero.
+        // Lane R holds row block R's mask of column blocks of L with a
+        // nonzero entry, set as each panel is written.  A block is 16 rows
+        // or columns up to 1024 rows and 32 up to 2048 (SH = log2 of the
+        // block), so R < 64 and a mask fits 64 bits.  The matrix-core left
+        // part visits only the column blocks where both the panel's rows and
+        // the chunk's rows have one: every skipped product is an exact zero.
+        constexpr int SH = C == 16 ? 4 : 5;
         unsigned long long rmask = 0;
         for (int i = lane; i < n; i += 64) pt[i] = H[(size_t)i * n + i];
         gsync();
@@ -1490,10 +1497,18 @@ struct BodyWave {
                     f64x4 t[4];
 #pragma unroll
                     for (int T = 0; T < 4; ++T) t[T] = f64x4{0.0, 0.0, 0.0, 0.0};
-                    const int kj = j0 >> 4, r16 = r0 >> 4;
-                    unsigned long long km = row_mask(rmask, kj) & (row_mask(rmask, r16) | row_mask(rmask, r16 + 1) |
-                                                                   row_mask(rmask, r16 + 2) | row_mask(rmask, r16 + 3));
-                    km &= (1ull << kj) - 1;  // column blocks left of the panel (kj <= 63)
+                    // The panel's rows lie in block j0 >> SH, the chunk's in
+                    // blocks r0 >> SH .. (r0 + 63) >> SH.
+                    unsigned long long rows_m = 0;
+                    for (int rb = r0 >> SH; rb <= (r0 + 63) >> SH; ++rb) rows_m |= row_mask(rmask, rb);
+                    unsigned long long km = row_mask(rmask, j0 >> SH) & rows_m;
+                    // Column blocks holding a column left of the panel.
+                    const int nbl = (j0 + (1 << SH) - 1) >> SH;
+                    if (nbl < 64) km &= (1ull << nbl) - 1;
+                    // Groups of sixteen columns go by 1 << GS per block, the
+                    // first j0 / 16 of them left of the panel.
+                    constexpr int GS = SH - 4;
+                    const int ngl = j0 >> 4;
                     const int lr = lane & 15, lk = lane >> 4;
                     int ar[4];
 #pragma unroll
@@ -1510,14 +1525,21 @@ struct BodyWave {
                             for (int T = 0; T < 4; ++T) a[u][T] = col[ar[T]];
                         }
                     };
-                    int kb = km ? __builtin_ctzll(km) : 0;
-                    if (km) load_group(16 * kb, ca, cb);
-                    while (km) {
-                        // The next visited group (the same group again at the
-                        // end: every load stays in bounds and unconditional).
-                        km &= km - 1;
-                        kb = km ? __builtin_ctzll(km) : kb;
-                        load_group(16 * kb, na, nb);
+                    int kb = km ? __builtin_ctzll(km) << GS : 0;
+                    bool more = km != 0;
+                    if (more) load_group(16 * kb, ca, cb);
+                    while (more) {
+                        // The next visited group: the next one of the same
+                        // block while it is left of the panel, else the first
+                        // of the next block (the same group again at the end:
+                        // every load stays in bounds and unconditional).
+                        int nk = kb + 1;
+                        if ((nk & ((1 << GS) - 1)) == 0 || nk >= ngl) {
+                            km &= km - 1;
+                            more = km != 0;
+                            nk = more ? __builtin_ctzll(km) << GS : kb;
+                        }
+                        load_group(16 * nk, na, nb);
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
                             const double bu = bon ? cb[u] : 0.0;
@@ -1530,6 +1552,7 @@ struct BodyWave {
 #pragma unroll
                             for (int T = 0; T < 4; ++T) ca[u][T] = na[u][T];
                         }
+                        kb = nk;
                     }
                     // To one row per lane: lane 16 q + m holds rows 16 T + m
                     // (all T) at columns 4 r + q; lane 16 q + m wants row
@@ -1614,8 +1637,11 @@ struct BodyWave {
 #pragma unroll
                     for (int c = 0; c < P; ++c) nz |= q[c] != 0.0;
                     const unsigned long long bm = __ballot(nz);
-                    const int g = lane - (r0 >> 4);
-                    if (g >= 0 && g < 4 && ((bm >> (16 * g)) & 0xffffull) != 0) rmask |= 1ull << (j0 >> 4);
+                    // Rows r0 + 16 g .. r0 + 16 g + 15 lie in block (r0 + 16 g) >> SH.
+#pragma unroll
+                    for (int g = 0; g < 4; ++g)
+                        if (lane == (r0 + 16 * g) >> SH && ((bm >> (16 * g)) & 0xffffull) != 0)
+                            rmask |= 1ull << (j0 >> SH);
                 }
 #pragma unroll
                 for (int c = 0; c < P; ++c) {
@@ -1656,9 +1682,9 @@ struct BodyWave {
             // terms in k order, all sixteen factor loads of a row in flight
             // together.
             {
-                double yr[16];
+                double yr[C];
 #pragma unroll
-                for (int c = 0; c < 16; ++c) {
+                for (int c = 0; c < C; ++c) {
                     const int i = 64 * c + lane;
                     yr[c] = i < n ? y[i] : 0.0;
                 }
@@ -1671,7 +1697,7 @@ struct BodyWave {
                     for (int u = 0; u < 16; ++u) lp[u] = u < kw ? Lt[(size_t)(k0 + u) * n + rpc] : 0.0;
                     double ycp = 0.0;
 #pragma unroll
-                    for (int c = 0; c < 16; ++c)
+                    for (int c = 0; c < C; ++c)
                         if (c == cp) ycp = yr[c];
 #pragma unroll
                     for (int u = 0; u < 16; ++u) {
@@ -1685,10 +1711,10 @@ struct BodyWave {
                         }
                     }
 #pragma unroll
-                    for (int c = 0; c < 16; ++c)
+                    for (int c = 0; c < C; ++c)
                         if (c == cp) yr[c] = ycp;
 #pragma unroll
-                    for (int c = 0; c < 16; ++c) {
+                    for (int c = 0; c < C; ++c) {
                         const int i = 64 * c + lane;
                         if (c > cp && 64 * c < n) {
                             const int ic = i < n ? i : n - 1;
@@ -1704,7 +1730,7 @@ struct BodyWave {
                     }
                 }
 #pragma unroll
-                for (int c = 0; c < 16; ++c) {
+                for (int c = 0; c < C; ++c) {
                     const int i = 64 * c + lane;
                     if (i < n) y[i] = yr[c];
                 }
@@ -1718,9 +1744,9 @@ struct BodyWave {
                 // rows' loads in flight together.  Each sum takes its terms in
                 // decreasing i, not newt's increasing order: within the
                 // drop-in's tolerance.
-                double yr[16];
+                double yr[C];
 #pragma unroll
-                for (int c = 0; c < 16; ++c) {
+                for (int c = 0; c < C; ++c) {
                     const int k = 64 * c + lane;
                     yr[c] = k < n ? y[k] : 0.0;
                 }
@@ -1733,7 +1759,7 @@ struct BodyWave {
                     for (int u = 0; u < 16; ++u) lp[u] = u < iw ? Lf[(size_t)(i0 + u) * n + kpc] : 0.0;
                     double ycp = 0.0;
 #pragma unroll
-                    for (int c = 0; c < 16; ++c)
+                    for (int c = 0; c < C; ++c)
                         if (c == cp) ycp = yr[c];
 #pragma unroll
                     for (int u = 15; u >= 0; --u) {
@@ -1747,10 +1773,10 @@ struct BodyWave {
                         }
                     }
 #pragma unroll
-                    for (int c = 0; c < 16; ++c)
+                    for (int c = 0; c < C; ++c)
                         if (c == cp) yr[c] = ycp;
 #pragma unroll
-                    for (int c = 0; c < 16; ++c) {
+                    for (int c = 0; c < C; ++c) {
                         const int k = 64 * c + lane;
                         if (c < cp) {
                             double l[16];
@@ -1763,28 +1789,28 @@ struct BodyWave {
                     }
                 }
 #pragma unroll
-                for (int c = 0; c < 16; ++c) {
+                for (int c = 0; c < C; ++c) {
                     const int k = 64 * c + lane;
                     if (k < n) dd[k] = yr[c];
                 }
             } else
             // backward: x[i] = (y[i] - sum_{k>i} L[k][i] x[k]) / L[i][i], the sum
             // in increasing k.  Lane k mod 64 keeps x[k] in a register (chunk
-            // k / 64 of at most 16) and forms the product L[k][i] x[k]; every
+            // k / 64 of at most C) and forms the product L[k][i] x[k]; every
             // lane runs the subtraction chain on the broadcast products in k
             // order.  Row i - 1's factor entries, y[i - 1] and L[i-1][i-1] load
             // while row i is summed, so no load sits on the chain.
             {
-                double xr[16], lr[16], ln[16];
+                double xr[C], lr[C], ln[C];
 #pragma unroll
-                for (int c = 0; c < 16; ++c) {
+                for (int c = 0; c < C; ++c) {
                     xr[c] = 0.0;
                     lr[c] = 0.0;
                 }
                 auto load_row = [&](int i, double* out) {
                     const auto lti = Lt + (size_t)i * n;
 #pragma unroll
-                    for (int c = 0; c < 16; ++c) {
+                    for (int c = 0; c < C; ++c) {
                         const int k = 64 * c + lane;
                         out[c] = (k > i && k < n) ? lti[k] : 0.0;
                     }
@@ -1804,7 +1830,7 @@ struct BodyWave {
                     }
                     double sum = yi;
 #pragma unroll
-                    for (int c = 0; c < 16; ++c) {
+                    for (int c = 0; c < C; ++c) {
                         const int b0 = 64 * c;
                         if (b0 + 63 <= i || b0 >= n) continue;
                         const double p = lr[c] * xr[c];
@@ -1823,14 +1849,14 @@ struct BodyWave {
                     }
                     const double xi = sum / di;
 #pragma unroll
-                    for (int c = 0; c < 16; ++c)
+                    for (int c = 0; c < C; ++c)
                         if (64 * c + lane == i) xr[c] = xi;
                     if (lane == (i & 63)) dd[i] = xi;
                     if (i > 0) {
                         yi = ny;
                         di = nd;
 #pragma unroll
-                        for (int c = 0; c < 16; ++c) lr[c] = ln[c];
+                        for (int c = 0; c < C; ++c) lr[c] = ln[c];
                     }
                 }
             }
@@ -2019,7 +2045,7 @@ struct BodyWave {
         gsync();
         if (nblk == 0) return 0;
         if (m.mode == NEWT_SOLVER_NEWTON) {
-            const int st = newton(nblk);
+            const int st = nblk <= 256 ? newton<16>(nblk) : newton<32>(nblk);
             if (st != 0) return st;
         } else {
             if (lane == 0)

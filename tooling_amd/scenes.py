@@ -520,6 +520,41 @@ def box_stack(n_boxes=3, dt=0.005, iterations=30, integrator=abi.INTEGRATOR_RK4)
                  f"{n_boxes} stacked free boxes on a plane, PGS {iterations}", bodies=bodies)
 
 
+def box_field(n_x=10, n_y=5, levels=2, half=0.1, spacing=0.35, dt=0.002, iterations=20, yaw=0.2) -> Scene:
+    """n_x x n_y stacks of `levels` cubes (half extent `half`, 1 kg) resting
+    on a plane 0.1 mm into contact, stacks `spacing` apart; Newton, pyramidal,
+    Euler.  Every cube rests on a face, so a world holds about 4 contacts per
+    cube (at the defaults 100 cubes, about 400 contacts and 1600 Newton
+    rows).  Worlds differ by a random yaw (up to `yaw` radians) and a
+    lateral offset (up to 5 mm) per cube."""
+    bodies, geoms = [], [geom(abi.SHAPE_PLANE, friction=0.8)]
+    hs = (half, half, half)
+    for i in range(n_x):
+        for j in range(n_y):
+            for k in range(levels):
+                p = ((i - (n_x - 1) / 2) * spacing, (j - (n_y - 1) / 2) * spacing, half - 1e-4 + k * (2 * half - 1e-4))
+                geoms.append(geom(abi.SHAPE_BOX, body=len(bodies), size=hs, friction=0.8))
+                bodies.append(body(1.0, solid_box_inertia(1.0, hs), p))
+    q0, v0 = _body_state(bodies)
+    nb = len(bodies)
+
+    def sampler(n, seed):
+        r = _rng(seed)
+        q = np.tile(q0, (n, 1))
+        v = np.tile(v0, (n, 1))
+        for b in range(nb):
+            q[:, 7 * b:7 * b + 2] += r.uniform(-0.005, 0.005, size=(n, 2))
+            z = r.uniform(-yaw, yaw, size=n)
+            q[:, 7 * b + 3:7 * b + 7] = np.stack([np.zeros(n), np.zeros(n), np.sin(z / 2), np.cos(z / 2)], axis=1)
+        return q, v
+
+    opts = dict(dt=dt, gravity=(0.0, 0.0, -9.81), integrator=abi.INTEGRATOR_EULER, solver=abi.SOLVER_NEWTON,
+                iterations=iterations)
+    return Scene(f"boxfield{nb}", opts, [], geoms, 0, sampler,
+                 f"{n_x}x{n_y} stacks of {levels} cubes resting on a plane, Newton {iterations}, Euler dt {dt}",
+                 bodies=bodies)
+
+
 def penalty_stack(dt=0.005, integrator=abi.INTEGRATOR_RK4) -> Scene:
     """newt's penalty contact golden scene (newt/tests/contacts_golden.rs:31-59):
     three 1 kg boxes of half extent 0.35 dropped onto a plane (friction 0.6,

@@ -5,8 +5,15 @@ serving layer.  This module runs many rollout requests of different models,
 world counts and horizons at once, with their state resident in device
 memory:
 
-* A *pool* per model: one ``newt_data`` whose ``capacity`` world slots are
-  allocated once, when the server starts, inside a device-memory budget.
+* A *pool* per model: one ``newt_data`` of ``capacity`` world slots.  With
+  ``lazy=False`` every pool is allocated when the server starts, inside the
+  device-memory budget.  With ``lazy=True`` a pool is allocated when the
+  first request for its model arrives, if its device bytes (measured once
+  per model on a one-world probe, times the capacity) fit what the budget
+  has left; otherwise the least recently used idle pools (no request queued
+  or running) are freed first, and if that is still not enough the requests
+  wait until running pools finish and become idle.  Admission therefore
+  follows the memory actually free, not a start-up reservation.
 * A request (model key, initial qpos/qvel of its worlds, horizon) is admitted
   when its pool has a contiguous free slot range (first fit) and is uploaded
   with ``newt_data_set_worlds_async``.  Requests that arrive while others run
@@ -164,7 +171,8 @@ class Server:
     """
 
     def __init__(self, pools: Dict[str, tuple], precision: int = abi.PRECISION_MIXED, chunk: int = 50,
-                 n_streams: int = 4, hbm_budget: int = 200 << 30, quantum: Optional[float] = None):
+                 n_streams: int = 4, hbm_budget: int = 200 << 30, quantum: Optional[float] = None,
+                 lazy: bool = False):
         if hip.device_count() < 1:
             raise RuntimeError("the serving loop needs a GPU")
         if chunk < 1:
@@ -173,16 +181,24 @@ class Server:
         self.chunk = chunk
         self.quantum = quantum
         self.hbm_budget = hbm_budget
+        self.specs: Dict[str, tuple] = dict(pools)
+        self.lazy = lazy
         self.pools: Dict[str, Pool] = {}
         self.device_bytes = 0
+        self.pool_bytes: Dict[str, int] = {}
+        self.pending: Dict[str, collections.deque] = {}  # requests whose pool is not allocated
+        self.last_use: Dict[str, int] = {}
+        self.tick = 0
+        self.created = 0
+        self.evicted = 0
+        self._probes: Dict[str, tuple] = {}
         try:
-            for key, (scene, capacity) in pools.items():
-                p = Pool(key, scene, capacity, precision)
-                self.pools[key] = p
-                self.device_bytes += p.data.device_bytes()
-                if self.device_bytes > hbm_budget:
-                    raise MemoryError(f"pools need more than the {hbm_budget} B device budget "
-                                      f"({self.device_bytes} B after pool {key!r})")
+            if not lazy:
+                for key in pools:
+                    self._create(key)
+                    if self.device_bytes > hbm_budget:
+                        raise MemoryError(f"pools need more than the {hbm_budget} B device budget "
+                                          f"({self.device_bytes} B after pool {key!r})")
         except BaseException:
             self.close()
             raise
@@ -194,6 +210,54 @@ class Server:
         for p in self.pools.values():
             p.close()
         self.pools = {}
+        for m, bpw in self._probes.values():
+            m.close()
+        self._probes = {}
+
+    # -- pools under the device budget -------------------------------------
+    def _probe(self, key: str):
+        """(model, device bytes per world slot) of a model, measured once on a
+        one-world data (model tables included, so the estimate is an upper
+        bound of a pool's bytes per slot)."""
+        if key not in self._probes:
+            m = Model(self.specs[key][0])
+            d = Data(m, 1, self.precision)
+            bpw = d.device_bytes()
+            d.close()
+            self._probes[key] = (m, bpw)
+        return self._probes[key]
+
+    def _create(self, key: str):
+        scene, capacity = self.specs[key]
+        p = Pool(key, scene, capacity, self.precision)
+        self.pools[key] = p
+        self.pool_bytes[key] = p.data.device_bytes()
+        self.device_bytes += self.pool_bytes[key]
+        self.created += 1
+        return p
+
+    def _idle(self, p: Pool) -> bool:
+        return not p.busy and not p.active and not p.queue
+
+    def _materialize(self):
+        """Allocate the pools that pending requests wait for, oldest request
+        first, freeing idle pools (least recently used first) when the budget
+        is short."""
+        waiting = sorted((q[0].arrival, k) for k, q in self.pending.items() if q)
+        for _, key in waiting:
+            need = self._probe(key)[1] * self.specs[key][1]
+            if self.device_bytes + need > self.hbm_budget:
+                idle = sorted((self.last_use.get(k, -1), k) for k, p in self.pools.items() if self._idle(p))
+                for _, k in idle:
+                    if self.device_bytes + need <= self.hbm_budget:
+                        break
+                    self.pools.pop(k).close()
+                    self.device_bytes -= self.pool_bytes.pop(k)
+                    self.evicted += 1
+            if self.device_bytes + need > self.hbm_budget:
+                continue  # waits for running pools to finish
+            p = self._create(key)
+            p.queue.extend(self.pending.pop(key))
 
     # --------------------------------------------------------------------
     def _finish(self, lane: _Lane, now: Callable[[], float]) -> int:
@@ -236,6 +300,8 @@ class Server:
         if best is None:
             return False
         p, s = best, lane.stream.handle
+        self.tick += 1
+        self.last_use[p.key] = self.tick
         for first, n in p.to_park:
             p.data.park_worlds(first, n, s)
         p.to_park.clear()
@@ -262,24 +328,32 @@ class Server:
         r.admitted = now
         p.active.append([r, first, r.horizon])
 
-    def check(self, r: Request) -> Pool:
-        """The request's pool; raises if the request can never be served."""
-        p = self.pools.get(r.key)
-        if p is None:
+    def check(self, r: Request):
+        """Raises if the request can never be served."""
+        if r.key not in self.specs:
             raise KeyError(f"no pool for model {r.key!r}")
-        if r.qpos.ndim != 2 or r.qvel.ndim != 2 or r.qpos.shape != (r.n_worlds, p.model.nq) \
-                or r.qvel.shape != (r.n_worlds, p.model.nv):
+        p = self.pools.get(r.key)
+        model = p.model if p is not None else self._probe(r.key)[0]
+        capacity = self.specs[r.key][1]
+        if r.qpos.ndim != 2 or r.qvel.ndim != 2 or r.qpos.shape != (r.n_worlds, model.nq) \
+                or r.qvel.shape != (r.n_worlds, model.nv):
             raise ValueError(f"request state shapes {r.qpos.shape}, {r.qvel.shape} do not match "
-                             f"model {r.key!r} (nq {p.model.nq}, nv {p.model.nv})")
-        if r.n_worlds < 1 or r.n_worlds > p.slots.capacity:
+                             f"model {r.key!r} (nq {model.nq}, nv {model.nv})")
+        if r.n_worlds < 1 or r.n_worlds > capacity:
             raise ValueError(f"request of {r.n_worlds} worlds does not fit pool {r.key!r} "
-                             f"of {p.slots.capacity} slots")
+                             f"of {capacity} slots")
         if r.horizon < 1:
             raise ValueError("horizon must be at least 1")
-        return p
+        if self.lazy and self._probe(r.key)[1] * capacity > self.hbm_budget:
+            raise MemoryError(f"pool {r.key!r} alone needs more than the {self.hbm_budget} B budget")
 
     def submit(self, r: Request):
-        self.check(r).queue.append(r)
+        self.check(r)
+        p = self.pools.get(r.key)
+        if p is not None:
+            p.queue.append(r)
+        else:
+            self.pending.setdefault(r.key, collections.deque()).append(r)
 
     def run(self, requests: List[Request], clock: Callable[[], float] = time.perf_counter,
             time_limit: float = float("inf"), progress: Optional[Callable[[int, float], None]] = None) -> float:
@@ -302,6 +376,8 @@ class Server:
                 nxt += 1
             progressed = False
             stamp = lambda: clock() - t0  # noqa: E731
+            if self.pending:
+                self._materialize()
             for lane in self.lanes:
                 if lane.pool is not None and lane.event.query():
                     n_done = self._finish(lane, stamp)
@@ -318,11 +394,13 @@ class Server:
 
 
 def rollout_mix(n_requests: int, seed: int, rate: float = 200.0, families=("pendulum3", "quadruped18", "chain"),
-                worlds=(1, 512), horizon=(100, 2000), chain_links=(2, 64)) -> List[Request]:
+                worlds=(1, 512), horizon=(100, 2000), chain_links=(2, 64), box_worlds=(1, 64)) -> List[Request]:
     """Poisson arrivals (`rate` per second) of rollout requests: model family
     uniform over `families` ("chain" draws its length uniformly from
-    `chain_links`), world count and horizon uniform over the inclusive
-    ranges.  Initial states come from each scene's seeded sampler."""
+    `chain_links`; "boxes" is the 64-box pile under Newton, with its world
+    count from `box_worlds`), world count and horizon uniform over the
+    inclusive ranges.  Initial states come from each scene's seeded
+    sampler."""
     from . import scenes
     rng = np.random.Generator(np.random.PCG64(seed))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
@@ -331,9 +409,12 @@ def rollout_mix(n_requests: int, seed: int, rate: float = 200.0, families=("pend
     for i in range(n_requests):
         fam = families[int(rng.integers(len(families)))]
         key = f"chain{int(rng.integers(chain_links[0], chain_links[1] + 1))}" if fam == "chain" else fam
+        if fam == "boxes":
+            key = "boxpile64_newton"
         if key not in cache:
             cache[key] = scene_for(key)
-        n = int(rng.integers(worlds[0], worlds[1] + 1))
+        wr = box_worlds if fam == "boxes" else worlds
+        n = int(rng.integers(wr[0], wr[1] + 1))
         h = int(rng.integers(horizon[0], horizon[1] + 1))
         q, v = cache[key].initial_state(n, seed * 1000003 + i)
         out.append(Request(key, np.asarray(q, dtype=np.float64), np.asarray(v, dtype=np.float64), h,
