@@ -1230,10 +1230,10 @@ template <class T> struct World {
         return out;
     }
 
-    struct Comp {  // WorldTreeJacobian (solver.rs:2768-2774)
-        int tree, link;
+    struct Comp {  // WorldJacobian (solver.rs:2754-2774): a tree link's or a free body's
+        int tree = -1, link = -1, body = -1;
         std::vector<T> coeff;
-        V3<T> wrench_linear, wrench_angular;
+        V3<T> wrench_linear, wrench_angular;  // a body's linear / angular Jacobian rows
     };
     struct Row {  // WorldContactRow (solver.rs:2746-2752)
         std::vector<Comp> comps;
@@ -1263,14 +1263,20 @@ template <class T> struct World {
         return {tr / T(3.0), rot / T(3.0)};
     }
 
-    // solve_tree_contacts (solver.rs:2805-3366), PGS, trees only.
+    // solve_tree_contacts (solver.rs:2805-3366), PGS or Newton, over the
+    // contacts touching a tree; a free body on the other side enters the
+    // rows as a body component (world_jacobian_for_side, 3537-3600).
     // Returns tree_qfrc per tree.
     // With `wrenches` / `normals`: also TreeContactSolution::tree_wrenches
     // (per tree and link, world force and torque at the COM) and
-    // contact_normal_forces (per input contact), solver.rs:3334-3364.
+    // contact_normal_forces (per input contact), solver.rs:3334-3364; with
+    // `body_wrenches`: TreeContactSolution::body_wrenches (per body, world
+    // force and torque).
     std::vector<std::vector<T>> solve_tree_contacts(const std::vector<Contact<T>>& contacts, int tree_implicit /*-1 none,0,1*/,
                                                     std::vector<std::vector<std::pair<V3<T>, V3<T>>>>* wrenches = nullptr,
-                                                    std::vector<T>* normals = nullptr) const {
+                                                    std::vector<T>* normals = nullptr,
+                                                    std::vector<std::pair<V3<T>, V3<T>>>* body_wrenches = nullptr) const {
+        if (body_wrenches) body_wrenches->assign(bodies.size(), {vzero<T>(), vzero<T>()});
         std::vector<std::vector<T>> qfrc(trees.size());
         for (size_t t = 0; t < trees.size(); ++t) qfrc[t].assign(trees[t].nv(), T(0.0));
         if (wrenches) {
@@ -1282,6 +1288,11 @@ template <class T> struct World {
         if (cone != C_PYRAMIDAL) fail("elliptic cone is not restated");
         std::vector<std::vector<Pose<T>>> poses;
         tree_poses(poses);
+        std::vector<M3<T>> inv_i_world(bodies.size());
+        for (size_t i = 0; i < bodies.size(); ++i) {
+            M3<T> r = to_mat3(bodies[i].q);
+            inv_i_world[i] = (r * bodies[i].inv_inertia) * transpose(r);
+        }
         std::vector<std::vector<T>> factors(trees.size()), dfactors(trees.size()), freev(trees.size());
         for (size_t t = 0; t < trees.size(); ++t) {
             int n = trees[t].nv();
@@ -1334,6 +1345,14 @@ template <class T> struct World {
                 for (int side = 0; side < 2; ++side) {
                     const Geom<T>& g = side == 0 ? ga : gb;
                     T sign = side == 0 ? T(1.0) : T(-1.0);
+                    if (g.is_body()) {
+                        Comp cp;
+                        cp.body = g.body;
+                        cp.wrench_linear = dirs[k] * sign;
+                        cp.wrench_angular = cross(c.pos - bodies[g.body].p, dirs[k]) * sign;
+                        r.comps.push_back(cp);
+                        continue;
+                    }
                     if (!g.is_link()) continue;
                     const Pose<T>& lp = poses[g.tree][g.link];
                     V3<T> local = inverse_rotate(lp.q, c.pos - lp.p);
@@ -1359,7 +1378,16 @@ template <class T> struct World {
             // apply_world_impulse (solver.rs:3799-3836) with impulse 1.
             std::vector<std::vector<T>> delta(trees.size());
             for (size_t t = 0; t < trees.size(); ++t) delta[t].assign(trees[t].nv(), T(0.0));
+            std::vector<std::pair<V3<T>, V3<T>>> bdelta(bodies.size(), {vzero<T>(), vzero<T>()});
             for (const Comp& cc : rows[col].comps) {
+                if (cc.body >= 0) {
+                    const Body<T>& bs = bodies[cc.body];
+                    V3<T> force = cc.wrench_linear * T(1.0), torque = cc.wrench_angular * T(1.0);
+                    V3<T> angular_velocity = inv_i_world[cc.body] * torque;
+                    bdelta[cc.body].first = bdelta[cc.body].first + force / bs.mass;
+                    bdelta[cc.body].second = bdelta[cc.body].second + inverse_rotate(bs.q, angular_velocity);
+                    continue;
+                }
                 int nv = trees[cc.tree].nv();
                 std::vector<T> gi(nv);
                 for (int s = 0; s < nv; ++s) gi[s] = cc.coeff[s] * T(1.0);
@@ -1370,6 +1398,13 @@ template <class T> struct World {
                 // world_velocity_from_delta (solver.rs:3687-3714).
                 RSum<T> outer;
                 for (const Comp& rc : rows[ri].comps) {
+                    if (rc.body >= 0) {
+                        const Body<T>& bs = bodies[rc.body];
+                        T bv = dot(rc.wrench_linear, bdelta[rc.body].first) +
+                               dot(rc.wrench_angular, rotate(bs.q, bdelta[rc.body].second));
+                        outer.add(bv + T(0.0));
+                        continue;
+                    }
                     RSum<T> s;
                     for (int k = 0; k < trees[rc.tree].nv(); ++k) s.add(rc.coeff[k] * delta[rc.tree][k]);
                     outer.add(T(0.0) + s.acc);
@@ -1385,6 +1420,11 @@ template <class T> struct World {
                 // tree_contact_diag_approx (solver.rs:3756-3796).
                 T tr = T(0.0), rot = T(0.0);
                 for (const Comp& cp : r.comps) {
+                    if (cp.body >= 0) {
+                        if (!is_zero_vec(cp.wrench_linear)) tr += T(1.0) / bodies[cp.body].mass;
+                        else rot += dot(cp.wrench_angular, inv_i_world[cp.body] * cp.wrench_angular);
+                        continue;
+                    }
                     auto iw = body_invweight0(cp.tree, cp.link, poses[cp.tree], dfactors[cp.tree]);
                     if (!is_zero_vec(cp.wrench_linear)) tr += iw.first; else rot += iw.second;
                 }
@@ -1395,6 +1435,12 @@ template <class T> struct World {
                 // world_current_velocity / world_free_velocity (solver.rs:3624-3685).
                 RSum<T> vel, fvel;
                 for (const Comp& cp : r.comps) {
+                    if (cp.body >= 0) {
+                        const Body<T>& bs = bodies[cp.body];
+                        vel.add(dot(cp.wrench_linear, bs.v) + dot(cp.wrench_angular, rotate(bs.q, bs.w)) + T(0.0));
+                        fvel.add(dot(cp.wrench_linear, body_gravity(bs, gravity) * dt) + T(0.0));
+                        continue;
+                    }
                     const Tree<T>& t = trees[cp.tree];
                     RSum<T> sv, sf;
                     for (int s = 0; s < t.nv(); ++s) sv.add(cp.coeff[s] * t.qdot[s]);
@@ -1449,6 +1495,14 @@ template <class T> struct World {
         for (int i = 0; i < nr; ++i) {
             T force = f[i] / dt;
             for (const Comp& cp : rows[i].comps) {
+                if (cp.body >= 0) {
+                    if (body_wrenches) {
+                        auto& w = (*body_wrenches)[cp.body];
+                        w.first = w.first + cp.wrench_linear * force;
+                        w.second = w.second + cp.wrench_angular * force;
+                    }
+                    continue;
+                }
                 for (int s = 0; s < trees[cp.tree].nv(); ++s) qfrc[cp.tree][s] += cp.coeff[s] * force;
                 if (wrenches) {
                     auto& w = (*wrenches)[cp.tree][cp.link];
@@ -2083,20 +2137,26 @@ template <class T> struct World {
         last_contacts.clear();
         if (constraint) {
             std::vector<Contact<T>> contacts = detect_contacts(pairs);
+            // compute_tree_contact_solution (world.rs:2154-2173) takes the
+            // contacts touching a tree, compute_solver_wrenches (2097-2149)
+            // the rest, and adds the tree solve's body wrenches to its own.
             std::vector<Contact<T>> tc, fc;
             for (auto& c : contacts) {
                 const Geom<T>& ga = geoms[c.geom_a];
                 const Geom<T>& gb = geoms[c.geom_b];
-                if ((ga.is_link() && gb.is_body()) || (ga.is_body() && gb.is_link()))
-                    fail("tree-body contacts are not restated");
                 if (ga.is_link() || gb.is_link()) tc.push_back(c);
-                else fc.push_back(c);  // compute_solver_wrenches (world.rs:2097-2149)
+                else fc.push_back(c);
             }
-            last_contacts = tc;
-            last_contacts.insert(last_contacts.end(), fc.begin(), fc.end());
+            last_contacts = contacts;
             int ti = integrator == I_RK4 ? -1 : integrator == I_IMPLICITFAST ? 1 : 0;
-            tq = solve_tree_contacts(tc, ti);
+            std::vector<std::pair<V3<T>, V3<T>>> tbw;
+            tq = solve_tree_contacts(tc, ti, nullptr, nullptr, &tbw);
             bw = solve_free_bodies(bodies, geoms, fc, gravity, dt, cone, iterations, mode == M_NEWTON);
+            if (!trees.empty())
+                for (size_t i = 0; i < bodies.size(); ++i) {
+                    bw[i].first = bw[i].first + tbw[i].first;
+                    bw[i].second = bw[i].second + tbw[i].second;
+                }
         } else {
             if (!pairs.empty() && !trees.empty() && !bodies.empty())
                 fail("penalty contacts between trees and bodies are not restated");

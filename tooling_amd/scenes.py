@@ -520,6 +520,118 @@ def box_stack(n_boxes=3, dt=0.005, iterations=30, integrator=abi.INTEGRATOR_RK4)
                  f"{n_boxes} stacked free boxes on a plane, PGS {iterations}", bodies=bodies)
 
 
+def _free_sphere_link(z=0.0, x=0.0):
+    """newt/tests/tree_contacts_solver.rs:17-28: a one-link tree on a free
+    joint at (x, 0, z), 1 kg, inertia diag 0.1."""
+    return link(-1, abi.JOINT_FREE, parent_pos=(x, 0.0, z), mass=1.0, inertia=_diag(0.1, 0.1, 0.1))
+
+
+def tree_body_spheres(solver=abi.SOLVER_PGS, as_tree=False) -> Scene:
+    """tree_body_contact_transfers_equal_and_opposite_impulses
+    (newt/tests/tree_contacts_solver.rs:404-448): a free-root sphere tree at
+    x = -0.4 and a 2 kg free sphere body at x = +0.4, radius 0.5 each and
+    frictionless, overlapping by 0.2 m at rest; no gravity, 30 iterations,
+    World::new's dt 5 ms and RK4.  With `as_tree` the body is a second
+    free-root tree of the same mass and inertia instead."""
+    links = [_free_sphere_link(0.0, -0.4)]
+    inertia = _diag(0.4 * 2.0 * 0.25, 0.4 * 2.0 * 0.25, 0.4 * 2.0 * 0.25)
+    geoms = [geom(abi.SHAPE_SPHERE, tree=0, link=0, size=(0.5, 0.0, 0.0), friction=0.0)]
+    bodies, more = [], []
+    if as_tree:
+        more = [([link(-1, abi.JOINT_FREE, parent_pos=(0.4, 0.0, 0.0), mass=2.0, inertia=inertia)], 0)]
+        geoms.append(geom(abi.SHAPE_SPHERE, tree=1, link=0, size=(0.5, 0.0, 0.0), friction=0.0))
+    else:
+        bodies = [body(2.0, inertia, (0.4, 0.0, 0.0))]
+        geoms.append(geom(abi.SHAPE_SPHERE, body=0, size=(0.5, 0.0, 0.0), friction=0.0))
+    opts = dict(dt=0.005, gravity=(0.0, 0.0, 0.0), integrator=abi.INTEGRATOR_RK4, solver=solver, iterations=30)
+    sc = Scene("tree_body_spheres" + ("_trees" if as_tree else ""), opts, links, geoms, 0, None,
+               "a free sphere tree and a free sphere body overlapping at rest", bodies=bodies, more_trees=more)
+
+    def sampler(n, seed):
+        q = np.zeros((n, sc.nq))
+        q[:, 0] = -0.4
+        q[:, 6] = 1.0
+        q[:, 7] = 0.4
+        q[:, 13] = 1.0
+        return q, np.zeros((n, sc.nv))
+
+    sc.sampler = sampler
+    return sc
+
+
+def tree_body_tilted_boxes(solver=abi.SOLVER_PGS, integrator=None, as_tree=False) -> Scene:
+    """tilted_tree_box_box_world (newt/tests/tree_contacts_solver.rs:105-143):
+    a free-root tree at z 1.48 tilted 0.005 rad about x carrying a box of
+    half extent 0.5, over a 1 kg free box body at z 0.5 tilted 0.01 rad;
+    frictionless, no gravity, 40 iterations, dt 5 ms, RK4 unless
+    `integrator` says otherwise.  With `as_tree` the body is a second
+    free-root tree of the same mass and inertia."""
+    half = (0.5, 0.5, 0.5)
+    q_tree = quat_axis_angle((1.0, 0.0, 0.0), 0.005)
+    q_body = quat_axis_angle((1.0, 0.0, 0.0), 0.01)
+    links = [_free_sphere_link(1.48)]
+    geoms = [geom(abi.SHAPE_BOX, tree=0, link=0, size=half, friction=0.0)]
+    bodies, more = [], []
+    if as_tree:
+        more = [([link(-1, abi.JOINT_FREE, parent_pos=(0.0, 0.0, 0.5), mass=1.0,
+                       inertia=solid_box_inertia(1.0, half))], 0)]
+        geoms.append(geom(abi.SHAPE_BOX, tree=1, link=0, size=half, friction=0.0))
+    else:
+        bodies = [body(1.0, solid_box_inertia(1.0, half), (0.0, 0.0, 0.5), q_body)]
+        geoms.append(geom(abi.SHAPE_BOX, body=0, size=half, friction=0.0))
+    opts = dict(dt=0.005, gravity=(0.0, 0.0, 0.0),
+                integrator=abi.INTEGRATOR_RK4 if integrator is None else integrator, solver=solver, iterations=40)
+    sc = Scene("tree_body_tilted_boxes" + ("_trees" if as_tree else ""), opts, links, geoms, 0, None,
+               "a tilted tree box resting on a tilted box body", bodies=bodies, more_trees=more)
+
+    def sampler(n, seed):
+        q = np.zeros((n, sc.nq))
+        q[:, 2] = 1.48
+        q[:, 3:7] = q_tree
+        q[:, 9] = 0.5
+        q[:, 10:14] = q_body
+        return q, np.zeros((n, sc.nv))
+
+    sc.sampler = sampler
+    return sc
+
+
+def chain_and_boxes(n_links=4, n_free=4, solver=abi.SOLVER_PGS, integrator=abi.INTEGRATOR_EULER, dt=0.002,
+                    iterations=30) -> Scene:
+    """A tree and free bodies in one world: the hinged box chain of
+    box_chain(n_links) lying on a plane with n_free free boxes (half extents
+    0.06 x 0.05 x 0.04, 0.5 kg) dropped onto its links from 0.15 m up, then
+    onto each other and the plane: tree-body, body-body, body-plane and
+    tree-plane contacts.  Worlds differ by the chain's pose and joint
+    angles (box_chain's sampler) and each box's lateral offset and
+    orientation."""
+    base = box_chain(n_links, dt=dt, iterations=iterations)
+    h = (0.06, 0.05, 0.04)
+    bodies = []
+    geoms = list(base.geoms)
+    for i in range(n_free):
+        x = (i % n_links - (n_links - 1) / 2.0) * 0.08
+        bodies.append(body(0.5, solid_box_inertia(0.5, h), (x, 0.0, 0.15 + 0.1 * i)))
+        geoms.append(geom(abi.SHAPE_BOX, body=i, size=h, friction=1.0))
+    opts = dict(base.options, solver=solver, integrator=integrator)
+    sc = Scene(f"chain{n_links}_boxes{n_free}", opts, base.links, geoms, 0, None,
+               f"{n_links} hinged boxes on a plane with {n_free} free boxes dropped on them", bodies=bodies)
+
+    def sampler(n, seed):
+        q, v = base.sampler(n, seed)
+        r = _rng(seed + 1)
+        bq = np.zeros((n, 7 * n_free))
+        for i in range(n_free):
+            b = bodies[i]
+            bq[:, 7 * i:7 * i + 3] = np.array(b["position"]) + np.c_[r.uniform(-0.03, 0.03, size=(n, 2)),
+                                                                     np.zeros(n)]
+            bq[:, 7 * i + 3:7 * i + 7] = random_quaternions(r, n)
+        return np.hstack([q, bq]), np.hstack([v, np.zeros((n, 6 * n_free))])
+
+    sc.sampler = sampler
+    return sc
+
+
 def box_field(n_x=10, n_y=5, levels=2, half=0.1, spacing=0.35, dt=0.002, iterations=20, yaw=0.2) -> Scene:
     """n_x x n_y stacks of `levels` cubes (half extent `half`, 1 kg) resting
     on a plane 0.1 mm into contact, stacks `spacing` apart; Newton, pyramidal,
