@@ -44,28 +44,9 @@ struct FreeBodies {
         for (int k = 0; k < 9; ++k) a.m[k] = w.dw[off + k];
         return a;
     }
-    static NEWT_HD M33<double> tbl(const double* p) {
-        M33<double> a;
-        for (int k = 0; k < 9; ++k) a.m[k] = p[k];
-        return a;
-    }
-    // Mat3 product and transpose (math.rs:523-537), sums from 0.0.
-    static NEWT_HD M33<double> mm(const M33<double>& a, const M33<double>& b) {
-        M33<double> o;
-        for (int c = 0; c < 3; ++c)
-            for (int r = 0; r < 3; ++r) {
-                double s = 0.0;
-                for (int k = 0; k < 3; ++k) s += a.m[r * 3 + k] * b.m[k * 3 + c];
-                o.m[r * 3 + c] = s;
-            }
-        return o;
-    }
-    static NEWT_HD M33<double> tr(const M33<double>& a) {
-        M33<double> o;
-        for (int r = 0; r < 3; ++r)
-            for (int c = 0; c < 3; ++c) o.m[r * 3 + c] = a.m[c * 3 + r];
-        return o;
-    }
+    static NEWT_HD M33<double> tbl(const double* p) { return m33_load(p); }
+    static NEWT_HD M33<double> mm(const M33<double>& a, const M33<double>& b) { return m33_mul(a, b); }
+    static NEWT_HD M33<double> tr(const M33<double>& a) { return m33_tr(a); }
     NEWT_HD V3<double> gravity_of(int b) const {
         const V3<double> g = mk3(w.m.gravity[0], w.m.gravity[1], w.m.gravity[2]);
         const double s = body(b).gravity_scale;
@@ -209,6 +190,8 @@ struct FreeBodies {
             const GeomT& gb = w.m.geoms[w.iw[w.L.cids + 3 * c + 1]];
             const int a = ga.body, b = gb.body;
             if (a < 0 && b < 0) continue;
+            // a link side puts the contact in the tree solve (world.rs:2107-2115)
+            if (World<double>::on_link(ga) || World<double>::on_link(gb)) continue;
             const double pen = w.dw[o + 6] - w.dw[o + 8];
             if (pen <= 0.0) continue;
             const double mu = ga.friction < gb.friction ? ga.friction : gb.friction;
@@ -580,6 +563,35 @@ struct FreeBodies {
         }
         w.iw[w.L.iwf + 1] = nc;
         set_poses();
+        w.populate(w.m.dt, false);
+        return 0;
+    }
+
+    // World::step for a model with trees and free bodies (world.rs:1459-1537),
+    // PGS, or Newton where every contact has a link side: the step's pairs
+    // and contacts, the tree solve with the bodies as components of the
+    // tree-body rows (World::pre), the trees' integration (World::post),
+    // then the free-body solve over the contacts with no link side, the
+    // tree solve's body wrenches added to its own (world.rs:2097-2149), the
+    // bodies' integration, and the broadphase refresh.  Trees and bodies
+    // integrate independent parts of the state, so newt's order (trees
+    // first under Euler, bodies first under RK4) does not change a bit.
+    NEWT_HD int step_with_trees() const {
+        // The tree solve's body wrenches: zero unless some contact row
+        // touches a body this step.
+        for (int i = 0; i < 6 * w.m.n_bodies; ++i) w.dw[w.L.tbw + i] = 0.0;
+        int st = w.pre();
+        if (st != 0) return st;
+        st = w.post(false);
+        if (st != 0) return st;
+        const int nc = w.iw[w.L.iwf + w.m.n_links + 1];
+        solve(nc);
+        for (int b = 0; b < w.m.n_bodies; ++b)
+            for (int k = 0; k < 6; ++k) w.dw[w.L.bwr + 6 * b + k] = w.dw[w.L.bwr + 6 * b + k] + w.dw[w.L.tbw + 6 * b + k];
+        integrate();
+        set_poses();
+        w.forward_kinematics();
+        w.link_velocities();
         w.populate(w.m.dt, false);
         return 0;
     }

@@ -155,7 +155,6 @@ inline bool body_inverse(const double* cm, double* out_rm) {
 // state layout, body tables, geoms riding on bodies, static candidate pairs.
 inline newt_status compile_bodies(HostModel* M, std::string* err) {
     const newt_options& o = M->opt;
-    if (!M->trees.empty()) return cfail(err, NEWT_ERR_UNSUPPORTED, "a model holds a tree or free bodies, not both");
     if (o.iterations < 1) return cfail(err, NEWT_ERR_INVALID_CONFIG, "solver iterations must be >= 1");
     if (o.penetration_slop != 0.0) return cfail(err, NEWT_ERR_UNSUPPORTED, "penetration_slop is not built");
     if (o.solver_mode == NEWT_SOLVER_NEWTON && o.cone == NEWT_CONE_ELLIPTIC)
@@ -265,6 +264,7 @@ inline newt_status compile_bodies(HostModel* M, std::string* err) {
     d.pairs = M->pairs.data();
     d.n_bodies = nb;
     d.bodies = M->hbodies.data();
+    d.tnq = d.tnv = 0;
     set_mesh_tables(M);
     if (d.has_mesh && o.solver_mode == NEWT_SOLVER_NEWTON)
         return cfail(err, NEWT_ERR_UNSUPPORTED, "the Newton free-body step runs one wavefront per world and has no "
@@ -283,6 +283,8 @@ inline newt_status compile_sensors(HostModel* M, std::string* err) {
     for (size_t k = 0; k < M->sensors.size(); ++k) {
         const newt_sensor_desc& s = M->sensors[k];
         const std::string who = "sensor " + std::to_string(k) + ": ";
+        if (d.n_bodies > 0 && d.n_links > 0)
+            return cfail(err, NEWT_ERR_UNSUPPORTED, who + "sensors on a model with trees and free bodies are not built");
         const int dim = sensor_kind_dim(s.kind);
         if (dim == 0) return cfail(err, NEWT_ERR_UNSUPPORTED, who + "this sensor kind is not built");
         // (tree, link) -> the link's index in the forest
@@ -333,8 +335,9 @@ inline newt_status compile_sensors(HostModel* M, std::string* err) {
 }
 
 inline newt_status compile_model(HostModel* M, std::string* err) {
-    if (!M->bodies.empty()) return compile_bodies(M, err);
+    if (!M->bodies.empty() && M->trees.empty()) return compile_bodies(M, err);
     if (M->trees.empty()) return cfail(err, NEWT_ERR_INVALID_CONFIG, "a model needs a tree or free bodies");
+    const int nb = (int)M->bodies.size();  // free bodies beside the trees
     const newt_options& o = M->opt;
     if (o.iterations < 1) return cfail(err, NEWT_ERR_INVALID_CONFIG, "solver iterations must be >= 1");
     if (o.penetration_slop != 0.0) return cfail(err, NEWT_ERR_UNSUPPORTED, "penetration_slop is not built");
@@ -487,13 +490,15 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
         if (g.tree >= (int)M->trees.size()) return cfail(err, NEWT_ERR_INVALID_ARG, "geom attached to a missing tree");
         if (g.tree >= 0 && (g.link < 0 || g.link >= (int)M->trees[g.tree].size()))
             return cfail(err, NEWT_ERR_INVALID_ARG, "geom link out of range");
-        if (g.body >= 0) return cfail(err, NEWT_ERR_INVALID_ARG, "geom attached to a missing body");
+        if (g.body >= nb) return cfail(err, NEWT_ERR_INVALID_ARG, "geom attached to a missing body");
+        if (g.tree >= 0 && g.body >= 0) return cfail(err, NEWT_ERR_INVALID_ARG, "a geom rides on a link or a body");
         if (g.condim != 3 && g.condim != 1 && g.condim != 4 && g.condim != 6)
             return cfail(err, NEWT_ERR_INVALID_ARG, "condim must be 1, 3, 4 or 6");
         GeomT h{};
         h.shape = g.shape;
-        h.link = g.tree >= 0 ? tree_base[g.tree] + g.link : -1;
-        h.body = -1;
+        // A body's geoms take the pose slot after the links.
+        h.link = g.tree >= 0 ? tree_base[g.tree] + g.link : g.body >= 0 ? (int)T.size() + g.body : -1;
+        h.body = g.body >= 0 ? g.body : -1;
         h.condim = g.condim;
         for (int k = 0; k < 3; ++k) {
             h.size[k] = g.size[k];
@@ -520,9 +525,25 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
             const newt_geom_desc& gb = M->geoms[b];
             if (!((ga.group & gb.mask) != 0 && (gb.group & ga.mask) != 0)) continue;
             bool la = ga.tree >= 0, lb = gb.tree >= 0;
+            const bool ba = ga.body >= 0, bb = gb.body >= 0;
             // disabled_self_collision holds a tree whose links skip each other
             if (la && lb && ga.tree == gb.tree && M->tree_collide[ga.tree] == 0) continue;
-            if (la == lb && (!la || (ga.tree == gb.tree && ga.link == gb.link))) continue;
+            // the same attachment: Static, Body(i) or Link(t, l)
+            if (!la && !lb && !ba && !bb) continue;
+            if (ba && bb && ga.body == gb.body) continue;
+            if (la && lb && ga.tree == gb.tree && ga.link == gb.link) continue;
+            // newt's penalty step runs the bodies against the trees' states
+            // of that moment (world.rs:1705-1785, 1826-1946): not built.
+            if (nb > 0 && o.solver_mode == NEWT_SOLVER_PENALTY)
+                return cfail(err, NEWT_ERR_UNSUPPORTED, "penalty contacts in a model with trees and free bodies are "
+                                                        "not built (geom " + std::to_string(a) + " x geom " +
+                                                            std::to_string(b) + ")");
+            // Contacts with no link side go to the free-body solve, whose
+            // Newton form runs on the wavefront step of free-body models only.
+            if (nb > 0 && !la && !lb && o.solver_mode == NEWT_SOLVER_NEWTON)
+                return cfail(err, NEWT_ERR_UNSUPPORTED, "Newton contacts between free bodies, or a free body and the "
+                                                        "world, in a model with trees are not built (geom " +
+                                                            std::to_string(a) + " x geom " + std::to_string(b) + ")");
             // newt steps trees one after another under the penalty solver
             // (step_trees, world.rs:1826-1946), each against the others'
             // states of that moment: not built.
@@ -539,13 +560,39 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
             cap_c += pair_max_contacts(ga.shape, gb.shape);
         }
     M->v_default.assign(nv, 0.0);
+    // Free bodies follow the trees in qpos / qvel (World state order).
+    M->hbodies.clear();
+    for (int i = 0; i < nb; ++i) {
+        const newt_body_desc& b = M->bodies[i];
+        if (!(b.mass > 0.0)) return cfail(err, NEWT_ERR_INVALID_ARG, "body mass must be positive");
+        BodyT t{};
+        t.mass = b.mass;
+        t.gravity_scale = b.gravity_scale;
+        t.has_max_lin = b.has_max_linear_velocity != 0;
+        t.has_max_ang = b.has_max_angular_velocity != 0;
+        t.max_lin = b.max_linear_velocity;
+        t.max_ang = b.max_angular_velocity;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) t.inertia[r * 3 + c] = b.inertia[c * 3 + r];
+        if (!body_inverse(b.inertia, t.inv_inertia))
+            return cfail(err, NEWT_ERR_INVALID_ARG, "body inertia tensor must be invertible");
+        t.q_off = nq + 7 * i;
+        t.v_off = nv + 6 * i;
+        M->hbodies.push_back(t);
+        for (int k = 0; k < 3; ++k) M->q_default.push_back(b.position[k]);
+        for (int k = 0; k < 4; ++k) M->q_default.push_back(b.orientation[k]);
+        for (int k = 0; k < 3; ++k) M->v_default.push_back(b.linear_velocity[k]);
+        for (int k = 0; k < 3; ++k) M->v_default.push_back(b.angular_velocity_body[k]);
+    }
     ModelT& d = M->dev;
     d.n_links = (int)T.size();
-    d.nq = nq;
-    d.nv = nv;
+    d.nq = nq + 7 * nb;
+    d.nv = nv + 6 * nb;
+    d.tnq = nq;
+    d.tnv = nv;
     d.forest = M->trees.size() > 1 ? 1 : 0;
-    d.n_bodies = 0;
-    d.bodies = nullptr;
+    d.n_bodies = nb;
+    d.bodies = nb > 0 ? M->hbodies.data() : nullptr;
     // Penalty limits and penalty contacts run on the lane-per-world FP64 step.
     d.penalty_limits = (!M->limits.empty() || !M->pairs.empty()) && o.solver_mode == NEWT_SOLVER_PENALTY;
     d.n_geoms = (int)M->geoms.size();
@@ -580,7 +627,8 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
 // under the penalty solver, several trees, the Newton contact solve and
 // convex meshes step on the lane-per-world FP64 kernel in every precision.
 inline bool lane_only(const ModelT& m) {
-    return m.n_bodies == 0 && (m.penalty_limits || m.forest || m.mode == NEWT_SOLVER_NEWTON || m.has_mesh);
+    return (m.n_bodies == 0 && (m.penalty_limits || m.forest || m.mode == NEWT_SOLVER_NEWTON || m.has_mesh)) ||
+           (m.n_bodies > 0 && m.n_links > 0);
 }
 
 // Per-world mesh-mesh workspace (newt_ccd.hpp, Work): the EPA polytope
@@ -607,11 +655,13 @@ inline Layout make_layout(const ModelT& m, int real_bytes) {
     auto R = [&](int n) { int o = r; r += n; return o; };
     auto D = [&](int n) { int o = dd; dd += n; return o; };
     auto I = [&](int n) { int o = ii; ii += n; return o; };
-    int n = m.n_links, nv = std::max(m.nv, 1);
+    // The tree part of the state; free bodies beside the trees add their
+    // pose slots and the free-body solve's workspace.
+    int n = m.n_links, nv = std::max(m.tnv, 1), nb = m.n_bodies;
     L.coef = R(2 * nv * m.cap_r);
     L.A = R(m.cap_r * m.cap_r);
     L.total_r = r;
-    L.poses = D(7 * n);
+    L.poses = D(7 * (n + nb));
     L.lvel = D(6 * n);
     L.xup = D(12 * n);
     L.S = D(6 * n);
@@ -641,10 +691,10 @@ inline Layout make_layout(const ModelT& m, int real_bytes) {
     L.dres = D(nv);
     // RK4 (tree.rs:1234-1364): start state, stage derivative, weighted sums.
     const bool rk4 = m.integrator == NEWT_INTEGRATOR_RK4;
-    L.rq0 = rk4 ? D(m.nq) : 0;
+    L.rq0 = rk4 ? D(m.tnq) : 0;
     L.rv0 = rk4 ? D(nv) : 0;
-    L.rdq = rk4 ? D(m.nq) : 0;
-    L.raq = rk4 ? D(m.nq) : 0;
+    L.rdq = rk4 ? D(m.tnq) : 0;
+    L.raq = rk4 ? D(m.tnq) : 0;
     L.rav = rk4 ? D(nv) : 0;
     const int nr = m.n_ranged;
     L.lim = D(nv);
@@ -665,6 +715,16 @@ inline Layout make_layout(const ModelT& m, int real_bytes) {
     L.nfac = newton ? D(m.cap_r * m.cap_r) : 0;
     L.nvec = newton ? D(9 * m.cap_r) : 0;
     L.nbrk = newton ? D(m.cap_r + 2) : 0;
+    if (nb > 0) {
+        L.bwr = D(6 * nb);
+        L.biw = D(9 * nb);
+        L.bbd = D(6 * nb);
+        L.brow = D(7 * m.cap_r);
+        L.bblk = D(8 * m.cap_c);
+        L.bcomp = D(12 * m.cap_r);
+        L.bdel = D(6);
+        L.tbw = D(6 * nb);
+    }
     add_ccd_layout(m, L, D);
     L.total_d = dd;
     L.cids = I(3 * m.cap_c);
@@ -673,6 +733,7 @@ inline Layout make_layout(const ModelT& m, int real_bytes) {
     L.iwf = I(n + 2);
     L.lsl = I(std::max(m.n_ranged, 1));
     L.llm = I(std::max(m.n_ranged, 1));
+    L.bbi = nb > 0 ? I(2 * m.cap_c) : 0;
     add_ccd_layout_i(m, L, I);
     L.total_i = ii;
     (void)real_bytes;

@@ -85,6 +85,36 @@ __global__ void __launch_bounds__(kBlock) k_step(ModelT m, Layout L, DataT d, R*
     d.ncontact[w] = iws[(size_t)(L.iwf + m.n_links + 1) * d.W + w];
 }
 
+// Models with trees and free bodies: n_steps World::steps per lane in FP64
+// (every precision), bit-identical to the FP64 oracle.
+__global__ void __launch_bounds__(kBlock) k_trees_bodies(ModelT m, Layout L, DataT d, double* ws, double* dws,
+                                                          int* iws, int n_steps, int init) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= d.W) return;
+    World<double> wd = world_of<double>(m, L, d, ws, dws, iws, w);
+    if (d.fault[w] != 0) return;
+    FreeBodies fb{wd};
+    wd.forward_kinematics();
+    wd.link_velocities();
+    fb.set_poses();
+    if (init) {
+        wd.populate(m.dt, true);
+        iws[(size_t)(L.iwf + m.n_links) * d.W + w] = 0;
+        iws[(size_t)(L.iwf + m.n_links + 1) * d.W + w] = 0;
+        return;
+    }
+    wd.populate(m.dt, false);
+    for (int s = 0; s < n_steps; ++s) {
+        const int st = fb.step_with_trees();
+        if (st != 0) {
+            d.fault[w] = st;
+            break;
+        }
+    }
+    d.npair[w] = iws[(size_t)(L.iwf + m.n_links) * d.W + w];
+    d.ncontact[w] = iws[(size_t)(L.iwf + m.n_links + 1) * d.W + w];
+}
+
 // Free-body models: n_steps World::steps per lane in FP64 (every precision).
 template <bool Penalty>
 __global__ void __launch_bounds__(kBlock) k_bodies(ModelT m, Layout L, DataT d, double* dws, int* iws, int n_steps,
@@ -191,6 +221,12 @@ newt_status launch(newt_data* D, int n_steps, hipStream_t stream, int init) {
     const newt_model* M = D->model;
     D->last_stream = stream;
     dim3 grid((D->W + kBlock - 1) / kBlock), block(kBlock);
+    if (M->dev.n_bodies > 0 && M->dev.n_links > 0) {
+        hipLaunchKernelGGL(k_trees_bodies, grid, block, 0, stream, M->dev, D->L, D->d, (double*)D->ws, D->dws,
+                           D->iws, n_steps, init);
+        HIP_TRY(hipGetLastError());
+        return NEWT_OK;
+    }
     if (M->dev.n_bodies > 0) {
         const bool newton = M->dev.mode == NEWT_SOLVER_NEWTON;  // the Newton solve is wavefront-only
         // Meshes run on the lane step only.
@@ -476,16 +512,18 @@ newt_status newt_data_create(const newt_model* model, int32_t n_worlds, int32_t 
     if (precision != NEWT_PRECISION_FP64 && precision != NEWT_PRECISION_MIXED &&
         precision != NEWT_PRECISION_FP64_EXACT)
         return fail(NEWT_ERR_INVALID_ARG, "unknown precision");
-    // Free-body models run the lane-per-world FP64 step in every precision.
-    const bool bodies = model->dev.n_bodies > 0;
+    // Free-body models run the lane-per-world FP64 step in every precision;
+    // so do models with trees and free bodies (lane_only), on the tree layout.
+    const bool bodies = model->dev.n_bodies > 0 && model->dev.n_links == 0;
+    const bool with_bodies = model->dev.n_bodies > 0;
     const bool team = !bodies && !lane_only(model->dev) && precision != NEWT_PRECISION_FP64_EXACT;
     if ((model->dev.nv > 64 || model->dev.n_links > 128) && team)
         return fail(NEWT_ERR_UNSUPPORTED, "the wavefront-per-world step handles nv <= 64 and <= 128 links; "
                                           "use NEWT_PRECISION_FP64_EXACT for larger trees");
-    const TeamLayout TL = bodies ? TeamLayout{} : make_team_layout(model->dev, precision == NEWT_PRECISION_MIXED ? 4 : 8);
-    const int cache_bytes = bodies ? 0 : model->cache.bytes;
+    const TeamLayout TL = with_bodies ? TeamLayout{} : make_team_layout(model->dev, precision == NEWT_PRECISION_MIXED ? 4 : 8);
+    const int cache_bytes = with_bodies ? 0 : model->cache.bytes;
     const int max_wpb = team_max_worlds_for(model->dev.nv);
-    const int wpb = bodies ? 1 : std::min(max_wpb, (kMaxLds - cache_bytes) / std::max(TL.bytes, 1));
+    const int wpb = with_bodies ? 1 : std::min(max_wpb, (kMaxLds - cache_bytes) / std::max(TL.bytes, 1));
     if (team && wpb < 1)
         return fail(NEWT_ERR_UNSUPPORTED, "one world's step state exceeds the 160 KiB of LDS of a workgroup");
     newt_data* D = new newt_data();

@@ -148,6 +148,29 @@ template <class R> NEWT_HD M33<R> qmat(Qt<R> q) {
     o.m[8] = R(1.0) - R(2.0) * (x * x + y * y);
     return o;
 }
+// A row-major table as a Mat3; Mat3 product and transpose (math.rs:523-537),
+// sums from 0.0.
+NEWT_HD inline M33<double> m33_load(const double* p) {
+    M33<double> a;
+    for (int k = 0; k < 9; ++k) a.m[k] = p[k];
+    return a;
+}
+NEWT_HD inline M33<double> m33_mul(const M33<double>& a, const M33<double>& b) {
+    M33<double> o;
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += a.m[r * 3 + k] * b.m[k * 3 + c];
+            o.m[r * 3 + c] = s;
+        }
+    return o;
+}
+NEWT_HD inline M33<double> m33_tr(const M33<double>& a) {
+    M33<double> o;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) o.m[r * 3 + c] = a.m[c * 3 + r];
+    return o;
+}
 template <class R> NEWT_HD V3<R> mv(const M33<R>& a, V3<R> v) {
     return {a.m[0] * v.x + a.m[1] * v.y + a.m[2] * v.z, a.m[3] * v.x + a.m[4] * v.y + a.m[5] * v.z,
             a.m[6] * v.x + a.m[7] * v.y + a.m[8] * v.z};

@@ -545,12 +545,10 @@ Loaded load(const Value& root) {
     const Value* trees = root.get("trees");
     const Value* bodies = root.get("bodies");
     const bool n_trees = trees && trees->kind == Value::Array && !trees->arr.empty();
-    if (bodies && bodies->kind == Value::Array && !bodies->arr.empty() && n_trees)
-        err("$.bodies", "the GPU step builds trees or free bodies, not both");
     std::map<std::string, int> body_index, tree_index;
     std::vector<std::map<std::string, int>> link_index;
     load_bodies(bodies, &L, &body_index);
-    if (L.bodies.empty()) {
+    if (n_trees || L.bodies.empty()) {
         if (!n_trees) err("$.trees", "expected a non-empty array of trees");
         for (size_t k = 0; k < trees->arr.size(); ++k) {
             const std::string tp = "$.trees[" + std::to_string(k) + "]";

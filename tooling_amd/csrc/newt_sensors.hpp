@@ -194,7 +194,7 @@ template <class R> struct SensorEval {
                 // aba (tree.rs:942-955): explicit, the user's applied force,
                 // the contact wrenches as external forces, penalty limits on
                 // (the step has restored disable_penalty_limits).
-                for (int s = 0; s < m.nv; ++s) wd.dw[L.tau + s] = wd.qfrc[s];
+                for (int s = 0; s < m.tnv; ++s) wd.dw[L.tau + s] = wd.qfrc[s];
                 wd.ext_wr = L.swr;
                 wd.aba(false, false, m.dt, L.sqdd);
                 wd.ext_wr = -1;

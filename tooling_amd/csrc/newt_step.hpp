@@ -146,6 +146,30 @@ template <class R> struct World {
     NEWT_HD V3<double> kld3(int off) const { return {dw[off], dw[off + 1], dw[off + 2]}; }
     NEWT_HD void kst3(int off, V3<double> v) const { dw[off] = v.x; dw[off + 1] = v.y; dw[off + 2] = v.z; }
     NEWT_HD SM<double> kldm(int off) const { return {kld3(off), kld3(off + 3)}; }
+    // Free bodies beside the trees: state at q_off / v_off of qpos / qvel.
+    NEWT_HD V3<double> body_pos(int b) const {
+        const int o = m.bodies[b].q_off;
+        return mk3(qpos[o], qpos[o + 1], qpos[o + 2]);
+    }
+    NEWT_HD Qt<double> body_ori(int b) const {
+        const int o = m.bodies[b].q_off;
+        return Qt<double>{qpos[o + 3], qpos[o + 4], qpos[o + 5], qpos[o + 6]};
+    }
+    NEWT_HD V3<double> body_vel(int b) const {
+        const int o = m.bodies[b].v_off;
+        return mk3(qvel[o], qvel[o + 1], qvel[o + 2]);
+    }
+    NEWT_HD V3<double> body_omega(int b) const {  // body frame
+        const int o = m.bodies[b].v_off;
+        return mk3(qvel[o + 3], qvel[o + 4], qvel[o + 5]);
+    }
+    NEWT_HD V3<double> body_gravity(int b) const {
+        const V3<double> g = mk3(m.gravity[0], m.gravity[1], m.gravity[2]);
+        const double s = m.bodies[b].gravity_scale;
+        return s == 1.0 ? g : g * s;
+    }
+    // A geom on a tree link (not a free body, not the static world).
+    static NEWT_HD bool on_link(const GeomT& g) { return g.link >= 0 && g.body < 0; }
     NEWT_HD void kstm(int off, SM<double> v) const { kst3(off, v.w); kst3(off + 3, v.v); }
     NEWT_HD SF<double> kldf(int off) const { return {kld3(off), kld3(off + 3)}; }
     NEWT_HD void kstf(int off, SF<double> v) const { kst3(off, v.t); kst3(off + 3, v.f); }
@@ -540,7 +564,7 @@ template <class R> struct World {
 
     // dynamics::mass_matrix (dynamics.rs:474-578) into dw[M] (row-major nv*nv).
     NEWT_HD void mass_matrix() const {
-        const int nv = m.nv, n = m.n_links;
+        const int nv = m.tnv, n = m.n_links;
         for (int k = 0; k < nv * nv; ++k) dw[L.M + k] = 0.0;
         for (int i = 0; i < n; ++i) {
             for (int k = 0; k < 36; ++k) dw[L.ia + i * 36 + k] = m.links[i].si[k];
@@ -597,7 +621,7 @@ template <class R> struct World {
 
     // Cholesky-Banachiewicz (dynamics.rs:602-622).  false when not PD.
     NEWT_HD bool cholesky(int src, int dst, bool add_damping, double dt) const {
-        const int n = m.nv;
+        const int n = m.tnv;
         for (int k = 0; k < n * n; ++k) dw[dst + k] = 0.0;
         for (int i = 0; i < n; ++i)
             for (int j = 0; j <= i; ++j) {
@@ -615,7 +639,7 @@ template <class R> struct World {
     }
     // cholesky_solve (dynamics.rs:626-648): b in dw[vec], x into dw[res].
     NEWT_HD void chol_solve(int fac, int vec, int res) const {
-        const int n = m.nv;
+        const int n = m.tnv;
         for (int i = 0; i < n; ++i) {
             double s = dw[vec + i];
             for (int k = 0; k < i; ++k) s = s - dw[fac + i * n + k] * dw[res + k];
@@ -806,7 +830,7 @@ template <class R> struct World {
             for (int k = 0; k < 36; ++k) dw[L.ia + p * 36 + k] = dw[L.ia + p * 36 + k] + out6[k];
             kstf(L.pa + p * 6, kldf(L.pa + p * 6) + xtforce(x, paf));
         }
-        for (int k = 0; k < m.nv; ++k) dw[out + k] = double(0);
+        for (int k = 0; k < m.tnv; ++k) dw[out + k] = double(0);
         // Each root: a free root solves its 6x6 articulated system, a fixed
         // root does not accelerate.
         for (int ri = 0; ri < n; ++ri) {
@@ -879,7 +903,7 @@ template <class R> struct World {
     // Translational Jacobian of a world point on `link` (jacobian.rs:44-110),
     // into dw[L.jt + 3*slot] (FP64 geometry).
     NEWT_HD void point_jacobian(int link, V3<double> local) const {
-        for (int s = 0; s < m.nv; ++s) kst3(L.jt + 3 * s, z3<double>());
+        for (int s = 0; s < m.tnv; ++s) kst3(L.jt + 3 * s, z3<double>());
         V3<double> point = pos(link) + qrot(ori(link), local);
         const LinkT& tl = m.links[link];
         for (int k = 0; k < tl.chain_len; ++k) {
@@ -911,7 +935,7 @@ template <class R> struct World {
     NEWT_HD double invweight(int link) const {
         if (iw[L.iwf + link]) return dw[L.iwc + link];
         point_jacobian(link, z3<double>());
-        const int nv = m.nv;
+        const int nv = m.tnv;
         double tr = 0.0;
         for (int axis = 0; axis < 3; ++axis) {
             for (int s = 0; s < nv; ++s) dw[L.dvec + s] = dw[L.jt + 3 * s + axis];
@@ -931,7 +955,7 @@ template <class R> struct World {
     // Returns 0, or the fault: 3 a factor not positive definite, 4 a failed
     // Newton solve.
     NEWT_HD int solve_contacts(int nc, R dt, int* nrows_out) const {
-        const int nv = m.nv;
+        const int nv = m.tnv;
         const double dtd = m.dt;
         for (int s = 0; s < nv; ++s) dw[L.delta + s] = 0.0;
         *nrows_out = 0;
@@ -950,6 +974,14 @@ template <class R> struct World {
         }
         if (!cholesky(L.M, L.Li, implicit && dtd != 0.0, dtd)) return 3;
         if (!cholesky(L.M, L.Ld, false, 0.0)) return 3;
+        // Free bodies beside the trees: I_world^-1 = R I^-1 R^T per body
+        // (solver.rs:2838-2845).  Their wrench accumulators (L.tbw) are
+        // cleared by the step before the solve.
+        for (int b = 0; b < m.n_bodies; ++b) {
+            const M33<double> r = qmat(body_ori(b));
+            const M33<double> iwm = m33_mul(m33_mul(r, m33_load(m.bodies[b].inv_inertia)), m33_tr(r));
+            for (int k = 0; k < 9; ++k) dw[L.biw + 9 * b + k] = iwm.m[k];
+        }
         // Free velocity: ABA at the current state, user forces only.
         for (int s = 0; s < nv; ++s) dw[L.tau + s] = qfrc[s];
         aba(true, implicit, dtd, L.qacc_free);
@@ -962,6 +994,8 @@ template <class R> struct World {
             if (pen <= 0.0) continue;
             const GeomT& ga = m.geoms[iw[L.cids + 3 * k]];
             const GeomT& gb = m.geoms[iw[L.cids + 3 * k + 1]];
+            // contacts with no link side belong to the free-body solve
+            if (!on_link(ga) && !on_link(gb)) continue;
             V3<double> cp = kld3(o), n = kld3(o + 3);
             double mu = ga.friction < gb.friction ? ga.friction : gb.friction;
             frictionless = frictionless || mu == 0.0;
@@ -973,8 +1007,24 @@ template <class R> struct World {
             for (int r = 0; r < 4; ++r) iw[L.rinfo + 3 * (nr + r)] = 0;
             for (int side = 0; side < 2; ++side) {
                 const GeomT& g = side == 0 ? ga : gb;
-                if (g.link < 0) continue;
                 double sign = side == 0 ? 1.0 : -1.0;
+                if (g.body >= 0) {
+                    // world_jacobian_for_side, Body (solver.rs:3547-3565): the
+                    // direction and its moment about the body's COM; the
+                    // component is coded -(body + 1).
+                    const V3<double> arm = cp - body_pos(g.body);
+                    for (int r = 0; r < 4; ++r) {
+                        int row = nr + r;
+                        int cidx = iw[L.rinfo + 3 * row];
+                        iw[L.rinfo + 3 * row + 1 + cidx] = -(g.body + 1);
+                        iw[L.rinfo + 3 * row] = cidx + 1;
+                        const int bo = L.bcomp + (row * 2 + cidx) * 6;
+                        kst3(bo, dirs[r] * sign);
+                        kst3(bo + 3, cross3(arm, dirs[r]) * sign);
+                    }
+                    continue;
+                }
+                if (g.link < 0) continue;
                 V3<double> local = qrot_inv(ori(g.link), cp - pos(g.link));
                 point_jacobian(g.link, local);
                 for (int r = 0; r < 4; ++r) {
@@ -1045,8 +1095,22 @@ template <class R> struct World {
                 double tr = 0.0;
                 Sum<double> vel, fvel;
                 for (int cidx = 0; cidx < ncomp; ++cidx) {
+                    const int who = iw[L.rinfo + 3 * row + 1 + cidx];
+                    if (who < 0) {
+                        // a body component (solver.rs:3624-3685, 3766-3771):
+                        // 1/m into diagApprox, its current velocity and its
+                        // gravity step along the row
+                        const int b = -who - 1;
+                        const int bo = L.bcomp + (row * 2 + cidx) * 6;
+                        const V3<double> lin = kld3(bo), ang = kld3(bo + 3);
+                        tr = tr + 1.0 / m.bodies[b].mass;
+                        const double bv = dot3(lin, body_vel(b)) + dot3(ang, qrot(body_ori(b), body_omega(b)));
+                        vel.add(bv + 0.0);
+                        fvel.add(dot3(lin, body_gravity(b) * dtd) + 0.0);
+                        continue;
+                    }
                     int co = L.coef + (row * 2 + cidx) * nv;
-                    tr = tr + invweight(iw[L.rinfo + 3 * row + 1 + cidx]);
+                    tr = tr + invweight(who);
                     Sum<double> sv, sf;
                     for (int s = 0; s < nv; ++s) sv.add((double)ws[co + s] * qvel[s]);
                     for (int s = 0; s < nv; ++s) sf.add((double)ws[co + s] * (dw[L.qacc_free + s] * dtd));
@@ -1069,7 +1133,24 @@ template <class R> struct World {
         for (int col = 0; col < nr; ++col) {
             int ncomp = iw[L.rinfo + 3 * col];
             for (int s = 0; s < nv; ++s) dw[L.gres + s] = 0.0;
+            // apply_world_impulse (solver.rs:3799-3836): a column's body
+            // component moves its body by (d / m, R^T I_world^-1 (r x d)).
+            int colb = -1;
+            if (m.n_bodies > 0)
+                for (int k = 0; k < 6; ++k) dw[L.bdel + k] = 0.0;
             for (int cidx = 0; cidx < ncomp; ++cidx) {
+                const int who = iw[L.rinfo + 3 * col + 1 + cidx];
+                if (who < 0) {
+                    colb = -who - 1;
+                    const int bo = L.bcomp + (col * 2 + cidx) * 6;
+                    const V3<double> force = kld3(bo) * 1.0, torque = kld3(bo + 3) * 1.0;
+                    M33<double> iwm;
+                    for (int k = 0; k < 9; ++k) iwm.m[k] = dw[L.biw + 9 * colb + k];
+                    const V3<double> av = mv(iwm, torque);
+                    kst3(L.bdel, kld3(L.bdel) + force / m.bodies[colb].mass);
+                    kst3(L.bdel + 3, kld3(L.bdel + 3) + qrot_inv(body_ori(colb), av));
+                    continue;
+                }
                 int co = L.coef + (col * 2 + cidx) * nv;
                 for (int s = 0; s < nv; ++s) dw[L.dvec + s] = (double)ws[co + s] * 1.0;
                 chol_solve(L.Li, L.dvec, L.dres);
@@ -1079,6 +1160,17 @@ template <class R> struct World {
                 int nrc = iw[L.rinfo + 3 * row];
                 Sum<double> outer;
                 for (int cidx = 0; cidx < nrc; ++cidx) {
+                    const int who = iw[L.rinfo + 3 * row + 1 + cidx];
+                    if (who < 0) {
+                        // world_velocity_from_delta (solver.rs:3687-3714)
+                        const int b = -who - 1;
+                        const int bo = L.bcomp + (row * 2 + cidx) * 6;
+                        const V3<double> dl = b == colb ? kld3(L.bdel) : z3<double>();
+                        const V3<double> da = b == colb ? kld3(L.bdel + 3) : z3<double>();
+                        const double bv = dot3(kld3(bo), dl) + dot3(kld3(bo + 3), qrot(body_ori(b), da));
+                        outer.add(bv + 0.0);
+                        continue;
+                    }
                     int co = L.coef + (row * 2 + cidx) * nv;
                     Sum<double> sm;
                     for (int s = 0; s < nv; ++s) sm.add((double)ws[co + s] * dw[L.gres + s]);
@@ -1109,6 +1201,14 @@ template <class R> struct World {
             double force = dw[L.f + i] / dtd;
             int ncomp = iw[L.rinfo + 3 * i];
             for (int cidx = 0; cidx < ncomp; ++cidx) {
+                const int who = iw[L.rinfo + 3 * i + 1 + cidx];
+                if (who < 0) {
+                    // TreeContactSolution::body_wrenches (solver.rs:3345-3350)
+                    const int bo = L.bcomp + (i * 2 + cidx) * 6, wo = L.tbw + 6 * (-who - 1);
+                    kst3(wo, kld3(wo) + kld3(bo) * force);
+                    kst3(wo + 3, kld3(wo + 3) + kld3(bo + 3) * force);
+                    continue;
+                }
                 int co = L.coef + (i * 2 + cidx) * nv;
                 for (int s = 0; s < nv; ++s) dw[L.delta + s] = dw[L.delta + s] + (double)ws[co + s] * force;
             }
@@ -1280,7 +1380,7 @@ template <class R> struct World {
     // code.
     NEWT_HD int pre() const {
         const R dt = (R)m.dt;
-        const int nv = m.nv;
+        const int nv = m.tnv;
         int np = active_pairs();
         int nc = 0;
         bool constraint = m.mode == 1 || m.mode == 2;
@@ -1308,7 +1408,7 @@ template <class R> struct World {
     // range, in link order; A from Cholesky solves of M; PGS with f >= 0.
     // The per-dof force into dw[L.lim] (zero without active rows).
     NEWT_HD void solve_limits() const {
-        const int nv = m.nv;
+        const int nv = m.tnv;
         const double dt = m.dt;
         for (int s = 0; s < nv; ++s) dw[L.lim + s] = 0.0;
         if (m.n_ranged == 0 || nv == 0 || !(dt > 0.0)) return;
@@ -1407,7 +1507,7 @@ template <class R> struct World {
     // Penalty contacts are sampled again at every stage (world.rs:1913-1931);
     // returns a capacity fault code or 0.
     NEWT_HD int rk4(bool constraint) const {
-        const int nv = m.nv, nq = m.nq;
+        const int nv = m.tnv, nq = m.tnq;
         const double dt = m.dt;
         for (int k = 0; k < nq; ++k) dw[L.rq0 + k] = qpos[k];
         for (int s = 0; s < nv; ++s) dw[L.rv0 + s] = qvel[s];
@@ -1458,9 +1558,11 @@ template <class R> struct World {
         return 0;
     }
 
-    NEWT_HD int post() const {
+    // With refresh false the caller refreshes the broadphase itself (after
+    // stepping the free bodies beside the trees).
+    NEWT_HD int post(bool refresh = true) const {
         const R dt = (R)m.dt;
-        const int nv = m.nv;
+        const int nv = m.tnv;
         bool constraint = m.mode == 1 || m.mode == 2;
         // step_trees_euler / step_trees (world.rs:1826-2051): the limit force
         // plus the contact force is added to qfrc_applied for the step and
@@ -1474,9 +1576,11 @@ template <class R> struct World {
             const int f = rk4(constraint);
             if (f != 0) return f;
             for (int s = 0; s < nv; ++s) qfrc[s] = dw[L.tau + s] - dw[L.lim + s];
-            forward_kinematics();
-            link_velocities();
-            populate(dt, false);
+            if (refresh) {
+                forward_kinematics();
+                link_velocities();
+                populate(dt, false);
+            }
             return 0;
         }
         if (penalty_contacts()) {  // step_trees_euler: sampled once at the current state
@@ -1511,9 +1615,11 @@ template <class R> struct World {
             }
         }
         // refresh_broadphase_for_query on the post-step state (world.rs:1523)
-        forward_kinematics();
-        link_velocities();
-        populate(dt, false);
+        if (refresh) {
+            forward_kinematics();
+            link_velocities();
+            populate(dt, false);
+        }
         return 0;
     }
 

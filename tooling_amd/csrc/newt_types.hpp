@@ -128,7 +128,8 @@ struct ModelT {
     int n_ranged;  // hinges/slides with a range
     const LimitT* limits;
     int ext;       // needs the extended persistent step: limits, RK4 or non-plane pairs
-    int n_bodies;  // free bodies (a model has a tree or free bodies, not both)
+    int n_bodies;  // free bodies; with trees too, their state follows the trees' (q_off / v_off)
+    int tnq, tnv;  // the trees' share of nq / nv (all of it without free bodies, 0 without trees)
     int penalty_limits;  // ranged joints under the penalty solver: the lane-per-world step only
     const BodyT* bodies;
     // Sensors, evaluated after the last step of a launch (sensor.rs:446-702).
@@ -162,6 +163,10 @@ struct Layout {
     // impulse; per contact block the two arms (6) and the two facet-pair
     // cross responses
     int bwr, biw, bbd, brow, bblk;
+    // trees with free bodies: body components of the tree rows (linear and
+    // angular Jacobian, 6 per row and side), one column's body response (6)
+    // and the tree solve's body wrenches (6 per body)
+    int bcomp, bdel, tbw;
     // sensors: per-link contact wrenches (6), v and a (12) and joint
     // wrenches (6); qddot [nv]; per-contact normal forces
     int swr, sva, sfl, sqdd, sfn;

@@ -306,8 +306,6 @@ def load(src) -> scenes.Scene:
     if root.tag != "mujoco":
         raise MjcfError("<root>", "expected <mujoco>")
     ld = _Loader(root)
-    if ld.trees and ld.bodies:
-        raise MjcfError("worldbody", "the GPU step builds trees or free bodies, not both")
     opts = dict(ld.opts)
     trees = ld.trees
     q_tree = [x for qs in ld.q_trees for x in qs]

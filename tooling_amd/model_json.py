@@ -223,11 +223,9 @@ def load(src) -> scenes.Scene:
     bodies_v = top.get("bodies", [])
     if not isinstance(bodies_v, list):
         raise ModelError("$.bodies", "expected an array")
-    if bodies_v and trees:
-        raise ModelError("$.bodies", "the GPU step builds trees or free bodies, not both")
     bodies, body_index = _bodies(bodies_v)
     tree_index, indexes, loaded = {}, [], []
-    if not bodies:
+    if trees or not bodies:
         if not isinstance(trees, list) or not trees:
             raise ModelError("$.trees", "expected a non-empty array of trees")
         for k, t in enumerate(trees):
@@ -255,7 +253,7 @@ def load(src) -> scenes.Scene:
     sc.extra["sites"] = sites
     sc.extra["link_names"] = list(index)
     sc.extra["body_names"] = list(body_index)
-    if bodies:
+    if bodies and not loaded:
         q0 = np.array([x for b in bodies for x in (*b["position"], *b["orientation"])])
         v0 = np.array([x for b in bodies for x in (*b["linear_velocity"], *b["angular_velocity_body"])])
         sc.sampler = lambda n, seed: (np.tile(q0, (n, 1)), np.tile(v0, (n, 1)))
