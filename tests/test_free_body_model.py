@@ -161,14 +161,13 @@ def test_newton_stack_scene_is_newts_golden(oracle_lib):
 
 
 def test_trees_with_bodies_unsupported_solvers_rejected():
-    """A model with trees and free bodies compiles under PGS; penalty
-    contacts, and Newton contacts with no link side, are rejected at
-    finalize before any device work."""
+    """A model with trees and free bodies compiles under PGS and Newton;
+    penalty contacts are rejected at finalize before any device work."""
     st = scenes.box_stack()
     pend = scenes.pendulum()
-    for solver in (abi.SOLVER_PENALTY, abi.SOLVER_NEWTON):
-        sc = scenes.Scene("mixed", dict(st.options, solver=solver), pend.links, st.geoms, 0, None, bodies=st.bodies)
-        assert _status(sc) == abi.ERR_UNSUPPORTED
+    sc = scenes.Scene("mixed", dict(st.options, solver=abi.SOLVER_PENALTY), pend.links, st.geoms, 0, None,
+                      bodies=st.bodies)
+    assert _status(sc) == abi.ERR_UNSUPPORTED
 
 
 def test_geom_on_missing_body_rejected():

@@ -11,10 +11,11 @@ rows newt's anchors and a reduction to the tree-tree solve pin
 - newt's sphere tree and sphere body pushed apart, and its tilted tree box
   on a box body, under PGS and Newton;
 - a hinged box chain on a plane with free boxes dropped on it, under PGS
-  with Euler and RK4: tree-body, body-body, body-plane and tree-plane
-  contacts, contact lists equal every 10th step, state and qacc bitwise
-  after 300 steps;
-- rejections: penalty contacts, Newton contacts with no link side, sensors.
+  with Euler and RK4 and under Newton with Euler: tree-body, body-body,
+  body-plane and tree-plane contacts, contact lists equal every 10th step,
+  state and qacc bitwise after 300 steps;
+- newt's pendulum model with a ball, loaded from JSON;
+- rejections: penalty contacts, sensors.
 """
 import sys
 
@@ -81,11 +82,13 @@ def test_tree_body_anchor_scenes(oracle_lib, solver):
             assert kinds.get(("body", "tree"), 0) > 0
 
 
-@pytest.mark.parametrize("integrator", [abi.INTEGRATOR_EULER, abi.INTEGRATOR_RK4])
+@pytest.mark.parametrize("solver,integrator", [(abi.SOLVER_PGS, abi.INTEGRATOR_EULER),
+                                              (abi.SOLVER_PGS, abi.INTEGRATOR_RK4),
+                                              (abi.SOLVER_NEWTON, abi.INTEGRATOR_EULER)])
 @pytest.mark.parametrize("precision", PRECISIONS)
-def test_chain_with_free_boxes(oracle_lib, integrator, precision):
-    sc = scenes.chain_and_boxes(integrator=integrator)
-    _say(f"{sc.name} integrator {integrator} precision {precision}: 32 worlds x 300 steps")
+def test_chain_with_free_boxes(oracle_lib, solver, integrator, precision):
+    sc = scenes.chain_and_boxes(solver=solver, integrator=integrator)
+    _say(f"{sc.name} solver {solver} integrator {integrator} precision {precision}: 32 worlds x 300 steps")
     kinds = _run(oracle_lib, sc, 32, 300, precision)
     _say(f"contact kinds {kinds}")
     for k in (("body", "tree"), ("body", "body"), ("body", "static"), ("static", "tree")):
@@ -94,11 +97,10 @@ def test_chain_with_free_boxes(oracle_lib, integrator, precision):
 
 def test_trees_with_bodies_rejections():
     base = scenes.chain_and_boxes()
-    for solver, what in ((abi.SOLVER_PENALTY, "penalty"), (abi.SOLVER_NEWTON, "Newton")):
-        sc = scenes.with_options(base, "_x", solver=solver)
-        with pytest.raises(abi.NewtError) as e:
-            Model(sc)
-        assert e.value.status == abi.ERR_UNSUPPORTED, what
+    sc = scenes.with_options(base, "_x", solver=abi.SOLVER_PENALTY)
+    with pytest.raises(abi.NewtError) as e:
+        Model(sc)
+    assert e.value.status == abi.ERR_UNSUPPORTED
     sc = scenes.with_options(base, "_sensed")
     sc.sensors = [scenes.sensor(abi.SENSOR_JOINT_POS, tree=0, link=1)]
     with pytest.raises(abi.NewtError) as e:

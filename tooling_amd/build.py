@@ -23,6 +23,9 @@ HIPCC_FLAGS = [
     "-fPIC",
     "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # falling off the end of a value-returning function is undefined
+    # behaviour that the compiler may turn into a jump anywhere
+    "-Werror=return-type",
 ]
 
 # Per translation unit: FP contraction.

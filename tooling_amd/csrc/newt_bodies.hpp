@@ -172,11 +172,46 @@ struct FreeBodies {
 
     // ---------------- solve_free_bodies (solver.rs:703-1257) ---------------
     // Per-body world wrenches (force, torque) into bwr.
-    NEWT_HD void solve(int nc) const {
+    // solve_free_body_newton_impulses (solver.rs:1295-1351) on a model with
+    // trees (free-body models run it on the wavefront step): A[i][j] =
+    // row_residual of row i after apply_impulse_delta of a unit impulse on
+    // row j, into the tree solve's Delassus storage (done with by now), H =
+    // A + diag(reg), lin = the row biases, and the World's nonnegative
+    // Newton (newton.rs:275-556) into the rows' impulses.  0, or
+    // NEWT_ERR_NEWTON_FAIL when H is not positive definite.
+    NEWT_HD int newton_impulses(int nr) const {
+        const int cap = w.m.cap_r;
+        const V3<double> z = z3<double>();
+        for (int j = 0; j < nr; ++j) {
+            const int kj = j >> 2, ja = ba(kj), jb = bb(kj);
+            V3<double> dva = z, dwa = z, dvb = z, dwb = z;
+            if (ja >= 0) side_delta(ja, rdir(j), arm_a(kj), 1.0, &dva, &dwa);
+            if (jb >= 0) side_delta(jb, -rdir(j), arm_b(kj), 1.0, &dvb, &dwb);
+            for (int i = 0; i < nr; ++i) {
+                const int ki = i >> 2, a = ba(ki), b = bb(ki);
+                const bool aa = a >= 0 && a == ja, ab = a >= 0 && a == jb;
+                const bool ba_ = b >= 0 && b == ja, bb_ = b >= 0 && b == jb;
+                const V3<double> av = aa ? dva : ab ? dvb : z, aw = aa ? dwa : ab ? dwb : z;
+                const V3<double> bv = ba_ ? dva : bb_ ? dvb : z, bw = ba_ ? dwa : bb_ ? dwb : z;
+                w.ws[w.L.A + i * cap + j] = residual_of(rdir(i), a, b, arm_a(ki), arm_b(ki), av, aw, bv, bw);
+            }
+        }
+        for (int i = 0; i < nr; ++i) {
+            w.dw[w.L.rdiag + i] = w.ws[w.L.A + i * cap + i] + rreg(i);
+            w.dw[w.L.rbias + i] = rbias(i);
+            w.dw[w.L.f + i] = 0.0;
+        }
+        if (!w.newton_contacts(nr, false)) return 4;  // NEWT_ERR_NEWTON_FAIL
+        for (int i = 0; i < nr; ++i) rimp(i) = w.dw[w.L.f + i];
+        return 0;
+    }
+
+    // The free-body solve: per-body wrenches into bwr; 0 or a fault code.
+    NEWT_HD int solve(int nc) const {
         const int nb = w.m.n_bodies;
         const double dt = w.m.dt;
         for (int i = 0; i < 6 * nb; ++i) w.dw[w.L.bwr + i] = 0.0;
-        if (nc == 0 || dt <= 0.0) return;
+        if (nc == 0 || dt <= 0.0) return 0;
         // I_world^-1 = R I^-1 R^T per body.
         for (int i = 0; i < nb; ++i) {
             M33<double> r = qmat(bori(i));
@@ -277,14 +312,19 @@ struct FreeBodies {
                 w.dw[w.L.bblk + 8 * k + 6 + p] = residual_of(dirs[2 * p], a, b, arma, armb, dva, dwa, dvb, dwb);
             }
         }
-        if (nblk == 0) return;
-        for (int i = 0; i < 6 * nb; ++i) w.dw[w.L.bbd + i] = 0.0;
-        for (int it = 0; it < w.m.iterations; ++it)
-            for (int k = 0; k < nblk; ++k) {
-                step_non_negative(4 * k);
-                step_pair(4 * k, w.dw[w.L.bblk + 8 * k + 6]);
-                step_pair(4 * k + 2, w.dw[w.L.bblk + 8 * k + 7]);
-            }
+        if (nblk == 0) return 0;
+        if (w.m.mode == NEWT_SOLVER_NEWTON && w.m.n_links > 0) {
+            const int st = newton_impulses(4 * nblk);
+            if (st != 0) return st;
+        } else {
+            for (int i = 0; i < 6 * nb; ++i) w.dw[w.L.bbd + i] = 0.0;
+            for (int it = 0; it < w.m.iterations; ++it)
+                for (int k = 0; k < nblk; ++k) {
+                    step_non_negative(4 * k);
+                    step_pair(4 * k, w.dw[w.L.bblk + 8 * k + 6]);
+                    step_pair(4 * k + 2, w.dw[w.L.bblk + 8 * k + 7]);
+                }
+        }
         // Per-body wrenches, rows in order (solver.rs:1198-1235).
         for (int ri = 0; ri < 4 * nblk; ++ri) {
             const double f = rimp(ri);
@@ -304,6 +344,7 @@ struct FreeBodies {
                 w.kst3(o + 3, w.kld3(o + 3) + cross3(arm_b(k), fb));
             }
         }
+        return 0;
     }
 
     // ---------------- integration (world.rs:1705-1817, 3223-3286) ---------
@@ -585,7 +626,8 @@ struct FreeBodies {
         st = w.post(false);
         if (st != 0) return st;
         const int nc = w.iw[w.L.iwf + w.m.n_links + 1];
-        solve(nc);
+        st = solve(nc);
+        if (st != 0) return st;
         for (int b = 0; b < w.m.n_bodies; ++b)
             for (int k = 0; k < 6; ++k) w.dw[w.L.bwr + 6 * b + k] = w.dw[w.L.bwr + 6 * b + k] + w.dw[w.L.tbw + 6 * b + k];
         integrate();
@@ -609,7 +651,8 @@ struct FreeBodies {
         }
         w.iw[w.L.iwf] = np;
         w.iw[w.L.iwf + 1] = nc;
-        solve(nc);
+        const int st = solve(nc);
+        if (st != 0) return st;
         integrate();
         set_poses();
         w.populate(w.m.dt, false);

@@ -538,12 +538,6 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
                 return cfail(err, NEWT_ERR_UNSUPPORTED, "penalty contacts in a model with trees and free bodies are "
                                                         "not built (geom " + std::to_string(a) + " x geom " +
                                                             std::to_string(b) + ")");
-            // Contacts with no link side go to the free-body solve, whose
-            // Newton form runs on the wavefront step of free-body models only.
-            if (nb > 0 && !la && !lb && o.solver_mode == NEWT_SOLVER_NEWTON)
-                return cfail(err, NEWT_ERR_UNSUPPORTED, "Newton contacts between free bodies, or a free body and the "
-                                                        "world, in a model with trees are not built (geom " +
-                                                            std::to_string(a) + " x geom " + std::to_string(b) + ")");
             // newt steps trees one after another under the penalty solver
             // (step_trees, world.rs:1826-1946), each against the others'
             // states of that moment: not built.
