@@ -101,7 +101,9 @@ typedef enum newt_precision {
 typedef struct newt_options {
     double dt;               /* World::dt */
     double gravity[3];       /* World::gravity */
-    double penetration_slop; /* World::penetration_slop */
+    double penetration_slop; /* World::penetration_slop (>= 0): raises every
+                                contact's force-free gap to at least itself
+                                (append_contacts, world.rs:2517-2528) */
     int32_t integrator;      /* newt_integrator */
     int32_t solver_mode;     /* newt_solver_mode */
     int32_t iterations;      /* SolverConfig::iterations */

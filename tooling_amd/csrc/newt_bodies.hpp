@@ -1041,6 +1041,7 @@ struct BodyWave {
         *fr_out = ga.friction < gb.friction ? ga.friction : gb.friction;
         const double margin = ga.margin > gb.margin ? ga.margin : gb.margin;
         *gap_out = ga.gap > gb.gap ? ga.gap : gb.gap;
+        if (m.slop > 0.0 && *gap_out < m.slop) *gap_out = m.slop;  // append_contacts (world.rs:2517-2528)
         if (ga.shape == SH_BOX && gb.shape == SH_BOX) {
             V3<double> pa, pb, n;
             Qt<double> qa, qb;

@@ -156,7 +156,7 @@ inline bool body_inverse(const double* cm, double* out_rm) {
 inline newt_status compile_bodies(HostModel* M, std::string* err) {
     const newt_options& o = M->opt;
     if (o.iterations < 1) return cfail(err, NEWT_ERR_INVALID_CONFIG, "solver iterations must be >= 1");
-    if (o.penetration_slop != 0.0) return cfail(err, NEWT_ERR_UNSUPPORTED, "penetration_slop is not built");
+    if (!(o.penetration_slop >= 0.0)) return cfail(err, NEWT_ERR_INVALID_ARG, "penetration_slop must be >= 0");
     if (o.solver_mode == NEWT_SOLVER_NEWTON && o.cone == NEWT_CONE_ELLIPTIC)
         return cfail(err, NEWT_ERR_INVALID_CONFIG, "solver=newton with cone=elliptic is not supported yet; use cone=pyramidal");
     if (o.integrator != NEWT_INTEGRATOR_EULER && o.integrator != NEWT_INTEGRATOR_IMPLICITFAST &&
@@ -340,7 +340,7 @@ inline newt_status compile_model(HostModel* M, std::string* err) {
     const int nb = (int)M->bodies.size();  // free bodies beside the trees
     const newt_options& o = M->opt;
     if (o.iterations < 1) return cfail(err, NEWT_ERR_INVALID_CONFIG, "solver iterations must be >= 1");
-    if (o.penetration_slop != 0.0) return cfail(err, NEWT_ERR_UNSUPPORTED, "penetration_slop is not built");
+    if (!(o.penetration_slop >= 0.0)) return cfail(err, NEWT_ERR_INVALID_ARG, "penetration_slop must be >= 0");
     if (o.solver_mode == NEWT_SOLVER_NEWTON && o.cone == NEWT_CONE_ELLIPTIC)
         return cfail(err, NEWT_ERR_INVALID_CONFIG, "solver=newton with cone=elliptic is not supported yet; use cone=pyramidal");
     // ImplicitFast differs from Euler only through actuator velocity terms;

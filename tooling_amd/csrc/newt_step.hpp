@@ -492,6 +492,9 @@ template <class R> struct World {
             double fr = (double)ga.friction < (double)gb.friction ? (double)ga.friction : (double)gb.friction;
             double margin = (double)ga.margin > (double)gb.margin ? (double)ga.margin : (double)gb.margin;
             double gap = (double)ga.gap > (double)gb.gap ? (double)ga.gap : (double)gb.gap;
+            // append_contacts (world.rs:2517-2528): the world's penetration
+            // slop raises every contact's force-free gap to at least itself.
+            if (m.slop > 0.0 && gap < m.slop) gap = m.slop;
             // (primitive, plane) in the given order, else swapped.
             if (ga.shape == SH_MESH || gb.shape == SH_MESH) {
                 mesh_pair(&nc, pr.a, ga, pr.b, gb, fr, margin, gap);

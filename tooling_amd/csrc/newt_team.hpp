@@ -472,6 +472,7 @@ struct Team {
         double fr = ga.friction < gb.friction ? ga.friction : gb.friction;
         double margin = ga.margin > gb.margin ? ga.margin : gb.margin;
         double gap = ga.gap > gb.gap ? ga.gap : gb.gap;
+        if (m.slop > 0.0 && gap < m.slop) gap = m.slop;  // append_contacts (world.rs:2517-2528)
         if constexpr (X) {
         if (ga.shape == SH_BOX && gb.shape == SH_BOX) {
             V3<double> pa, pb, nrm, cp[4];
