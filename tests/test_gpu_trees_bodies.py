@@ -15,7 +15,8 @@ rows newt's anchors and a reduction to the tree-tree solve pin
   body-plane and tree-plane contacts, contact lists equal every 10th step,
   state and qacc bitwise after 300 steps;
 - newt's pendulum model with a ball, loaded from JSON;
-- rejections: penalty contacts, sensors.
+- state sensors on links and bodies, bitwise with the oracle;
+- rejections: penalty contacts, acceleration/force/touch sensors.
 """
 import sys
 
@@ -101,11 +102,6 @@ def test_trees_with_bodies_rejections():
     with pytest.raises(abi.NewtError) as e:
         Model(sc)
     assert e.value.status == abi.ERR_UNSUPPORTED
-    sc = scenes.with_options(base, "_sensed")
-    sc.sensors = [scenes.sensor(abi.SENSOR_JOINT_POS, tree=0, link=1)]
-    with pytest.raises(abi.NewtError) as e:
-        Model(sc)
-    assert e.value.status == abi.ERR_UNSUPPORTED
 
 
 def _pendulum_with_ball():
@@ -171,3 +167,40 @@ def test_trees_with_bodies_from_json(oracle_lib):
     assert tree_body > 0
     d.close()
     ow.close()
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_sensors_on_trees_with_bodies(oracle_lib, precision):
+    """State sensors on links and on free bodies of one model (sensor.rs:
+    446-702, evaluated after the step): joint position, frame position and
+    orientation, gyro, velocimeter, frame linear and angular velocity with
+    site offsets; bitwise with the oracle after 150 steps.  Acceleration,
+    force and touch sensors on such models are rejected at finalize."""
+    S = scenes.sensor
+    sc = scenes.chain_and_boxes()
+    sc.sensors = [S(abi.SENSOR_JOINT_POS, tree=0, link=1),
+                  S(abi.SENSOR_FRAME_POS, tree=0, link=2, offset=(0.05, 0.0, 0.01)),
+                  S(abi.SENSOR_FRAME_QUAT, tree=0, link=3),
+                  S(abi.SENSOR_GYRO, body=0),
+                  S(abi.SENSOR_VELOCIMETER, body=1, offset=(0.02, -0.01, 0.0)),
+                  S(abi.SENSOR_FRAME_LINVEL, body=2, offset=(0.0, 0.03, 0.0)),
+                  S(abi.SENSOR_FRAME_ANGVEL, tree=0, link=0)]
+    n = 16
+    q0, v0 = sc.initial_state(n, 5)
+    m = Model(sc)
+    d = Data(m, n, precision)
+    d.set_state(q0, v0)
+    ow = oracle_lib.OracleWorlds(sc, n, 0)
+    ow.set_state(q0, v0)
+    step(m, d, 150)
+    ow.step(150, threads=8)
+    assert not d.faults().any()
+    np.testing.assert_array_equal(d.state()[0], ow.state()[0])
+    np.testing.assert_array_equal(d.sensordata(), ow.sensordata())
+    d.close()
+    ow.close()
+    bad = scenes.chain_and_boxes()
+    bad.sensors = [S(abi.SENSOR_ACCELEROMETER, tree=0, link=1)]
+    with pytest.raises(abi.NewtError) as e:
+        Model(bad)
+    assert e.value.status == abi.ERR_UNSUPPORTED

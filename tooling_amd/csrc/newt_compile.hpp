@@ -283,8 +283,6 @@ inline newt_status compile_sensors(HostModel* M, std::string* err) {
     for (size_t k = 0; k < M->sensors.size(); ++k) {
         const newt_sensor_desc& s = M->sensors[k];
         const std::string who = "sensor " + std::to_string(k) + ": ";
-        if (d.n_bodies > 0 && d.n_links > 0)
-            return cfail(err, NEWT_ERR_UNSUPPORTED, who + "sensors on a model with trees and free bodies are not built");
         const int dim = sensor_kind_dim(s.kind);
         if (dim == 0) return cfail(err, NEWT_ERR_UNSUPPORTED, who + "this sensor kind is not built");
         // (tree, link) -> the link's index in the forest
@@ -311,7 +309,8 @@ inline newt_status compile_sensors(HostModel* M, std::string* err) {
         if (sensor_kind_dynamic(s.kind)) {
             dyn = true;
             if (d.n_bodies > 0)
-                return cfail(err, NEWT_ERR_UNSUPPORTED, who + "acceleration, force and touch sensors are built for trees only");
+                return cfail(err, NEWT_ERR_UNSUPPORTED, who + "acceleration, force and touch sensors are built for "
+                                                              "models without free bodies only");
             if (d.n_pairs > 0 && d.mode != NEWT_SOLVER_PGS)
                 return cfail(err, NEWT_ERR_UNSUPPORTED,
                              who + "acceleration, force and touch sensors with contacts are built for the PGS solver only");

@@ -185,7 +185,7 @@ template <class R> struct SensorEval {
             va = va || sensor_kind_va(m.sensors[k].kind);
             dyn = dyn || sensor_kind_dynamic(m.sensors[k].kind);
         }
-        const bool tree = m.n_bodies == 0;
+        const bool tree = m.n_links > 0;  // trees, alone or beside free bodies
         if (tree) wd.forward_kinematics();
         bool ok = true;
         if (tree && dyn) {
