@@ -207,8 +207,9 @@ newt_status newt_model_add_tree(newt_model* model, const newt_link_desc* links,
  * a contact with a link on one side is solved in the tree solve with the
  * body as a component of its rows (solver.rs:2805-3366), the others in the
  * free-body solve (world.rs:2097-2149).  Such models step on the
- * lane-per-world FP64 step in every precision, under PGS or Newton;
- * penalty contacts and sensors on them are rejected at finalize with
+ * lane-per-world FP64 step in every precision, under PGS or Newton, with
+ * state sensors on links and bodies; penalty contacts and acceleration,
+ * force or touch sensors on them are rejected at finalize with
  * NEWT_ERR_UNSUPPORTED. */
 newt_status newt_model_add_body(newt_model* model, const newt_body_desc* body,
                                 int32_t* body_index);
