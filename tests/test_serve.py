@@ -117,15 +117,18 @@ def test_served_equals_alone(precision):
 @pytest.mark.gpu
 def test_served_free_bodies_equal_alone():
     """Free-body pools (the golden stack under PGS, the 64-box pile under
-    Newton) through the slot entry points: every served rollout equals the
-    same rollout run alone, bit for bit."""
+    Newton) and a pool of trees with free bodies (a hinged box chain with
+    free boxes) through the slot entry points: every served rollout equals
+    the same rollout run alone, bit for bit."""
     spec = [("boxstack3", 20, 60, 0.0), ("boxpile64_newton", 6, 40, 0.0), ("boxstack3", 9, 45, 0.001),
-            ("boxpile64_newton", 5, 30, 0.002), ("boxstack3", 30, 20, 0.003)]
+            ("boxpile64_newton", 5, 30, 0.002), ("boxstack3", 30, 20, 0.003), ("chain4_boxes4", 12, 50, 0.0),
+            ("chain4_boxes4", 7, 35, 0.001)]
     reqs = []
     for i, (key, n, h, t) in enumerate(spec):
         q, v = serve.scene_for(key).initial_state(n, 300 + i)
         reqs.append(serve.Request(key, np.asarray(q, float), np.asarray(v, float), h, t))
-    pools = {"boxstack3": (serve.scene_for("boxstack3"), 40), "boxpile64_newton": (serve.scene_for("boxpile64_newton"), 8)}
+    pools = {"boxstack3": (serve.scene_for("boxstack3"), 40), "boxpile64_newton": (serve.scene_for("boxpile64_newton"), 8),
+             "chain4_boxes4": (serve.scene_for("chain4_boxes4"), 24)}
     say(f"serve free bodies: {len(reqs)} requests, chunk 16")
     srv = serve.Server(pools, precision=abi.PRECISION_FP64, chunk=16, n_streams=2, hbm_budget=8 << 30)
     try:

@@ -861,7 +861,8 @@ def mesh_tree(dt=0.002, iterations=30) -> Scene:
 SCENES = {"pendulum3": pendulum, "quadruped18": quadruped, "boxchain6": box_chain, "coil10": hinge_coil,
           "chain40": lambda: hinge_chain(40), "chain64": lambda: hinge_chain(64), "boxstack3": box_stack,
           "boxpile64": box_pile,
-          "boxpile64_newton": lambda: with_options(box_pile(), "_newton", solver=abi.SOLVER_NEWTON)}
+          "boxpile64_newton": lambda: with_options(box_pile(), "_newton", solver=abi.SOLVER_NEWTON),
+          "chain4_boxes4": chain_and_boxes}
 
 
 def sensor(kind, link=-1, body=-1, geom=-1, offset=(0.0, 0.0, 0.0), quat=(0.0, 0.0, 0.0, 1.0), tree=0):
